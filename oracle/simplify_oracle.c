@@ -1,0 +1,1736 @@
+/*
+ * simplify_oracle.c -- CPU restatement of meshoptimizer 1.2's quadric edge-collapse
+ * simplifier (meshopt_simplify / meshopt_simplifyWithAttributes).
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker for the HIP implementation in
+ * meshoptimizer_amd/: only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
+ * load it.  The product library never links or calls it.
+ *
+ * Every function names the reference lines it restates (src/simplifier.cpp unless stated).
+ * Floating-point expressions keep the reference's operation order, because any change of
+ * association perturbs the collapse order; build with -ffp-contract=off and no -march so that
+ * no FMA is formed (gcc -O3 -ffp-contract=off matches the reference bit for bit).
+ *
+ * Besides the sequential greedy commit (src/simplifier.cpp:1498-1636) this file carries the
+ * round-based commit that the GPU uses (oracle_perform_rounds) so the equivalence of the two
+ * can be checked on the CPU, and per-pass statistics that mirror the reference's TRACE output
+ * (src/simplifier.cpp:1625-1633).
+ *
+ * Parity pins: tests/test_oracle.py checks this file against the known-answer vectors of
+ * demo/tests.cpp (tests/golden/known_answers.json), against the reference library compiled
+ * from its own sources (oracle/_ref, when built), and against closed forms.
+ */
+#include <float.h>
+#include <math.h>
+#include <stddef.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "simplify_oracle.h"
+
+enum
+{
+	OPT_LOCK_BORDER = 1u << 0,
+	OPT_SPARSE = 1u << 1,
+	OPT_ERROR_ABSOLUTE = 1u << 2,
+	OPT_PRUNE = 1u << 3,
+	OPT_REGULARIZE = 1u << 4,
+	OPT_PERMISSIVE = 1u << 5,
+	OPT_REGULARIZE_LIGHT = 1u << 6,
+	OPT_INTERNAL_DEBUG = 1u << 30,
+};
+
+enum
+{
+	VLOCK_LOCK = 1,
+	VLOCK_PROTECT = 2,
+	VLOCK_PRIORITY = 4,
+};
+
+/* vertex kinds, src/simplifier.cpp:296-305 */
+enum
+{
+	K_MANIFOLD = 0,
+	K_BORDER = 1,
+	K_SEAM = 2,
+	K_COMPLEX = 3,
+	K_LOCKED = 4,
+};
+
+#define NONE 0xffffffffu
+#define MAX_ATTR 32
+
+/* collapse permission and opposite-edge tables: same data as src/simplifier.cpp:312-331 */
+static const unsigned char can_collapse[5][5] = {
+    {1, 1, 1, 1, 1},
+    {0, 1, 0, 0, 1},
+    {0, 0, 1, 0, 1},
+    {0, 0, 0, 1, 1},
+    {0, 0, 0, 0, 0},
+};
+
+static const unsigned char has_opposite[5][5] = {
+    {1, 1, 1, 1, 1},
+    {1, 0, 1, 0, 0},
+    {1, 1, 1, 0, 1},
+    {1, 0, 0, 0, 0},
+    {1, 0, 1, 0, 0},
+};
+
+typedef struct
+{
+	float x, y, z;
+} vec3;
+
+typedef struct
+{
+	float a00, a11, a22;
+	float a10, a20, a21;
+	float b0, b1, b2, c;
+	float w;
+} quadric;
+
+typedef struct
+{
+	float gx, gy, gz, gw;
+} qgrad;
+
+/* half-edge CSR: per vertex, the (next, prev) of every corner at that vertex */
+typedef struct
+{
+	unsigned int* offsets; /* V + 1 */
+	unsigned int* next;    /* I */
+	unsigned int* prev;    /* I */
+} corner_csr;
+
+static void* xalloc(size_t n)
+{
+	void* p = malloc(n ? n : 1);
+	if (!p)
+		abort();
+	return p;
+}
+
+static unsigned int fbits(float f)
+{
+	unsigned int u;
+	memcpy(&u, &f, 4);
+	return u;
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* adjacency: src/simplifier.cpp:46-109 (counting sort of corners by vertex, corner order kept) */
+
+static void csr_build(corner_csr* adj, const unsigned int* indices, size_t index_count, size_t vertex_count, const unsigned int* remap)
+{
+	unsigned int* off = adj->offsets;
+	memset(off, 0, (vertex_count + 1) * sizeof(unsigned int));
+
+	for (size_t i = 0; i < index_count; ++i)
+	{
+		unsigned int v = remap ? remap[indices[i]] : indices[i];
+		off[v + 1]++;
+	}
+
+	for (size_t i = 0; i < vertex_count; ++i)
+		off[i + 1] += off[i];
+
+	unsigned int* cursor = (unsigned int*)xalloc(vertex_count * sizeof(unsigned int) + 4);
+	memcpy(cursor, off, vertex_count * sizeof(unsigned int));
+
+	for (size_t t = 0; t < index_count / 3; ++t)
+	{
+		unsigned int c[3];
+		for (int k = 0; k < 3; ++k)
+			c[k] = remap ? remap[indices[t * 3 + k]] : indices[t * 3 + k];
+
+		for (int k = 0; k < 3; ++k)
+		{
+			unsigned int slot = cursor[c[k]]++;
+			adj->next[slot] = c[(k + 1) % 3];
+			adj->prev[slot] = c[(k + 2) % 3];
+		}
+	}
+
+	free(cursor);
+}
+
+/* hasEdge, src/simplifier.cpp:333-343 */
+static int csr_has_edge(const corner_csr* adj, unsigned int a, unsigned int b)
+{
+	for (unsigned int j = adj->offsets[a]; j < adj->offsets[a + 1]; ++j)
+		if (adj->next[j] == b)
+			return 1;
+	return 0;
+}
+
+/* hasEdge over all wedges of a, src/simplifier.cpp:345-362 */
+static int csr_has_edge_wedged(const corner_csr* adj, unsigned int a, unsigned int b, const unsigned int* remap, const unsigned int* wedge)
+{
+	unsigned int v = a;
+	do
+	{
+		for (unsigned int j = adj->offsets[v]; j < adj->offsets[v + 1]; ++j)
+			if (remap[adj->next[j]] == remap[b])
+				return 1;
+		v = wedge[v];
+	} while (v != a);
+	return 0;
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* sparse remap, src/simplifier.cpp:242-294: dense id = order of first occurrence in the index buffer */
+
+static unsigned int* sparse_remap_build(unsigned int* indices, size_t index_count, size_t vertex_count, size_t* out_count)
+{
+	unsigned int* dense = (unsigned int*)xalloc(vertex_count * sizeof(unsigned int));
+	memset(dense, 0xff, vertex_count * sizeof(unsigned int));
+
+	unsigned int* back = (unsigned int*)xalloc(index_count * sizeof(unsigned int) + 4);
+	size_t n = 0;
+
+	for (size_t i = 0; i < index_count; ++i)
+	{
+		unsigned int v = indices[i];
+		if (dense[v] == NONE)
+		{
+			dense[v] = (unsigned int)n;
+			back[n++] = v;
+		}
+		indices[i] = dense[v];
+	}
+
+	free(dense);
+	*out_count = n;
+	return back;
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* position remap, src/simplifier.cpp:111-148, 201-240.
+ * remap[i] = lowest index whose xyz compares equal (float ==, so +0 == -0 and NaN matches nothing).
+ * wedge[] is the cyclic list r -> a_m -> ... -> a_1 -> r over the group r < a_1 < ... < a_m.
+ * Restated with a linear-probing table over canonical keys instead of the reference's hash. */
+
+static int pos_has_nan(const float* p)
+{
+	return p[0] != p[0] || p[1] != p[1] || p[2] != p[2];
+}
+
+static unsigned int canon_bits(float f)
+{
+	unsigned int u = fbits(f);
+	return u == 0x80000000u ? 0u : u;
+}
+
+static void position_remap_build(unsigned int* remap, unsigned int* wedge, const float* pos, size_t vertex_count, size_t stride_float, const unsigned int* sparse)
+{
+	size_t cap = 1;
+	while (cap < vertex_count * 2 + 2)
+		cap *= 2;
+
+	unsigned int* table = (unsigned int*)xalloc(cap * sizeof(unsigned int));
+	memset(table, 0xff, cap * sizeof(unsigned int));
+
+	for (size_t i = 0; i < vertex_count; ++i)
+	{
+		const float* p = pos + (size_t)(sparse ? sparse[i] : i) * stride_float;
+
+		if (pos_has_nan(p))
+		{
+			remap[i] = (unsigned int)i;
+			continue;
+		}
+
+		unsigned int kx = canon_bits(p[0]), ky = canon_bits(p[1]), kz = canon_bits(p[2]);
+		uint64_t h = (uint64_t)kx * 0x9E3779B97F4A7C15ull ^ (uint64_t)ky * 0xC2B2AE3D27D4EB4Full ^ (uint64_t)kz * 0x165667B19E3779F9ull;
+		size_t slot = (size_t)((h ^ (h >> 29)) & (cap - 1));
+
+		for (;;)
+		{
+			unsigned int e = table[slot];
+			if (e == NONE)
+			{
+				table[slot] = (unsigned int)i;
+				remap[i] = (unsigned int)i;
+				break;
+			}
+
+			const float* q = pos + (size_t)(sparse ? sparse[e] : e) * stride_float;
+			if (q[0] == p[0] && q[1] == p[1] && q[2] == p[2])
+			{
+				remap[i] = e;
+				break;
+			}
+
+			slot = (slot + 1) & (cap - 1);
+		}
+	}
+
+	free(table);
+
+	for (size_t i = 0; i < vertex_count; ++i)
+		wedge[i] = (unsigned int)i;
+
+	/* insert each secondary right after its primary: ring r -> newest -> ... -> oldest -> r */
+	for (size_t i = 0; i < vertex_count; ++i)
+		if (remap[i] != i)
+		{
+			unsigned int r = remap[i];
+			wedge[i] = wedge[r];
+			wedge[r] = (unsigned int)i;
+		}
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* classification, src/simplifier.cpp:364-542 */
+
+static void classify(unsigned char* kind, unsigned int* loop, unsigned int* loopback, size_t vertex_count, const corner_csr* adj, const unsigned int* remap, const unsigned int* wedge, const unsigned char* vertex_lock, const unsigned int* sparse, unsigned int options)
+{
+	/* open incoming/outgoing half-edge per vertex: NONE if none, the other endpoint if exactly
+	 * one, the vertex itself if several or if it has a self edge (src/simplifier.cpp:379-400) */
+	unsigned int* openinc = loopback;
+	unsigned int* openout = loop;
+
+	for (size_t i = 0; i < vertex_count; ++i)
+		openinc[i] = openout[i] = NONE;
+
+	for (unsigned int v = 0; v < vertex_count; ++v)
+	{
+		for (unsigned int j = adj->offsets[v]; j < adj->offsets[v + 1]; ++j)
+		{
+			unsigned int t = adj->next[j];
+
+			if (t == v)
+				openinc[v] = openout[v] = v;
+			else if (!csr_has_edge(adj, t, v))
+			{
+				openinc[t] = (openinc[t] == NONE) ? v : t;
+				openout[v] = (openout[v] == NONE) ? t : v;
+			}
+		}
+	}
+
+	for (unsigned int i = 0; i < vertex_count; ++i)
+	{
+		if (remap[i] != i)
+		{
+			kind[i] = kind[remap[i]];
+			continue;
+		}
+
+		if (wedge[i] == i)
+		{
+			unsigned int oi = openinc[i], oo = openout[i];
+
+			if (oi == NONE && oo == NONE)
+				kind[i] = K_MANIFOLD;
+			else if (oi != NONE && oo != NONE && remap[oi] == remap[oo] && oi != i)
+				kind[i] = K_SEAM; /* half-seam */
+			else if (oi != i && oo != i)
+				kind[i] = K_BORDER;
+			else
+				kind[i] = K_LOCKED;
+		}
+		else if (wedge[wedge[i]] == i)
+		{
+			unsigned int w = wedge[i];
+			unsigned int iv = openinc[i], ov = openout[i];
+			unsigned int iw = openinc[w], ow = openout[w];
+
+			int open_ok = iv != NONE && iv != i && ov != NONE && ov != i &&
+			              iw != NONE && iw != w && ow != NONE && ow != w;
+
+			if (open_ok && remap[iv] == remap[ow] && remap[ov] == remap[iw] && remap[iv] != remap[ov])
+				kind[i] = K_SEAM;
+			else
+				kind[i] = K_LOCKED;
+		}
+		else
+			kind[i] = K_LOCKED; /* three or more wedges */
+	}
+
+	if (options & OPT_PERMISSIVE)
+	{
+		/* src/simplifier.cpp:481-515 */
+		for (unsigned int i = 0; i < vertex_count; ++i)
+		{
+			if (kind[i] != K_SEAM && kind[i] != K_LOCKED)
+				continue;
+
+			if (remap[i] != i)
+			{
+				kind[i] = kind[remap[i]];
+				continue;
+			}
+
+			int protect = 0;
+			unsigned int v = i;
+			do
+			{
+				unsigned int rv = sparse ? sparse[v] : v;
+				protect |= vertex_lock && (vertex_lock[rv] & VLOCK_PROTECT) != 0;
+				v = wedge[v];
+			} while (v != i);
+
+			do
+			{
+				for (unsigned int j = adj->offsets[v]; j < adj->offsets[v + 1]; ++j)
+					protect |= !csr_has_edge_wedged(adj, adj->next[j], v, remap, wedge);
+				v = wedge[v];
+			} while (v != i);
+
+			if (!protect)
+				kind[i] = K_COMPLEX;
+		}
+	}
+
+	if (vertex_lock)
+	{
+		/* src/simplifier.cpp:517-531 */
+		for (size_t i = 0; i < vertex_count; ++i)
+		{
+			unsigned int ri = sparse ? sparse[i] : (unsigned int)i;
+			if (vertex_lock[ri] & VLOCK_LOCK)
+				kind[remap[i]] = K_LOCKED;
+		}
+
+		for (size_t i = 0; i < vertex_count; ++i)
+			if (kind[remap[i]] == K_LOCKED)
+				kind[i] = K_LOCKED;
+	}
+
+	if (options & OPT_LOCK_BORDER)
+		for (size_t i = 0; i < vertex_count; ++i)
+			if (kind[i] == K_BORDER)
+				kind[i] = K_LOCKED;
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* rescale, src/simplifier.cpp:549-621 */
+
+static float rescale_positions(vec3* out, const float* pos, size_t vertex_count, size_t stride_float, const unsigned int* sparse)
+{
+	float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX};
+	float hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+
+	for (size_t i = 0; i < vertex_count; ++i)
+	{
+		const float* p = pos + (size_t)(sparse ? sparse[i] : i) * stride_float;
+
+		if (out)
+		{
+			out[i].x = p[0];
+			out[i].y = p[1];
+			out[i].z = p[2];
+		}
+
+		for (int k = 0; k < 3; ++k)
+		{
+			lo[k] = lo[k] > p[k] ? p[k] : lo[k];
+			hi[k] = hi[k] < p[k] ? p[k] : hi[k];
+		}
+	}
+
+	float extent = 0.f;
+	extent = (hi[0] - lo[0]) < extent ? extent : (hi[0] - lo[0]);
+	extent = (hi[1] - lo[1]) < extent ? extent : (hi[1] - lo[1]);
+	extent = (hi[2] - lo[2]) < extent ? extent : (hi[2] - lo[2]);
+
+	if (out)
+	{
+		float scale = extent == 0 ? 0.f : 1.f / extent;
+
+		for (size_t i = 0; i < vertex_count; ++i)
+		{
+			out[i].x = (out[i].x - lo[0]) * scale;
+			out[i].y = (out[i].y - lo[1]) * scale;
+			out[i].z = (out[i].z - lo[2]) * scale;
+		}
+	}
+
+	return extent;
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* quadric algebra, src/simplifier.cpp:702-965 (same expression trees) */
+
+static float vnormalize(vec3* v)
+{
+	float length = sqrtf(v->x * v->x + v->y * v->y + v->z * v->z);
+
+	if (length > 0)
+	{
+		v->x /= length;
+		v->y /= length;
+		v->z /= length;
+	}
+
+	return length;
+}
+
+static void q_add(quadric* Q, const quadric* R)
+{
+	Q->a00 += R->a00;
+	Q->a11 += R->a11;
+	Q->a22 += R->a22;
+	Q->a10 += R->a10;
+	Q->a20 += R->a20;
+	Q->a21 += R->a21;
+	Q->b0 += R->b0;
+	Q->b1 += R->b1;
+	Q->b2 += R->b2;
+	Q->c += R->c;
+	Q->w += R->w;
+}
+
+static void g_add(qgrad* G, const qgrad* R, size_t n)
+{
+	for (size_t k = 0; k < n; ++k)
+	{
+		G[k].gx += R[k].gx;
+		G[k].gy += R[k].gy;
+		G[k].gz += R[k].gz;
+		G[k].gw += R[k].gw;
+	}
+}
+
+static float q_eval(const quadric* Q, vec3 v)
+{
+	float rx = Q->b0;
+	float ry = Q->b1;
+	float rz = Q->b2;
+
+	rx += Q->a10 * v.y;
+	ry += Q->a21 * v.z;
+	rz += Q->a20 * v.x;
+
+	rx *= 2;
+	ry *= 2;
+	rz *= 2;
+
+	rx += Q->a00 * v.x;
+	ry += Q->a11 * v.y;
+	rz += Q->a22 * v.z;
+
+	float r = Q->c;
+	r += rx * v.x;
+	r += ry * v.y;
+	r += rz * v.z;
+	return r;
+}
+
+static float q_error(const quadric* Q, vec3 v)
+{
+	float r = q_eval(Q, v);
+	float s = Q->w == 0.f ? 0.f : 1.f / Q->w;
+	return fabsf(r) * s;
+}
+
+static float q_error_attr(const quadric* Q, const qgrad* G, size_t n, vec3 v, const float* va)
+{
+	float r = q_eval(Q, v);
+
+	for (size_t k = 0; k < n; ++k)
+	{
+		float a = va[k];
+		float g = v.x * G[k].gx + v.y * G[k].gy + v.z * G[k].gz + G[k].gw;
+		r += a * (a * Q->w - 2 * g);
+	}
+
+	return fabsf(r);
+}
+
+static void q_from_plane(quadric* Q, float a, float b, float c, float d, float w)
+{
+	float aw = a * w, bw = b * w, cw = c * w, dw = d * w;
+
+	Q->a00 = a * aw;
+	Q->a11 = b * bw;
+	Q->a22 = c * cw;
+	Q->a10 = a * bw;
+	Q->a20 = a * cw;
+	Q->a21 = b * cw;
+	Q->b0 = a * dw;
+	Q->b1 = b * dw;
+	Q->b2 = c * dw;
+	Q->c = d * dw;
+	Q->w = w;
+}
+
+static void q_from_point(quadric* Q, float x, float y, float z, float w)
+{
+	Q->a00 = Q->a11 = Q->a22 = w;
+	Q->a10 = Q->a20 = Q->a21 = 0;
+	Q->b0 = -x * w;
+	Q->b1 = -y * w;
+	Q->b2 = -z * w;
+	Q->c = (x * x + y * y + z * z) * w;
+	Q->w = w;
+}
+
+static void q_from_triangle(quadric* Q, vec3 p0, vec3 p1, vec3 p2, float weight)
+{
+	vec3 p10 = {p1.x - p0.x, p1.y - p0.y, p1.z - p0.z};
+	vec3 p20 = {p2.x - p0.x, p2.y - p0.y, p2.z - p0.z};
+	vec3 n = {p10.y * p20.z - p10.z * p20.y, p10.z * p20.x - p10.x * p20.z, p10.x * p20.y - p10.y * p20.x};
+	float area = vnormalize(&n);
+	float distance = n.x * p0.x + n.y * p0.y + n.z * p0.z;
+
+	q_from_plane(Q, n.x, n.y, n.z, -distance, sqrtf(area) * weight);
+}
+
+static void q_from_triangle_edge(quadric* Q, vec3 p0, vec3 p1, vec3 p2, float weight)
+{
+	vec3 p10 = {p1.x - p0.x, p1.y - p0.y, p1.z - p0.z};
+	float lengthsq = p10.x * p10.x + p10.y * p10.y + p10.z * p10.z;
+	float length = sqrtf(lengthsq);
+
+	vec3 p20 = {p2.x - p0.x, p2.y - p0.y, p2.z - p0.z};
+	float p20p = p20.x * p10.x + p20.y * p10.y + p20.z * p10.z;
+
+	vec3 perp = {p20.x * lengthsq - p10.x * p20p, p20.y * lengthsq - p10.y * p20p, p20.z * lengthsq - p10.z * p20p};
+	vnormalize(&perp);
+
+	float distance = perp.x * p0.x + perp.y * p0.y + perp.z * p0.z;
+
+	q_from_plane(Q, perp.x, perp.y, perp.z, -distance, length * weight);
+}
+
+static void q_from_attributes(quadric* Q, qgrad* G, vec3 p0, vec3 p1, vec3 p2, const float* va0, const float* va1, const float* va2, size_t n)
+{
+	vec3 p10 = {p1.x - p0.x, p1.y - p0.y, p1.z - p0.z};
+	vec3 p20 = {p2.x - p0.x, p2.y - p0.y, p2.z - p0.z};
+	vec3 nn = {p10.y * p20.z - p10.z * p20.y, p10.z * p20.x - p10.x * p20.z, p10.x * p20.y - p10.y * p20.x};
+	float area = sqrtf(nn.x * nn.x + nn.y * nn.y + nn.z * nn.z) * 0.5f;
+	float w = area;
+
+	vec3 v0 = p10, v1 = p20;
+	float d00 = v0.x * v0.x + v0.y * v0.y + v0.z * v0.z;
+	float d01 = v0.x * v1.x + v0.y * v1.y + v0.z * v1.z;
+	float d11 = v1.x * v1.x + v1.y * v1.y + v1.z * v1.z;
+	float denom = d00 * d11 - d01 * d01;
+	float denomr = denom == 0 ? 0.f : 1.f / denom;
+
+	float gx1 = (d11 * v0.x - d01 * v1.x) * denomr;
+	float gx2 = (d00 * v1.x - d01 * v0.x) * denomr;
+	float gy1 = (d11 * v0.y - d01 * v1.y) * denomr;
+	float gy2 = (d00 * v1.y - d01 * v0.y) * denomr;
+	float gz1 = (d11 * v0.z - d01 * v1.z) * denomr;
+	float gz2 = (d00 * v1.z - d01 * v0.z) * denomr;
+
+	memset(Q, 0, sizeof(*Q));
+	Q->w = w;
+
+	for (size_t k = 0; k < n; ++k)
+	{
+		float a0 = va0[k], a1 = va1[k], a2 = va2[k];
+
+		float gx = gx1 * (a1 - a0) + gx2 * (a2 - a0);
+		float gy = gy1 * (a1 - a0) + gy2 * (a2 - a0);
+		float gz = gz1 * (a1 - a0) + gz2 * (a2 - a0);
+		float gw = a0 - p0.x * gx - p0.y * gy - p0.z * gz;
+
+		Q->a00 += w * (gx * gx);
+		Q->a11 += w * (gy * gy);
+		Q->a22 += w * (gz * gz);
+
+		Q->a10 += w * (gy * gx);
+		Q->a20 += w * (gz * gx);
+		Q->a21 += w * (gz * gy);
+
+		Q->b0 += w * (gx * gw);
+		Q->b1 += w * (gy * gw);
+		Q->b2 += w * (gz * gw);
+
+		Q->c += w * (gw * gw);
+
+		G[k].gx = w * gx;
+		G[k].gy = w * gy;
+		G[k].gz = w * gz;
+		G[k].gw = w * gw;
+	}
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* quadric fills, src/simplifier.cpp:1057-1179 */
+
+static void fill_face_quadrics(quadric* vq, const unsigned int* indices, size_t index_count, const vec3* vp, const unsigned int* remap)
+{
+	for (size_t i = 0; i < index_count; i += 3)
+	{
+		unsigned int i0 = indices[i], i1 = indices[i + 1], i2 = indices[i + 2];
+		quadric Q;
+		q_from_triangle(&Q, vp[i0], vp[i1], vp[i2], 1.f);
+
+		q_add(&vq[remap[i0]], &Q);
+		q_add(&vq[remap[i1]], &Q);
+		q_add(&vq[remap[i2]], &Q);
+	}
+}
+
+static void fill_vertex_quadrics(quadric* vq, const vec3* vp, size_t vertex_count, const unsigned int* remap, const unsigned char* vertex_lock, const unsigned int* sparse, unsigned int options)
+{
+	float factor = (options & OPT_REGULARIZE_LIGHT) ? 1e-2f : ((options & OPT_REGULARIZE) ? 1e-1f : 1e-7f);
+
+	for (size_t i = 0; i < vertex_count; ++i)
+	{
+		if (remap[i] != i)
+			continue;
+
+		unsigned int ri = sparse ? sparse[i] : (unsigned int)i;
+		int priority = vertex_lock && (vertex_lock[ri] & VLOCK_PRIORITY) != 0;
+
+		float w = vq[i].w * (priority ? 1.0f : factor);
+		quadric Q;
+		q_from_point(&Q, vp[i].x, vp[i].y, vp[i].z, w);
+		q_add(&vq[i], &Q);
+	}
+}
+
+static void fill_edge_quadrics(quadric* vq, const unsigned int* indices, size_t index_count, const vec3* vp, const unsigned int* remap, const unsigned char* kind, const unsigned int* loop, const unsigned int* loopback)
+{
+	for (size_t i = 0; i < index_count; i += 3)
+	{
+		for (int e = 0; e < 3; ++e)
+		{
+			unsigned int i0 = indices[i + e];
+			unsigned int i1 = indices[i + (e + 1) % 3];
+			unsigned char k0 = kind[i0], k1 = kind[i1];
+
+			int b0 = (k0 == K_BORDER || k0 == K_SEAM);
+			int b1 = (k1 == K_BORDER || k1 == K_SEAM);
+
+			if (!b0 && !b1)
+				continue;
+			if (b0 && loop[i0] != i1)
+				continue;
+			if (b1 && loopback[i1] != i0)
+				continue;
+
+			unsigned int i2 = indices[i + (e + 2) % 3];
+			float ew = (k0 == K_BORDER || k1 == K_BORDER) ? 10.f : 0.5f;
+
+			quadric Q, QT;
+			q_from_triangle_edge(&Q, vp[i0], vp[i1], vp[i2], ew);
+			q_from_triangle(&QT, vp[i0], vp[i1], vp[i2], ew);
+			QT.w = 0;
+			q_add(&Q, &QT);
+
+			q_add(&vq[remap[i0]], &Q);
+			q_add(&vq[remap[i1]], &Q);
+		}
+	}
+}
+
+static void fill_attribute_quadrics(quadric* aq, qgrad* ag, const unsigned int* indices, size_t index_count, const vec3* vp, const float* va, size_t n)
+{
+	for (size_t i = 0; i < index_count; i += 3)
+	{
+		unsigned int i0 = indices[i], i1 = indices[i + 1], i2 = indices[i + 2];
+		quadric QA;
+		qgrad G[MAX_ATTR];
+		q_from_attributes(&QA, G, vp[i0], vp[i1], vp[i2], &va[(size_t)i0 * n], &va[(size_t)i1 * n], &va[(size_t)i2 * n], n);
+
+		q_add(&aq[i0], &QA);
+		q_add(&aq[i1], &QA);
+		q_add(&aq[i2], &QA);
+		g_add(&ag[(size_t)i0 * n], G, n);
+		g_add(&ag[(size_t)i1 * n], G, n);
+		g_add(&ag[(size_t)i2 * n], G, n);
+	}
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* flip test, src/simplifier.cpp:1182-1232 */
+
+static int tri_flips(vec3 a, vec3 b, vec3 c, vec3 d)
+{
+	vec3 eb = {b.x - a.x, b.y - a.y, b.z - a.z};
+	vec3 ec = {c.x - a.x, c.y - a.y, c.z - a.z};
+	vec3 ed = {d.x - a.x, d.y - a.y, d.z - a.z};
+
+	vec3 nbc = {eb.y * ec.z - eb.z * ec.y, eb.z * ec.x - eb.x * ec.z, eb.x * ec.y - eb.y * ec.x};
+	vec3 nbd = {eb.y * ed.z - eb.z * ed.y, eb.z * ed.x - eb.x * ed.z, eb.x * ed.y - eb.y * ed.x};
+
+	float ndp = nbc.x * nbd.x + nbc.y * nbd.y + nbc.z * nbd.z;
+	float abc = nbc.x * nbc.x + nbc.y * nbc.y + nbc.z * nbc.z;
+	float abd = nbd.x * nbd.x + nbd.y * nbd.y + nbd.z * nbd.z;
+
+	return ndp <= 0.25f * sqrtf(abc * abd);
+}
+
+static int collapse_flips(const corner_csr* adj, const vec3* vp, const unsigned int* cremap, unsigned int i0, unsigned int i1)
+{
+	vec3 v0 = vp[i0], v1 = vp[i1];
+
+	for (unsigned int j = adj->offsets[i0]; j < adj->offsets[i0 + 1]; ++j)
+	{
+		unsigned int a = cremap[adj->next[j]];
+		unsigned int b = cremap[adj->prev[j]];
+
+		if (a == i1 || b == i1 || a == b)
+			continue;
+
+		if (tri_flips(vp[a], vp[b], v0, v1))
+			return 1;
+	}
+
+	return 0;
+}
+
+/* getComplexTarget, src/simplifier.cpp:1278-1290 */
+static unsigned int complex_target(unsigned int v, unsigned int target, const unsigned int* remap, const unsigned int* loop, const unsigned int* loopback)
+{
+	unsigned int r = remap[target];
+
+	if (loop[v] != NONE && remap[loop[v]] == r)
+		return loop[v];
+	else if (loopback[v] != NONE && remap[loopback[v]] == r)
+		return loopback[v];
+	else
+		return target;
+}
+
+/* boundEdgeCollapses, src/simplifier.cpp:1292-1308 */
+static size_t collapse_capacity(const corner_csr* adj, size_t vertex_count, size_t index_count, const unsigned char* kind)
+{
+	size_t dual = 0;
+	for (size_t i = 0; i < vertex_count; ++i)
+		if (kind[i] == K_MANIFOLD || kind[i] == K_SEAM)
+			dual += adj->offsets[i + 1] - adj->offsets[i];
+
+	return (index_count - dual / 2) + 3;
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* candidate list: v0, v1 and the error (or the bidirectional flag before ranking) */
+
+typedef struct
+{
+	unsigned int v0, v1;
+	unsigned int bidi;
+	float error;
+} cand;
+
+/* pickEdgeCollapses, src/simplifier.cpp:1310-1372 */
+static size_t pick_collapses(cand* out, size_t capacity, const unsigned int* indices, size_t index_count, const unsigned int* remap, const unsigned char* kind, const unsigned int* loop, const unsigned int* loopback)
+{
+	size_t count = 0;
+
+	for (size_t i = 0; i < index_count; i += 3)
+	{
+		if (count + 3 > capacity)
+			break;
+
+		for (int e = 0; e < 3; ++e)
+		{
+			unsigned int i0 = indices[i + e];
+			unsigned int i1 = indices[i + (e + 1) % 3];
+
+			if (remap[i0] == remap[i1])
+				continue;
+
+			unsigned char k0 = kind[i0], k1 = kind[i1];
+
+			if (!(can_collapse[k0][k1] | can_collapse[k1][k0]))
+				continue;
+
+			if (has_opposite[k0][k1] && remap[i1] > remap[i0])
+				continue;
+
+			if ((k0 == K_BORDER || k0 == K_SEAM) && k1 != K_MANIFOLD && loop[i0] != i1)
+				continue;
+			if ((k1 == K_BORDER || k1 == K_SEAM) && k0 != K_MANIFOLD && loopback[i1] != i0)
+				continue;
+
+			cand c;
+			if (can_collapse[k0][k1] & can_collapse[k1][k0])
+			{
+				c.v0 = i0;
+				c.v1 = i1;
+				c.bidi = 1;
+			}
+			else
+			{
+				c.v0 = can_collapse[k0][k1] ? i0 : i1;
+				c.v1 = can_collapse[k0][k1] ? i1 : i0;
+				c.bidi = 0;
+			}
+			c.error = 0;
+			out[count++] = c;
+		}
+	}
+
+	return count;
+}
+
+/* rankEdgeCollapses, src/simplifier.cpp:1374-1450 */
+static void rank_collapses(cand* cs, size_t count, const vec3* vp, const float* va, const quadric* vq, const quadric* aq, const qgrad* ag, size_t n, const unsigned int* remap, const unsigned int* wedge, const unsigned char* kind, const unsigned int* loop, const unsigned int* loopback)
+{
+	for (size_t i = 0; i < count; ++i)
+	{
+		cand* c = &cs[i];
+		unsigned int i0 = c->v0, i1 = c->v1;
+		int bidi = c->bidi != 0;
+
+		float ei = q_error(&vq[remap[i0]], vp[i1]);
+		float ej = bidi ? q_error(&vq[remap[i1]], vp[i0]) : FLT_MAX;
+
+		if (n)
+		{
+			ei += q_error_attr(&aq[i0], &ag[(size_t)i0 * n], n, vp[i1], &va[(size_t)i1 * n]);
+			ej += bidi ? q_error_attr(&aq[i1], &ag[(size_t)i1 * n], n, vp[i0], &va[(size_t)i0 * n]) : 0;
+
+			if (kind[i0] == K_SEAM)
+			{
+				unsigned int s0 = wedge[i0];
+				unsigned int s1 = loop[i0] == i1 ? loopback[s0] : loop[s0];
+				s1 = (s1 != NONE) ? s1 : wedge[i1];
+
+				ei += q_error_attr(&aq[s0], &ag[(size_t)s0 * n], n, vp[s1], &va[(size_t)s1 * n]);
+				ej += bidi ? q_error_attr(&aq[s1], &ag[(size_t)s1 * n], n, vp[s0], &va[(size_t)s0 * n]) : 0;
+			}
+			else
+			{
+				if (kind[i0] == K_COMPLEX)
+					for (unsigned int v = wedge[i0]; v != i0; v = wedge[v])
+					{
+						unsigned int t = complex_target(v, i1, remap, loop, loopback);
+						ei += q_error_attr(&aq[v], &ag[(size_t)v * n], n, vp[t], &va[(size_t)t * n]);
+					}
+
+				if (kind[i1] == K_COMPLEX && bidi)
+					for (unsigned int v = wedge[i1]; v != i1; v = wedge[v])
+					{
+						unsigned int t = complex_target(v, i0, remap, loop, loopback);
+						ej += q_error_attr(&aq[v], &ag[(size_t)v * n], n, vp[t], &va[(size_t)t * n]);
+					}
+			}
+		}
+
+		int rev = bidi & (ej < ei);
+		c->v0 = rev ? i1 : i0;
+		c->v1 = rev ? i0 : i1;
+		c->error = ej < ei ? ej : ei;
+	}
+}
+
+/* sortEdgeCollapses key, src/simplifier.cpp:1452-1496: 12 top bits below the sign, clamped */
+static unsigned int sort_key(float e)
+{
+	unsigned int k = (fbits(e) << 1) >> 20;
+	return k < 2560 ? k : 2559;
+}
+
+static void sort_collapses(unsigned int* order, const cand* cs, size_t count)
+{
+	unsigned int hist[2560];
+	memset(hist, 0, sizeof(hist));
+
+	for (size_t i = 0; i < count; ++i)
+		hist[sort_key(cs[i].error)]++;
+
+	unsigned int sum = 0;
+	for (int b = 0; b < 2560; ++b)
+	{
+		unsigned int c = hist[b];
+		hist[b] = sum;
+		sum += c;
+	}
+
+	for (size_t i = 0; i < count; ++i)
+		order[hist[sort_key(cs[i].error)]++] = (unsigned int)i;
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* commit of one collapse, src/simplifier.cpp:1573-1611 */
+
+static void commit_collapse(unsigned int* cremap, const cand* c, const unsigned int* remap, const unsigned int* wedge, const unsigned char* kind, const unsigned int* loop, const unsigned int* loopback)
+{
+	unsigned int i0 = c->v0, i1 = c->v1;
+	unsigned char k = kind[i0];
+
+	if (k == K_COMPLEX)
+	{
+		unsigned int v = i0;
+		do
+		{
+			cremap[v] = complex_target(v, i1, remap, loop, loopback);
+			v = wedge[v];
+		} while (v != i0);
+	}
+	else if (k == K_SEAM)
+	{
+		unsigned int s0 = wedge[i0];
+		unsigned int s1 = loop[i0] == i1 ? loopback[s0] : loop[s0];
+		s1 = (s1 != NONE) ? s1 : wedge[i1];
+
+		cremap[i0] = i1;
+		cremap[s0] = s1;
+	}
+	else
+		cremap[i0] = i1;
+}
+
+/* undo of commit_collapse: every entry it wrote belongs to the source's wedge ring */
+static void revert_collapse(unsigned int* cremap, const cand* c, const unsigned int* wedge)
+{
+	unsigned int v = c->v0;
+	do
+	{
+		cremap[v] = v;
+		v = wedge[v];
+	} while (v != c->v0);
+}
+
+typedef struct
+{
+	size_t committed, evaluated, skipped, rejected, rounds, window;
+	int stop; /* 0 out of collapses, 1 error limit, 2 count limit, 3 error goal */
+	size_t tri_collapses;
+} perform_result;
+
+/* performEdgeCollapses, src/simplifier.cpp:1498-1636 (sequential greedy) */
+static perform_result perform_sequential(unsigned int* cremap, unsigned char* locked, const cand* cs, size_t count, const unsigned int* order, const unsigned int* remap, const unsigned int* wedge, const unsigned char* kind, const unsigned int* loop, const unsigned int* loopback, const vec3* vp, const corner_csr* adj, size_t goal, float error_limit, float* result_error)
+{
+	perform_result pr;
+	memset(&pr, 0, sizeof(pr));
+
+	size_t edge_goal = goal / 2;
+
+	for (size_t i = 0; i < count; ++i)
+	{
+		const cand* c = &cs[order[i]];
+		pr.evaluated++;
+
+		if (c->error > error_limit)
+		{
+			pr.stop = 1;
+			break;
+		}
+
+		if (pr.tri_collapses >= goal)
+		{
+			pr.stop = 2;
+			break;
+		}
+
+		float error_goal = edge_goal < count ? 1.5f * cs[order[edge_goal]].error : FLT_MAX;
+
+		if (c->error > error_goal && c->error > *result_error && pr.tri_collapses > goal / 6)
+		{
+			pr.stop = 3;
+			break;
+		}
+
+		unsigned int r0 = remap[c->v0], r1 = remap[c->v1];
+
+		if (locked[r0] | locked[r1])
+		{
+			pr.skipped++;
+			continue;
+		}
+
+		if (collapse_flips(adj, vp, cremap, r0, r1))
+		{
+			edge_goal++;
+			pr.rejected++;
+			continue;
+		}
+
+		commit_collapse(cremap, c, remap, wedge, kind, loop, loopback);
+
+		locked[r0] = 1;
+		locked[r1] = 1;
+
+		pr.tri_collapses += (kind[c->v0] == K_BORDER) ? 1 : 2;
+		pr.committed++;
+
+		*result_error = *result_error < c->error ? c->error : *result_error;
+	}
+
+	return pr;
+}
+
+/*
+ * Round-based restatement of the same greedy (the algorithm the GPU runs).
+ *
+ * Candidate k (sorted rank) depends only on earlier candidates j < k that share an endpoint
+ * class with it (the lock test) or whose source class is a welded neighbour of k's source
+ * class (the flip test reads collapse_remap only there, and only sources change it; the welded
+ * neighbour relation is symmetric).  In round e every undecided candidate of the rank window
+ * publishes its rank as a per-vertex minimum -- endpoint role at r0 and r1, source role at r0 --
+ * into buffer e&1, tagged with e so that stale entries read as "none".  In round e+1 a candidate
+ * whose endpoints are locked is skipped (a lock can only come from an earlier commit), and a
+ * candidate that is the published minimum at both endpoints and below every source-role entry
+ * in its neighbourhood decides: flip-rejected or committed.  Candidates that decide in the same
+ * round never touch each other's state, so the order of a sweep does not matter; the sweep
+ * below visits them in a shuffled order to stand in for the GPU's concurrency.  The lowest
+ * undecided rank is always ready in the next round, so every round makes progress.
+ * After the window is decided, prefix sums (triangles removed, flip rejects) and a prefix max
+ * (committed error) locate the first rank where the reference loop breaks; commits at or past
+ * it are reverted.  If the window ends before any break, it is doubled.
+ */
+typedef struct
+{
+	uint64_t* pend_any[2];
+	uint64_t* pend_src[2];
+	unsigned int epoch;
+	unsigned int* perm;
+	size_t perm_cap;
+	uint64_t rng;
+} round_state;
+
+static void round_state_init(round_state* rs, size_t vertex_count)
+{
+	for (int b = 0; b < 2; ++b)
+	{
+		rs->pend_any[b] = (uint64_t*)xalloc(vertex_count * sizeof(uint64_t) + 8);
+		rs->pend_src[b] = (uint64_t*)xalloc(vertex_count * sizeof(uint64_t) + 8);
+		memset(rs->pend_any[b], 0xff, vertex_count * sizeof(uint64_t));
+		memset(rs->pend_src[b], 0xff, vertex_count * sizeof(uint64_t));
+	}
+	rs->epoch = 0;
+	rs->perm = NULL;
+	rs->perm_cap = 0;
+	rs->rng = 0x243F6A8885A308D3ull;
+}
+
+static void round_state_free(round_state* rs)
+{
+	for (int b = 0; b < 2; ++b)
+	{
+		free(rs->pend_any[b]);
+		free(rs->pend_src[b]);
+	}
+	free(rs->perm);
+}
+
+static uint64_t rs_next(round_state* rs)
+{
+	rs->rng ^= rs->rng << 13;
+	rs->rng ^= rs->rng >> 7;
+	rs->rng ^= rs->rng << 17;
+	return rs->rng;
+}
+
+static uint64_t pend_key(unsigned int epoch, size_t k)
+{
+	return ((uint64_t)(~epoch) << 32) | (uint64_t)k;
+}
+
+static perform_result perform_rounds(round_state* rs, unsigned int* cremap, unsigned char* locked, const cand* cs, size_t count, const unsigned int* order, const unsigned int* remap, const unsigned int* wedge, const unsigned char* kind, const unsigned int* loop, const unsigned int* loopback, const vec3* vp, const corner_csr* adj, size_t goal, float error_limit, float* result_error, size_t window0)
+{
+	perform_result pr;
+	memset(&pr, 0, sizeof(pr));
+
+	unsigned char* st = (unsigned char*)xalloc(count + 1); /* 0 undecided, 1 skip, 2 reject, 3 commit */
+	memset(st, 0, count + 1);
+
+	if (rs->perm_cap < count)
+	{
+		free(rs->perm);
+		rs->perm = (unsigned int*)xalloc(count * sizeof(unsigned int));
+		rs->perm_cap = count;
+	}
+
+	size_t window = window0 ? window0 : count;
+	if (window > count)
+		window = count;
+
+	size_t lo = 0;
+	size_t stop = count;
+	int stop_reason = 0;
+
+	size_t tri = 0, rejects = 0;
+	float maxerr = *result_error;
+
+	for (;;)
+	{
+		/* shuffled visiting order for this window */
+		size_t span = window - lo;
+		for (size_t i = 0; i < span; ++i)
+			rs->perm[i] = (unsigned int)(lo + i);
+		for (size_t i = span; i > 1; --i)
+		{
+			size_t j = (size_t)(rs_next(rs) % i);
+			unsigned int t = rs->perm[i - 1];
+			rs->perm[i - 1] = rs->perm[j];
+			rs->perm[j] = t;
+		}
+
+		for (;;)
+		{
+			unsigned int e = ++rs->epoch;
+			uint64_t* rd_any = rs->pend_any[(e - 1) & 1];
+			uint64_t* rd_src = rs->pend_src[(e - 1) & 1];
+			uint64_t* wr_any = rs->pend_any[e & 1];
+			uint64_t* wr_src = rs->pend_src[e & 1];
+			uint64_t prev_tag = (uint64_t)(~(e - 1));
+			size_t undecided = 0;
+
+			for (size_t p = 0; p < span; ++p)
+			{
+				size_t k = rs->perm[p];
+				if (st[k])
+					continue;
+
+				const cand* c = &cs[order[k]];
+				unsigned int r0 = remap[c->v0], r1 = remap[c->v1];
+
+				if (locked[r0] | locked[r1])
+				{
+					st[k] = 1;
+					continue;
+				}
+
+				uint64_t mine = pend_key(e - 1, k);
+				int ready = rd_any[r0] == mine && rd_any[r1] == mine;
+
+				for (unsigned int j = adj->offsets[r0]; ready && j < adj->offsets[r0 + 1]; ++j)
+				{
+					uint64_t pa = rd_src[adj->next[j]], pb = rd_src[adj->prev[j]];
+					if ((pa >> 32) == prev_tag && (pa & 0xffffffffu) < k)
+						ready = 0;
+					if ((pb >> 32) == prev_tag && (pb & 0xffffffffu) < k)
+						ready = 0;
+				}
+
+				if (ready)
+				{
+					if (collapse_flips(adj, vp, cremap, r0, r1))
+						st[k] = 2;
+					else
+					{
+						commit_collapse(cremap, c, remap, wedge, kind, loop, loopback);
+						locked[r0] = 1;
+						locked[r1] = 1;
+						st[k] = 3;
+					}
+					continue;
+				}
+
+				uint64_t key = pend_key(e, k);
+				if (wr_any[r0] > key)
+					wr_any[r0] = key;
+				if (wr_any[r1] > key)
+					wr_any[r1] = key;
+				if (wr_src[r0] > key)
+					wr_src[r0] = key;
+				undecided++;
+			}
+
+			pr.rounds++;
+			if (!undecided)
+				break;
+		}
+
+		/* stop scan over [lo, window) */
+		size_t k = lo;
+		for (; k < window; ++k)
+		{
+			const cand* c = &cs[order[k]];
+
+			if (c->error > error_limit)
+			{
+				stop_reason = 1;
+				break;
+			}
+			if (tri >= goal)
+			{
+				stop_reason = 2;
+				break;
+			}
+			size_t eg = goal / 2 + rejects;
+			float error_goal = eg < count ? 1.5f * cs[order[eg]].error : FLT_MAX;
+			if (c->error > error_goal && c->error > maxerr && tri > goal / 6)
+			{
+				stop_reason = 3;
+				break;
+			}
+
+			if (st[k] == 3)
+			{
+				tri += (kind[c->v0] == K_BORDER) ? 1 : 2;
+				maxerr = maxerr < c->error ? c->error : maxerr;
+			}
+			else if (st[k] == 2)
+				rejects++;
+		}
+
+		if (stop_reason || window == count)
+		{
+			stop = k;
+			break;
+		}
+
+		lo = window;
+		window = window * 2 < count ? window * 2 : count;
+	}
+
+	for (size_t k = stop; k < window; ++k)
+		if (st[k] == 3)
+			revert_collapse(cremap, &cs[order[k]], wedge);
+
+	for (size_t k = 0; k < stop; ++k)
+	{
+		pr.committed += st[k] == 3;
+		pr.skipped += st[k] == 1;
+		pr.rejected += st[k] == 2;
+	}
+
+	pr.evaluated = stop_reason ? stop + 1 : stop;
+	pr.stop = stop_reason;
+	pr.tri_collapses = tri;
+	pr.window = window;
+	*result_error = maxerr;
+
+	free(st);
+	return pr;
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* updateQuadrics, src/simplifier.cpp:1638-1673 */
+
+static void update_quadrics(const unsigned int* cremap, size_t vertex_count, quadric* vq, quadric* aq, qgrad* ag, size_t n, const vec3* vp, const unsigned int* remap, float* vertex_error)
+{
+	for (size_t i = 0; i < vertex_count; ++i)
+	{
+		if (cremap[i] == i)
+			continue;
+
+		unsigned int i0 = (unsigned int)i, i1 = cremap[i];
+		unsigned int r0 = remap[i0], r1 = remap[i1];
+
+		if (i0 == r0)
+			q_add(&vq[r1], &vq[r0]);
+
+		if (n)
+		{
+			q_add(&aq[i1], &aq[i0]);
+			g_add(&ag[(size_t)i1 * n], &ag[(size_t)i0 * n], n);
+
+			if (i0 == r0)
+			{
+				float derr = q_error(&vq[r0], vp[r1]);
+				*vertex_error = *vertex_error < derr ? derr : *vertex_error;
+			}
+		}
+	}
+}
+
+/* remapIndexBuffer, src/simplifier.cpp:1811-1843 */
+static size_t remap_index_buffer(unsigned int* indices, size_t index_count, const unsigned int* cremap, const unsigned int* remap)
+{
+	size_t w = 0;
+	for (size_t i = 0; i < index_count; i += 3)
+	{
+		unsigned int v0 = cremap[indices[i]], v1 = cremap[indices[i + 1]], v2 = cremap[indices[i + 2]];
+		unsigned int r0 = remap[v0], r1 = remap[v1], r2 = remap[v2];
+
+		if (r0 != r1 && r0 != r2 && r1 != r2)
+		{
+			indices[w] = v0;
+			indices[w + 1] = v1;
+			indices[w + 2] = v2;
+			w += 3;
+		}
+	}
+	return w;
+}
+
+/* remapEdgeLoops, src/simplifier.cpp:1845-1864 */
+static void remap_edge_loops(unsigned int* loop, size_t vertex_count, const unsigned int* cremap)
+{
+	for (size_t i = 0; i < vertex_count; ++i)
+	{
+		if (loop[i] == NONE)
+			continue;
+
+		unsigned int l = loop[i];
+		unsigned int r = cremap[l];
+
+		if (i == r)
+			loop[i] = (loop[l] != NONE) ? cremap[loop[l]] : NONE;
+		else
+			loop[i] = r;
+	}
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* components for the Prune option, src/simplifier.cpp:1866-2024.
+ * Component ids number the connected components of the welded mesh in order of their lowest
+ * primary vertex; any union-find gives the same partition. */
+
+static unsigned int uf_find(unsigned int* p, unsigned int x)
+{
+	while (p[x] != x)
+	{
+		p[x] = p[p[x]];
+		x = p[x];
+	}
+	return x;
+}
+
+static size_t build_components(unsigned int* comp, size_t vertex_count, const unsigned int* indices, size_t index_count, const unsigned int* remap)
+{
+	unsigned int* parent = (unsigned int*)xalloc(vertex_count * sizeof(unsigned int) + 4);
+	for (size_t i = 0; i < vertex_count; ++i)
+		parent[i] = (unsigned int)i;
+
+	for (size_t i = 0; i < index_count; i += 3)
+		for (int e = 0; e < 3; ++e)
+		{
+			unsigned int a = uf_find(parent, remap[indices[i + e]]);
+			unsigned int b = uf_find(parent, remap[indices[i + (e + 1) % 3]]);
+			if (a != b)
+			{
+				if (a < b)
+					parent[b] = a;
+				else
+					parent[a] = b;
+			}
+		}
+
+	size_t next = 0;
+	for (size_t i = 0; i < vertex_count; ++i)
+	{
+		if (remap[i] == i)
+		{
+			unsigned int root = uf_find(parent, (unsigned int)i);
+			comp[i] = (root == i) ? (unsigned int)next++ : comp[root];
+		}
+		else
+			comp[i] = comp[remap[i]];
+	}
+
+	free(parent);
+	return next;
+}
+
+static void measure_components(float* cerr, size_t ccount, const unsigned int* comp, const vec3* vp, size_t vertex_count)
+{
+	float* acc = (float*)xalloc(ccount * 4 * sizeof(float) + 16);
+	memset(acc, 0, ccount * 4 * sizeof(float));
+
+	for (size_t i = 0; i < vertex_count; ++i)
+	{
+		unsigned int c = comp[i];
+		acc[c * 4 + 0] += vp[i].x;
+		acc[c * 4 + 1] += vp[i].y;
+		acc[c * 4 + 2] += vp[i].z;
+		acc[c * 4 + 3] += 1;
+	}
+
+	for (size_t c = 0; c < ccount; ++c)
+	{
+		float w = acc[c * 4 + 3];
+		float iw = w == 0.f ? 0.f : 1.f / w;
+		acc[c * 4 + 0] *= iw;
+		acc[c * 4 + 1] *= iw;
+		acc[c * 4 + 2] *= iw;
+		acc[c * 4 + 3] = 0;
+	}
+
+	for (size_t i = 0; i < vertex_count; ++i)
+	{
+		unsigned int c = comp[i];
+		float dx = vp[i].x - acc[c * 4 + 0];
+		float dy = vp[i].y - acc[c * 4 + 1];
+		float dz = vp[i].z - acc[c * 4 + 2];
+		float r = dx * dx + dy * dy + dz * dz;
+		acc[c * 4 + 3] = acc[c * 4 + 3] < r ? r : acc[c * 4 + 3];
+	}
+
+	for (size_t c = 0; c < ccount; ++c)
+		cerr[c] = acc[c * 4 + 3];
+
+	free(acc);
+}
+
+static size_t prune_components(unsigned int* indices, size_t index_count, const unsigned int* comp, const float* cerr, float cutoff, float* nexterror)
+{
+	size_t w = 0;
+	float min_error = FLT_MAX;
+
+	for (size_t i = 0; i < index_count; i += 3)
+	{
+		unsigned int v0 = indices[i], v1 = indices[i + 1], v2 = indices[i + 2];
+		unsigned int c = comp[v0];
+
+		if (cerr[c] > cutoff)
+		{
+			min_error = min_error > cerr[c] ? cerr[c] : min_error;
+			indices[w] = v0;
+			indices[w + 1] = v1;
+			indices[w + 2] = v2;
+			w += 3;
+		}
+	}
+
+	*nexterror = min_error;
+	return w;
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* driver, src/simplifier.cpp:2344-2626 */
+
+size_t moracle_simplify_edge(unsigned int* destination, const unsigned int* indices, size_t index_count, const float* vertex_positions, size_t vertex_count, size_t vertex_positions_stride, const float* vertex_attributes, size_t vertex_attributes_stride, const float* attribute_weights, size_t attribute_count, const unsigned char* vertex_lock, size_t target_index_count, float target_error, unsigned int options, float* out_result_error, const moracle_config* config, moracle_stats* stats)
+{
+	int use_rounds = config ? config->perform_rounds : 0;
+	size_t window0 = config ? config->round_window : 0;
+
+	if (stats)
+		memset(stats, 0, sizeof(*stats));
+
+	unsigned int* result = destination;
+	if (result != indices)
+		memmove(result, indices, index_count * sizeof(unsigned int));
+
+	unsigned int* sparse = NULL;
+	if (options & OPT_SPARSE)
+		sparse = sparse_remap_build(result, index_count, vertex_count, &vertex_count);
+
+	size_t stride_f = vertex_positions_stride / sizeof(float);
+
+	corner_csr adj;
+	adj.offsets = (unsigned int*)xalloc((vertex_count + 1) * sizeof(unsigned int));
+	adj.next = (unsigned int*)xalloc(index_count * sizeof(unsigned int) + 4);
+	adj.prev = (unsigned int*)xalloc(index_count * sizeof(unsigned int) + 4);
+	csr_build(&adj, result, index_count, vertex_count, NULL);
+
+	unsigned int* remap = (unsigned int*)xalloc(vertex_count * sizeof(unsigned int) + 4);
+	unsigned int* wedge = (unsigned int*)xalloc(vertex_count * sizeof(unsigned int) + 4);
+	position_remap_build(remap, wedge, vertex_positions, vertex_count, stride_f, sparse);
+
+	unsigned char* kind = (unsigned char*)xalloc(vertex_count + 1);
+	unsigned int* loop = (unsigned int*)xalloc(vertex_count * sizeof(unsigned int) + 4);
+	unsigned int* loopback = (unsigned int*)xalloc(vertex_count * sizeof(unsigned int) + 4);
+	classify(kind, loop, loopback, vertex_count, &adj, remap, wedge, vertex_lock, sparse, options);
+
+	if (stats)
+	{
+		for (size_t i = 0; i < vertex_count; ++i)
+			if (remap[i] == i)
+			{
+				stats->unique_positions++;
+				stats->kinds[kind[i]]++;
+			}
+	}
+
+	vec3* vp = (vec3*)xalloc(vertex_count * sizeof(vec3) + 16);
+	float vertex_scale = rescale_positions(vp, vertex_positions, vertex_count, stride_f, sparse);
+
+	float* va = NULL;
+	size_t n = 0;
+	if (attribute_count)
+	{
+		unsigned int amap[MAX_ATTR];
+		for (size_t k = 0; k < attribute_count; ++k)
+			if (attribute_weights[k] > 0)
+				amap[n++] = (unsigned int)k;
+
+		va = (float*)xalloc(vertex_count * n * sizeof(float) + 4);
+		size_t astride = vertex_attributes_stride / sizeof(float);
+		for (size_t i = 0; i < vertex_count; ++i)
+		{
+			size_t ri = sparse ? sparse[i] : i;
+			for (size_t k = 0; k < n; ++k)
+				va[i * n + k] = vertex_attributes[ri * astride + amap[k]] * attribute_weights[amap[k]];
+		}
+	}
+
+	quadric* vq = (quadric*)xalloc(vertex_count * sizeof(quadric) + 4);
+	memset(vq, 0, vertex_count * sizeof(quadric));
+
+	quadric* aq = NULL;
+	qgrad* ag = NULL;
+	if (n)
+	{
+		aq = (quadric*)xalloc(vertex_count * sizeof(quadric));
+		memset(aq, 0, vertex_count * sizeof(quadric));
+		ag = (qgrad*)xalloc(vertex_count * n * sizeof(qgrad));
+		memset(ag, 0, vertex_count * n * sizeof(qgrad));
+	}
+
+	fill_face_quadrics(vq, result, index_count, vp, remap);
+	fill_vertex_quadrics(vq, vp, vertex_count, remap, vertex_lock, sparse, options);
+	fill_edge_quadrics(vq, result, index_count, vp, remap, kind, loop, loopback);
+	if (n)
+		fill_attribute_quadrics(aq, ag, result, index_count, vp, va, n);
+
+	unsigned int* comp = NULL;
+	float* cerr = NULL;
+	size_t ccount = 0;
+	float cnext = 0;
+
+	if (options & OPT_PRUNE)
+	{
+		comp = (unsigned int*)xalloc(vertex_count * sizeof(unsigned int) + 4);
+		ccount = build_components(comp, vertex_count, result, index_count, remap);
+		cerr = (float*)xalloc(ccount * sizeof(float) + 4);
+		measure_components(cerr, ccount, comp, vp, vertex_count);
+
+		cnext = FLT_MAX;
+		for (size_t c = 0; c < ccount; ++c)
+			cnext = cnext > cerr[c] ? cerr[c] : cnext;
+	}
+
+	size_t capacity = collapse_capacity(&adj, vertex_count, index_count, kind);
+	cand* cs = (cand*)xalloc(capacity * sizeof(cand) + 16);
+	unsigned int* order = (unsigned int*)xalloc(capacity * sizeof(unsigned int) + 4);
+	unsigned int* cremap = (unsigned int*)xalloc(vertex_count * sizeof(unsigned int) + 4);
+	unsigned char* locked = (unsigned char*)xalloc(vertex_count + 1);
+
+	size_t result_count = index_count;
+	float result_error = 0;
+	float vertex_error = 0;
+
+	round_state rs;
+	if (use_rounds)
+		round_state_init(&rs, vertex_count);
+
+	float error_scale = (options & OPT_ERROR_ABSOLUTE) ? vertex_scale : 1.f;
+	float error_limit = (target_error * target_error) / (error_scale * error_scale);
+
+	while (result_count > target_index_count)
+	{
+		csr_build(&adj, result, result_count, vertex_count, remap);
+
+		size_t ccnt = pick_collapses(cs, capacity, result, result_count, remap, kind, loop, loopback);
+		if (ccnt == 0)
+			break;
+
+		rank_collapses(cs, ccnt, vp, va, vq, aq, ag, n, remap, wedge, kind, loop, loopback);
+		sort_collapses(order, cs, ccnt);
+
+		size_t goal = (result_count - target_index_count) / 3;
+
+		for (size_t i = 0; i < vertex_count; ++i)
+			cremap[i] = (unsigned int)i;
+		memset(locked, 0, vertex_count);
+
+		perform_result pr;
+		if (use_rounds)
+			pr = perform_rounds(&rs, cremap, locked, cs, ccnt, order, remap, wedge, kind, loop, loopback, vp, &adj, goal, error_limit, &result_error, window0);
+		else
+			pr = perform_sequential(cremap, locked, cs, ccnt, order, remap, wedge, kind, loop, loopback, vp, &adj, goal, error_limit, &result_error);
+
+		if (stats && stats->passes < MORACLE_MAX_PASSES)
+		{
+			int p = stats->passes;
+			stats->pass[p].candidates = (unsigned int)ccnt;
+			stats->pass[p].evaluated = (unsigned int)pr.evaluated;
+			stats->pass[p].committed = (unsigned int)pr.committed;
+			stats->pass[p].skipped = (unsigned int)pr.skipped;
+			stats->pass[p].rejected = (unsigned int)pr.rejected;
+			stats->pass[p].rounds = (unsigned int)pr.rounds;
+			stats->pass[p].window = (unsigned int)pr.window;
+			stats->pass[p].stop = (unsigned int)pr.stop;
+			stats->pass[p].tri_collapses = (unsigned int)pr.tri_collapses;
+			stats->pass[p].live_triangles = (unsigned int)(result_count / 3);
+		}
+		if (stats)
+			stats->passes++;
+
+		if (pr.committed == 0)
+			break;
+
+		update_quadrics(cremap, vertex_count, vq, aq, ag, n, vp, remap, &vertex_error);
+		vertex_error = n == 0 ? result_error : vertex_error;
+
+		remap_edge_loops(loop, vertex_count, cremap);
+		remap_edge_loops(loopback, vertex_count, cremap);
+
+		result_count = remap_index_buffer(result, result_count, cremap, remap);
+
+		if ((options & OPT_PRUNE) && result_count > target_index_count && cnext <= vertex_error)
+			result_count = prune_components(result, result_count, comp, cerr, vertex_error, &cnext);
+	}
+
+	int next_stale = 1;
+	while ((options & OPT_PRUNE) && result_count > target_index_count && cnext <= error_limit)
+	{
+		float cutoff = cnext * 1.5f < error_limit ? cnext * 1.5f : error_limit;
+
+		float cmax = 0;
+		for (size_t c = 0; c < ccount; ++c)
+			if (cerr[c] > cmax && cerr[c] <= cutoff)
+				cmax = cerr[c];
+
+		size_t new_count = prune_components(result, result_count, comp, cerr, cutoff, &cnext);
+		if (new_count == result_count && !next_stale)
+			break;
+
+		next_stale = 0;
+		result_count = new_count;
+		result_error = result_error < cmax ? cmax : result_error;
+		vertex_error = vertex_error < cmax ? cmax : vertex_error;
+	}
+
+	if ((options & OPT_INTERNAL_DEBUG) && !sparse)
+	{
+		for (size_t i = 0; i < result_count; i += 3)
+		{
+			unsigned int a = result[i], b = result[i + 1], c = result[i + 2];
+			result[i + 0] |= ((unsigned int)kind[a] << 28) | ((unsigned int)(loop[a] == b || loopback[b] == a) << 31);
+			result[i + 1] |= ((unsigned int)kind[b] << 28) | ((unsigned int)(loop[b] == c || loopback[c] == b) << 31);
+			result[i + 2] |= ((unsigned int)kind[c] << 28) | ((unsigned int)(loop[c] == a || loopback[a] == c) << 31);
+		}
+	}
+
+	if (sparse)
+		for (size_t i = 0; i < result_count; ++i)
+			result[i] = sparse[result[i]];
+
+	if (out_result_error)
+		*out_result_error = sqrtf(result_error) * error_scale;
+
+	if (stats)
+	{
+		stats->result_error_sq = result_error;
+		stats->vertex_error_sq = vertex_error;
+	}
+
+	free(adj.offsets);
+	free(adj.next);
+	free(adj.prev);
+	free(remap);
+	free(wedge);
+	free(kind);
+	free(loop);
+	free(loopback);
+	free(vp);
+	free(va);
+	free(vq);
+	free(aq);
+	free(ag);
+	free(comp);
+	free(cerr);
+	free(cs);
+	free(order);
+	free(cremap);
+	free(locked);
+	free(sparse);
+	if (use_rounds)
+		round_state_free(&rs);
+
+	return result_count;
+}
+
+size_t moracle_simplify(unsigned int* destination, const unsigned int* indices, size_t index_count, const float* vertex_positions, size_t vertex_count, size_t vertex_positions_stride, size_t target_index_count, float target_error, unsigned int options, float* result_error)
+{
+	return moracle_simplify_edge(destination, indices, index_count, vertex_positions, vertex_count, vertex_positions_stride, NULL, 0, NULL, 0, NULL, target_index_count, target_error, options, result_error, NULL, NULL);
+}
+
+size_t moracle_simplify_with_attributes(unsigned int* destination, const unsigned int* indices, size_t index_count, const float* vertex_positions, size_t vertex_count, size_t vertex_positions_stride, const float* vertex_attributes, size_t vertex_attributes_stride, const float* attribute_weights, size_t attribute_count, const unsigned char* vertex_lock, size_t target_index_count, float target_error, unsigned int options, float* result_error)
+{
+	return moracle_simplify_edge(destination, indices, index_count, vertex_positions, vertex_count, vertex_positions_stride, vertex_attributes, vertex_attributes_stride, attribute_weights, attribute_count, vertex_lock, target_index_count, target_error, options, result_error, NULL, NULL);
+}
+
+/* meshopt_simplifyScale, src/simplifier.cpp:2938-2948 */
+float moracle_simplify_scale(const float* vertex_positions, size_t vertex_count, size_t vertex_positions_stride)
+{
+	return rescale_positions(NULL, vertex_positions, vertex_count, vertex_positions_stride / sizeof(float), NULL);
+}

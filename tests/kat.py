@@ -1,0 +1,48 @@
+"""Helpers that turn the known-answer fixtures (tests/golden/known_answers.json) into arrays."""
+from __future__ import annotations
+
+import json
+import os
+
+import numpy as np
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def load_cases():
+    with open(os.path.join(GOLDEN, "known_answers.json")) as f:
+        return json.load(f)["cases"]
+
+
+def arrays(case):
+    """Returns dict(indices, positions, attributes, weights, lock) ready for a simplify call.
+    positions/attributes are float32 row views (row stride = vertex / attribute stride)."""
+    v = np.asarray(case["vertices"], dtype=np.float32)
+    positions = v
+    attributes = None
+    a = case.get("attributes")
+    if a is not None:
+        if "interleaved" in a:
+            off = a["interleaved"]
+            attributes = v[:, off:off + a["count"]]
+        else:
+            attributes = np.asarray(a["separate"], dtype=np.float32)
+    weights = None if case.get("weights") is None else np.asarray(case["weights"], dtype=np.float32)
+    lock = None if case.get("lock") is None else np.asarray(case["lock"], dtype=np.uint8)
+    if case["call"] == "simplifyWithAttributes" and attributes is None:
+        attributes = np.zeros((v.shape[0], 0), dtype=np.float32)
+    return dict(indices=np.asarray(case["indices"], dtype=np.uint32), positions=positions,
+                attributes=attributes, weights=weights, lock=lock)
+
+
+def check(case, result, error):
+    """Asserts the reference test's expectations on (result indices, result_error)."""
+    name = case["name"]
+    assert result.size == case["expected_count"], f"{name}: count {result.size} != {case['expected_count']}"
+    if case["expected_indices"] is not None:
+        assert result.tolist() == case["expected_indices"], f"{name}: {result.tolist()} != {case['expected_indices']}"
+    if "error_lt" in case:
+        assert error < case["error_lt"], f"{name}: error {error}"
+    if "error_approx" in case:
+        val, tol = case["error_approx"]
+        assert abs(error - val) < tol, f"{name}: error {error} vs {val}"
