@@ -1,0 +1,64 @@
+/**
+ * meshopt_hip.h -- entry points of libmeshopt_hip.so that have no counterpart in the
+ * reference library: device-resident calls, per-pass statistics and error reporting.
+ * Kept apart from meshoptimizer.h so that the drop-in header stays byte-compatible in shape.
+ */
+#ifndef MESHOPT_HIP_H
+#define MESHOPT_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C"
+{
+#endif
+
+/**
+ * meshopt_simplifyWithAttributes on buffers that already live in device memory (HBM):
+ * destination, indices, vertex_positions, vertex_attributes and vertex_lock are device
+ * pointers; attribute_weights and result_error are host pointers.  The call is synchronous.
+ * Semantics and result are those of meshopt_simplifyWithAttributes (src/meshoptimizer.h:537);
+ * with vertex_attributes == NULL, attribute_count == 0 and vertex_lock == NULL it is
+ * meshopt_simplify (src/meshoptimizer.h:513).
+ * Returns the index count, or (size_t)-1 on a HIP error (see meshopt_hip_last_error).
+ */
+size_t meshopt_hip_simplify_device(unsigned int* destination, const unsigned int* indices, size_t index_count, const float* vertex_positions, size_t vertex_count, size_t vertex_positions_stride, const float* vertex_attributes, size_t vertex_attributes_stride, const float* attribute_weights, size_t attribute_count, const unsigned char* vertex_lock, size_t target_index_count, float target_error, unsigned int options, float* result_error);
+
+/* Per-pass counters of the last call made on the calling thread; the same quantities the
+ * reference prints under TRACE (src/simplifier.cpp:1625-1633). */
+typedef struct meshopt_hip_pass_stats
+{
+	unsigned int live_triangles; /* T_p at the start of the pass */
+	unsigned int candidates;     /* C_p, picked edge collapses */
+	unsigned int evaluated;      /* ranks looked at by the greedy loop, the stopping one included */
+	unsigned int committed;
+	unsigned int skipped;        /* endpoint already locked by a commit */
+	unsigned int rejected;       /* would flip a triangle */
+	unsigned int stop;           /* 0 out of collapses, 1 error limit, 2 count limit, 3 error goal */
+	unsigned int rounds;         /* priority rounds the GPU commit used */
+} meshopt_hip_pass_stats;
+
+typedef struct meshopt_hip_call_stats
+{
+	unsigned int passes;
+	unsigned int vertex_count;     /* after the optional sparse remap */
+	unsigned int unique_positions; /* primaries of the position remap */
+	unsigned int kinds[5];         /* primaries per kind: manifold, border, seam, complex, locked */
+	float result_error_sq;
+	float vertex_error_sq;
+} meshopt_hip_call_stats;
+
+/* Copies up to max_passes pass records of the calling thread's last call; returns the pass count. */
+int meshopt_hip_last_call_stats(meshopt_hip_call_stats* call, meshopt_hip_pass_stats* passes, int max_passes);
+
+/* Text of the last HIP error seen by the calling thread (empty string if none). */
+const char* meshopt_hip_last_error(void);
+
+/* Releases the device arenas and streams held by the library (they are re-created on demand). */
+void meshopt_hip_release(void);
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif
+
+#endif /* MESHOPT_HIP_H */

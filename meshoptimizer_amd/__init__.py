@@ -1,0 +1,188 @@
+"""meshoptimizer_amd -- MI355X (gfx950) implementation of meshoptimizer's quadric
+edge-collapse simplifier behind the reference's own C interface.
+
+The product is the shared library libmeshopt_hip.so (C ABI declared in include/
+meshoptimizer.h and include/meshopt_hip.h).  This module is a thin Python view of that ABI
+with the reference's names and argument meaning:
+
+  simplify(indices, positions, target_index_count, target_error, options)
+      -> meshopt_simplify (src/meshoptimizer.h:513)
+  simplify_with_attributes(indices, positions, attributes, weights, lock, target_index_count,
+                           target_error, options)
+      -> meshopt_simplifyWithAttributes (src/meshoptimizer.h:537)
+  simplify_scale(positions)
+      -> meshopt_simplifyScale (src/meshoptimizer.h:611)
+  simplify_device(...)  device-resident variant (buffers already in HBM)
+
+There is no CPU fallback: if the HIP library is missing the import of the library raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmeshopt_hip.so")
+
+# option bits, src/meshoptimizer.h:466-482
+SimplifyLockBorder = 1 << 0
+SimplifySparse = 1 << 1
+SimplifyErrorAbsolute = 1 << 2
+SimplifyPrune = 1 << 3
+SimplifyRegularize = 1 << 4
+SimplifyPermissive = 1 << 5
+SimplifyRegularizeLight = 1 << 6
+SimplifyInternalDebug = 1 << 30
+
+# vertex_lock flags, src/meshoptimizer.h:487-495
+SimplifyVertex_Lock = 1 << 0
+SimplifyVertex_Protect = 1 << 1
+SimplifyVertex_Priority = 1 << 2
+
+# every symbol the headers in include/ declare
+EXPORTED_SYMBOLS = (
+    "meshopt_simplify",
+    "meshopt_simplifyWithAttributes",
+    "meshopt_simplifyScale",
+    "meshopt_setAllocator",
+    "meshopt_hip_simplify_device",
+    "meshopt_hip_last_call_stats",
+    "meshopt_hip_last_error",
+    "meshopt_hip_release",
+)
+
+
+class PassStats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint) for n in (
+        "live_triangles", "candidates", "evaluated", "committed", "skipped", "rejected", "stop", "rounds")]
+
+
+class CallStats(ctypes.Structure):
+    _fields_ = [
+        ("passes", ctypes.c_uint),
+        ("vertex_count", ctypes.c_uint),
+        ("unique_positions", ctypes.c_uint),
+        ("kinds", ctypes.c_uint * 5),
+        ("result_error_sq", ctypes.c_float),
+        ("vertex_error_sq", ctypes.c_float),
+    ]
+
+
+_P = ctypes.c_void_p
+_SZ = ctypes.c_size_t
+_F = ctypes.c_float
+_U = ctypes.c_uint
+_lib = None
+
+
+def library() -> ctypes.CDLL:
+    """Load libmeshopt_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with `python -m meshoptimizer_amd.build` "
+                          "(the simplifier has no CPU fallback)")
+    lib = ctypes.CDLL(LIB_PATH)
+    lib.meshopt_simplify.restype = _SZ
+    lib.meshopt_simplify.argtypes = [_P, _P, _SZ, _P, _SZ, _SZ, _SZ, _F, _U, _P]
+    lib.meshopt_simplifyWithAttributes.restype = _SZ
+    lib.meshopt_simplifyWithAttributes.argtypes = [_P, _P, _SZ, _P, _SZ, _SZ, _P, _SZ, _P, _SZ, _P, _SZ, _F, _U, _P]
+    lib.meshopt_simplifyScale.restype = _F
+    lib.meshopt_simplifyScale.argtypes = [_P, _SZ, _SZ]
+    lib.meshopt_hip_simplify_device.restype = _SZ
+    lib.meshopt_hip_simplify_device.argtypes = [_P, _P, _SZ, _P, _SZ, _SZ, _P, _SZ, _P, _SZ, _P, _SZ, _F, _U, _P]
+    lib.meshopt_hip_last_call_stats.restype = ctypes.c_int
+    lib.meshopt_hip_last_call_stats.argtypes = [_P, _P, ctypes.c_int]
+    lib.meshopt_hip_last_error.restype = ctypes.c_char_p
+    lib.meshopt_hip_last_error.argtypes = []
+    lib.meshopt_hip_release.restype = None
+    lib.meshopt_hip_release.argtypes = []
+    _lib = lib
+    return lib
+
+
+def _rows(a, name):
+    a = np.asarray(a)
+    if a.dtype != np.float32:
+        raise TypeError(f"{name} must be float32")
+    if a.ndim == 1:
+        a = a.reshape(-1, 3)
+    if a.shape[1] and (a.strides[1] != 4 or a.strides[0] % 4 or a.strides[0] < 4 * a.shape[1]):
+        raise ValueError(f"{name} rows must be packed float32 with a row stride >= their width")
+    return a
+
+
+def simplify(indices, positions, target_index_count, target_error=1e-2, options=0):
+    """meshopt_simplify.  positions: float32 [V, >=3] (row stride = vertex stride).
+    Returns (indices uint32 [n], result_error)."""
+    lib = library()
+    indices = np.ascontiguousarray(indices, dtype=np.uint32)
+    positions = _rows(positions, "positions")
+    dest = np.zeros(max(indices.size, 1), dtype=np.uint32)
+    err = ctypes.c_float(0)
+    n = lib.meshopt_simplify(dest.ctypes.data, indices.ctypes.data, indices.size, positions.ctypes.data,
+                             positions.shape[0], positions.strides[0], int(target_index_count), float(target_error),
+                             int(options), ctypes.byref(err))
+    return dest[:n].copy(), err.value
+
+
+def simplify_with_attributes(indices, positions, attributes, weights, lock, target_index_count,
+                             target_error=1e-2, options=0):
+    """meshopt_simplifyWithAttributes.  attributes: float32 [V, A] view or None; weights [A];
+    lock: uint8 [V] or None.  Returns (indices, result_error)."""
+    lib = library()
+    indices = np.ascontiguousarray(indices, dtype=np.uint32)
+    positions = _rows(positions, "positions")
+    aptr, astride, acount = None, 0, 0
+    if attributes is not None and np.asarray(attributes).size:
+        attributes = _rows(attributes, "attributes")
+        aptr, astride, acount = attributes.ctypes.data, attributes.strides[0], attributes.shape[1]
+    w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float32)
+    lk = None if lock is None else np.ascontiguousarray(lock, dtype=np.uint8)
+    dest = np.zeros(max(indices.size, 1), dtype=np.uint32)
+    err = ctypes.c_float(0)
+    n = lib.meshopt_simplifyWithAttributes(
+        dest.ctypes.data, indices.ctypes.data, indices.size, positions.ctypes.data, positions.shape[0],
+        positions.strides[0], aptr, astride, None if w is None else w.ctypes.data, acount,
+        None if lk is None else lk.ctypes.data, int(target_index_count), float(target_error), int(options),
+        ctypes.byref(err))
+    return dest[:n].copy(), err.value
+
+
+def simplify_scale(positions):
+    """meshopt_simplifyScale."""
+    lib = library()
+    positions = _rows(positions, "positions")
+    return lib.meshopt_simplifyScale(positions.ctypes.data, positions.shape[0], positions.strides[0])
+
+
+def simplify_device(dest_ptr, indices_ptr, index_count, positions_ptr, vertex_count, position_stride,
+                    target_index_count, target_error, options=0, attributes_ptr=None, attribute_stride=0,
+                    weights=None, lock_ptr=None):
+    """meshopt_hip_simplify_device on device pointers (e.g. torch tensor data_ptr()).
+    Returns (index_count, result_error)."""
+    lib = library()
+    w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float32)
+    acount = 0 if w is None else w.size
+    err = ctypes.c_float(0)
+    n = lib.meshopt_hip_simplify_device(
+        dest_ptr, indices_ptr, index_count, positions_ptr, vertex_count, position_stride,
+        attributes_ptr, attribute_stride, None if w is None else w.ctypes.data, acount, lock_ptr,
+        int(target_index_count), float(target_error), int(options), ctypes.byref(err))
+    if n == ctypes.c_size_t(-1).value:
+        raise RuntimeError(lib.meshopt_hip_last_error().decode())
+    return n, err.value
+
+
+def last_call_stats():
+    """(call stats dict, [pass stats dict]) of the calling thread's last simplify call."""
+    lib = library()
+    call = CallStats()
+    passes = (PassStats * 64)()
+    n = lib.meshopt_hip_last_call_stats(ctypes.byref(call), passes, 64)
+    cd = {f: getattr(call, f) for f, _ in CallStats._fields_}
+    cd["kinds"] = list(call.kinds)
+    return cd, [{f: getattr(passes[i], f) for f, _ in PassStats._fields_} for i in range(min(n, 64))]

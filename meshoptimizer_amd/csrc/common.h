@@ -1,0 +1,133 @@
+// Shared definitions for the gfx950 simplifier kernels and their host launchers.
+//
+// Every device file is compiled with -ffp-contract=off (and the pragma below as a second
+// guard): the quadric sums must round exactly like the reference's x86 build, and a fused
+// multiply-add would change the collapse order.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <float.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#pragma clang fp contract(off)
+
+namespace mhip
+{
+
+constexpr uint32_t kNone = 0xffffffffu;
+constexpr int kMaxAttributes = 32;
+
+// vertex kinds (src/simplifier.cpp:296-305)
+enum : uint8_t
+{
+	kManifold = 0,
+	kBorder = 1,
+	kSeam = 2,
+	kComplex = 3,
+	kLocked = 4,
+};
+
+// option bits (src/meshoptimizer.h:466-482, src/simplifier.cpp:2337-2342)
+enum : uint32_t
+{
+	kOptLockBorder = 1u << 0,
+	kOptSparse = 1u << 1,
+	kOptErrorAbsolute = 1u << 2,
+	kOptPrune = 1u << 3,
+	kOptRegularize = 1u << 4,
+	kOptPermissive = 1u << 5,
+	kOptRegularizeLight = 1u << 6,
+	kOptInternalSolve = 1u << 29,
+	kOptInternalDebug = 1u << 30,
+};
+
+// vertex_lock flags (src/meshoptimizer.h:487-495)
+enum : uint8_t
+{
+	kVertexLock = 1,
+	kVertexProtect = 2,
+	kVertexPriority = 4,
+};
+
+// commit status of a ranked candidate during the priority rounds
+enum : uint8_t
+{
+	kUndecided = 0,
+	kSkipped = 1,
+	kRejected = 2,
+	kCommitted = 3,
+};
+
+// 11-float quadric padded to 48 bytes so a gather is three 16-byte loads
+struct __align__(16) Quad
+{
+	float a00, a11, a22, a10;
+	float a20, a21, b0, b1;
+	float b2, c, w, pad;
+};
+
+struct __align__(16) Grad
+{
+	float gx, gy, gz, gw;
+};
+
+// scalars of one simplify call that live on the device
+struct CallScalars
+{
+	float lo[3];          // bounding box minimum (rescale offset)
+	float scale;          // 1 / extent (0 for a flat mesh)
+	float extent;         // largest bounding box side
+	float result_error;   // squared error so far (prefix max of committed collapses)
+	float vertex_error;   // squared positional error (attribute path)
+	uint32_t vertex_error_bits; // atomicMax target for vertex_error (non-negative floats order as uints)
+	uint64_t box_min[3];  // (ordered key << 32 | index) reductions for the bounding box
+	uint64_t box_max[3];
+	uint32_t candidates;  // C_p of the current pass
+	uint32_t triangles;   // live triangles T_p
+	uint32_t next_triangles;
+	uint32_t capacity_tris; // triangles processed by pick before the capacity cut
+	uint32_t committed;   // commits below the stop rank
+	uint32_t stop;        // stop rank of the pass
+	uint32_t stop_reason; // 0 out of collapses, 1 error limit, 2 count limit, 3 error goal
+	uint32_t skipped;
+	uint32_t rejected;
+	uint32_t tri_collapses;
+	uint32_t list_count[3]; // undecided work-list counters of the rounds (rotating)
+	uint32_t dual_count;  // boundEdgeCollapses reduction
+	uint32_t unique_count; // sparse remap: number of referenced vertices
+	uint32_t rounds;
+	uint32_t list_zero;   // third rotating work-list counter (cleared one round ahead)
+	uint32_t work[8];     // small counters of setup kernels
+};
+
+#define HIP_CHECK(expr)                                                                               \
+	do                                                                                                \
+	{                                                                                                 \
+		hipError_t err_ = (expr);                                                                     \
+		if (err_ != hipSuccess)                                                                       \
+			::mhip::fatal_hip(err_, #expr, __FILE__, __LINE__);                                       \
+	} while (0)
+
+[[noreturn]] void fatal_hip(hipError_t err, const char* expr, const char* file, int line);
+
+__device__ __forceinline__ uint32_t float_bits(float f)
+{
+	return __float_as_uint(f);
+}
+
+// order-preserving map of a float to uint32 with +0 and -0 mapped to the same key
+__device__ __forceinline__ uint32_t ordered_key(float f)
+{
+	uint32_t u = __float_as_uint(f);
+	if (u == 0x80000000u)
+		u = 0;
+	return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+inline unsigned int div_up(size_t a, size_t b)
+{
+	return (unsigned int)((a + b - 1) / b);
+}
+
+} // namespace mhip

@@ -1,0 +1,759 @@
+// Host orchestrator and C ABI of libmeshopt_hip.so.
+//
+// One simplify call = the setup kernels, then the pass loop of src/simplifier.cpp:2493-2541
+// run as kernel sequences on one HIP stream, with the per-pass loop conditions read back
+// through a pinned scalar block.  All state lives in one device arena per context.
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/meshopt_hip.h"
+#include "../../include/meshoptimizer.h"
+#include "common.h"
+#include "kernels.h"
+#include "prims.h"
+
+namespace mhip
+{
+
+struct HipError : std::runtime_error
+{
+	using std::runtime_error::runtime_error;
+};
+
+void fatal_hip(hipError_t err, const char* expr, const char* file, int line)
+{
+	char buf[512];
+	snprintf(buf, sizeof(buf), "HIP error %d (%s) in %s at %s:%d", int(err), hipGetErrorString(err), expr, file, line);
+	throw HipError(buf);
+}
+
+// ------------------------------------------------------------------------------------------
+// arena: bump allocation of 256-byte aligned sub-buffers; a dry run measures the footprint
+
+struct Arena
+{
+	char* base = nullptr;
+	size_t size = 0;
+	size_t used = 0;
+
+	template <typename T>
+	T* take(size_t count)
+	{
+		size_t off = (used + 255) & ~size_t(255);
+		used = off + count * sizeof(T) + 16;
+		return base ? reinterpret_cast<T*>(base + off) : nullptr;
+	}
+};
+
+struct PassRecord
+{
+	meshopt_hip_pass_stats s;
+};
+
+struct Context
+{
+	std::mutex mutex;
+	int device = -1;
+	hipStream_t stream = nullptr;
+	char* arena = nullptr;
+	size_t arena_size = 0;
+	CallScalars* sc_host = nullptr; // pinned mirror of the device scalar block
+	std::vector<meshopt_hip_pass_stats> passes;
+	meshopt_hip_call_stats call{};
+	int last_rounds = 16;
+
+	void ensure_init()
+	{
+		if (stream)
+			return;
+		HIP_CHECK(hipGetDevice(&device));
+		HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+		HIP_CHECK(hipHostMalloc((void**)&sc_host, sizeof(CallScalars), hipHostMallocDefault));
+	}
+
+	void ensure_arena(size_t bytes)
+	{
+		if (bytes <= arena_size)
+			return;
+		if (arena)
+			HIP_CHECK(hipFree(arena));
+		arena = nullptr;
+		arena_size = 0;
+		size_t grow = bytes + bytes / 4;
+		HIP_CHECK(hipMalloc((void**)&arena, grow));
+		arena_size = grow;
+	}
+
+	void release()
+	{
+		if (arena)
+			(void)hipFree(arena);
+		if (sc_host)
+			(void)hipHostFree(sc_host);
+		if (stream)
+			(void)hipStreamDestroy(stream);
+		arena = nullptr;
+		arena_size = 0;
+		sc_host = nullptr;
+		stream = nullptr;
+	}
+};
+
+static Context& context()
+{
+	static Context ctx;
+	return ctx;
+}
+
+static thread_local std::string t_last_error;
+static thread_local std::vector<meshopt_hip_pass_stats> t_passes;
+static thread_local meshopt_hip_call_stats t_call{};
+
+// host allocation hooks (meshopt_setAllocator)
+static void* (*g_allocate)(size_t) = nullptr;
+static void (*g_deallocate)(void*) = nullptr;
+
+struct HostBuffer
+{
+	void* p = nullptr;
+	explicit HostBuffer(size_t bytes)
+	{
+		p = g_allocate ? g_allocate(bytes) : ::operator new(bytes);
+	}
+	~HostBuffer()
+	{
+		if (g_deallocate)
+			g_deallocate(p);
+		else
+			::operator delete(p);
+	}
+};
+
+// ------------------------------------------------------------------------------------------
+// buffer plan of one call
+
+struct Plan
+{
+	Mesh m;
+	uint32_t* sp_first;
+	uint32_t* sp_flags;
+	uint32_t* sp_dense;
+	uint32_t* sparse_remap;
+	uint32_t* table;
+	size_t table_size;
+	uint32_t* v_flags;
+	uint32_t* v_a;
+	uint32_t* v_b;
+	uint32_t* v_c;
+	uint32_t* v_d;
+	uint32_t* raw_off;
+	uint32_t* raw_keys;
+	uint32_t* raw_corners;
+	uint32_t* w_off;
+	uint32_t* w_keys;
+	uint32_t* w_corners;
+	uint32_t* i_alt0;
+	uint32_t* i_alt1;
+	uint32_t* kind_counts;
+};
+
+static size_t pow2_at_least(size_t n)
+{
+	size_t p = 1;
+	while (p < n)
+		p <<= 1;
+	return p;
+}
+
+static void carve(Arena& a, Plan& p, size_t V_in, size_t V, size_t I, size_t A, bool sparse)
+{
+	Mesh& m = p.m;
+	size_t T = I / 3 + 1;
+	size_t cap = I + 4;
+	size_t Vp = V + 1;
+
+	m.ib[0] = a.take<uint32_t>(I + 4);
+	m.ib[1] = a.take<uint32_t>(I + 4);
+	p.sp_first = sparse ? a.take<uint32_t>(V_in + 1) : nullptr;
+	p.sp_flags = sparse ? a.take<uint32_t>(I + 1) : nullptr;
+	p.sp_dense = sparse ? a.take<uint32_t>(I + 1) : nullptr;
+	p.sparse_remap = sparse ? a.take<uint32_t>(I + 1) : nullptr;
+
+	m.praw = a.take<float4>(Vp);
+	m.pos = a.take<float4>(Vp);
+	m.remap = a.take<uint32_t>(Vp);
+	m.wedge = a.take<uint32_t>(Vp);
+	m.kind = a.take<uint8_t>(Vp);
+	for (int k = 0; k < 2; ++k)
+	{
+		m.loop[k] = a.take<uint32_t>(Vp);
+		m.loopback[k] = a.take<uint32_t>(Vp);
+	}
+	m.vq = a.take<Quad>(Vp);
+	m.aq = A ? a.take<Quad>(Vp) : nullptr;
+	m.ag = A ? a.take<Grad>(Vp * A) : nullptr;
+	m.va = A ? a.take<float>(Vp * A) : nullptr;
+	m.cremap = a.take<uint32_t>(Vp);
+	m.locked = a.take<uint32_t>(Vp);
+	for (int k = 0; k < 2; ++k)
+	{
+		m.pend_any[k] = a.take<uint64_t>(Vp);
+		m.pend_src[k] = a.take<uint64_t>(Vp);
+	}
+	m.adj_off = a.take<uint32_t>(Vp + 1);
+	m.adj = a.take<uint2>(I + 4);
+	m.adj_cursor = a.take<uint32_t>(Vp);
+
+	m.tri_cnt = a.take<uint32_t>(T);
+	m.tri_off = a.take<uint32_t>(T);
+
+	m.c_v0 = a.take<uint32_t>(cap);
+	m.c_v1 = a.take<uint32_t>(cap);
+	m.c_err = a.take<float>(cap);
+	m.keys = a.take<uint32_t>(cap);
+	m.vals = a.take<uint32_t>(cap);
+	m.keys_alt = a.take<uint32_t>(cap);
+	m.vals_alt = a.take<uint32_t>(cap);
+	m.r_v0 = a.take<uint32_t>(cap);
+	m.r_v1 = a.take<uint32_t>(cap);
+	m.r_r0 = a.take<uint32_t>(cap);
+	m.r_r1 = a.take<uint32_t>(cap);
+	m.r_err = a.take<float>(cap);
+	m.status = a.take<uint8_t>(cap);
+	m.list[0] = a.take<uint32_t>(cap);
+	m.list[1] = a.take<uint32_t>(cap);
+	m.stop_tile = a.take<uint32_t>(3 * (cap / kScanTile + 2));
+
+	// setup-only buffers alias the candidate arrays (not live at the same time)
+	p.table_size = pow2_at_least(2 * V + 2);
+	p.table = a.take<uint32_t>(p.table_size);
+	p.v_flags = a.take<uint32_t>(Vp);
+	p.v_a = a.take<uint32_t>(Vp);
+	p.v_b = a.take<uint32_t>(Vp);
+	p.v_c = a.take<uint32_t>(Vp);
+	p.v_d = a.take<uint32_t>(Vp);
+	p.raw_off = a.take<uint32_t>(Vp + 1);
+	p.raw_keys = m.r_v0;
+	p.raw_corners = m.r_v1;
+	p.w_off = a.take<uint32_t>(Vp + 1);
+	p.w_keys = m.r_r0;
+	p.w_corners = m.r_r1;
+	p.i_alt0 = m.keys_alt;
+	p.i_alt1 = m.vals_alt;
+	p.kind_counts = a.take<uint32_t>(8);
+
+	size_t biggest = std::max(std::max(I, cap), Vp + 1);
+	size_t temp_words = std::max(sort_temp_words(biggest), scan_temp_words(biggest + 16));
+	m.temp = a.take<uint32_t>(temp_words + 64);
+	m.sc = a.take<CallScalars>(1);
+}
+
+// ------------------------------------------------------------------------------------------
+
+__global__ void k_count_kinds(const uint32_t* remap, const uint8_t* kind, size_t n, uint32_t* counts)
+{
+	size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n && remap[i] == (uint32_t)i)
+	{
+		atomicAdd(&counts[kind[i]], 1u);
+		atomicAdd(&counts[5], 1u);
+	}
+}
+
+__global__ void k_scalars_init(CallScalars* sc)
+{
+	sc->result_error = 0.f;
+	sc->vertex_error = 0.f;
+	sc->vertex_error_bits = 0u;
+	sc->candidates = 0;
+	sc->dual_count = 0;
+	sc->work[2] = 0;
+}
+
+static void sync_scalars(Context& ctx, const Mesh& m)
+{
+	HIP_CHECK(hipMemcpyAsync(ctx.sc_host, m.sc, sizeof(CallScalars), hipMemcpyDeviceToHost, ctx.stream));
+	HIP_CHECK(hipStreamSynchronize(ctx.stream));
+}
+
+struct CallArgs
+{
+	unsigned int* destination; // device
+	const unsigned int* indices; // device
+	size_t index_count;
+	const float* positions; // device
+	size_t vertex_count;
+	size_t position_stride;
+	const float* attributes; // device
+	size_t attribute_stride;
+	const float* weights; // host
+	size_t attribute_count;
+	const unsigned char* lock; // device
+	size_t target_index_count;
+	float target_error;
+	unsigned int options;
+};
+
+// the whole simplify call on device buffers; returns the index count, sets *result_error_sq
+static size_t simplify_on_device(Context& ctx, const CallArgs& c, float* result_error_out)
+{
+	hipStream_t s = ctx.stream;
+	const bool sparse = (c.options & kOptSparse) != 0;
+	const size_t I = c.index_count;
+	const size_t V_in = c.vertex_count;
+
+	uint32_t amap[kMaxAttributes];
+	float aw[kMaxAttributes];
+	size_t A = 0;
+	for (size_t k = 0; k < c.attribute_count && k < (size_t)kMaxAttributes; ++k)
+		if (c.weights[k] > 0)
+		{
+			amap[A] = (uint32_t)k;
+			aw[A] = c.weights[k];
+			A++;
+		}
+
+	size_t V_alloc = sparse ? std::min(V_in, I) : V_in;
+
+	Arena dry;
+	Plan plan;
+	carve(dry, plan, V_in, V_alloc, I, A, sparse);
+	ctx.ensure_arena(dry.used + 4096);
+	Arena arena;
+	arena.base = ctx.arena;
+	arena.size = ctx.arena_size;
+	carve(arena, plan, V_in, V_alloc, I, A, sparse);
+	Mesh& m = plan.m;
+
+	m.index_count = (uint32_t)I;
+	m.attr_count = (uint32_t)A;
+	m.ib_cur = 0;
+	m.loop_cur = 0;
+	m.epoch = 0;
+
+	k_scalars_init<<<1, 1, 0, s>>>(m.sc);
+	if (I)
+	{
+		HIP_CHECK(hipMemcpyAsync(m.ib[0], c.indices, I * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+		// every later gather trusts the indices: reject out-of-range ones before any kernel uses them
+		launch_check_indices(s, m.ib[0], I, V_in, &m.sc->work[2]);
+		sync_scalars(ctx, m);
+		if (ctx.sc_host->work[2])
+			throw std::invalid_argument("index buffer references a vertex >= vertex_count");
+	}
+
+	size_t V = V_in;
+	const uint32_t* sparse_remap = nullptr;
+	if (sparse)
+	{
+		launch_sparse_remap(s, m.ib[0], m.ib[0], I, V_in, plan.sp_first, plan.sp_flags, plan.sp_dense, plan.sparse_remap, &m.sc->unique_count, m.temp);
+		sync_scalars(ctx, m);
+		V = I ? ctx.sc_host->unique_count : 0;
+		sparse_remap = plan.sparse_remap;
+	}
+	m.vertex_count = (uint32_t)V;
+
+	launch_gather_positions(s, c.positions, c.position_stride / sizeof(float), sparse_remap, m.praw, V);
+	launch_position_remap(s, m, plan.table, pow2_at_least(2 * V + 2));
+	launch_wedges(s, m, plan.v_flags, plan.v_a, plan.v_b, plan.v_c, plan.v_d);
+
+	launch_corner_csr(s, m.ib[0], I, nullptr, V, plan.raw_off, plan.raw_keys, plan.raw_corners, plan.i_alt0, plan.i_alt1, m.temp);
+	launch_classify(s, m, plan.raw_off, plan.raw_corners, c.lock, sparse_remap, c.options);
+
+	launch_rescale(s, m);
+	if (A)
+		launch_rescale_attributes(s, m, c.attributes, c.attribute_stride / sizeof(float), sparse_remap, amap, aw);
+
+	launch_corner_csr(s, m.ib[0], I, m.remap, V, plan.w_off, plan.w_keys, plan.w_corners, plan.i_alt0, plan.i_alt1, m.temp);
+	HIP_CHECK(hipMemsetAsync(m.vq, 0, (V + 1) * sizeof(Quad), s));
+	launch_vertex_quadrics(s, m, plan.w_off, plan.w_corners, c.lock, sparse_remap, c.options);
+	if (A)
+		launch_attribute_quadrics(s, m, plan.raw_off, plan.raw_corners);
+
+	// attribute quadrics read the raw corner list; everything after may reuse its space
+	launch_dual_count(s, m, plan.raw_off);
+	HIP_CHECK(hipMemsetAsync(plan.kind_counts, 0, 8 * sizeof(uint32_t), s));
+	if (V)
+		k_count_kinds<<<div_up(V, 256), 256, 0, s>>>(m.remap, m.kind, V, plan.kind_counts);
+	uint32_t kind_counts[8];
+	HIP_CHECK(hipMemcpyAsync(kind_counts, plan.kind_counts, sizeof(kind_counts), hipMemcpyDeviceToHost, s));
+
+	// round minima start as "no entry" (tag of epoch 0, rank all-ones)
+	HIP_CHECK(hipMemsetAsync(m.pend_any[0], 0xff, (V + 1) * sizeof(uint64_t), s));
+	HIP_CHECK(hipMemsetAsync(m.pend_any[1], 0xff, (V + 1) * sizeof(uint64_t), s));
+	HIP_CHECK(hipMemsetAsync(m.pend_src[0], 0xff, (V + 1) * sizeof(uint64_t), s));
+	HIP_CHECK(hipMemsetAsync(m.pend_src[1], 0xff, (V + 1) * sizeof(uint64_t), s));
+
+	sync_scalars(ctx, m);
+	const CallScalars& h = *ctx.sc_host;
+
+	t_call = meshopt_hip_call_stats{};
+	t_call.vertex_count = (unsigned int)V;
+	t_call.unique_positions = kind_counts[5];
+	for (int k = 0; k < 5; ++k)
+		t_call.kinds[k] = kind_counts[k];
+	t_passes.clear();
+
+	size_t capacity = (I - h.dual_count / 2) + 3;
+	m.capacity = (uint32_t)std::min<size_t>(capacity, 0xffffffffu);
+
+	float vertex_scale = h.extent;
+	float error_scale = (c.options & kOptErrorAbsolute) ? vertex_scale : 1.f;
+	float error_limit = (c.target_error * c.target_error) / (error_scale * error_scale);
+
+	size_t result_count = I;
+	float result_error_sq = 0.f;
+
+	while (result_count > c.target_index_count)
+	{
+		size_t T = result_count / 3;
+		launch_adjacency(s, m, T);
+		launch_pick(s, m, T);
+		sync_scalars(ctx, m);
+		size_t C = h.candidates;
+		if (C == 0)
+			break;
+
+		launch_rank_sort(s, m, C);
+		uint32_t goal = (uint32_t)((result_count - c.target_index_count) / 3);
+		launch_perform_init(s, m, C);
+
+		int round = 0;
+		size_t bound = C;
+		int batch = std::max(ctx.last_rounds + 4, 8);
+		for (;;)
+		{
+			for (int r = 0; r < batch; ++r, ++round)
+				launch_round(s, m, bound, round);
+			launch_stop_scan(s, m, C, goal, error_limit);
+			sync_scalars(ctx, m);
+			uint32_t left = h.list_count[round % 3];
+			if (left == 0)
+				break;
+			// some ranks are still undecided: the stop scan above saw an unfinished window
+			bound = left;
+			batch = 8;
+		}
+		launch_stop_commit(s, m, C, goal, error_limit);
+		ctx.last_rounds = round;
+
+		meshopt_hip_pass_stats ps{};
+		ps.live_triangles = (unsigned int)T;
+		ps.candidates = (unsigned int)C;
+		ps.stop = h.stop_reason;
+		ps.evaluated = h.stop_reason ? h.stop + 1 : h.stop;
+		ps.committed = h.committed;
+		ps.skipped = h.skipped;
+		ps.rejected = h.rejected;
+		ps.rounds = (unsigned int)round;
+		t_passes.push_back(ps);
+
+		if (h.committed == 0)
+			break;
+
+		launch_apply(s, m, C);
+		launch_remap_triangles(s, m, T);
+		sync_scalars(ctx, m);
+		result_count = (size_t)h.next_triangles * 3;
+	}
+
+	// the stop scan advanced result_error even in a pass that broke off without commits
+	sync_scalars(ctx, m);
+	result_error_sq = h.result_error;
+
+	if ((c.options & kOptInternalDebug) && !sparse)
+		launch_debug_bits(s, m, result_count);
+	if (sparse)
+		launch_sparse_backmap(s, m.ib[m.ib_cur], result_count, sparse_remap);
+	if (result_count)
+		HIP_CHECK(hipMemcpyAsync(c.destination, m.ib[m.ib_cur], result_count * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+	sync_scalars(ctx, m);
+
+	t_call.passes = (unsigned int)t_passes.size();
+	t_call.result_error_sq = result_error_sq;
+	t_call.vertex_error_sq = h.vertex_error;
+
+	if (result_error_out)
+		*result_error_out = sqrtf(result_error_sq) * error_scale;
+	return result_count;
+}
+
+// ------------------------------------------------------------------------------------------
+// host-pointer entry: stage the used spans of the caller's buffers in device memory
+
+struct DeviceStaging
+{
+	std::vector<void*> ptrs;
+	~DeviceStaging()
+	{
+		for (void* p : ptrs)
+			(void)hipFree(p);
+	}
+	void* alloc(size_t bytes)
+	{
+		void* p = nullptr;
+		HIP_CHECK(hipMalloc(&p, bytes ? bytes : 16));
+		ptrs.push_back(p);
+		return p;
+	}
+};
+
+struct StagingCache
+{
+	void* buf = nullptr;
+	size_t size = 0;
+	void* get(size_t bytes)
+	{
+		if (bytes > size)
+		{
+			if (buf)
+				HIP_CHECK(hipFree(buf));
+			buf = nullptr;
+			size = 0;
+			HIP_CHECK(hipMalloc(&buf, bytes + bytes / 4 + 256));
+			size = bytes + bytes / 4 + 256;
+		}
+		return buf;
+	}
+};
+
+static StagingCache& staging()
+{
+	static StagingCache s;
+	return s;
+}
+
+static size_t simplify_host(const unsigned int* indices, unsigned int* destination, size_t index_count, const float* positions, size_t vertex_count,
+                            size_t position_stride, const float* attributes, size_t attribute_stride, const float* weights, size_t attribute_count,
+                            const unsigned char* lock, size_t target_index_count, float target_error, unsigned int options, float* result_error)
+{
+	Context& ctx = context();
+	std::lock_guard<std::mutex> guard(ctx.mutex);
+	ctx.ensure_init();
+	hipStream_t s = ctx.stream;
+
+	size_t pos_span = vertex_count ? (vertex_count - 1) * position_stride + 12 : 0;
+	const char* pos_lo = reinterpret_cast<const char*>(positions);
+	bool attr_inside = attributes && attribute_count && reinterpret_cast<const char*>(attributes) >= pos_lo &&
+	                   reinterpret_cast<const char*>(attributes) + (vertex_count ? (vertex_count - 1) * attribute_stride + attribute_count * 4 : 0) <=
+	                       pos_lo + (vertex_count ? (vertex_count - 1) * position_stride + position_stride : 0);
+	size_t pos_bytes = attr_inside ? vertex_count * position_stride : pos_span;
+	size_t attr_span = (attributes && attribute_count && vertex_count && !attr_inside) ? (vertex_count - 1) * attribute_stride + attribute_count * 4 : 0;
+	size_t lock_bytes = lock ? vertex_count : 0;
+
+	size_t off_idx = 0;
+	size_t off_dst = (off_idx + index_count * 4 + 255) & ~size_t(255);
+	size_t off_pos = (off_dst + index_count * 4 + 255) & ~size_t(255);
+	size_t off_attr = (off_pos + pos_bytes + 255) & ~size_t(255);
+	size_t off_lock = (off_attr + attr_span + 255) & ~size_t(255);
+	size_t total = off_lock + lock_bytes + 256;
+
+	char* d = static_cast<char*>(staging().get(total));
+	if (index_count)
+		HIP_CHECK(hipMemcpyAsync(d + off_idx, indices, index_count * 4, hipMemcpyHostToDevice, s));
+	if (pos_bytes)
+		HIP_CHECK(hipMemcpyAsync(d + off_pos, positions, pos_bytes, hipMemcpyHostToDevice, s));
+	const float* d_attr = nullptr;
+	if (attr_inside)
+		d_attr = reinterpret_cast<const float*>(d + off_pos + (reinterpret_cast<const char*>(attributes) - pos_lo));
+	else if (attr_span)
+	{
+		HIP_CHECK(hipMemcpyAsync(d + off_attr, attributes, attr_span, hipMemcpyHostToDevice, s));
+		d_attr = reinterpret_cast<const float*>(d + off_attr);
+	}
+	if (lock_bytes)
+		HIP_CHECK(hipMemcpyAsync(d + off_lock, lock, lock_bytes, hipMemcpyHostToDevice, s));
+
+	CallArgs c;
+	c.destination = reinterpret_cast<unsigned int*>(d + off_dst);
+	c.indices = reinterpret_cast<const unsigned int*>(d + off_idx);
+	c.index_count = index_count;
+	c.positions = reinterpret_cast<const float*>(d + off_pos);
+	c.vertex_count = vertex_count;
+	c.position_stride = position_stride;
+	c.attributes = d_attr;
+	c.attribute_stride = attribute_stride;
+	c.weights = weights;
+	c.attribute_count = attribute_count;
+	c.lock = lock_bytes ? reinterpret_cast<const unsigned char*>(d + off_lock) : nullptr;
+	c.target_index_count = target_index_count;
+	c.target_error = target_error;
+	c.options = options;
+
+	size_t n = simplify_on_device(ctx, c, result_error);
+	if (n)
+		HIP_CHECK(hipMemcpyAsync(destination, c.destination, n * 4, hipMemcpyDeviceToHost, s));
+	HIP_CHECK(hipStreamSynchronize(s));
+	return n;
+}
+
+static void check_args(size_t index_count, size_t stride, size_t target, float target_error, size_t attr_stride, size_t attr_count, const float* weights)
+{
+	// preconditions of src/simplifier.cpp:2348-2357 (asserts in the reference)
+	if (index_count % 3 != 0 || stride < 12 || stride > 256 || stride % 4 != 0 || target > index_count || !(target_error >= 0) ||
+	    attr_count > (size_t)kMaxAttributes || (attr_count && (attr_stride < attr_count * 4 || attr_stride > 256 || attr_stride % 4 != 0)))
+		throw std::invalid_argument("meshopt_simplify: invalid arguments");
+	for (size_t k = 0; k < attr_count; ++k)
+		if (!(weights[k] >= 0))
+			throw std::invalid_argument("meshopt_simplify: negative attribute weight");
+}
+
+} // namespace mhip
+
+using namespace mhip;
+
+[[noreturn]] static void die(const char* what)
+{
+	fprintf(stderr, "libmeshopt_hip: %s\n", what);
+	fflush(stderr);
+	abort();
+}
+
+extern "C" size_t meshopt_simplify(unsigned int* destination, const unsigned int* indices, size_t index_count, const float* vertex_positions, size_t vertex_count,
+                                   size_t vertex_positions_stride, size_t target_index_count, float target_error, unsigned int options, float* result_error)
+{
+	try
+	{
+		check_args(index_count, vertex_positions_stride, target_index_count, target_error, 0, 0, nullptr);
+		if (options & kOptInternalSolve)
+			throw std::invalid_argument("meshopt_simplify: the solve bit is not accepted here");
+		return simplify_host(indices, destination, index_count, vertex_positions, vertex_count, vertex_positions_stride, nullptr, 0, nullptr, 0,
+		                     nullptr, target_index_count, target_error, options, result_error);
+	}
+	catch (const std::exception& e)
+	{
+		die(e.what());
+	}
+}
+
+extern "C" size_t meshopt_simplifyWithAttributes(unsigned int* destination, const unsigned int* indices, size_t index_count, const float* vertex_positions,
+                                                 size_t vertex_count, size_t vertex_positions_stride, const float* vertex_attributes,
+                                                 size_t vertex_attributes_stride, const float* attribute_weights, size_t attribute_count,
+                                                 const unsigned char* vertex_lock, size_t target_index_count, float target_error, unsigned int options,
+                                                 float* result_error)
+{
+	try
+	{
+		check_args(index_count, vertex_positions_stride, target_index_count, target_error, vertex_attributes_stride, attribute_count, attribute_weights);
+		if (options & kOptInternalSolve)
+			throw std::invalid_argument("meshopt_simplifyWithAttributes: the solve bit is not accepted here");
+		return simplify_host(indices, destination, index_count, vertex_positions, vertex_count, vertex_positions_stride, vertex_attributes,
+		                     vertex_attributes_stride, attribute_weights, attribute_count, vertex_lock, target_index_count, target_error, options,
+		                     result_error);
+	}
+	catch (const std::exception& e)
+	{
+		die(e.what());
+	}
+}
+
+extern "C" size_t meshopt_hip_simplify_device(unsigned int* destination, const unsigned int* indices, size_t index_count, const float* vertex_positions,
+                                              size_t vertex_count, size_t vertex_positions_stride, const float* vertex_attributes,
+                                              size_t vertex_attributes_stride, const float* attribute_weights, size_t attribute_count,
+                                              const unsigned char* vertex_lock, size_t target_index_count, float target_error, unsigned int options,
+                                              float* result_error)
+{
+	try
+	{
+		check_args(index_count, vertex_positions_stride, target_index_count, target_error, vertex_attributes_stride, attribute_count, attribute_weights);
+		Context& ctx = context();
+		std::lock_guard<std::mutex> guard(ctx.mutex);
+		ctx.ensure_init();
+		CallArgs c;
+		c.destination = destination;
+		c.indices = indices;
+		c.index_count = index_count;
+		c.positions = vertex_positions;
+		c.vertex_count = vertex_count;
+		c.position_stride = vertex_positions_stride;
+		c.attributes = attribute_count ? vertex_attributes : nullptr;
+		c.attribute_stride = vertex_attributes_stride;
+		c.weights = attribute_weights;
+		c.attribute_count = attribute_count;
+		c.lock = vertex_lock;
+		c.target_index_count = target_index_count;
+		c.target_error = target_error;
+		c.options = options & ~(unsigned int)kOptInternalSolve;
+		size_t n = simplify_on_device(ctx, c, result_error);
+		t_last_error.clear();
+		return n;
+	}
+	catch (const std::exception& e)
+	{
+		t_last_error = e.what();
+		return (size_t)-1;
+	}
+}
+
+extern "C" float meshopt_simplifyScale(const float* vertex_positions, size_t vertex_count, size_t vertex_positions_stride)
+{
+	try
+	{
+		Context& ctx = context();
+		std::lock_guard<std::mutex> guard(ctx.mutex);
+		ctx.ensure_init();
+		hipStream_t s = ctx.stream;
+		size_t span = vertex_count ? (vertex_count - 1) * vertex_positions_stride + 12 : 0;
+		size_t off_p = (span + 255) & ~size_t(255);
+		size_t off_sc = (off_p + (vertex_count + 1) * sizeof(float4) + 255) & ~size_t(255);
+		char* d = static_cast<char*>(staging().get(off_sc + sizeof(CallScalars) + 256));
+		if (span)
+			HIP_CHECK(hipMemcpyAsync(d, vertex_positions, span, hipMemcpyHostToDevice, s));
+		Mesh m{};
+		m.praw = reinterpret_cast<float4*>(d + off_p);
+		m.sc = reinterpret_cast<CallScalars*>(d + off_sc);
+		m.vertex_count = (uint32_t)vertex_count;
+		launch_gather_positions(s, reinterpret_cast<const float*>(d), vertex_positions_stride / 4, nullptr, m.praw, vertex_count);
+		launch_extent_only(s, m);
+		HIP_CHECK(hipMemcpyAsync(ctx.sc_host, m.sc, sizeof(CallScalars), hipMemcpyDeviceToHost, s));
+		HIP_CHECK(hipStreamSynchronize(s));
+		return ctx.sc_host->extent;
+	}
+	catch (const std::exception& e)
+	{
+		die(e.what());
+	}
+}
+
+extern "C" void meshopt_setAllocator(void* (*allocate)(size_t), void (*deallocate)(void*))
+{
+	g_allocate = allocate;
+	g_deallocate = deallocate;
+}
+
+extern "C" int meshopt_hip_last_call_stats(meshopt_hip_call_stats* call, meshopt_hip_pass_stats* passes, int max_passes)
+{
+	if (call)
+		*call = t_call;
+	int n = (int)t_passes.size();
+	for (int i = 0; i < n && i < max_passes && passes; ++i)
+		passes[i] = t_passes[i];
+	return n;
+}
+
+extern "C" const char* meshopt_hip_last_error(void)
+{
+	return t_last_error.c_str();
+}
+
+extern "C" void meshopt_hip_release(void)
+{
+	Context& ctx = context();
+	std::lock_guard<std::mutex> guard(ctx.mutex);
+	ctx.release();
+	StagingCache& st = staging();
+	if (st.buf)
+		(void)hipFree(st.buf);
+	st.buf = nullptr;
+	st.size = 0;
+}

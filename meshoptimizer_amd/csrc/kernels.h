@@ -1,0 +1,101 @@
+// Host-side launchers for the simplifier's device kernels (setup.hip, pass.hip).
+#pragma once
+
+#include "common.h"
+
+namespace mhip
+{
+
+// device state of one simplify call; every pointer is carved from one arena
+struct Mesh
+{
+	uint32_t vertex_count;   // V (after the optional sparse remap)
+	uint32_t index_count;    // I of the input
+	uint32_t attr_count;     // attributes with weight > 0
+	uint32_t capacity;       // collapse capacity (boundEdgeCollapses)
+
+	float4* praw;            // input positions (x, y, z, 0)
+	float4* pos;             // rescaled positions
+	uint32_t* remap;         // position remap: lowest index with equal xyz
+	uint32_t* wedge;         // cyclic wedge list
+	uint8_t* kind;
+	uint32_t* loop[2];       // loop / loopback (edge loops), ping-pong for remapEdgeLoops
+	uint32_t* loopback[2];
+	int loop_cur;
+
+	Quad* vq;                // vertex quadrics (primaries)
+	Quad* aq;                // attribute quadrics (per wedge)
+	Grad* ag;                // attribute gradients (V x A)
+	float* va;               // weighted attributes (V x A)
+
+	uint32_t* cremap;        // collapse_remap
+	uint32_t* locked;        // collapse_locked
+	uint64_t* pend_any[2];   // round minima, endpoint role
+	uint64_t* pend_src[2];   // round minima, source role
+	uint32_t epoch;
+
+	uint32_t* adj_off;       // V + 1: welded corner CSR offsets of the current pass
+	uint2* adj;              // I: (next, prev) per corner
+	uint32_t* adj_cursor;    // V
+
+	uint32_t* ib[2];         // index buffer ping-pong
+	int ib_cur;
+
+	uint32_t* tri_cnt;       // per-triangle counts (pick / remap compaction)
+	uint32_t* tri_off;
+
+	uint32_t* c_v0;          // candidates in pick order
+	uint32_t* c_v1;
+	float* c_err;
+	uint32_t* keys;          // sort buffers
+	uint32_t* vals;
+	uint32_t* keys_alt;
+	uint32_t* vals_alt;
+
+	uint32_t* r_v0;          // candidates in rank order
+	uint32_t* r_v1;
+	uint32_t* r_r0;          // remap[r_v0]: source class
+	uint32_t* r_r1;          // remap[r_v1]: target class
+	float* r_err;
+	uint8_t* status;
+	uint32_t* list[2];       // undecided work lists
+	uint32_t* stop_tile;     // per-tile aggregates of the stop scan (3 words each)
+
+	uint32_t* temp;          // scan / sort temporaries
+	CallScalars* sc;
+};
+
+// ---------------------------------------------------------------- setup (setup.hip)
+void launch_check_indices(hipStream_t s, const uint32_t* ib, size_t index_count, size_t vertex_count, uint32_t* flag);
+void launch_sparse_remap(hipStream_t s, const uint32_t* indices_in, uint32_t* indices_out, size_t index_count, size_t vertex_count,
+                         uint32_t* first, uint32_t* flags, uint32_t* dense, uint32_t* sparse_remap, uint32_t* unique_count, uint32_t* temp);
+void launch_gather_positions(hipStream_t s, const float* positions, size_t stride_floats, const uint32_t* sparse_remap, float4* out, size_t vertex_count);
+void launch_position_remap(hipStream_t s, Mesh& m, uint32_t* table, size_t table_size);
+void launch_wedges(hipStream_t s, Mesh& m, uint32_t* flags, uint32_t* scratch_a, uint32_t* scratch_b, uint32_t* scratch_c, uint32_t* scratch_d);
+void launch_corner_csr(hipStream_t s, const uint32_t* ib, size_t index_count, const uint32_t* remap, size_t vertex_count,
+                       uint32_t* offsets, uint32_t* keys, uint32_t* corners, uint32_t* keys_alt, uint32_t* vals_alt, uint32_t* temp);
+void launch_classify(hipStream_t s, Mesh& m, const uint32_t* raw_off, const uint32_t* raw_corners, const uint8_t* vertex_lock,
+                     const uint32_t* sparse_remap, uint32_t options);
+void launch_rescale(hipStream_t s, Mesh& m);
+void launch_extent_only(hipStream_t s, Mesh& m);
+void launch_rescale_attributes(hipStream_t s, Mesh& m, const float* attributes, size_t stride_floats, const uint32_t* sparse_remap,
+                               const uint32_t* attr_index, const float* attr_weight);
+void launch_vertex_quadrics(hipStream_t s, Mesh& m, const uint32_t* welded_off, const uint32_t* welded_corners, const uint8_t* vertex_lock,
+                            const uint32_t* sparse_remap, uint32_t options);
+void launch_attribute_quadrics(hipStream_t s, Mesh& m, const uint32_t* raw_off, const uint32_t* raw_corners);
+void launch_dual_count(hipStream_t s, Mesh& m, const uint32_t* raw_off);
+
+// ---------------------------------------------------------------- passes (pass.hip)
+void launch_adjacency(hipStream_t s, Mesh& m, size_t tri_count);
+void launch_pick(hipStream_t s, Mesh& m, size_t tri_count);
+void launch_rank_sort(hipStream_t s, Mesh& m, size_t cand_count);
+void launch_perform_init(hipStream_t s, Mesh& m, size_t cand_count);
+void launch_round(hipStream_t s, Mesh& m, size_t list_bound, int parity);
+void launch_stop_scan(hipStream_t s, Mesh& m, size_t cand_count, uint32_t goal, float error_limit);
+void launch_stop_commit(hipStream_t s, Mesh& m, size_t cand_count, uint32_t goal, float error_limit);
+void launch_apply(hipStream_t s, Mesh& m, size_t cand_count);
+void launch_remap_triangles(hipStream_t s, Mesh& m, size_t tri_count);
+void launch_debug_bits(hipStream_t s, Mesh& m, size_t index_count);
+void launch_sparse_backmap(hipStream_t s, uint32_t* ib, size_t index_count, const uint32_t* sparse_remap);
+
+} // namespace mhip

@@ -1,0 +1,987 @@
+// Per-pass kernels of the edge-collapse simplifier (src/simplifier.cpp:2493-2541):
+// welded adjacency, pick, rank, sort, the priority-round greedy commit with its prefix-scan
+// stop, quadric update, edge-loop remap and index compaction.
+#include "kernels.h"
+#include "prims.h"
+#include "quadric.h"
+
+namespace mhip
+{
+
+static constexpr int kThreads = 256;
+
+static inline unsigned int grid_for(size_t n)
+{
+	return div_up(n ? n : 1, kThreads);
+}
+
+// kCanCollapse / kHasOpposite, src/simplifier.cpp:312-331
+__constant__ uint8_t c_can_collapse[5][5] = {
+    {1, 1, 1, 1, 1},
+    {0, 1, 0, 0, 1},
+    {0, 0, 1, 0, 1},
+    {0, 0, 0, 1, 1},
+    {0, 0, 0, 0, 0},
+};
+
+__constant__ uint8_t c_has_opposite[5][5] = {
+    {1, 1, 1, 1, 1},
+    {1, 0, 1, 0, 0},
+    {1, 1, 1, 0, 1},
+    {1, 0, 0, 0, 0},
+    {1, 0, 1, 0, 0},
+};
+
+// ------------------------------------------------------------------------------------------
+// welded corner CSR for this pass (src/simplifier.cpp:52-109).  Segment order is free: every
+// consumer (open-edge tests, flip tests, neighbourhood scans) is a count or an any().
+
+__global__ void k_adj_count(const uint32_t* ib, size_t n, const uint32_t* remap, uint32_t* counts)
+{
+	size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (j < n)
+		atomicAdd(&counts[remap[ib[j]]], 1u);
+}
+
+__global__ void k_adj_fill(const uint32_t* ib, size_t tris, const uint32_t* remap, uint32_t* cursor, uint2* adj)
+{
+	size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= tris)
+		return;
+	uint32_t a = remap[ib[t * 3 + 0]], b = remap[ib[t * 3 + 1]], c = remap[ib[t * 3 + 2]];
+	adj[atomicAdd(&cursor[a], 1u)] = make_uint2(b, c);
+	adj[atomicAdd(&cursor[b], 1u)] = make_uint2(c, a);
+	adj[atomicAdd(&cursor[c], 1u)] = make_uint2(a, b);
+}
+
+void launch_adjacency(hipStream_t s, Mesh& m, size_t tri_count)
+{
+	size_t n = m.vertex_count;
+	fill_u32(s, m.adj_off, 0, n + 1);
+	if (tri_count)
+		k_adj_count<<<grid_for(tri_count * 3), kThreads, 0, s>>>(m.ib[m.ib_cur], tri_count * 3, m.remap, m.adj_off);
+	scan_exclusive(s, m.adj_off, m.adj_off, n, nullptr, m.adj_off + n, m.temp);
+	HIP_CHECK(hipMemcpyAsync(m.adj_cursor, m.adj_off, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+	if (tri_count)
+		k_adj_fill<<<grid_for(tri_count), kThreads, 0, s>>>(m.ib[m.ib_cur], tri_count, m.remap, m.adj_cursor, m.adj);
+}
+
+// ------------------------------------------------------------------------------------------
+// pickEdgeCollapses, src/simplifier.cpp:1310-1372: candidates of one triangle in edge order
+
+__device__ __forceinline__ int pick_triangle(const uint32_t* ib, size_t t, const uint32_t* remap, const uint8_t* kind, const uint32_t* loop,
+                                             const uint32_t* loopback, uint32_t v0[3], uint32_t v1[3], uint32_t bidi[3])
+{
+	uint32_t tri[3] = {ib[t * 3 + 0], ib[t * 3 + 1], ib[t * 3 + 2]};
+	int count = 0;
+#pragma unroll
+	for (int e = 0; e < 3; ++e)
+	{
+		uint32_t i0 = tri[e];
+		uint32_t i1 = tri[e == 2 ? 0 : e + 1];
+		uint32_t r0 = remap[i0], r1 = remap[i1];
+		if (r0 == r1)
+			continue;
+		uint8_t k0 = kind[i0], k1 = kind[i1];
+		uint8_t c01 = c_can_collapse[k0][k1], c10 = c_can_collapse[k1][k0];
+		if (!(c01 | c10))
+			continue;
+		if (c_has_opposite[k0][k1] && r1 > r0)
+			continue;
+		if ((k0 == kBorder || k0 == kSeam) && k1 != kManifold && loop[i0] != i1)
+			continue;
+		if ((k1 == kBorder || k1 == kSeam) && k0 != kManifold && loopback[i1] != i0)
+			continue;
+		if (c01 & c10)
+		{
+			v0[count] = i0;
+			v1[count] = i1;
+			bidi[count] = 1;
+		}
+		else
+		{
+			v0[count] = c01 ? i0 : i1;
+			v1[count] = c01 ? i1 : i0;
+			bidi[count] = 0;
+		}
+		count++;
+	}
+	return count;
+}
+
+__global__ void k_pick_count(const uint32_t* ib, size_t tris, const uint32_t* remap, const uint8_t* kind, const uint32_t* loop,
+                             const uint32_t* loopback, uint32_t* counts)
+{
+	size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= tris)
+		return;
+	uint32_t v0[3], v1[3], bd[3];
+	counts[t] = (uint32_t)pick_triangle(ib, t, remap, kind, loop, loopback, v0, v1, bd);
+}
+
+// triangles are taken while `written + 3 <= capacity` (src/simplifier.cpp:1318-1320)
+__global__ void k_pick_write(const uint32_t* ib, size_t tris, const uint32_t* remap, const uint8_t* kind, const uint32_t* loop,
+                             const uint32_t* loopback, const uint32_t* offsets, uint32_t capacity, uint32_t* c_v0, uint32_t* c_v1,
+                             float* c_err, uint32_t* out_count)
+{
+	size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= tris)
+		return;
+	uint32_t base = offsets[t];
+	if ((uint64_t)base + 3 > capacity)
+		return;
+	uint32_t v0[3], v1[3], bd[3];
+	int count = pick_triangle(ib, t, remap, kind, loop, loopback, v0, v1, bd);
+	for (int k = 0; k < count; ++k)
+	{
+		c_v0[base + k] = v0[k];
+		c_v1[base + k] = v1[k];
+		c_err[base + k] = __uint_as_float(bd[k]); // the bidirectional flag until ranked
+	}
+	atomicMax(out_count, base + (uint32_t)count);
+}
+
+void launch_pick(hipStream_t s, Mesh& m, size_t tri_count)
+{
+	uint32_t* loop = m.loop[m.loop_cur];
+	uint32_t* loopback = m.loopback[m.loop_cur];
+	HIP_CHECK(hipMemsetAsync(&m.sc->candidates, 0, sizeof(uint32_t), s));
+	if (!tri_count)
+		return;
+	k_pick_count<<<grid_for(tri_count), kThreads, 0, s>>>(m.ib[m.ib_cur], tri_count, m.remap, m.kind, loop, loopback, m.tri_cnt);
+	scan_exclusive(s, m.tri_cnt, m.tri_off, tri_count, nullptr, nullptr, m.temp);
+	k_pick_write<<<grid_for(tri_count), kThreads, 0, s>>>(m.ib[m.ib_cur], tri_count, m.remap, m.kind, loop, loopback, m.tri_off, m.capacity,
+	                                                      m.c_v0, m.c_v1, m.c_err, &m.sc->candidates);
+}
+
+// ------------------------------------------------------------------------------------------
+// rankEdgeCollapses, src/simplifier.cpp:1374-1450
+
+__device__ __forceinline__ uint32_t complex_target(uint32_t v, uint32_t target, const uint32_t* remap, const uint32_t* loop, const uint32_t* loopback)
+{
+	// getComplexTarget, src/simplifier.cpp:1278-1290
+	uint32_t r = remap[target];
+	uint32_t l = loop[v];
+	if (l != kNone && remap[l] == r)
+		return l;
+	uint32_t lb = loopback[v];
+	if (lb != kNone && remap[lb] == r)
+		return lb;
+	return target;
+}
+
+struct RankArgs
+{
+	const float4* pos;
+	const float* va;
+	const Quad* vq;
+	const Quad* aq;
+	const Grad* ag;
+	int attr_count;
+	const uint32_t* remap;
+	const uint32_t* wedge;
+	const uint8_t* kind;
+	const uint32_t* loop;
+	const uint32_t* loopback;
+};
+
+__device__ __forceinline__ float attr_error(const RankArgs& a, uint32_t v, uint32_t t)
+{
+	const int A = a.attr_count;
+	return quad_error_attr(load_quad(a.aq, v), a.ag + (size_t)v * A, A, load_pos(a.pos, t), a.va + (size_t)t * A);
+}
+
+__global__ void k_rank(RankArgs a, size_t count, uint32_t* c_v0, uint32_t* c_v1, float* c_err, uint32_t* keys, uint32_t* vals)
+{
+	size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (c >= count)
+		return;
+	uint32_t i0 = c_v0[c], i1 = c_v1[c];
+	bool bidi = __float_as_uint(c_err[c]) != 0;
+
+	float ei = quad_error(load_quad(a.vq, a.remap[i0]), load_pos(a.pos, i1));
+	float ej = bidi ? quad_error(load_quad(a.vq, a.remap[i1]), load_pos(a.pos, i0)) : FLT_MAX;
+
+	if (a.attr_count)
+	{
+		ei += attr_error(a, i0, i1);
+		ej += bidi ? attr_error(a, i1, i0) : 0;
+
+		uint8_t k0 = a.kind[i0];
+		if (k0 == kSeam)
+		{
+			uint32_t s0 = a.wedge[i0];
+			uint32_t s1 = a.loop[i0] == i1 ? a.loopback[s0] : a.loop[s0];
+			s1 = (s1 != kNone) ? s1 : a.wedge[i1];
+
+			ei += attr_error(a, s0, s1);
+			ej += bidi ? attr_error(a, s1, s0) : 0;
+		}
+		else
+		{
+			if (k0 == kComplex)
+				for (uint32_t v = a.wedge[i0]; v != i0; v = a.wedge[v])
+					ei += attr_error(a, v, complex_target(v, i1, a.remap, a.loop, a.loopback));
+
+			if (a.kind[i1] == kComplex && bidi)
+				for (uint32_t v = a.wedge[i1]; v != i1; v = a.wedge[v])
+					ej += attr_error(a, v, complex_target(v, i0, a.remap, a.loop, a.loopback));
+		}
+	}
+
+	bool rev = bidi & (ej < ei);
+	float err = ej < ei ? ej : ei;
+	c_v0[c] = rev ? i1 : i0;
+	c_v1[c] = rev ? i0 : i1;
+	c_err[c] = err;
+	keys[c] = collapse_sort_key(err);
+	vals[c] = (uint32_t)c;
+}
+
+__global__ void k_gather_ranked(const uint32_t* order, size_t count, const uint32_t* c_v0, const uint32_t* c_v1, const float* c_err,
+                                const uint32_t* remap, uint32_t* r_v0, uint32_t* r_v1, uint32_t* r_r0, uint32_t* r_r1, float* r_err)
+{
+	size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (k >= count)
+		return;
+	uint32_t c = order[k];
+	uint32_t v0 = c_v0[c], v1 = c_v1[c];
+	r_v0[k] = v0;
+	r_v1[k] = v1;
+	r_r0[k] = remap[v0];
+	r_r1[k] = remap[v1];
+	r_err[k] = c_err[c];
+}
+
+void launch_rank_sort(hipStream_t s, Mesh& m, size_t cand_count)
+{
+	if (!cand_count)
+		return;
+	RankArgs a;
+	a.pos = m.pos;
+	a.va = m.va;
+	a.vq = m.vq;
+	a.aq = m.aq;
+	a.ag = m.ag;
+	a.attr_count = (int)m.attr_count;
+	a.remap = m.remap;
+	a.wedge = m.wedge;
+	a.kind = m.kind;
+	a.loop = m.loop[m.loop_cur];
+	a.loopback = m.loopback[m.loop_cur];
+	k_rank<<<grid_for(cand_count), kThreads, 0, s>>>(a, cand_count, m.c_v0, m.c_v1, m.c_err, m.keys, m.vals);
+	// sortEdgeCollapses (src/simplifier.cpp:1452-1496): stable by the 12-bit key, ties in pick order
+	radix_sort_pairs(s, m.keys, m.vals, m.keys_alt, m.vals_alt, cand_count, nullptr, 12, m.temp);
+	k_gather_ranked<<<grid_for(cand_count), kThreads, 0, s>>>(m.vals, cand_count, m.c_v0, m.c_v1, m.c_err, m.remap, m.r_v0, m.r_v1, m.r_r0,
+	                                                          m.r_r1, m.r_err);
+}
+
+// ------------------------------------------------------------------------------------------
+// performEdgeCollapses (src/simplifier.cpp:1498-1636) as priority rounds.
+//
+// Rank k's decision depends only on lower ranks j that share an endpoint class (the lock
+// test) or whose source class is a welded neighbour of k's source class (the flip test reads
+// collapse_remap only there; only sources are written, and the neighbour relation is
+// symmetric).  Round e: an undecided rank whose endpoints are locked is skipped (a lock can
+// only come from a lower commit); one that was the minimum published at both endpoints in
+// round e-1 and is below every source-role entry of e-1 in its neighbourhood decides now
+// (flip-rejected or committed); the rest publish their rank for round e as 64-bit minima
+// tagged with ~e in the high word, into the buffer of parity e, and stay on the work list.
+// Ranks that decide in one round never touch each other's state.
+
+__device__ __forceinline__ uint64_t round_key(uint32_t epoch, uint32_t k)
+{
+	return ((uint64_t)(~epoch) << 32) | k;
+}
+
+// hasTriangleFlips, src/simplifier.cpp:1200-1232
+__device__ __forceinline__ bool collapse_flips(const uint32_t* off, const uint2* adj, const float4* pos, const uint32_t* cremap, uint32_t i0, uint32_t i1)
+{
+	V3 v0 = load_pos(pos, i0), v1 = load_pos(pos, i1);
+	for (uint32_t p = off[i0], e = off[i0 + 1]; p < e; ++p)
+	{
+		uint2 nb = adj[p];
+		uint32_t a = cremap[nb.x];
+		uint32_t b = cremap[nb.y];
+		if (a == i1 || b == i1 || a == b)
+			continue;
+		if (triangle_flips(load_pos(pos, a), load_pos(pos, b), v0, v1))
+			return true;
+	}
+	return false;
+}
+
+// commit writes of one collapse, src/simplifier.cpp:1573-1611
+__device__ __forceinline__ void commit_collapse(uint32_t* cremap, uint32_t i0, uint32_t i1, const uint8_t* kind, const uint32_t* wedge,
+                                                const uint32_t* remap, const uint32_t* loop, const uint32_t* loopback)
+{
+	uint8_t k = kind[i0];
+	if (k == kComplex)
+	{
+		uint32_t v = i0;
+		do
+		{
+			cremap[v] = complex_target(v, i1, remap, loop, loopback);
+			v = wedge[v];
+		} while (v != i0);
+	}
+	else if (k == kSeam)
+	{
+		uint32_t s0 = wedge[i0];
+		uint32_t s1 = loop[i0] == i1 ? loopback[s0] : loop[s0];
+		s1 = (s1 != kNone) ? s1 : wedge[i1];
+		cremap[i0] = i1;
+		cremap[s0] = s1;
+	}
+	else
+		cremap[i0] = i1;
+}
+
+struct RoundArgs
+{
+	const uint32_t* r_v0;
+	const uint32_t* r_v1;
+	const uint32_t* r_r0;
+	const uint32_t* r_r1;
+	const uint32_t* adj_off;
+	const uint2* adj;
+	const float4* pos;
+	const uint8_t* kind;
+	const uint32_t* wedge;
+	const uint32_t* remap;
+	const uint32_t* loop;
+	const uint32_t* loopback;
+	uint32_t* cremap;
+	uint32_t* locked;
+	uint8_t* status;
+	const uint64_t* rd_any;
+	const uint64_t* rd_src;
+	uint64_t* wr_any;
+	uint64_t* wr_src;
+	const uint32_t* in_list;
+	const uint32_t* in_count;
+	uint32_t* out_list;
+	uint32_t* out_count;
+	uint32_t* zero_count;
+	uint32_t epoch;
+};
+
+__global__ void __launch_bounds__(kThreads) k_round(RoundArgs a)
+{
+	if (blockIdx.x == 0 && threadIdx.x == 0)
+		*a.zero_count = 0;
+
+	const uint32_t n = *a.in_count;
+	const uint32_t prev_tag = ~(a.epoch - 1);
+	const int lane = __lane_id();
+
+	for (uint32_t base = blockIdx.x * kThreads; base < n; base += gridDim.x * kThreads)
+	{
+		uint32_t idx = base + threadIdx.x;
+		bool keep = false;
+		uint32_t k = 0;
+		if (idx < n)
+		{
+			k = a.in_list[idx];
+			uint32_t r0 = a.r_r0[k], r1 = a.r_r1[k];
+			if (a.locked[r0] | a.locked[r1])
+				a.status[k] = kSkipped;
+			else
+			{
+				uint64_t mine = round_key(a.epoch - 1, k);
+				bool ready = a.rd_any[r0] == mine && a.rd_any[r1] == mine;
+				if (ready)
+				{
+					for (uint32_t p = a.adj_off[r0], e = a.adj_off[r0 + 1]; p < e; ++p)
+					{
+						uint2 nb = a.adj[p];
+						uint64_t pa = a.rd_src[nb.x], pb = a.rd_src[nb.y];
+						if (((uint32_t)(pa >> 32) == prev_tag && (uint32_t)pa < k) || ((uint32_t)(pb >> 32) == prev_tag && (uint32_t)pb < k))
+						{
+							ready = false;
+							break;
+						}
+					}
+				}
+
+				if (ready)
+				{
+					if (collapse_flips(a.adj_off, a.adj, a.pos, a.cremap, r0, r1))
+						a.status[k] = kRejected;
+					else
+					{
+						commit_collapse(a.cremap, a.r_v0[k], a.r_v1[k], a.kind, a.wedge, a.remap, a.loop, a.loopback);
+						a.locked[r0] = 1;
+						a.locked[r1] = 1;
+						a.status[k] = kCommitted;
+					}
+				}
+				else
+				{
+					uint64_t key = round_key(a.epoch, k);
+					atomicMin((unsigned long long*)&a.wr_any[r0], (unsigned long long)key);
+					atomicMin((unsigned long long*)&a.wr_any[r1], (unsigned long long)key);
+					atomicMin((unsigned long long*)&a.wr_src[r0], (unsigned long long)key);
+					keep = true;
+				}
+			}
+		}
+
+		// wave-aggregated append of the ranks that stay undecided
+		uint64_t mask = __ballot(keep);
+		if (mask)
+		{
+			uint32_t slot0 = 0;
+			int leader = __ffsll((long long)mask) - 1;
+			if (lane == leader)
+				slot0 = atomicAdd(a.out_count, (uint32_t)__popcll(mask));
+			slot0 = __shfl(slot0, leader, 64);
+			if (keep)
+				a.out_list[slot0 + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull))] = k;
+		}
+	}
+}
+
+__global__ void k_perform_init_status(uint8_t* status, size_t count)
+{
+	size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (k < count)
+		status[k] = kUndecided;
+}
+
+__global__ void k_perform_init_scalars(CallScalars* sc, uint32_t count)
+{
+	sc->list_count[0] = count;
+	sc->list_count[1] = 0;
+	sc->list_count[2] = 0;
+	sc->stop_reason = 0;
+}
+
+void launch_perform_init(hipStream_t s, Mesh& m, size_t cand_count)
+{
+	iota_u32(s, m.cremap, m.vertex_count);
+	fill_u32(s, m.locked, 0, m.vertex_count);
+	if (cand_count)
+	{
+		k_perform_init_status<<<grid_for(cand_count), kThreads, 0, s>>>(m.status, cand_count);
+		iota_u32(s, m.list[0], cand_count);
+	}
+	k_perform_init_scalars<<<1, 1, 0, s>>>(m.sc, (uint32_t)cand_count);
+}
+
+// round r reads list (r & 1) with counter (r % 3), writes list ((r + 1) & 1) with counter
+// ((r + 1) % 3) and clears counter ((r + 2) % 3) for the round after
+void launch_round(hipStream_t s, Mesh& m, size_t list_bound, int round)
+{
+	RoundArgs a;
+	a.r_v0 = m.r_v0;
+	a.r_v1 = m.r_v1;
+	a.r_r0 = m.r_r0;
+	a.r_r1 = m.r_r1;
+	a.adj_off = m.adj_off;
+	a.adj = m.adj;
+	a.pos = m.pos;
+	a.kind = m.kind;
+	a.wedge = m.wedge;
+	a.remap = m.remap;
+	a.loop = m.loop[m.loop_cur];
+	a.loopback = m.loopback[m.loop_cur];
+	a.cremap = m.cremap;
+	a.locked = m.locked;
+	a.status = m.status;
+	uint32_t e = ++m.epoch;
+	a.epoch = e;
+	a.rd_any = m.pend_any[(e - 1) & 1];
+	a.rd_src = m.pend_src[(e - 1) & 1];
+	a.wr_any = m.pend_any[e & 1];
+	a.wr_src = m.pend_src[e & 1];
+	a.in_list = m.list[round & 1];
+	a.out_list = m.list[(round + 1) & 1];
+	a.in_count = &m.sc->list_count[round % 3];
+	a.out_count = &m.sc->list_count[(round + 1) % 3];
+	a.zero_count = &m.sc->list_count[(round + 2) % 3];
+	unsigned int blocks = grid_for(list_bound);
+	blocks = blocks < 1 ? 1 : (blocks > 4096 ? 4096 : blocks);
+	k_round<<<blocks, kThreads, 0, s>>>(a);
+}
+
+// ------------------------------------------------------------------------------------------
+// stop scan: the first rank where the sequential loop breaks (src/simplifier.cpp:1517-1539)
+//   S1 err > error_limit
+//   S2 tri_collapses >= goal
+//   S3 err > 1.5 * err[goal/2 + rejects] && err > result_error && tri_collapses > goal/6
+// with tri_collapses, rejects and result_error the exclusive prefix sums / max of the
+// decided ranks.  Tiles of 4096 ranks: reduce, scan the tile aggregates, then test.
+
+struct StopAgg
+{
+	uint32_t tri, rej, err;
+};
+
+__device__ __forceinline__ StopAgg stop_item(uint8_t st, uint8_t k0, float err)
+{
+	StopAgg a;
+	a.tri = st == kCommitted ? (k0 == kBorder ? 1u : 2u) : 0u;
+	a.rej = st == kRejected ? 1u : 0u;
+	// result_error keeps a NaN error out (its `<` test fails); errors are >= 0 otherwise
+	a.err = (st == kCommitted && err == err) ? __float_as_uint(err) : 0u;
+	return a;
+}
+
+template <bool Max>
+__device__ __forceinline__ uint32_t wave_incl(uint32_t v)
+{
+	const int lane = __lane_id();
+#pragma unroll
+	for (int o = 1; o < 64; o <<= 1)
+	{
+		uint32_t t = __shfl_up(v, o, 64);
+		if (lane >= o)
+			v = Max ? (t > v ? t : v) : v + t;
+	}
+	return v;
+}
+
+// exclusive block scan of three values (sum, sum, max) for 256 threads
+__device__ __forceinline__ StopAgg block_scan3(StopAgg v, StopAgg& total)
+{
+	__shared__ uint32_t lds[3][4];
+	uint32_t it = wave_incl<false>(v.tri);
+	uint32_t ir = wave_incl<false>(v.rej);
+	uint32_t ie = wave_incl<true>(v.err);
+	const int wave = threadIdx.x >> 6;
+	if (__lane_id() == 63)
+	{
+		lds[0][wave] = it;
+		lds[1][wave] = ir;
+		lds[2][wave] = ie;
+	}
+	__syncthreads();
+	uint32_t bt = 0, br = 0, be = 0, at = 0, ar = 0, ae = 0;
+#pragma unroll
+	for (int w = 0; w < 4; ++w)
+	{
+		uint32_t t = lds[0][w], r = lds[1][w], e = lds[2][w];
+		if (w < wave)
+		{
+			bt += t;
+			br += r;
+			be = e > be ? e : be;
+		}
+		at += t;
+		ar += r;
+		ae = e > ae ? e : ae;
+	}
+	__syncthreads();
+	total = StopAgg{at, ar, ae};
+	// exclusive: inclusive minus own (sums); for the max, the inclusive value of the previous lane
+	uint32_t pe = __shfl_up(ie, 1, 64);
+	if (__lane_id() == 0)
+		pe = 0;
+	StopAgg ex;
+	ex.tri = bt + it - v.tri;
+	ex.rej = br + ir - v.rej;
+	ex.err = be > pe ? be : pe;
+	return ex;
+}
+
+__global__ void __launch_bounds__(kThreads) k_stop_reduce(const uint8_t* status, const uint32_t* r_v0, const uint8_t* kind, const float* r_err,
+                                                          size_t count, uint32_t* tiles)
+{
+	size_t base = (size_t)blockIdx.x * kScanTile;
+	StopAgg acc{0, 0, 0};
+	for (int j = 0; j < kScanItems; ++j)
+	{
+		size_t k = base + (size_t)j * kThreads + threadIdx.x;
+		if (k < count)
+		{
+			StopAgg a = stop_item(status[k], kind[r_v0[k]], r_err[k]);
+			acc.tri += a.tri;
+			acc.rej += a.rej;
+			acc.err = a.err > acc.err ? a.err : acc.err;
+		}
+	}
+	StopAgg total;
+	block_scan3(acc, total);
+	if (threadIdx.x == 0)
+	{
+		tiles[blockIdx.x * 3 + 0] = total.tri;
+		tiles[blockIdx.x * 3 + 1] = total.rej;
+		tiles[blockIdx.x * 3 + 2] = total.err;
+	}
+}
+
+// single block: exclusive prefix of the tile aggregates; the max starts at the incoming result_error
+__global__ void __launch_bounds__(kThreads) k_stop_tiles(uint32_t* tiles, uint32_t ntiles, const CallScalars* sc)
+{
+	StopAgg carry{0, 0, __float_as_uint(sc->result_error)};
+	for (uint32_t b = 0; b < ntiles; b += kThreads)
+	{
+		uint32_t i = b + threadIdx.x;
+		StopAgg v{0, 0, 0};
+		if (i < ntiles)
+			v = StopAgg{tiles[i * 3 + 0], tiles[i * 3 + 1], tiles[i * 3 + 2]};
+		StopAgg total;
+		StopAgg ex = block_scan3(v, total);
+		if (i < ntiles)
+		{
+			tiles[i * 3 + 0] = carry.tri + ex.tri;
+			tiles[i * 3 + 1] = carry.rej + ex.rej;
+			uint32_t e = ex.err > carry.err ? ex.err : carry.err;
+			tiles[i * 3 + 2] = e;
+		}
+		carry.tri += total.tri;
+		carry.rej += total.rej;
+		carry.err = total.err > carry.err ? total.err : carry.err;
+	}
+}
+
+__global__ void __launch_bounds__(kThreads) k_stop_find(const uint8_t* status, const uint32_t* r_v0, const uint8_t* kind, const float* r_err,
+                                                        size_t count, const uint32_t* tiles, uint32_t goal, float error_limit, CallScalars* sc)
+{
+	size_t base = (size_t)blockIdx.x * kScanTile;
+	StopAgg carry{tiles[blockIdx.x * 3 + 0], tiles[blockIdx.x * 3 + 1], tiles[blockIdx.x * 3 + 2]};
+	const uint32_t half = goal / 2, sixth = goal / 6;
+	for (int j = 0; j < kScanItems; ++j)
+	{
+		size_t k = base + (size_t)j * kThreads + threadIdx.x;
+		StopAgg v{0, 0, 0};
+		float err = 0.f;
+		if (k < count)
+		{
+			err = r_err[k];
+			v = stop_item(status[k], kind[r_v0[k]], err);
+		}
+		StopAgg total;
+		StopAgg ex = block_scan3(v, total);
+		if (k < count)
+		{
+			uint32_t tri = carry.tri + ex.tri;
+			uint32_t rej = carry.rej + ex.rej;
+			uint32_t eb = ex.err > carry.err ? ex.err : carry.err;
+			float result_error = __uint_as_float(eb);
+			size_t eg = (size_t)half + rej;
+			float error_goal = eg < count ? 1.5f * r_err[eg] : FLT_MAX;
+			bool stop = err > error_limit || tri >= goal || (err > error_goal && err > result_error && tri > sixth);
+			if (stop)
+				atomicMin(&sc->stop, (uint32_t)k);
+		}
+		carry.tri += total.tri;
+		carry.rej += total.rej;
+		carry.err = total.err > carry.err ? total.err : carry.err;
+	}
+}
+
+// totals below the stop rank
+__global__ void __launch_bounds__(kThreads) k_stop_totals(const uint8_t* status, const uint32_t* r_v0, const uint8_t* kind, const float* r_err,
+                                                          CallScalars* sc)
+{
+	uint32_t stop = sc->stop;
+	uint32_t committed = 0, skipped = 0, rejected = 0, tri = 0, eb = 0;
+	for (uint32_t k = blockIdx.x * kThreads + threadIdx.x; k < stop; k += gridDim.x * kThreads)
+	{
+		uint8_t st = status[k];
+		StopAgg a = stop_item(st, kind[r_v0[k]], r_err[k]);
+		committed += st == kCommitted;
+		skipped += st == kSkipped;
+		rejected += st == kRejected;
+		tri += a.tri;
+		eb = a.err > eb ? a.err : eb;
+	}
+#pragma unroll
+	for (int o = 32; o > 0; o >>= 1)
+	{
+		committed += __shfl_xor(committed, o, 64);
+		skipped += __shfl_xor(skipped, o, 64);
+		rejected += __shfl_xor(rejected, o, 64);
+		tri += __shfl_xor(tri, o, 64);
+		uint32_t t = __shfl_xor(eb, o, 64);
+		eb = t > eb ? t : eb;
+	}
+	if (__lane_id() == 0)
+	{
+		if (committed)
+			atomicAdd(&sc->committed, committed);
+		if (skipped)
+			atomicAdd(&sc->skipped, skipped);
+		if (rejected)
+			atomicAdd(&sc->rejected, rejected);
+		if (tri)
+			atomicAdd(&sc->tri_collapses, tri);
+		if (eb)
+			atomicMax(&sc->work[1], eb);
+	}
+}
+
+__global__ void k_stop_finish(const float* r_err, uint32_t count, uint32_t goal, float error_limit, CallScalars* sc)
+{
+	uint32_t stop = sc->stop;
+	float prev = sc->result_error;
+	float best = __uint_as_float(sc->work[1]);
+	sc->result_error = prev < best ? best : prev;
+	uint32_t reason = 0;
+	if (stop < count)
+	{
+		float err = r_err[stop];
+		if (err > error_limit)
+			reason = 1;
+		else if (sc->tri_collapses >= goal)
+			reason = 2;
+		else
+			reason = 3;
+	}
+	sc->stop_reason = reason;
+}
+
+// undo commits at or past the stop rank: a commit wrote only its source class's wedges
+__global__ void k_revert(const uint8_t* status, const uint32_t* r_r0, const uint32_t* wedge, size_t count, const CallScalars* sc, uint32_t* cremap)
+{
+	size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (k >= count || k < sc->stop || status[k] != kCommitted)
+		return;
+	uint32_t r = r_r0[k];
+	uint32_t v = r;
+	do
+	{
+		cremap[v] = v;
+		v = wedge[v];
+	} while (v != r);
+}
+
+__global__ void k_stop_reset(CallScalars* sc, uint32_t count)
+{
+	sc->stop = count;
+	sc->committed = 0;
+	sc->skipped = 0;
+	sc->rejected = 0;
+	sc->tri_collapses = 0;
+	sc->work[1] = 0;
+}
+
+// repeatable: reads the decisions and writes only the stop rank and the totals below it
+void launch_stop_scan(hipStream_t s, Mesh& m, size_t cand_count, uint32_t goal, float error_limit)
+{
+	k_stop_reset<<<1, 1, 0, s>>>(m.sc, (uint32_t)cand_count);
+	if (!cand_count)
+		return;
+	unsigned int tiles = div_up(cand_count, kScanTile);
+	k_stop_reduce<<<tiles, kThreads, 0, s>>>(m.status, m.r_v0, m.kind, m.r_err, cand_count, m.stop_tile);
+	k_stop_tiles<<<1, kThreads, 0, s>>>(m.stop_tile, tiles, m.sc);
+	k_stop_find<<<tiles, kThreads, 0, s>>>(m.status, m.r_v0, m.kind, m.r_err, cand_count, m.stop_tile, goal, error_limit, m.sc);
+	unsigned int blocks = grid_for(cand_count);
+	blocks = blocks > 1024 ? 1024 : blocks;
+	k_stop_totals<<<blocks, kThreads, 0, s>>>(m.status, m.r_v0, m.kind, m.r_err, m.sc);
+}
+
+// once every rank of the window is decided: advance result_error and undo the commits at or
+// past the stop rank
+void launch_stop_commit(hipStream_t s, Mesh& m, size_t cand_count, uint32_t goal, float error_limit)
+{
+	k_stop_finish<<<1, 1, 0, s>>>(m.r_err, (uint32_t)cand_count, goal, error_limit, m.sc);
+	if (cand_count)
+		k_revert<<<grid_for(cand_count), kThreads, 0, s>>>(m.status, m.r_r0, m.wedge, cand_count, m.sc, m.cremap);
+}
+
+// ------------------------------------------------------------------------------------------
+// updateQuadrics (src/simplifier.cpp:1638-1673), one thread per commit below the stop.
+// The target class receives exactly one positional add (its primary is locked); attribute
+// quadrics move wedge by wedge in ascending wedge order, as the reference's vertex loop does.
+
+__device__ __forceinline__ uint32_t next_wedge_above(const uint32_t* wedge, uint32_t r, uint32_t after)
+{
+	// smallest member of r's ring greater than `after` (kNone if none)
+	uint32_t best = kNone;
+	uint32_t v = r;
+	do
+	{
+		if (v > after && v < best)
+			best = v;
+		v = wedge[v];
+	} while (v != r);
+	return best;
+}
+
+__global__ void k_update_quadrics(const uint8_t* status, const uint32_t* r_r0, const uint32_t* r_r1, const CallScalars* sc_in, const uint32_t* cremap,
+                                  const uint32_t* wedge, const float4* pos, int attr_count, Quad* vq, Quad* aq, Grad* ag, CallScalars* sc)
+{
+	size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (k >= sc_in->stop || status[k] != kCommitted)
+		return;
+	uint32_t r0 = r_r0[k], r1 = r_r1[k];
+
+	Quad Q0 = load_quad(vq, r0);
+	Quad Q1 = load_quad(vq, r1);
+	quad_add(Q1, Q0);
+	store_quad(vq, r1, Q1);
+
+	if (attr_count)
+	{
+		const int A = attr_count;
+		// r0 is the lowest index of its class, so ascending order starts there
+		for (uint32_t v = r0; v != kNone; v = next_wedge_above(wedge, r0, v))
+		{
+			uint32_t t = cremap[v];
+			if (t == v)
+				continue;
+			Quad Qt = load_quad(aq, t);
+			quad_add(Qt, load_quad(aq, v));
+			store_quad(aq, t, Qt);
+			for (int a = 0; a < A; ++a)
+			{
+				Grad g = ag[(size_t)t * A + a];
+				grad_add(g, ag[(size_t)v * A + a]);
+				ag[(size_t)t * A + a] = g;
+			}
+		}
+
+		float derr = quad_error(Q0, load_pos(pos, r1));
+		if (derr == derr)
+			atomicMax(&sc->vertex_error_bits, __float_as_uint(derr));
+	}
+}
+
+// remapEdgeLoops, src/simplifier.cpp:1845-1864.  collapse_remap is idempotent within a pass
+// (targets are locked, never sources), so reading loop[l] before or after its own update
+// gives the same result and the in-place loop becomes a double-buffered map.
+__global__ void k_remap_loops(const uint32_t* cremap, size_t n, const uint32_t* loop_in, uint32_t* loop_out, const uint32_t* back_in, uint32_t* back_out)
+{
+	size_t ii = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (ii >= n)
+		return;
+	uint32_t i = (uint32_t)ii;
+	const uint32_t* in[2] = {loop_in, back_in};
+	uint32_t* out[2] = {loop_out, back_out};
+#pragma unroll
+	for (int q = 0; q < 2; ++q)
+	{
+		uint32_t l = in[q][i];
+		uint32_t res = kNone;
+		if (l != kNone)
+		{
+			uint32_t r = cremap[l];
+			if (i == r)
+			{
+				uint32_t ll = in[q][l];
+				res = ll != kNone ? cremap[ll] : kNone;
+			}
+			else
+				res = r;
+		}
+		out[q][i] = res;
+	}
+}
+
+__global__ void k_vertex_error_from_result(CallScalars* sc)
+{
+	sc->vertex_error = sc->result_error;
+}
+
+__global__ void k_vertex_error_from_bits(CallScalars* sc)
+{
+	float v = __uint_as_float(sc->vertex_error_bits);
+	float prev = sc->vertex_error;
+	sc->vertex_error = prev < v ? v : prev;
+}
+
+void launch_apply(hipStream_t s, Mesh& m, size_t cand_count)
+{
+	if (cand_count)
+		k_update_quadrics<<<grid_for(cand_count), kThreads, 0, s>>>(m.status, m.r_r0, m.r_r1, m.sc, m.cremap, m.wedge, m.pos, (int)m.attr_count,
+		                                                           m.vq, m.aq, m.ag, m.sc);
+	if (m.attr_count)
+		k_vertex_error_from_bits<<<1, 1, 0, s>>>(m.sc);
+	else
+		k_vertex_error_from_result<<<1, 1, 0, s>>>(m.sc);
+
+	int cur = m.loop_cur;
+	if (m.vertex_count)
+		k_remap_loops<<<grid_for(m.vertex_count), kThreads, 0, s>>>(m.cremap, m.vertex_count, m.loop[cur], m.loop[cur ^ 1], m.loopback[cur],
+		                                                          m.loopback[cur ^ 1]);
+	m.loop_cur = cur ^ 1;
+}
+
+// ------------------------------------------------------------------------------------------
+// remapIndexBuffer, src/simplifier.cpp:1811-1843: apply collapse_remap, drop triangles whose
+// primaries are no longer distinct, keep the survivors' order
+
+__device__ __forceinline__ bool remap_triangle(const uint32_t* ib, size_t t, const uint32_t* cremap, const uint32_t* remap, uint32_t v[3])
+{
+	v[0] = cremap[ib[t * 3 + 0]];
+	v[1] = cremap[ib[t * 3 + 1]];
+	v[2] = cremap[ib[t * 3 + 2]];
+	uint32_t r0 = remap[v[0]], r1 = remap[v[1]], r2 = remap[v[2]];
+	return r0 != r1 && r0 != r2 && r1 != r2;
+}
+
+__global__ void k_tri_keep(const uint32_t* ib, size_t tris, const uint32_t* cremap, const uint32_t* remap, uint32_t* keep)
+{
+	size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= tris)
+		return;
+	uint32_t v[3];
+	keep[t] = remap_triangle(ib, t, cremap, remap, v);
+}
+
+__global__ void k_tri_write(const uint32_t* ib, size_t tris, const uint32_t* cremap, const uint32_t* remap, const uint32_t* offsets, uint32_t* out)
+{
+	size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= tris)
+		return;
+	uint32_t v[3];
+	if (remap_triangle(ib, t, cremap, remap, v))
+	{
+		size_t o = (size_t)offsets[t] * 3;
+		out[o + 0] = v[0];
+		out[o + 1] = v[1];
+		out[o + 2] = v[2];
+	}
+}
+
+void launch_remap_triangles(hipStream_t s, Mesh& m, size_t tri_count)
+{
+	HIP_CHECK(hipMemsetAsync(&m.sc->next_triangles, 0, sizeof(uint32_t), s));
+	if (!tri_count)
+		return;
+	const uint32_t* in = m.ib[m.ib_cur];
+	uint32_t* out = m.ib[m.ib_cur ^ 1];
+	k_tri_keep<<<grid_for(tri_count), kThreads, 0, s>>>(in, tri_count, m.cremap, m.remap, m.tri_cnt);
+	scan_exclusive(s, m.tri_cnt, m.tri_off, tri_count, nullptr, &m.sc->next_triangles, m.temp);
+	k_tri_write<<<grid_for(tri_count), kThreads, 0, s>>>(in, tri_count, m.cremap, m.remap, m.tri_off, out);
+	m.ib_cur ^= 1;
+}
+
+// ------------------------------------------------------------------------------------------
+// finalisation (src/simplifier.cpp:2601-2619)
+
+__global__ void k_debug_bits(uint32_t* ib, size_t tris, const uint8_t* kind, const uint32_t* loop, const uint32_t* loopback)
+{
+	size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= tris)
+		return;
+	uint32_t a = ib[t * 3 + 0], b = ib[t * 3 + 1], c = ib[t * 3 + 2];
+	ib[t * 3 + 0] = a | ((uint32_t)kind[a] << 28) | ((uint32_t)(loop[a] == b || loopback[b] == a) << 31);
+	ib[t * 3 + 1] = b | ((uint32_t)kind[b] << 28) | ((uint32_t)(loop[b] == c || loopback[c] == b) << 31);
+	ib[t * 3 + 2] = c | ((uint32_t)kind[c] << 28) | ((uint32_t)(loop[c] == a || loopback[a] == c) << 31);
+}
+
+void launch_debug_bits(hipStream_t s, Mesh& m, size_t index_count)
+{
+	size_t tris = index_count / 3;
+	if (tris)
+		k_debug_bits<<<grid_for(tris), kThreads, 0, s>>>(m.ib[m.ib_cur], tris, m.kind, m.loop[m.loop_cur], m.loopback[m.loop_cur]);
+}
+
+__global__ void k_sparse_backmap(uint32_t* ib, size_t n, const uint32_t* sparse_remap)
+{
+	size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n)
+		ib[i] = sparse_remap[ib[i]];
+}
+
+void launch_sparse_backmap(hipStream_t s, uint32_t* ib, size_t index_count, const uint32_t* sparse_remap)
+{
+	if (index_count)
+		k_sparse_backmap<<<grid_for(index_count), kThreads, 0, s>>>(ib, index_count, sparse_remap);
+}
+
+} // namespace mhip

@@ -1,0 +1,40 @@
+"""Small random meshes for parity fuzzing (the pattern of tools/simplifyfuzz.cpp:12-30:
+random vertices, random triangles, every option combination), plus helpers that compare a
+simplifier run against the oracle."""
+from __future__ import annotations
+
+import numpy as np
+
+from meshoptimizer_amd import workloads as W
+
+
+def random_soup(rng, nv=100, ntri=300, dup_frac=0.1, grid_snap=True):
+    """Random vertices (some snapped to a coarse grid so positions repeat and form wedges),
+    random triangles (some degenerate)."""
+    v = rng.random((nv, 4), dtype=np.float32)
+    if grid_snap:
+        v[:, :3] = np.floor(v[:, :3] * 4) / 4
+    dup = rng.random(nv) < dup_frac
+    src = rng.integers(0, nv, size=nv)
+    v[dup, :3] = v[src[dup], :3]
+    ib = rng.integers(0, nv, size=ntri * 3).astype(np.uint32)
+    return ib, v
+
+
+def random_grid(rng, n=12, noise=0.05, seams=2):
+    """Grid patch with random heights and a few UV seams (duplicated vertex lines)."""
+    m = W.seamed_grid(n, seams=seams, seed=int(rng.integers(0, 1 << 30)))
+    v = m.vertices.copy()
+    v[:, 2] += (rng.random(v.shape[0]).astype(np.float32) * np.float32(noise))
+    # duplicates must keep identical positions: copy heights from the base vertex
+    pos = v[:, :3]
+    _, inv = np.unique(m.vertices[:, :3], axis=0, return_inverse=True)
+    first = np.full(inv.max() + 1, -1)
+    for i in range(len(inv) - 1, -1, -1):
+        first[inv[i]] = i
+    v[:, 2] = pos[first[inv.reshape(-1)], 2]
+    return m.indices.copy(), v
+
+
+def same_bits(a, b):
+    return np.float32(a).tobytes() == np.float32(b).tobytes()

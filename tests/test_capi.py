@@ -1,0 +1,50 @@
+"""CPU checks of the drop-in boundary: the library loads and exports every symbol the
+headers in include/ declare (no compute calls: this host has no GPU)."""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+import meshoptimizer_amd as M
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared_symbols():
+    names = set()
+    for h in ("meshoptimizer.h", "meshopt_hip.h"):
+        text = open(os.path.join(ROOT, "include", h)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        c_part = text.split("#if defined(__cplusplus) && !defined(MESHOPTIMIZER_NO_WRAPPERS)")[0]
+        for m in re.finditer(r"\b(meshopt_\w+)\s*\(", c_part):
+            names.add(m.group(1))
+    return names
+
+
+def test_library_exports_declared_symbols():
+    if not os.path.exists(M.LIB_PATH):
+        pytest.skip("libmeshopt_hip.so not built")
+    lib = ctypes.CDLL(M.LIB_PATH)
+    declared = _declared_symbols()
+    assert declared == set(M.EXPORTED_SYMBOLS)
+    for name in declared:
+        assert hasattr(lib, name), name
+
+
+def test_header_is_c89_clean(tmp_path):
+    src = tmp_path / "ansi.c"
+    src.write_text('#include "meshoptimizer.h"\n#include "meshopt_hip.h"\nint main(void) { return 0; }\n')
+    res = subprocess.run(["gcc", "-std=c89", "-pedantic", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"),
+                          "-c", str(src), "-o", str(tmp_path / "ansi.o")], capture_output=True, text=True)
+    assert res.returncode == 0, res.stderr
+
+
+def test_product_does_not_link_the_oracle():
+    if not os.path.exists(M.LIB_PATH):
+        pytest.skip("libmeshopt_hip.so not built")
+    out = subprocess.run(["nm", "-D", M.LIB_PATH], capture_output=True, text=True).stdout
+    assert "moracle" not in out
+    ldd = subprocess.run(["ldd", M.LIB_PATH], capture_output=True, text=True).stdout
+    assert "oracle" not in ldd and "meshopt_ref" not in ldd
