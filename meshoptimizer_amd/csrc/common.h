@@ -96,8 +96,8 @@ struct CallScalars
 	uint32_t list_count[3]; // undecided work-list counters of the rounds (rotating)
 	uint32_t dual_count;  // boundEdgeCollapses reduction
 	uint32_t unique_count; // sparse remap: number of referenced vertices
-	uint32_t rounds;
-	uint32_t list_zero;   // third rotating work-list counter (cleared one round ahead)
+	uint32_t tail_rounds;   // rounds run by the single-workgroup tail of the commit
+	uint32_t head_sizes[16]; // work-list size at the start of each head round
 	uint32_t work[8];     // small counters of setup kernels
 };
 

@@ -70,7 +70,8 @@ struct Context
 	CallScalars* sc_host = nullptr; // pinned mirror of the device scalar block
 	std::vector<meshopt_hip_pass_stats> passes;
 	meshopt_hip_call_stats call{};
-	int last_rounds = 16;
+	int head_rounds = 8;
+	double window_fraction = 0; // evaluated / candidates of the previous pass (0: none yet)
 
 	void ensure_init()
 	{
@@ -204,12 +205,13 @@ static void carve(Arena& a, Plan& p, size_t V_in, size_t V, size_t I, size_t A, 
 	m.ag = A ? a.take<Grad>(Vp * A) : nullptr;
 	m.va = A ? a.take<float>(Vp * A) : nullptr;
 	m.cremap = a.take<uint32_t>(Vp);
-	m.locked = a.take<uint32_t>(Vp);
+	m.lock_rank = a.take<uint32_t>(Vp);
 	for (int k = 0; k < 2; ++k)
 	{
 		m.pend_any[k] = a.take<uint64_t>(Vp);
 		m.pend_src[k] = a.take<uint64_t>(Vp);
 	}
+	m.src_off = a.take<uint32_t>(Vp + 1);
 	m.adj_off = a.take<uint32_t>(Vp + 1);
 	m.adj = a.take<uint2>(I + 4);
 	m.adj_cursor = a.take<uint32_t>(Vp);
@@ -230,8 +232,10 @@ static void carve(Arena& a, Plan& p, size_t V_in, size_t V, size_t I, size_t A, 
 	m.r_r1 = a.take<uint32_t>(cap);
 	m.r_err = a.take<float>(cap);
 	m.status = a.take<uint8_t>(cap);
-	m.list[0] = a.take<uint32_t>(cap);
-	m.list[1] = a.take<uint32_t>(cap);
+	m.src_rank = a.take<uint32_t>(cap);
+	m.src_val = a.take<uint32_t>(cap);
+	m.list[0] = a.take<uint4>(cap);
+	m.list[1] = a.take<uint4>(cap);
 	m.stop_tile = a.take<uint32_t>(3 * (cap / kScanTile + 2));
 
 	// setup-only buffers alias the candidate arrays (not live at the same time)
@@ -260,14 +264,31 @@ static void carve(Arena& a, Plan& p, size_t V_in, size_t V, size_t I, size_t A, 
 
 // ------------------------------------------------------------------------------------------
 
-__global__ void k_count_kinds(const uint32_t* remap, const uint8_t* kind, size_t n, uint32_t* counts)
+__global__ void __launch_bounds__(256) k_count_kinds(const uint32_t* remap, const uint8_t* kind, size_t n, uint32_t* counts)
 {
-	size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-	if (i < n && remap[i] == (uint32_t)i)
+	__shared__ uint32_t h[6];
+	if (threadIdx.x < 6)
+		h[threadIdx.x] = 0;
+	__syncthreads();
+	uint32_t local[6] = {0, 0, 0, 0, 0, 0};
+	for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+		if (remap[i] == (uint32_t)i)
+		{
+			uint8_t k = kind[i];
+			local[k < 5 ? k : 4]++;
+			local[5]++;
+		}
+	for (int q = 0; q < 6; ++q)
 	{
-		atomicAdd(&counts[kind[i]], 1u);
-		atomicAdd(&counts[5], 1u);
+		uint32_t v = local[q];
+		for (int o = 32; o > 0; o >>= 1)
+			v += __shfl_xor(v, o, 64);
+		if (__lane_id() == 0 && v)
+			atomicAdd(&h[q], v);
 	}
+	__syncthreads();
+	if (threadIdx.x < 6 && h[threadIdx.x])
+		atomicAdd(&counts[threadIdx.x], h[threadIdx.x]);
 }
 
 __global__ void k_scalars_init(CallScalars* sc)
@@ -341,6 +362,7 @@ static size_t simplify_on_device(Context& ctx, const CallArgs& c, float* result_
 	m.loop_cur = 0;
 	m.epoch = 0;
 
+	HIP_CHECK(hipMemsetAsync(m.sc, 0, sizeof(CallScalars), s));
 	k_scalars_init<<<1, 1, 0, s>>>(m.sc);
 	if (I)
 	{
@@ -384,19 +406,21 @@ static size_t simplify_on_device(Context& ctx, const CallArgs& c, float* result_
 	launch_dual_count(s, m, plan.raw_off);
 	HIP_CHECK(hipMemsetAsync(plan.kind_counts, 0, 8 * sizeof(uint32_t), s));
 	if (V)
-		k_count_kinds<<<div_up(V, 256), 256, 0, s>>>(m.remap, m.kind, V, plan.kind_counts);
+		k_count_kinds<<<std::min(div_up(V, 256 * 8), 512u), 256, 0, s>>>(m.remap, m.kind, V, plan.kind_counts);
 	uint32_t kind_counts[8];
 	HIP_CHECK(hipMemcpyAsync(kind_counts, plan.kind_counts, sizeof(kind_counts), hipMemcpyDeviceToHost, s));
 
 	// round minima start as "no entry" (tag of epoch 0, rank all-ones)
-	HIP_CHECK(hipMemsetAsync(m.pend_any[0], 0xff, (V + 1) * sizeof(uint64_t), s));
-	HIP_CHECK(hipMemsetAsync(m.pend_any[1], 0xff, (V + 1) * sizeof(uint64_t), s));
-	HIP_CHECK(hipMemsetAsync(m.pend_src[0], 0xff, (V + 1) * sizeof(uint64_t), s));
-	HIP_CHECK(hipMemsetAsync(m.pend_src[1], 0xff, (V + 1) * sizeof(uint64_t), s));
+	for (int b = 0; b < 2; ++b)
+	{
+		HIP_CHECK(hipMemsetAsync(m.pend_any[b], 0xff, (V + 1) * sizeof(uint64_t), s));
+		HIP_CHECK(hipMemsetAsync(m.pend_src[b], 0xff, (V + 1) * sizeof(uint64_t), s));
+	}
 
 	sync_scalars(ctx, m);
 	const CallScalars& h = *ctx.sc_host;
 
+	ctx.window_fraction = 0;
 	t_call = meshopt_hip_call_stats{};
 	t_call.vertex_count = (unsigned int)V;
 	t_call.unique_positions = kind_counts[5];
@@ -426,26 +450,48 @@ static size_t simplify_on_device(Context& ctx, const CallArgs& c, float* result_
 
 		launch_rank_sort(s, m, C);
 		uint32_t goal = (uint32_t)((result_count - c.target_index_count) / 3);
-		launch_perform_init(s, m, C);
+
+		// rank window: the first pass guesses from the triangle goal, later passes from the
+		// fraction of candidates the previous pass evaluated; doubled while no stop is found
+		size_t W = ctx.window_fraction > 0 ? (size_t)(ctx.window_fraction * (double)C * 1.25) : (size_t)goal * 3 / 4;
+		W = std::min(C, std::max<size_t>(W, 32768));
+		launch_perform_init(s, m, C, W);
 
 		int round = 0;
-		size_t bound = C;
-		int batch = std::max(ctx.last_rounds + 4, 8);
+		size_t lo = 0;
+		launch_window_extend(s, m, 0, W, round);
 		for (;;)
 		{
-			for (int r = 0; r < batch; ++r, ++round)
-				launch_round(s, m, bound, round);
-			launch_stop_scan(s, m, C, goal, error_limit);
+			int heads = ctx.head_rounds;
+			for (int hd = 0; hd < heads; ++hd, ++round)
+				launch_round(s, m, W - lo, round, hd);
+			launch_round_tail(s, m, round, (uint32_t)(W - lo + 2));
+			launch_resolve(s, m, lo, W);
+			launch_stop_scan(s, m, W, C, goal, error_limit);
 			sync_scalars(ctx, m);
-			uint32_t left = h.list_count[round % 3];
-			if (left == 0)
+			if (h.work[3])
+				throw std::runtime_error("priority rounds did not converge (internal error)");
+			round += (int)h.tail_rounds;
+			m.epoch += h.tail_rounds;
+
+			// head rounds for the next window / pass: enough to bring the list under the tail size
+			int next = heads + 4;
+			for (int hd = 0; hd < heads && hd < 16; ++hd)
+				if (h.head_sizes[hd] < 4096)
+				{
+					next = hd;
+					break;
+				}
+			ctx.head_rounds = std::max(2, std::min(16, next));
+
+			if (h.stop < W || W == C)
 				break;
-			// some ranks are still undecided: the stop scan above saw an unfinished window
-			bound = left;
-			batch = 8;
+			lo = W;
+			W = std::min(C, W * 2);
+			launch_window_extend(s, m, lo, W, round);
 		}
-		launch_stop_commit(s, m, C, goal, error_limit);
-		ctx.last_rounds = round;
+		launch_stop_commit(s, m, W, goal, error_limit);
+		ctx.window_fraction = std::min(1.0, (double)(h.stop + 1) / (double)C);
 
 		meshopt_hip_pass_stats ps{};
 		ps.live_triangles = (unsigned int)T;

@@ -29,9 +29,12 @@ struct Mesh
 	float* va;               // weighted attributes (V x A)
 
 	uint32_t* cremap;        // collapse_remap
-	uint32_t* locked;        // collapse_locked
-	uint64_t* pend_any[2];   // round minima, endpoint role
-	uint64_t* pend_src[2];   // round minima, source role
+	uint32_t* lock_rank;     // rank of the commit that locked a class (collapse_locked)
+	uint64_t* pend_any[2];   // per-class round minima of the commit rounds
+	uint64_t* pend_src[2];   // the same over ranks whose source is the class
+	uint32_t* src_off;       // V + 1: candidate ranks grouped by source class
+	uint32_t* src_rank;      // candidates
+	uint32_t* src_val;       // the value each listed candidate's commit writes at the class primary
 	uint32_t epoch;
 
 	uint32_t* adj_off;       // V + 1: welded corner CSR offsets of the current pass
@@ -58,7 +61,7 @@ struct Mesh
 	uint32_t* r_r1;          // remap[r_v1]: target class
 	float* r_err;
 	uint8_t* status;
-	uint32_t* list[2];       // undecided work lists
+	uint4* list[2];          // undecided work lists: (rank, source class, target class, 0)
 	uint32_t* stop_tile;     // per-tile aggregates of the stop scan (3 words each)
 
 	uint32_t* temp;          // scan / sort temporaries
@@ -89,10 +92,13 @@ void launch_dual_count(hipStream_t s, Mesh& m, const uint32_t* raw_off);
 void launch_adjacency(hipStream_t s, Mesh& m, size_t tri_count);
 void launch_pick(hipStream_t s, Mesh& m, size_t tri_count);
 void launch_rank_sort(hipStream_t s, Mesh& m, size_t cand_count);
-void launch_perform_init(hipStream_t s, Mesh& m, size_t cand_count);
-void launch_round(hipStream_t s, Mesh& m, size_t list_bound, int parity);
-void launch_stop_scan(hipStream_t s, Mesh& m, size_t cand_count, uint32_t goal, float error_limit);
-void launch_stop_commit(hipStream_t s, Mesh& m, size_t cand_count, uint32_t goal, float error_limit);
+void launch_perform_init(hipStream_t s, Mesh& m, size_t cand_count, size_t window);
+void launch_round(hipStream_t s, Mesh& m, size_t list_bound, int round, int head_index);
+void launch_round_tail(hipStream_t s, Mesh& m, int round, uint32_t max_rounds);
+void launch_window_extend(hipStream_t s, Mesh& m, size_t lo, size_t hi, int round);
+void launch_resolve(hipStream_t s, Mesh& m, size_t lo, size_t hi);
+void launch_stop_scan(hipStream_t s, Mesh& m, size_t window, size_t cand_count, uint32_t goal, float error_limit);
+void launch_stop_commit(hipStream_t s, Mesh& m, size_t window, uint32_t goal, float error_limit);
 void launch_apply(hipStream_t s, Mesh& m, size_t cand_count);
 void launch_remap_triangles(hipStream_t s, Mesh& m, size_t tri_count);
 void launch_debug_bits(hipStream_t s, Mesh& m, size_t index_count);

@@ -138,7 +138,9 @@ __global__ void k_pick_write(const uint32_t* ib, size_t tris, const uint32_t* re
 		c_v1[base + k] = v1[k];
 		c_err[base + k] = __uint_as_float(bd[k]); // the bidirectional flag until ranked
 	}
-	atomicMax(out_count, base + (uint32_t)count);
+	// offsets are non-decreasing, so the processed triangles form a prefix; its last one writes C
+	if (t + 1 == tris || (uint64_t)offsets[t + 1] + 3 > capacity)
+		*out_count = base + (uint32_t)count;
 }
 
 void launch_pick(hipStream_t s, Mesh& m, size_t tri_count)
@@ -337,12 +339,18 @@ __device__ __forceinline__ void commit_collapse(uint32_t* cremap, uint32_t i0, u
 		cremap[i0] = i1;
 }
 
-struct RoundArgs
+enum : uint8_t
+{
+	kFlipPass = 4, // flip test passes in every remaining scenario; commits once lowest at its endpoints
+	kFlipFail = 5, // flip test fails in every scenario; resolved as skipped or rejected at the end
+};
+
+constexpr int kMaxAlternatives = 4;
+
+struct RoundState
 {
 	const uint32_t* r_v0;
 	const uint32_t* r_v1;
-	const uint32_t* r_r0;
-	const uint32_t* r_r1;
 	const uint32_t* adj_off;
 	const uint2* adj;
 	const float4* pos;
@@ -351,94 +359,279 @@ struct RoundArgs
 	const uint32_t* remap;
 	const uint32_t* loop;
 	const uint32_t* loopback;
+	const uint32_t* src_off;  // candidate ranks grouped by source class
+	const uint32_t* src_rank;
+	const uint32_t* src_val;  // what each such candidate's commit writes at the class primary
 	uint32_t* cremap;
-	uint32_t* locked;
+	uint32_t* lock_rank;      // rank of the commit that locked a class (kNone: unlocked)
 	uint8_t* status;
-	const uint64_t* rd_any;
-	const uint64_t* rd_src;
-	uint64_t* wr_any;
-	uint64_t* wr_src;
-	const uint32_t* in_list;
-	const uint32_t* in_count;
-	uint32_t* out_list;
-	uint32_t* out_count;
-	uint32_t* zero_count;
-	uint32_t epoch;
+	uint64_t* pend[2];        // per-class minimum rank published in a round (tagged with ~round)
+	uint64_t* pend_src[2];    // the same minimum over ranks whose source is the class
+	uint4* list[2];           // work-list entries: (rank, source class, target class, 0)
+	CallScalars* sc;
 };
 
-__global__ void __launch_bounds__(kThreads) k_round(RoundArgs a)
+// collapse_remap[x] as the sequential loop sees it at rank k: a move committed by a higher
+// rank (possible once flip tests are decided early) reads as "not moved"
+__device__ __forceinline__ uint32_t remap_seen(const RoundState& a, uint32_t x, uint32_t k)
 {
-	if (blockIdx.x == 0 && threadIdx.x == 0)
-		*a.zero_count = 0;
+	uint32_t v = a.cremap[x];
+	return (v != x && a.lock_rank[x] > k) ? x : v;
+}
 
-	const uint32_t n = *a.in_count;
-	const uint32_t prev_tag = ~(a.epoch - 1);
-	const int lane = __lane_id();
-
-	for (uint32_t base = blockIdx.x * kThreads; base < n; base += gridDim.x * kThreads)
+// the value rank r's commit writes into collapse_remap[p], p the primary of its source class
+// (src/simplifier.cpp:1573-1605)
+__device__ __forceinline__ uint32_t commit_value_at(const RoundState& a, uint32_t r, uint32_t p)
+{
+	uint32_t i0 = a.r_v0[r], i1 = a.r_v1[r];
+	uint8_t kd = a.kind[i0];
+	if (kd == kComplex)
+		return complex_target(p, i1, a.remap, a.loop, a.loopback);
+	if (kd == kSeam)
 	{
-		uint32_t idx = base + threadIdx.x;
-		bool keep = false;
-		uint32_t k = 0;
-		if (idx < n)
-		{
-			k = a.in_list[idx];
-			uint32_t r0 = a.r_r0[k], r1 = a.r_r1[k];
-			if (a.locked[r0] | a.locked[r1])
-				a.status[k] = kSkipped;
-			else
-			{
-				uint64_t mine = round_key(a.epoch - 1, k);
-				bool ready = a.rd_any[r0] == mine && a.rd_any[r1] == mine;
-				if (ready)
-				{
-					for (uint32_t p = a.adj_off[r0], e = a.adj_off[r0 + 1]; p < e; ++p)
-					{
-						uint2 nb = a.adj[p];
-						uint64_t pa = a.rd_src[nb.x], pb = a.rd_src[nb.y];
-						if (((uint32_t)(pa >> 32) == prev_tag && (uint32_t)pa < k) || ((uint32_t)(pb >> 32) == prev_tag && (uint32_t)pb < k))
-						{
-							ready = false;
-							break;
-						}
-					}
-				}
+		uint32_t s0 = a.wedge[i0];
+		uint32_t s1 = a.loop[i0] == i1 ? a.loopback[s0] : a.loop[s0];
+		s1 = (s1 != kNone) ? s1 : a.wedge[i1];
+		uint32_t v = p;
+		if (i0 == p)
+			v = i1;
+		if (s0 == p)
+			v = s1;
+		return v;
+	}
+	return i1;
+}
 
-				if (ready)
-				{
-					if (collapse_flips(a.adj_off, a.adj, a.pos, a.cremap, r0, r1))
-						a.status[k] = kRejected;
-					else
-					{
-						commit_collapse(a.cremap, a.r_v0[k], a.r_v1[k], a.kind, a.wedge, a.remap, a.loop, a.loopback);
-						a.locked[r0] = 1;
-						a.locked[r1] = 1;
-						a.status[k] = kCommitted;
-					}
-				}
-				else
-				{
-					uint64_t key = round_key(a.epoch, k);
-					atomicMin((unsigned long long*)&a.wr_any[r0], (unsigned long long)key);
-					atomicMin((unsigned long long*)&a.wr_any[r1], (unsigned long long)key);
-					atomicMin((unsigned long long*)&a.wr_src[r0], (unsigned long long)key);
-					keep = true;
-				}
+// flip test (src/simplifier.cpp:1200-1232) of rank k over every way the still-committable
+// lower ranks could move its neighbours: 0 undetermined, 1 passes in all, 2 fails in all.
+// A neighbour whose source-role minimum of the previous round is not below k has no pending
+// lower candidate (every pending rank publishes every round), so its list is not scanned.
+__device__ int flip_outcome(const RoundState& a, uint32_t i0, uint32_t i1, uint32_t k, uint32_t epoch)
+{
+	const uint64_t* prev_src = a.pend_src[(epoch - 1) & 1];
+	const uint32_t prev_tag = ~(epoch - 1);
+	V3 v0 = load_pos(a.pos, i0), v1 = load_pos(a.pos, i1);
+	bool all_pass = true;
+	for (uint32_t p = a.adj_off[i0], e = a.adj_off[i0 + 1]; p < e; ++p)
+	{
+		uint2 nb2 = a.adj[p];
+		uint32_t nb[2] = {nb2.x, nb2.y};
+		uint32_t alt[2][kMaxAlternatives];
+		int na[2];
+#pragma unroll
+		for (int q = 0; q < 2; ++q)
+		{
+			uint32_t x = nb[q];
+			na[q] = 0;
+			alt[q][na[q]++] = remap_seen(a, x, k);
+			uint64_t ps = prev_src[x];
+			if ((uint32_t)(ps >> 32) != prev_tag || (uint32_t)ps >= k)
+				continue;
+			for (uint32_t t = a.src_off[x], te = a.src_off[x + 1]; t < te; ++t)
+			{
+				uint32_t r = a.src_rank[t];
+				if (r >= k)
+					continue;
+				uint8_t st = a.status[r];
+				if (st != kUndecided && st != kFlipPass)
+					continue;
+				if (na[q] == kMaxAlternatives)
+					return 0;
+				alt[q][na[q]++] = a.src_val[t];
 			}
 		}
 
-		// wave-aggregated append of the ranks that stay undecided
-		uint64_t mask = __ballot(keep);
-		if (mask)
+		bool any_flip = false, any_ok = false;
+		for (int x = 0; x < na[0]; ++x)
+			for (int y = 0; y < na[1]; ++y)
+			{
+				if (nb[0] == nb[1] && x != y)
+					continue; // one vertex moves one way only
+				uint32_t va = alt[0][x], vb = alt[1][y];
+				if (va == i1 || vb == i1 || va == vb)
+					any_ok = true;
+				else if (triangle_flips(load_pos(a.pos, va), load_pos(a.pos, vb), v0, v1))
+					any_flip = true;
+				else
+					any_ok = true;
+			}
+		if (any_flip && !any_ok)
+			return 2;
+		if (any_flip)
+			all_pass = false;
+	}
+	return all_pass ? 1 : 0;
+}
+
+// phase X of round `epoch`: no commit happens during it.  Ranks at a locked endpoint are
+// skipped; pending flip tests are evaluated; ranks that can still commit publish their rank at
+// both endpoint classes.  True = stays on the list.
+__device__ __forceinline__ bool round_x(const RoundState& a, uint4 w, uint32_t epoch)
+{
+	const uint32_t k = w.x, r0 = w.y, r1 = w.z;
+	uint8_t st = a.status[k];
+	if (st != kUndecided && st != kFlipPass)
+		return false;
+	if (a.lock_rank[r0] != kNone || a.lock_rank[r1] != kNone)
+	{
+		a.status[k] = kSkipped;
+		return false;
+	}
+	// the flip test is only worth evaluating for a rank that was the lowest published at one of
+	// its endpoints in the previous round (others cannot commit yet)
+	const uint64_t* prev = a.pend[(epoch - 1) & 1];
+	uint64_t was = round_key(epoch - 1, k);
+	if (st == kUndecided && (prev[r0] == was || prev[r1] == was))
+	{
+		int o = flip_outcome(a, r0, r1, k, epoch);
+		if (o == 2)
 		{
-			uint32_t slot0 = 0;
-			int leader = __ffsll((long long)mask) - 1;
-			if (lane == leader)
-				slot0 = atomicAdd(a.out_count, (uint32_t)__popcll(mask));
-			slot0 = __shfl(slot0, leader, 64);
-			if (keep)
-				a.out_list[slot0 + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull))] = k;
+			a.status[k] = kFlipFail;
+			return false;
 		}
+		if (o == 1)
+			a.status[k] = kFlipPass;
+	}
+	uint64_t key = round_key(epoch, k);
+	unsigned long long* wr = (unsigned long long*)a.pend[epoch & 1];
+	atomicMin(&wr[r0], (unsigned long long)key);
+	atomicMin(&wr[r1], (unsigned long long)key);
+	atomicMin((unsigned long long*)&a.pend_src[epoch & 1][r0], (unsigned long long)key);
+	return true;
+}
+
+// phase Y of round `epoch`: a flip-passed rank that is the minimum published at both endpoint
+// classes commits (such ranks share no endpoint, so they never touch each other's state)
+__device__ __forceinline__ void round_y(const RoundState& a, uint4 w, uint32_t epoch)
+{
+	const uint32_t k = w.x, r0 = w.y, r1 = w.z;
+	if (a.status[k] != kFlipPass)
+		return;
+	const uint64_t* rd = a.pend[epoch & 1];
+	uint64_t mine = round_key(epoch, k);
+	if (rd[r0] != mine || rd[r1] != mine)
+		return;
+	commit_collapse(a.cremap, a.r_v0[k], a.r_v1[k], a.kind, a.wedge, a.remap, a.loop, a.loopback);
+	a.lock_rank[r0] = k;
+	a.lock_rank[r1] = k;
+	a.status[k] = kCommitted;
+}
+
+__device__ __forceinline__ void append_wave(bool keep, uint4 w, uint4* out_list, uint32_t* out_count)
+{
+	const int lane = __lane_id();
+	uint64_t mask = __ballot(keep);
+	if (mask)
+	{
+		uint32_t slot0 = 0;
+		int leader = __ffsll((long long)mask) - 1;
+		if (lane == leader)
+			slot0 = atomicAdd(out_count, (uint32_t)__popcll(mask));
+		slot0 = __shfl(slot0, leader, 64);
+		if (keep)
+			out_list[slot0 + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull))] = w;
+	}
+}
+
+// below this many listed ranks a head kernel runs on one workgroup
+constexpr uint32_t kSoloThreshold = 2048;
+
+// head round, phase X.  Round r reads list (r & 1) / counter (r % 3), appends the survivors
+// to list ((r + 1) & 1) / counter ((r + 1) % 3) and clears counter ((r + 2) % 3).
+__global__ void __launch_bounds__(kThreads) k_round_x(RoundState a, uint32_t round, uint32_t epoch, uint32_t head_index)
+{
+	CallScalars* sc = a.sc;
+	const uint32_t n = sc->list_count[round % 3];
+	if (blockIdx.x == 0 && threadIdx.x == 0)
+	{
+		sc->list_count[(round + 2) % 3] = 0;
+		if (head_index < 16)
+			sc->head_sizes[head_index] = n;
+	}
+	const bool solo = n < kSoloThreshold;
+	if (solo && blockIdx.x != 0)
+		return;
+	const uint4* in_list = a.list[round & 1];
+	uint4* out_list = a.list[(round + 1) & 1];
+	uint32_t* out_count = &sc->list_count[(round + 1) % 3];
+	const uint32_t first = solo ? 0 : blockIdx.x * kThreads;
+	const uint32_t step = solo ? kThreads : gridDim.x * kThreads;
+	for (uint32_t base = first; base < n; base += step)
+	{
+		uint32_t idx = base + threadIdx.x;
+		uint4 w = idx < n ? in_list[idx] : make_uint4(0, 0, 0, 0);
+		bool keep = idx < n && round_x(a, w, epoch);
+		append_wave(keep, w, out_list, out_count);
+	}
+}
+
+// head round, phase Y over the survivors of phase X
+__global__ void __launch_bounds__(kThreads) k_round_y(RoundState a, uint32_t round, uint32_t epoch)
+{
+	const uint32_t n = a.sc->list_count[(round + 1) % 3];
+	const bool solo = n < kSoloThreshold;
+	if (solo && blockIdx.x != 0)
+		return;
+	const uint4* list = a.list[(round + 1) & 1];
+	const uint32_t first = solo ? threadIdx.x : blockIdx.x * kThreads + threadIdx.x;
+	const uint32_t step = solo ? kThreads : gridDim.x * kThreads;
+	for (uint32_t idx = first; idx < n; idx += step)
+		round_y(a, list[idx], epoch);
+}
+
+constexpr int kTailThreads = 1024;
+
+// tail: one workgroup runs the remaining rounds back to back (phases separated by barriers)
+// until the list is empty.  The lowest undecided rank decides in every round, so a list of n
+// ranks empties within n + 1 rounds; max_rounds bounds the loop regardless.
+__global__ void __launch_bounds__(kTailThreads) k_round_tail(RoundState a, uint32_t round0, uint32_t epoch0, uint32_t max_rounds)
+{
+	__shared__ uint32_t s_n, s_out;
+	CallScalars* sc = a.sc;
+	if (threadIdx.x == 0)
+	{
+		s_n = sc->list_count[round0 % 3];
+		s_out = 0;
+	}
+	__syncthreads();
+
+	uint32_t round = round0, epoch = epoch0, done = 0;
+	for (;;)
+	{
+		uint32_t n = s_n;
+		if (n == 0 || done >= max_rounds)
+			break;
+		const uint4* in_list = a.list[round & 1];
+		uint4* out_list = a.list[(round + 1) & 1];
+		for (uint32_t base = 0; base < n; base += kTailThreads)
+		{
+			uint32_t idx = base + threadIdx.x;
+			uint4 w = idx < n ? in_list[idx] : make_uint4(0, 0, 0, 0);
+			bool keep = idx < n && round_x(a, w, epoch);
+			append_wave(keep, w, out_list, &s_out);
+		}
+		__syncthreads();
+		uint32_t m = s_out;
+		for (uint32_t idx = threadIdx.x; idx < m; idx += kTailThreads)
+			round_y(a, out_list[idx], epoch);
+		__syncthreads();
+		if (threadIdx.x == 0)
+		{
+			s_n = s_out;
+			s_out = 0;
+		}
+		round++;
+		epoch++;
+		done++;
+		__syncthreads();
+	}
+
+	if (threadIdx.x == 0)
+	{
+		sc->list_count[round % 3] = s_n;
+		sc->tail_rounds = done;
+		if (s_n)
+			sc->work[3] = 1; // round guard hit: the host reports an error
 	}
 }
 
@@ -449,35 +642,39 @@ __global__ void k_perform_init_status(uint8_t* status, size_t count)
 		status[k] = kUndecided;
 }
 
-__global__ void k_perform_init_scalars(CallScalars* sc, uint32_t count)
+__global__ void k_perform_init_scalars(CallScalars* sc)
 {
-	sc->list_count[0] = count;
+	sc->list_count[0] = 0;
 	sc->list_count[1] = 0;
 	sc->list_count[2] = 0;
 	sc->stop_reason = 0;
+	sc->tail_rounds = 0;
 }
 
-void launch_perform_init(hipStream_t s, Mesh& m, size_t cand_count)
+// candidate ranks grouped by source class (order inside a group is irrelevant)
+__global__ void k_src_count(const uint32_t* r_r0, size_t count, uint32_t* counts)
 {
-	iota_u32(s, m.cremap, m.vertex_count);
-	fill_u32(s, m.locked, 0, m.vertex_count);
-	if (cand_count)
-	{
-		k_perform_init_status<<<grid_for(cand_count), kThreads, 0, s>>>(m.status, cand_count);
-		iota_u32(s, m.list[0], cand_count);
-	}
-	k_perform_init_scalars<<<1, 1, 0, s>>>(m.sc, (uint32_t)cand_count);
+	size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (k < count)
+		atomicAdd(&counts[r_r0[k]], 1u);
 }
 
-// round r reads list (r & 1) with counter (r % 3), writes list ((r + 1) & 1) with counter
-// ((r + 1) % 3) and clears counter ((r + 2) % 3) for the round after
-void launch_round(hipStream_t s, Mesh& m, size_t list_bound, int round)
+__global__ void k_src_fill(RoundState a, const uint32_t* r_r0, size_t count, uint32_t* cursor, uint32_t* src_rank, uint32_t* src_val)
 {
-	RoundArgs a;
+	size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (k >= count)
+		return;
+	uint32_t r0 = r_r0[k];
+	uint32_t slot = atomicAdd(&cursor[r0], 1u);
+	src_rank[slot] = (uint32_t)k;
+	src_val[slot] = commit_value_at(a, (uint32_t)k, r0);
+}
+
+static RoundState round_state(Mesh& m)
+{
+	RoundState a;
 	a.r_v0 = m.r_v0;
 	a.r_v1 = m.r_v1;
-	a.r_r0 = m.r_r0;
-	a.r_r1 = m.r_r1;
 	a.adj_off = m.adj_off;
 	a.adj = m.adj;
 	a.pos = m.pos;
@@ -486,23 +683,92 @@ void launch_round(hipStream_t s, Mesh& m, size_t list_bound, int round)
 	a.remap = m.remap;
 	a.loop = m.loop[m.loop_cur];
 	a.loopback = m.loopback[m.loop_cur];
+	a.src_off = m.src_off;
+	a.src_rank = m.src_rank;
+	a.src_val = m.src_val;
 	a.cremap = m.cremap;
-	a.locked = m.locked;
+	a.lock_rank = m.lock_rank;
 	a.status = m.status;
+	for (int b = 0; b < 2; ++b)
+	{
+		a.pend[b] = m.pend_any[b];
+		a.pend_src[b] = m.pend_src[b];
+		a.list[b] = m.list[b];
+	}
+	a.sc = m.sc;
+	return a;
+}
+
+void launch_perform_init(hipStream_t s, Mesh& m, size_t cand_count, size_t window)
+{
+	(void)window;
+	size_t n = m.vertex_count;
+	iota_u32(s, m.cremap, n);
+	fill_u32(s, m.lock_rank, kNone, n);
+	fill_u32(s, m.src_off, 0, n + 1);
+	if (cand_count)
+	{
+		k_perform_init_status<<<grid_for(cand_count), kThreads, 0, s>>>(m.status, cand_count);
+		k_src_count<<<grid_for(cand_count), kThreads, 0, s>>>(m.r_r0, cand_count, m.src_off);
+	}
+	scan_exclusive(s, m.src_off, m.src_off, n, nullptr, m.src_off + n, m.temp);
+	if (cand_count)
+	{
+		HIP_CHECK(hipMemcpyAsync(m.adj_cursor, m.src_off, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+		k_src_fill<<<grid_for(cand_count), kThreads, 0, s>>>(round_state(m), m.r_r0, cand_count, m.adj_cursor, m.src_rank, m.src_val);
+	}
+	k_perform_init_scalars<<<1, 1, 0, s>>>(m.sc);
+}
+
+// ranks [lo, hi) become the list of round `round` (its counter reads 0 when this runs)
+__global__ void k_window_open(uint4* list, uint32_t lo, uint32_t hi, const uint32_t* r_r0, const uint32_t* r_r1, uint32_t* count, uint32_t* next_count)
+{
+	uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i == 0)
+	{
+		*count = hi - lo;
+		*next_count = 0;
+	}
+	uint32_t k = lo + i;
+	if (k < hi)
+		list[i] = make_uint4(k, r_r0[k], r_r1[k], 0);
+}
+
+void launch_window_extend(hipStream_t s, Mesh& m, size_t lo, size_t hi, int round)
+{
+	k_window_open<<<grid_for(hi > lo ? hi - lo : 1), kThreads, 0, s>>>(m.list[round & 1], (uint32_t)lo, (uint32_t)hi, m.r_r0, m.r_r1,
+	                                                                    &m.sc->list_count[round % 3], &m.sc->list_count[(round + 1) % 3]);
+}
+
+void launch_round(hipStream_t s, Mesh& m, size_t list_bound, int round, int head_index)
+{
 	uint32_t e = ++m.epoch;
-	a.epoch = e;
-	a.rd_any = m.pend_any[(e - 1) & 1];
-	a.rd_src = m.pend_src[(e - 1) & 1];
-	a.wr_any = m.pend_any[e & 1];
-	a.wr_src = m.pend_src[e & 1];
-	a.in_list = m.list[round & 1];
-	a.out_list = m.list[(round + 1) & 1];
-	a.in_count = &m.sc->list_count[round % 3];
-	a.out_count = &m.sc->list_count[(round + 1) % 3];
-	a.zero_count = &m.sc->list_count[(round + 2) % 3];
 	unsigned int blocks = grid_for(list_bound);
-	blocks = blocks < 1 ? 1 : (blocks > 4096 ? 4096 : blocks);
-	k_round<<<blocks, kThreads, 0, s>>>(a);
+	blocks = blocks < 1 ? 1 : (blocks > 1024 ? 1024 : blocks);
+	RoundState a = round_state(m);
+	k_round_x<<<blocks, kThreads, 0, s>>>(a, (uint32_t)round, e, (uint32_t)head_index);
+	k_round_y<<<blocks, kThreads, 0, s>>>(a, (uint32_t)round, e);
+}
+
+void launch_round_tail(hipStream_t s, Mesh& m, int round, uint32_t max_rounds)
+{
+	k_round_tail<<<1, kTailThreads, 0, s>>>(round_state(m), (uint32_t)round, m.epoch + 1, max_rounds);
+}
+
+// a failed flip test is a rejection unless a lower commit locked an endpoint first
+__global__ void k_resolve_flip_fail(uint8_t* status, const uint32_t* r_r0, const uint32_t* r_r1, const uint32_t* lock_rank, uint32_t lo, uint32_t hi)
+{
+	uint32_t k = lo + blockIdx.x * blockDim.x + threadIdx.x;
+	if (k >= hi || status[k] != kFlipFail)
+		return;
+	uint32_t l0 = lock_rank[r_r0[k]], l1 = lock_rank[r_r1[k]];
+	status[k] = ((l0 != kNone && l0 < k) || (l1 != kNone && l1 < k)) ? kSkipped : kRejected;
+}
+
+void launch_resolve(hipStream_t s, Mesh& m, size_t lo, size_t hi)
+{
+	if (hi > lo)
+		k_resolve_flip_fail<<<grid_for(hi - lo), kThreads, 0, s>>>(m.status, m.r_r0, m.r_r1, m.lock_rank, (uint32_t)lo, (uint32_t)hi);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -637,7 +903,8 @@ __global__ void __launch_bounds__(kThreads) k_stop_tiles(uint32_t* tiles, uint32
 }
 
 __global__ void __launch_bounds__(kThreads) k_stop_find(const uint8_t* status, const uint32_t* r_v0, const uint8_t* kind, const float* r_err,
-                                                        size_t count, const uint32_t* tiles, uint32_t goal, float error_limit, CallScalars* sc)
+                                                        size_t count, size_t cand_count, const uint32_t* tiles, uint32_t goal, float error_limit,
+                                                        CallScalars* sc)
 {
 	size_t base = (size_t)blockIdx.x * kScanTile;
 	StopAgg carry{tiles[blockIdx.x * 3 + 0], tiles[blockIdx.x * 3 + 1], tiles[blockIdx.x * 3 + 2]};
@@ -661,7 +928,7 @@ __global__ void __launch_bounds__(kThreads) k_stop_find(const uint8_t* status, c
 			uint32_t eb = ex.err > carry.err ? ex.err : carry.err;
 			float result_error = __uint_as_float(eb);
 			size_t eg = (size_t)half + rej;
-			float error_goal = eg < count ? 1.5f * r_err[eg] : FLT_MAX;
+			float error_goal = eg < cand_count ? 1.5f * r_err[eg] : FLT_MAX;
 			bool stop = err > error_limit || tri >= goal || (err > error_goal && err > result_error && tri > sixth);
 			if (stop)
 				atomicMin(&sc->stop, (uint32_t)k);
@@ -672,44 +939,56 @@ __global__ void __launch_bounds__(kThreads) k_stop_find(const uint8_t* status, c
 	}
 }
 
-// totals below the stop rank
+// totals below the stop rank (one atomic per workgroup and counter)
 __global__ void __launch_bounds__(kThreads) k_stop_totals(const uint8_t* status, const uint32_t* r_v0, const uint8_t* kind, const float* r_err,
                                                           CallScalars* sc)
 {
+	__shared__ uint32_t red[5][4];
 	uint32_t stop = sc->stop;
-	uint32_t committed = 0, skipped = 0, rejected = 0, tri = 0, eb = 0;
+	uint32_t v[5] = {0, 0, 0, 0, 0}; // committed, skipped, rejected, tri, max error bits
 	for (uint32_t k = blockIdx.x * kThreads + threadIdx.x; k < stop; k += gridDim.x * kThreads)
 	{
 		uint8_t st = status[k];
 		StopAgg a = stop_item(st, kind[r_v0[k]], r_err[k]);
-		committed += st == kCommitted;
-		skipped += st == kSkipped;
-		rejected += st == kRejected;
-		tri += a.tri;
-		eb = a.err > eb ? a.err : eb;
+		v[0] += st == kCommitted;
+		v[1] += st == kSkipped;
+		v[2] += st == kRejected;
+		v[3] += a.tri;
+		v[4] = a.err > v[4] ? a.err : v[4];
 	}
 #pragma unroll
 	for (int o = 32; o > 0; o >>= 1)
 	{
-		committed += __shfl_xor(committed, o, 64);
-		skipped += __shfl_xor(skipped, o, 64);
-		rejected += __shfl_xor(rejected, o, 64);
-		tri += __shfl_xor(tri, o, 64);
-		uint32_t t = __shfl_xor(eb, o, 64);
-		eb = t > eb ? t : eb;
+#pragma unroll
+		for (int q = 0; q < 4; ++q)
+			v[q] += __shfl_xor(v[q], o, 64);
+		uint32_t t = __shfl_xor(v[4], o, 64);
+		v[4] = t > v[4] ? t : v[4];
 	}
+	const int wave = threadIdx.x >> 6;
 	if (__lane_id() == 0)
+		for (int q = 0; q < 5; ++q)
+			red[q][wave] = v[q];
+	__syncthreads();
+	if (threadIdx.x == 0)
 	{
-		if (committed)
-			atomicAdd(&sc->committed, committed);
-		if (skipped)
-			atomicAdd(&sc->skipped, skipped);
-		if (rejected)
-			atomicAdd(&sc->rejected, rejected);
-		if (tri)
-			atomicAdd(&sc->tri_collapses, tri);
-		if (eb)
-			atomicMax(&sc->work[1], eb);
+		uint32_t t[5] = {0, 0, 0, 0, 0};
+		for (int w = 0; w < kThreads / 64; ++w)
+		{
+			for (int q = 0; q < 4; ++q)
+				t[q] += red[q][w];
+			t[4] = red[4][w] > t[4] ? red[4][w] : t[4];
+		}
+		if (t[0])
+			atomicAdd(&sc->committed, t[0]);
+		if (t[1])
+			atomicAdd(&sc->skipped, t[1]);
+		if (t[2])
+			atomicAdd(&sc->rejected, t[2]);
+		if (t[3])
+			atomicAdd(&sc->tri_collapses, t[3]);
+		if (t[4])
+			atomicMax(&sc->work[1], t[4]);
 	}
 }
 
@@ -748,9 +1027,9 @@ __global__ void k_revert(const uint8_t* status, const uint32_t* r_r0, const uint
 	} while (v != r);
 }
 
-__global__ void k_stop_reset(CallScalars* sc, uint32_t count)
+__global__ void k_stop_reset(CallScalars* sc, uint32_t window)
 {
-	sc->stop = count;
+	sc->stop = window;
 	sc->committed = 0;
 	sc->skipped = 0;
 	sc->rejected = 0;
@@ -758,28 +1037,29 @@ __global__ void k_stop_reset(CallScalars* sc, uint32_t count)
 	sc->work[1] = 0;
 }
 
-// repeatable: reads the decisions and writes only the stop rank and the totals below it
-void launch_stop_scan(hipStream_t s, Mesh& m, size_t cand_count, uint32_t goal, float error_limit)
+// repeatable: reads the decisions of ranks [0, window) and writes only the stop rank (window
+// if no rank in it breaks the loop) and the totals below it
+void launch_stop_scan(hipStream_t s, Mesh& m, size_t window, size_t cand_count, uint32_t goal, float error_limit)
 {
-	k_stop_reset<<<1, 1, 0, s>>>(m.sc, (uint32_t)cand_count);
-	if (!cand_count)
+	k_stop_reset<<<1, 1, 0, s>>>(m.sc, (uint32_t)window);
+	if (!window)
 		return;
-	unsigned int tiles = div_up(cand_count, kScanTile);
-	k_stop_reduce<<<tiles, kThreads, 0, s>>>(m.status, m.r_v0, m.kind, m.r_err, cand_count, m.stop_tile);
+	unsigned int tiles = div_up(window, kScanTile);
+	k_stop_reduce<<<tiles, kThreads, 0, s>>>(m.status, m.r_v0, m.kind, m.r_err, window, m.stop_tile);
 	k_stop_tiles<<<1, kThreads, 0, s>>>(m.stop_tile, tiles, m.sc);
-	k_stop_find<<<tiles, kThreads, 0, s>>>(m.status, m.r_v0, m.kind, m.r_err, cand_count, m.stop_tile, goal, error_limit, m.sc);
-	unsigned int blocks = grid_for(cand_count);
-	blocks = blocks > 1024 ? 1024 : blocks;
+	k_stop_find<<<tiles, kThreads, 0, s>>>(m.status, m.r_v0, m.kind, m.r_err, window, cand_count, m.stop_tile, goal, error_limit, m.sc);
+	unsigned int blocks = div_up(window, kThreads * 4);
+	blocks = blocks < 1 ? 1 : (blocks > 512 ? 512 : blocks);
 	k_stop_totals<<<blocks, kThreads, 0, s>>>(m.status, m.r_v0, m.kind, m.r_err, m.sc);
 }
 
 // once every rank of the window is decided: advance result_error and undo the commits at or
 // past the stop rank
-void launch_stop_commit(hipStream_t s, Mesh& m, size_t cand_count, uint32_t goal, float error_limit)
+void launch_stop_commit(hipStream_t s, Mesh& m, size_t window, uint32_t goal, float error_limit)
 {
-	k_stop_finish<<<1, 1, 0, s>>>(m.r_err, (uint32_t)cand_count, goal, error_limit, m.sc);
-	if (cand_count)
-		k_revert<<<grid_for(cand_count), kThreads, 0, s>>>(m.status, m.r_r0, m.wedge, cand_count, m.sc, m.cremap);
+	k_stop_finish<<<1, 1, 0, s>>>(m.r_err, (uint32_t)window, goal, error_limit, m.sc);
+	if (window)
+		k_revert<<<grid_for(window), kThreads, 0, s>>>(m.status, m.r_r0, m.wedge, window, m.sc, m.cremap);
 }
 
 // ------------------------------------------------------------------------------------------

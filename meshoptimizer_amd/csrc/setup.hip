@@ -486,6 +486,7 @@ __device__ __forceinline__ uint64_t wave_max64(uint64_t v)
 
 __global__ void __launch_bounds__(256) k_bbox(const float4* praw, size_t n, CallScalars* sc)
 {
+	__shared__ uint64_t red[6][4];
 	uint64_t mn[3], mx[3];
 #pragma unroll
 	for (int k = 0; k < 3; ++k)
@@ -509,6 +510,7 @@ __global__ void __launch_bounds__(256) k_bbox(const float4* praw, size_t n, Call
 			mx[k] = b > mx[k] ? b : mx[k];
 		}
 	}
+	const int wave = threadIdx.x >> 6;
 #pragma unroll
 	for (int k = 0; k < 3; ++k)
 	{
@@ -516,11 +518,24 @@ __global__ void __launch_bounds__(256) k_bbox(const float4* praw, size_t n, Call
 		uint64_t b = wave_max64(mx[k]);
 		if (__lane_id() == 0)
 		{
-			if (a != ~0ull)
-				atomicMin((unsigned long long*)&sc->box_min[k], (unsigned long long)a);
-			if (b != 0ull)
-				atomicMax((unsigned long long*)&sc->box_max[k], (unsigned long long)b);
+			red[k][wave] = a;
+			red[3 + k][wave] = b;
 		}
+	}
+	__syncthreads();
+	if (threadIdx.x < 3)
+	{
+		int k = threadIdx.x;
+		uint64_t a = ~0ull, b = 0ull;
+		for (int w = 0; w < 4; ++w)
+		{
+			a = red[k][w] < a ? red[k][w] : a;
+			b = red[3 + k][w] > b ? red[3 + k][w] : b;
+		}
+		if (a != ~0ull)
+			atomicMin((unsigned long long*)&sc->box_min[k], (unsigned long long)a);
+		if (b != 0ull)
+			atomicMax((unsigned long long*)&sc->box_max[k], (unsigned long long)b);
 	}
 }
 
@@ -584,8 +599,8 @@ void launch_rescale(hipStream_t s, Mesh& m)
 	k_bbox_init<<<1, 1, 0, s>>>(m.sc);
 	if (n)
 	{
-		unsigned int blocks = grid_for(n);
-		blocks = blocks > 2048 ? 2048 : blocks;
+		unsigned int blocks = div_up(n, 256 * 8);
+		blocks = blocks > 512 ? 512 : blocks;
 		k_bbox<<<blocks, 256, 0, s>>>(m.praw, n, m.sc);
 	}
 	k_extent<<<1, 1, 0, s>>>(m.praw, m.sc);
@@ -600,8 +615,8 @@ void launch_extent_only(hipStream_t s, Mesh& m)
 	k_bbox_init<<<1, 1, 0, s>>>(m.sc);
 	if (n)
 	{
-		unsigned int blocks = grid_for(n);
-		blocks = blocks > 2048 ? 2048 : blocks;
+		unsigned int blocks = div_up(n, 256 * 8);
+		blocks = blocks > 512 ? 512 : blocks;
 		k_bbox<<<blocks, 256, 0, s>>>(m.praw, n, m.sc);
 	}
 	k_extent<<<1, 1, 0, s>>>(m.praw, m.sc);
@@ -819,26 +834,37 @@ void launch_attribute_quadrics(hipStream_t s, Mesh& m, const uint32_t* raw_off, 
 }
 
 // boundEdgeCollapses dual count, src/simplifier.cpp:1292-1308
-__global__ void k_dual_count(const uint8_t* kind, const uint32_t* off, size_t n, uint32_t* out)
+__global__ void __launch_bounds__(256) k_dual_count(const uint8_t* kind, const uint32_t* off, size_t n, uint32_t* out)
 {
-	size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+	__shared__ uint32_t red[4];
 	uint32_t v = 0;
-	if (i < n)
+	for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
 	{
 		uint8_t k = kind[i];
-		v = (k == kManifold || k == kSeam) ? off[i + 1] - off[i] : 0u;
+		v += (k == kManifold || k == kSeam) ? off[i + 1] - off[i] : 0u;
 	}
 	for (int o = 32; o > 0; o >>= 1)
 		v += __shfl_xor(v, o, 64);
-	if (__lane_id() == 0 && v)
-		atomicAdd(out, v);
+	if (__lane_id() == 0)
+		red[threadIdx.x >> 6] = v;
+	__syncthreads();
+	if (threadIdx.x == 0)
+	{
+		uint32_t t = red[0] + red[1] + red[2] + red[3];
+		if (t)
+			atomicAdd(out, t);
+	}
 }
 
 void launch_dual_count(hipStream_t s, Mesh& m, const uint32_t* raw_off)
 {
 	HIP_CHECK(hipMemsetAsync(&m.sc->dual_count, 0, sizeof(uint32_t), s));
 	if (m.vertex_count)
-		k_dual_count<<<grid_for(m.vertex_count), kThreads, 0, s>>>(m.kind, raw_off, m.vertex_count, &m.sc->dual_count);
+	{
+		unsigned int blocks = div_up(m.vertex_count, 256 * 8);
+		blocks = blocks > 512 ? 512 : blocks;
+		k_dual_count<<<blocks, 256, 0, s>>>(m.kind, raw_off, m.vertex_count, &m.sc->dual_count);
+	}
 }
 
 } // namespace mhip

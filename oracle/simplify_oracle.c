@@ -1056,55 +1056,91 @@ static perform_result perform_sequential(unsigned int* cremap, unsigned char* lo
 /*
  * Round-based restatement of the same greedy (the algorithm the GPU runs).
  *
- * Candidate k (sorted rank) depends only on earlier candidates j < k that share an endpoint
- * class with it (the lock test) or whose source class is a welded neighbour of k's source
- * class (the flip test reads collapse_remap only there, and only sources change it; the welded
- * neighbour relation is symmetric).  In round e every undecided candidate of the rank window
- * publishes its rank as a per-vertex minimum -- endpoint role at r0 and r1, source role at r0 --
- * into buffer e&1, tagged with e so that stale entries read as "none".  In round e+1 a candidate
- * whose endpoints are locked is skipped (a lock can only come from an earlier commit), and a
- * candidate that is the published minimum at both endpoints and below every source-role entry
- * in its neighbourhood decides: flip-rejected or committed.  Candidates that decide in the same
- * round never touch each other's state, so the order of a sweep does not matter; the sweep
- * below visits them in a shuffled order to stand in for the GPU's concurrency.  The lowest
- * undecided rank is always ready in the next round, so every round makes progress.
- * After the window is decided, prefix sums (triangles removed, flip rejects) and a prefix max
- * (committed error) locate the first rank where the reference loop breaks; commits at or past
- * it are reverted.  If the window ends before any break, it is doubled.
+ * Sequentially, rank k is skipped if a lower commit locked one of its endpoint classes,
+ * rejected if its flip test fails, committed otherwise.  Its flip test reads collapse_remap
+ * only at the welded neighbours x of its source class, and collapse_remap[x] changes only when
+ * a candidate whose source class is x commits.  So:
+ *   - once every lower candidate with a source in k's neighbourhood is decided, k's flip result
+ *     is fixed; it is fixed even earlier when it comes out the same for every way the pending
+ *     ones could still move (each pending lower candidate j at x either leaves x or moves it to
+ *     the target j's commit would write) -- "outcome-invariant" evaluation;
+ *   - a candidate whose flip fails can never commit, so it blocks nobody; it is resolved at the
+ *     end as skipped (a lower commit locked an endpoint) or rejected;
+ *   - a candidate whose flip passes commits once it is the lowest still-committable candidate at
+ *     both endpoint classes; a candidate at a locked endpoint is skipped.
+ * Higher-ranked commits may happen before a lower candidate's flip test; collapse_remap reads
+ * are therefore rank-tagged: a move made by a commit ranked above the reader reads as "not
+ * moved" (every class is locked by at most one commit, so lock_rank[x] tags x's move).
+ * Each round has two phases with no commit in the first:
+ *   X: skip candidates at locked endpoints; evaluate the pending flip tests (outcome-invariant)
+ *      of candidates that were the lowest published at one of their endpoints in round e-1;
+ *      candidates still able to commit publish their rank as a per-vertex minimum at both
+ *      endpoint classes into buffer e&1, tagged with the round e (stale entries read as none);
+ *   Y: a flip-passed candidate that is the minimum published at both endpoints commits.
+ * Phase sweeps visit candidates in a shuffled order, standing in for the GPU's concurrency.
+ * The lowest undecided candidate always decides within a round, so rounds make progress.
+ * A window of ranks [0, W) is decided first; prefix sums (triangles removed, flip rejects) and
+ * a prefix max (committed error) locate the first rank where the reference loop breaks, and the
+ * commits at or past it are reverted.  If the window holds no break, [W, 2W) is opened next.
  */
+enum
+{
+	ST_UNDECIDED = 0,
+	ST_SKIPPED = 1,
+	ST_REJECTED = 2,
+	ST_COMMITTED = 3,
+	ST_FLIP_PASS = 4,
+	ST_FLIP_FAIL = 5,
+};
+
+#define MAX_ALTERNATIVES 4
+
 typedef struct
 {
-	uint64_t* pend_any[2];
-	uint64_t* pend_src[2];
+	uint64_t* pend[2];
+	uint64_t* pend_src[2];   /* the same minimum over ranks whose source is the class */
+	unsigned int* lock_rank; /* rank of the commit that locked the class (NONE: unlocked) */
 	unsigned int epoch;
-	unsigned int* perm;
-	size_t perm_cap;
+	unsigned int* list[2];
+	size_t list_cap;
+	unsigned int* src_off;   /* per source class: candidate ranks of this pass */
+	unsigned int* src_rank;
+	size_t src_cap;
 	uint64_t rng;
+	size_t vertex_count;
 } round_state;
 
 static void round_state_init(round_state* rs, size_t vertex_count)
 {
 	for (int b = 0; b < 2; ++b)
 	{
-		rs->pend_any[b] = (uint64_t*)xalloc(vertex_count * sizeof(uint64_t) + 8);
+		rs->pend[b] = (uint64_t*)xalloc(vertex_count * sizeof(uint64_t) + 8);
+		memset(rs->pend[b], 0xff, vertex_count * sizeof(uint64_t));
 		rs->pend_src[b] = (uint64_t*)xalloc(vertex_count * sizeof(uint64_t) + 8);
-		memset(rs->pend_any[b], 0xff, vertex_count * sizeof(uint64_t));
 		memset(rs->pend_src[b], 0xff, vertex_count * sizeof(uint64_t));
 	}
+	rs->lock_rank = (unsigned int*)xalloc(vertex_count * sizeof(unsigned int) + 4);
+	rs->src_off = (unsigned int*)xalloc((vertex_count + 2) * sizeof(unsigned int));
 	rs->epoch = 0;
-	rs->perm = NULL;
-	rs->perm_cap = 0;
+	rs->list[0] = rs->list[1] = NULL;
+	rs->list_cap = 0;
+	rs->src_rank = NULL;
+	rs->src_cap = 0;
 	rs->rng = 0x243F6A8885A308D3ull;
+	rs->vertex_count = vertex_count;
 }
 
 static void round_state_free(round_state* rs)
 {
-	for (int b = 0; b < 2; ++b)
-	{
-		free(rs->pend_any[b]);
-		free(rs->pend_src[b]);
-	}
-	free(rs->perm);
+	free(rs->pend[0]);
+	free(rs->pend[1]);
+	free(rs->pend_src[0]);
+	free(rs->pend_src[1]);
+	free(rs->lock_rank);
+	free(rs->src_off);
+	free(rs->src_rank);
+	free(rs->list[0]);
+	free(rs->list[1]);
 }
 
 static uint64_t rs_next(round_state* rs)
@@ -1120,19 +1156,183 @@ static uint64_t pend_key(unsigned int epoch, size_t k)
 	return ((uint64_t)(~epoch) << 32) | (uint64_t)k;
 }
 
+static void shuffle(round_state* rs, unsigned int* v, size_t n)
+{
+	for (size_t i = n; i > 1; --i)
+	{
+		size_t j = (size_t)(rs_next(rs) % i);
+		unsigned int t = v[i - 1];
+		v[i - 1] = v[j];
+		v[j] = t;
+	}
+}
+
+/* collapse_remap[x] as the sequential loop sees it at rank k */
+static unsigned int remap_seen(const unsigned int* cremap, const unsigned int* lock_rank, unsigned int x, size_t k)
+{
+	unsigned int v = cremap[x];
+	return (v != x && lock_rank[x] > k) ? x : v;
+}
+
+/* the value candidate c's commit writes into collapse_remap[p], p the primary of its source
+ * class (src/simplifier.cpp:1573-1605) */
+static unsigned int commit_value_at(const cand* c, unsigned int p, const unsigned int* remap, const unsigned int* wedge, const unsigned char* kind, const unsigned int* loop, const unsigned int* loopback)
+{
+	unsigned int i0 = c->v0, i1 = c->v1;
+	unsigned char kd = kind[i0];
+	if (kd == K_COMPLEX)
+		return complex_target(p, i1, remap, loop, loopback);
+	if (kd == K_SEAM)
+	{
+		unsigned int s0 = wedge[i0];
+		unsigned int s1 = loop[i0] == i1 ? loopback[s0] : loop[s0];
+		s1 = (s1 != NONE) ? s1 : wedge[i1];
+		unsigned int v = p;
+		if (i0 == p)
+			v = i1;
+		if (s0 == p)
+			v = s1;
+		return v;
+	}
+	return i1;
+}
+
+/* flip test of rank k (source class i0, target class i1) over every way the pending lower
+ * candidates could still move its neighbours: 0 undetermined, 1 passes in all, 2 fails in all */
+static int flip_outcome(const round_state* rs, unsigned int e, const corner_csr* adj, const vec3* vp, const unsigned int* cremap, unsigned int i0, unsigned int i1, size_t k, const unsigned char* st, const cand* cs, const unsigned int* order, const unsigned int* remap, const unsigned int* wedge, const unsigned char* kind, const unsigned int* loop, const unsigned int* loopback)
+{
+	vec3 v0 = vp[i0], v1 = vp[i1];
+	int all_pass = 1;
+
+	for (unsigned int j = adj->offsets[i0]; j < adj->offsets[i0 + 1]; ++j)
+	{
+		unsigned int nb[2] = {adj->next[j], adj->prev[j]};
+		unsigned int alt[2][MAX_ALTERNATIVES];
+		int na[2];
+
+		for (int q = 0; q < 2; ++q)
+		{
+			unsigned int x = nb[q];
+			na[q] = 0;
+			alt[q][na[q]++] = remap_seen(cremap, rs->lock_rank, x, k);
+			/* every pending rank publishes its source role each round: a class whose minimum of
+			 * the previous round is not below k has no pending lower candidate */
+			uint64_t ps = rs->pend_src[(e - 1) & 1][x];
+			if ((ps >> 32) != (uint64_t)(unsigned int)~(e - 1) || (ps & 0xffffffffu) >= k)
+				continue;
+			for (unsigned int t = rs->src_off[x]; t < rs->src_off[x + 1]; ++t)
+			{
+				unsigned int r = rs->src_rank[t];
+				if (r < k && (st[r] == ST_UNDECIDED || st[r] == ST_FLIP_PASS))
+				{
+					if (na[q] == MAX_ALTERNATIVES)
+						return 0;
+					alt[q][na[q]++] = commit_value_at(&cs[order[r]], x, remap, wedge, kind, loop, loopback);
+				}
+			}
+		}
+
+		int any_flip = 0, any_ok = 0;
+		for (int x = 0; x < na[0]; ++x)
+			for (int y = 0; y < na[1]; ++y)
+			{
+				if (nb[0] == nb[1] && x != y)
+					continue; /* the same vertex moves one way only */
+				unsigned int a = alt[0][x], b = alt[1][y];
+				if (a == i1 || b == i1 || a == b)
+					any_ok = 1;
+				else if (tri_flips(vp[a], vp[b], v0, v1))
+					any_flip = 1;
+				else
+					any_ok = 1;
+			}
+
+		if (any_flip && !any_ok)
+			return 2;
+		if (any_flip)
+			all_pass = 0;
+	}
+
+	return all_pass ? 1 : 0;
+}
+
+/* phase X for rank k in round e; returns 1 if it stays on the list */
+static int round_x(round_state* rs, unsigned char* st, size_t k, unsigned int r0, unsigned int r1, unsigned int e, const corner_csr* adj, const vec3* vp, const unsigned int* cremap, const cand* cs, const unsigned int* order, const unsigned int* remap, const unsigned int* wedge, const unsigned char* kind, const unsigned int* loop, const unsigned int* loopback)
+{
+	if (st[k] != ST_UNDECIDED && st[k] != ST_FLIP_PASS)
+		return 0;
+	if (rs->lock_rank[r0] != NONE || rs->lock_rank[r1] != NONE)
+	{
+		st[k] = ST_SKIPPED;
+		return 0;
+	}
+	/* the flip test is only worth evaluating for a rank that was the lowest published at one of
+	 * its endpoints in the previous round: others cannot commit yet, and evaluating them later
+	 * costs few rounds and saves most of the work */
+	const uint64_t* prev = rs->pend[(e - 1) & 1];
+	uint64_t was = pend_key(e - 1, k);
+	if (st[k] == ST_UNDECIDED && (prev[r0] == was || prev[r1] == was))
+	{
+		int o = flip_outcome(rs, e, adj, vp, cremap, r0, r1, k, st, cs, order, remap, wedge, kind, loop, loopback);
+		if (o == 2)
+		{
+			st[k] = ST_FLIP_FAIL;
+			return 0;
+		}
+		if (o == 1)
+			st[k] = ST_FLIP_PASS;
+	}
+	uint64_t key = pend_key(e, k);
+	uint64_t* w = rs->pend[e & 1];
+	if (w[r0] > key)
+		w[r0] = key;
+	if (w[r1] > key)
+		w[r1] = key;
+	uint64_t* ws = rs->pend_src[e & 1];
+	if (ws[r0] > key)
+		ws[r0] = key;
+	return 1;
+}
+
 static perform_result perform_rounds(round_state* rs, unsigned int* cremap, unsigned char* locked, const cand* cs, size_t count, const unsigned int* order, const unsigned int* remap, const unsigned int* wedge, const unsigned char* kind, const unsigned int* loop, const unsigned int* loopback, const vec3* vp, const corner_csr* adj, size_t goal, float error_limit, float* result_error, size_t window0)
 {
 	perform_result pr;
 	memset(&pr, 0, sizeof(pr));
+	(void)locked;
 
-	unsigned char* st = (unsigned char*)xalloc(count + 1); /* 0 undecided, 1 skip, 2 reject, 3 commit */
+	size_t V = rs->vertex_count;
+	memset(rs->lock_rank, 0xff, V * sizeof(unsigned int));
+
+	unsigned char* st = (unsigned char*)xalloc(count + 1);
 	memset(st, 0, count + 1);
 
-	if (rs->perm_cap < count)
+	if (rs->list_cap < count)
 	{
-		free(rs->perm);
-		rs->perm = (unsigned int*)xalloc(count * sizeof(unsigned int));
-		rs->perm_cap = count;
+		free(rs->list[0]);
+		free(rs->list[1]);
+		rs->list[0] = (unsigned int*)xalloc(count * sizeof(unsigned int) + 4);
+		rs->list[1] = (unsigned int*)xalloc(count * sizeof(unsigned int) + 4);
+		rs->list_cap = count;
+	}
+	if (rs->src_cap < count)
+	{
+		free(rs->src_rank);
+		rs->src_rank = (unsigned int*)xalloc(count * sizeof(unsigned int) + 4);
+		rs->src_cap = count;
+	}
+
+	/* candidate ranks grouped by source class */
+	memset(rs->src_off, 0, (V + 2) * sizeof(unsigned int));
+	for (size_t k = 0; k < count; ++k)
+		rs->src_off[remap[cs[order[k]].v0] + 1]++;
+	for (size_t v = 0; v < V; ++v)
+		rs->src_off[v + 1] += rs->src_off[v];
+	{
+		unsigned int* cur = (unsigned int*)xalloc((V + 1) * sizeof(unsigned int));
+		memcpy(cur, rs->src_off, V * sizeof(unsigned int));
+		for (size_t k = 0; k < count; ++k)
+			rs->src_rank[cur[remap[cs[order[k]].v0]]++] = (unsigned int)k;
+		free(cur);
 	}
 
 	size_t window = window0 ? window0 : count;
@@ -1142,91 +1342,70 @@ static perform_result perform_rounds(round_state* rs, unsigned int* cremap, unsi
 	size_t lo = 0;
 	size_t stop = count;
 	int stop_reason = 0;
-
 	size_t tri = 0, rejects = 0;
 	float maxerr = *result_error;
 
+	unsigned int* list = rs->list[0];
+	unsigned int* next = rs->list[1];
+
 	for (;;)
 	{
-		/* shuffled visiting order for this window */
-		size_t span = window - lo;
-		for (size_t i = 0; i < span; ++i)
-			rs->perm[i] = (unsigned int)(lo + i);
-		for (size_t i = span; i > 1; --i)
-		{
-			size_t j = (size_t)(rs_next(rs) % i);
-			unsigned int t = rs->perm[i - 1];
-			rs->perm[i - 1] = rs->perm[j];
-			rs->perm[j] = t;
-		}
+		size_t n = 0;
+		for (size_t k = lo; k < window; ++k)
+			list[n++] = (unsigned int)k;
 
-		for (;;)
+		while (n)
 		{
 			unsigned int e = ++rs->epoch;
-			uint64_t* rd_any = rs->pend_any[(e - 1) & 1];
-			uint64_t* rd_src = rs->pend_src[(e - 1) & 1];
-			uint64_t* wr_any = rs->pend_any[e & 1];
-			uint64_t* wr_src = rs->pend_src[e & 1];
-			uint64_t prev_tag = (uint64_t)(~(e - 1));
-			size_t undecided = 0;
 
-			for (size_t p = 0; p < span; ++p)
+			/* phase X */
+			shuffle(rs, list, n);
+			size_t m = 0;
+			for (size_t p = 0; p < n; ++p)
 			{
-				size_t k = rs->perm[p];
-				if (st[k])
-					continue;
-
+				size_t k = list[p];
 				const cand* c = &cs[order[k]];
-				unsigned int r0 = remap[c->v0], r1 = remap[c->v1];
-
-				if (locked[r0] | locked[r1])
-				{
-					st[k] = 1;
-					continue;
-				}
-
-				uint64_t mine = pend_key(e - 1, k);
-				int ready = rd_any[r0] == mine && rd_any[r1] == mine;
-
-				for (unsigned int j = adj->offsets[r0]; ready && j < adj->offsets[r0 + 1]; ++j)
-				{
-					uint64_t pa = rd_src[adj->next[j]], pb = rd_src[adj->prev[j]];
-					if ((pa >> 32) == prev_tag && (pa & 0xffffffffu) < k)
-						ready = 0;
-					if ((pb >> 32) == prev_tag && (pb & 0xffffffffu) < k)
-						ready = 0;
-				}
-
-				if (ready)
-				{
-					if (collapse_flips(adj, vp, cremap, r0, r1))
-						st[k] = 2;
-					else
-					{
-						commit_collapse(cremap, c, remap, wedge, kind, loop, loopback);
-						locked[r0] = 1;
-						locked[r1] = 1;
-						st[k] = 3;
-					}
-					continue;
-				}
-
-				uint64_t key = pend_key(e, k);
-				if (wr_any[r0] > key)
-					wr_any[r0] = key;
-				if (wr_any[r1] > key)
-					wr_any[r1] = key;
-				if (wr_src[r0] > key)
-					wr_src[r0] = key;
-				undecided++;
+				if (round_x(rs, st, k, remap[c->v0], remap[c->v1], e, adj, vp, cremap, cs, order, remap, wedge, kind, loop, loopback))
+					next[m++] = (unsigned int)k;
 			}
 
+			/* phase Y */
+			shuffle(rs, next, m);
+			uint64_t* rd = rs->pend[e & 1];
+			for (size_t p = 0; p < m; ++p)
+			{
+				size_t k = next[p];
+				if (st[k] != ST_FLIP_PASS)
+					continue;
+				const cand* c = &cs[order[k]];
+				unsigned int r0 = remap[c->v0], r1 = remap[c->v1];
+				uint64_t mine = pend_key(e, k);
+				if (rd[r0] == mine && rd[r1] == mine)
+				{
+					commit_collapse(cremap, c, remap, wedge, kind, loop, loopback);
+					rs->lock_rank[r0] = (unsigned int)k;
+					rs->lock_rank[r1] = (unsigned int)k;
+					st[k] = ST_COMMITTED;
+				}
+			}
+
+			unsigned int* t = list;
+			list = next;
+			next = t;
+			n = m;
 			pr.rounds++;
-			if (!undecided)
-				break;
 		}
 
-		/* stop scan over [lo, window) */
+		/* a failed flip test is a rejection unless a lower commit locked an endpoint first */
+		for (size_t k = lo; k < window; ++k)
+			if (st[k] == ST_FLIP_FAIL)
+			{
+				const cand* c = &cs[order[k]];
+				unsigned int l0 = rs->lock_rank[remap[c->v0]], l1 = rs->lock_rank[remap[c->v1]];
+				st[k] = ((l0 != NONE && l0 < k) || (l1 != NONE && l1 < k)) ? ST_SKIPPED : ST_REJECTED;
+			}
+
+		/* stop scan over [lo, window) (prefix state carried from earlier windows) */
 		size_t k = lo;
 		for (; k < window; ++k)
 		{
@@ -1250,12 +1429,12 @@ static perform_result perform_rounds(round_state* rs, unsigned int* cremap, unsi
 				break;
 			}
 
-			if (st[k] == 3)
+			if (st[k] == ST_COMMITTED)
 			{
 				tri += (kind[c->v0] == K_BORDER) ? 1 : 2;
 				maxerr = maxerr < c->error ? c->error : maxerr;
 			}
-			else if (st[k] == 2)
+			else if (st[k] == ST_REJECTED)
 				rejects++;
 		}
 
@@ -1270,14 +1449,14 @@ static perform_result perform_rounds(round_state* rs, unsigned int* cremap, unsi
 	}
 
 	for (size_t k = stop; k < window; ++k)
-		if (st[k] == 3)
+		if (st[k] == ST_COMMITTED)
 			revert_collapse(cremap, &cs[order[k]], wedge);
 
 	for (size_t k = 0; k < stop; ++k)
 	{
-		pr.committed += st[k] == 3;
-		pr.skipped += st[k] == 1;
-		pr.rejected += st[k] == 2;
+		pr.committed += st[k] == ST_COMMITTED;
+		pr.skipped += st[k] == ST_SKIPPED;
+		pr.rejected += st[k] == ST_REJECTED;
 	}
 
 	pr.evaluated = stop_reason ? stop + 1 : stop;
