@@ -25,6 +25,11 @@ import numpy as np
 
 HBM_PEAK_GBS = 8000.0
 
+# compulsory bytes of one rank visit in phase X of the commit rounds (the dominant kernel):
+# work-list entry 16 + status 1 + two lock ranks 8 + two previous-round minima 16 +
+# three 64-bit minima published 24 + survivor entry written 16
+PHASE_X_BYTES_PER_RANK = 81
+
 
 def make_workload(name):
     from meshoptimizer_amd import workloads as W
@@ -79,6 +84,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="icosphere8")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run one device copy of 256 MiB (reference point for PMC byte counters)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -106,12 +113,21 @@ def main():
         step()
     torch.cuda.synchronize()
 
+    if args.calibrate_copy:
+        # a plain copy of known size: FETCH_SIZE / WRITE_SIZE of this dispatch should read 262144 KiB
+        src = torch.ones(64 << 20, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+
     # bit-exactness of the measured configuration against the CPU reference (untimed)
     n_gpu, e_gpu = step()
     gpu_out = dest[:n_gpu].cpu().numpy().astype(np.uint32)
     call, passes = M.last_call_stats()
 
     times = []
+    M.kernel_timing(True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -120,6 +136,7 @@ def main():
         times.append(time.perf_counter() - ts)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
+    x_ms, x_launches, x_ranks = M.kernel_timing(False)
 
     ms = elapsed / args.steps * 1e3
     value = m.triangle_count / (elapsed / args.steps) / 1e6
@@ -147,6 +164,17 @@ def main():
         "call_roofline": {"bound": "hbm", "alg_bytes": ab, "achieved_GBs": round(ab / (ms * 1e-3) / 1e9, 2),
                           "peak_GBs": HBM_PEAK_GBS, "frac": round(ab / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5)},
     }
+
+    if x_launches:
+        per_launch_bytes = PHASE_X_BYTES_PER_RANK * x_ranks / x_launches
+        per_launch_s = x_ms * 1e-3 / x_launches
+        achieved = per_launch_bytes / per_launch_s / 1e9
+        out["roofline"] = {"bound": "hbm", "kernel": "k_round_x (commit rounds, phase X)",
+                           "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                           "launches_per_step": x_launches / args.steps,
+                           "avg_launch_us": round(per_launch_s * 1e6, 2),
+                           "ranks_per_launch": round(x_ranks / x_launches, 1)}
 
     if not args.no_cpu:
         from oracle import oracle as O

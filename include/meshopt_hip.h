@@ -51,6 +51,11 @@ typedef struct meshopt_hip_call_stats
 /* Copies up to max_passes pass records of the calling thread's last call; returns the pass count. */
 int meshopt_hip_last_call_stats(meshopt_hip_call_stats* call, meshopt_hip_pass_stats* passes, int max_passes);
 
+/* HIP-event timing of the dominant kernel (phase X of the commit rounds) on the library's
+ * stream.  Each call returns the totals gathered since the previous call (summed kernel time in
+ * ms, launches, ranks listed over those launches), resets them, and turns timing on or off. */
+void meshopt_hip_kernel_timing(int enable, double* total_ms, unsigned long long* launches, unsigned long long* ranks);
+
 /* Text of the last HIP error seen by the calling thread (empty string if none). */
 const char* meshopt_hip_last_error(void);
 

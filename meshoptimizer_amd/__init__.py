@@ -50,6 +50,7 @@ EXPORTED_SYMBOLS = (
     "meshopt_hip_simplify_device",
     "meshopt_hip_last_call_stats",
     "meshopt_hip_last_error",
+    "meshopt_hip_kernel_timing",
     "meshopt_hip_release",
 )
 
@@ -98,6 +99,8 @@ def library() -> ctypes.CDLL:
     lib.meshopt_hip_last_call_stats.argtypes = [_P, _P, ctypes.c_int]
     lib.meshopt_hip_last_error.restype = ctypes.c_char_p
     lib.meshopt_hip_last_error.argtypes = []
+    lib.meshopt_hip_kernel_timing.restype = None
+    lib.meshopt_hip_kernel_timing.argtypes = [ctypes.c_int, _P, _P, _P]
     lib.meshopt_hip_release.restype = None
     lib.meshopt_hip_release.argtypes = []
     _lib = lib
@@ -175,6 +178,17 @@ def simplify_device(dest_ptr, indices_ptr, index_count, positions_ptr, vertex_co
     if n == ctypes.c_size_t(-1).value:
         raise RuntimeError(lib.meshopt_hip_last_error().decode())
     return n, err.value
+
+
+def kernel_timing(enable: bool):
+    """Turns HIP-event timing of the phase-X round kernel on/off; returns (ms, launches, ranks)
+    gathered since the previous call."""
+    lib = library()
+    ms = ctypes.c_double(0)
+    n = ctypes.c_ulonglong(0)
+    r = ctypes.c_ulonglong(0)
+    lib.meshopt_hip_kernel_timing(1 if enable else 0, ctypes.byref(ms), ctypes.byref(n), ctypes.byref(r))
+    return ms.value, n.value, r.value
 
 
 def last_call_stats():

@@ -98,6 +98,8 @@ struct CallScalars
 	uint32_t unique_count; // sparse remap: number of referenced vertices
 	uint32_t tail_rounds;   // rounds run by the single-workgroup tail of the commit
 	uint32_t head_sizes[16]; // work-list size at the start of each head round
+	uint32_t x_visits;      // ranks listed over every phase-X launch of the call (roofline accounting)
+	uint32_t x_launches;
 	uint32_t work[8];     // small counters of setup kernels
 };
 

@@ -71,6 +71,11 @@ struct Context
 	std::vector<meshopt_hip_pass_stats> passes;
 	meshopt_hip_call_stats call{};
 	int head_rounds = 8;
+	// phase-X kernel timing (meshopt_hip_kernel_timing)
+	bool timing = false;
+	std::vector<hipEvent_t> ev_begin, ev_end;
+	double timed_ms = 0;
+	unsigned long long timed_launches = 0, timed_visits = 0;
 	double window_fraction = 0; // evaluated / candidates of the previous pass (0: none yet)
 
 	void ensure_init()
@@ -358,6 +363,27 @@ static size_t simplify_on_device(Context& ctx, const CallArgs& c, float* result_
 
 	m.index_count = (uint32_t)I;
 	m.attr_count = (uint32_t)A;
+	KernelTimer timer{};
+	if (ctx.timing)
+	{
+		if (ctx.ev_begin.empty())
+		{
+			ctx.ev_begin.resize(4096);
+			ctx.ev_end.resize(4096);
+			for (size_t i = 0; i < ctx.ev_begin.size(); ++i)
+			{
+				HIP_CHECK(hipEventCreate(&ctx.ev_begin[i]));
+				HIP_CHECK(hipEventCreate(&ctx.ev_end[i]));
+			}
+		}
+		timer.begin = ctx.ev_begin.data();
+		timer.end = ctx.ev_end.data();
+		timer.capacity = (int)ctx.ev_begin.size();
+		timer.count = 0;
+		m.timer = &timer;
+	}
+	else
+		m.timer = nullptr;
 	m.ib_cur = 0;
 	m.loop_cur = 0;
 	m.epoch = 0;
@@ -463,8 +489,12 @@ static size_t simplify_on_device(Context& ctx, const CallArgs& c, float* result_
 		for (;;)
 		{
 			int heads = ctx.head_rounds;
+			double estimate = (double)(W - lo);
 			for (int hd = 0; hd < heads; ++hd, ++round)
-				launch_round(s, m, W - lo, round, hd);
+			{
+				launch_round(s, m, (size_t)estimate + 256, round, hd);
+				estimate *= hd == 0 ? 0.9 : 0.75; // undecided ranks shrink geometrically
+			}
 			launch_round_tail(s, m, round, (uint32_t)(W - lo + 2));
 			launch_resolve(s, m, lo, W);
 			launch_stop_scan(s, m, W, C, goal, error_limit);
@@ -474,10 +504,11 @@ static size_t simplify_on_device(Context& ctx, const CallArgs& c, float* result_
 			round += (int)h.tail_rounds;
 			m.epoch += h.tail_rounds;
 
-			// head rounds for the next window / pass: enough to bring the list under the tail size
-			int next = heads + 4;
+			// head rounds for the next window / pass: enough to bring the list under the size one
+			// tail workgroup takes in a single sweep
+			int next = heads + 3;
 			for (int hd = 0; hd < heads && hd < 16; ++hd)
-				if (h.head_sizes[hd] < 4096)
+				if (h.head_sizes[hd] < 1024)
 				{
 					next = hd;
 					break;
@@ -524,6 +555,18 @@ static size_t simplify_on_device(Context& ctx, const CallArgs& c, float* result_
 	if (result_count)
 		HIP_CHECK(hipMemcpyAsync(c.destination, m.ib[m.ib_cur], result_count * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
 	sync_scalars(ctx, m);
+
+	if (ctx.timing)
+	{
+		for (int i = 0; i < timer.count; ++i)
+		{
+			float ms = 0;
+			HIP_CHECK(hipEventElapsedTime(&ms, timer.begin[i], timer.end[i]));
+			ctx.timed_ms += ms;
+		}
+		ctx.timed_launches += (unsigned long long)timer.count;
+		ctx.timed_visits += h.x_visits;
+	}
 
 	t_call.passes = (unsigned int)t_passes.size();
 	t_call.result_error_sq = result_error_sq;
@@ -785,6 +828,22 @@ extern "C" int meshopt_hip_last_call_stats(meshopt_hip_call_stats* call, meshopt
 	for (int i = 0; i < n && i < max_passes && passes; ++i)
 		passes[i] = t_passes[i];
 	return n;
+}
+
+extern "C" void meshopt_hip_kernel_timing(int enable, double* total_ms, unsigned long long* launches, unsigned long long* ranks)
+{
+	Context& ctx = context();
+	std::lock_guard<std::mutex> guard(ctx.mutex);
+	if (total_ms)
+		*total_ms = ctx.timed_ms;
+	if (launches)
+		*launches = ctx.timed_launches;
+	if (ranks)
+		*ranks = ctx.timed_visits;
+	ctx.timing = enable != 0;
+	ctx.timed_ms = 0;
+	ctx.timed_launches = 0;
+	ctx.timed_visits = 0;
 }
 
 extern "C" const char* meshopt_hip_last_error(void)

@@ -6,9 +6,19 @@
 namespace mhip
 {
 
+// optional HIP-event timing of the phase-X round kernel (the dominant kernel)
+struct KernelTimer
+{
+	hipEvent_t* begin;   // event pairs recorded around each launch
+	hipEvent_t* end;
+	int capacity;
+	int count;
+};
+
 // device state of one simplify call; every pointer is carved from one arena
 struct Mesh
 {
+	KernelTimer* timer;
 	uint32_t vertex_count;   // V (after the optional sparse remap)
 	uint32_t index_count;    // I of the input
 	uint32_t attr_count;     // attributes with weight > 0

@@ -345,7 +345,8 @@ enum : uint8_t
 	kFlipFail = 5, // flip test fails in every scenario; resolved as skipped or rejected at the end
 };
 
-constexpr int kMaxAlternatives = 4;
+constexpr int kMaxAlternatives = 2; // per neighbour: its current value and one pending move
+constexpr int kMaxPendingSlots = 4; // adjacency slots whose neighbour has a pending lower source
 
 struct RoundState
 {
@@ -410,9 +411,23 @@ __device__ int flip_outcome(const RoundState& a, uint32_t i0, uint32_t i1, uint3
 {
 	const uint64_t* prev_src = a.pend_src[(epoch - 1) & 1];
 	const uint32_t prev_tag = ~(epoch - 1);
+	const uint32_t begin = a.adj_off[i0], end = a.adj_off[i0 + 1];
+
+	// too many moving neighbours: the outcome is rarely invariant, wait instead
+	int pending = 0;
+	for (uint32_t p = begin; p < end; ++p)
+	{
+		uint2 nb2 = a.adj[p];
+		uint64_t pa = prev_src[nb2.x], pb = prev_src[nb2.y];
+		pending += (uint32_t)(pa >> 32) == prev_tag && (uint32_t)pa < k;
+		pending += (uint32_t)(pb >> 32) == prev_tag && (uint32_t)pb < k;
+	}
+	if (pending > kMaxPendingSlots)
+		return 0;
+
 	V3 v0 = load_pos(a.pos, i0), v1 = load_pos(a.pos, i1);
 	bool all_pass = true;
-	for (uint32_t p = a.adj_off[i0], e = a.adj_off[i0 + 1]; p < e; ++p)
+	for (uint32_t p = begin; p < end; ++p)
 	{
 		uint2 nb2 = a.adj[p];
 		uint32_t nb[2] = {nb2.x, nb2.y};
@@ -533,11 +548,9 @@ __device__ __forceinline__ void append_wave(bool keep, uint4 w, uint4* out_list,
 	}
 }
 
-// below this many listed ranks a head kernel runs on one workgroup
-constexpr uint32_t kSoloThreshold = 2048;
-
 // head round, phase X.  Round r reads list (r & 1) / counter (r % 3), appends the survivors
-// to list ((r + 1) & 1) / counter ((r + 1) % 3) and clears counter ((r + 2) % 3).
+// to list ((r + 1) & 1) / counter ((r + 1) % 3) and clears counter ((r + 2) % 3).  One rank
+// per thread when the grid is large enough; the grid-stride loop covers any list length.
 __global__ void __launch_bounds__(kThreads) k_round_x(RoundState a, uint32_t round, uint32_t epoch, uint32_t head_index)
 {
 	CallScalars* sc = a.sc;
@@ -547,16 +560,13 @@ __global__ void __launch_bounds__(kThreads) k_round_x(RoundState a, uint32_t rou
 		sc->list_count[(round + 2) % 3] = 0;
 		if (head_index < 16)
 			sc->head_sizes[head_index] = n;
+		sc->x_visits += n;
+		sc->x_launches += 1;
 	}
-	const bool solo = n < kSoloThreshold;
-	if (solo && blockIdx.x != 0)
-		return;
 	const uint4* in_list = a.list[round & 1];
 	uint4* out_list = a.list[(round + 1) & 1];
 	uint32_t* out_count = &sc->list_count[(round + 1) % 3];
-	const uint32_t first = solo ? 0 : blockIdx.x * kThreads;
-	const uint32_t step = solo ? kThreads : gridDim.x * kThreads;
-	for (uint32_t base = first; base < n; base += step)
+	for (uint32_t base = blockIdx.x * kThreads; base < n; base += gridDim.x * kThreads)
 	{
 		uint32_t idx = base + threadIdx.x;
 		uint4 w = idx < n ? in_list[idx] : make_uint4(0, 0, 0, 0);
@@ -569,13 +579,8 @@ __global__ void __launch_bounds__(kThreads) k_round_x(RoundState a, uint32_t rou
 __global__ void __launch_bounds__(kThreads) k_round_y(RoundState a, uint32_t round, uint32_t epoch)
 {
 	const uint32_t n = a.sc->list_count[(round + 1) % 3];
-	const bool solo = n < kSoloThreshold;
-	if (solo && blockIdx.x != 0)
-		return;
 	const uint4* list = a.list[(round + 1) & 1];
-	const uint32_t first = solo ? threadIdx.x : blockIdx.x * kThreads + threadIdx.x;
-	const uint32_t step = solo ? kThreads : gridDim.x * kThreads;
-	for (uint32_t idx = first; idx < n; idx += step)
+	for (uint32_t idx = blockIdx.x * kThreads + threadIdx.x; idx < n; idx += gridDim.x * kThreads)
 		round_y(a, list[idx], epoch);
 }
 
@@ -740,13 +745,20 @@ void launch_window_extend(hipStream_t s, Mesh& m, size_t lo, size_t hi, int roun
 	                                                                    &m.sc->list_count[round % 3], &m.sc->list_count[(round + 1) % 3]);
 }
 
-void launch_round(hipStream_t s, Mesh& m, size_t list_bound, int round, int head_index)
+// list_estimate sizes the grid (one rank per thread); the kernels stay correct for any length
+void launch_round(hipStream_t s, Mesh& m, size_t list_estimate, int round, int head_index)
 {
 	uint32_t e = ++m.epoch;
-	unsigned int blocks = grid_for(list_bound);
-	blocks = blocks < 1 ? 1 : (blocks > 1024 ? 1024 : blocks);
+	unsigned int blocks = grid_for(list_estimate);
+	blocks = blocks < 1 ? 1 : (blocks > 8192 ? 8192 : blocks);
 	RoundState a = round_state(m);
+	KernelTimer* t = m.timer;
+	bool timed = t && t->count < t->capacity;
+	if (timed)
+		HIP_CHECK(hipEventRecord(t->begin[t->count], s));
 	k_round_x<<<blocks, kThreads, 0, s>>>(a, (uint32_t)round, e, (uint32_t)head_index);
+	if (timed)
+		HIP_CHECK(hipEventRecord(t->end[t->count++], s));
 	k_round_y<<<blocks, kThreads, 0, s>>>(a, (uint32_t)round, e);
 }
 

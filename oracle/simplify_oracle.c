@@ -1093,7 +1093,8 @@ enum
 	ST_FLIP_FAIL = 5,
 };
 
-#define MAX_ALTERNATIVES 4
+#define MAX_ALTERNATIVES 2 /* per neighbour: its current value and one pending move */
+#define MAX_PENDING_SLOTS 4 /* adjacency slots whose neighbour has a pending lower source */
 
 typedef struct
 {
@@ -1203,6 +1204,19 @@ static int flip_outcome(const round_state* rs, unsigned int e, const corner_csr*
 {
 	vec3 v0 = vp[i0], v1 = vp[i1];
 	int all_pass = 1;
+	const uint64_t* prev_src = rs->pend_src[(e - 1) & 1];
+	const uint64_t prev_tag = (uint64_t)(unsigned int)~(e - 1);
+
+	/* too many moving neighbours: the outcome is rarely invariant, wait instead */
+	int pending = 0;
+	for (unsigned int j = adj->offsets[i0]; j < adj->offsets[i0 + 1]; ++j)
+	{
+		uint64_t pa = prev_src[adj->next[j]], pb = prev_src[adj->prev[j]];
+		pending += (pa >> 32) == prev_tag && (pa & 0xffffffffu) < k;
+		pending += (pb >> 32) == prev_tag && (pb & 0xffffffffu) < k;
+	}
+	if (pending > MAX_PENDING_SLOTS)
+		return 0;
 
 	for (unsigned int j = adj->offsets[i0]; j < adj->offsets[i0 + 1]; ++j)
 	{
@@ -1217,8 +1231,8 @@ static int flip_outcome(const round_state* rs, unsigned int e, const corner_csr*
 			alt[q][na[q]++] = remap_seen(cremap, rs->lock_rank, x, k);
 			/* every pending rank publishes its source role each round: a class whose minimum of
 			 * the previous round is not below k has no pending lower candidate */
-			uint64_t ps = rs->pend_src[(e - 1) & 1][x];
-			if ((ps >> 32) != (uint64_t)(unsigned int)~(e - 1) || (ps & 0xffffffffu) >= k)
+			uint64_t ps = prev_src[x];
+			if ((ps >> 32) != prev_tag || (ps & 0xffffffffu) >= k)
 				continue;
 			for (unsigned int t = rs->src_off[x]; t < rs->src_off[x + 1]; ++t)
 			{
