@@ -58,6 +58,17 @@ def alg_bytes(T0, V, A, passes):
     return total
 
 
+def measured_traffic(workload):
+    """HBM bytes per k_round_x launch from the committed rocprofv3 PMC summary of this workload
+    (FETCH_SIZE and WRITE_SIZE runs, calibrated against a copy of known size), or None."""
+    import re
+    path = os.path.join(ROOT, "profiles", f"k_round_x_{workload}_traffic.md")
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"Traffic per launch = ([0-9.]+) MB", open(path).read())
+    return float(m.group(1)) * 1e6 if m else None
+
+
 def cpu_reference(m, target, weights, budget_s=12.0):
     """The reference library (compiled from its own sources into oracle/_ref) on one host core."""
     from oracle import oracle as O
@@ -171,7 +182,7 @@ def main():
         achieved = per_launch_bytes / per_launch_s / 1e9
         out["roofline"] = {"bound": "hbm", "kernel": "k_round_x (commit rounds, phase X)",
                            "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                           "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                           "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(args.workload),
                            "launches_per_step": x_launches / args.steps,
                            "avg_launch_us": round(per_launch_s * 1e6, 2),
                            "ranks_per_launch": round(x_ranks / x_launches, 1)}
