@@ -1,7 +1,7 @@
 """Benchmark of the MI355X simplifier: input triangles per second for one plain
 meshopt_simplify call on a device-resident mesh, next to the reference CPU library.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload icosphere8|grid256|seamed1024]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload icosphere8|grid256|seamed1024|patches4096]
 
 A step is one whole simplify call (setup, every pass, compaction of the result) on inputs
 already resident in HBM; the timed region is bracketed by device synchronisation.  Prints one
@@ -58,11 +58,11 @@ def alg_bytes(T0, V, A, passes):
     return total
 
 
-def measured_traffic(workload):
-    """HBM bytes per k_round_x launch from the committed rocprofv3 PMC summary of this workload
+def measured_traffic(workload, kernel="k_round_x"):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary of this workload
     (FETCH_SIZE and WRITE_SIZE runs, calibrated against a copy of known size), or None."""
     import re
-    path = os.path.join(ROOT, "profiles", f"k_round_x_{workload}_traffic.md")
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_traffic.md")
     if not os.path.exists(path):
         return None
     m = re.search(r"Traffic per launch = ([0-9.]+) MB", open(path).read())
@@ -88,6 +88,138 @@ def cpu_reference(m, target, weights, budget_s=12.0):
             "sample": f"{len(times)} single-threaded calls on the same mesh, median {med * 1e3:.1f} ms"}
 
 
+# compulsory HBM bytes of one mesh in the batched kernel: its indices read (4 I) and positions
+# read (12 V, xyz), the result written (4 I_out), offsets/target read and count/error/status
+# written (6 x 4)
+def batch_alg_bytes(index_counts, vertex_counts, out_counts):
+    return int(4 * np.sum(index_counts) + 12 * np.sum(vertex_counts) + 4 * np.sum(out_counts) + 24 * len(index_counts))
+
+
+def bench_patches(args):
+    """The batched workload: 4096 clusterlod-style patches in one meshopt_hip_simplify_batch_device
+    call per step."""
+    import torch
+    import meshoptimizer_amd as M
+    from meshoptimizer_amd import workloads as W
+
+    count, opts, err = 4096, M.SimplifyLockBorder, 1e-2
+    meshes = W.patches(count=count)
+    idx, io, verts, vo, stride = M.pack_meshes([(m.indices, m.vertices) for m in meshes])
+    targets = np.array([W.target_count(m.index_count, 1, 2) for m in meshes], np.uint32)
+    tris = int(sum(m.triangle_count for m in meshes))
+    desc = (f"{count} clusterlod-style patches of 64-512 triangles ({tris:,} T) cut from a displaced 2048x2048 grid, "
+            "LockBorder, target 50% per patch, error 1e-2, one batched call")
+
+    dev = torch.device("cuda:0")
+    as_dev = lambda a, dt: torch.from_numpy(a.view(dt)).to(dev)
+    d_idx = as_dev(idx, np.int32)
+    d_io = as_dev(io, np.int32)
+    d_vo = as_dev(vo, np.int32)
+    d_tg = as_dev(targets, np.int32)
+    d_v = torch.from_numpy(verts).to(dev)
+    d_dst = torch.empty(max(idx.size, 1), dtype=torch.int32, device=dev)
+    d_cnt = torch.empty(count, dtype=torch.int32, device=dev)
+    d_err = torch.empty(count, dtype=torch.float32, device=dev)
+
+    def step():
+        M.simplify_batch_device(d_dst.data_ptr(), d_cnt.data_ptr(), d_err.data_ptr(), d_idx.data_ptr(), d_io.data_ptr(),
+                                d_v.data_ptr(), d_vo.data_ptr(), stride, d_tg.data_ptr(), count, err, opts)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    step()
+    torch.cuda.synchronize()
+    g_dst = d_dst.cpu().numpy().view(np.uint32)
+    g_cnt = d_cnt.cpu().numpy().view(np.uint32)
+    g_err = d_err.cpu().numpy()
+
+    times = []
+    M.kernel_timing(True)
+    M.batch_kernel_timing()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ts = time.perf_counter()
+        step()
+        times.append(time.perf_counter() - ts)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    k_ms, k_launches, k_meshes = M.batch_kernel_timing()
+    M.kernel_timing(False)
+    ms = elapsed / args.steps * 1e3
+    value = tris / (elapsed / args.steps) / 1e6
+
+    # the plain per-call device path on the same patches (a bounded sample, per-patch mean scaled)
+    sample = min(count, args.plain_sample)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for p in range(sample):
+        M.simplify_device(d_dst.data_ptr() + 4 * int(io[p]), d_idx.data_ptr() + 4 * int(io[p]), int(io[p + 1] - io[p]),
+                          d_v.data_ptr() + stride * int(vo[p]), int(vo[p + 1] - vo[p]), stride, int(targets[p]), err, opts)
+    torch.cuda.synchronize()
+    plain_s = (time.perf_counter() - t0) / sample * count
+    sample_tris = int(sum(meshes[p].triangle_count for p in range(sample)))
+    plain_value = sample_tris / (plain_s * sample / count) / 1e6
+
+    out = {
+        "metric": "input Mtri/s simplified (bit-exact vs CPU), 4096-patch batch",
+        "value": round(value, 3),
+        "unit": "Mtri/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded generator)",
+        "config": {"workload": desc, "input_triangles": tris, "output_triangles": int(g_cnt.sum()) // 3, "patches": count},
+        "p50_ms": round(statistics.median(times) * 1e3, 4),
+        "p99_ms": round(float(np.percentile(np.array(times) * 1e3, 99)), 4),
+        "plain_per_call_gpu": {"value": round(plain_value, 3), "unit": "Mtri/s",
+                               "sample": f"meshopt_hip_simplify_device on the first {sample} patches, one call each",
+                               "batch_speedup": round(value / plain_value, 2)},
+    }
+    if k_launches:
+        ab = batch_alg_bytes(np.diff(io), np.diff(vo), g_cnt)
+        per_launch_s = k_ms * 1e-3 / k_launches
+        achieved = ab / per_launch_s / 1e9
+        out["roofline"] = {"bound": "hbm", "kernel": "k_simplify_patches (one workgroup per patch, LDS-resident)",
+                           "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": measured_traffic("patches4096", "k_simplify_patches"),
+                           "alg_bytes_per_launch": ab, "avg_launch_us": round(per_launch_s * 1e6, 2)}
+
+    if not args.no_cpu:
+        from oracle import oracle as O
+        impl = "ref" if O.have_ref() else "oracle"
+        kind = "reference" if impl == "ref" else "port"
+        r_dst, r_cnt, r_err = O.simplify_batch(idx, io, verts, vo, targets, err, opts, threads=1, impl=impl)
+        exact = bool(np.array_equal(r_cnt, g_cnt) and np.array_equal(r_err.view(np.uint32), g_err.view(np.uint32)))
+        for p in range(count):
+            if not exact:
+                break
+            exact = np.array_equal(r_dst[io[p]:io[p] + r_cnt[p]], g_dst[io[p]:io[p] + g_cnt[p]])
+        out["bit_exact"] = bool(exact)
+        cores = max(1, min(args.cpu_threads, os.cpu_count() or 1))
+        base = {}
+        for th in (1, cores):
+            reps, t_all = 0, 0.0
+            while t_all < args.cpu_budget / 2 or reps < 1:
+                t0 = time.perf_counter()
+                O.simplify_batch(idx, io, verts, vo, targets, err, opts, threads=th, impl=impl)
+                t_all += time.perf_counter() - t0
+                reps += 1
+            base[th] = (tris * reps / t_all / 1e6, reps, t_all)
+        v1, vN = base[1], base[cores]
+        out["cpu_baseline"] = {"value": round(vN[0], 3), "unit": "Mtri/s", "cores": cores, "kind": kind,
+                               "sample": f"the full 4096-patch batch, {vN[1]} runs on {cores} threads (one patch per work item)",
+                               "single_thread": round(v1[0], 3), "single_thread_sample": f"{v1[1]} runs on 1 thread"}
+        out["speedup_vs_cpu_all_cores"] = round(value / vN[0], 2)
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -95,6 +227,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="icosphere8")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--plain-sample", type=int, default=256, help="patches run through the plain per-call path")
+    ap.add_argument("--cpu-threads", type=int, default=16, help="threads of the all-core CPU baseline (batches)")
+    ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU baseline timing")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one device copy of 256 MiB (reference point for PMC byte counters)")
     args = ap.parse_args()
@@ -103,6 +238,9 @@ def main():
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               "(meshes and patch batches shard across GPUs with no exchange, see the design notes)", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "patches4096":
+        return bench_patches(args)
 
     import torch
     import meshoptimizer_amd as M

@@ -24,6 +24,20 @@ extern "C"
  */
 size_t meshopt_hip_simplify_device(unsigned int* destination, const unsigned int* indices, size_t index_count, const float* vertex_positions, size_t vertex_count, size_t vertex_positions_stride, const float* vertex_attributes, size_t vertex_attributes_stride, const float* attribute_weights, size_t attribute_count, const unsigned char* vertex_lock, size_t target_index_count, float target_error, unsigned int options, float* result_error);
 
+/**
+ * Simplifies many independent meshes (for example the cluster groups of a clusterlod build) in
+ * one launch; every mesh gets exactly what meshopt_simplify (src/meshoptimizer.h:513) returns for
+ * it.  Mesh p owns indices [index_offsets[p], index_offsets[p+1]) (local to its vertices) and
+ * vertices [vertex_offsets[p], vertex_offsets[p+1]) of the packed arrays, and writes its result
+ * at destination + index_offsets[p]; result_counts[p] and result_errors[p] (may be NULL) receive
+ * the index count and error.  target_error and options are shared by all meshes; Prune is not
+ * accepted.  Returns 0, or -1 with meshopt_hip_last_error set.
+ * The _device variant takes device pointers for every array; the other copies host arrays in
+ * and out around the same launch.
+ */
+int meshopt_hip_simplify_batch_device(unsigned int* destination, unsigned int* result_counts, float* result_errors, const unsigned int* indices, const unsigned int* index_offsets, const float* vertex_positions, const unsigned int* vertex_offsets, size_t vertex_positions_stride, const unsigned int* target_index_counts, size_t mesh_count, float target_error, unsigned int options);
+int meshopt_hip_simplify_batch(unsigned int* destination, unsigned int* result_counts, float* result_errors, const unsigned int* indices, const unsigned int* index_offsets, const float* vertex_positions, const unsigned int* vertex_offsets, size_t vertex_positions_stride, const unsigned int* target_index_counts, size_t mesh_count, float target_error, unsigned int options);
+
 /* Per-pass counters of the last call made on the calling thread; the same quantities the
  * reference prints under TRACE (src/simplifier.cpp:1625-1633). */
 typedef struct meshopt_hip_pass_stats
@@ -54,7 +68,12 @@ int meshopt_hip_last_call_stats(meshopt_hip_call_stats* call, meshopt_hip_pass_s
 /* HIP-event timing of the dominant kernel (phase X of the commit rounds) on the library's
  * stream.  Each call returns the totals gathered since the previous call (summed kernel time in
  * ms, launches, ranks listed over those launches), resets them, and turns timing on or off. */
-void meshopt_hip_kernel_timing(int enable, double* total_ms, unsigned long long* launches, unsigned long long* ranks);
+void meshopt_hip_kernel_timing(int enable, double* total_ms, size_t* launches, size_t* ranks);
+
+/* HIP-event timing of the batched kernel (meshopt_hip_simplify_batch*), gathered while
+ * meshopt_hip_kernel_timing has timing on: summed kernel time in ms, launches and meshes since
+ * the previous call; resets the totals. */
+void meshopt_hip_batch_kernel_timing(double* total_ms, size_t* launches, size_t* meshes);
 
 /* Text of the last HIP error seen by the calling thread (empty string if none). */
 const char* meshopt_hip_last_error(void);

@@ -48,6 +48,9 @@ EXPORTED_SYMBOLS = (
     "meshopt_simplifyScale",
     "meshopt_setAllocator",
     "meshopt_hip_simplify_device",
+    "meshopt_hip_simplify_batch",
+    "meshopt_hip_simplify_batch_device",
+    "meshopt_hip_batch_kernel_timing",
     "meshopt_hip_last_call_stats",
     "meshopt_hip_last_error",
     "meshopt_hip_kernel_timing",
@@ -99,6 +102,12 @@ def library() -> ctypes.CDLL:
     lib.meshopt_hip_last_call_stats.argtypes = [_P, _P, ctypes.c_int]
     lib.meshopt_hip_last_error.restype = ctypes.c_char_p
     lib.meshopt_hip_last_error.argtypes = []
+    for name in ("meshopt_hip_simplify_batch", "meshopt_hip_simplify_batch_device"):
+        fn = getattr(lib, name)
+        fn.restype = ctypes.c_int
+        fn.argtypes = [_P, _P, _P, _P, _P, _P, _P, _SZ, _P, _SZ, _F, _U]
+    lib.meshopt_hip_batch_kernel_timing.restype = None
+    lib.meshopt_hip_batch_kernel_timing.argtypes = [_P, _P, _P]
     lib.meshopt_hip_kernel_timing.restype = None
     lib.meshopt_hip_kernel_timing.argtypes = [ctypes.c_int, _P, _P, _P]
     lib.meshopt_hip_release.restype = None
@@ -180,15 +189,72 @@ def simplify_device(dest_ptr, indices_ptr, index_count, positions_ptr, vertex_co
     return n, err.value
 
 
+def pack_meshes(meshes):
+    """Packs a list of (indices, positions[V, >=3] float32) into batch arrays:
+    (indices, index_offsets, vertices, vertex_offsets, stride_bytes)."""
+    io = np.zeros(len(meshes) + 1, dtype=np.uint32)
+    vo = np.zeros(len(meshes) + 1, dtype=np.uint32)
+    for p, (ib, v) in enumerate(meshes):
+        io[p + 1] = io[p] + len(ib)
+        vo[p + 1] = vo[p] + len(v)
+    width = max((np.asarray(v).shape[1] for _, v in meshes), default=3)
+    verts = np.zeros((int(vo[-1]), width), dtype=np.float32)
+    idx = np.zeros(int(io[-1]), dtype=np.uint32)
+    for p, (ib, v) in enumerate(meshes):
+        idx[io[p]:io[p + 1]] = ib
+        verts[vo[p]:vo[p + 1], :np.asarray(v).shape[1]] = v
+    return idx, io, verts, vo, width * 4
+
+
+def simplify_batch(meshes, target_index_counts, target_error=1e-2, options=0):
+    """Simplifies every (indices, positions) mesh in one batched launch (meshopt_simplify per
+    mesh).  Returns [(indices, result_error)]."""
+    lib = library()
+    idx, io, verts, vo, stride = pack_meshes(meshes)
+    tg = np.ascontiguousarray(target_index_counts, dtype=np.uint32)
+    dest = np.zeros(max(idx.size, 1), dtype=np.uint32)
+    counts = np.zeros(max(len(meshes), 1), dtype=np.uint32)
+    errors = np.zeros(max(len(meshes), 1), dtype=np.float32)
+    rc = lib.meshopt_hip_simplify_batch(dest.ctypes.data, counts.ctypes.data, errors.ctypes.data, idx.ctypes.data, io.ctypes.data,
+                                        verts.ctypes.data, vo.ctypes.data, stride, tg.ctypes.data, len(meshes),
+                                        float(target_error), int(options))
+    if rc != 0:
+        raise RuntimeError(lib.meshopt_hip_last_error().decode())
+    return [(dest[io[p]:io[p] + counts[p]].copy(), float(errors[p])) for p in range(len(meshes))]
+
+
 def kernel_timing(enable: bool):
     """Turns HIP-event timing of the phase-X round kernel on/off; returns (ms, launches, ranks)
     gathered since the previous call."""
     lib = library()
     ms = ctypes.c_double(0)
-    n = ctypes.c_ulonglong(0)
-    r = ctypes.c_ulonglong(0)
+    n = ctypes.c_size_t(0)
+    r = ctypes.c_size_t(0)
     lib.meshopt_hip_kernel_timing(1 if enable else 0, ctypes.byref(ms), ctypes.byref(n), ctypes.byref(r))
     return ms.value, n.value, r.value
+
+
+def batch_kernel_timing():
+    """(ms, launches, meshes) of the batched kernel gathered since the previous call, while
+    kernel_timing(True) is on."""
+    lib = library()
+    ms = ctypes.c_double(0)
+    n = ctypes.c_size_t(0)
+    k = ctypes.c_size_t(0)
+    lib.meshopt_hip_batch_kernel_timing(ctypes.byref(ms), ctypes.byref(n), ctypes.byref(k))
+    return ms.value, n.value, k.value
+
+
+def simplify_batch_device(destination_ptr, result_counts_ptr, result_errors_ptr, indices_ptr, index_offsets_ptr,
+                          positions_ptr, vertex_offsets_ptr, stride, target_counts_ptr, mesh_count,
+                          target_error, options=0):
+    """meshopt_hip_simplify_batch_device on device pointers (integers); raises on error."""
+    lib = library()
+    rc = lib.meshopt_hip_simplify_batch_device(destination_ptr, result_counts_ptr, result_errors_ptr, indices_ptr,
+                                               index_offsets_ptr, positions_ptr, vertex_offsets_ptr, stride,
+                                               target_counts_ptr, mesh_count, float(target_error), int(options))
+    if rc != 0:
+        raise RuntimeError(lib.meshopt_hip_last_error().decode())
 
 
 def last_call_stats():

@@ -18,6 +18,7 @@
 
 #include "../../include/meshopt_hip.h"
 #include "../../include/meshoptimizer.h"
+#include "batch.h"
 #include "common.h"
 #include "kernels.h"
 #include "prims.h"
@@ -75,7 +76,10 @@ struct Context
 	bool timing = false;
 	std::vector<hipEvent_t> ev_begin, ev_end;
 	double timed_ms = 0;
-	unsigned long long timed_launches = 0, timed_visits = 0;
+	size_t timed_launches = 0, timed_visits = 0;
+	double batch_ms = 0; // the batched kernel, timed when timing is on
+	size_t batch_launches = 0, batch_meshes = 0;
+	hipEvent_t batch_ev[2] = {nullptr, nullptr};
 	double window_fraction = 0; // evaluated / candidates of the previous pass (0: none yet)
 
 	void ensure_init()
@@ -564,7 +568,7 @@ static size_t simplify_on_device(Context& ctx, const CallArgs& c, float* result_
 			HIP_CHECK(hipEventElapsedTime(&ms, timer.begin[i], timer.end[i]));
 			ctx.timed_ms += ms;
 		}
-		ctx.timed_launches += (unsigned long long)timer.count;
+		ctx.timed_launches += (size_t)timer.count;
 		ctx.timed_visits += h.x_visits;
 	}
 
@@ -784,6 +788,223 @@ extern "C" size_t meshopt_hip_simplify_device(unsigned int* destination, const u
 	}
 }
 
+// ------------------------------------------------------------------------------------------
+// batch of independent meshes (clusterlod patches)
+
+namespace mhip
+{
+
+// dynamic LDS per workgroup; the kernel's static LDS (scan partials, bounds) stays under 512 bytes
+static const size_t kBatchLdsLimit = 64 * 1024 - 512;
+
+static int simplify_batch_on_device(Context& ctx, unsigned int* destination, unsigned int* result_counts, float* result_errors,
+                                    const unsigned int* indices, const unsigned int* index_offsets, const float* positions,
+                                    const unsigned int* vertex_offsets, size_t stride, const unsigned int* target_index_counts,
+                                    size_t patch_count, float target_error, unsigned int options, const unsigned int* h_index_offsets,
+                                    const unsigned int* h_vertex_offsets, const unsigned int* h_targets)
+{
+	hipStream_t s = ctx.stream;
+	if (options & kOptPrune)
+		throw std::invalid_argument("meshopt_hip_simplify_batch: the Prune option is not supported");
+	if (stride < 12 || stride > 256 || stride % 4)
+		throw std::invalid_argument("meshopt_hip_simplify_batch: invalid vertex stride");
+
+	// per-mesh LDS need; the launch's layout is the largest prefix (by need) whose combined
+	// maxima fit one workgroup's LDS; meshes outside it take the plain path
+	auto round_v = [](uint32_t v) { return (v + 31) & ~31u; };
+	auto round_i = [](uint32_t i) { return (i + 95) / 96 * 96; };
+	std::vector<std::pair<size_t, uint32_t>> need; // (bytes, mesh)
+	need.reserve(patch_count);
+	for (size_t p = 0; p < patch_count; ++p)
+	{
+		uint32_t ni = h_index_offsets[p + 1] - h_index_offsets[p];
+		uint32_t nv = h_vertex_offsets[p + 1] - h_vertex_offsets[p];
+		if (h_index_offsets[p + 1] < h_index_offsets[p] || h_vertex_offsets[p + 1] < h_vertex_offsets[p])
+			throw std::invalid_argument("meshopt_hip_simplify_batch: offsets must not decrease");
+		if (ni % 3 || h_targets[p] > ni)
+			throw std::invalid_argument("meshopt_hip_simplify_batch: invalid index count or target");
+		if (nv < 0xffffu && ni < 0xffffu)
+			need.push_back({batch_lds_bytes(round_v(nv), round_i(ni)), (uint32_t)p});
+	}
+	std::sort(need.begin(), need.end());
+	uint32_t vmax = 32, imax = 96;
+	for (auto& e : need)
+	{
+		uint32_t nv = round_v(h_vertex_offsets[e.second + 1] - h_vertex_offsets[e.second]);
+		uint32_t ni = round_i(h_index_offsets[e.second + 1] - h_index_offsets[e.second]);
+		uint32_t v2 = std::max(vmax, nv), i2 = std::max(imax, ni);
+		if (batch_lds_bytes(v2, i2) > kBatchLdsLimit)
+			break;
+		vmax = v2;
+		imax = i2;
+	}
+
+	uint32_t* status = nullptr;
+	HIP_CHECK(hipMallocAsync((void**)&status, std::max<size_t>(patch_count, 1) * sizeof(uint32_t), s));
+	BatchArgs a;
+	a.destination = destination;
+	a.result_counts = result_counts;
+	a.result_errors = result_errors;
+	a.status = status;
+	a.passes = nullptr;
+	a.indices = indices;
+	a.index_offsets = index_offsets;
+	a.positions = positions;
+	a.vertex_offsets = vertex_offsets;
+	a.position_stride_floats = stride / sizeof(float);
+	a.target_index_counts = target_index_counts;
+	a.patch_count = patch_count;
+	a.target_error = target_error;
+	a.options = options;
+
+	std::vector<uint32_t> h_status(patch_count, 1);
+	if (!(options & kOptSparse))
+	{
+		const bool timed = ctx.timing && patch_count;
+		if (timed && !ctx.batch_ev[0])
+		{
+			HIP_CHECK(hipEventCreate(&ctx.batch_ev[0]));
+			HIP_CHECK(hipEventCreate(&ctx.batch_ev[1]));
+		}
+		if (timed)
+			HIP_CHECK(hipEventRecord(ctx.batch_ev[0], s));
+		launch_batch(s, a, vmax, imax);
+		if (timed)
+			HIP_CHECK(hipEventRecord(ctx.batch_ev[1], s));
+		HIP_CHECK(hipGetLastError());
+		if (patch_count)
+			HIP_CHECK(hipMemcpyAsync(h_status.data(), status, patch_count * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+	}
+	HIP_CHECK(hipFreeAsync(status, s));
+	HIP_CHECK(hipStreamSynchronize(s));
+	if (ctx.timing && patch_count && !(options & kOptSparse))
+	{
+		float ms = 0;
+		HIP_CHECK(hipEventElapsedTime(&ms, ctx.batch_ev[0], ctx.batch_ev[1]));
+		ctx.batch_ms += ms;
+		ctx.batch_launches += 1;
+		ctx.batch_meshes += patch_count;
+	}
+
+	for (size_t p = 0; p < patch_count; ++p)
+	{
+		if (h_status[p] == 2)
+			throw std::invalid_argument("meshopt_hip_simplify_batch: index buffer references a vertex >= vertex_count");
+		if (h_status[p] == 3)
+			throw std::runtime_error("meshopt_hip_simplify_batch: internal error (pass bound exceeded)");
+		if (h_status[p] == 0)
+			continue;
+		// too large for the LDS kernel (or a Sparse batch): the plain device path on this mesh
+		CallArgs c;
+		c.destination = destination + h_index_offsets[p];
+		c.indices = indices + h_index_offsets[p];
+		c.index_count = h_index_offsets[p + 1] - h_index_offsets[p];
+		c.positions = positions + (size_t)h_vertex_offsets[p] * (stride / sizeof(float));
+		c.vertex_count = h_vertex_offsets[p + 1] - h_vertex_offsets[p];
+		c.position_stride = stride;
+		c.attributes = nullptr;
+		c.attribute_stride = 0;
+		c.weights = nullptr;
+		c.attribute_count = 0;
+		c.lock = nullptr;
+		c.target_index_count = h_targets[p];
+		c.target_error = target_error;
+		c.options = options;
+		float err = 0;
+		uint32_t n = (uint32_t)simplify_on_device(ctx, c, &err);
+		HIP_CHECK(hipMemcpyAsync(result_counts + p, &n, sizeof(uint32_t), hipMemcpyHostToDevice, s));
+		if (result_errors)
+			HIP_CHECK(hipMemcpyAsync(result_errors + p, &err, sizeof(float), hipMemcpyHostToDevice, s));
+		HIP_CHECK(hipStreamSynchronize(s));
+	}
+	return 0;
+}
+
+} // namespace mhip
+
+extern "C" int meshopt_hip_simplify_batch_device(unsigned int* destination, unsigned int* result_counts, float* result_errors, const unsigned int* indices,
+                                                 const unsigned int* index_offsets, const float* vertex_positions, const unsigned int* vertex_offsets,
+                                                 size_t vertex_positions_stride, const unsigned int* target_index_counts, size_t mesh_count,
+                                                 float target_error, unsigned int options)
+{
+	try
+	{
+		Context& ctx = context();
+		std::lock_guard<std::mutex> guard(ctx.mutex);
+		ctx.ensure_init();
+		std::vector<unsigned int> io(mesh_count + 1), vo(mesh_count + 1), tg(mesh_count + 1);
+		HIP_CHECK(hipMemcpyAsync(io.data(), index_offsets, (mesh_count + 1) * sizeof(unsigned int), hipMemcpyDeviceToHost, ctx.stream));
+		HIP_CHECK(hipMemcpyAsync(vo.data(), vertex_offsets, (mesh_count + 1) * sizeof(unsigned int), hipMemcpyDeviceToHost, ctx.stream));
+		if (mesh_count)
+			HIP_CHECK(hipMemcpyAsync(tg.data(), target_index_counts, mesh_count * sizeof(unsigned int), hipMemcpyDeviceToHost, ctx.stream));
+		HIP_CHECK(hipStreamSynchronize(ctx.stream));
+		simplify_batch_on_device(ctx, destination, result_counts, result_errors, indices, index_offsets, vertex_positions, vertex_offsets,
+		                         vertex_positions_stride, target_index_counts, mesh_count, target_error, options, io.data(), vo.data(), tg.data());
+		t_last_error.clear();
+		return 0;
+	}
+	catch (const std::exception& e)
+	{
+		t_last_error = e.what();
+		return -1;
+	}
+}
+
+extern "C" int meshopt_hip_simplify_batch(unsigned int* destination, unsigned int* result_counts, float* result_errors, const unsigned int* indices,
+                                          const unsigned int* index_offsets, const float* vertex_positions, const unsigned int* vertex_offsets,
+                                          size_t vertex_positions_stride, const unsigned int* target_index_counts, size_t mesh_count, float target_error,
+                                          unsigned int options)
+{
+	try
+	{
+		Context& ctx = context();
+		std::lock_guard<std::mutex> guard(ctx.mutex);
+		ctx.ensure_init();
+		hipStream_t s = ctx.stream;
+		size_t I = index_offsets[mesh_count], V = vertex_offsets[mesh_count];
+		size_t pos_bytes = V ? (V - 1) * vertex_positions_stride + 12 : 0;
+		size_t n1 = mesh_count + 1;
+		size_t off_idx = 0, off_dst = (I * 4 + 255) & ~size_t(255);
+		size_t off_io = off_dst + ((I * 4 + 255) & ~size_t(255));
+		size_t off_vo = off_io + ((n1 * 4 + 255) & ~size_t(255));
+		size_t off_tg = off_vo + ((n1 * 4 + 255) & ~size_t(255));
+		size_t off_rc = off_tg + ((n1 * 4 + 255) & ~size_t(255));
+		size_t off_re = off_rc + ((n1 * 4 + 255) & ~size_t(255));
+		size_t off_pos = off_re + ((n1 * 4 + 255) & ~size_t(255));
+		char* d = static_cast<char*>(staging().get(off_pos + pos_bytes + 256));
+		if (I)
+			HIP_CHECK(hipMemcpyAsync(d + off_idx, indices, I * 4, hipMemcpyHostToDevice, s));
+		HIP_CHECK(hipMemcpyAsync(d + off_io, index_offsets, n1 * 4, hipMemcpyHostToDevice, s));
+		HIP_CHECK(hipMemcpyAsync(d + off_vo, vertex_offsets, n1 * 4, hipMemcpyHostToDevice, s));
+		if (mesh_count)
+			HIP_CHECK(hipMemcpyAsync(d + off_tg, target_index_counts, mesh_count * 4, hipMemcpyHostToDevice, s));
+		if (pos_bytes)
+			HIP_CHECK(hipMemcpyAsync(d + off_pos, vertex_positions, pos_bytes, hipMemcpyHostToDevice, s));
+		simplify_batch_on_device(ctx, reinterpret_cast<unsigned int*>(d + off_dst), reinterpret_cast<unsigned int*>(d + off_rc),
+		                         reinterpret_cast<float*>(d + off_re), reinterpret_cast<const unsigned int*>(d + off_idx),
+		                         reinterpret_cast<const unsigned int*>(d + off_io), reinterpret_cast<const float*>(d + off_pos),
+		                         reinterpret_cast<const unsigned int*>(d + off_vo), vertex_positions_stride,
+		                         reinterpret_cast<const unsigned int*>(d + off_tg), mesh_count, target_error, options, index_offsets, vertex_offsets,
+		                         target_index_counts);
+		if (mesh_count)
+		{
+			HIP_CHECK(hipMemcpyAsync(result_counts, d + off_rc, mesh_count * 4, hipMemcpyDeviceToHost, s));
+			if (result_errors)
+				HIP_CHECK(hipMemcpyAsync(result_errors, d + off_re, mesh_count * 4, hipMemcpyDeviceToHost, s));
+		}
+		if (I)
+			HIP_CHECK(hipMemcpyAsync(destination, d + off_dst, I * 4, hipMemcpyDeviceToHost, s));
+		HIP_CHECK(hipStreamSynchronize(s));
+		t_last_error.clear();
+		return 0;
+	}
+	catch (const std::exception& e)
+	{
+		t_last_error = e.what();
+		return -1;
+	}
+}
+
 extern "C" float meshopt_simplifyScale(const float* vertex_positions, size_t vertex_count, size_t vertex_positions_stride)
 {
 	try
@@ -830,7 +1051,22 @@ extern "C" int meshopt_hip_last_call_stats(meshopt_hip_call_stats* call, meshopt
 	return n;
 }
 
-extern "C" void meshopt_hip_kernel_timing(int enable, double* total_ms, unsigned long long* launches, unsigned long long* ranks)
+extern "C" void meshopt_hip_batch_kernel_timing(double* total_ms, size_t* launches, size_t* meshes)
+{
+	Context& ctx = context();
+	std::lock_guard<std::mutex> guard(ctx.mutex);
+	if (total_ms)
+		*total_ms = ctx.batch_ms;
+	if (launches)
+		*launches = ctx.batch_launches;
+	if (meshes)
+		*meshes = ctx.batch_meshes;
+	ctx.batch_ms = 0;
+	ctx.batch_launches = 0;
+	ctx.batch_meshes = 0;
+}
+
+extern "C" void meshopt_hip_kernel_timing(int enable, double* total_ms, size_t* launches, size_t* ranks)
 {
 	Context& ctx = context();
 	std::lock_guard<std::mutex> guard(ctx.mutex);

@@ -114,6 +114,9 @@ def _rows(a, name):
         raise TypeError(f"{name} must be float32")
     if a.ndim == 1:
         a = a.reshape(-1, 3)
+    if a.shape[0] == 0:
+        # numpy gives empty arrays zero strides; keep a packed row stride for the C side
+        return np.zeros((1, a.shape[1]), dtype=np.float32)[:0]
     if a.strides[1] != 4 or a.strides[0] % 4 or a.strides[0] < 4 * a.shape[1]:
         raise ValueError(f"{name} rows must be packed float32 with a row stride >= its width")
     return a
@@ -179,3 +182,37 @@ def simplify_scale(vertices, impl="oracle"):
     lib = oracle_lib() if impl == "oracle" else ref_lib()
     fn = lib.moracle_simplify_scale if impl == "oracle" else lib.meshopt_simplifyScale
     return fn(vertices.ctypes.data, vertices.shape[0], vertices.shape[1] * 4)
+
+
+BATCH_SO = os.path.join(HERE, "libbatch_driver.so")
+
+
+def simplify_batch(indices, index_offsets, positions, vertex_offsets, targets, target_error, options=0, threads=1,
+                   impl="ref"):
+    """Runs the reference library (impl="ref") or the C restatement (impl="oracle") over a packed
+    batch of meshes on `threads` POSIX threads (batch_driver.c).  positions is [V, k] float32.
+    Returns (destination, result_counts, result_errors)."""
+    drv = _load(BATCH_SO)
+    if impl == "ref":
+        fn = ctypes.cast(ref_lib().meshopt_simplify, ctypes.c_void_p).value
+    elif impl == "oracle":
+        fn = ctypes.cast(oracle_lib().moracle_simplify, ctypes.c_void_p).value
+    else:
+        raise ValueError(impl)
+    indices = np.ascontiguousarray(indices, dtype=np.uint32)
+    io = np.ascontiguousarray(index_offsets, dtype=np.uint32)
+    vo = np.ascontiguousarray(vertex_offsets, dtype=np.uint32)
+    tg = np.ascontiguousarray(targets, dtype=np.uint32)
+    positions = np.ascontiguousarray(positions, dtype=np.float32)
+    count = tg.size
+    dest = np.zeros(max(indices.size, 1), dtype=np.uint32)
+    counts = np.zeros(max(count, 1), dtype=np.uint32)
+    errors = np.zeros(max(count, 1), dtype=np.float32)
+    drv.batch_simplify.restype = ctypes.c_int
+    drv.batch_simplify.argtypes = [_P, ctypes.c_int, _P, _P, _P, _P, _P, _P, _P, _SZ, _P, _SZ, _F, _U]
+    rc = drv.batch_simplify(fn, int(threads), dest.ctypes.data, counts.ctypes.data, errors.ctypes.data,
+                            indices.ctypes.data, io.ctypes.data, positions.ctypes.data, vo.ctypes.data,
+                            positions.strides[0], tg.ctypes.data, count, float(target_error), int(options))
+    if rc != 0:
+        raise RuntimeError("batch_simplify could not start its threads")
+    return dest, counts[:count], errors[:count]
