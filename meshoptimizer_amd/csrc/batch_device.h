@@ -1,0 +1,1247 @@
+// Device code of the batched simplification of many small meshes (clusterlod-style patches):
+// one workgroup per patch, the whole patch state resident in LDS, the entire simplify (setup,
+// every pass, output) in one launch.  Each patch gets exactly what meshopt_simplify (src/simplifier.cpp:2628) returns
+// for it: same indices, count and result_error.
+//
+// Parallel phases (remap, adjacency, classification, quadrics, pick, rank, sort, compaction) use
+// the whole workgroup; the greedy commit (src/simplifier.cpp:1498-1636) is decided in rank order
+// on wavefront 0, 64 candidates per step, exactly as the reference decides it.  All local ids
+// are 16-bit.
+#pragma once
+
+#include "batch.h"
+#include "quadric.h"
+
+namespace mhip
+{
+
+constexpr int kBatchThreads = 256;
+
+// kCanCollapse / kHasOpposite, src/simplifier.cpp:312-331, as 25-bit masks (bit k0 * 5 + k1)
+// so that the lookups compile to shifts instead of per-lane memory loads
+constexpr uint8_t kCanCollapseTable[5][5] = {
+    {1, 1, 1, 1, 1},
+    {0, 1, 0, 0, 1},
+    {0, 0, 1, 0, 1},
+    {0, 0, 0, 1, 1},
+    {0, 0, 0, 0, 0},
+};
+
+constexpr uint8_t kHasOppositeTable[5][5] = {
+    {1, 1, 1, 1, 1},
+    {1, 0, 1, 0, 0},
+    {1, 1, 1, 0, 1},
+    {1, 0, 0, 0, 0},
+    {1, 0, 1, 0, 0},
+};
+
+constexpr uint32_t kind_pair_mask(const uint8_t (&t)[5][5])
+{
+	uint32_t m = 0;
+	for (int a = 0; a < 5; ++a)
+		for (int b = 0; b < 5; ++b)
+			m |= uint32_t(t[a][b] != 0) << (a * 5 + b);
+	return m;
+}
+
+constexpr uint32_t kCanCollapseMask = kind_pair_mask(kCanCollapseTable);
+constexpr uint32_t kHasOppositeMask = kind_pair_mask(kHasOppositeTable);
+
+__device__ __forceinline__ uint32_t kind_pair(uint32_t mask, uint32_t k0, uint32_t k1)
+{
+	return (mask >> (k0 * 5 + k1)) & 1u;
+}
+
+constexpr uint16_t kNone16 = 0xffff;
+
+// Optional phase timing (build with -DMESHOPT_BATCH_PROFILE): thread 0 of every workgroup adds
+// the shader-clock cycles spent in each phase, and event counts, into g_batch_profile.
+#ifdef MESHOPT_BATCH_PROFILE
+__device__ unsigned long long g_batch_profile[32];
+#define PROFILE_DECL unsigned long long prof_t = clock64(), prof_acc[32] = {}
+#define PROFILE_PHASE(k)                          \
+	do                                            \
+	{                                             \
+		unsigned long long now_ = clock64();      \
+		prof_acc[k] += now_ - prof_t;             \
+		prof_t = now_;                            \
+	} while (0)
+#define PROFILE_COUNT(k, n) (prof_acc[k] += (n))
+#define PROFILE_FLUSH()                                        \
+	do                                                         \
+	{                                                          \
+		if (threadIdx.x == 0)                                  \
+			for (int k_ = 0; k_ < 32; ++k_)                    \
+				atomicAdd(&g_batch_profile[k_], prof_acc[k_]); \
+	} while (0)
+#else
+#define PROFILE_DECL
+#define PROFILE_PHASE(k)
+#define PROFILE_COUNT(k, n)
+#define PROFILE_FLUSH()
+#endif
+
+// LDS layout of one patch, in bytes from the start of dynamic shared memory
+struct PatchLayout
+{
+	uint32_t vmax, imax, cmax;
+	uint32_t ib0, ib1, px, py, pz, remap, wedge, loop, loopback, cremap, kind, lock, off, adj, vq, cv0, cv1, cerr, order, scratch, total;
+	uint32_t scratch_bytes;
+};
+
+inline uint32_t align16(uint32_t x)
+{
+	return (x + 15) & ~15u;
+}
+
+inline PatchLayout patch_layout(uint32_t vmax, uint32_t imax)
+{
+	PatchLayout L{};
+	L.vmax = vmax;
+	L.imax = imax;
+	L.cmax = imax + 4;
+	uint32_t o = 0;
+	auto take = [&](uint32_t bytes) {
+		uint32_t at = o;
+		o = align16(o + bytes);
+		return at;
+	};
+	L.ib0 = take(imax * 2);
+	L.ib1 = take(imax * 2);
+	L.px = take(vmax * 4);
+	L.py = take(vmax * 4);
+	L.pz = take(vmax * 4);
+	L.remap = take(vmax * 2);
+	L.wedge = take(vmax * 2);
+	L.loop = take(vmax * 2);
+	L.loopback = take(vmax * 2);
+	L.cremap = take(vmax * 2);
+	L.kind = take(vmax);
+	L.lock = take(vmax);
+	L.off = take((vmax + 2) * 2);
+	L.adj = take(imax * 4);
+	L.vq = take(vmax * 11 * 4);
+	L.cv0 = take(L.cmax * 2);
+	L.cv1 = take(L.cmax * 2);
+	L.cerr = take(L.cmax * 4);
+	L.order = take(L.cmax * 2);
+	// scratch: hash table / corner ids / sort ping-pong / scan partials
+	uint32_t hsize = 16;
+	while (hsize < 2 * vmax + 2)
+		hsize <<= 1;
+	uint32_t hash = 4 * hsize;
+	uint32_t sortb = align16(L.cmax * 2) + 5 * 64 * 4;
+	uint32_t corners = imax * 2 + (vmax + 2) * 4;
+	uint32_t sb = hash;
+	sb = sb > sortb ? sb : sortb;
+	sb = sb > corners ? sb : corners;
+	sb += 256 * 4;
+	L.scratch_bytes = sb;
+	L.scratch = take(sb);
+	L.total = o;
+	return L;
+}
+
+struct PatchSmem
+{
+	uint16_t *ib0, *ib1, *remap, *wedge, *loop, *loopback, *cremap, *off, *cv0, *cv1, *order;
+	float *px, *py, *pz, *vq, *cerr;
+	uint8_t *kind, *lock;
+	uint32_t* adj; // next | prev << 16
+	uint8_t* scratch;
+	uint32_t vmax, imax, cmax;
+};
+
+__device__ __forceinline__ PatchSmem bind(uint8_t* base, const PatchLayout& L)
+{
+	PatchSmem s;
+	s.ib0 = (uint16_t*)(base + L.ib0);
+	s.ib1 = (uint16_t*)(base + L.ib1);
+	s.px = (float*)(base + L.px);
+	s.py = (float*)(base + L.py);
+	s.pz = (float*)(base + L.pz);
+	s.remap = (uint16_t*)(base + L.remap);
+	s.wedge = (uint16_t*)(base + L.wedge);
+	s.loop = (uint16_t*)(base + L.loop);
+	s.loopback = (uint16_t*)(base + L.loopback);
+	s.cremap = (uint16_t*)(base + L.cremap);
+	s.kind = base + L.kind;
+	s.lock = base + L.lock;
+	s.off = (uint16_t*)(base + L.off);
+	s.adj = (uint32_t*)(base + L.adj);
+	s.vq = (float*)(base + L.vq);
+	s.cv0 = (uint16_t*)(base + L.cv0);
+	s.cv1 = (uint16_t*)(base + L.cv1);
+	s.cerr = (float*)(base + L.cerr);
+	s.order = (uint16_t*)(base + L.order);
+	s.scratch = base + L.scratch;
+	s.vmax = L.vmax;
+	s.imax = L.imax;
+	s.cmax = L.cmax;
+	return s;
+}
+
+__device__ __forceinline__ V3 P(const PatchSmem& s, uint32_t i)
+{
+	return V3{s.px[i], s.py[i], s.pz[i]};
+}
+
+__device__ __forceinline__ Quad QL(const PatchSmem& s, uint32_t v)
+{
+	const uint32_t n = s.vmax;
+	Quad q;
+	q.a00 = s.vq[0 * n + v];
+	q.a11 = s.vq[1 * n + v];
+	q.a22 = s.vq[2 * n + v];
+	q.a10 = s.vq[3 * n + v];
+	q.a20 = s.vq[4 * n + v];
+	q.a21 = s.vq[5 * n + v];
+	q.b0 = s.vq[6 * n + v];
+	q.b1 = s.vq[7 * n + v];
+	q.b2 = s.vq[8 * n + v];
+	q.c = s.vq[9 * n + v];
+	q.w = s.vq[10 * n + v];
+	q.pad = 0.f;
+	return q;
+}
+
+__device__ __forceinline__ void QS(const PatchSmem& s, uint32_t v, const Quad& q)
+{
+	const uint32_t n = s.vmax;
+	s.vq[0 * n + v] = q.a00;
+	s.vq[1 * n + v] = q.a11;
+	s.vq[2 * n + v] = q.a22;
+	s.vq[3 * n + v] = q.a10;
+	s.vq[4 * n + v] = q.a20;
+	s.vq[5 * n + v] = q.a21;
+	s.vq[6 * n + v] = q.b0;
+	s.vq[7 * n + v] = q.b1;
+	s.vq[8 * n + v] = q.b2;
+	s.vq[9 * n + v] = q.c;
+	s.vq[10 * n + v] = q.w;
+}
+
+// exclusive scan of one value per thread across the workgroup; `part` holds kBatchThreads/64 words
+__device__ __forceinline__ uint32_t wg_scan(uint32_t v, uint32_t* part, uint32_t& total)
+{
+	const int lane = __lane_id();
+	uint32_t x = v;
+#pragma unroll
+	for (int o = 1; o < 64; o <<= 1)
+	{
+		uint32_t t = __shfl_up(x, o, 64);
+		if (lane >= o)
+			x += t;
+	}
+	const int wave = threadIdx.x >> 6;
+	__syncthreads();
+	if (lane == 63)
+		part[wave] = x;
+	__syncthreads();
+	uint32_t before = 0, all = 0;
+#pragma unroll
+	for (int w = 0; w < kBatchThreads / 64; ++w)
+	{
+		uint32_t c = part[w];
+		before += w < wave ? c : 0;
+		all += c;
+	}
+	total = all;
+	return before + x - v;
+}
+
+// counting placement of `count` items into per-key segments: off[key] = segment start, off[n] = total.
+// keys come from key_of(i); ids of the items in each segment come out in ascending item order.
+// cursor: n + 1 words of scratch.  All threads call it.
+template <typename KeyOf>
+__device__ void segment_items(uint16_t* off, uint16_t* items, uint32_t n, uint32_t count, KeyOf key_of, uint32_t* cursor, uint32_t* part)
+{
+	for (uint32_t v = threadIdx.x; v <= n; v += kBatchThreads)
+		cursor[v] = 0;
+	__syncthreads();
+	for (uint32_t i = threadIdx.x; i < count; i += kBatchThreads)
+		atomicAdd(&cursor[key_of(i)], 1u);
+	__syncthreads();
+	// exclusive scan of cursor[0..n) (n <= a few thousand): chunks of kBatchThreads
+	uint32_t carry = 0;
+	for (uint32_t base = 0; base < n; base += kBatchThreads)
+	{
+		uint32_t v = base + threadIdx.x;
+		uint32_t c = v < n ? cursor[v] : 0;
+		uint32_t tot;
+		uint32_t ex = wg_scan(c, part, tot);
+		if (v < n)
+			off[v] = (uint16_t)(carry + ex);
+		carry += tot;
+		__syncthreads();
+	}
+	if (threadIdx.x == 0)
+		off[n] = (uint16_t)carry;
+	__syncthreads();
+	for (uint32_t v = threadIdx.x; v < n; v += kBatchThreads)
+		cursor[v] = off[v];
+	__syncthreads();
+	for (uint32_t i = threadIdx.x; i < count; i += kBatchThreads)
+		items[atomicAdd(&cursor[key_of(i)], 1u)] = (uint16_t)i;
+	__syncthreads();
+	// ascending item order inside each segment (segments are short)
+	for (uint32_t v = threadIdx.x; v < n; v += kBatchThreads)
+	{
+		uint32_t b = off[v], e = off[v + 1];
+		for (uint32_t x = b + 1; x < e; ++x)
+		{
+			uint16_t t = items[x];
+			uint32_t y = x;
+			while (y > b && items[y - 1] > t)
+			{
+				items[y] = items[y - 1];
+				--y;
+			}
+			items[y] = t;
+		}
+	}
+	__syncthreads();
+}
+
+__device__ __forceinline__ bool lds_has_edge(const PatchSmem& s, uint32_t a, uint32_t b)
+{
+	for (uint32_t p = s.off[a], e = s.off[a + 1]; p < e; ++p)
+		if ((s.adj[p] & 0xffffu) == b)
+			return true;
+	return false;
+}
+
+__device__ __forceinline__ bool lds_has_edge_wedged(const PatchSmem& s, uint32_t a, uint32_t b)
+{
+	uint32_t rb = s.remap[b];
+	uint32_t v = a;
+	do
+	{
+		for (uint32_t p = s.off[v], e = s.off[v + 1]; p < e; ++p)
+			if (s.remap[s.adj[p] & 0xffffu] == rb)
+				return true;
+		v = s.wedge[v];
+	} while (v != a);
+	return false;
+}
+
+__device__ __forceinline__ uint32_t lds_complex_target(const PatchSmem& s, uint32_t v, uint32_t target)
+{
+	uint32_t r = s.remap[target];
+	uint32_t l = s.loop[v];
+	if (l != kNone16 && s.remap[l] == r)
+		return l;
+	uint32_t lb = s.loopback[v];
+	if (lb != kNone16 && s.remap[lb] == r)
+		return lb;
+	return target;
+}
+
+__device__ __forceinline__ uint32_t canon_bits(float f)
+{
+	uint32_t u = __float_as_uint(f);
+	return u == 0x80000000u ? 0u : u;
+}
+
+// build the corner CSR of the current index buffer keyed by `remap_key ? remap[v] : v`, sorted by
+// corner order when `ordered`; adj[p] = next | prev << 16 (both keyed the same way)
+__device__ void build_adjacency(const PatchSmem& s, const uint16_t* ib, uint32_t icount, uint32_t vcount, bool welded, uint16_t* corners,
+                                uint32_t* cursor, uint32_t* part)
+{
+	auto key_of = [&](uint32_t j) -> uint32_t {
+		uint32_t v = ib[j];
+		return welded ? s.remap[v] : v;
+	};
+	segment_items(s.off, corners, vcount, icount, key_of, cursor, part);
+	for (uint32_t p = threadIdx.x; p < icount; p += kBatchThreads)
+	{
+		uint32_t j = corners[p];
+		uint32_t t = j / 3, c = j - t * 3;
+		uint32_t nx = ib[t * 3 + (c == 2 ? 0 : c + 1)];
+		uint32_t pv = ib[t * 3 + (c == 0 ? 2 : c - 1)];
+		if (welded)
+		{
+			nx = s.remap[nx];
+			pv = s.remap[pv];
+		}
+		s.adj[p] = nx | (pv << 16);
+	}
+	__syncthreads();
+}
+
+// The whole simplify of one mesh by the calling workgroup; every return is workgroup-uniform.
+// sh[0..4] and shf[] are scratch words of this function.
+__device__ __forceinline__ void simplify_patch(const BatchArgs& a, const PatchLayout& L, uint32_t patch, uint8_t* smem, uint32_t* part,
+                                               uint32_t* sh, uint64_t* box, float* shf)
+{
+	const uint32_t ib_begin = a.index_offsets[patch], ib_end = a.index_offsets[patch + 1];
+	const uint32_t vb_begin = a.vertex_offsets[patch], vb_end = a.vertex_offsets[patch + 1];
+	const uint32_t I = ib_end - ib_begin;
+	const uint32_t V = vb_end - vb_begin;
+	const uint32_t target = a.target_index_counts[patch];
+	const uint32_t options = a.options;
+	const int tid = threadIdx.x;
+
+	if (V > L.vmax || I > L.imax || V >= kNone16)
+	{
+		if (tid == 0)
+			a.status[patch] = 1; // does not fit: the host runs it through the plain path
+		return;
+	}
+
+	PatchSmem s = bind(smem, L);
+	uint32_t* cursor = (uint32_t*)s.scratch; // V + 1 words
+	PROFILE_DECL;
+
+	// ---------------------------------------------------------------- load
+	if (tid == 0)
+		sh[0] = 0;
+	__syncthreads();
+	for (uint32_t j = tid; j < I; j += kBatchThreads)
+	{
+		uint32_t v = a.indices[ib_begin + j];
+		if (v >= V)
+			atomicOr(&sh[0], 1u);
+		s.ib0[j] = (uint16_t)(v < V ? v : 0);
+	}
+	const size_t stride = a.position_stride_floats;
+	for (uint32_t i = tid; i < V; i += kBatchThreads)
+	{
+		const float* p = a.positions + (size_t)(vb_begin + i) * stride;
+		s.px[i] = p[0];
+		s.py[i] = p[1];
+		s.pz[i] = p[2];
+	}
+	__syncthreads();
+	if (sh[0])
+	{
+		if (tid == 0)
+			a.status[patch] = 2; // index out of range
+		return;
+	}
+
+	// ---------------------------------------------------------------- position remap (src/simplifier.cpp:201-240)
+	{
+		uint32_t* table = (uint32_t*)s.scratch;
+		uint32_t hsize = 16;
+		while (hsize < 2 * V + 2)
+			hsize <<= 1;
+		const uint32_t mask = hsize - 1;
+		for (uint32_t h = tid; h < hsize; h += kBatchThreads)
+			table[h] = kNone;
+		__syncthreads();
+		for (uint32_t i = tid; i < V; i += kBatchThreads)
+		{
+			float x = s.px[i], y = s.py[i], z = s.pz[i];
+			if (x != x || y != y || z != z)
+				continue;
+			uint64_t h = (uint64_t)canon_bits(x) * 0x9E3779B97F4A7C15ull ^ (uint64_t)canon_bits(y) * 0xC2B2AE3D27D4EB4Full ^
+			             (uint64_t)canon_bits(z) * 0x165667B19E3779F9ull;
+			uint32_t slot = ((uint32_t)h ^ (uint32_t)(h >> 32)) & mask;
+			for (uint32_t probe = 0; probe <= mask; ++probe)
+			{
+				uint32_t old = atomicCAS(&table[slot], kNone, i);
+				if (old == kNone)
+					break;
+				if (s.px[old] == x && s.py[old] == y && s.pz[old] == z)
+				{
+					atomicMin(&table[slot], i);
+					break;
+				}
+				slot = (slot + 1) & mask;
+			}
+		}
+		__syncthreads();
+		for (uint32_t i = tid; i < V; i += kBatchThreads)
+		{
+			float x = s.px[i], y = s.py[i], z = s.pz[i];
+			uint32_t r = i;
+			if (!(x != x || y != y || z != z))
+			{
+				uint64_t h = (uint64_t)canon_bits(x) * 0x9E3779B97F4A7C15ull ^ (uint64_t)canon_bits(y) * 0xC2B2AE3D27D4EB4Full ^
+				             (uint64_t)canon_bits(z) * 0x165667B19E3779F9ull;
+				uint32_t slot = ((uint32_t)h ^ (uint32_t)(h >> 32)) & mask;
+				for (uint32_t probe = 0; probe <= mask; ++probe)
+				{
+					uint32_t e = table[slot];
+					if (e == kNone)
+						break;
+					if (s.px[e] == x && s.py[e] == y && s.pz[e] == z)
+					{
+						r = e;
+						break;
+					}
+					slot = (slot + 1) & mask;
+				}
+			}
+			s.remap[i] = (uint16_t)r;
+		}
+		__syncthreads();
+		// wedge rings r -> a_m -> ... -> a_1 -> r: a secondary links to the next lower member
+		uint32_t* wmax = (uint32_t*)s.scratch;
+		for (uint32_t i = tid; i < V; i += kBatchThreads)
+			wmax[i] = 0;
+		__syncthreads();
+		for (uint32_t i = tid; i < V; i += kBatchThreads)
+		{
+			uint32_t r = s.remap[i];
+			uint32_t w = i;
+			if (r != i)
+			{
+				uint32_t j = i - 1;
+				while (s.remap[j] != r)
+					--j;
+				w = j;
+				atomicMax(&wmax[r], i);
+			}
+			s.wedge[i] = (uint16_t)w;
+		}
+		__syncthreads();
+		for (uint32_t i = tid; i < V; i += kBatchThreads)
+			if (s.remap[i] == i && wmax[i])
+				s.wedge[i] = (uint16_t)wmax[i];
+		__syncthreads();
+	}
+
+	PROFILE_PHASE(0);
+	// ---------------------------------------------------------------- unwelded adjacency, classification
+	uint16_t* corners = (uint16_t*)(s.scratch + (V + 2) * 4);
+	build_adjacency(s, s.ib0, I, V, false, corners, cursor, part);
+
+	for (uint32_t v = tid; v < V; v += kBatchThreads)
+	{
+		uint32_t out_count = 0, out_last = kNone, in_count = 0, in_last = kNone;
+		bool self = false;
+		for (uint32_t p = s.off[v], e = s.off[v + 1]; p < e; ++p)
+		{
+			uint32_t nx = s.adj[p] & 0xffffu, pv = s.adj[p] >> 16;
+			if (nx == v)
+				self = true;
+			else if (!lds_has_edge(s, nx, v))
+			{
+				out_count++;
+				out_last = nx;
+			}
+			if (pv == v)
+				self = true;
+			else if (!lds_has_edge(s, v, pv))
+			{
+				in_count++;
+				in_last = pv;
+			}
+		}
+		s.loop[v] = (uint16_t)((self || out_count > 1) ? v : (out_count == 1 ? out_last : kNone16));
+		s.loopback[v] = (uint16_t)((self || in_count > 1) ? v : (in_count == 1 ? in_last : kNone16));
+	}
+	__syncthreads();
+
+	for (uint32_t i = tid; i < V; i += kBatchThreads)
+	{
+		if (s.remap[i] != i)
+			continue;
+		uint8_t k;
+		uint32_t oi = s.loopback[i], oo = s.loop[i];
+		if (s.wedge[i] == i)
+		{
+			if (oi == kNone16 && oo == kNone16)
+				k = kManifold;
+			else if (oi != kNone16 && oo != kNone16 && s.remap[oi] == s.remap[oo] && oi != i)
+				k = kSeam;
+			else if (oi != i && oo != i)
+				k = kBorder;
+			else
+				k = kLocked;
+		}
+		else if (s.wedge[s.wedge[i]] == i)
+		{
+			uint32_t w = s.wedge[i];
+			uint32_t iw = s.loopback[w], ow = s.loop[w];
+			bool open = oi != kNone16 && oi != i && oo != kNone16 && oo != i && iw != kNone16 && iw != w && ow != kNone16 && ow != w;
+			k = (open && s.remap[oi] == s.remap[ow] && s.remap[oo] == s.remap[iw] && s.remap[oi] != s.remap[oo]) ? kSeam : kLocked;
+		}
+		else
+			k = kLocked;
+		s.kind[i] = k;
+	}
+	__syncthreads();
+	if (options & kOptPermissive)
+	{
+		for (uint32_t i = tid; i < V; i += kBatchThreads)
+		{
+			if (s.remap[i] != i || (s.kind[i] != kSeam && s.kind[i] != kLocked))
+				continue;
+			bool protect = false;
+			uint32_t v = i;
+			do
+			{
+				for (uint32_t p = s.off[v], e = s.off[v + 1]; p < e && !protect; ++p)
+					protect |= !lds_has_edge_wedged(s, s.adj[p] & 0xffffu, v);
+				v = s.wedge[v];
+			} while (v != i);
+			if (!protect)
+				s.kind[i] = kComplex;
+		}
+		__syncthreads();
+	}
+	for (uint32_t i = tid; i < V; i += kBatchThreads)
+	{
+		uint8_t k = s.kind[s.remap[i]];
+		if ((options & kOptLockBorder) && k == kBorder)
+			k = kLocked;
+		s.kind[i] = k;
+	}
+	__syncthreads();
+
+	// boundEdgeCollapses (src/simplifier.cpp:1292-1308) on the unwelded adjacency
+	uint32_t capacity;
+	{
+		uint32_t d = 0;
+		for (uint32_t i = tid; i < V; i += kBatchThreads)
+			d += (s.kind[i] == kManifold || s.kind[i] == kSeam) ? s.off[i + 1] - s.off[i] : 0u;
+		uint32_t tot;
+		wg_scan(d, part, tot);
+		capacity = (I - tot / 2) + 3;
+	}
+
+	PROFILE_PHASE(1);
+	// ---------------------------------------------------------------- rescale (src/simplifier.cpp:549-603)
+	if (tid < 6)
+		box[tid] = tid < 3 ? ~0ull : 0ull;
+	__syncthreads();
+	{
+		uint64_t mn[3] = {~0ull, ~0ull, ~0ull}, mx[3] = {0, 0, 0};
+		for (uint32_t i = tid; i < V; i += kBatchThreads)
+		{
+			float c[3] = {s.px[i], s.py[i], s.pz[i]};
+			for (int k = 0; k < 3; ++k)
+			{
+				if (c[k] != c[k])
+					continue;
+				uint64_t key = (uint64_t)ordered_key(c[k]) << 32;
+				uint64_t lo = key | i, hi = key | (0xffffffffu - i);
+				mn[k] = lo < mn[k] ? lo : mn[k];
+				mx[k] = hi > mx[k] ? hi : mx[k];
+			}
+		}
+		// one LDS atomic per wavefront and bound
+		for (int k = 0; k < 3; ++k)
+		{
+			const uint64_t wmn = wave_min64(mn[k]), wmx = wave_max64(mx[k]);
+			if (__lane_id() == 0)
+			{
+				if (wmn != ~0ull)
+					atomicMin((unsigned long long*)&box[k], (unsigned long long)wmn);
+				if (wmx != 0ull)
+					atomicMax((unsigned long long*)&box[3 + k], (unsigned long long)wmx);
+			}
+		}
+	}
+	__syncthreads();
+	if (tid == 0)
+	{
+		float lo[3], hi[3];
+		const float* comp[3] = {s.px, s.py, s.pz};
+		for (int k = 0; k < 3; ++k)
+		{
+			lo[k] = FLT_MAX;
+			hi[k] = -FLT_MAX;
+			if (box[k] != ~0ull)
+			{
+				float v = comp[k][(uint32_t)box[k]];
+				lo[k] = FLT_MAX > v ? v : FLT_MAX;
+			}
+			if (box[3 + k] != 0ull)
+			{
+				float v = comp[k][0xffffffffu - (uint32_t)box[3 + k]];
+				hi[k] = -FLT_MAX < v ? v : -FLT_MAX;
+			}
+		}
+		float extent = 0.f;
+		extent = (hi[0] - lo[0]) < extent ? extent : (hi[0] - lo[0]);
+		extent = (hi[1] - lo[1]) < extent ? extent : (hi[1] - lo[1]);
+		extent = (hi[2] - lo[2]) < extent ? extent : (hi[2] - lo[2]);
+		shf[0] = lo[0];
+		shf[1] = lo[1];
+		shf[2] = lo[2];
+		shf[3] = extent == 0 ? 0.f : 1.f / extent;
+		shf[4] = extent;
+	}
+	__syncthreads();
+	for (uint32_t i = tid; i < V; i += kBatchThreads)
+	{
+		float sc = shf[3];
+		s.px[i] = (s.px[i] - shf[0]) * sc;
+		s.py[i] = (s.py[i] - shf[1]) * sc;
+		s.pz[i] = (s.pz[i] - shf[2]) * sc;
+	}
+	const float extent = shf[4];
+	__syncthreads();
+	PROFILE_PHASE(12);
+
+	// ---------------------------------------------------------------- vertex quadrics (src/simplifier.cpp:1057-1157)
+	build_adjacency(s, s.ib0, I, V, true, corners, cursor, part);
+	PROFILE_PHASE(13);
+	{
+		const float factor = (options & kOptRegularizeLight) ? 1e-2f : ((options & kOptRegularize) ? 1e-1f : 1e-7f);
+		for (uint32_t r = tid; r < V; r += kBatchThreads)
+		{
+			Quad Q;
+			quad_zero(Q);
+			if (s.remap[r] == r)
+			{
+				uint32_t b = s.off[r], e = s.off[r + 1];
+				for (uint32_t p = b; p < e; ++p)
+				{
+					uint32_t t = corners[p] / 3;
+					Quad F;
+					quad_from_triangle(F, P(s, s.ib0[t * 3 + 0]), P(s, s.ib0[t * 3 + 1]), P(s, s.ib0[t * 3 + 2]), 1.f);
+					quad_add(Q, F);
+				}
+				{
+					V3 v = P(s, r);
+					Quad Pq;
+					quad_from_point(Pq, v.x, v.y, v.z, Q.w * factor);
+					quad_add(Q, Pq);
+				}
+				uint32_t last_t = kNone;
+				for (uint32_t p = b; p < e; ++p)
+				{
+					uint32_t t = corners[p] / 3;
+					if (t == last_t)
+						continue;
+					last_t = t;
+					uint32_t tri[3] = {s.ib0[t * 3 + 0], s.ib0[t * 3 + 1], s.ib0[t * 3 + 2]};
+					for (int ed = 0; ed < 3; ++ed)
+					{
+						uint32_t i0 = tri[ed], i1 = tri[ed == 2 ? 0 : ed + 1], i2 = tri[ed == 0 ? 2 : ed - 1];
+						bool at0 = s.remap[i0] == r, at1 = s.remap[i1] == r;
+						if (!at0 && !at1)
+							continue;
+						uint8_t k0 = s.kind[i0], k1 = s.kind[i1];
+						bool b0 = k0 == kBorder || k0 == kSeam, b1 = k1 == kBorder || k1 == kSeam;
+						if (!b0 && !b1)
+							continue;
+						if (b0 && s.loop[i0] != i1)
+							continue;
+						if (b1 && s.loopback[i1] != i0)
+							continue;
+						float weight = (k0 == kBorder || k1 == kBorder) ? 10.f : 0.5f;
+						Quad E;
+						quad_edge_term(E, P(s, i0), P(s, i1), P(s, i2), weight);
+						if (at0)
+							quad_add(Q, E);
+						if (at1)
+							quad_add(Q, E);
+					}
+				}
+			}
+			QS(s, r, Q);
+		}
+	}
+	__syncthreads();
+
+	PROFILE_PHASE(2);
+	const float error_scale = (options & kOptErrorAbsolute) ? extent : 1.f;
+	const float error_limit = (a.target_error * a.target_error) / (error_scale * error_scale);
+
+	// ---------------------------------------------------------------- pass loop (src/simplifier.cpp:2493-2541)
+	uint16_t* ib = s.ib0;
+	uint16_t* ib_alt = s.ib1;
+	uint32_t result_count = I;
+	float result_error = 0.f;
+	uint32_t passes = 0;
+
+	// every pass that commits removes at least one triangle, so I / 3 + 1 passes always suffice;
+	// the bound is an exit every wavefront reaches even on inconsistent state
+	const uint32_t pass_limit = I / 3 + 1;
+	bool overrun = false;
+	while (result_count > target)
+	{
+		if (passes >= pass_limit)
+		{
+			overrun = true;
+			break;
+		}
+		const uint32_t T = result_count / 3;
+		build_adjacency(s, ib, result_count, V, true, corners, cursor, part);
+		PROFILE_PHASE(3);
+
+		// pick (src/simplifier.cpp:1310-1372): per triangle, order-preserving compaction
+		uint32_t C = 0;
+		if (tid == 0)
+			sh[1] = 0;
+		__syncthreads();
+		{
+			uint32_t carry = 0;
+			for (uint32_t base = 0; base < T; base += kBatchThreads)
+			{
+				uint32_t t = base + tid;
+				// up to three candidates per triangle, packed v0 | v1 << 16, in named registers
+				uint32_t e0 = 0, e1 = 0, e2 = 0, bidi = 0;
+				int cnt = 0;
+				if (t < T)
+				{
+					uint32_t tri[3] = {ib[t * 3 + 0], ib[t * 3 + 1], ib[t * 3 + 2]};
+#pragma unroll
+					for (int ed = 0; ed < 3; ++ed)
+					{
+						uint32_t i0 = tri[ed], i1 = tri[ed == 2 ? 0 : ed + 1];
+						uint32_t r0 = s.remap[i0], r1 = s.remap[i1];
+						if (r0 == r1)
+							continue;
+						uint8_t k0 = s.kind[i0], k1 = s.kind[i1];
+						uint32_t c01 = kind_pair(kCanCollapseMask, k0, k1), c10 = kind_pair(kCanCollapseMask, k1, k0);
+						if (!(c01 | c10))
+							continue;
+						if (kind_pair(kHasOppositeMask, k0, k1) && r1 > r0)
+							continue;
+						if ((k0 == kBorder || k0 == kSeam) && k1 != kManifold && s.loop[i0] != i1)
+							continue;
+						if ((k1 == kBorder || k1 == kSeam) && k0 != kManifold && s.loopback[i1] != i0)
+							continue;
+						uint32_t packed, bd;
+						if (c01 & c10)
+						{
+							packed = i0 | (i1 << 16);
+							bd = 1;
+						}
+						else
+						{
+							packed = c01 ? (i0 | (i1 << 16)) : (i1 | (i0 << 16));
+							bd = 0;
+						}
+						if (cnt == 0)
+							e0 = packed;
+						else if (cnt == 1)
+							e1 = packed;
+						else
+							e2 = packed;
+						bidi |= bd << cnt;
+						cnt++;
+					}
+				}
+				uint32_t tot;
+				uint32_t ex = wg_scan((uint32_t)cnt, part, tot);
+				uint32_t at = carry + ex;
+				// triangles are taken while `written + 3 <= capacity` (src/simplifier.cpp:1318-1320);
+				// the running offset only grows, so the taken triangles form a prefix
+				if (t < T && at + 3 <= capacity)
+				{
+					if (cnt > 0)
+					{
+						s.cv0[at] = (uint16_t)(e0 & 0xffffu);
+						s.cv1[at] = (uint16_t)(e0 >> 16);
+						s.cerr[at] = __uint_as_float(bidi & 1u);
+					}
+					if (cnt > 1)
+					{
+						s.cv0[at + 1] = (uint16_t)(e1 & 0xffffu);
+						s.cv1[at + 1] = (uint16_t)(e1 >> 16);
+						s.cerr[at + 1] = __uint_as_float((bidi >> 1) & 1u);
+					}
+					if (cnt > 2)
+					{
+						s.cv0[at + 2] = (uint16_t)(e2 & 0xffffu);
+						s.cv1[at + 2] = (uint16_t)(e2 >> 16);
+						s.cerr[at + 2] = __uint_as_float((bidi >> 2) & 1u);
+					}
+					// the last taken triangle (the next one would not fit, or there is none) ends the list
+					if (t + 1 == T || at + cnt + 3 > capacity)
+						sh[1] = at + cnt;
+				}
+				carry += tot;
+				__syncthreads();
+			}
+			C = sh[1];
+			__syncthreads();
+		}
+
+		PROFILE_PHASE(4);
+		PROFILE_COUNT(20, C);
+		if (C == 0)
+			break;
+
+		// rank (src/simplifier.cpp:1374-1450)
+		for (uint32_t c = tid; c < C; c += kBatchThreads)
+		{
+			uint32_t i0 = s.cv0[c], i1 = s.cv1[c];
+			bool bidi = __float_as_uint(s.cerr[c]) != 0;
+			float ei = quad_error(QL(s, s.remap[i0]), P(s, i1));
+			float ej = bidi ? quad_error(QL(s, s.remap[i1]), P(s, i0)) : FLT_MAX;
+			bool rev = bidi & (ej < ei);
+			s.cv0[c] = (uint16_t)(rev ? i1 : i0);
+			s.cv1[c] = (uint16_t)(rev ? i0 : i1);
+			s.cerr[c] = ej < ei ? ej : ei;
+		}
+		__syncthreads();
+
+		PROFILE_PHASE(5);
+		// stable sort by the 12-bit key (src/simplifier.cpp:1452-1496): LSD, two 6-bit digits
+		{
+			uint16_t* src = s.order;
+			uint16_t* dst = (uint16_t*)s.scratch;
+			for (uint32_t c = tid; c < C; c += kBatchThreads)
+				src[c] = (uint16_t)c;
+			__syncthreads();
+			uint32_t* hist = (uint32_t*)(s.scratch + ((s.cmax * 2 + 15) & ~15u)); // 64 digit offsets + [4][64] wave counts
+			for (int pass = 0; pass < 2; ++pass)
+			{
+				const int shift = pass * 6;
+				for (uint32_t b = tid; b < 64; b += kBatchThreads)
+					hist[b] = 0;
+				__syncthreads();
+				for (uint32_t c = tid; c < C; c += kBatchThreads)
+					atomicAdd(&hist[(collapse_sort_key(s.cerr[src[c]]) >> shift) & 63u], 1u);
+				__syncthreads();
+				if (tid < 64)
+				{
+					uint32_t v = hist[tid];
+					uint32_t x = v;
+					for (int o = 1; o < 64; o <<= 1)
+					{
+						uint32_t t = __shfl_up(x, o, 64);
+						if ((int)__lane_id() >= o)
+							x += t;
+					}
+					hist[tid] = x - v;
+				}
+				__syncthreads();
+				// stable scatter: chunks of kBatchThreads in order; rank among equal digits by ballots
+				for (uint32_t base = 0; base < C; base += kBatchThreads)
+				{
+					uint32_t c = base + tid;
+					bool valid = c < C;
+					uint32_t item = valid ? src[c] : 0;
+					uint32_t d = valid ? (collapse_sort_key(s.cerr[item]) >> shift) & 63u : 0;
+					uint64_t peers = __ballot(valid);
+					for (int b = 0; b < 6; ++b)
+					{
+						uint64_t bb = __ballot((d >> b) & 1u);
+						peers &= ((d >> b) & 1u) ? bb : ~bb;
+					}
+					uint32_t lane = __lane_id();
+					uint32_t rank_in_wave = (uint32_t)__popcll(peers & ((1ull << lane) - 1ull));
+					uint32_t* wcount = hist + 64; // [4][64]
+					const int wave = tid >> 6;
+					for (uint32_t b = tid; b < 4 * 64; b += kBatchThreads)
+						wcount[b] = 0;
+					__syncthreads();
+					if (valid && (63 - __clzll((long long)peers)) == (int)lane)
+						wcount[wave * 64 + d] = (uint32_t)__popcll(peers);
+					__syncthreads();
+					uint32_t before = 0;
+					for (int w = 0; w < wave; ++w)
+						before += wcount[w * 64 + d];
+					if (valid)
+						dst[hist[d] + before + rank_in_wave] = (uint16_t)item;
+					__syncthreads();
+					if (tid < 64)
+						hist[tid] += wcount[0 * 64 + tid] + wcount[1 * 64 + tid] + wcount[2 * 64 + tid] + wcount[3 * 64 + tid];
+					__syncthreads();
+				}
+				uint16_t* t = src;
+				src = dst;
+				dst = t;
+			}
+			// after two passes the sorted order sits back in s.order
+		}
+
+		PROFILE_PHASE(6);
+		// greedy commit (src/simplifier.cpp:1498-1636) on wavefront 0, 64 ranks at a time: every
+		// lane loads one candidate and evaluates its lock state, its commit targets and its flip
+		// test against the collapse_remap of the chunk start; the lanes are then resolved in rank
+		// order with scalar state, exactly as the sequential loop decides them.  A lane's flip
+		// result can only change if an earlier commit of the same chunk moved one of the welded
+		// neighbours its test reads (the source class of that commit); each lane keeps its first
+		// kNearSlots adjacency entries in registers to detect this exactly, and re-runs the test
+		// on the current collapse_remap only then.
+		for (uint32_t v = tid; v < V; v += kBatchThreads)
+		{
+			s.cremap[v] = (uint16_t)v;
+			s.lock[v] = 0;
+		}
+		__syncthreads();
+		if (tid < 64)
+		{
+			constexpr int kNearSlots = 8;
+			const uint32_t lane = tid;
+			const uint32_t goal = (result_count - target) / 3;
+			uint32_t edge_goal = goal / 2;
+			float error_goal = edge_goal < C ? 1.5f * s.cerr[s.order[edge_goal]] : FLT_MAX;
+			uint32_t tri_collapses = 0, committed = 0, evaluated = 0;
+			bool stop = false;
+			for (uint32_t base = 0; base < C && !stop; base += 64)
+			{
+				PROFILE_PHASE(15);
+				PROFILE_COUNT(24, 1);
+				const uint32_t k = base + lane;
+				uint32_t i0 = 0, i1 = 0, r0 = kNone16, r1 = kNone16, s0 = kNone16, s1 = 0, ab = 0, ae = 0;
+				float err = 0.f;
+				bool locked = true, flips = false, border = false, complex = false;
+				uint32_t near[kNearSlots];
+#pragma unroll
+				for (int q = 0; q < kNearSlots; ++q)
+					near[q] = 0xffffffffu; // 0xffff never equals a vertex id
+				if (k < C)
+				{
+					const uint32_t c = s.order[k];
+					i0 = s.cv0[c];
+					i1 = s.cv1[c];
+					err = s.cerr[c];
+					r0 = s.remap[i0];
+					r1 = s.remap[i1];
+					locked = (s.lock[r0] | s.lock[r1]) != 0;
+					const uint8_t kd = s.kind[i0];
+					border = kd == kBorder;
+					complex = kd == kComplex;
+					if (kd == kSeam)
+					{
+						s0 = s.wedge[i0];
+						uint32_t t = s.loop[i0] == i1 ? s.loopback[s0] : s.loop[s0];
+						s1 = t != kNone16 ? t : s.wedge[i1];
+					}
+					ab = s.off[r0];
+					ae = s.off[r0 + 1];
+					if (!locked)
+					{
+						// hasTriangleFlips (src/simplifier.cpp:1200-1232); a boolean OR over the
+						// corners, evaluated without early exit so the loads overlap
+						const V3 v0 = P(s, r0), v1 = P(s, r1);
+#pragma unroll
+						for (int q = 0; q < kNearSlots; ++q)
+						{
+							if (ab + q < ae)
+							{
+								const uint32_t ad = s.adj[ab + q];
+								near[q] = ad;
+								const uint32_t a2 = s.cremap[ad & 0xffffu], b2 = s.cremap[ad >> 16];
+								if (!(a2 == r1 || b2 == r1 || a2 == b2))
+									flips |= triangle_flips(P(s, a2), P(s, b2), v0, v1);
+							}
+						}
+						for (uint32_t p = ab + kNearSlots; p < ae; ++p)
+						{
+							const uint32_t ad = s.adj[p];
+							const uint32_t a2 = s.cremap[ad & 0xffffu], b2 = s.cremap[ad >> 16];
+							if (!(a2 == r1 || b2 == r1 || a2 == b2))
+								flips |= triangle_flips(P(s, a2), P(s, b2), v0, v1);
+						}
+					}
+				}
+				uint64_t lock_mask = __ballot(locked);
+				const uint64_t flip_mask = __ballot(flips);
+				// lanes whose corners do not all fit the register slots count as stale after any commit
+				const uint64_t wide_mask = __ballot(ae - ab > (uint32_t)kNearSlots);
+				uint64_t stale_mask = 0;
+				const uint32_t n = C - base < 64 ? C - base : 64;
+				PROFILE_PHASE(14);
+				for (uint32_t j = 0; j < n; ++j)
+				{
+					evaluated++;
+					const float ej = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(err), j));
+					if (ej > error_limit || tri_collapses >= goal || (ej > error_goal && ej > result_error && tri_collapses > goal / 6))
+					{
+						stop = true;
+						break;
+					}
+					if ((lock_mask >> j) & 1)
+						continue;
+					const uint32_t r0j = __builtin_amdgcn_readlane(r0, j), r1j = __builtin_amdgcn_readlane(r1, j);
+					bool fj = (flip_mask >> j) & 1;
+					if ((stale_mask >> j) & 1)
+					{
+						// re-run on the current collapse_remap, one adjacency entry per lane
+						PROFILE_COUNT(25, 1);
+						fj = false;
+						const uint32_t b = __builtin_amdgcn_readlane(ab, j), e = __builtin_amdgcn_readlane(ae, j);
+						const V3 v0 = P(s, r0j), v1 = P(s, r1j);
+						for (uint32_t p0 = b; p0 < e; p0 += 64)
+						{
+							const uint32_t p = p0 + lane;
+							bool f = false;
+							if (p < e)
+							{
+								const uint32_t ad = s.adj[p];
+								const uint32_t a2 = s.cremap[ad & 0xffffu], b2 = s.cremap[ad >> 16];
+								if (!(a2 == r1j || b2 == r1j || a2 == b2))
+									f = triangle_flips(P(s, a2), P(s, b2), v0, v1);
+							}
+							if (__ballot(f))
+							{
+								fj = true;
+								break;
+							}
+						}
+					}
+					if (fj)
+					{
+						edge_goal++;
+						error_goal = edge_goal < C ? 1.5f * s.cerr[s.order[edge_goal]] : FLT_MAX;
+						continue;
+					}
+					// commit (src/simplifier.cpp:1573-1611) from the candidate's own lane
+					if (lane == j)
+					{
+						if (complex)
+						{
+							uint32_t v = i0;
+							do
+							{
+								s.cremap[v] = (uint16_t)lds_complex_target(s, v, i1);
+								v = s.wedge[v];
+							} while (v != i0);
+						}
+						else
+						{
+							s.cremap[i0] = (uint16_t)i1;
+							if (s0 != kNone16)
+								s.cremap[s0] = (uint16_t)s1;
+						}
+						s.lock[r0] = 1;
+						s.lock[r1] = 1;
+					}
+					// the writes are read by later stale re-runs and by the next chunk's loads
+					__builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+					__builtin_amdgcn_wave_barrier();
+					lock_mask |= __ballot(r0 == r0j || r0 == r1j || r1 == r0j || r1 == r1j);
+					bool moved = false;
+#pragma unroll
+					for (int q = 0; q < kNearSlots; ++q)
+						moved |= (near[q] & 0xffffu) == r0j || (near[q] >> 16) == r0j;
+					stale_mask |= __ballot(moved) | wide_mask;
+					tri_collapses += __builtin_amdgcn_readlane((int)border, j) ? 1 : 2;
+					committed++;
+					result_error = result_error < ej ? ej : result_error;
+				}
+			}
+			if (lane == 0)
+			{
+				sh[4] = evaluated;
+				sh[3] = committed;
+				shf[5] = result_error;
+			}
+		}
+		__syncthreads();
+		PROFILE_PHASE(7);
+		PROFILE_COUNT(21, sh[4]);
+		PROFILE_COUNT(22, sh[3]);
+		result_error = shf[5];
+		passes++;
+		if (sh[3] == 0)
+			break;
+
+		// updateQuadrics (src/simplifier.cpp:1638-1673): each target class receives one add
+		for (uint32_t i = tid; i < V; i += kBatchThreads)
+		{
+			uint32_t t = s.cremap[i];
+			if (t != i && s.remap[i] == i)
+			{
+				Quad Q1 = QL(s, s.remap[t]);
+				quad_add(Q1, QL(s, i));
+				QS(s, s.remap[t], Q1);
+			}
+		}
+		__syncthreads();
+
+		// remapEdgeLoops (src/simplifier.cpp:1845-1864), double-buffered through scratch
+		{
+			uint16_t* nl = (uint16_t*)s.scratch;
+			uint16_t* nb = nl + s.vmax;
+			for (uint32_t i = tid; i < V; i += kBatchThreads)
+			{
+				const uint16_t* in[2] = {s.loop, s.loopback};
+				uint16_t* out[2] = {nl, nb};
+				for (int q = 0; q < 2; ++q)
+				{
+					uint32_t l = in[q][i];
+					uint32_t res = kNone16;
+					if (l != kNone16)
+					{
+						uint32_t r = s.cremap[l];
+						if (i == r)
+						{
+							uint32_t ll = in[q][l];
+							res = ll != kNone16 ? s.cremap[ll] : kNone16;
+						}
+						else
+							res = r;
+					}
+					out[q][i] = (uint16_t)res;
+				}
+			}
+			__syncthreads();
+			for (uint32_t i = tid; i < V; i += kBatchThreads)
+			{
+				s.loop[i] = nl[i];
+				s.loopback[i] = nb[i];
+			}
+			__syncthreads();
+		}
+
+		PROFILE_PHASE(8);
+		// remapIndexBuffer (src/simplifier.cpp:1811-1843): stable compaction into the other buffer
+		{
+			uint32_t carry = 0;
+			for (uint32_t base = 0; base < T; base += kBatchThreads)
+			{
+				uint32_t t = base + tid;
+				uint32_t v[3] = {0, 0, 0};
+				bool keep = false;
+				if (t < T)
+				{
+					v[0] = s.cremap[ib[t * 3 + 0]];
+					v[1] = s.cremap[ib[t * 3 + 1]];
+					v[2] = s.cremap[ib[t * 3 + 2]];
+					uint32_t q0 = s.remap[v[0]], q1 = s.remap[v[1]], q2 = s.remap[v[2]];
+					keep = q0 != q1 && q0 != q2 && q1 != q2;
+				}
+				uint32_t tot;
+				uint32_t ex = wg_scan(keep ? 1u : 0u, part, tot);
+				if (keep)
+				{
+					uint32_t o = (carry + ex) * 3;
+					ib_alt[o + 0] = (uint16_t)v[0];
+					ib_alt[o + 1] = (uint16_t)v[1];
+					ib_alt[o + 2] = (uint16_t)v[2];
+				}
+				carry += tot;
+				__syncthreads();
+			}
+			uint16_t* t = ib;
+			ib = ib_alt;
+			ib_alt = t;
+			result_count = carry * 3;
+		}
+		__syncthreads();
+		PROFILE_PHASE(9);
+	}
+	PROFILE_COUNT(23, passes);
+	PROFILE_PHASE(10);
+
+	// ---------------------------------------------------------------- output (src/simplifier.cpp:2601-2625)
+	const bool debug = (options & kOptInternalDebug) != 0;
+	for (uint32_t j = tid; j < result_count; j += kBatchThreads)
+	{
+		uint32_t v = ib[j];
+		if (debug)
+		{
+			uint32_t t = j / 3, c = j - t * 3;
+			uint32_t nxt = ib[t * 3 + (c == 2 ? 0 : c + 1)];
+			v |= ((uint32_t)s.kind[v] << 28) | ((uint32_t)(s.loop[v] == nxt || s.loopback[nxt] == v) << 31);
+		}
+		a.destination[ib_begin + j] = v;
+	}
+	if (tid == 0)
+	{
+		a.result_counts[patch] = result_count;
+		if (a.result_errors)
+			a.result_errors[patch] = sqrtf(result_error) * error_scale;
+		a.status[patch] = overrun ? 3u : 0u;
+		if (a.passes)
+			a.passes[patch] = passes;
+	}
+	PROFILE_PHASE(11);
+	PROFILE_FLUSH();
+}
+
+} // namespace mhip

@@ -127,6 +127,29 @@ __device__ __forceinline__ uint32_t ordered_key(float f)
 	return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
 }
 
+// 64-bit minimum / maximum over the 64 lanes of a wavefront
+__device__ __forceinline__ uint64_t wave_min64(uint64_t v)
+{
+#pragma unroll
+	for (int o = 32; o > 0; o >>= 1)
+	{
+		uint64_t t = __shfl_xor(v, o, 64);
+		v = t < v ? t : v;
+	}
+	return v;
+}
+
+__device__ __forceinline__ uint64_t wave_max64(uint64_t v)
+{
+#pragma unroll
+	for (int o = 32; o > 0; o >>= 1)
+	{
+		uint64_t t = __shfl_xor(v, o, 64);
+		v = t > v ? t : v;
+	}
+	return v;
+}
+
 inline unsigned int div_up(size_t a, size_t b)
 {
 	return (unsigned int)((a + b - 1) / b);

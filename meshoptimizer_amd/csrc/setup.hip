@@ -462,28 +462,6 @@ void launch_classify(hipStream_t s, Mesh& m, const uint32_t* raw_off, const uint
 // order) that compares below the running minimum, so among +0/-0 ties the earliest wins:
 // reduce (ordered key, index) pairs, then read the winning element's own bits.
 
-__device__ __forceinline__ uint64_t wave_min64(uint64_t v)
-{
-#pragma unroll
-	for (int o = 32; o > 0; o >>= 1)
-	{
-		uint64_t t = __shfl_xor(v, o, 64);
-		v = t < v ? t : v;
-	}
-	return v;
-}
-
-__device__ __forceinline__ uint64_t wave_max64(uint64_t v)
-{
-#pragma unroll
-	for (int o = 32; o > 0; o >>= 1)
-	{
-		uint64_t t = __shfl_xor(v, o, 64);
-		v = t > v ? t : v;
-	}
-	return v;
-}
-
 __global__ void __launch_bounds__(256) k_bbox(const float4* praw, size_t n, CallScalars* sc)
 {
 	__shared__ uint64_t red[6][4];

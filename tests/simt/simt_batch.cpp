@@ -1,0 +1,324 @@
+// Runs the batched kernel's device code (simplify_patch) on the CPU: one fiber per GPU thread,
+// rendezvous at every barrier and wavefront operation (shim/hip/hip_runtime.h).  A deadlock (a
+// barrier or wavefront operation that not every thread of its group reaches) is reported with the
+// source line each fiber waits at.  Test infrastructure only.
+#include <signal.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <sys/mman.h>
+#include <ucontext.h>
+#include <unistd.h>
+
+#include <vector>
+
+#include "batch_device.h"
+
+namespace simt
+{
+
+enum State
+{
+	kRunnable,
+	kAtBarrier,
+	kAtWave,
+	kDone
+};
+
+struct Fiber
+{
+	ucontext_t ctx;
+	Dim idx;
+	State state;
+	int line; // source line of the last rendezvous
+	unsigned long long syncs;
+	std::vector<char> stack;
+};
+
+struct Group
+{
+	int threads = 0;
+	std::vector<Fiber> fibers;
+	int current = -1;
+	ucontext_t scheduler;
+	// workgroup barrier
+	int barrier_count = 0;
+	// wavefront rendezvous, double-buffered by generation
+	std::vector<int> wave_count;
+	std::vector<unsigned long long> wave_gen;
+	std::vector<uint64_t> wave_buf; // [waves][2][64]
+};
+
+static Group g;
+static volatile sig_atomic_t g_watchdog = 0;
+
+const Dim& thread_idx()
+{
+	return g.fibers[g.current].idx;
+}
+
+unsigned int lane()
+{
+	return g.fibers[g.current].idx.x & 63;
+}
+
+static void park(State st, int line)
+{
+	Fiber& f = g.fibers[g.current];
+	f.state = st;
+	f.line = line;
+	f.syncs++;
+	swapcontext(&f.ctx, &g.scheduler);
+}
+
+void block_barrier(int line)
+{
+	g.barrier_count++;
+	if (g.barrier_count == g.threads)
+	{
+		g.barrier_count = 0;
+		for (auto& f : g.fibers)
+			if (f.state == kAtBarrier)
+				f.state = kRunnable;
+		Fiber& me = g.fibers[g.current];
+		me.line = line;
+		me.syncs++;
+		return;
+	}
+	park(kAtBarrier, line);
+}
+
+const uint64_t* wave_sync(uint64_t v, int line)
+{
+	const int w = (int)(g.fibers[g.current].idx.x >> 6);
+	const int members = (g.threads - w * 64) < 64 ? (g.threads - w * 64) : 64;
+	const unsigned long long gen = g.wave_gen[w];
+	uint64_t* buf = &g.wave_buf[((size_t)w * 2 + (gen & 1)) * 64];
+	buf[lane()] = v;
+	g.wave_count[w]++;
+	if (g.wave_count[w] == members)
+	{
+		g.wave_count[w] = 0;
+		g.wave_gen[w]++;
+		for (int l = 0; l < members; ++l)
+		{
+			Fiber& f = g.fibers[w * 64 + l];
+			if (f.state == kAtWave)
+				f.state = kRunnable;
+		}
+		Fiber& me = g.fibers[g.current];
+		me.line = line;
+		me.syncs++;
+	}
+	else
+		park(kAtWave, line);
+	return buf;
+}
+
+struct Task
+{
+	const mhip::BatchArgs* a;
+	const mhip::PatchLayout* L;
+	uint32_t patch;
+	uint8_t* smem;
+	uint32_t* part;
+	uint32_t* sh;
+	uint64_t* box;
+	float* shf;
+};
+
+static Task g_task;
+
+static void fiber_main()
+{
+	mhip::simplify_patch(*g_task.a, *g_task.L, g_task.patch, g_task.smem, g_task.part, g_task.sh, g_task.box, g_task.shf);
+	g.fibers[g.current].state = kDone;
+	swapcontext(&g.fibers[g.current].ctx, &g.scheduler);
+}
+
+static void report(const char* why)
+{
+	fprintf(stderr, "simt: %s (patch %u)\n", why, g_task.patch);
+	for (int t = 0; t < g.threads; ++t)
+	{
+		const Fiber& f = g.fibers[t];
+		if (t < 4 || t % 64 == 0 || f.state != g.fibers[0].state || f.line != g.fibers[0].line)
+			fprintf(stderr, "  thread %3d state %d line %d syncs %llu%s\n", t, (int)f.state, f.line, f.syncs, t == g.current ? " (running)" : "");
+	}
+}
+
+static void on_alarm(int)
+{
+	g_watchdog = 1;
+	report("watchdog: no completion within the time limit");
+	_exit(3);
+}
+
+static void on_segv(int)
+{
+	report("access fault (past the LDS image or elsewhere)");
+	_exit(4);
+}
+
+// the LDS image ends exactly at an inaccessible page, so any access past the layout faults
+struct GuardedImage
+{
+	uint8_t* base = nullptr;
+	size_t mapped = 0;
+	uint8_t* data = nullptr;
+
+	explicit GuardedImage(size_t bytes)
+	{
+		const size_t page = (size_t)sysconf(_SC_PAGESIZE);
+		const size_t body = (bytes + page - 1) / page * page;
+		mapped = body + page;
+		base = (uint8_t*)mmap(nullptr, mapped, PROT_READ | PROT_WRITE, MAP_PRIVATE | MAP_ANONYMOUS, -1, 0);
+		if (base == MAP_FAILED)
+			abort();
+		mprotect(base + body, page, PROT_NONE);
+		data = base + body - bytes;
+	}
+	~GuardedImage()
+	{
+		munmap(base, mapped);
+	}
+};
+
+static uint64_t g_rng = 0;
+
+static uint64_t next_random()
+{
+	g_rng ^= g_rng << 13;
+	g_rng ^= g_rng >> 7;
+	g_rng ^= g_rng << 17;
+	return g_rng;
+}
+
+// runs one workgroup of `threads` fibers over one mesh; returns 0, or -1 on a deadlock.  With
+// shuffle set, every scheduling sweep visits the fibers in a fresh random order (a different
+// interleaving of the threads between rendezvous points).
+static int run_group(int threads, bool shuffle)
+{
+	g.threads = threads;
+	g.fibers.resize(threads);
+	const int waves = (threads + 63) / 64;
+	g.wave_count.assign(waves, 0);
+	g.wave_gen.assign(waves, 0);
+	g.wave_buf.assign((size_t)waves * 2 * 64, 0);
+	g.barrier_count = 0;
+	for (int t = 0; t < threads; ++t)
+	{
+		Fiber& f = g.fibers[t];
+		f.idx = Dim{(unsigned int)t, 0, 0};
+		f.state = kRunnable;
+		f.line = 0;
+		f.syncs = 0;
+		f.stack.resize(256 * 1024);
+		getcontext(&f.ctx);
+		f.ctx.uc_stack.ss_sp = f.stack.data();
+		f.ctx.uc_stack.ss_size = f.stack.size();
+		f.ctx.uc_link = nullptr;
+		makecontext(&f.ctx, fiber_main, 0);
+	}
+	std::vector<int> order(threads);
+	for (int t = 0; t < threads; ++t)
+		order[t] = t;
+	for (;;)
+	{
+		bool progressed = false, all_done = true;
+		if (shuffle)
+			for (int t = threads - 1; t > 0; --t)
+			{
+				int u = (int)(next_random() % (uint64_t)(t + 1));
+				int x = order[t];
+				order[t] = order[u];
+				order[u] = x;
+			}
+		for (int i = 0; i < threads; ++i)
+		{
+			const int t = order[i];
+			Fiber& f = g.fibers[t];
+			if (f.state != kDone)
+				all_done = false;
+			if (f.state != kRunnable)
+				continue;
+			g.current = t;
+			swapcontext(&g.scheduler, &f.ctx);
+			progressed = true;
+		}
+		if (all_done)
+			return 0;
+		if (!progressed)
+		{
+			report("deadlock: no thread can proceed");
+			return -1;
+		}
+	}
+}
+
+} // namespace simt
+
+// Simplifies every mesh of a packed batch with the kernel's device code, mesh by mesh, using the
+// launch's LDS layout for (vmax, imax).  status[p] receives the kernel's status word.
+extern "C" int simt_simplify_batch(unsigned int* destination, unsigned int* result_counts, float* result_errors, unsigned int* status,
+                                   unsigned int* passes, const unsigned int* indices, const unsigned int* index_offsets, const float* positions,
+                                   const unsigned int* vertex_offsets, size_t stride, const unsigned int* targets, size_t count,
+                                   float target_error, unsigned int options, unsigned int vmax, unsigned int imax, int watchdog_s,
+                                   unsigned int mode, unsigned long long seed)
+{
+	// mode bit 0: fill the LDS image with random bytes before every mesh (LDS starts undefined);
+	// mode bit 1: random fiber order in every scheduling sweep
+	simt::g_rng = seed * 0x9E3779B97F4A7C15ull + 1;
+	mhip::BatchArgs a = {};
+	a.destination = destination;
+	a.result_counts = result_counts;
+	a.result_errors = result_errors;
+	a.status = status;
+	a.passes = passes;
+	a.indices = indices;
+	a.index_offsets = index_offsets;
+	a.positions = positions;
+	a.vertex_offsets = vertex_offsets;
+	a.position_stride_floats = stride / sizeof(float);
+	a.target_index_counts = targets;
+	a.patch_count = count;
+	a.target_error = target_error;
+	a.options = options;
+	mhip::PatchLayout L = mhip::patch_layout(vmax, imax);
+
+	simt::GuardedImage image(L.total);
+	uint8_t* smem = image.data;
+	uint32_t part[mhip::kBatchThreads / 64];
+	uint32_t sh[16];
+	uint64_t box[6];
+	float shf[8];
+	signal(SIGSEGV, simt::on_segv);
+	signal(SIGBUS, simt::on_segv);
+	if (watchdog_s > 0)
+	{
+		signal(SIGALRM, simt::on_alarm);
+		alarm((unsigned int)watchdog_s);
+	}
+	int rc = 0;
+	for (size_t p = 0; p < count && rc == 0; ++p)
+	{
+		if (mode & 1)
+		{
+			for (size_t b = 0; b < L.total; ++b)
+				smem[b] = (uint8_t)simt::next_random();
+			for (auto& w : sh)
+				w = (uint32_t)simt::next_random();
+			for (auto& w : part)
+				w = (uint32_t)simt::next_random();
+		}
+		simt::g_task = simt::Task{&a, &L, (uint32_t)p, smem, part, sh, box, shf};
+		rc = simt::run_group(mhip::kBatchThreads, (mode & 2) != 0);
+	}
+	if (watchdog_s > 0)
+		alarm(0);
+	return rc;
+}
+
+extern "C" unsigned int simt_layout_bytes(unsigned int vmax, unsigned int imax)
+{
+	return mhip::patch_layout(vmax, imax).total;
+}

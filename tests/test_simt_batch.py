@@ -1,0 +1,87 @@
+"""The batched kernel's device code (meshoptimizer_amd/csrc/batch_device.h) run on the CPU through
+the SIMT harness in tests/simt: one fiber per GPU thread, rendezvous at every barrier and
+wavefront operation, the LDS image followed by an inaccessible page, LDS filled with random
+bytes before each mesh and a random thread interleaving.  Each mesh must come out exactly as the
+oracle's meshopt_simplify (0 ulp); a deadlock, an access past the LDS layout or a hang fails the
+test.  Runs without a GPU."""
+import ctypes
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from meshes import random_grid, random_soup, same_bits
+from oracle import oracle as O
+import meshoptimizer_amd as M
+from meshoptimizer_amd import workloads as W
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SIMT = os.path.join(HERE, "simt")
+CSRC = os.path.join(os.path.dirname(HERE), "meshoptimizer_amd", "csrc")
+OUT = os.path.join(SIMT, "_build", "libsimt_batch.so")
+
+
+@pytest.fixture(scope="module")
+def simt():
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("g++ not available")
+    os.makedirs(os.path.dirname(OUT), exist_ok=True)
+    cmd = [gxx, "-std=c++17", "-O1", "-g", "-fPIC", "-shared", "-ffp-contract=off", "-fno-fast-math", "-Wno-unknown-pragmas",
+           "-I", os.path.join(SIMT, "shim"), "-I", CSRC, os.path.join(SIMT, "simt_batch.cpp"), "-o", OUT]
+    subprocess.run(cmd, check=True)
+    lib = ctypes.CDLL(OUT)
+    P, SZ = ctypes.c_void_p, ctypes.c_size_t
+    lib.simt_simplify_batch.restype = ctypes.c_int
+    lib.simt_simplify_batch.argtypes = [P, P, P, P, P, P, P, P, P, SZ, P, SZ, ctypes.c_float, ctypes.c_uint, ctypes.c_uint,
+                                        ctypes.c_uint, ctypes.c_int, ctypes.c_uint, ctypes.c_ulonglong]
+    return lib
+
+
+def run_simt(lib, meshes, targets, err, opts, mode=3, seed=1):
+    idx, io, verts, vo, stride = M.pack_meshes(meshes)
+    tg = np.ascontiguousarray(targets, dtype=np.uint32)
+    n = len(meshes)
+    vmax = (int(np.diff(vo).max()) + 31) & ~31
+    imax = (int(np.diff(io).max()) + 95) // 96 * 96
+    dest = np.zeros(max(idx.size, 1), np.uint32)
+    cnt = np.zeros(n, np.uint32)
+    errs = np.zeros(n, np.float32)
+    status = np.full(n, 99, np.uint32)
+    rc = lib.simt_simplify_batch(dest.ctypes.data, cnt.ctypes.data, errs.ctypes.data, status.ctypes.data, None,
+                                 idx.ctypes.data, io.ctypes.data, verts.ctypes.data, vo.ctypes.data, stride, tg.ctypes.data,
+                                 n, float(err), int(opts), vmax, imax, 120, mode, seed)
+    assert rc == 0, "deadlock in the device code (see stderr)"
+    return [(dest[io[p]:io[p] + cnt[p]].copy(), float(errs[p]), int(status[p])) for p in range(n)]
+
+
+def check(lib, meshes, targets, err, opts, **kw):
+    for p, ((ib, v), t, (r, e, st)) in enumerate(zip(meshes, targets, run_simt(lib, meshes, targets, err, opts, **kw))):
+        assert st == 0, f"mesh {p}: status {st}"
+        ro, eo = O.simplify(ib, v, int(t), err, opts)
+        assert r.size == ro.size and np.array_equal(r, ro), f"mesh {p}: indices differ from the oracle"
+        assert same_bits(e, eo), f"mesh {p}: result_error {e!r} != {eo!r}"
+
+
+def test_patches(simt):
+    ms = W.patches(count=24, seed=4)
+    check(simt, [(m.indices, m.vertices) for m in ms], [W.target_count(m.index_count, 1, 2) for m in ms], 1e-2,
+          M.SimplifyLockBorder)
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_random_meshes(simt, seed):
+    rng = np.random.default_rng(9100 + seed)
+    opts = [0, M.SimplifyPermissive, M.SimplifyRegularize | M.SimplifyLockBorder, M.SimplifyErrorAbsolute,
+            M.SimplifyInternalDebug, M.SimplifyRegularizeLight | M.SimplifyPermissive][seed]
+    meshes, targets = [], []
+    for k in range(8):
+        if k % 2:
+            ib, v = random_soup(rng, nv=int(rng.integers(3, 120)), ntri=int(rng.integers(1, 240)))
+        else:
+            ib, v = random_grid(rng, n=int(rng.integers(3, 14)))
+        meshes.append((ib, v))
+        targets.append((ib.size // 3 * int(rng.integers(0, 4)) // 4) * 3)
+    check(simt, meshes, targets, float(rng.choice([1e-3, 1e-2, 1e-1, 1.0])), opts, seed=seed)
