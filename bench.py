@@ -91,6 +91,16 @@ def cpu_reference(m, target, weights, budget_s=12.0):
 # compulsory HBM bytes of one mesh in the batched kernel: its indices read (4 I) and positions
 # read (12 V, xyz), the result written (4 I_out), offsets/target read and count/error/status
 # written (6 x 4)
+def calibrate_copy(torch, dev):
+    """One device copy of 256 MiB: the reference point for the PMC byte counters (its FETCH_SIZE
+    and WRITE_SIZE should read 262144 KiB)."""
+    src = torch.ones(64 << 20, dtype=torch.float32, device=dev)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
+
+
 def batch_alg_bytes(index_counts, vertex_counts, out_counts):
     return int(4 * np.sum(index_counts) + 12 * np.sum(vertex_counts) + 4 * np.sum(out_counts) + 24 * len(index_counts))
 
@@ -128,6 +138,8 @@ def bench_patches(args):
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
+    if args.calibrate_copy:
+        calibrate_copy(torch, dev)
     step()
     torch.cuda.synchronize()
     g_dst = d_dst.cpu().numpy().view(np.uint32)
@@ -263,12 +275,7 @@ def main():
     torch.cuda.synchronize()
 
     if args.calibrate_copy:
-        # a plain copy of known size: FETCH_SIZE / WRITE_SIZE of this dispatch should read 262144 KiB
-        src = torch.ones(64 << 20, dtype=torch.float32, device=dev)
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
+        calibrate_copy(torch, dev)
 
     # bit-exactness of the measured configuration against the CPU reference (untimed)
     n_gpu, e_gpu = step()
