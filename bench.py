@@ -1,7 +1,7 @@
 """Benchmark of the MI355X simplifier: input triangles per second for one plain
 meshopt_simplify call on a device-resident mesh, next to the reference CPU library.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload icosphere8|grid256|seamed1024|patches4096]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload icosphere8|grid256|seamed1024|patches4096|serving]
 
 A step is one whole simplify call (setup, every pass, compaction of the result) on inputs
 already resident in HBM; the timed region is bracketed by device synchronisation.  Prints one
@@ -232,6 +232,130 @@ def bench_patches(args):
     print(json.dumps(out), flush=True)
 
 
+def replay_cpu(reqs, arrivals, threads):
+    """The reference library (oracle/_ref, or the restatement) serving the same arrival trace on a
+    pool of `threads` workers; returns ([(indices, error)], [latency s], makespan s)."""
+    import concurrent.futures as cf
+    import threading
+    from oracle import oracle as O
+    impl = "ref" if O.have_ref() else "oracle"
+    out = [None] * len(reqs)
+    lat = [0.0] * len(reqs)
+    t0 = time.perf_counter()
+
+    def run(k):
+        r = reqs[k]
+        m = r.mesh
+        attrs = m.vertices[:, 3:3 + m.attr_count] if r.weights is not None else None
+        out[k] = O.simplify(m.indices, m.vertices, r.target_index_count, 1e-2, 0, attributes=attrs, weights=r.weights, impl=impl)
+        lat[k] = time.perf_counter() - t0 - arrivals[k]
+
+    futs = []
+    with cf.ThreadPoolExecutor(max_workers=threads) as ex:
+        for k in range(len(reqs)):
+            delay = arrivals[k] - (time.perf_counter() - t0)
+            if delay > 0:
+                time.sleep(delay)
+            futs.append(ex.submit(run, k))
+        for f in futs:
+            f.result()
+    return out, lat, time.perf_counter() - t0 - arrivals[0]
+
+
+def bench_serving(args):
+    """The serving stream: Poisson arrivals of requests of 1k-2M triangles (grids and icospheres,
+    a third with normal+UV attributes, target ratio 0.5/0.25/0.1) served by meshopt_hip_server
+    with `--slots` concurrent device contexts and a 64 GB admission budget.  First the closed-loop
+    capacity (every request queued at once), then an open-loop replay at `--load` of it."""
+    import threading
+    import torch
+    import meshoptimizer_amd as M
+    from meshoptimizer_amd import workloads as W
+
+    count, slots, budget, load = args.requests, args.slots, 64 << 30, args.load
+    reqs = W.request_stream(count, 1.0)
+    unit = np.array([r.arrival_s for r in reqs])  # arrival times at one request per second
+    tris = np.array([r.mesh.triangle_count for r in reqs], dtype=np.float64)
+    total = float(tris.sum())
+    torch.cuda.init()
+
+    srv = M.Server(budget_bytes=budget, slots=slots)
+
+    def serve(arrivals):
+        res = [None] * count
+        tickets = [None] * count
+        done = threading.Event()
+
+        def waiter():
+            for k in range(count):
+                while tickets[k] is None:
+                    time.sleep(0.0002)
+                res[k] = srv.wait(tickets[k])
+            done.set()
+
+        th = threading.Thread(target=waiter)
+        th.start()
+        t0 = time.perf_counter()
+        for k, r in enumerate(reqs):
+            delay = arrivals[k] - (time.perf_counter() - t0)
+            if delay > 0:
+                time.sleep(delay)
+            m = r.mesh
+            attrs = m.vertices[:, 3:3 + m.attr_count] if r.weights is not None else None
+            tickets[k] = srv.submit(m.indices, m.vertices, r.target_index_count, 1e-2, 0, attributes=attrs, weights=r.weights)
+        done.wait()
+        th.join()
+        return res
+
+    serve(np.zeros(count))  # warm-up: every slot's arena grows to its working size
+    closed = serve(np.zeros(count))
+    cap_s = (max(i["finish_ms"] for _, _, i in closed) - min(i["submit_ms"] for _, _, i in closed)) * 1e-3
+    capacity = total / cap_s
+    rate = load * capacity / tris.mean()  # requests per second
+    arrivals = unit / rate
+    opened = serve(arrivals)
+    srv.close()
+    lat_ms = np.array([i["finish_ms"] - i["submit_ms"] for _, _, i in opened])
+    span_s = (max(i["finish_ms"] for _, _, i in opened) - min(i["submit_ms"] for _, _, i in opened)) * 1e-3
+    value = total / span_s / 1e6
+
+    out = {
+        "metric": "input Mtri/s served (bit-exact vs CPU), serving stream",
+        "value": round(value, 3),
+        "unit": "Mtri/s",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": 1,
+        "ms_per_step": round(span_s * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded generator)",
+        "config": {"workload": f"serving stream: {count} requests, log-uniform 1k-2M triangles (grids and icospheres, "
+                               f"1/3 with normal+UV), target ratio 0.5/0.25/0.1, Poisson arrivals at {load:.0%} of the "
+                               f"measured capacity, {slots} slots, 64 GB admission budget",
+                   "requests": count, "input_triangles": int(total), "slots": slots,
+                   "offered_requests_per_s": round(rate, 2)},
+        "p50_ms": round(float(np.percentile(lat_ms, 50)), 3),
+        "p99_ms": round(float(np.percentile(lat_ms, 99)), 3),
+        "closed_loop_capacity": round(capacity / 1e6, 3),
+    }
+    if not args.no_cpu:
+        cores = max(1, min(args.cpu_threads, os.cpu_count() or 1))
+        cpu_res, cpu_lat, cpu_span = replay_cpu(reqs, arrivals, cores)
+        exact = all(np.array_equal(g[0], c[0]) and np.float32(g[1]).tobytes() == np.float32(c[1]).tobytes()
+                    for g, c in zip(opened, cpu_res))
+        out["bit_exact"] = bool(exact)
+        from oracle import oracle as O
+        out["cpu_baseline"] = {"value": round(total / cpu_span / 1e6, 3), "unit": "Mtri/s", "cores": cores,
+                               "kind": "reference" if O.have_ref() else "port",
+                               "sample": f"the same {count}-request arrival trace on {cores} worker threads",
+                               "p50_ms": round(float(np.percentile(np.array(cpu_lat) * 1e3, 50)), 3),
+                               "p99_ms": round(float(np.percentile(np.array(cpu_lat) * 1e3, 99)), 3)}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -240,6 +364,9 @@ def main():
     ap.add_argument("--workload", default="icosphere8")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-sample", type=int, default=256, help="patches run through the plain per-call path")
+    ap.add_argument("--requests", type=int, default=64, help="serving: requests in the stream")
+    ap.add_argument("--slots", type=int, default=4, help="serving: concurrent device contexts")
+    ap.add_argument("--load", type=float, default=0.7, help="serving: offered load as a fraction of capacity")
     ap.add_argument("--cpu-threads", type=int, default=16, help="threads of the all-core CPU baseline (batches)")
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU baseline timing")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -253,6 +380,8 @@ def main():
 
     if args.workload == "patches4096":
         return bench_patches(args)
+    if args.workload == "serving":
+        return bench_serving(args)
 
     import torch
     import meshoptimizer_amd as M

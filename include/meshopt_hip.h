@@ -75,6 +75,52 @@ void meshopt_hip_kernel_timing(int enable, double* total_ms, size_t* launches, s
  * the previous call; resets the totals. */
 void meshopt_hip_batch_kernel_timing(double* total_ms, size_t* launches, size_t* meshes);
 
+/* Serving.  A server runs simplify requests of any size as they arrive, on `slots` concurrent
+ * device contexts (each its own HIP stream, arena and staging buffer), admitting them in arrival
+ * order against a device-memory budget: a request starts once a slot is idle and the slot's
+ * memory, next to what the other slots hold, fits the budget.  Each result is exactly what
+ * meshopt_simplify / meshopt_simplifyWithAttributes (src/meshoptimizer.h:513, 537) returns for
+ * the request alone.  This interface has no counterpart in the reference. */
+typedef struct meshopt_hip_server meshopt_hip_server;
+
+typedef struct meshopt_hip_request
+{
+	unsigned int* destination; /* host array with room for index_count indices */
+	const unsigned int* indices;
+	size_t index_count;
+	const float* vertex_positions;
+	size_t vertex_count;
+	size_t vertex_positions_stride;
+	const float* vertex_attributes; /* NULL, or as in meshopt_simplifyWithAttributes */
+	size_t vertex_attributes_stride;
+	const float* attribute_weights; /* copied at submit */
+	size_t attribute_count;
+	const unsigned char* vertex_lock; /* may be NULL */
+	size_t target_index_count;
+	float target_error;
+	unsigned int options;
+} meshopt_hip_request;
+
+typedef struct meshopt_hip_response
+{
+	size_t index_count; /* indices written to the request's destination; (size_t)-1 on failure */
+	float result_error;
+	int slot;
+	double submit_ms, start_ms, finish_ms; /* server clock (meshopt_hip_server_clock_ms) */
+	size_t device_bytes;                   /* the admission estimate of the request */
+} meshopt_hip_response;
+
+/* Returns NULL on failure (meshopt_hip_last_error). */
+meshopt_hip_server* meshopt_hip_server_create(size_t memory_budget_bytes, int slots);
+/* Queues a request; its arrays must stay valid until the ticket is waited on.  Returns the
+ * ticket, or 0 if the request is invalid or needs more than the whole budget. */
+size_t meshopt_hip_server_submit(meshopt_hip_server* server, const meshopt_hip_request* request);
+/* Blocks until the ticket's request has finished; fills *response.  Returns 0, or -1 on failure. */
+int meshopt_hip_server_wait(meshopt_hip_server* server, size_t ticket, meshopt_hip_response* response);
+double meshopt_hip_server_clock_ms(const meshopt_hip_server* server);
+/* Finishes every queued request, then frees the server's contexts. */
+void meshopt_hip_server_destroy(meshopt_hip_server* server);
+
 /* Text of the last HIP error seen by the calling thread (empty string if none). */
 const char* meshopt_hip_last_error(void);
 

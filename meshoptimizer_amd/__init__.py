@@ -51,6 +51,11 @@ EXPORTED_SYMBOLS = (
     "meshopt_hip_simplify_batch",
     "meshopt_hip_simplify_batch_device",
     "meshopt_hip_batch_kernel_timing",
+    "meshopt_hip_server_create",
+    "meshopt_hip_server_submit",
+    "meshopt_hip_server_wait",
+    "meshopt_hip_server_clock_ms",
+    "meshopt_hip_server_destroy",
     "meshopt_hip_last_call_stats",
     "meshopt_hip_last_error",
     "meshopt_hip_kernel_timing",
@@ -112,6 +117,16 @@ def library() -> ctypes.CDLL:
     lib.meshopt_hip_kernel_timing.argtypes = [ctypes.c_int, _P, _P, _P]
     lib.meshopt_hip_release.restype = None
     lib.meshopt_hip_release.argtypes = []
+    lib.meshopt_hip_server_create.restype = _P
+    lib.meshopt_hip_server_create.argtypes = [_SZ, ctypes.c_int]
+    lib.meshopt_hip_server_submit.restype = _SZ
+    lib.meshopt_hip_server_submit.argtypes = [_P, _P]
+    lib.meshopt_hip_server_wait.restype = ctypes.c_int
+    lib.meshopt_hip_server_wait.argtypes = [_P, _SZ, _P]
+    lib.meshopt_hip_server_clock_ms.restype = ctypes.c_double
+    lib.meshopt_hip_server_clock_ms.argtypes = [_P]
+    lib.meshopt_hip_server_destroy.restype = None
+    lib.meshopt_hip_server_destroy.argtypes = [_P]
     _lib = lib
     return lib
 
@@ -122,6 +137,9 @@ def _rows(a, name):
         raise TypeError(f"{name} must be float32")
     if a.ndim == 1:
         a = a.reshape(-1, 3)
+    if a.shape[0] == 0:
+        # numpy gives empty arrays zero strides; keep a packed row stride for the C side
+        return np.zeros((1, a.shape[1]), dtype=np.float32)[:0]
     if a.shape[1] and (a.strides[1] != 4 or a.strides[0] % 4 or a.strides[0] < 4 * a.shape[1]):
         raise ValueError(f"{name} rows must be packed float32 with a row stride >= their width")
     return a
@@ -266,3 +284,85 @@ def last_call_stats():
     cd = {f: getattr(call, f) for f, _ in CallStats._fields_}
     cd["kinds"] = list(call.kinds)
     return cd, [{f: getattr(passes[i], f) for f, _ in PassStats._fields_} for i in range(min(n, 64))]
+
+
+class ServerRequest(ctypes.Structure):
+    """meshopt_hip_request."""
+    _fields_ = [("destination", _P), ("indices", _P), ("index_count", _SZ), ("vertex_positions", _P),
+                ("vertex_count", _SZ), ("vertex_positions_stride", _SZ), ("vertex_attributes", _P),
+                ("vertex_attributes_stride", _SZ), ("attribute_weights", _P), ("attribute_count", _SZ),
+                ("vertex_lock", _P), ("target_index_count", _SZ), ("target_error", _F), ("options", _U)]
+
+
+class ServerResponse(ctypes.Structure):
+    """meshopt_hip_response."""
+    _fields_ = [("index_count", _SZ), ("result_error", _F), ("slot", ctypes.c_int), ("submit_ms", ctypes.c_double),
+                ("start_ms", ctypes.c_double), ("finish_ms", ctypes.c_double), ("device_bytes", _SZ)]
+
+
+class Server:
+    """A simplify server (meshopt_hip_server_*): requests run concurrently on `slots` device
+    contexts and are admitted in arrival order against `budget_bytes` of device memory.  Each
+    result equals simplify / simplify_with_attributes on the request alone."""
+
+    def __init__(self, budget_bytes=64 << 30, slots=4):
+        self._lib = library()
+        self._h = self._lib.meshopt_hip_server_create(int(budget_bytes), int(slots))
+        if not self._h:
+            raise RuntimeError(self._lib.meshopt_hip_last_error().decode())
+        self._live = {}
+
+    def submit(self, indices, positions, target_index_count, target_error=1e-2, options=0, attributes=None,
+               weights=None, lock=None):
+        """Queues one request; returns its ticket.  The arrays are kept alive until wait()."""
+        indices = np.ascontiguousarray(indices, dtype=np.uint32)
+        positions = _rows(positions, "positions")
+        aptr, astride, acount = None, 0, 0
+        if attributes is not None and np.asarray(attributes).size:
+            attributes = _rows(attributes, "attributes")
+            aptr, astride, acount = attributes.ctypes.data, attributes.strides[0], attributes.shape[1]
+        w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float32)
+        lk = None if lock is None else np.ascontiguousarray(lock, dtype=np.uint8)
+        dest = np.zeros(max(indices.size, 1), dtype=np.uint32)
+        req = ServerRequest(dest.ctypes.data, indices.ctypes.data, indices.size, positions.ctypes.data,
+                            positions.shape[0], positions.strides[0], aptr, astride,
+                            None if w is None else w.ctypes.data, acount if w is not None else 0,
+                            None if lk is None else lk.ctypes.data, int(target_index_count), float(target_error),
+                            int(options))
+        ticket = self._lib.meshopt_hip_server_submit(self._h, ctypes.byref(req))
+        if ticket == 0:
+            raise RuntimeError(self._lib.meshopt_hip_last_error().decode())
+        self._live[ticket] = (dest, indices, positions, attributes, w, lk)
+        return ticket
+
+    def wait(self, ticket):
+        """Blocks until the request is done; returns (indices, result_error, info)."""
+        resp = ServerResponse()
+        rc = self._lib.meshopt_hip_server_wait(self._h, ticket, ctypes.byref(resp))
+        dest = self._live.pop(ticket)[0]
+        if rc != 0:
+            raise RuntimeError(self._lib.meshopt_hip_last_error().decode())
+        info = {"slot": resp.slot, "submit_ms": resp.submit_ms, "start_ms": resp.start_ms,
+                "finish_ms": resp.finish_ms, "device_bytes": resp.device_bytes}
+        return dest[:resp.index_count].copy(), resp.result_error, info
+
+    def clock_ms(self):
+        return self._lib.meshopt_hip_server_clock_ms(self._h)
+
+    def close(self):
+        if self._h:
+            self._lib.meshopt_hip_server_destroy(self._h)
+            self._h = None
+            self._live.clear()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

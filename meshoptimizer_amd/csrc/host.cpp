@@ -11,9 +11,15 @@
 #include <string.h>
 
 #include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <memory>
 #include <mutex>
 #include <stdexcept>
 #include <string>
+#include <thread>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/meshopt_hip.h"
@@ -61,6 +67,33 @@ struct PassRecord
 	meshopt_hip_pass_stats s;
 };
 
+// device buffer for the host entry points' input/output staging (grows; released explicitly)
+struct StagingCache
+{
+	void* buf = nullptr;
+	size_t size = 0;
+	void* get(size_t bytes)
+	{
+		if (bytes > size)
+		{
+			if (buf)
+				HIP_CHECK(hipFree(buf));
+			buf = nullptr;
+			size = 0;
+			HIP_CHECK(hipMalloc(&buf, bytes + bytes / 4 + 256));
+			size = bytes + bytes / 4 + 256;
+		}
+		return buf;
+	}
+	void release()
+	{
+		if (buf)
+			(void)hipFree(buf);
+		buf = nullptr;
+		size = 0;
+	}
+};
+
 struct Context
 {
 	std::mutex mutex;
@@ -68,6 +101,7 @@ struct Context
 	hipStream_t stream = nullptr;
 	char* arena = nullptr;
 	size_t arena_size = 0;
+	StagingCache staging;
 	CallScalars* sc_host = nullptr; // pinned mirror of the device scalar block
 	std::vector<meshopt_hip_pass_stats> passes;
 	meshopt_hip_call_stats call{};
@@ -104,8 +138,24 @@ struct Context
 		arena_size = grow;
 	}
 
+	// device memory held between calls (arena and staging)
+	size_t held_bytes() const
+	{
+		return arena_size + staging.size;
+	}
+
+	void release_memory()
+	{
+		if (arena)
+			(void)hipFree(arena);
+		arena = nullptr;
+		arena_size = 0;
+		staging.release();
+	}
+
 	void release()
 	{
+		staging.release();
 		if (arena)
 			(void)hipFree(arena);
 		if (sc_host)
@@ -601,67 +651,60 @@ struct DeviceStaging
 	}
 };
 
-struct StagingCache
+
+// Byte offsets of the staged streams of one host call inside the context's staging buffer.
+struct HostStaging
 {
-	void* buf = nullptr;
-	size_t size = 0;
-	void* get(size_t bytes)
-	{
-		if (bytes > size)
-		{
-			if (buf)
-				HIP_CHECK(hipFree(buf));
-			buf = nullptr;
-			size = 0;
-			HIP_CHECK(hipMalloc(&buf, bytes + bytes / 4 + 256));
-			size = bytes + bytes / 4 + 256;
-		}
-		return buf;
-	}
+	bool attr_inside; // attributes interleaved in the position stream: staged with it
+	size_t pos_bytes, attr_span, lock_bytes;
+	size_t off_idx, off_dst, off_pos, off_attr, off_lock, total;
 };
 
-static StagingCache& staging()
+static HostStaging plan_staging(size_t index_count, const float* positions, size_t vertex_count, size_t position_stride, const float* attributes,
+                                size_t attribute_stride, size_t attribute_count, const unsigned char* lock)
 {
-	static StagingCache s;
-	return s;
+	HostStaging h;
+	size_t pos_span = vertex_count ? (vertex_count - 1) * position_stride + 12 : 0;
+	const char* pos_lo = reinterpret_cast<const char*>(positions);
+	h.attr_inside = attributes && attribute_count && reinterpret_cast<const char*>(attributes) >= pos_lo &&
+	                reinterpret_cast<const char*>(attributes) + (vertex_count ? (vertex_count - 1) * attribute_stride + attribute_count * 4 : 0) <=
+	                    pos_lo + (vertex_count ? (vertex_count - 1) * position_stride + position_stride : 0);
+	h.pos_bytes = h.attr_inside ? vertex_count * position_stride : pos_span;
+	h.attr_span = (attributes && attribute_count && vertex_count && !h.attr_inside) ? (vertex_count - 1) * attribute_stride + attribute_count * 4 : 0;
+	h.lock_bytes = lock ? vertex_count : 0;
+	h.off_idx = 0;
+	h.off_dst = (h.off_idx + index_count * 4 + 255) & ~size_t(255);
+	h.off_pos = (h.off_dst + index_count * 4 + 255) & ~size_t(255);
+	h.off_attr = (h.off_pos + h.pos_bytes + 255) & ~size_t(255);
+	h.off_lock = (h.off_attr + h.attr_span + 255) & ~size_t(255);
+	h.total = h.off_lock + h.lock_bytes + 256;
+	return h;
 }
 
-static size_t simplify_host(const unsigned int* indices, unsigned int* destination, size_t index_count, const float* positions, size_t vertex_count,
-                            size_t position_stride, const float* attributes, size_t attribute_stride, const float* weights, size_t attribute_count,
-                            const unsigned char* lock, size_t target_index_count, float target_error, unsigned int options, float* result_error)
+// One host call on the given context (the caller holds whatever lock guards it): stage the used
+// spans of the inputs, run the device path, copy the result back.
+static size_t simplify_host_on(Context& ctx, const unsigned int* indices, unsigned int* destination, size_t index_count, const float* positions,
+                               size_t vertex_count, size_t position_stride, const float* attributes, size_t attribute_stride, const float* weights,
+                               size_t attribute_count, const unsigned char* lock, size_t target_index_count, float target_error, unsigned int options,
+                               float* result_error)
 {
-	Context& ctx = context();
-	std::lock_guard<std::mutex> guard(ctx.mutex);
 	ctx.ensure_init();
 	hipStream_t s = ctx.stream;
 
-	size_t pos_span = vertex_count ? (vertex_count - 1) * position_stride + 12 : 0;
-	const char* pos_lo = reinterpret_cast<const char*>(positions);
-	bool attr_inside = attributes && attribute_count && reinterpret_cast<const char*>(attributes) >= pos_lo &&
-	                   reinterpret_cast<const char*>(attributes) + (vertex_count ? (vertex_count - 1) * attribute_stride + attribute_count * 4 : 0) <=
-	                       pos_lo + (vertex_count ? (vertex_count - 1) * position_stride + position_stride : 0);
-	size_t pos_bytes = attr_inside ? vertex_count * position_stride : pos_span;
-	size_t attr_span = (attributes && attribute_count && vertex_count && !attr_inside) ? (vertex_count - 1) * attribute_stride + attribute_count * 4 : 0;
-	size_t lock_bytes = lock ? vertex_count : 0;
-
-	size_t off_idx = 0;
-	size_t off_dst = (off_idx + index_count * 4 + 255) & ~size_t(255);
-	size_t off_pos = (off_dst + index_count * 4 + 255) & ~size_t(255);
-	size_t off_attr = (off_pos + pos_bytes + 255) & ~size_t(255);
-	size_t off_lock = (off_attr + attr_span + 255) & ~size_t(255);
-	size_t total = off_lock + lock_bytes + 256;
-
-	char* d = static_cast<char*>(staging().get(total));
+	const HostStaging h = plan_staging(index_count, positions, vertex_count, position_stride, attributes, attribute_stride, attribute_count, lock);
+	const size_t off_idx = h.off_idx, off_dst = h.off_dst, off_pos = h.off_pos, off_attr = h.off_attr, off_lock = h.off_lock;
+	const size_t lock_bytes = h.lock_bytes;
+	char* d = static_cast<char*>(ctx.staging.get(h.total));
 	if (index_count)
 		HIP_CHECK(hipMemcpyAsync(d + off_idx, indices, index_count * 4, hipMemcpyHostToDevice, s));
-	if (pos_bytes)
-		HIP_CHECK(hipMemcpyAsync(d + off_pos, positions, pos_bytes, hipMemcpyHostToDevice, s));
+	if (h.pos_bytes)
+		HIP_CHECK(hipMemcpyAsync(d + off_pos, positions, h.pos_bytes, hipMemcpyHostToDevice, s));
 	const float* d_attr = nullptr;
-	if (attr_inside)
-		d_attr = reinterpret_cast<const float*>(d + off_pos + (reinterpret_cast<const char*>(attributes) - pos_lo));
-	else if (attr_span)
+	if (h.attr_inside)
+		d_attr = reinterpret_cast<const float*>(d + off_pos + (reinterpret_cast<const char*>(attributes) - reinterpret_cast<const char*>(positions)));
+	else if (h.attr_span)
 	{
-		HIP_CHECK(hipMemcpyAsync(d + off_attr, attributes, attr_span, hipMemcpyHostToDevice, s));
+		HIP_CHECK(hipMemcpyAsync(d + off_attr, attributes, h.attr_span, hipMemcpyHostToDevice, s));
 		d_attr = reinterpret_cast<const float*>(d + off_attr);
 	}
 	if (lock_bytes)
@@ -688,6 +731,16 @@ static size_t simplify_host(const unsigned int* indices, unsigned int* destinati
 		HIP_CHECK(hipMemcpyAsync(destination, c.destination, n * 4, hipMemcpyDeviceToHost, s));
 	HIP_CHECK(hipStreamSynchronize(s));
 	return n;
+}
+
+static size_t simplify_host(const unsigned int* indices, unsigned int* destination, size_t index_count, const float* positions, size_t vertex_count,
+                            size_t position_stride, const float* attributes, size_t attribute_stride, const float* weights, size_t attribute_count,
+                            const unsigned char* lock, size_t target_index_count, float target_error, unsigned int options, float* result_error)
+{
+	Context& ctx = context();
+	std::lock_guard<std::mutex> guard(ctx.mutex);
+	return simplify_host_on(ctx, indices, destination, index_count, positions, vertex_count, position_stride, attributes, attribute_stride, weights,
+	                        attribute_count, lock, target_index_count, target_error, options, result_error);
 }
 
 static void check_args(size_t index_count, size_t stride, size_t target, float target_error, size_t attr_stride, size_t attr_count, const float* weights)
@@ -971,7 +1024,7 @@ extern "C" int meshopt_hip_simplify_batch(unsigned int* destination, unsigned in
 		size_t off_rc = off_tg + ((n1 * 4 + 255) & ~size_t(255));
 		size_t off_re = off_rc + ((n1 * 4 + 255) & ~size_t(255));
 		size_t off_pos = off_re + ((n1 * 4 + 255) & ~size_t(255));
-		char* d = static_cast<char*>(staging().get(off_pos + pos_bytes + 256));
+		char* d = static_cast<char*>(ctx.staging.get(off_pos + pos_bytes + 256));
 		if (I)
 			HIP_CHECK(hipMemcpyAsync(d + off_idx, indices, I * 4, hipMemcpyHostToDevice, s));
 		HIP_CHECK(hipMemcpyAsync(d + off_io, index_offsets, n1 * 4, hipMemcpyHostToDevice, s));
@@ -1016,7 +1069,7 @@ extern "C" float meshopt_simplifyScale(const float* vertex_positions, size_t ver
 		size_t span = vertex_count ? (vertex_count - 1) * vertex_positions_stride + 12 : 0;
 		size_t off_p = (span + 255) & ~size_t(255);
 		size_t off_sc = (off_p + (vertex_count + 1) * sizeof(float4) + 255) & ~size_t(255);
-		char* d = static_cast<char*>(staging().get(off_sc + sizeof(CallScalars) + 256));
+		char* d = static_cast<char*>(ctx.staging.get(off_sc + sizeof(CallScalars) + 256));
 		if (span)
 			HIP_CHECK(hipMemcpyAsync(d, vertex_positions, span, hipMemcpyHostToDevice, s));
 		Mesh m{};
@@ -1092,9 +1145,284 @@ extern "C" void meshopt_hip_release(void)
 	Context& ctx = context();
 	std::lock_guard<std::mutex> guard(ctx.mutex);
 	ctx.release();
-	StagingCache& st = staging();
-	if (st.buf)
-		(void)hipFree(st.buf);
-	st.buf = nullptr;
-	st.size = 0;
+}
+
+// ------------------------------------------------------------------------------------------
+// serving: a request loop over several contexts (one HIP stream, arena and staging buffer each)
+
+namespace mhip
+{
+
+// device bytes a context holds for one host call of this shape (arena with its growth slack,
+// staging buffer with its slack)
+static size_t request_device_bytes(const meshopt_hip_request& r)
+{
+	size_t A = 0;
+	for (size_t k = 0; k < r.attribute_count && k < (size_t)kMaxAttributes; ++k)
+		A += r.attribute_weights[k] > 0;
+	const bool sparse = (r.options & kOptSparse) != 0;
+	Arena dry;
+	Plan plan;
+	carve(dry, plan, r.vertex_count, sparse ? std::min(r.vertex_count, r.index_count) : r.vertex_count, r.index_count, A, sparse);
+	size_t arena = dry.used + 4096;
+	arena += arena / 4;
+	const HostStaging h = plan_staging(r.index_count, r.vertex_positions, r.vertex_count, r.vertex_positions_stride, r.vertex_attributes,
+	                                   r.vertex_attributes_stride, r.attribute_count, r.vertex_lock);
+	return arena + h.total + h.total / 4 + 256;
+}
+
+struct ServerTicket
+{
+	meshopt_hip_request req{};
+	std::vector<float> weights; // the request's weights, copied at submit
+	size_t need = 0;
+	double submit_ms = 0, start_ms = 0, finish_ms = 0;
+	size_t result = 0;
+	float error = 0;
+	int slot = -1;
+	bool done = false;
+	std::string failure;
+};
+
+// Requests are admitted in arrival order (FIFO, no overtaking) to the first idle slot whose
+// memory, next to what the other slots hold, fits the budget.  A slot keeps its arena between
+// requests unless it holds more than budget / slots; idle slots give theirs up when the head of
+// the queue needs the room.  Slots run concurrently, each on its own HIP stream, so the device
+// overlaps their work.
+struct Server
+{
+	size_t budget = 0;
+	int slots = 0;
+	int device = 0;
+	std::vector<std::unique_ptr<Context>> ctxs;
+	std::vector<size_t> held;
+	std::vector<char> busy;
+	std::vector<std::thread> workers;
+	std::mutex m;
+	std::condition_variable cv_work, cv_done;
+	std::deque<uint64_t> queue;
+	std::unordered_map<uint64_t, std::unique_ptr<ServerTicket>> tickets;
+	uint64_t next_id = 1;
+	bool stopping = false;
+	std::chrono::steady_clock::time_point epoch = std::chrono::steady_clock::now();
+
+	double now_ms() const
+	{
+		return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - epoch).count();
+	}
+
+	// the queue head fits on slot s (caller holds m)
+	bool head_fits(int s) const
+	{
+		const ServerTicket& t = *tickets.at(queue.front());
+		size_t others = 0;
+		for (int k = 0; k < slots; ++k)
+			if (k != s)
+				others += held[k];
+		return others + std::max(held[s], t.need) <= budget;
+	}
+
+	void worker(int s)
+	{
+		(void)hipSetDevice(device);
+		Context& ctx = *ctxs[s];
+		std::unique_lock<std::mutex> lk(m);
+		for (;;)
+		{
+			cv_work.wait(lk, [&] { return stopping || !queue.empty(); });
+			if (queue.empty())
+			{
+				if (stopping)
+					break;
+				continue;
+			}
+			if (!head_fits(s))
+			{
+				// make room from idle slots (this one included), then wait for running ones
+				for (int k = 0; k < slots; ++k)
+					if (!busy[k] && held[k])
+					{
+						ctxs[k]->release_memory();
+						held[k] = 0;
+					}
+				if (!head_fits(s))
+				{
+					cv_work.wait(lk);
+					continue;
+				}
+			}
+			const uint64_t id = queue.front();
+			queue.pop_front();
+			ServerTicket& t = *tickets[id];
+			busy[s] = 1;
+			held[s] = std::max(held[s], t.need);
+			t.start_ms = now_ms();
+			t.slot = s;
+			lk.unlock();
+
+			size_t n = 0;
+			float err = 0;
+			std::string failure;
+			try
+			{
+				const meshopt_hip_request& r = t.req;
+				n = simplify_host_on(ctx, r.indices, r.destination, r.index_count, r.vertex_positions, r.vertex_count, r.vertex_positions_stride,
+				                     r.attribute_count ? r.vertex_attributes : nullptr, r.vertex_attributes_stride,
+				                     t.weights.empty() ? nullptr : t.weights.data(), r.attribute_count, r.vertex_lock, r.target_index_count,
+				                     r.target_error, r.options, &err);
+			}
+			catch (const std::exception& e)
+			{
+				failure = e.what();
+			}
+			size_t now_held = ctx.held_bytes();
+			if (now_held > budget / (size_t)slots)
+			{
+				ctx.release_memory();
+				now_held = 0;
+			}
+
+			lk.lock();
+			held[s] = now_held;
+			busy[s] = 0;
+			t.result = n;
+			t.error = err;
+			t.failure = failure;
+			t.finish_ms = now_ms();
+			t.done = true;
+			cv_done.notify_all();
+			cv_work.notify_all();
+		}
+	}
+};
+
+} // namespace mhip
+
+extern "C" meshopt_hip_server* meshopt_hip_server_create(size_t memory_budget_bytes, int slots)
+{
+	try
+	{
+		if (slots < 1 || slots > 64 || memory_budget_bytes == 0)
+			throw std::invalid_argument("meshopt_hip_server_create: slots must be in [1, 64] and the budget positive");
+		std::unique_ptr<Server> srv(new Server());
+		srv->budget = memory_budget_bytes;
+		srv->slots = slots;
+		HIP_CHECK(hipGetDevice(&srv->device));
+		for (int s = 0; s < slots; ++s)
+		{
+			srv->ctxs.emplace_back(new Context());
+			srv->ctxs.back()->ensure_init();
+		}
+		srv->held.assign(slots, 0);
+		srv->busy.assign(slots, 0);
+		for (int s = 0; s < slots; ++s)
+			srv->workers.emplace_back(&Server::worker, srv.get(), s);
+		t_last_error.clear();
+		return reinterpret_cast<meshopt_hip_server*>(srv.release());
+	}
+	catch (const std::exception& e)
+	{
+		t_last_error = e.what();
+		return nullptr;
+	}
+}
+
+extern "C" size_t meshopt_hip_server_submit(meshopt_hip_server* server, const meshopt_hip_request* request)
+{
+	Server* srv = reinterpret_cast<Server*>(server);
+	try
+	{
+		if (!srv || !request)
+			throw std::invalid_argument("meshopt_hip_server_submit: null server or request");
+		const meshopt_hip_request& r = *request;
+		check_args(r.index_count, r.vertex_positions_stride, r.target_index_count, r.target_error, r.vertex_attributes_stride, r.attribute_count,
+		           r.attribute_weights);
+		std::unique_ptr<ServerTicket> t(new ServerTicket());
+		t->req = r;
+		if (r.attribute_count)
+			t->weights.assign(r.attribute_weights, r.attribute_weights + r.attribute_count);
+		t->req.attribute_weights = t->weights.empty() ? nullptr : t->weights.data();
+		t->need = request_device_bytes(t->req);
+		if (t->need > srv->budget)
+			throw std::invalid_argument("meshopt_hip_server_submit: the request needs more device memory than the server's budget");
+		std::lock_guard<std::mutex> lk(srv->m);
+		if (srv->stopping)
+			throw std::runtime_error("meshopt_hip_server_submit: the server is shutting down");
+		const uint64_t id = srv->next_id++;
+		t->submit_ms = srv->now_ms();
+		srv->tickets[id] = std::move(t);
+		srv->queue.push_back(id);
+		srv->cv_work.notify_all();
+		t_last_error.clear();
+		return id;
+	}
+	catch (const std::exception& e)
+	{
+		t_last_error = e.what();
+		return 0;
+	}
+}
+
+extern "C" int meshopt_hip_server_wait(meshopt_hip_server* server, size_t ticket, meshopt_hip_response* response)
+{
+	Server* srv = reinterpret_cast<Server*>(server);
+	if (!srv)
+	{
+		t_last_error = "meshopt_hip_server_wait: null server";
+		return -1;
+	}
+	std::unique_lock<std::mutex> lk(srv->m);
+	auto it = srv->tickets.find(ticket);
+	if (it == srv->tickets.end())
+	{
+		t_last_error = "meshopt_hip_server_wait: unknown ticket";
+		return -1;
+	}
+	// the ticket object keeps its address; the map may rehash while this thread waits
+	ServerTicket* tp = it->second.get();
+	srv->cv_done.wait(lk, [&] { return tp->done; });
+	it = srv->tickets.find(ticket);
+	std::unique_ptr<ServerTicket> t = std::move(it->second);
+	srv->tickets.erase(it);
+	lk.unlock();
+	if (response)
+	{
+		response->index_count = t->failure.empty() ? t->result : (size_t)-1;
+		response->result_error = t->error;
+		response->slot = t->slot;
+		response->submit_ms = t->submit_ms;
+		response->start_ms = t->start_ms;
+		response->finish_ms = t->finish_ms;
+		response->device_bytes = t->need;
+	}
+	if (!t->failure.empty())
+	{
+		t_last_error = t->failure;
+		return -1;
+	}
+	t_last_error.clear();
+	return 0;
+}
+
+extern "C" double meshopt_hip_server_clock_ms(const meshopt_hip_server* server)
+{
+	const Server* srv = reinterpret_cast<const Server*>(server);
+	return srv ? srv->now_ms() : 0.0;
+}
+
+extern "C" void meshopt_hip_server_destroy(meshopt_hip_server* server)
+{
+	Server* srv = reinterpret_cast<Server*>(server);
+	if (!srv)
+		return;
+	{
+		std::lock_guard<std::mutex> lk(srv->m);
+		srv->stopping = true;
+		srv->cv_work.notify_all();
+	}
+	for (auto& w : srv->workers)
+		w.join();
+	for (auto& c : srv->ctxs)
+		c->release();
+	delete srv;
 }

@@ -1,0 +1,118 @@
+"""GPU tests of the serving loop (meshopt_hip_server_*): each request's result in a concurrent
+mix equals the same request run alone (meshopt_simplify / meshopt_simplifyWithAttributes) and the
+oracle, bit for bit; admission honours the memory budget; invalid requests are rejected."""
+import sys
+
+import numpy as np
+import pytest
+
+from meshes import random_grid, random_soup, same_bits
+from oracle import oracle as O
+import meshoptimizer_amd as M
+from meshoptimizer_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def announce(what):
+    print(f"[case] {what}", flush=True)
+    sys.stdout.flush()
+
+
+def mixed_requests(seed=3):
+    """Requests of very different sizes and kinds: grids, icospheres, seamed grids with
+    attributes, random soups with vertex locks."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(14):
+        kind = k % 5
+        if kind == 0:
+            m = W.grid(int(rng.integers(20, 160)), seed=int(rng.integers(1, 1000)))
+            out.append(dict(ib=m.indices, v=m.vertices, target=W.target_count(m.index_count, 1, 4), err=1e-2, opts=0))
+        elif kind == 1:
+            m = W.icosphere(int(rng.integers(2, 7)), seed=int(rng.integers(1, 1000)))
+            out.append(dict(ib=m.indices, v=m.vertices, target=W.target_count(m.index_count, 1, 10), err=1e-2, opts=0))
+        elif kind == 2:
+            m = W.seamed_grid(int(rng.integers(16, 96)), seams=2, seed=int(rng.integers(1, 1000)))
+            out.append(dict(ib=m.indices, v=m.vertices, target=W.target_count(m.index_count, 1, 5), err=1e-2, opts=0,
+                            attrs=m.vertices[:, 3:8], weights=W.SEAMED_GRID_WEIGHTS))
+        elif kind == 3:
+            ib, v = random_soup(rng, nv=int(rng.integers(50, 400)), ntri=int(rng.integers(50, 800)))
+            lock = rng.integers(0, 8, size=v.shape[0]).astype(np.uint8)
+            out.append(dict(ib=ib, v=v, target=ib.size // 2 // 3 * 3, err=1e-1, opts=M.SimplifyPermissive,
+                            attrs=v[:, 3:4], weights=np.array([0.5], np.float32), lock=lock))
+        else:
+            ib, v = random_grid(rng, n=int(rng.integers(6, 30)))
+            out.append(dict(ib=ib, v=v, target=0, err=float(rng.choice([1e-3, 1e-2, 1e-1])),
+                            opts=int(rng.choice([0, M.SimplifyLockBorder, M.SimplifyRegularize, M.SimplifyErrorAbsolute]))))
+    return out
+
+
+def alone(r):
+    if "attrs" in r or "lock" in r:
+        return M.simplify_with_attributes(r["ib"], r["v"], r.get("attrs"), r.get("weights"), r.get("lock"), r["target"],
+                                          r["err"], r["opts"])
+    return M.simplify(r["ib"], r["v"], r["target"], r["err"], r["opts"])
+
+
+def submit(srv, r):
+    return srv.submit(r["ib"], r["v"], r["target"], r["err"], r["opts"], attributes=r.get("attrs"), weights=r.get("weights"),
+                      lock=r.get("lock"))
+
+
+def test_mix_equals_alone_and_oracle():
+    reqs = mixed_requests()
+    announce(f"serving mix of {len(reqs)} requests on 4 slots")
+    with M.Server(budget_bytes=8 << 30, slots=4) as srv:
+        tickets = [submit(srv, r) for r in reqs]
+        results = [srv.wait(t) for t in tickets]
+    for k, (r, (ri, re, info)) in enumerate(zip(reqs, results)):
+        ai, ae = alone(r)
+        assert np.array_equal(ri, ai) and same_bits(re, ae), f"request {k}: mix differs from the request alone"
+        oi, oe = O.simplify(r["ib"], r["v"], r["target"], r["err"], r["opts"], attributes=r.get("attrs"),
+                            weights=r.get("weights"), lock=r.get("lock"))
+        assert np.array_equal(ri, oi) and same_bits(re, oe), f"request {k}: differs from the oracle"
+    # the slots ran side by side: some request started before another one on a different slot finished
+    spans = [(i["start_ms"], i["finish_ms"], i["slot"]) for _, _, i in results]
+    overlap = any(a[0] < b[1] and b[0] < a[1] and a[2] != b[2] for x, a in enumerate(spans) for b in spans[x + 1:])
+    assert overlap, "no two requests overlapped in time"
+    assert len({s for _, _, s in spans}) > 1
+
+
+def test_budget_serialises_admission():
+    m = W.icosphere(5, seed=7)
+    target = W.target_count(m.index_count, 1, 4)
+    announce("budget admission")
+    with M.Server(budget_bytes=8 << 30, slots=1) as srv:
+        _, _, info = srv.wait(srv.submit(m.indices, m.vertices, target))
+    need = info["device_bytes"]
+    expect = M.simplify(m.indices, m.vertices, target)
+    # room for one request at a time: three slots, but the requests must run one after another
+    with M.Server(budget_bytes=need * 3 // 2, slots=3) as srv:
+        tickets = [srv.submit(m.indices, m.vertices, target) for _ in range(3)]
+        results = [srv.wait(t) for t in tickets]
+    for ri, re, _ in results:
+        assert np.array_equal(ri, expect[0]) and same_bits(re, expect[1])
+    spans = sorted((i["start_ms"], i["finish_ms"]) for _, _, i in results)
+    for a, b in zip(spans, spans[1:]):
+        assert b[0] >= a[1], f"requests overlapped under a budget that fits one: {spans}"
+
+
+def test_rejections():
+    announce("rejections")
+    m = W.grid(64)
+    with M.Server(budget_bytes=1 << 20, slots=2) as srv:
+        with pytest.raises(RuntimeError, match="budget"):
+            srv.submit(m.indices, m.vertices, 0)
+    with M.Server(budget_bytes=1 << 30, slots=2) as srv:
+        with pytest.raises(RuntimeError, match="invalid"):
+            srv.submit(m.indices, m.vertices, m.index_count + 3)
+        bad = m.indices.copy()
+        bad[5] = m.vertex_count + 10
+        t_bad = srv.submit(bad, m.vertices, 0)
+        t_ok = srv.submit(m.indices, m.vertices, W.target_count(m.index_count, 1, 4))
+        with pytest.raises(RuntimeError, match="vertex"):
+            srv.wait(t_bad)
+        ri, re, _ = srv.wait(t_ok)
+    e = M.simplify(m.indices, m.vertices, W.target_count(m.index_count, 1, 4))
+    assert np.array_equal(ri, e[0]) and same_bits(re, e[1])
