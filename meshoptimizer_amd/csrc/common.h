@@ -97,6 +97,9 @@ struct CallScalars
 	uint32_t dual_count;  // boundEdgeCollapses reduction
 	uint32_t unique_count; // sparse remap: number of referenced vertices
 	uint32_t tail_rounds;   // rounds run by the single-workgroup tail of the commit
+	uint32_t component_count; // Prune: connected components of the welded mesh
+	uint32_t prune_min_bits;  // Prune: smallest component error among kept triangles (float bits)
+	uint32_t prune_max_bits;  // Prune: largest component error at or below a cutoff (float bits)
 	uint32_t head_sizes[16]; // work-list size at the start of each head round
 	uint32_t x_visits;      // ranks listed over every phase-X launch of the call (roofline accounting)
 	uint32_t x_launches;

@@ -225,6 +225,8 @@ struct Plan
 	uint32_t* i_alt0;
 	uint32_t* i_alt1;
 	uint32_t* kind_counts;
+	uint32_t* comp; // Prune: component per vertex
+	float* cerr;    // Prune: error per component
 };
 
 static size_t pow2_at_least(size_t n)
@@ -235,7 +237,7 @@ static size_t pow2_at_least(size_t n)
 	return p;
 }
 
-static void carve(Arena& a, Plan& p, size_t V_in, size_t V, size_t I, size_t A, bool sparse)
+static void carve(Arena& a, Plan& p, size_t V_in, size_t V, size_t I, size_t A, bool sparse, bool prune)
 {
 	Mesh& m = p.m;
 	size_t T = I / 3 + 1;
@@ -314,6 +316,8 @@ static void carve(Arena& a, Plan& p, size_t V_in, size_t V, size_t I, size_t A, 
 	p.i_alt0 = m.keys_alt;
 	p.i_alt1 = m.vals_alt;
 	p.kind_counts = a.take<uint32_t>(8);
+	p.comp = prune ? a.take<uint32_t>(Vp) : nullptr;
+	p.cerr = prune ? a.take<float>(Vp) : nullptr;
 
 	size_t biggest = std::max(std::max(I, cap), Vp + 1);
 	size_t temp_words = std::max(sort_temp_words(biggest), scan_temp_words(biggest + 16));
@@ -360,6 +364,13 @@ __global__ void k_scalars_init(CallScalars* sc)
 	sc->work[2] = 0;
 }
 
+static float host_float_bits(uint32_t u)
+{
+	float f;
+	memcpy(&f, &u, sizeof(f));
+	return f;
+}
+
 static void sync_scalars(Context& ctx, const Mesh& m)
 {
 	HIP_CHECK(hipMemcpyAsync(ctx.sc_host, m.sc, sizeof(CallScalars), hipMemcpyDeviceToHost, ctx.stream));
@@ -389,6 +400,7 @@ static size_t simplify_on_device(Context& ctx, const CallArgs& c, float* result_
 {
 	hipStream_t s = ctx.stream;
 	const bool sparse = (c.options & kOptSparse) != 0;
+	const bool prune = (c.options & kOptPrune) != 0;
 	const size_t I = c.index_count;
 	const size_t V_in = c.vertex_count;
 
@@ -407,12 +419,12 @@ static size_t simplify_on_device(Context& ctx, const CallArgs& c, float* result_
 
 	Arena dry;
 	Plan plan;
-	carve(dry, plan, V_in, V_alloc, I, A, sparse);
+	carve(dry, plan, V_in, V_alloc, I, A, sparse, prune);
 	ctx.ensure_arena(dry.used + 4096);
 	Arena arena;
 	arena.base = ctx.arena;
 	arena.size = ctx.arena_size;
-	carve(arena, plan, V_in, V_alloc, I, A, sparse);
+	carve(arena, plan, V_in, V_alloc, I, A, sparse, prune);
 	Mesh& m = plan.m;
 
 	m.index_count = (uint32_t)I;
@@ -475,6 +487,14 @@ static size_t simplify_on_device(Context& ctx, const CallArgs& c, float* result_
 	launch_rescale(s, m);
 	if (A)
 		launch_rescale_attributes(s, m, c.attributes, c.attribute_stride / sizeof(float), sparse_remap, amap, aw);
+	if (prune)
+	{
+		// buildComponents / measureComponents (src/simplifier.cpp:2457-2472) on the input
+		// index buffer and the rescaled positions; the wedge scratch is free again
+		uint32_t* scratch[5] = {plan.v_flags, plan.v_a, plan.v_b, plan.v_c, plan.v_d};
+		launch_components(s, m, I, plan.comp, scratch);
+		launch_measure_components(s, m, plan.comp, plan.cerr, scratch);
+	}
 
 	launch_corner_csr(s, m.ib[0], I, m.remap, V, plan.w_off, plan.w_keys, plan.w_corners, plan.i_alt0, plan.i_alt1, m.temp);
 	HIP_CHECK(hipMemsetAsync(m.vq, 0, (V + 1) * sizeof(Quad), s));
@@ -517,6 +537,8 @@ static size_t simplify_on_device(Context& ctx, const CallArgs& c, float* result_
 
 	size_t result_count = I;
 	float result_error_sq = 0.f;
+	// Prune: smallest component error still present (src/simplifier.cpp:2466-2470)
+	float prune_next = prune ? host_float_bits(h.prune_min_bits) : 0.f;
 
 	while (result_count > c.target_index_count)
 	{
@@ -596,11 +618,40 @@ static size_t simplify_on_device(Context& ctx, const CallArgs& c, float* result_
 		launch_remap_triangles(s, m, T);
 		sync_scalars(ctx, m);
 		result_count = (size_t)h.next_triangles * 3;
+
+		// components whose error fell under the pass's error (src/simplifier.cpp:2539-2541)
+		if (prune && result_count > c.target_index_count && prune_next <= h.vertex_error)
+		{
+			launch_prune_triangles(s, m, result_count / 3, plan.comp, plan.cerr, h.vertex_error);
+			sync_scalars(ctx, m);
+			result_count = (size_t)h.next_triangles * 3;
+			prune_next = host_float_bits(h.prune_min_bits);
+		}
 	}
 
 	// the stop scan advanced result_error even in a pass that broke off without commits
 	sync_scalars(ctx, m);
 	result_error_sq = h.result_error;
+	float vertex_error_sq = h.vertex_error;
+
+	// remaining components up to the error limit (src/simplifier.cpp:2545-2569)
+	bool prune_stale = true;
+	while (prune && result_count > c.target_index_count && prune_next <= error_limit)
+	{
+		float cutoff = prune_next * 1.5f < error_limit ? prune_next * 1.5f : error_limit;
+		launch_component_max(s, m, plan.cerr, cutoff);
+		launch_prune_triangles(s, m, result_count / 3, plan.comp, plan.cerr, cutoff);
+		sync_scalars(ctx, m);
+		float cmax = host_float_bits(h.prune_max_bits);
+		size_t new_count = (size_t)h.next_triangles * 3;
+		prune_next = host_float_bits(h.prune_min_bits);
+		if (new_count == result_count && !prune_stale)
+			break;
+		prune_stale = false;
+		result_count = new_count;
+		result_error_sq = result_error_sq < cmax ? cmax : result_error_sq;
+		vertex_error_sq = vertex_error_sq < cmax ? cmax : vertex_error_sq;
+	}
 
 	if ((c.options & kOptInternalDebug) && !sparse)
 		launch_debug_bits(s, m, result_count);
@@ -624,7 +675,7 @@ static size_t simplify_on_device(Context& ctx, const CallArgs& c, float* result_
 
 	t_call.passes = (unsigned int)t_passes.size();
 	t_call.result_error_sq = result_error_sq;
-	t_call.vertex_error_sq = h.vertex_error;
+	t_call.vertex_error_sq = vertex_error_sq;
 
 	if (result_error_out)
 		*result_error_out = sqrtf(result_error_sq) * error_scale;
@@ -1163,7 +1214,8 @@ static size_t request_device_bytes(const meshopt_hip_request& r)
 	const bool sparse = (r.options & kOptSparse) != 0;
 	Arena dry;
 	Plan plan;
-	carve(dry, plan, r.vertex_count, sparse ? std::min(r.vertex_count, r.index_count) : r.vertex_count, r.index_count, A, sparse);
+	carve(dry, plan, r.vertex_count, sparse ? std::min(r.vertex_count, r.index_count) : r.vertex_count, r.index_count, A, sparse,
+	      (r.options & kOptPrune) != 0);
 	size_t arena = dry.used + 4096;
 	arena += arena / 4;
 	const HostStaging h = plan_staging(r.index_count, r.vertex_positions, r.vertex_count, r.vertex_positions_stride, r.vertex_attributes,

@@ -114,4 +114,17 @@ void launch_remap_triangles(hipStream_t s, Mesh& m, size_t tri_count);
 void launch_debug_bits(hipStream_t s, Mesh& m, size_t index_count);
 void launch_sparse_backmap(hipStream_t s, uint32_t* ib, size_t index_count, const uint32_t* sparse_remap);
 
+// ---------------------------------------------------------------- Prune (prune.hip)
+// components of the welded mesh over the input index buffer: comp[V] (ids in order of each
+// component's lowest primary), sc->component_count; scratch: five arrays of V + 1 words
+void launch_components(hipStream_t s, Mesh& m, size_t index_count, uint32_t* comp, uint32_t* scratch[5]);
+// squared bounding-sphere radius per component around its vertex mean, sums in vertex order;
+// sc->prune_min_bits = the smallest (FLT_MAX bits when there is none)
+void launch_measure_components(hipStream_t s, Mesh& m, const uint32_t* comp, float* cerr, uint32_t* scratch[5]);
+// keep the triangles whose component error (of their first vertex) is above cutoff, in order;
+// sc->next_triangles and sc->prune_min_bits (over the kept ones) are the results
+void launch_prune_triangles(hipStream_t s, Mesh& m, size_t tri_count, const uint32_t* comp, const float* cerr, float cutoff);
+// sc->prune_max_bits = largest component error <= cutoff (0 if none)
+void launch_component_max(hipStream_t s, Mesh& m, const float* cerr, float cutoff);
+
 } // namespace mhip

@@ -38,8 +38,6 @@ def assert_parity(ib, v, target, err, opts, attrs=None, weights=None, lock=None,
 
 @pytest.mark.parametrize("case", CASES, ids=[c["name"] for c in CASES])
 def test_known_answers(case):
-    if case["options"] & M.SimplifyPrune:
-        pytest.skip("the Prune option is not implemented on the GPU path yet")
     announce(case["name"])
     a = kat.arrays(case)
     with_attr = case["call"] == "simplifyWithAttributes"
@@ -49,7 +47,7 @@ def test_known_answers(case):
     kat.check(case, r, e)
 
 
-OPTION_SETS = [0, 1, 2, 4, 16, 32, 64, 1 | 32, 32 | 16, 1 | 2 | 4]
+OPTION_SETS = [0, 1, 2, 4, 16, 32, 64, 1 | 32, 32 | 16, 1 | 2 | 4, 8, 8 | 1, 8 | 32, 8 | 2 | 4]
 
 
 @pytest.mark.parametrize("seed", range(60))
@@ -117,3 +115,34 @@ def test_workloads_full(name):
     A = m.vertices[:, 3:8] if attrs else None
     w = W.SEAMED_GRID_WEIGHTS if attrs else None
     assert_parity(m.indices, m.vertices, t, 1e-2, 0, A, w)
+
+
+def scattered_components(rng, pieces):
+    """One mesh made of `pieces` disjoint grid patches and icospheres at different scales and
+    offsets, in shuffled triangle order (components with very different errors)."""
+    ibs, vs, base = [], [], 0
+    for k in range(pieces):
+        if k % 3 == 2:
+            m = W.icosphere(int(rng.integers(1, 4)), seed=int(rng.integers(1, 1000)))
+        else:
+            m = W.grid(int(rng.integers(3, 12)), seed=int(rng.integers(1, 1000)))
+        v = m.vertices[:, :3] * np.float32(rng.choice([0.0005, 0.002, 0.01, 0.05, 0.3]))
+        v = v + rng.random(3).astype(np.float32) * np.float32(4.0)
+        ibs.append(m.indices.reshape(-1, 3) + base)
+        vs.append(v.astype(np.float32))
+        base += v.shape[0]
+    tri = np.concatenate(ibs)
+    tri = tri[rng.permutation(tri.shape[0])]
+    return tri.reshape(-1).astype(np.uint32), np.ascontiguousarray(np.concatenate(vs))
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_prune_components(seed):
+    """Prune (src/simplifier.cpp:1866-2024, 2457-2472, 2539-2569) on meshes of many components."""
+    rng = np.random.default_rng(500 + seed)
+    ib, v = scattered_components(rng, int(rng.integers(4, 40)))
+    target = 0 if seed % 2 else (ib.size // 3 * int(rng.integers(1, 3)) // 4) * 3
+    err = float(rng.choice([1e-3, 1e-2, 5e-2, 0.2]))
+    opts = M.SimplifyPrune | int(rng.choice([0, M.SimplifyLockBorder, M.SimplifySparse, M.SimplifyErrorAbsolute]))
+    announce(f"prune seed={seed} I={ib.size} V={v.shape[0]} opts={opts} err={err}")
+    assert_parity(ib, v, target, err, opts)
