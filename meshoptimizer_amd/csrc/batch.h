@@ -24,7 +24,11 @@ struct BatchArgs
 	uint32_t options;
 };
 
+// LDS size classes of one batched call (launched side by side)
+constexpr int kBatchClasses = 3;
+
 size_t batch_lds_bytes(uint32_t vmax, uint32_t imax);
-void launch_batch(hipStream_t s, const BatchArgs& a, uint32_t vmax, uint32_t imax);
+// one workgroup per listed mesh (ids: count device words), every one with the (vmax, imax) layout
+void launch_batch(hipStream_t s, const BatchArgs& a, uint32_t vmax, uint32_t imax, const uint32_t* ids, size_t count);
 
 } // namespace mhip

@@ -4,8 +4,8 @@
 namespace mhip
 {
 
-// One workgroup per mesh.
-__global__ void __launch_bounds__(kBatchThreads) k_simplify_patches(BatchArgs a, PatchLayout L)
+// One workgroup per mesh of one LDS size class; ids lists the class's meshes.
+__global__ void __launch_bounds__(kBatchThreads) k_simplify_patches(BatchArgs a, PatchLayout L, const uint32_t* ids)
 {
 	extern __shared__ __align__(16) uint8_t smem[];
 	__shared__ uint32_t part[kBatchThreads / 64];
@@ -13,9 +13,7 @@ __global__ void __launch_bounds__(kBatchThreads) k_simplify_patches(BatchArgs a,
 	__shared__ uint64_t box[6];
 	__shared__ float shf[8];
 
-	const uint32_t patch = blockIdx.x;
-	if (patch >= a.patch_count)
-		return;
+	const uint32_t patch = ids[blockIdx.x];
 	simplify_patch(a, L, patch, smem, part, sh, box, shf);
 }
 
@@ -36,11 +34,11 @@ size_t batch_lds_bytes(uint32_t vmax, uint32_t imax)
 	return patch_layout(vmax, imax).total;
 }
 
-void launch_batch(hipStream_t s, const BatchArgs& a, uint32_t vmax, uint32_t imax)
+void launch_batch(hipStream_t s, const BatchArgs& a, uint32_t vmax, uint32_t imax, const uint32_t* ids, size_t count)
 {
 	PatchLayout L = patch_layout(vmax, imax);
-	if (a.patch_count)
-		k_simplify_patches<<<(unsigned int)a.patch_count, kBatchThreads, L.total, s>>>(a, L);
+	if (count)
+		k_simplify_patches<<<(unsigned int)count, kBatchThreads, L.total, s>>>(a, L, ids);
 }
 
 } // namespace mhip

@@ -115,6 +115,20 @@ struct Context
 	size_t batch_launches = 0, batch_meshes = 0;
 	hipEvent_t batch_ev[2] = {nullptr, nullptr};
 	double window_fraction = 0; // evaluated / candidates of the previous pass (0: none yet)
+	hipStream_t aux[kBatchClasses - 1] = {}; // batch size classes beyond the first
+	hipEvent_t fork_ev = nullptr, join_ev[kBatchClasses - 1] = {};
+
+	void ensure_aux_streams()
+	{
+		if (fork_ev)
+			return;
+		for (int k = 0; k < kBatchClasses - 1; ++k)
+		{
+			HIP_CHECK(hipStreamCreateWithFlags(&aux[k], hipStreamNonBlocking));
+			HIP_CHECK(hipEventCreateWithFlags(&join_ev[k], hipEventDisableTiming));
+		}
+		HIP_CHECK(hipEventCreateWithFlags(&fork_ev, hipEventDisableTiming));
+	}
 
 	void ensure_init()
 	{
@@ -155,6 +169,18 @@ struct Context
 
 	void release()
 	{
+		for (int k = 0; k < kBatchClasses - 1; ++k)
+		{
+			if (aux[k])
+				(void)hipStreamDestroy(aux[k]);
+			if (join_ev[k])
+				(void)hipEventDestroy(join_ev[k]);
+			aux[k] = nullptr;
+			join_ev[k] = nullptr;
+		}
+		if (fork_ev)
+			(void)hipEventDestroy(fork_ev);
+		fork_ev = nullptr;
 		staging.release();
 		if (arena)
 			(void)hipFree(arena);
@@ -931,20 +957,47 @@ static int simplify_batch_on_device(Context& ctx, unsigned int* destination, uns
 			need.push_back({batch_lds_bytes(round_v(nv), round_i(ni)), (uint32_t)p});
 	}
 	std::sort(need.begin(), need.end());
-	uint32_t vmax = 32, imax = 96;
-	for (auto& e : need)
+
+	// LDS size classes: meshes in ascending need, each class as large as its layout (the maxima
+	// of its members) stays within the class limit; a class limit is what lets `per CU`
+	// workgroups share one CU's LDS.  Meshes beyond the last class take the plain path.
+	struct BatchClass
 	{
-		uint32_t nv = round_v(h_vertex_offsets[e.second + 1] - h_vertex_offsets[e.second]);
-		uint32_t ni = round_i(h_index_offsets[e.second + 1] - h_index_offsets[e.second]);
-		uint32_t v2 = std::max(vmax, nv), i2 = std::max(imax, ni);
-		if (batch_lds_bytes(v2, i2) > kBatchLdsLimit)
-			break;
-		vmax = v2;
-		imax = i2;
+		uint32_t vmax, imax;
+		size_t first, count; // range of the class in `ids`
+	};
+	const size_t cu_lds = 160 * 1024;
+	const size_t limits[kBatchClasses] = {cu_lds / 4, cu_lds / 3 - 256, kBatchLdsLimit};
+	std::vector<BatchClass> classes;
+	std::vector<uint32_t> ids;
+	ids.reserve(need.size());
+	std::vector<uint8_t> launched(patch_count, 0);
+	{
+		size_t k = 0;
+		for (int c = 0; c < kBatchClasses && k < need.size(); ++c)
+		{
+			BatchClass bc{32, 96, ids.size(), 0};
+			for (; k < need.size(); ++k)
+			{
+				uint32_t p = need[k].second;
+				uint32_t v2 = std::max(bc.vmax, round_v(h_vertex_offsets[p + 1] - h_vertex_offsets[p]));
+				uint32_t i2 = std::max(bc.imax, round_i(h_index_offsets[p + 1] - h_index_offsets[p]));
+				if (batch_lds_bytes(v2, i2) > limits[c])
+					break;
+				bc.vmax = v2;
+				bc.imax = i2;
+				ids.push_back(p);
+				launched[p] = 1;
+			}
+			bc.count = ids.size() - bc.first;
+			if (bc.count)
+				classes.push_back(bc);
+		}
 	}
 
-	uint32_t* status = nullptr;
-	HIP_CHECK(hipMallocAsync((void**)&status, std::max<size_t>(patch_count, 1) * sizeof(uint32_t), s));
+	uint32_t* status = nullptr; // status[patch_count], then the class id lists
+	HIP_CHECK(hipMallocAsync((void**)&status, (std::max<size_t>(patch_count, 1) + ids.size()) * sizeof(uint32_t), s));
+	uint32_t* d_ids = status + std::max<size_t>(patch_count, 1);
 	BatchArgs a;
 	a.destination = destination;
 	a.result_counts = result_counts;
@@ -970,9 +1023,26 @@ static int simplify_batch_on_device(Context& ctx, unsigned int* destination, uns
 			HIP_CHECK(hipEventCreate(&ctx.batch_ev[0]));
 			HIP_CHECK(hipEventCreate(&ctx.batch_ev[1]));
 		}
+		if (!ids.empty())
+			HIP_CHECK(hipMemcpyAsync(d_ids, ids.data(), ids.size() * sizeof(uint32_t), hipMemcpyHostToDevice, s));
 		if (timed)
 			HIP_CHECK(hipEventRecord(ctx.batch_ev[0], s));
-		launch_batch(s, a, vmax, imax);
+		// the classes run side by side: the first on the call's stream, the others on auxiliary
+		// streams forked from it and joined back before the status read-back
+		ctx.ensure_aux_streams();
+		HIP_CHECK(hipEventRecord(ctx.fork_ev, s));
+		for (size_t c = 0; c < classes.size(); ++c)
+		{
+			hipStream_t cs = c == 0 ? s : ctx.aux[c - 1];
+			if (c)
+				HIP_CHECK(hipStreamWaitEvent(cs, ctx.fork_ev, 0));
+			launch_batch(cs, a, classes[c].vmax, classes[c].imax, d_ids + classes[c].first, classes[c].count);
+			if (c)
+			{
+				HIP_CHECK(hipEventRecord(ctx.join_ev[c - 1], cs));
+				HIP_CHECK(hipStreamWaitEvent(s, ctx.join_ev[c - 1], 0));
+			}
+		}
 		if (timed)
 			HIP_CHECK(hipEventRecord(ctx.batch_ev[1], s));
 		HIP_CHECK(hipGetLastError());
@@ -994,6 +1064,8 @@ static int simplify_batch_on_device(Context& ctx, unsigned int* destination, uns
 	{
 		if (h_status[p] == 2)
 			throw std::invalid_argument("meshopt_hip_simplify_batch: index buffer references a vertex >= vertex_count");
+		if (!launched[p])
+			h_status[p] = 1;
 		if (h_status[p] == 3)
 			throw std::runtime_error("meshopt_hip_simplify_batch: internal error (pass bound exceeded)");
 		if (h_status[p] == 0)
