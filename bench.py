@@ -198,7 +198,8 @@ def bench_patches(args):
         ab = batch_alg_bytes(np.diff(io), np.diff(vo), g_cnt)
         per_launch_s = k_ms * 1e-3 / k_launches
         achieved = ab / per_launch_s / 1e9
-        out["roofline"] = {"bound": "hbm", "kernel": "k_simplify_patches (one workgroup per patch, LDS-resident)",
+        out["roofline"] = {"bound": "hbm", "kernel": "k_simplify_patches: one LDS-resident workgroup per patch, 3 size classes "
+                                                  "launched side by side, timed with the classify kernel before them",
                            "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                            "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": measured_traffic("patches4096", "k_simplify_patches"),
                            "alg_bytes_per_launch": ab, "avg_launch_us": round(per_launch_s * 1e6, 2)}

@@ -6,12 +6,15 @@
 namespace mhip
 {
 
+// LDS size classes of one batched call (launched side by side)
+constexpr int kBatchClasses = 3;
+
 struct BatchArgs
 {
 	uint32_t* destination;             // packed: mesh p writes at index_offsets[p]
 	uint32_t* result_counts;           // per mesh
 	float* result_errors;              // per mesh (may be null)
-	uint32_t* status;                  // per mesh: 0 done, 1 too large for LDS, 2 bad index, 3 pass bound hit (internal error)
+	uint32_t* status;                  // per mesh: 0 done, 1 plain path, 2 bad index, 3 pass bound hit (internal error), 4 invalid sizes
 	uint32_t* passes;                  // per mesh (may be null)
 	const uint32_t* indices;           // packed local indices
 	const uint32_t* index_offsets;     // mesh_count + 1
@@ -24,11 +27,23 @@ struct BatchArgs
 	uint32_t options;
 };
 
-// LDS size classes of one batched call (launched side by side)
-constexpr int kBatchClasses = 3;
-
 size_t batch_lds_bytes(uint32_t vmax, uint32_t imax);
-// one workgroup per listed mesh (ids: count device words), every one with the (vmax, imax) layout
-void launch_batch(hipStream_t s, const BatchArgs& a, uint32_t vmax, uint32_t imax, const uint32_t* ids, size_t count);
+// size classes: a mesh belongs to the first class whose vertex and index caps it fits
+struct BatchClasses
+{
+	uint32_t vcap[kBatchClasses];
+	uint32_t icap[kBatchClasses];
+};
+
+// the largest caps per class (V:I at most 1:4) whose LDS layout fits each class's share of a CU
+BatchClasses batch_classes();
+
+// per mesh: validate sizes (status 4), pick its class (ids[c * patch_count + slot], counts[c])
+// or mark it for the plain path (status 1); counts must be zero on entry
+void launch_batch_classify(hipStream_t s, const BatchArgs& a, const BatchClasses& classes, uint32_t* ids, uint32_t* counts);
+
+// one workgroup per mesh of a class (ids, *count_dev of them; grid = upper bound), every one with
+// the (vmax, imax) layout
+void launch_batch(hipStream_t s, const BatchArgs& a, uint32_t vmax, uint32_t imax, const uint32_t* ids, const uint32_t* count_dev, size_t grid);
 
 } // namespace mhip

@@ -4,8 +4,9 @@
 namespace mhip
 {
 
-// One workgroup per mesh of one LDS size class; ids lists the class's meshes.
-__global__ void __launch_bounds__(kBatchThreads) k_simplify_patches(BatchArgs a, PatchLayout L, const uint32_t* ids)
+// One workgroup per mesh of one LDS size class; ids lists the class's *count meshes (the grid
+// is an upper bound, workgroups past the count leave at once).
+__global__ void __launch_bounds__(kBatchThreads) k_simplify_patches(BatchArgs a, PatchLayout L, const uint32_t* ids, const uint32_t* count)
 {
 	extern __shared__ __align__(16) uint8_t smem[];
 	__shared__ uint32_t part[kBatchThreads / 64];
@@ -13,8 +14,61 @@ __global__ void __launch_bounds__(kBatchThreads) k_simplify_patches(BatchArgs a,
 	__shared__ uint64_t box[6];
 	__shared__ float shf[8];
 
+	if (blockIdx.x >= *count)
+		return;
 	const uint32_t patch = ids[blockIdx.x];
 	simplify_patch(a, L, patch, smem, part, sh, box, shf);
+}
+
+__global__ void k_batch_classify(BatchArgs a, BatchClasses c, uint32_t* ids, uint32_t* counts)
+{
+	size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (p >= a.patch_count)
+		return;
+	const uint32_t ib = a.index_offsets[p], ie = a.index_offsets[p + 1];
+	const uint32_t vb = a.vertex_offsets[p], ve = a.vertex_offsets[p + 1];
+	if (ie < ib || ve < vb || (ie - ib) % 3 != 0 || a.target_index_counts[p] > ie - ib)
+	{
+		a.status[p] = 4;
+		return;
+	}
+	const uint32_t ni = ie - ib, nv = ve - vb;
+	for (int k = 0; k < kBatchClasses; ++k)
+		if (nv <= c.vcap[k] && ni <= c.icap[k])
+		{
+			uint32_t slot = atomicAdd(&counts[k], 1u);
+			ids[(size_t)k * a.patch_count + slot] = (uint32_t)p;
+			return;
+		}
+	a.status[p] = 1; // too large for the LDS layouts: the plain path
+}
+
+BatchClasses batch_classes()
+{
+	const uint32_t cu_lds = 160 * 1024;
+	const uint32_t limits[kBatchClasses] = {cu_lds / 4, cu_lds / 3 - 256, 64 * 1024 - 512};
+	BatchClasses c{};
+	for (int k = 0; k < kBatchClasses; ++k)
+	{
+		uint32_t vbest = 32, ibest = 96;
+		for (uint32_t icap = 96; icap < 0xffffu; icap += 96)
+		{
+			uint32_t vcap = ((icap / 4) + 31) & ~31u;
+			if (patch_layout(vcap, icap).total > limits[k])
+				break;
+			vbest = vcap;
+			ibest = icap;
+		}
+		c.vcap[k] = vbest;
+		c.icap[k] = ibest;
+	}
+	return c;
+}
+
+void launch_batch_classify(hipStream_t s, const BatchArgs& a, const BatchClasses& classes, uint32_t* ids, uint32_t* counts)
+{
+	if (a.patch_count)
+		k_batch_classify<<<(unsigned int)((a.patch_count + 255) / 256), 256, 0, s>>>(a, classes, ids, counts);
 }
 
 #ifdef MESHOPT_BATCH_PROFILE
@@ -34,11 +88,11 @@ size_t batch_lds_bytes(uint32_t vmax, uint32_t imax)
 	return patch_layout(vmax, imax).total;
 }
 
-void launch_batch(hipStream_t s, const BatchArgs& a, uint32_t vmax, uint32_t imax, const uint32_t* ids, size_t count)
+void launch_batch(hipStream_t s, const BatchArgs& a, uint32_t vmax, uint32_t imax, const uint32_t* ids, const uint32_t* count_dev, size_t grid)
 {
 	PatchLayout L = patch_layout(vmax, imax);
-	if (count)
-		k_simplify_patches<<<(unsigned int)count, kBatchThreads, L.total, s>>>(a, L, ids);
+	if (grid)
+		k_simplify_patches<<<(unsigned int)grid, kBatchThreads, L.total, s>>>(a, L, ids, count_dev);
 }
 
 } // namespace mhip

@@ -115,8 +115,35 @@ struct Context
 	size_t batch_launches = 0, batch_meshes = 0;
 	hipEvent_t batch_ev[2] = {nullptr, nullptr};
 	double window_fraction = 0; // evaluated / candidates of the previous pass (0: none yet)
+	uint32_t* batch_dev = nullptr;    // batched calls: status, class counts and class id lists
+	size_t batch_dev_words = 0;
+	uint32_t* batch_pinned = nullptr; // batched calls: status read-back
+	size_t batch_pinned_words = 0;
 	hipStream_t aux[kBatchClasses - 1] = {}; // batch size classes beyond the first
 	hipEvent_t fork_ev = nullptr, join_ev[kBatchClasses - 1] = {};
+
+	void ensure_batch_buffers(size_t meshes)
+	{
+		size_t words = meshes + 4 + (size_t)kBatchClasses * meshes;
+		if (words > batch_dev_words)
+		{
+			if (batch_dev)
+				HIP_CHECK(hipFree(batch_dev));
+			batch_dev = nullptr;
+			batch_dev_words = 0;
+			HIP_CHECK(hipMalloc((void**)&batch_dev, words * sizeof(uint32_t)));
+			batch_dev_words = words;
+		}
+		if (meshes > batch_pinned_words)
+		{
+			if (batch_pinned)
+				HIP_CHECK(hipHostFree(batch_pinned));
+			batch_pinned = nullptr;
+			batch_pinned_words = 0;
+			HIP_CHECK(hipHostMalloc((void**)&batch_pinned, meshes * sizeof(uint32_t), hipHostMallocDefault));
+			batch_pinned_words = meshes;
+		}
+	}
 
 	void ensure_aux_streams()
 	{
@@ -169,6 +196,13 @@ struct Context
 
 	void release()
 	{
+		if (batch_dev)
+			(void)hipFree(batch_dev);
+		if (batch_pinned)
+			(void)hipHostFree(batch_pinned);
+		batch_dev = nullptr;
+		batch_pinned = nullptr;
+		batch_dev_words = batch_pinned_words = 0;
 		for (int k = 0; k < kBatchClasses - 1; ++k)
 		{
 			if (aux[k])
@@ -924,9 +958,12 @@ extern "C" size_t meshopt_hip_simplify_device(unsigned int* destination, const u
 namespace mhip
 {
 
-// dynamic LDS per workgroup; the kernel's static LDS (scan partials, bounds) stays under 512 bytes
-static const size_t kBatchLdsLimit = 64 * 1024 - 512;
 
+// Batched call: sizes are validated and meshes sorted into LDS size classes on the device, the
+// classes run side by side (the first on the call's stream, the others on auxiliary streams
+// forked from it), and only the per-mesh status comes back to the host.  Meshes that fit no
+// class (and Sparse batches) take the plain device path afterwards; their offsets and targets
+// are read back only then (h_* may be null: device-resident offsets).
 static int simplify_batch_on_device(Context& ctx, unsigned int* destination, unsigned int* result_counts, float* result_errors,
                                     const unsigned int* indices, const unsigned int* index_offsets, const float* positions,
                                     const unsigned int* vertex_offsets, size_t stride, const unsigned int* target_index_counts,
@@ -938,66 +975,17 @@ static int simplify_batch_on_device(Context& ctx, unsigned int* destination, uns
 		throw std::invalid_argument("meshopt_hip_simplify_batch: the Prune option is not supported");
 	if (stride < 12 || stride > 256 || stride % 4)
 		throw std::invalid_argument("meshopt_hip_simplify_batch: invalid vertex stride");
+	if (!patch_count)
+		return 0;
 
-	// per-mesh LDS need; the launch's layout is the largest prefix (by need) whose combined
-	// maxima fit one workgroup's LDS; meshes outside it take the plain path
-	auto round_v = [](uint32_t v) { return (v + 31) & ~31u; };
-	auto round_i = [](uint32_t i) { return (i + 95) / 96 * 96; };
-	std::vector<std::pair<size_t, uint32_t>> need; // (bytes, mesh)
-	need.reserve(patch_count);
-	for (size_t p = 0; p < patch_count; ++p)
-	{
-		uint32_t ni = h_index_offsets[p + 1] - h_index_offsets[p];
-		uint32_t nv = h_vertex_offsets[p + 1] - h_vertex_offsets[p];
-		if (h_index_offsets[p + 1] < h_index_offsets[p] || h_vertex_offsets[p + 1] < h_vertex_offsets[p])
-			throw std::invalid_argument("meshopt_hip_simplify_batch: offsets must not decrease");
-		if (ni % 3 || h_targets[p] > ni)
-			throw std::invalid_argument("meshopt_hip_simplify_batch: invalid index count or target");
-		if (nv < 0xffffu && ni < 0xffffu)
-			need.push_back({batch_lds_bytes(round_v(nv), round_i(ni)), (uint32_t)p});
-	}
-	std::sort(need.begin(), need.end());
+	static const BatchClasses classes = batch_classes();
+	const size_t P = patch_count;
+	ctx.ensure_batch_buffers(P);
+	uint32_t* status = ctx.batch_dev;
+	uint32_t* counts = ctx.batch_dev + P;
+	uint32_t* ids = ctx.batch_dev + P + 4;
+	uint32_t* h_status = ctx.batch_pinned;
 
-	// LDS size classes: meshes in ascending need, each class as large as its layout (the maxima
-	// of its members) stays within the class limit; a class limit is what lets `per CU`
-	// workgroups share one CU's LDS.  Meshes beyond the last class take the plain path.
-	struct BatchClass
-	{
-		uint32_t vmax, imax;
-		size_t first, count; // range of the class in `ids`
-	};
-	const size_t cu_lds = 160 * 1024;
-	const size_t limits[kBatchClasses] = {cu_lds / 4, cu_lds / 3 - 256, kBatchLdsLimit};
-	std::vector<BatchClass> classes;
-	std::vector<uint32_t> ids;
-	ids.reserve(need.size());
-	std::vector<uint8_t> launched(patch_count, 0);
-	{
-		size_t k = 0;
-		for (int c = 0; c < kBatchClasses && k < need.size(); ++c)
-		{
-			BatchClass bc{32, 96, ids.size(), 0};
-			for (; k < need.size(); ++k)
-			{
-				uint32_t p = need[k].second;
-				uint32_t v2 = std::max(bc.vmax, round_v(h_vertex_offsets[p + 1] - h_vertex_offsets[p]));
-				uint32_t i2 = std::max(bc.imax, round_i(h_index_offsets[p + 1] - h_index_offsets[p]));
-				if (batch_lds_bytes(v2, i2) > limits[c])
-					break;
-				bc.vmax = v2;
-				bc.imax = i2;
-				ids.push_back(p);
-				launched[p] = 1;
-			}
-			bc.count = ids.size() - bc.first;
-			if (bc.count)
-				classes.push_back(bc);
-		}
-	}
-
-	uint32_t* status = nullptr; // status[patch_count], then the class id lists
-	HIP_CHECK(hipMallocAsync((void**)&status, (std::max<size_t>(patch_count, 1) + ids.size()) * sizeof(uint32_t), s));
-	uint32_t* d_ids = status + std::max<size_t>(patch_count, 1);
 	BatchArgs a;
 	a.destination = destination;
 	a.result_counts = result_counts;
@@ -1014,29 +1002,27 @@ static int simplify_batch_on_device(Context& ctx, unsigned int* destination, uns
 	a.target_error = target_error;
 	a.options = options;
 
-	std::vector<uint32_t> h_status(patch_count, 1);
-	if (!(options & kOptSparse))
+	const bool batched = !(options & kOptSparse);
+	if (batched)
 	{
-		const bool timed = ctx.timing && patch_count;
+		const bool timed = ctx.timing;
 		if (timed && !ctx.batch_ev[0])
 		{
 			HIP_CHECK(hipEventCreate(&ctx.batch_ev[0]));
 			HIP_CHECK(hipEventCreate(&ctx.batch_ev[1]));
 		}
-		if (!ids.empty())
-			HIP_CHECK(hipMemcpyAsync(d_ids, ids.data(), ids.size() * sizeof(uint32_t), hipMemcpyHostToDevice, s));
 		if (timed)
 			HIP_CHECK(hipEventRecord(ctx.batch_ev[0], s));
-		// the classes run side by side: the first on the call's stream, the others on auxiliary
-		// streams forked from it and joined back before the status read-back
+		HIP_CHECK(hipMemsetAsync(counts, 0, 4 * sizeof(uint32_t), s));
+		launch_batch_classify(s, a, classes, ids, counts);
 		ctx.ensure_aux_streams();
 		HIP_CHECK(hipEventRecord(ctx.fork_ev, s));
-		for (size_t c = 0; c < classes.size(); ++c)
+		for (int c = 0; c < kBatchClasses; ++c)
 		{
 			hipStream_t cs = c == 0 ? s : ctx.aux[c - 1];
 			if (c)
 				HIP_CHECK(hipStreamWaitEvent(cs, ctx.fork_ev, 0));
-			launch_batch(cs, a, classes[c].vmax, classes[c].imax, d_ids + classes[c].first, classes[c].count);
+			launch_batch(cs, a, classes.vcap[c], classes.icap[c], ids + (size_t)c * P, counts + c, P);
 			if (c)
 			{
 				HIP_CHECK(hipEventRecord(ctx.join_ev[c - 1], cs));
@@ -1046,31 +1032,56 @@ static int simplify_batch_on_device(Context& ctx, unsigned int* destination, uns
 		if (timed)
 			HIP_CHECK(hipEventRecord(ctx.batch_ev[1], s));
 		HIP_CHECK(hipGetLastError());
-		if (patch_count)
-			HIP_CHECK(hipMemcpyAsync(h_status.data(), status, patch_count * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+		HIP_CHECK(hipMemcpyAsync(h_status, status, P * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+		HIP_CHECK(hipStreamSynchronize(s));
+		if (timed)
+		{
+			float ms = 0;
+			HIP_CHECK(hipEventElapsedTime(&ms, ctx.batch_ev[0], ctx.batch_ev[1]));
+			ctx.batch_ms += ms;
+			ctx.batch_launches += 1;
+			ctx.batch_meshes += patch_count;
+		}
 	}
-	HIP_CHECK(hipFreeAsync(status, s));
-	HIP_CHECK(hipStreamSynchronize(s));
-	if (ctx.timing && patch_count && !(options & kOptSparse))
-	{
-		float ms = 0;
-		HIP_CHECK(hipEventElapsedTime(&ms, ctx.batch_ev[0], ctx.batch_ev[1]));
-		ctx.batch_ms += ms;
-		ctx.batch_launches += 1;
-		ctx.batch_meshes += patch_count;
-	}
+	else
+		std::fill(h_status, h_status + P, 1u);
 
-	for (size_t p = 0; p < patch_count; ++p)
+	bool fallback = false;
+	for (size_t p = 0; p < P; ++p)
 	{
 		if (h_status[p] == 2)
 			throw std::invalid_argument("meshopt_hip_simplify_batch: index buffer references a vertex >= vertex_count");
-		if (!launched[p])
-			h_status[p] = 1;
+		if (h_status[p] == 4)
+			throw std::invalid_argument("meshopt_hip_simplify_batch: invalid index count, target or offsets");
 		if (h_status[p] == 3)
 			throw std::runtime_error("meshopt_hip_simplify_batch: internal error (pass bound exceeded)");
-		if (h_status[p] == 0)
+		fallback |= h_status[p] == 1;
+	}
+	if (!fallback)
+		return 0;
+
+	// the plain device path for the meshes the batched kernel did not take
+	std::vector<unsigned int> io, vo, tg;
+	if (!h_index_offsets)
+	{
+		io.resize(P + 1);
+		vo.resize(P + 1);
+		tg.resize(P);
+		HIP_CHECK(hipMemcpyAsync(io.data(), index_offsets, (P + 1) * sizeof(unsigned int), hipMemcpyDeviceToHost, s));
+		HIP_CHECK(hipMemcpyAsync(vo.data(), vertex_offsets, (P + 1) * sizeof(unsigned int), hipMemcpyDeviceToHost, s));
+		HIP_CHECK(hipMemcpyAsync(tg.data(), target_index_counts, P * sizeof(unsigned int), hipMemcpyDeviceToHost, s));
+		HIP_CHECK(hipStreamSynchronize(s));
+		h_index_offsets = io.data();
+		h_vertex_offsets = vo.data();
+		h_targets = tg.data();
+	}
+	for (size_t p = 0; p < P; ++p)
+	{
+		if (h_status[p] != 1)
 			continue;
-		// too large for the LDS kernel (or a Sparse batch): the plain device path on this mesh
+		if (h_index_offsets[p + 1] < h_index_offsets[p] || h_vertex_offsets[p + 1] < h_vertex_offsets[p] ||
+		    (h_index_offsets[p + 1] - h_index_offsets[p]) % 3 || h_targets[p] > h_index_offsets[p + 1] - h_index_offsets[p])
+			throw std::invalid_argument("meshopt_hip_simplify_batch: invalid index count, target or offsets");
 		CallArgs c;
 		c.destination = destination + h_index_offsets[p];
 		c.indices = indices + h_index_offsets[p];
@@ -1108,14 +1119,8 @@ extern "C" int meshopt_hip_simplify_batch_device(unsigned int* destination, unsi
 		Context& ctx = context();
 		std::lock_guard<std::mutex> guard(ctx.mutex);
 		ctx.ensure_init();
-		std::vector<unsigned int> io(mesh_count + 1), vo(mesh_count + 1), tg(mesh_count + 1);
-		HIP_CHECK(hipMemcpyAsync(io.data(), index_offsets, (mesh_count + 1) * sizeof(unsigned int), hipMemcpyDeviceToHost, ctx.stream));
-		HIP_CHECK(hipMemcpyAsync(vo.data(), vertex_offsets, (mesh_count + 1) * sizeof(unsigned int), hipMemcpyDeviceToHost, ctx.stream));
-		if (mesh_count)
-			HIP_CHECK(hipMemcpyAsync(tg.data(), target_index_counts, mesh_count * sizeof(unsigned int), hipMemcpyDeviceToHost, ctx.stream));
-		HIP_CHECK(hipStreamSynchronize(ctx.stream));
 		simplify_batch_on_device(ctx, destination, result_counts, result_errors, indices, index_offsets, vertex_positions, vertex_offsets,
-		                         vertex_positions_stride, target_index_counts, mesh_count, target_error, options, io.data(), vo.data(), tg.data());
+		                         vertex_positions_stride, target_index_counts, mesh_count, target_error, options, nullptr, nullptr, nullptr);
 		t_last_error.clear();
 		return 0;
 	}
