@@ -251,10 +251,12 @@ __device__ __forceinline__ uint32_t wg_scan(uint32_t v, uint32_t* part, uint32_t
 }
 
 // counting placement of `count` items into per-key segments: off[key] = segment start, off[n] = total.
-// keys come from key_of(i); ids of the items in each segment come out in ascending item order.
-// cursor: n + 1 words of scratch.  All threads call it.
+// keys come from key_of(i); with `ordered` the ids of the items in each segment come out in
+// ascending item order, otherwise in any order.  cursor: n + 1 words of scratch.  All threads
+// call it.
 template <typename KeyOf>
-__device__ void segment_items(uint16_t* off, uint16_t* items, uint32_t n, uint32_t count, KeyOf key_of, uint32_t* cursor, uint32_t* part)
+__device__ void segment_items(uint16_t* off, uint16_t* items, uint32_t n, uint32_t count, KeyOf key_of, uint32_t* cursor, uint32_t* part,
+                              bool ordered)
 {
 	for (uint32_t v = threadIdx.x; v <= n; v += kBatchThreads)
 		cursor[v] = 0;
@@ -284,6 +286,8 @@ __device__ void segment_items(uint16_t* off, uint16_t* items, uint32_t n, uint32
 	for (uint32_t i = threadIdx.x; i < count; i += kBatchThreads)
 		items[atomicAdd(&cursor[key_of(i)], 1u)] = (uint16_t)i;
 	__syncthreads();
+	if (!ordered)
+		return;
 	// ascending item order inside each segment (segments are short)
 	for (uint32_t v = threadIdx.x; v < n; v += kBatchThreads)
 	{
@@ -343,16 +347,18 @@ __device__ __forceinline__ uint32_t canon_bits(float f)
 	return u == 0x80000000u ? 0u : u;
 }
 
-// build the corner CSR of the current index buffer keyed by `remap_key ? remap[v] : v`, sorted by
-// corner order when `ordered`; adj[p] = next | prev << 16 (both keyed the same way)
-__device__ void build_adjacency(const PatchSmem& s, const uint16_t* ib, uint32_t icount, uint32_t vcount, bool welded, uint16_t* corners,
-                                uint32_t* cursor, uint32_t* part)
+// build the corner CSR of the current index buffer keyed by `welded ? remap[v] : v`, each
+// vertex's corners in corner order when `ordered` (the quadric sums need it; classification and
+// the flip test only count, pick the single open edge, or OR over the corners);
+// adj[p] = next | prev << 16 (both keyed the same way)
+__device__ void build_adjacency(const PatchSmem& s, const uint16_t* ib, uint32_t icount, uint32_t vcount, bool welded, bool ordered,
+                                uint16_t* corners, uint32_t* cursor, uint32_t* part)
 {
 	auto key_of = [&](uint32_t j) -> uint32_t {
 		uint32_t v = ib[j];
 		return welded ? s.remap[v] : v;
 	};
-	segment_items(s.off, corners, vcount, icount, key_of, cursor, part);
+	segment_items(s.off, corners, vcount, icount, key_of, cursor, part, ordered);
 	for (uint32_t p = threadIdx.x; p < icount; p += kBatchThreads)
 	{
 		uint32_t j = corners[p];
@@ -506,7 +512,7 @@ __device__ __forceinline__ void simplify_patch(const BatchArgs& a, const PatchLa
 	PROFILE_PHASE(0);
 	// ---------------------------------------------------------------- unwelded adjacency, classification
 	uint16_t* corners = (uint16_t*)(s.scratch + (V + 2) * 4);
-	build_adjacency(s, s.ib0, I, V, false, corners, cursor, part);
+	build_adjacency(s, s.ib0, I, V, false, false, corners, cursor, part);
 
 	for (uint32_t v = tid; v < V; v += kBatchThreads)
 	{
@@ -679,7 +685,7 @@ __device__ __forceinline__ void simplify_patch(const BatchArgs& a, const PatchLa
 	PROFILE_PHASE(12);
 
 	// ---------------------------------------------------------------- vertex quadrics (src/simplifier.cpp:1057-1157)
-	build_adjacency(s, s.ib0, I, V, true, corners, cursor, part);
+	build_adjacency(s, s.ib0, I, V, true, true, corners, cursor, part);
 	PROFILE_PHASE(13);
 	{
 		const float factor = (options & kOptRegularizeLight) ? 1e-2f : ((options & kOptRegularize) ? 1e-1f : 1e-7f);
@@ -763,7 +769,7 @@ __device__ __forceinline__ void simplify_patch(const BatchArgs& a, const PatchLa
 			break;
 		}
 		const uint32_t T = result_count / 3;
-		build_adjacency(s, ib, result_count, V, true, corners, cursor, part);
+		build_adjacency(s, ib, result_count, V, true, false, corners, cursor, part);
 		PROFILE_PHASE(3);
 
 		// pick (src/simplifier.cpp:1310-1372): per triangle, order-preserving compaction
