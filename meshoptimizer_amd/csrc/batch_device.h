@@ -973,7 +973,11 @@ __device__ __forceinline__ void simplify_patch(const BatchArgs& a, const PatchLa
 			const uint32_t lane = tid;
 			const uint32_t goal = (result_count - target) / 3;
 			uint32_t edge_goal = goal / 2;
-			float error_goal = edge_goal < C ? 1.5f * s.cerr[s.order[edge_goal]] : FLT_MAX;
+			// cerr of the sorted candidates edge_goal .. edge_goal + 63, one per lane: the error
+			// goal follows edge_goal one step per rejection and is read from this window
+			uint32_t gbase = edge_goal;
+			float gwin = gbase + lane < C ? s.cerr[s.order[gbase + lane]] : 0.f;
+			float error_goal = edge_goal < C ? 1.5f * __int_as_float(__builtin_amdgcn_readlane(__float_as_int(gwin), 0)) : FLT_MAX;
 			uint32_t tri_collapses = 0, committed = 0, evaluated = 0;
 			bool stop = false;
 			for (uint32_t base = 0; base < C && !stop; base += 64)
@@ -1040,24 +1044,36 @@ __device__ __forceinline__ void simplify_patch(const BatchArgs& a, const PatchLa
 				const uint64_t wide_mask = __ballot(ae - ab > (uint32_t)kNearSlots);
 				uint64_t stale_mask = 0;
 				const uint32_t n = C - base < 64 ? C - base : 64;
+				const uint64_t valid = n == 64 ? ~0ull : (1ull << n) - 1;
 				PROFILE_PHASE(14);
+				// the stop test of every lane at the current scalar state; that state changes only on
+				// a rejection or a commit, so the locked candidates between two such events are
+				// passed over by one scan of the masks (they change nothing but the evaluated count)
+				uint64_t stop_mask = __ballot(err > error_limit || tri_collapses >= goal || (err > error_goal && err > result_error && tri_collapses > goal / 6));
 				for (uint32_t j = 0; j < n; ++j)
 				{
-					evaluated++;
-					const float ej = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(err), j));
-					if (ej > error_limit || tri_collapses >= goal || (ej > error_goal && ej > result_error && tri_collapses > goal / 6))
+					const uint64_t events = (stop_mask | ~lock_mask) & valid & (~0ull << j);
+					if (!events)
+					{
+						evaluated += n - j;
+						break;
+					}
+					const uint32_t jn = (uint32_t)__builtin_ctzll(events);
+					evaluated += jn - j + 1;
+					j = jn;
+					if ((stop_mask >> j) & 1)
 					{
 						stop = true;
 						break;
 					}
-					if ((lock_mask >> j) & 1)
-						continue;
+					const float ej = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(err), j));
 					const uint32_t r0j = __builtin_amdgcn_readlane(r0, j), r1j = __builtin_amdgcn_readlane(r1, j);
 					bool fj = (flip_mask >> j) & 1;
 					if ((stale_mask >> j) & 1)
 					{
 						// re-run on the current collapse_remap, one adjacency entry per lane
 						PROFILE_COUNT(25, 1);
+						PROFILE_PHASE(15);
 						fj = false;
 						const uint32_t b = __builtin_amdgcn_readlane(ab, j), e = __builtin_amdgcn_readlane(ae, j);
 						const V3 v0 = P(s, r0j), v1 = P(s, r1j);
@@ -1078,11 +1094,18 @@ __device__ __forceinline__ void simplify_patch(const BatchArgs& a, const PatchLa
 								break;
 							}
 						}
+						PROFILE_PHASE(16);
 					}
 					if (fj)
 					{
 						edge_goal++;
-						error_goal = edge_goal < C ? 1.5f * s.cerr[s.order[edge_goal]] : FLT_MAX;
+						if (edge_goal - gbase >= 64)
+						{
+							gbase = edge_goal;
+							gwin = gbase + lane < C ? s.cerr[s.order[gbase + lane]] : 0.f;
+						}
+						error_goal = edge_goal < C ? 1.5f * __int_as_float(__builtin_amdgcn_readlane(__float_as_int(gwin), edge_goal - gbase)) : FLT_MAX;
+						stop_mask = __ballot(err > error_limit || tri_collapses >= goal || (err > error_goal && err > result_error && tri_collapses > goal / 6));
 						continue;
 					}
 					// commit (src/simplifier.cpp:1573-1611) from the candidate's own lane
@@ -1118,6 +1141,7 @@ __device__ __forceinline__ void simplify_patch(const BatchArgs& a, const PatchLa
 					tri_collapses += __builtin_amdgcn_readlane((int)border, j) ? 1 : 2;
 					committed++;
 					result_error = result_error < ej ? ej : result_error;
+					stop_mask = __ballot(err > error_limit || tri_collapses >= goal || (err > error_goal && err > result_error && tri_collapses > goal / 6));
 				}
 			}
 			if (lane == 0)
