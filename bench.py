@@ -133,7 +133,7 @@ def bench_patches(args):
 
     def step():
         M.simplify_batch_device(d_dst.data_ptr(), d_cnt.data_ptr(), d_err.data_ptr(), d_idx.data_ptr(), d_io.data_ptr(),
-                                d_v.data_ptr(), d_vo.data_ptr(), stride, d_tg.data_ptr(), count, err, opts)
+                                d_v.data_ptr(), d_vo.data_ptr(), int(vo[-1]), stride, d_tg.data_ptr(), count, err, opts)
 
     for _ in range(args.warmup):
         step()

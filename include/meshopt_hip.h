@@ -32,10 +32,11 @@ size_t meshopt_hip_simplify_device(unsigned int* destination, const unsigned int
  * at destination + index_offsets[p]; result_counts[p] and result_errors[p] (may be NULL) receive
  * the index count and error.  target_error and options are shared by all meshes; Prune is not
  * accepted.  Returns 0, or -1 with meshopt_hip_last_error set.
- * The _device variant takes device pointers for every array; the other copies host arrays in
- * and out around the same launch.
+ * The _device variant takes device pointers for every array and the number of vertices the
+ * positions hold (vertex_offsets[mesh_count] must not exceed it); the other copies host arrays
+ * in and out around the same launch.
  */
-int meshopt_hip_simplify_batch_device(unsigned int* destination, unsigned int* result_counts, float* result_errors, const unsigned int* indices, const unsigned int* index_offsets, const float* vertex_positions, const unsigned int* vertex_offsets, size_t vertex_positions_stride, const unsigned int* target_index_counts, size_t mesh_count, float target_error, unsigned int options);
+int meshopt_hip_simplify_batch_device(unsigned int* destination, unsigned int* result_counts, float* result_errors, const unsigned int* indices, const unsigned int* index_offsets, const float* vertex_positions, const unsigned int* vertex_offsets, size_t vertex_count, size_t vertex_positions_stride, const unsigned int* target_index_counts, size_t mesh_count, float target_error, unsigned int options);
 int meshopt_hip_simplify_batch(unsigned int* destination, unsigned int* result_counts, float* result_errors, const unsigned int* indices, const unsigned int* index_offsets, const float* vertex_positions, const unsigned int* vertex_offsets, size_t vertex_positions_stride, const unsigned int* target_index_counts, size_t mesh_count, float target_error, unsigned int options);
 
 /* Per-pass counters of the last call made on the calling thread; the same quantities the

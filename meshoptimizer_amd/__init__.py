@@ -107,10 +107,10 @@ def library() -> ctypes.CDLL:
     lib.meshopt_hip_last_call_stats.argtypes = [_P, _P, ctypes.c_int]
     lib.meshopt_hip_last_error.restype = ctypes.c_char_p
     lib.meshopt_hip_last_error.argtypes = []
-    for name in ("meshopt_hip_simplify_batch", "meshopt_hip_simplify_batch_device"):
-        fn = getattr(lib, name)
-        fn.restype = ctypes.c_int
-        fn.argtypes = [_P, _P, _P, _P, _P, _P, _P, _SZ, _P, _SZ, _F, _U]
+    lib.meshopt_hip_simplify_batch.restype = ctypes.c_int
+    lib.meshopt_hip_simplify_batch.argtypes = [_P, _P, _P, _P, _P, _P, _P, _SZ, _P, _SZ, _F, _U]
+    lib.meshopt_hip_simplify_batch_device.restype = ctypes.c_int
+    lib.meshopt_hip_simplify_batch_device.argtypes = [_P, _P, _P, _P, _P, _P, _P, _SZ, _SZ, _P, _SZ, _F, _U]
     lib.meshopt_hip_batch_kernel_timing.restype = None
     lib.meshopt_hip_batch_kernel_timing.argtypes = [_P, _P, _P]
     lib.meshopt_hip_kernel_timing.restype = None
@@ -264,12 +264,13 @@ def batch_kernel_timing():
 
 
 def simplify_batch_device(destination_ptr, result_counts_ptr, result_errors_ptr, indices_ptr, index_offsets_ptr,
-                          positions_ptr, vertex_offsets_ptr, stride, target_counts_ptr, mesh_count,
+                          positions_ptr, vertex_offsets_ptr, vertex_count, stride, target_counts_ptr, mesh_count,
                           target_error, options=0):
-    """meshopt_hip_simplify_batch_device on device pointers (integers); raises on error."""
+    """meshopt_hip_simplify_batch_device on device pointers (integers); vertex_count is the number of
+    vertices the positions hold.  Raises on error."""
     lib = library()
     rc = lib.meshopt_hip_simplify_batch_device(destination_ptr, result_counts_ptr, result_errors_ptr, indices_ptr,
-                                               index_offsets_ptr, positions_ptr, vertex_offsets_ptr, stride,
+                                               index_offsets_ptr, positions_ptr, vertex_offsets_ptr, vertex_count, stride,
                                                target_counts_ptr, mesh_count, float(target_error), int(options))
     if rc != 0:
         raise RuntimeError(lib.meshopt_hip_last_error().decode())

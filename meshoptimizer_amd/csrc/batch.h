@@ -14,12 +14,15 @@ struct BatchArgs
 	uint32_t* destination;             // packed: mesh p writes at index_offsets[p]
 	uint32_t* result_counts;           // per mesh
 	float* result_errors;              // per mesh (may be null)
-	uint32_t* status;                  // per mesh: 0 done, 1 plain path, 2 bad index, 3 pass bound hit (internal error), 4 invalid sizes
+	uint32_t* status;                  // per mesh: 0 done, 1 plain path, 2 bad index, 3 pass bound hit (internal error), 4 invalid sizes,
+	                                   // 5 more candidates than the layout holds (run again in the last class)
 	uint32_t* passes;                  // per mesh (may be null)
 	const uint32_t* indices;           // packed local indices
 	const uint32_t* index_offsets;     // mesh_count + 1
 	const float* positions;            // packed vertices (position in the first 12 bytes)
 	const uint32_t* vertex_offsets;    // mesh_count + 1
+	size_t vertex_count;               // vertices the positions hold; every vertex_offsets[p + 1] is at most this
+	Quad* quadrics;                    // vertex_count quadrics: mesh p keeps its vertex quadrics at vertex_offsets[p]
 	size_t position_stride_floats;
 	const uint32_t* target_index_counts; // per mesh
 	size_t patch_count;
@@ -27,15 +30,17 @@ struct BatchArgs
 	uint32_t options;
 };
 
-size_t batch_lds_bytes(uint32_t vmax, uint32_t imax);
-// size classes: a mesh belongs to the first class whose vertex and index caps it fits
+size_t batch_lds_bytes(uint32_t vmax, uint32_t imax, uint32_t cmax);
+// size classes: a mesh belongs to the first class whose vertex and index caps it fits; ccap is the
+// class's candidate capacity (the last class holds the full bound, so it never returns status 5)
 struct BatchClasses
 {
 	uint32_t vcap[kBatchClasses];
 	uint32_t icap[kBatchClasses];
+	uint32_t ccap[kBatchClasses];
 };
 
-// the largest caps per class (V:I at most 1:4) whose LDS layout fits each class's share of a CU
+// the largest caps per class whose LDS layout lets 5, 4 and 2 workgroups share a CU
 BatchClasses batch_classes();
 
 // per mesh: validate sizes (status 4), pick its class (ids[c * patch_count + slot], counts[c])
@@ -43,7 +48,8 @@ BatchClasses batch_classes();
 void launch_batch_classify(hipStream_t s, const BatchArgs& a, const BatchClasses& classes, uint32_t* ids, uint32_t* counts);
 
 // one workgroup per mesh of a class (ids, *count_dev of them; grid = upper bound), every one with
-// the (vmax, imax) layout
-void launch_batch(hipStream_t s, const BatchArgs& a, uint32_t vmax, uint32_t imax, const uint32_t* ids, const uint32_t* count_dev, size_t grid);
+// the (vmax, imax, cmax) layout
+void launch_batch(hipStream_t s, const BatchArgs& a, uint32_t vmax, uint32_t imax, uint32_t cmax, const uint32_t* ids, const uint32_t* count_dev,
+                  size_t grid);
 
 } // namespace mhip

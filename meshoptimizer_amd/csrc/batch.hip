@@ -27,7 +27,7 @@ __global__ void k_batch_classify(BatchArgs a, BatchClasses c, uint32_t* ids, uin
 		return;
 	const uint32_t ib = a.index_offsets[p], ie = a.index_offsets[p + 1];
 	const uint32_t vb = a.vertex_offsets[p], ve = a.vertex_offsets[p + 1];
-	if (ie < ib || ve < vb || (ie - ib) % 3 != 0 || a.target_index_counts[p] > ie - ib)
+	if (ie < ib || ve < vb || ve > a.vertex_count || (ie - ib) % 3 != 0 || a.target_index_counts[p] > ie - ib)
 	{
 		a.status[p] = 4;
 		return;
@@ -43,24 +43,39 @@ __global__ void k_batch_classify(BatchArgs a, BatchClasses c, uint32_t* ids, uin
 	a.status[p] = 1; // too large for the LDS layouts: the plain path
 }
 
+// LDS bytes one workgroup may take so that `per_cu` of them fit a CU (160 KB, allocated in
+// 512-byte granules, the kernel's static words included)
+static uint32_t lds_limit(uint32_t per_cu)
+{
+	const uint32_t cu_lds = 160 * 1024, granule = 512, static_words = 256;
+	uint32_t share = cu_lds / per_cu / granule * granule;
+	share = share < 64 * 1024 ? share : 64 * 1024;
+	return share - static_words;
+}
+
 BatchClasses batch_classes()
 {
-	const uint32_t cu_lds = 160 * 1024;
-	const uint32_t limits[kBatchClasses] = {cu_lds / 4, cu_lds / 3 - 256, 64 * 1024 - 512};
+	// 5, 4 and 2 workgroups per CU (at most 84 VGPRs allow 5 wavefronts per SIMD); the last class
+	// takes meshes with relatively more vertices and holds full candidate arrays
+	const uint32_t limits[kBatchClasses] = {lds_limit(5), lds_limit(4), lds_limit(2)};
+	const uint32_t vdiv[kBatchClasses] = {4, 4, 3};
 	BatchClasses c{};
 	for (int k = 0; k < kBatchClasses; ++k)
 	{
+		const bool last = k == kBatchClasses - 1;
 		uint32_t vbest = 32, ibest = 96;
 		for (uint32_t icap = 96; icap < 0xffffu; icap += 96)
 		{
-			uint32_t vcap = ((icap / 4) + 31) & ~31u;
-			if (patch_layout(vcap, icap).total > limits[k])
+			uint32_t vcap = ((icap / vdiv[k]) + 31) & ~31u;
+			uint32_t ccap = last ? icap + 4 : batch_candidate_cap(icap);
+			if (patch_layout(vcap, icap, ccap).total > limits[k])
 				break;
 			vbest = vcap;
 			ibest = icap;
 		}
 		c.vcap[k] = vbest;
 		c.icap[k] = ibest;
+		c.ccap[k] = last ? ibest + 4 : batch_candidate_cap(ibest);
 	}
 	return c;
 }
@@ -83,14 +98,15 @@ extern "C" void meshopt_hip_batch_profile(unsigned long long* out, int reset)
 }
 #endif
 
-size_t batch_lds_bytes(uint32_t vmax, uint32_t imax)
+size_t batch_lds_bytes(uint32_t vmax, uint32_t imax, uint32_t cmax)
 {
-	return patch_layout(vmax, imax).total;
+	return patch_layout(vmax, imax, cmax).total;
 }
 
-void launch_batch(hipStream_t s, const BatchArgs& a, uint32_t vmax, uint32_t imax, const uint32_t* ids, const uint32_t* count_dev, size_t grid)
+void launch_batch(hipStream_t s, const BatchArgs& a, uint32_t vmax, uint32_t imax, uint32_t cmax, const uint32_t* ids, const uint32_t* count_dev,
+                  size_t grid)
 {
-	PatchLayout L = patch_layout(vmax, imax);
+	PatchLayout L = patch_layout(vmax, imax, cmax);
 	if (grid)
 		k_simplify_patches<<<(unsigned int)grid, kBatchThreads, L.total, s>>>(a, L, ids, count_dev);
 }

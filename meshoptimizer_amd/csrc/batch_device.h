@@ -85,8 +85,7 @@ __device__ unsigned long long g_batch_profile[32];
 struct PatchLayout
 {
 	uint32_t vmax, imax, cmax;
-	uint32_t ib0, ib1, px, py, pz, remap, wedge, loop, loopback, cremap, kind, lock, off, adj, vq, cv0, cv1, cerr, order, scratch, total;
-	uint32_t scratch_bytes;
+	uint32_t ib, px, py, pz, remap, wedge, loop, loopback, cremap, kind, lock, off, adj, sortbuf, region, cv0, cv1, order, cerr, total;
 };
 
 inline uint32_t align16(uint32_t x)
@@ -94,20 +93,27 @@ inline uint32_t align16(uint32_t x)
 	return (x + 15) & ~15u;
 }
 
-inline PatchLayout patch_layout(uint32_t vmax, uint32_t imax)
+// candidate arrays of a layout for meshes of up to imax indices: a pass has about one candidate
+// per edge (I / 2 on a closed manifold, up to 0.58 I on open grids); a mesh with more leaves the
+// kernel with status 5 and runs again with arrays of imax + 4, the bound boundEdgeCollapses gives
+inline uint32_t batch_candidate_cap(uint32_t imax)
+{
+	return imax * 5 / 8 + 8;
+}
+
+inline PatchLayout patch_layout(uint32_t vmax, uint32_t imax, uint32_t cmax)
 {
 	PatchLayout L{};
 	L.vmax = vmax;
 	L.imax = imax;
-	L.cmax = imax + 4;
+	L.cmax = cmax;
 	uint32_t o = 0;
 	auto take = [&](uint32_t bytes) {
 		uint32_t at = o;
 		o = align16(o + bytes);
 		return at;
 	};
-	L.ib0 = take(imax * 2);
-	L.ib1 = take(imax * 2);
+	L.ib = take(imax * 2);
 	L.px = take(vmax * 4);
 	L.py = take(vmax * 4);
 	L.pz = take(vmax * 4);
@@ -120,43 +126,43 @@ inline PatchLayout patch_layout(uint32_t vmax, uint32_t imax)
 	L.lock = take(vmax);
 	L.off = take((vmax + 2) * 2);
 	L.adj = take(imax * 4);
-	L.vq = take(vmax * 11 * 4);
-	L.cv0 = take(L.cmax * 2);
-	L.cv1 = take(L.cmax * 2);
-	L.cerr = take(L.cmax * 4);
-	L.order = take(L.cmax * 2);
-	// scratch: hash table / corner ids / sort ping-pong / scan partials
+	// the sort's ping-pong half and digit counts (64 offsets + 4 x 64 wavefront counts)
+	L.sortbuf = take(align16(cmax * 2) + 5 * 64 * 4);
+	// one region for uses whose lifetimes do not overlap: the candidate arrays (pick to commit)
+	// and the scratch of setup (position hash, wedge maxima), of build_adjacency (cursor, corner
+	// ids; the ordered welded corners live on through the quadrics) and of the edge-loop update
 	uint32_t hsize = 16;
 	while (hsize < 2 * vmax + 2)
 		hsize <<= 1;
-	uint32_t hash = 4 * hsize;
-	uint32_t sortb = align16(L.cmax * 2) + 5 * 64 * 4;
-	uint32_t corners = imax * 2 + (vmax + 2) * 4;
-	uint32_t sb = hash;
-	sb = sb > sortb ? sb : sortb;
-	sb = sb > corners ? sb : corners;
-	sb += 256 * 4;
-	L.scratch_bytes = sb;
-	L.scratch = take(sb);
+	const uint32_t cand = 3 * align16(cmax * 2) + align16(cmax * 4);
+	const uint32_t uses[4] = {4 * hsize, (vmax + 2) * 4 + imax * 2, 2 * align16(vmax * 2), vmax * 4};
+	uint32_t rb = cand;
+	for (uint32_t u : uses)
+		rb = rb > u ? rb : u;
+	L.region = take(rb);
+	L.cv0 = L.region;
+	L.cv1 = L.cv0 + align16(cmax * 2);
+	L.order = L.cv1 + align16(cmax * 2);
+	L.cerr = L.order + align16(cmax * 2);
 	L.total = o;
 	return L;
 }
 
 struct PatchSmem
 {
-	uint16_t *ib0, *ib1, *remap, *wedge, *loop, *loopback, *cremap, *off, *cv0, *cv1, *order;
-	float *px, *py, *pz, *vq, *cerr;
+	uint16_t *ib, *remap, *wedge, *loop, *loopback, *cremap, *off, *cv0, *cv1, *order, *sortbuf;
+	float *px, *py, *pz, *cerr;
+	Quad* gq; // the mesh's vertex quadrics in global memory (read and written by this workgroup only)
 	uint8_t *kind, *lock;
 	uint32_t* adj; // next | prev << 16
-	uint8_t* scratch;
+	uint8_t* scratch; // the region the candidate arrays share with setup and adjacency scratch
 	uint32_t vmax, imax, cmax;
 };
 
 __device__ __forceinline__ PatchSmem bind(uint8_t* base, const PatchLayout& L)
 {
 	PatchSmem s;
-	s.ib0 = (uint16_t*)(base + L.ib0);
-	s.ib1 = (uint16_t*)(base + L.ib1);
+	s.ib = (uint16_t*)(base + L.ib);
 	s.px = (float*)(base + L.px);
 	s.py = (float*)(base + L.py);
 	s.pz = (float*)(base + L.pz);
@@ -169,12 +175,13 @@ __device__ __forceinline__ PatchSmem bind(uint8_t* base, const PatchLayout& L)
 	s.lock = base + L.lock;
 	s.off = (uint16_t*)(base + L.off);
 	s.adj = (uint32_t*)(base + L.adj);
-	s.vq = (float*)(base + L.vq);
+	s.sortbuf = (uint16_t*)(base + L.sortbuf);
 	s.cv0 = (uint16_t*)(base + L.cv0);
 	s.cv1 = (uint16_t*)(base + L.cv1);
-	s.cerr = (float*)(base + L.cerr);
 	s.order = (uint16_t*)(base + L.order);
-	s.scratch = base + L.scratch;
+	s.cerr = (float*)(base + L.cerr);
+	s.scratch = base + L.region;
+	s.gq = nullptr;
 	s.vmax = L.vmax;
 	s.imax = L.imax;
 	s.cmax = L.cmax;
@@ -188,37 +195,12 @@ __device__ __forceinline__ V3 P(const PatchSmem& s, uint32_t i)
 
 __device__ __forceinline__ Quad QL(const PatchSmem& s, uint32_t v)
 {
-	const uint32_t n = s.vmax;
-	Quad q;
-	q.a00 = s.vq[0 * n + v];
-	q.a11 = s.vq[1 * n + v];
-	q.a22 = s.vq[2 * n + v];
-	q.a10 = s.vq[3 * n + v];
-	q.a20 = s.vq[4 * n + v];
-	q.a21 = s.vq[5 * n + v];
-	q.b0 = s.vq[6 * n + v];
-	q.b1 = s.vq[7 * n + v];
-	q.b2 = s.vq[8 * n + v];
-	q.c = s.vq[9 * n + v];
-	q.w = s.vq[10 * n + v];
-	q.pad = 0.f;
-	return q;
+	return load_quad(s.gq, v);
 }
 
 __device__ __forceinline__ void QS(const PatchSmem& s, uint32_t v, const Quad& q)
 {
-	const uint32_t n = s.vmax;
-	s.vq[0 * n + v] = q.a00;
-	s.vq[1 * n + v] = q.a11;
-	s.vq[2 * n + v] = q.a22;
-	s.vq[3 * n + v] = q.a10;
-	s.vq[4 * n + v] = q.a20;
-	s.vq[5 * n + v] = q.a21;
-	s.vq[6 * n + v] = q.b0;
-	s.vq[7 * n + v] = q.b1;
-	s.vq[8 * n + v] = q.b2;
-	s.vq[9 * n + v] = q.c;
-	s.vq[10 * n + v] = q.w;
+	store_quad(s.gq, v, q);
 }
 
 // exclusive scan of one value per thread across the workgroup; `part` holds kBatchThreads/64 words
@@ -396,6 +378,7 @@ __device__ __forceinline__ void simplify_patch(const BatchArgs& a, const PatchLa
 	}
 
 	PatchSmem s = bind(smem, L);
+	s.gq = a.quadrics + vb_begin;
 	uint32_t* cursor = (uint32_t*)s.scratch; // V + 1 words
 	PROFILE_DECL;
 
@@ -408,7 +391,7 @@ __device__ __forceinline__ void simplify_patch(const BatchArgs& a, const PatchLa
 		uint32_t v = a.indices[ib_begin + j];
 		if (v >= V)
 			atomicOr(&sh[0], 1u);
-		s.ib0[j] = (uint16_t)(v < V ? v : 0);
+		s.ib[j] = (uint16_t)(v < V ? v : 0);
 	}
 	const size_t stride = a.position_stride_floats;
 	for (uint32_t i = tid; i < V; i += kBatchThreads)
@@ -512,7 +495,7 @@ __device__ __forceinline__ void simplify_patch(const BatchArgs& a, const PatchLa
 	PROFILE_PHASE(0);
 	// ---------------------------------------------------------------- unwelded adjacency, classification
 	uint16_t* corners = (uint16_t*)(s.scratch + (V + 2) * 4);
-	build_adjacency(s, s.ib0, I, V, false, false, corners, cursor, part);
+	build_adjacency(s, s.ib, I, V, false, false, corners, cursor, part);
 
 	for (uint32_t v = tid; v < V; v += kBatchThreads)
 	{
@@ -685,7 +668,7 @@ __device__ __forceinline__ void simplify_patch(const BatchArgs& a, const PatchLa
 	PROFILE_PHASE(12);
 
 	// ---------------------------------------------------------------- vertex quadrics (src/simplifier.cpp:1057-1157)
-	build_adjacency(s, s.ib0, I, V, true, true, corners, cursor, part);
+	build_adjacency(s, s.ib, I, V, true, true, corners, cursor, part);
 	PROFILE_PHASE(13);
 	{
 		const float factor = (options & kOptRegularizeLight) ? 1e-2f : ((options & kOptRegularize) ? 1e-1f : 1e-7f);
@@ -700,7 +683,7 @@ __device__ __forceinline__ void simplify_patch(const BatchArgs& a, const PatchLa
 				{
 					uint32_t t = corners[p] / 3;
 					Quad F;
-					quad_from_triangle(F, P(s, s.ib0[t * 3 + 0]), P(s, s.ib0[t * 3 + 1]), P(s, s.ib0[t * 3 + 2]), 1.f);
+					quad_from_triangle(F, P(s, s.ib[t * 3 + 0]), P(s, s.ib[t * 3 + 1]), P(s, s.ib[t * 3 + 2]), 1.f);
 					quad_add(Q, F);
 				}
 				{
@@ -716,7 +699,7 @@ __device__ __forceinline__ void simplify_patch(const BatchArgs& a, const PatchLa
 					if (t == last_t)
 						continue;
 					last_t = t;
-					uint32_t tri[3] = {s.ib0[t * 3 + 0], s.ib0[t * 3 + 1], s.ib0[t * 3 + 2]};
+					uint32_t tri[3] = {s.ib[t * 3 + 0], s.ib[t * 3 + 1], s.ib[t * 3 + 2]};
 					for (int ed = 0; ed < 3; ++ed)
 					{
 						uint32_t i0 = tri[ed], i1 = tri[ed == 2 ? 0 : ed + 1], i2 = tri[ed == 0 ? 2 : ed - 1];
@@ -751,8 +734,7 @@ __device__ __forceinline__ void simplify_patch(const BatchArgs& a, const PatchLa
 	const float error_limit = (a.target_error * a.target_error) / (error_scale * error_scale);
 
 	// ---------------------------------------------------------------- pass loop (src/simplifier.cpp:2493-2541)
-	uint16_t* ib = s.ib0;
-	uint16_t* ib_alt = s.ib1;
+	uint16_t* ib = s.ib;
 	uint32_t result_count = I;
 	float result_error = 0.f;
 	uint32_t passes = 0;
@@ -760,7 +742,7 @@ __device__ __forceinline__ void simplify_patch(const BatchArgs& a, const PatchLa
 	// every pass that commits removes at least one triangle, so I / 3 + 1 passes always suffice;
 	// the bound is an exit every wavefront reaches even on inconsistent state
 	const uint32_t pass_limit = I / 3 + 1;
-	bool overrun = false;
+	bool overrun = false, overflow = false;
 	while (result_count > target)
 	{
 		if (passes >= pass_limit)
@@ -833,19 +815,21 @@ __device__ __forceinline__ void simplify_patch(const BatchArgs& a, const PatchLa
 				// the running offset only grows, so the taken triangles form a prefix
 				if (t < T && at + 3 <= capacity)
 				{
-					if (cnt > 0)
+					// past the layout's candidate arrays nothing is written; the count then
+					// exceeds cmax and the mesh leaves with status 5
+					if (cnt > 0 && at < s.cmax)
 					{
 						s.cv0[at] = (uint16_t)(e0 & 0xffffu);
 						s.cv1[at] = (uint16_t)(e0 >> 16);
 						s.cerr[at] = __uint_as_float(bidi & 1u);
 					}
-					if (cnt > 1)
+					if (cnt > 1 && at + 1 < s.cmax)
 					{
 						s.cv0[at + 1] = (uint16_t)(e1 & 0xffffu);
 						s.cv1[at + 1] = (uint16_t)(e1 >> 16);
 						s.cerr[at + 1] = __uint_as_float((bidi >> 1) & 1u);
 					}
-					if (cnt > 2)
+					if (cnt > 2 && at + 2 < s.cmax)
 					{
 						s.cv0[at + 2] = (uint16_t)(e2 & 0xffffu);
 						s.cv1[at + 2] = (uint16_t)(e2 >> 16);
@@ -860,6 +844,11 @@ __device__ __forceinline__ void simplify_patch(const BatchArgs& a, const PatchLa
 			}
 			C = sh[1];
 			__syncthreads();
+		}
+		if (C > s.cmax)
+		{
+			overflow = true;
+			break;
 		}
 
 		PROFILE_PHASE(4);
@@ -885,11 +874,11 @@ __device__ __forceinline__ void simplify_patch(const BatchArgs& a, const PatchLa
 		// stable sort by the 12-bit key (src/simplifier.cpp:1452-1496): LSD, two 6-bit digits
 		{
 			uint16_t* src = s.order;
-			uint16_t* dst = (uint16_t*)s.scratch;
+			uint16_t* dst = s.sortbuf;
 			for (uint32_t c = tid; c < C; c += kBatchThreads)
 				src[c] = (uint16_t)c;
 			__syncthreads();
-			uint32_t* hist = (uint32_t*)(s.scratch + ((s.cmax * 2 + 15) & ~15u)); // 64 digit offsets + [4][64] wave counts
+			uint32_t* hist = (uint32_t*)((uint8_t*)s.sortbuf + ((s.cmax * 2 + 15) & ~15u)); // 64 digit offsets + [4][64] wave counts
 			for (int pass = 0; pass < 2; ++pass)
 			{
 				const int shift = pass * 6;
@@ -1209,7 +1198,9 @@ __device__ __forceinline__ void simplify_patch(const BatchArgs& a, const PatchLa
 		}
 
 		PROFILE_PHASE(8);
-		// remapIndexBuffer (src/simplifier.cpp:1811-1843): stable compaction into the other buffer
+		// remapIndexBuffer (src/simplifier.cpp:1811-1843): stable compaction in place.  Each chunk's
+		// triangles are read before the scan's barriers and written after them, at or below the
+		// chunk's own range, so nothing unread is overwritten
 		{
 			uint32_t carry = 0;
 			for (uint32_t base = 0; base < T; base += kBatchThreads)
@@ -1230,16 +1221,13 @@ __device__ __forceinline__ void simplify_patch(const BatchArgs& a, const PatchLa
 				if (keep)
 				{
 					uint32_t o = (carry + ex) * 3;
-					ib_alt[o + 0] = (uint16_t)v[0];
-					ib_alt[o + 1] = (uint16_t)v[1];
-					ib_alt[o + 2] = (uint16_t)v[2];
+					ib[o + 0] = (uint16_t)v[0];
+					ib[o + 1] = (uint16_t)v[1];
+					ib[o + 2] = (uint16_t)v[2];
 				}
 				carry += tot;
 				__syncthreads();
 			}
-			uint16_t* t = ib;
-			ib = ib_alt;
-			ib_alt = t;
 			result_count = carry * 3;
 		}
 		__syncthreads();
@@ -1247,6 +1235,14 @@ __device__ __forceinline__ void simplify_patch(const BatchArgs& a, const PatchLa
 	}
 	PROFILE_COUNT(23, passes);
 	PROFILE_PHASE(10);
+
+	if (overflow)
+	{
+		// more candidates than this layout holds: the host runs the mesh again with full arrays
+		if (tid == 0)
+			a.status[patch] = 5;
+		return;
+	}
 
 	// ---------------------------------------------------------------- output (src/simplifier.cpp:2601-2625)
 	const bool debug = (options & kOptInternalDebug) != 0;

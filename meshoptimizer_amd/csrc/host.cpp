@@ -119,6 +119,8 @@ struct Context
 	size_t batch_dev_words = 0;
 	uint32_t* batch_pinned = nullptr; // batched calls: status read-back
 	size_t batch_pinned_words = 0;
+	Quad* batch_quadrics = nullptr; // batched calls: one vertex quadric per vertex of the batch
+	size_t batch_quadric_count = 0;
 	hipStream_t aux[kBatchClasses - 1] = {}; // batch size classes beyond the first
 	hipEvent_t fork_ev = nullptr, join_ev[kBatchClasses - 1] = {};
 
@@ -143,6 +145,19 @@ struct Context
 			HIP_CHECK(hipHostMalloc((void**)&batch_pinned, meshes * sizeof(uint32_t), hipHostMallocDefault));
 			batch_pinned_words = meshes;
 		}
+	}
+
+	void ensure_batch_quadrics(size_t vertices)
+	{
+		if (vertices <= batch_quadric_count)
+			return;
+		if (batch_quadrics)
+			HIP_CHECK(hipFree(batch_quadrics));
+		batch_quadrics = nullptr;
+		batch_quadric_count = 0;
+		size_t grow = vertices + vertices / 4;
+		HIP_CHECK(hipMalloc((void**)&batch_quadrics, grow * sizeof(Quad)));
+		batch_quadric_count = grow;
 	}
 
 	void ensure_aux_streams()
@@ -200,9 +215,12 @@ struct Context
 			(void)hipFree(batch_dev);
 		if (batch_pinned)
 			(void)hipHostFree(batch_pinned);
+		if (batch_quadrics)
+			(void)hipFree(batch_quadrics);
 		batch_dev = nullptr;
 		batch_pinned = nullptr;
-		batch_dev_words = batch_pinned_words = 0;
+		batch_quadrics = nullptr;
+		batch_dev_words = batch_pinned_words = batch_quadric_count = 0;
 		for (int k = 0; k < kBatchClasses - 1; ++k)
 		{
 			if (aux[k])
@@ -966,7 +984,7 @@ namespace mhip
 // are read back only then (h_* may be null: device-resident offsets).
 static int simplify_batch_on_device(Context& ctx, unsigned int* destination, unsigned int* result_counts, float* result_errors,
                                     const unsigned int* indices, const unsigned int* index_offsets, const float* positions,
-                                    const unsigned int* vertex_offsets, size_t stride, const unsigned int* target_index_counts,
+                                    const unsigned int* vertex_offsets, size_t vertex_count, size_t stride, const unsigned int* target_index_counts,
                                     size_t patch_count, float target_error, unsigned int options, const unsigned int* h_index_offsets,
                                     const unsigned int* h_vertex_offsets, const unsigned int* h_targets)
 {
@@ -996,6 +1014,8 @@ static int simplify_batch_on_device(Context& ctx, unsigned int* destination, uns
 	a.index_offsets = index_offsets;
 	a.positions = positions;
 	a.vertex_offsets = vertex_offsets;
+	a.vertex_count = vertex_count;
+	a.quadrics = nullptr;
 	a.position_stride_floats = stride / sizeof(float);
 	a.target_index_counts = target_index_counts;
 	a.patch_count = patch_count;
@@ -1005,6 +1025,8 @@ static int simplify_batch_on_device(Context& ctx, unsigned int* destination, uns
 	const bool batched = !(options & kOptSparse);
 	if (batched)
 	{
+		ctx.ensure_batch_quadrics(vertex_count ? vertex_count : 1);
+		a.quadrics = ctx.batch_quadrics;
 		const bool timed = ctx.timing;
 		if (timed && !ctx.batch_ev[0])
 		{
@@ -1022,7 +1044,7 @@ static int simplify_batch_on_device(Context& ctx, unsigned int* destination, uns
 			hipStream_t cs = c == 0 ? s : ctx.aux[c - 1];
 			if (c)
 				HIP_CHECK(hipStreamWaitEvent(cs, ctx.fork_ev, 0));
-			launch_batch(cs, a, classes.vcap[c], classes.icap[c], ids + (size_t)c * P, counts + c, P);
+			launch_batch(cs, a, classes.vcap[c], classes.icap[c], classes.ccap[c], ids + (size_t)c * P, counts + c, P);
 			if (c)
 			{
 				HIP_CHECK(hipEventRecord(ctx.join_ev[c - 1], cs));
@@ -1042,6 +1064,24 @@ static int simplify_batch_on_device(Context& ctx, unsigned int* destination, uns
 			ctx.batch_launches += 1;
 			ctx.batch_meshes += patch_count;
 		}
+		// meshes with more candidates than their class's arrays hold (status 5) run again in the
+		// last class, whose arrays hold the full bound; its workgroups take the plain-path exit
+		// (status 1) for meshes beyond its vertex or index caps
+		std::vector<uint32_t> retry;
+		for (size_t p = 0; p < P; ++p)
+			if (h_status[p] == 5)
+				retry.push_back((uint32_t)p);
+		if (!retry.empty())
+		{
+			const int c = kBatchClasses - 1;
+			const uint32_t n = (uint32_t)retry.size();
+			HIP_CHECK(hipMemcpyAsync(ids, retry.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+			HIP_CHECK(hipMemcpyAsync(counts, &n, sizeof(uint32_t), hipMemcpyHostToDevice, s));
+			launch_batch(s, a, classes.vcap[c], classes.icap[c], classes.ccap[c], ids, counts, n);
+			HIP_CHECK(hipGetLastError());
+			HIP_CHECK(hipMemcpyAsync(h_status, status, P * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+			HIP_CHECK(hipStreamSynchronize(s));
+		}
 	}
 	else
 		std::fill(h_status, h_status + P, 1u);
@@ -1055,6 +1095,8 @@ static int simplify_batch_on_device(Context& ctx, unsigned int* destination, uns
 			throw std::invalid_argument("meshopt_hip_simplify_batch: invalid index count, target or offsets");
 		if (h_status[p] == 3)
 			throw std::runtime_error("meshopt_hip_simplify_batch: internal error (pass bound exceeded)");
+		if (h_status[p] == 5)
+			throw std::runtime_error("meshopt_hip_simplify_batch: internal error (candidate arrays of the last class exceeded)");
 		fallback |= h_status[p] == 1;
 	}
 	if (!fallback)
@@ -1111,8 +1153,8 @@ static int simplify_batch_on_device(Context& ctx, unsigned int* destination, uns
 
 extern "C" int meshopt_hip_simplify_batch_device(unsigned int* destination, unsigned int* result_counts, float* result_errors, const unsigned int* indices,
                                                  const unsigned int* index_offsets, const float* vertex_positions, const unsigned int* vertex_offsets,
-                                                 size_t vertex_positions_stride, const unsigned int* target_index_counts, size_t mesh_count,
-                                                 float target_error, unsigned int options)
+                                                 size_t vertex_count, size_t vertex_positions_stride, const unsigned int* target_index_counts,
+                                                 size_t mesh_count, float target_error, unsigned int options)
 {
 	try
 	{
@@ -1120,7 +1162,8 @@ extern "C" int meshopt_hip_simplify_batch_device(unsigned int* destination, unsi
 		std::lock_guard<std::mutex> guard(ctx.mutex);
 		ctx.ensure_init();
 		simplify_batch_on_device(ctx, destination, result_counts, result_errors, indices, index_offsets, vertex_positions, vertex_offsets,
-		                         vertex_positions_stride, target_index_counts, mesh_count, target_error, options, nullptr, nullptr, nullptr);
+		                         vertex_count, vertex_positions_stride, target_index_counts, mesh_count, target_error, options, nullptr, nullptr,
+		                         nullptr);
 		t_last_error.clear();
 		return 0;
 	}
@@ -1164,7 +1207,7 @@ extern "C" int meshopt_hip_simplify_batch(unsigned int* destination, unsigned in
 		simplify_batch_on_device(ctx, reinterpret_cast<unsigned int*>(d + off_dst), reinterpret_cast<unsigned int*>(d + off_rc),
 		                         reinterpret_cast<float*>(d + off_re), reinterpret_cast<const unsigned int*>(d + off_idx),
 		                         reinterpret_cast<const unsigned int*>(d + off_io), reinterpret_cast<const float*>(d + off_pos),
-		                         reinterpret_cast<const unsigned int*>(d + off_vo), vertex_positions_stride,
+		                         reinterpret_cast<const unsigned int*>(d + off_vo), V, vertex_positions_stride,
 		                         reinterpret_cast<const unsigned int*>(d + off_tg), mesh_count, target_error, options, index_offsets, vertex_offsets,
 		                         target_index_counts);
 		if (mesh_count)

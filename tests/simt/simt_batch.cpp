@@ -184,6 +184,7 @@ struct GuardedImage
 };
 
 static uint64_t g_rng = 0;
+static unsigned int g_retries = 0;
 
 static uint64_t next_random()
 {
@@ -278,15 +279,22 @@ extern "C" int simt_simplify_batch(unsigned int* destination, unsigned int* resu
 	a.index_offsets = index_offsets;
 	a.positions = positions;
 	a.vertex_offsets = vertex_offsets;
+	a.vertex_count = count ? vertex_offsets[count] : 0;
+	// the quadric buffer in global memory, undefined at the start like a fresh device allocation
+	std::vector<mhip::Quad> quadrics(a.vertex_count + 1);
+	for (auto& q : quadrics)
+		q.a00 = q.a11 = q.a22 = q.a10 = q.a20 = q.a21 = q.b0 = q.b1 = q.b2 = q.c = q.w = q.pad = (float)(simt::next_random() % 1000) - 500.f;
+	a.quadrics = quadrics.data();
 	a.position_stride_floats = stride / sizeof(float);
 	a.target_index_counts = targets;
 	a.patch_count = count;
 	a.target_error = target_error;
 	a.options = options;
-	mhip::PatchLayout L = mhip::patch_layout(vmax, imax);
-
-	simt::GuardedImage image(L.total);
-	uint8_t* smem = image.data;
+	// the host's two layouts for this size: capped candidate arrays first, full arrays for a
+	// mesh that leaves with status 5; each LDS image ends at its own guard page
+	const mhip::PatchLayout layouts[2] = {mhip::patch_layout(vmax, imax, mhip::batch_candidate_cap(imax)), mhip::patch_layout(vmax, imax, imax + 4)};
+	simt::GuardedImage image0(layouts[0].total), image1(layouts[1].total);
+	uint8_t* images[2] = {image0.data, image1.data};
 	uint32_t part[mhip::kBatchThreads / 64];
 	uint32_t sh[16];
 	uint64_t box[6];
@@ -299,26 +307,41 @@ extern "C" int simt_simplify_batch(unsigned int* destination, unsigned int* resu
 		alarm((unsigned int)watchdog_s);
 	}
 	int rc = 0;
+	simt::g_retries = 0;
 	for (size_t p = 0; p < count && rc == 0; ++p)
 	{
-		if (mode & 1)
+		for (int attempt = 0; attempt < 2 && rc == 0; ++attempt)
 		{
-			for (size_t b = 0; b < L.total; ++b)
-				smem[b] = (uint8_t)simt::next_random();
-			for (auto& w : sh)
-				w = (uint32_t)simt::next_random();
-			for (auto& w : part)
-				w = (uint32_t)simt::next_random();
+			const mhip::PatchLayout& L = layouts[attempt];
+			uint8_t* smem = images[attempt];
+			if (mode & 1)
+			{
+				for (size_t b = 0; b < L.total; ++b)
+					smem[b] = (uint8_t)simt::next_random();
+				for (auto& w : sh)
+					w = (uint32_t)simt::next_random();
+				for (auto& w : part)
+					w = (uint32_t)simt::next_random();
+			}
+			simt::g_task = simt::Task{&a, &L, (uint32_t)p, smem, part, sh, box, shf};
+			rc = simt::run_group(mhip::kBatchThreads, (mode & 2) != 0);
+			if (status[p] != 5)
+				break;
+			simt::g_retries++;
 		}
-		simt::g_task = simt::Task{&a, &L, (uint32_t)p, smem, part, sh, box, shf};
-		rc = simt::run_group(mhip::kBatchThreads, (mode & 2) != 0);
 	}
 	if (watchdog_s > 0)
 		alarm(0);
 	return rc;
 }
 
-extern "C" unsigned int simt_layout_bytes(unsigned int vmax, unsigned int imax)
+extern "C" unsigned int simt_layout_bytes(unsigned int vmax, unsigned int imax, unsigned int cmax)
 {
-	return mhip::patch_layout(vmax, imax).total;
+	return mhip::patch_layout(vmax, imax, cmax).total;
+}
+
+// meshes the last simt_simplify_batch call ran again with full candidate arrays
+extern "C" unsigned int simt_retries()
+{
+	return simt::g_retries;
 }

@@ -37,6 +37,8 @@ def simt():
     lib.simt_simplify_batch.restype = ctypes.c_int
     lib.simt_simplify_batch.argtypes = [P, P, P, P, P, P, P, P, P, SZ, P, SZ, ctypes.c_float, ctypes.c_uint, ctypes.c_uint,
                                         ctypes.c_uint, ctypes.c_int, ctypes.c_uint, ctypes.c_ulonglong]
+    lib.simt_retries.restype = ctypes.c_uint
+    lib.simt_retries.argtypes = []
     return lib
 
 
@@ -85,3 +87,16 @@ def test_random_meshes(simt, seed):
         meshes.append((ib, v))
         targets.append((ib.size // 3 * int(rng.integers(0, 4)) // 4) * 3)
     check(simt, meshes, targets, float(rng.choice([1e-3, 1e-2, 1e-1, 1.0])), opts, seed=seed)
+
+
+def test_candidate_overflow_runs_again(simt):
+    """Separate triangles make about one candidate per index, more than the capped candidate
+    arrays hold: such a mesh leaves with status 5 and runs again with full arrays."""
+    rng = np.random.default_rng(77)
+    ntri = 60
+    v = rng.random((ntri * 3, 3), dtype=np.float32)
+    ib = np.arange(ntri * 3, dtype=np.uint32)
+    g = W.grid(5)  # smaller than the soup: the soup sets the layout's index cap
+    meshes = [(ib, v), (g.indices, g.vertices)]
+    check(simt, meshes, [ib.size // 2 // 3 * 3, W.target_count(g.index_count, 1, 2)], 1.0, 0)
+    assert simt.simt_retries() >= 1
