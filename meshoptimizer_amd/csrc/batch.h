@@ -40,7 +40,7 @@ struct BatchClasses
 	uint32_t ccap[kBatchClasses];
 };
 
-// the largest caps per class whose LDS layout lets 5, 4 and 2 workgroups share a CU
+// the largest caps per class whose LDS layout lets 6, 5 and 2 workgroups share a CU
 BatchClasses batch_classes();
 
 // per mesh: validate sizes (status 4), pick its class (ids[c * patch_count + slot], counts[c])

@@ -5,8 +5,9 @@ namespace mhip
 {
 
 // One workgroup per mesh of one LDS size class; ids lists the class's *count meshes (the grid
-// is an upper bound, workgroups past the count leave at once).
-__global__ void __launch_bounds__(kBatchThreads) k_simplify_patches(BatchArgs a, PatchLayout L, const uint32_t* ids, const uint32_t* count)
+// is an upper bound, workgroups past the count leave at once).  At most 80 VGPRs, so that six
+// wavefronts share a SIMD and six workgroups a CU when their LDS layouts allow it.
+__global__ void __launch_bounds__(kBatchThreads) __attribute__((amdgpu_waves_per_eu(6))) k_simplify_patches(BatchArgs a, PatchLayout L, const uint32_t* ids, const uint32_t* count)
 {
 	extern __shared__ __align__(16) uint8_t smem[];
 	__shared__ uint32_t part[kBatchThreads / 64];
@@ -55,9 +56,9 @@ static uint32_t lds_limit(uint32_t per_cu)
 
 BatchClasses batch_classes()
 {
-	// 5, 4 and 2 workgroups per CU (at most 84 VGPRs allow 5 wavefronts per SIMD); the last class
-	// takes meshes with relatively more vertices and holds full candidate arrays
-	const uint32_t limits[kBatchClasses] = {lds_limit(5), lds_limit(4), lds_limit(2)};
+	// 6, 5 and 2 workgroups per CU (the kernel's 80 VGPRs allow 6 wavefronts per SIMD); the last
+	// class takes meshes with relatively more vertices and holds full candidate arrays
+	const uint32_t limits[kBatchClasses] = {lds_limit(6), lds_limit(5), lds_limit(2)};
 	const uint32_t vdiv[kBatchClasses] = {4, 4, 3};
 	BatchClasses c{};
 	for (int k = 0; k < kBatchClasses; ++k)

@@ -383,23 +383,65 @@ __device__ __forceinline__ void simplify_patch(const BatchArgs& a, const PatchLa
 	PROFILE_DECL;
 
 	// ---------------------------------------------------------------- load
+	// each thread issues a batch of global loads before it uses any of them, so the memory
+	// latency is paid once per batch rather than once per element
 	if (tid == 0)
 		sh[0] = 0;
 	__syncthreads();
-	for (uint32_t j = tid; j < I; j += kBatchThreads)
 	{
-		uint32_t v = a.indices[ib_begin + j];
-		if (v >= V)
+		const size_t stride = a.position_stride_floats;
+		constexpr int kPosBatch = 2, kIdxBatch = 8;
+		for (uint32_t i0 = tid; i0 < V; i0 += kPosBatch * kBatchThreads)
+		{
+			float x[kPosBatch], y[kPosBatch], z[kPosBatch];
+#pragma unroll
+			for (int u = 0; u < kPosBatch; ++u)
+			{
+				const uint32_t i = i0 + u * kBatchThreads;
+				x[u] = y[u] = z[u] = 0.f;
+				if (i < V)
+				{
+					const float* p = a.positions + (size_t)(vb_begin + i) * stride;
+					x[u] = p[0];
+					y[u] = p[1];
+					z[u] = p[2];
+				}
+			}
+#pragma unroll
+			for (int u = 0; u < kPosBatch; ++u)
+			{
+				const uint32_t i = i0 + u * kBatchThreads;
+				if (i < V)
+				{
+					s.px[i] = x[u];
+					s.py[i] = y[u];
+					s.pz[i] = z[u];
+				}
+			}
+		}
+		bool bad = false;
+		for (uint32_t j0 = tid; j0 < I; j0 += kIdxBatch * kBatchThreads)
+		{
+			uint32_t v[kIdxBatch];
+#pragma unroll
+			for (int u = 0; u < kIdxBatch; ++u)
+			{
+				const uint32_t j = j0 + u * kBatchThreads;
+				v[u] = j < I ? a.indices[ib_begin + j] : 0u;
+			}
+#pragma unroll
+			for (int u = 0; u < kIdxBatch; ++u)
+			{
+				const uint32_t j = j0 + u * kBatchThreads;
+				if (j < I)
+				{
+					bad |= v[u] >= V;
+					s.ib[j] = (uint16_t)(v[u] < V ? v[u] : 0);
+				}
+			}
+		}
+		if (bad)
 			atomicOr(&sh[0], 1u);
-		s.ib[j] = (uint16_t)(v < V ? v : 0);
-	}
-	const size_t stride = a.position_stride_floats;
-	for (uint32_t i = tid; i < V; i += kBatchThreads)
-	{
-		const float* p = a.positions + (size_t)(vb_begin + i) * stride;
-		s.px[i] = p[0];
-		s.py[i] = p[1];
-		s.pz[i] = p[2];
 	}
 	__syncthreads();
 	if (sh[0])
