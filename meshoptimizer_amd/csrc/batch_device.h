@@ -796,7 +796,10 @@ __device__ __forceinline__ void simplify_patch(const BatchArgs& a, const PatchLa
 		build_adjacency(s, ib, result_count, V, true, false, corners, cursor, part);
 		PROFILE_PHASE(3);
 
-		// pick (src/simplifier.cpp:1310-1372): per triangle, order-preserving compaction
+		// pick (src/simplifier.cpp:1310-1372) and rank (src/simplifier.cpp:1374-1450) in one pass over
+		// the triangles: each thread finds its triangle's candidates, fetches the quadrics of the
+		// corners that can be a collapse source while the edge checks run, and writes each taken
+		// candidate once, oriented and with its error; order-preserving compaction
 		uint32_t C = 0;
 		if (tid == 0)
 			sh[1] = 0;
@@ -807,20 +810,39 @@ __device__ __forceinline__ void simplify_patch(const BatchArgs& a, const PatchLa
 			{
 				uint32_t t = base + tid;
 				// up to three candidates per triangle, packed v0 | v1 << 16, in named registers
-				uint32_t e0 = 0, e1 = 0, e2 = 0, bidi = 0;
+				uint32_t e0 = 0, e1 = 0, e2 = 0;
+				float f0 = 0.f, f1 = 0.f, f2 = 0.f;
 				int cnt = 0;
 				if (t < T)
 				{
-					uint32_t tri[3] = {ib[t * 3 + 0], ib[t * 3 + 1], ib[t * 3 + 2]};
+					const uint32_t tri[3] = {ib[t * 3 + 0], ib[t * 3 + 1], ib[t * 3 + 2]};
+					const uint32_t rv[3] = {s.remap[tri[0]], s.remap[tri[1]], s.remap[tri[2]]};
+					const uint8_t kv[3] = {s.kind[tri[0]], s.kind[tri[1]], s.kind[tri[2]]};
+					const V3 pv[3] = {P(s, tri[0]), P(s, tri[1]), P(s, tri[2])};
+					// collapse errors of each corner toward its next and its previous corner (a
+					// locked corner is never a collapse source); each quadric is dropped once used
+					float to_next[3], to_prev[3];
+#pragma unroll
+					for (int k = 0; k < 3; ++k)
+					{
+						to_next[k] = to_prev[k] = 0.f;
+						if (kv[k] != kLocked)
+						{
+							const Quad q = QL(s, rv[k]);
+							to_next[k] = quad_error(q, pv[k == 2 ? 0 : k + 1]);
+							to_prev[k] = quad_error(q, pv[k == 0 ? 2 : k - 1]);
+						}
+					}
 #pragma unroll
 					for (int ed = 0; ed < 3; ++ed)
 					{
-						uint32_t i0 = tri[ed], i1 = tri[ed == 2 ? 0 : ed + 1];
-						uint32_t r0 = s.remap[i0], r1 = s.remap[i1];
+						const int a = ed, b = ed == 2 ? 0 : ed + 1;
+						const uint32_t i0 = tri[a], i1 = tri[b];
+						const uint32_t r0 = rv[a], r1 = rv[b];
 						if (r0 == r1)
 							continue;
-						uint8_t k0 = s.kind[i0], k1 = s.kind[i1];
-						uint32_t c01 = kind_pair(kCanCollapseMask, k0, k1), c10 = kind_pair(kCanCollapseMask, k1, k0);
+						const uint8_t k0 = kv[a], k1 = kv[b];
+						const uint32_t c01 = kind_pair(kCanCollapseMask, k0, k1), c10 = kind_pair(kCanCollapseMask, k1, k0);
 						if (!(c01 | c10))
 							continue;
 						if (kind_pair(kHasOppositeMask, k0, k1) && r1 > r0)
@@ -829,24 +851,31 @@ __device__ __forceinline__ void simplify_patch(const BatchArgs& a, const PatchLa
 							continue;
 						if ((k1 == kBorder || k1 == kSeam) && k0 != kManifold && s.loopback[i1] != i0)
 							continue;
-						uint32_t packed, bd;
-						if (c01 & c10)
-						{
-							packed = i0 | (i1 << 16);
-							bd = 1;
-						}
-						else
-						{
-							packed = c01 ? (i0 | (i1 << 16)) : (i1 | (i0 << 16));
-							bd = 0;
-						}
+						// the candidate as pickEdgeCollapses lists it (i0 -> i1 when both ways are
+						// allowed), then rankEdgeCollapses: the cheaper direction of a two-way edge
+						const bool bidi = c01 & c10;
+						const bool fwd = bidi || c01;
+						const uint32_t v0 = fwd ? i0 : i1, v1 = fwd ? i1 : i0;
+						const float ei = fwd ? to_next[a] : to_prev[b];
+						const float ej = bidi ? to_prev[b] : FLT_MAX;
+						const bool rev = bidi & (ej < ei);
+						const uint32_t packed = rev ? (v1 | (v0 << 16)) : (v0 | (v1 << 16));
+						const float err = ej < ei ? ej : ei;
 						if (cnt == 0)
+						{
 							e0 = packed;
+							f0 = err;
+						}
 						else if (cnt == 1)
+						{
 							e1 = packed;
+							f1 = err;
+						}
 						else
+						{
 							e2 = packed;
-						bidi |= bd << cnt;
+							f2 = err;
+						}
 						cnt++;
 					}
 				}
@@ -863,19 +892,19 @@ __device__ __forceinline__ void simplify_patch(const BatchArgs& a, const PatchLa
 					{
 						s.cv0[at] = (uint16_t)(e0 & 0xffffu);
 						s.cv1[at] = (uint16_t)(e0 >> 16);
-						s.cerr[at] = __uint_as_float(bidi & 1u);
+						s.cerr[at] = f0;
 					}
 					if (cnt > 1 && at + 1 < s.cmax)
 					{
 						s.cv0[at + 1] = (uint16_t)(e1 & 0xffffu);
 						s.cv1[at + 1] = (uint16_t)(e1 >> 16);
-						s.cerr[at + 1] = __uint_as_float((bidi >> 1) & 1u);
+						s.cerr[at + 1] = f1;
 					}
 					if (cnt > 2 && at + 2 < s.cmax)
 					{
 						s.cv0[at + 2] = (uint16_t)(e2 & 0xffffu);
 						s.cv1[at + 2] = (uint16_t)(e2 >> 16);
-						s.cerr[at + 2] = __uint_as_float((bidi >> 2) & 1u);
+						s.cerr[at + 2] = f2;
 					}
 					// the last taken triangle (the next one would not fit, or there is none) ends the list
 					if (t + 1 == T || at + cnt + 3 > capacity)
@@ -897,20 +926,6 @@ __device__ __forceinline__ void simplify_patch(const BatchArgs& a, const PatchLa
 		PROFILE_COUNT(20, C);
 		if (C == 0)
 			break;
-
-		// rank (src/simplifier.cpp:1374-1450)
-		for (uint32_t c = tid; c < C; c += kBatchThreads)
-		{
-			uint32_t i0 = s.cv0[c], i1 = s.cv1[c];
-			bool bidi = __float_as_uint(s.cerr[c]) != 0;
-			float ei = quad_error(QL(s, s.remap[i0]), P(s, i1));
-			float ej = bidi ? quad_error(QL(s, s.remap[i1]), P(s, i0)) : FLT_MAX;
-			bool rev = bidi & (ej < ei);
-			s.cv0[c] = (uint16_t)(rev ? i1 : i0);
-			s.cv1[c] = (uint16_t)(rev ? i0 : i1);
-			s.cerr[c] = ej < ei ? ej : ei;
-		}
-		__syncthreads();
 
 		PROFILE_PHASE(5);
 		// stable sort by the 12-bit key (src/simplifier.cpp:1452-1496): LSD, two 6-bit digits
