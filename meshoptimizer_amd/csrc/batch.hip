@@ -21,27 +21,43 @@ __global__ void __launch_bounds__(kBatchThreads) __attribute__((amdgpu_waves_per
 	simplify_patch(a, L, patch, smem, part, sh, box, shf);
 }
 
+// one slot per mesh in its class's id list; the slots of a wavefront's meshes come from one
+// atomic per class (same-address atomics from every thread serialise in L2)
 __global__ void k_batch_classify(BatchArgs a, BatchClasses c, uint32_t* ids, uint32_t* counts)
 {
 	size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-	if (p >= a.patch_count)
-		return;
-	const uint32_t ib = a.index_offsets[p], ie = a.index_offsets[p + 1];
-	const uint32_t vb = a.vertex_offsets[p], ve = a.vertex_offsets[p + 1];
-	if (ie < ib || ve < vb || ve > a.vertex_count || (ie - ib) % 3 != 0 || a.target_index_counts[p] > ie - ib)
+	int cls = -1;
+	if (p < a.patch_count)
 	{
-		a.status[p] = 4;
-		return;
-	}
-	const uint32_t ni = ie - ib, nv = ve - vb;
-	for (int k = 0; k < kBatchClasses; ++k)
-		if (nv <= c.vcap[k] && ni <= c.icap[k])
+		const uint32_t ib = a.index_offsets[p], ie = a.index_offsets[p + 1];
+		const uint32_t vb = a.vertex_offsets[p], ve = a.vertex_offsets[p + 1];
+		if (ie < ib || ve < vb || ve > a.vertex_count || (ie - ib) % 3 != 0 || a.target_index_counts[p] > ie - ib)
+			a.status[p] = 4;
+		else
 		{
-			uint32_t slot = atomicAdd(&counts[k], 1u);
-			ids[(size_t)k * a.patch_count + slot] = (uint32_t)p;
-			return;
+			const uint32_t ni = ie - ib, nv = ve - vb;
+			for (int k = 0; k < kBatchClasses && cls < 0; ++k)
+				if (nv <= c.vcap[k] && ni <= c.icap[k])
+					cls = k;
+			if (cls < 0)
+				a.status[p] = 1; // too large for the LDS layouts: the plain path
 		}
-	a.status[p] = 1; // too large for the LDS layouts: the plain path
+	}
+	const uint32_t lane = __lane_id();
+	const uint64_t below = (1ull << lane) - 1ull;
+	for (int k = 0; k < kBatchClasses; ++k)
+	{
+		const uint64_t mine = __ballot(cls == k);
+		if (!mine)
+			continue;
+		const uint32_t leader = (uint32_t)__builtin_ctzll(mine);
+		uint32_t base = 0;
+		if (lane == leader)
+			base = atomicAdd(&counts[k], (uint32_t)__popcll(mine));
+		base = __shfl(base, (int)leader, 64);
+		if (cls == k)
+			ids[(size_t)k * a.patch_count + base + (uint32_t)__popcll(mine & below)] = (uint32_t)p;
+	}
 }
 
 // LDS bytes one workgroup may take so that `per_cu` of them fit a CU (160 KB, allocated in

