@@ -30,8 +30,8 @@ size_t meshopt_hip_simplify_device(unsigned int* destination, const unsigned int
  * it.  Mesh p owns indices [index_offsets[p], index_offsets[p+1]) (local to its vertices) and
  * vertices [vertex_offsets[p], vertex_offsets[p+1]) of the packed arrays, and writes its result
  * at destination + index_offsets[p]; result_counts[p] and result_errors[p] (may be NULL) receive
- * the index count and error.  target_error and options are shared by all meshes; Prune is not
- * accepted.  Returns 0, or -1 with meshopt_hip_last_error set.
+ * the index count and error.  target_error and options are shared by all meshes (Sparse and
+ * Prune batches run mesh by mesh on the plain path).  Returns 0, or -1 with meshopt_hip_last_error set.
  * The _device variant takes device pointers for every array and the number of vertices the
  * positions hold (vertex_offsets[mesh_count] must not exceed it); the other copies host arrays
  * in and out around the same launch.

@@ -989,8 +989,6 @@ static int simplify_batch_on_device(Context& ctx, unsigned int* destination, uns
                                     const unsigned int* h_vertex_offsets, const unsigned int* h_targets)
 {
 	hipStream_t s = ctx.stream;
-	if (options & kOptPrune)
-		throw std::invalid_argument("meshopt_hip_simplify_batch: the Prune option is not supported");
 	if (stride < 12 || stride > 256 || stride % 4)
 		throw std::invalid_argument("meshopt_hip_simplify_batch: invalid vertex stride");
 	if (!patch_count)
@@ -1022,7 +1020,9 @@ static int simplify_batch_on_device(Context& ctx, unsigned int* destination, uns
 	a.target_error = target_error;
 	a.options = options;
 
-	const bool batched = !(options & kOptSparse);
+	// Sparse and Prune meshes take the plain path, mesh by mesh (the batched kernel's LDS layouts
+	// hold neither the sparse remap nor the component arrays)
+	const bool batched = !(options & (kOptSparse | kOptPrune));
 	if (batched)
 	{
 		ctx.ensure_batch_quadrics(vertex_count ? vertex_count : 1);

@@ -6,7 +6,7 @@ import sys
 import numpy as np
 import pytest
 
-from meshes import random_grid, random_soup, same_bits
+from meshes import random_grid, random_soup, same_bits, scattered_components
 from oracle import oracle as O
 import meshoptimizer_amd as M
 from meshoptimizer_amd import workloads as W
@@ -112,8 +112,14 @@ def test_bad_index_raises():
         M.simplify_batch([(m.indices, m.vertices), (ib, m.vertices)], [0, 0], 1e-2, 0)
 
 
-def test_prune_rejected():
-    announce("prune rejected")
-    m = W.grid(5)
-    with pytest.raises(RuntimeError, match="Prune"):
-        M.simplify_batch([(m.indices, m.vertices)], [0], 1e-2, M.SimplifyPrune)
+def test_prune_batch():
+    """Prune batches run mesh by mesh on the plain path inside the call, bit-exact per mesh, on
+    meshes of many components at very different scales (Prune removes the small ones)."""
+    rng = np.random.default_rng(31)
+    meshes, targets = [], []
+    for k in range(5):
+        ib, v = scattered_components(rng, int(rng.integers(4, 16)))
+        meshes.append((ib, v))
+        targets.append(0 if k % 2 else (ib.size // 3 // 2) * 3)
+    announce("prune batch")
+    check_batch(meshes, targets, 5e-2, M.SimplifyPrune)

@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 
 import kat
-from meshes import random_grid, random_soup, same_bits
+from meshes import scattered_components, random_grid, random_soup, same_bits
 from oracle import oracle as O
 import meshoptimizer_amd as M
 from meshoptimizer_amd import workloads as W
@@ -115,25 +115,6 @@ def test_workloads_full(name):
     A = m.vertices[:, 3:8] if attrs else None
     w = W.SEAMED_GRID_WEIGHTS if attrs else None
     assert_parity(m.indices, m.vertices, t, 1e-2, 0, A, w)
-
-
-def scattered_components(rng, pieces):
-    """One mesh made of `pieces` disjoint grid patches and icospheres at different scales and
-    offsets, in shuffled triangle order (components with very different errors)."""
-    ibs, vs, base = [], [], 0
-    for k in range(pieces):
-        if k % 3 == 2:
-            m = W.icosphere(int(rng.integers(1, 4)), seed=int(rng.integers(1, 1000)))
-        else:
-            m = W.grid(int(rng.integers(3, 12)), seed=int(rng.integers(1, 1000)))
-        v = m.vertices[:, :3] * np.float32(rng.choice([0.0005, 0.002, 0.01, 0.05, 0.3]))
-        v = v + rng.random(3).astype(np.float32) * np.float32(4.0)
-        ibs.append(m.indices.reshape(-1, 3) + base)
-        vs.append(v.astype(np.float32))
-        base += v.shape[0]
-    tri = np.concatenate(ibs)
-    tri = tri[rng.permutation(tri.shape[0])]
-    return tri.reshape(-1).astype(np.uint32), np.ascontiguousarray(np.concatenate(vs))
 
 
 @pytest.mark.parametrize("seed", range(8))
