@@ -175,6 +175,42 @@ def simplify(indices, positions, target_index_count, target_error, options=0, *,
     return out, err.value
 
 
+class SloppyTrace(ctypes.Structure):
+    _fields_ = [("passes", ctypes.c_int), ("final_grid", ctypes.c_int), ("cells", ctypes.c_uint),
+                ("grid", ctypes.c_int * MAX_PASSES), ("triangles", ctypes.c_uint * MAX_PASSES)]
+
+
+def simplify_sloppy(indices, positions, target_index_count, target_error, *, lock=None, impl="oracle", trace=False):
+    """meshopt_simplifySloppy on the CPU: the C restatement (impl="oracle") or the compiled reference
+    (impl="ref").  Returns (result_indices, result_error[, SloppyTrace])."""
+    indices = np.ascontiguousarray(indices, dtype=np.uint32)
+    positions = _rows(positions, "positions")
+    vcount = positions.shape[0]
+    dest = np.zeros(max(indices.size, 1), dtype=np.uint32)
+    err = ctypes.c_float(0)
+    lk = None if lock is None else np.ascontiguousarray(lock, dtype=np.uint8)
+    tr = SloppyTrace()
+    if impl == "oracle":
+        lib = oracle_lib()
+        fn = lib.moracle_simplify_sloppy
+        fn.restype = _SZ
+        fn.argtypes = [_P, _P, _SZ, _P, _SZ, _SZ, _P, _SZ, _F, _P, _P]
+        n = fn(dest.ctypes.data, indices.ctypes.data, indices.size, positions.ctypes.data, vcount, positions.strides[0],
+               _ptr(lk), target_index_count, target_error, ctypes.byref(err), ctypes.byref(tr))
+    elif impl == "ref":
+        fn = ref_lib().meshopt_simplifySloppy
+        fn.restype = _SZ
+        fn.argtypes = [_P, _P, _SZ, _P, _SZ, _SZ, _P, _SZ, _F, _P]
+        n = fn(dest.ctypes.data, indices.ctypes.data, indices.size, positions.ctypes.data, vcount, positions.strides[0],
+               _ptr(lk), target_index_count, target_error, ctypes.byref(err))
+    else:
+        raise ValueError(impl)
+    out = dest[:n].copy()
+    if trace:
+        return out, err.value, tr
+    return out, err.value
+
+
 def simplify_scale(vertices, impl="oracle"):
     vertices = np.ascontiguousarray(vertices, dtype=np.float32)
     if vertices.ndim == 1:

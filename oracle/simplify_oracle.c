@@ -1927,3 +1927,295 @@ float moracle_simplify_scale(const float* vertex_positions, size_t vertex_count,
 {
 	return rescale_positions(NULL, vertex_positions, vertex_count, vertex_positions_stride / sizeof(float), NULL);
 }
+
+/* ------------------------------------------------------------------------------------------ */
+/* sloppy simplifier: meshopt_simplifySloppy (src/simplifier.cpp:2647-2778) and the helpers it
+ * uses (2087-2334).  Vertices snap to the cells of a grid whose size comes from a guided
+ * search on the triangle count; each cell keeps its lowest-error vertex under the cell's
+ * quadric; collapsed and duplicate triangles are dropped in input order. */
+
+/* computeVertexIds, 2087-2105: round-half-up cell coordinates, 10 bits each; a locked vertex
+ * gets an id of its own */
+static void sloppy_vertex_ids(unsigned int* ids, const vec3* vp, const unsigned char* vertex_lock, size_t vertex_count, int grid_size)
+{
+	float cell_scale = (float)(grid_size - 1);
+
+	for (size_t i = 0; i < vertex_count; ++i)
+	{
+		int xi = (int)(vp[i].x * cell_scale + 0.5f);
+		int yi = (int)(vp[i].y * cell_scale + 0.5f);
+		int zi = (int)(vp[i].z * cell_scale + 0.5f);
+
+		if (vertex_lock && (vertex_lock[i] & VLOCK_LOCK))
+			ids[i] = (1u << 30) | (unsigned int)i;
+		else
+			ids[i] = ((unsigned int)xi << 20) | ((unsigned int)yi << 10) | (unsigned int)zi;
+	}
+}
+
+/* countTriangles, 2107-2121: triangles whose three vertex ids differ */
+static size_t sloppy_count(const unsigned int* ids, const unsigned int* indices, size_t index_count)
+{
+	size_t n = 0;
+	for (size_t i = 0; i < index_count; i += 3)
+	{
+		unsigned int a = ids[indices[i]], b = ids[indices[i + 1]], c = ids[indices[i + 2]];
+		n += (a != b) & (a != c) & (b != c);
+	}
+	return n;
+}
+
+/* open-addressing table of 32-bit keys (any hash gives the same numbering: cells and kept
+ * triangles are numbered by first occurrence in input order) */
+static size_t pow2_at_least(size_t n)
+{
+	size_t s = 16;
+	while (s < n)
+		s *= 2;
+	return s;
+}
+
+static unsigned int mix32(unsigned int h)
+{
+	h ^= h >> 16;
+	h *= 0x7feb352du;
+	h ^= h >> 15;
+	h *= 0x846ca68bu;
+	h ^= h >> 16;
+	return h;
+}
+
+/* fillVertexCells, 2123-2146: cell numbers in order of the first vertex of each id */
+static size_t sloppy_vertex_cells(unsigned int* cells, const unsigned int* ids, size_t vertex_count)
+{
+	size_t size = pow2_at_least(vertex_count + vertex_count / 4 + 1);
+	unsigned int* table = (unsigned int*)xalloc(size * sizeof(unsigned int));
+	memset(table, 0xff, size * sizeof(unsigned int));
+	size_t count = 0;
+
+	for (size_t i = 0; i < vertex_count; ++i)
+	{
+		size_t slot = mix32(ids[i]) & (size - 1);
+		while (table[slot] != NONE && ids[table[slot]] != ids[i])
+			slot = (slot + 1) & (size - 1);
+
+		if (table[slot] == NONE)
+		{
+			table[slot] = (unsigned int)i;
+			cells[i] = (unsigned int)count++;
+		}
+		else
+			cells[i] = cells[table[slot]];
+	}
+
+	free(table);
+	return count;
+}
+
+/* fillCellQuadrics, 2169-2199: each triangle's quadric (weight 3 when it lies in one cell) added
+ * to the cells of its corners, in triangle and corner order */
+static void sloppy_cell_quadrics(quadric* cq, const unsigned int* indices, size_t index_count, const vec3* vp, const unsigned int* cells)
+{
+	for (size_t i = 0; i < index_count; i += 3)
+	{
+		unsigned int i0 = indices[i], i1 = indices[i + 1], i2 = indices[i + 2];
+		unsigned int c0 = cells[i0], c1 = cells[i1], c2 = cells[i2];
+		int single = (c0 == c1) & (c0 == c2);
+
+		quadric Q;
+		q_from_triangle(&Q, vp[i0], vp[i1], vp[i2], single ? 3.f : 1.f);
+
+		q_add(&cq[c0], &Q);
+		if (!single)
+		{
+			q_add(&cq[c1], &Q);
+			q_add(&cq[c2], &Q);
+		}
+	}
+}
+
+/* fillCellRemap, 2237-2254: per cell the first vertex with the smallest error (a strictly
+ * smaller error replaces the current one) */
+static void sloppy_cell_remap(unsigned int* cremap, float* cerr, size_t cell_count, const unsigned int* cells, const quadric* cq, const vec3* vp, size_t vertex_count)
+{
+	for (size_t c = 0; c < cell_count; ++c)
+		cremap[c] = NONE;
+
+	for (size_t i = 0; i < vertex_count; ++i)
+	{
+		unsigned int c = cells[i];
+		float e = q_error(&cq[c], vp[i]);
+
+		if (cremap[c] == NONE || cerr[c] > e)
+		{
+			cremap[c] = (unsigned int)i;
+			cerr[c] = e;
+		}
+	}
+}
+
+/* filterTriangles, 2282-2325: triangles over three distinct cells, mapped to the cells' vertices
+ * and rotated so the smallest index leads; a triangle equal to an earlier kept one is dropped */
+static size_t sloppy_filter(unsigned int* destination, const unsigned int* indices, size_t index_count, const unsigned int* cells, const unsigned int* cremap, size_t max_triangles)
+{
+	size_t size = pow2_at_least(max_triangles + max_triangles / 4 + 1);
+	unsigned int* table = (unsigned int*)xalloc(size * sizeof(unsigned int));
+	memset(table, 0xff, size * sizeof(unsigned int));
+	size_t count = 0;
+
+	for (size_t i = 0; i < index_count; i += 3)
+	{
+		unsigned int c0 = cells[indices[i]], c1 = cells[indices[i + 1]], c2 = cells[indices[i + 2]];
+		if (c0 == c1 || c0 == c2 || c1 == c2)
+			continue;
+
+		unsigned int a = cremap[c0], b = cremap[c1], c = cremap[c2];
+		if (b < a && b < c)
+		{
+			unsigned int t = a;
+			a = b, b = c, c = t;
+		}
+		else if (c < a && c < b)
+		{
+			unsigned int t = c;
+			c = b, b = a, a = t;
+		}
+
+		size_t slot = mix32(a * 0x9e3779b1u ^ mix32(b) ^ mix32(c * 0x85ebca6bu)) & (size - 1);
+		int dup = 0;
+		while (table[slot] != NONE)
+		{
+			const unsigned int* t = destination + (size_t)table[slot] * 3;
+			if (t[0] == a && t[1] == b && t[2] == c)
+			{
+				dup = 1;
+				break;
+			}
+			slot = (slot + 1) & (size - 1);
+		}
+		if (dup)
+			continue;
+
+		destination[count * 3 + 0] = a;
+		destination[count * 3 + 1] = b;
+		destination[count * 3 + 2] = c;
+		table[slot] = (unsigned int)count++;
+	}
+
+	free(table);
+	return count * 3;
+}
+
+/* interpolate, 2327-2334: three-point interpolation search step */
+static float sloppy_interpolate(float y, float x0, float y0, float x1, float y1, float x2, float y2)
+{
+	float num = (y1 - y) * (x1 - x2) * (x1 - x0) * (y2 - y0);
+	float den = (y2 - y) * (x1 - x2) * (y0 - y1) + (y0 - y) * (x1 - x0) * (y1 - y2);
+	return x1 + (den == 0.f ? 0.f : num / den);
+}
+
+/* meshopt_simplifySloppy, 2647-2778 */
+size_t moracle_simplify_sloppy(unsigned int* destination, const unsigned int* indices, size_t index_count, const float* vertex_positions, size_t vertex_count, size_t vertex_positions_stride, const unsigned char* vertex_lock, size_t target_index_count, float target_error, float* out_result_error, moracle_sloppy_trace* trace)
+{
+	size_t target_cell_count = target_index_count / 6;
+
+	vec3* vp = (vec3*)xalloc((vertex_count + 1) * sizeof(vec3));
+	rescale_positions(vp, vertex_positions, vertex_count, vertex_positions_stride / sizeof(float), NULL);
+
+	unsigned int* ids = (unsigned int*)xalloc((vertex_count + 1) * sizeof(unsigned int));
+
+	const int interpolation_passes = 5;
+	int min_grid = (int)(1.f / (target_error < 1e-3f ? 1e-3f : (target_error < 1.f ? target_error : 1.f)));
+	int max_grid = 1025;
+	size_t min_triangles = 0;
+	size_t max_triangles = index_count / 3;
+
+	if (min_grid > 1 || vertex_lock)
+	{
+		sloppy_vertex_ids(ids, vp, vertex_lock, vertex_count, min_grid);
+		min_triangles = sloppy_count(ids, indices, index_count);
+	}
+
+	int next_grid_size = (int)(sqrtf((float)target_cell_count) + 0.5f);
+	int passes = 0;
+
+	for (int pass = 0; pass < 10 + interpolation_passes; ++pass)
+	{
+		if (min_triangles >= target_index_count / 3 || max_grid - min_grid <= 1)
+			break;
+
+		int grid_size = next_grid_size;
+		grid_size = (grid_size <= min_grid) ? min_grid + 1 : (grid_size >= max_grid ? max_grid - 1 : grid_size);
+
+		sloppy_vertex_ids(ids, vp, vertex_lock, vertex_count, grid_size);
+		size_t triangles = sloppy_count(ids, indices, index_count);
+		if (trace && passes < MORACLE_MAX_PASSES)
+		{
+			trace->grid[passes] = grid_size;
+			trace->triangles[passes] = (unsigned int)triangles;
+		}
+		passes++;
+
+		float tip = sloppy_interpolate((float)(size_t)(target_index_count / 3), (float)min_grid, (float)min_triangles, (float)grid_size, (float)triangles, (float)max_grid, (float)max_triangles);
+
+		if (triangles <= target_index_count / 3)
+		{
+			min_grid = grid_size;
+			min_triangles = triangles;
+		}
+		else
+		{
+			max_grid = grid_size;
+			max_triangles = triangles;
+		}
+
+		next_grid_size = (pass < interpolation_passes) ? (int)(tip + 0.5f) : (min_grid + max_grid) / 2;
+	}
+
+	if (trace)
+	{
+		trace->passes = passes;
+		trace->final_grid = min_grid;
+		trace->cells = 0;
+	}
+
+	if (min_triangles == 0)
+	{
+		if (out_result_error)
+			*out_result_error = 1.f;
+		free(vp);
+		free(ids);
+		return 0;
+	}
+
+	unsigned int* cells = (unsigned int*)xalloc((vertex_count + 1) * sizeof(unsigned int));
+	sloppy_vertex_ids(ids, vp, vertex_lock, vertex_count, min_grid);
+	size_t cell_count = sloppy_vertex_cells(cells, ids, vertex_count);
+	if (trace)
+		trace->cells = (unsigned int)cell_count;
+
+	quadric* cq = (quadric*)xalloc((cell_count + 1) * sizeof(quadric));
+	memset(cq, 0, (cell_count + 1) * sizeof(quadric));
+	sloppy_cell_quadrics(cq, indices, index_count, vp, cells);
+
+	unsigned int* cremap = (unsigned int*)xalloc((cell_count + 1) * sizeof(unsigned int));
+	float* cerr = (float*)xalloc((cell_count + 1) * sizeof(float));
+	sloppy_cell_remap(cremap, cerr, cell_count, cells, cq, vp, vertex_count);
+
+	float result_error = 0.f;
+	for (size_t c = 0; c < cell_count; ++c)
+		result_error = result_error < cerr[c] ? cerr[c] : result_error;
+
+	size_t write = sloppy_filter(destination, indices, index_count, cells, cremap, min_triangles);
+
+	if (out_result_error)
+		*out_result_error = sqrtf(result_error);
+
+	free(vp);
+	free(ids);
+	free(cells);
+	free(cq);
+	free(cremap);
+	free(cerr);
+	return write;
+}

@@ -43,6 +43,18 @@ size_t moracle_simplify_with_attributes(unsigned int* destination, const unsigne
 
 float moracle_simplify_scale(const float* vertex_positions, size_t vertex_count, size_t vertex_positions_stride);
 
+/* the grid-size search of meshopt_simplifySloppy, pass by pass (grid size tried, triangles counted) */
+typedef struct
+{
+	int passes;
+	int final_grid;
+	unsigned int cells;
+	int grid[MORACLE_MAX_PASSES];
+	unsigned int triangles[MORACLE_MAX_PASSES];
+} moracle_sloppy_trace;
+
+size_t moracle_simplify_sloppy(unsigned int* destination, const unsigned int* indices, size_t index_count, const float* vertex_positions, size_t vertex_count, size_t vertex_positions_stride, const unsigned char* vertex_lock, size_t target_index_count, float target_error, float* result_error, moracle_sloppy_trace* trace);
+
 #ifdef __cplusplus
 }
 #endif

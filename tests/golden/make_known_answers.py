@@ -3,6 +3,7 @@ reference's own tests, as data.
 
 Sources (inputs and expected outputs transcribed as values):
   demo/tests.cpp:1771-2558           C known-answer tests of meshopt_simplify* (1.2)
+  demo/tests.cpp:1833-1882           meshopt_simplifySloppy (written to sloppy_known_answers.json)
   js/meshopt_simplifier.test.js:25-163  the WASM build's plain-text expected arrays
 
 Each case: call ("simplify" | "simplifyWithAttributes"), vertices [V][F] float32 values,
@@ -222,12 +223,35 @@ def build():
     return cases
 
 
+def build_sloppy():
+    """meshopt_simplifySloppy known answers (demo/tests.cpp:1833-1882)."""
+    cases = []
+    zero_vb = [0, 0, 0, 0, 0, 0, 0, 0, 0]
+    cases.append(case("simplifySloppyStuck.zero_target", "demo/tests.cpp:1833-1841", zero_vb, [0, 1, 2], 0, 0.0,
+                      call="simplifySloppy", expected_count=0))
+    cases.append(case("simplifySloppyStuck.degenerate", "demo/tests.cpp:1843-1844", zero_vb, [0, 1, 2, 0, 1, 2], 6, 0.0,
+                      call="simplifySloppy", expected_count=0))
+    fan_vb = [0, 4, 0, 0, 1, 0, 2, 2, 0, 0, 0, 0, 1, 0, 0, 4, 0, 0]
+    fan_ib = [0, 2, 1, 1, 2, 3, 3, 2, 4, 2, 5, 4]
+    c = case("simplifySloppyLocks", "demo/tests.cpp:1847-1882", fan_vb, fan_ib, 3, 1.0, call="simplifySloppy",
+             lock=[1, 0, 1, 0, 0, 1], expected_count=6, expected=[0, 2, 1, 1, 2, 5])
+    c["error_eq"] = 0.0
+    cases.append(c)
+    return cases
+
+
 def main():
     cases = build()
-    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "known_answers.json")
+    here = os.path.dirname(os.path.abspath(__file__))
+    path = os.path.join(here, "known_answers.json")
     with open(path, "w") as f:
         json.dump({"reference": "meshoptimizer 1.2 (MESHOPTIMIZER_VERSION 1020)", "cases": cases}, f, indent=1)
     print(f"wrote {len(cases)} cases to {path}")
+    sloppy = build_sloppy()
+    path = os.path.join(here, "sloppy_known_answers.json")
+    with open(path, "w") as f:
+        json.dump({"reference": "meshoptimizer 1.2 (MESHOPTIMIZER_VERSION 1020)", "cases": sloppy}, f, indent=1)
+    print(f"wrote {len(sloppy)} cases to {path}")
 
 
 if __name__ == "__main__":

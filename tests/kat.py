@@ -9,8 +9,8 @@ import numpy as np
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
-def load_cases():
-    with open(os.path.join(GOLDEN, "known_answers.json")) as f:
+def load_cases(name="known_answers.json"):
+    with open(os.path.join(GOLDEN, name)) as f:
         return json.load(f)["cases"]
 
 
@@ -41,6 +41,8 @@ def check(case, result, error):
     assert result.size == case["expected_count"], f"{name}: count {result.size} != {case['expected_count']}"
     if case["expected_indices"] is not None:
         assert result.tolist() == case["expected_indices"], f"{name}: {result.tolist()} != {case['expected_indices']}"
+    if "error_eq" in case:
+        assert error == case["error_eq"], f"{name}: error {error} != {case['error_eq']}"
     if "error_lt" in case:
         assert error < case["error_lt"], f"{name}: error {error}"
     if "error_approx" in case:

@@ -130,3 +130,51 @@ def test_plausible_mistakes_are_caught():
     v2[:, 2] *= np.float32(1.0001)
     r2, e2 = O.simplify(m.indices, v2, t, 1e-2)
     assert not (r1.tolist() == r2.tolist() and same_bits(e1, e2))
+
+
+SLOPPY_CASES = kat.load_cases("sloppy_known_answers.json")
+
+
+@pytest.mark.parametrize("case", SLOPPY_CASES, ids=[c["name"] for c in SLOPPY_CASES])
+def test_sloppy_known_answers(case):
+    """meshopt_simplifySloppy known answers of demo/tests.cpp:1833-1882 on the C restatement."""
+    a = kat.arrays(case)
+    r, e = O.simplify_sloppy(a["indices"], a["positions"], case["target_index_count"], case["target_error"], lock=a["lock"])
+    kat.check(case, r, e)
+
+
+@pytest.mark.parametrize("seed", range(60))
+def test_sloppy_oracle_matches_reference(seed):
+    """The sloppy restatement against the compiled reference: index buffer and error bits."""
+    _ref_or_skip()
+    rng = np.random.default_rng(seed)
+    if seed % 2:
+        ib, v = random_soup(rng, nv=int(rng.integers(3, 200)), ntri=int(rng.integers(1, 400)))
+    else:
+        ib, v = random_grid(rng, n=int(rng.integers(3, 30)))
+    v = np.ascontiguousarray(v[:, :3])
+    target = (ib.size // 3 * int(rng.integers(0, 5)) // 4) * 3
+    err = float(rng.choice([1e-4, 1e-3, 1e-2, 0.1, 1.0]))
+    lock = (rng.random(v.shape[0]) < 0.1).astype(np.uint8) if seed % 3 == 0 else None
+    r1, e1 = O.simplify_sloppy(ib, v, target, err, lock=lock)
+    r2, e2 = O.simplify_sloppy(ib, v, target, err, lock=lock, impl="ref")
+    assert np.array_equal(r1, r2)
+    assert same_bits(e1, e2)
+
+
+def test_sloppy_closed_form():
+    """A flat (n+1)^2 grid with every triangle as the target and error 1e-3: the minimum grid is
+    int(1 / 1e-3) = 1000 cells per axis, where each vertex has a cell of its own, so the search
+    stops before its first pass (the count already meets the target), there are V cells, and
+    every triangle survives (rotated to lead with its smallest index)."""
+    n = 16
+    m = W.grid(n + 1)
+    v = np.ascontiguousarray(m.vertices[:, :3]).copy()
+    v[:, 2] = 0.0
+    r, e, tr = O.simplify_sloppy(m.indices, v, m.index_count, 1e-3, trace=True)
+    assert tr.passes == 0 and tr.final_grid == 1000
+    assert tr.cells == v.shape[0]
+    assert r.size == m.index_count
+    tri = m.indices.reshape(-1, 3)
+    rot = [tuple(np.roll(t, -int(np.argmin(t)))) for t in tri]
+    assert [tuple(x) for x in r.reshape(-1, 3)] == rot
