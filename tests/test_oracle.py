@@ -164,7 +164,8 @@ def test_sloppy_oracle_matches_reference(seed):
 
 def test_sloppy_closed_form():
     """A flat (n+1)^2 grid with every triangle as the target and error 1e-3: the minimum grid is
-    int(1 / 1e-3) = 1000 cells per axis, where each vertex has a cell of its own, so the search
+    int(1.f / 1e-3f) = 999 cells per axis (1e-3f is a little above 0.001, so the quotient
+    truncates to 999), where each vertex has a cell of its own, so the search
     stops before its first pass (the count already meets the target), there are V cells, and
     every triangle survives (rotated to lead with its smallest index)."""
     n = 16
@@ -172,7 +173,7 @@ def test_sloppy_closed_form():
     v = np.ascontiguousarray(m.vertices[:, :3]).copy()
     v[:, 2] = 0.0
     r, e, tr = O.simplify_sloppy(m.indices, v, m.index_count, 1e-3, trace=True)
-    assert tr.passes == 0 and tr.final_grid == 1000
+    assert tr.passes == 0 and tr.final_grid == 999
     assert tr.cells == v.shape[0]
     assert r.size == m.index_count
     tri = m.indices.reshape(-1, 3)
