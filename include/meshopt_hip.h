@@ -25,6 +25,13 @@ extern "C"
 size_t meshopt_hip_simplify_device(unsigned int* destination, const unsigned int* indices, size_t index_count, const float* vertex_positions, size_t vertex_count, size_t vertex_positions_stride, const float* vertex_attributes, size_t vertex_attributes_stride, const float* attribute_weights, size_t attribute_count, const unsigned char* vertex_lock, size_t target_index_count, float target_error, unsigned int options, float* result_error);
 
 /**
+ * meshopt_simplifySloppy on device-resident arrays (every pointer but result_error is a device
+ * pointer); no counterpart in the reference (its entry point takes host arrays).  Returns the
+ * number of indices written, or (size_t)-1 with meshopt_hip_last_error set.
+ */
+size_t meshopt_hip_simplify_sloppy_device(unsigned int* destination, const unsigned int* indices, size_t index_count, const float* vertex_positions, size_t vertex_count, size_t vertex_positions_stride, const unsigned char* vertex_lock, size_t target_index_count, float target_error, float* result_error);
+
+/**
  * Simplifies many independent meshes (for example the cluster groups of a clusterlod build) in
  * one launch; every mesh gets exactly what meshopt_simplify (src/meshoptimizer.h:513) returns for
  * it.  Mesh p owns indices [index_offsets[p], index_offsets[p+1]) (local to its vertices) and

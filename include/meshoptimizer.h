@@ -68,6 +68,15 @@ MESHOPTIMIZER_API size_t meshopt_simplify(unsigned int* destination, const unsig
 MESHOPTIMIZER_API size_t meshopt_simplifyWithAttributes(unsigned int* destination, const unsigned int* indices, size_t index_count, const float* vertex_positions, size_t vertex_count, size_t vertex_positions_stride, const float* vertex_attributes, size_t vertex_attributes_stride, const float* attribute_weights, size_t attribute_count, const unsigned char* vertex_lock, size_t target_index_count, float target_error, unsigned int options, float* result_error);
 
 /**
+ * Sloppy simplifier: vertices snap to a regular grid whose size is searched for the target, each
+ * grid cell keeps its lowest-error vertex, collapsed and duplicate triangles are dropped.
+ * Replaces meshopt_simplifySloppy, src/meshoptimizer.h:576 (implementation
+ * src/simplifier.cpp:2647-2778).  destination needs room for index_count indices and may alias
+ * indices; vertex_lock may be NULL; result_error may be NULL.  Returns the number of indices written.
+ */
+MESHOPTIMIZER_API size_t meshopt_simplifySloppy(unsigned int* destination, const unsigned int* indices, size_t index_count, const float* vertex_positions, size_t vertex_count, size_t vertex_positions_stride, const unsigned char* vertex_lock, size_t target_index_count, float target_error, float* result_error);
+
+/**
  * Largest side of the positions' bounding box: the factor between absolute and relative error.
  * Replaces meshopt_simplifyScale, src/meshoptimizer.h:611 (implementation src/simplifier.cpp:2938-2948).
  */

@@ -46,7 +46,9 @@ EXPORTED_SYMBOLS = (
     "meshopt_simplify",
     "meshopt_simplifyWithAttributes",
     "meshopt_simplifyScale",
+    "meshopt_simplifySloppy",
     "meshopt_setAllocator",
+    "meshopt_hip_simplify_sloppy_device",
     "meshopt_hip_simplify_device",
     "meshopt_hip_simplify_batch",
     "meshopt_hip_simplify_batch_device",
@@ -101,6 +103,10 @@ def library() -> ctypes.CDLL:
     lib.meshopt_simplifyWithAttributes.argtypes = [_P, _P, _SZ, _P, _SZ, _SZ, _P, _SZ, _P, _SZ, _P, _SZ, _F, _U, _P]
     lib.meshopt_simplifyScale.restype = _F
     lib.meshopt_simplifyScale.argtypes = [_P, _SZ, _SZ]
+    for name in ("meshopt_simplifySloppy", "meshopt_hip_simplify_sloppy_device"):
+        fn = getattr(lib, name)
+        fn.restype = _SZ
+        fn.argtypes = [_P, _P, _SZ, _P, _SZ, _SZ, _P, _SZ, _F, _P]
     lib.meshopt_hip_simplify_device.restype = _SZ
     lib.meshopt_hip_simplify_device.argtypes = [_P, _P, _SZ, _P, _SZ, _SZ, _P, _SZ, _P, _SZ, _P, _SZ, _F, _U, _P]
     lib.meshopt_hip_last_call_stats.restype = ctypes.c_int
@@ -180,6 +186,34 @@ def simplify_with_attributes(indices, positions, attributes, weights, lock, targ
         None if lk is None else lk.ctypes.data, int(target_index_count), float(target_error), int(options),
         ctypes.byref(err))
     return dest[:n].copy(), err.value
+
+
+def simplify_sloppy(indices, positions, target_index_count, target_error=1e-2, lock=None):
+    """meshopt_simplifySloppy.  positions: float32 [V, >=3] (row stride = vertex stride); lock: uint8 [V]
+    or None.  Returns (indices uint32 [n], result_error)."""
+    lib = library()
+    indices = np.ascontiguousarray(indices, dtype=np.uint32)
+    positions = _rows(positions, "positions")
+    lk = None if lock is None else np.ascontiguousarray(lock, dtype=np.uint8)
+    dest = np.zeros(max(indices.size, 1), dtype=np.uint32)
+    err = ctypes.c_float(0)
+    n = lib.meshopt_simplifySloppy(dest.ctypes.data, indices.ctypes.data, indices.size, positions.ctypes.data,
+                                   positions.shape[0], positions.strides[0], None if lk is None else lk.ctypes.data,
+                                   int(target_index_count), float(target_error), ctypes.byref(err))
+    return dest[:n].copy(), err.value
+
+
+def simplify_sloppy_device(destination_ptr, indices_ptr, index_count, positions_ptr, vertex_count, stride, lock_ptr,
+                           target_index_count, target_error):
+    """meshopt_hip_simplify_sloppy_device on device pointers (integers).  Returns (index count,
+    result_error); raises on error."""
+    lib = library()
+    err = ctypes.c_float(0)
+    n = lib.meshopt_hip_simplify_sloppy_device(destination_ptr, indices_ptr, index_count, positions_ptr, vertex_count, stride,
+                                               lock_ptr, int(target_index_count), float(target_error), ctypes.byref(err))
+    if n == ctypes.c_size_t(-1).value:
+        raise RuntimeError(lib.meshopt_hip_last_error().decode())
+    return n, err.value
 
 
 def simplify_scale(positions):

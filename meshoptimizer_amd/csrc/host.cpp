@@ -25,6 +25,7 @@
 #include "../../include/meshopt_hip.h"
 #include "../../include/meshoptimizer.h"
 #include "batch.h"
+#include "sloppy.h"
 #include "common.h"
 #include "kernels.h"
 #include "prims.h"
@@ -121,6 +122,14 @@ struct Context
 	size_t batch_pinned_words = 0;
 	Quad* batch_quadrics = nullptr; // batched calls: one vertex quadric per vertex of the batch
 	size_t batch_quadric_count = 0;
+	uint32_t* sloppy_words = nullptr; // sloppy calls: pinned read-back words
+
+	uint32_t* sloppy_pinned()
+	{
+		if (!sloppy_words)
+			HIP_CHECK(hipHostMalloc((void**)&sloppy_words, 8 * sizeof(uint32_t), hipHostMallocDefault));
+		return sloppy_words;
+	}
 	hipStream_t aux[kBatchClasses - 1] = {}; // batch size classes beyond the first
 	hipEvent_t fork_ev = nullptr, join_ev[kBatchClasses - 1] = {};
 
@@ -217,6 +226,9 @@ struct Context
 			(void)hipHostFree(batch_pinned);
 		if (batch_quadrics)
 			(void)hipFree(batch_quadrics);
+		if (sloppy_words)
+			(void)hipHostFree(sloppy_words);
+		sloppy_words = nullptr;
 		batch_dev = nullptr;
 		batch_pinned = nullptr;
 		batch_quadrics = nullptr;
@@ -872,6 +884,166 @@ static size_t simplify_host(const unsigned int* indices, unsigned int* destinati
 	                        attribute_count, lock, target_index_count, target_error, options, result_error);
 }
 
+// ------------------------------------------------------------------------------------------
+// meshopt_simplifySloppy (src/simplifier.cpp:2647-2778) on device-resident inputs: the grid-size
+// search runs on the host exactly as the reference's loop (one device count per pass), the
+// cell and triangle steps on the device.
+
+// interpolate, src/simplifier.cpp:2327-2334 (host arithmetic, no contraction)
+static float sloppy_interpolate(float y, float x0, float y0, float x1, float y1, float x2, float y2)
+{
+	float num = (y1 - y) * (x1 - x2) * (x1 - x0) * (y2 - y0);
+	float den = (y2 - y) * (x1 - x2) * (y0 - y1) + (y0 - y) * (x1 - x0) * (y1 - y2);
+	return x1 + (den == 0.f ? 0.f : num / den);
+}
+
+static size_t sloppy_on_device(Context& ctx, unsigned int* destination, const unsigned int* indices, size_t index_count, const float* positions,
+                               size_t vertex_count, size_t stride, const unsigned char* lock, size_t target_index_count, float target_error,
+                               float* out_result_error)
+{
+	hipStream_t s = ctx.stream;
+	const size_t V = vertex_count, I = index_count, T = I / 3;
+	if (V >= 0xffffffffu || I >= 0xffffffffu)
+		throw std::invalid_argument("meshopt_simplifySloppy: more than 2^32 - 1 vertices or indices");
+
+	// carve the call's buffers from the context arena (256-byte aligned)
+	const uint32_t vtable_size = (uint32_t)std::max<size_t>(pow2_at_least(2 * V + 2), 16), ttable_size = (uint32_t)std::max<size_t>(pow2_at_least(2 * T + 2), 16);
+	const size_t sort_words = sort_temp_words(3 * T + 1);
+	const size_t scan_words = std::max(scan_temp_words(V + 1), scan_temp_words(T + 1));
+	size_t off = 0;
+	auto take = [&](size_t bytes) {
+		size_t at = off;
+		off = (off + bytes + 255) & ~size_t(255);
+		return at;
+	};
+	const size_t o_praw = take(16 * (V + 1)), o_pos = take(16 * (V + 1)), o_sc = take(sizeof(CallScalars));
+	const size_t o_words = take(4 * 8), o_ids = take(4 * (V + 1)), o_vtable = take(4 * (size_t)vtable_size), o_rep = take(4 * (V + 1));
+	const size_t o_cscan = take(4 * (V + 2)), o_cells = take(4 * (V + 1)), o_cfirst = take(4 * (V + 1));
+	const size_t o_ek = take(4 * (3 * T + 1)), o_ev = take(4 * (3 * T + 1)), o_eka = take(4 * (3 * T + 1)), o_eva = take(4 * (3 * T + 1));
+	const size_t o_coff = take(4 * (V + 2)), o_stemp = take(4 * sort_words), o_ctemp = take(4 * scan_words), o_cq = take(sizeof(Quad) * (V + 1));
+	const size_t o_best = take(8 * (V + 1)), o_cremap = take(4 * (V + 1)), o_tri = take(16 * (T + 1)), o_ttable = take(4 * (size_t)ttable_size);
+	const size_t o_keep = take(4 * (T + 2));
+	ctx.ensure_arena(off);
+	char* base = ctx.arena;
+
+	Mesh m{};
+	m.vertex_count = (uint32_t)V;
+	m.praw = reinterpret_cast<float4*>(base + o_praw);
+	m.pos = reinterpret_cast<float4*>(base + o_pos);
+	m.sc = reinterpret_cast<CallScalars*>(base + o_sc);
+	uint32_t* words = reinterpret_cast<uint32_t*>(base + o_words); // count, cell count, kept, error bits, index check
+
+	SloppyBuffers b{};
+	b.indices = indices;
+	b.pos = m.pos;
+	b.lock = lock;
+	b.vertex_count = (uint32_t)V;
+	b.index_count = (uint32_t)I;
+	b.count = words + 0;
+	b.ids = reinterpret_cast<uint32_t*>(base + o_ids);
+	b.vtable = reinterpret_cast<uint32_t*>(base + o_vtable);
+	b.vtable_size = vtable_size;
+	b.rep = reinterpret_cast<uint32_t*>(base + o_rep);
+	b.cell_scan = reinterpret_cast<uint32_t*>(base + o_cscan);
+	b.cells = reinterpret_cast<uint32_t*>(base + o_cells);
+	b.cell_first = reinterpret_cast<uint32_t*>(base + o_cfirst);
+	b.entry_keys = reinterpret_cast<uint32_t*>(base + o_ek);
+	b.entry_vals = reinterpret_cast<uint32_t*>(base + o_ev);
+	b.entry_keys_alt = reinterpret_cast<uint32_t*>(base + o_eka);
+	b.entry_vals_alt = reinterpret_cast<uint32_t*>(base + o_eva);
+	b.cell_off = reinterpret_cast<uint32_t*>(base + o_coff);
+	b.sort_temp = reinterpret_cast<uint32_t*>(base + o_stemp);
+	b.scan_temp = reinterpret_cast<uint32_t*>(base + o_ctemp);
+	b.cq = reinterpret_cast<Quad*>(base + o_cq);
+	b.best = reinterpret_cast<uint64_t*>(base + o_best);
+	b.cell_remap = reinterpret_cast<uint32_t*>(base + o_cremap);
+	b.error_bits = words + 3;
+	b.tri = reinterpret_cast<uint4*>(base + o_tri);
+	b.ttable = reinterpret_cast<uint32_t*>(base + o_ttable);
+	b.ttable_size = ttable_size;
+	b.keep = reinterpret_cast<uint32_t*>(base + o_keep);
+
+	uint32_t* h = ctx.sloppy_pinned();
+	auto read_words = [&](int first, int count) {
+		HIP_CHECK(hipMemcpyAsync(h + first, words + first, count * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+		HIP_CHECK(hipStreamSynchronize(s));
+	};
+
+	// every later gather trusts the indices: reject out-of-range ones before any kernel uses them
+	if (I)
+	{
+		HIP_CHECK(hipMemsetAsync(words + 4, 0, sizeof(uint32_t), s));
+		launch_check_indices(s, indices, I, V, words + 4);
+		read_words(4, 1);
+		if (h[4])
+			throw std::invalid_argument("meshopt_simplifySloppy: index buffer references a vertex >= vertex_count");
+	}
+
+	// rescalePositions (src/simplifier.cpp:549-603)
+	if (V)
+		launch_gather_positions(s, positions, stride / sizeof(float), nullptr, m.praw, V);
+	launch_rescale(s, m);
+
+	auto count_at = [&](int grid_size) -> size_t {
+		launch_sloppy_count(s, b, grid_size);
+		read_words(0, 1);
+		return h[0];
+	};
+
+	// the guided grid-size search (src/simplifier.cpp:2667-2723)
+	const size_t target_cell_count = target_index_count / 6;
+	const int interpolation_passes = 5;
+	int min_grid = int(1.f / (target_error < 1e-3f ? 1e-3f : (target_error < 1.f ? target_error : 1.f)));
+	int max_grid = 1025;
+	size_t min_triangles = 0;
+	size_t max_triangles = T;
+	if (min_grid > 1 || lock)
+		min_triangles = count_at(min_grid);
+
+	int next_grid_size = int(sqrtf(float(target_cell_count)) + 0.5f);
+	for (int pass = 0; pass < 10 + interpolation_passes; ++pass)
+	{
+		if (min_triangles >= target_index_count / 3 || max_grid - min_grid <= 1)
+			break;
+		int grid_size = next_grid_size;
+		grid_size = (grid_size <= min_grid) ? min_grid + 1 : (grid_size >= max_grid ? max_grid - 1 : grid_size);
+		const size_t triangles = count_at(grid_size);
+		const float tip = sloppy_interpolate(float(size_t(target_index_count / 3)), float(min_grid), float(min_triangles), float(grid_size),
+		                                     float(triangles), float(max_grid), float(max_triangles));
+		if (triangles <= target_index_count / 3)
+		{
+			min_grid = grid_size;
+			min_triangles = triangles;
+		}
+		else
+		{
+			max_grid = grid_size;
+			max_triangles = triangles;
+		}
+		next_grid_size = (pass < interpolation_passes) ? int(tip + 0.5f) : (min_grid + max_grid) / 2;
+	}
+
+	if (min_triangles == 0)
+	{
+		if (out_result_error)
+			*out_result_error = 1.f;
+		return 0;
+	}
+
+	launch_sloppy_cells(s, b, min_grid, words + 1);
+	read_words(1, 1);
+	const uint32_t cell_count = h[1];
+	launch_sloppy_quadrics(s, b, cell_count);
+	launch_sloppy_filter(s, b, cell_count, destination, words + 2);
+	read_words(2, 2);
+	const uint32_t kept = h[2];
+	float err_sq;
+	memcpy(&err_sq, &h[3], sizeof(float));
+	if (out_result_error)
+		*out_result_error = sqrtf(err_sq);
+	return (size_t)kept * 3;
+}
+
 static void check_args(size_t index_count, size_t stride, size_t target, float target_error, size_t attr_stride, size_t attr_count, const float* weights)
 {
 	// preconditions of src/simplifier.cpp:2348-2357 (asserts in the reference)
@@ -929,6 +1101,64 @@ extern "C" size_t meshopt_simplifyWithAttributes(unsigned int* destination, cons
 	catch (const std::exception& e)
 	{
 		die(e.what());
+	}
+}
+
+extern "C" size_t meshopt_simplifySloppy(unsigned int* destination, const unsigned int* indices, size_t index_count, const float* vertex_positions,
+                                         size_t vertex_count, size_t vertex_positions_stride, const unsigned char* vertex_lock, size_t target_index_count,
+                                         float target_error, float* result_error)
+{
+	try
+	{
+		check_args(index_count, vertex_positions_stride, target_index_count, target_error, 0, 0, nullptr);
+		Context& ctx = context();
+		std::lock_guard<std::mutex> guard(ctx.mutex);
+		ctx.ensure_init();
+		hipStream_t s = ctx.stream;
+		const HostStaging h = plan_staging(index_count, vertex_positions, vertex_count, vertex_positions_stride, nullptr, 0, 0, vertex_lock);
+		char* d = static_cast<char*>(ctx.staging.get(h.total));
+		if (index_count)
+			HIP_CHECK(hipMemcpyAsync(d + h.off_idx, indices, index_count * 4, hipMemcpyHostToDevice, s));
+		if (h.pos_bytes)
+			HIP_CHECK(hipMemcpyAsync(d + h.off_pos, vertex_positions, h.pos_bytes, hipMemcpyHostToDevice, s));
+		if (h.lock_bytes)
+			HIP_CHECK(hipMemcpyAsync(d + h.off_lock, vertex_lock, h.lock_bytes, hipMemcpyHostToDevice, s));
+		unsigned int* d_dst = reinterpret_cast<unsigned int*>(d + h.off_dst);
+		size_t n = sloppy_on_device(ctx, d_dst, reinterpret_cast<const unsigned int*>(d + h.off_idx), index_count,
+		                            reinterpret_cast<const float*>(d + h.off_pos), vertex_count, vertex_positions_stride,
+		                            h.lock_bytes ? reinterpret_cast<const unsigned char*>(d + h.off_lock) : nullptr, target_index_count, target_error,
+		                            result_error);
+		if (n)
+			HIP_CHECK(hipMemcpyAsync(destination, d_dst, n * 4, hipMemcpyDeviceToHost, s));
+		HIP_CHECK(hipStreamSynchronize(s));
+		return n;
+	}
+	catch (const std::exception& e)
+	{
+		die(e.what());
+	}
+}
+
+extern "C" size_t meshopt_hip_simplify_sloppy_device(unsigned int* destination, const unsigned int* indices, size_t index_count, const float* vertex_positions,
+                                                     size_t vertex_count, size_t vertex_positions_stride, const unsigned char* vertex_lock,
+                                                     size_t target_index_count, float target_error, float* result_error)
+{
+	try
+	{
+		check_args(index_count, vertex_positions_stride, target_index_count, target_error, 0, 0, nullptr);
+		Context& ctx = context();
+		std::lock_guard<std::mutex> guard(ctx.mutex);
+		ctx.ensure_init();
+		size_t n = sloppy_on_device(ctx, destination, indices, index_count, vertex_positions, vertex_count, vertex_positions_stride, vertex_lock,
+		                            target_index_count, target_error, result_error);
+		HIP_CHECK(hipStreamSynchronize(ctx.stream));
+		t_last_error.clear();
+		return n;
+	}
+	catch (const std::exception& e)
+	{
+		t_last_error = e.what();
+		return (size_t)-1;
 	}
 }
 
