@@ -1,7 +1,7 @@
 """Benchmark of the MI355X simplifier: input triangles per second for one plain
 meshopt_simplify call on a device-resident mesh, next to the reference CPU library.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload icosphere8|grid256|seamed1024|patches4096|serving]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload icosphere8|grid256|seamed1024|patches4096|serving|sloppy]
 
 A step is one whole simplify call (setup, every pass, compaction of the result) on inputs
 already resident in HBM; the timed region is bracketed by device synchronisation.  Prints one
@@ -357,6 +357,68 @@ def bench_serving(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_sloppy(args):
+    """meshopt_simplifySloppy on the icosphere L8 mesh (target 10%, error 1e-2) with inputs resident in
+    HBM: one step is one whole call (the grid-size search, cells, cell quadrics, filtering)."""
+    import torch
+    import meshoptimizer_amd as M
+    from meshoptimizer_amd import workloads as W
+
+    m = W.icosphere(8)
+    target = W.target_count(m.index_count, 1, 10)
+    dev = torch.device("cuda:0")
+    ib = torch.from_numpy(m.indices.astype(np.int32)).to(dev)
+    vb = torch.from_numpy(np.ascontiguousarray(m.vertices[:, :3])).to(dev)
+    dest = torch.empty(m.index_count, dtype=torch.int32, device=dev)
+
+    def step():
+        return M.simplify_sloppy_device(dest.data_ptr(), ib.data_ptr(), m.index_count, vb.data_ptr(), m.vertex_count, 12, None,
+                                        target, 1e-2)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    n_gpu, e_gpu = step()
+    gpu_out = dest[:n_gpu].cpu().numpy().astype(np.uint32)
+    times = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ts = time.perf_counter()
+        step()
+        times.append(time.perf_counter() - ts)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms = elapsed / args.steps * 1e3
+    out = {
+        "metric": "input Mtri/s simplified (bit-exact vs CPU), meshopt_simplifySloppy",
+        "value": round(m.triangle_count / (elapsed / args.steps) / 1e6, 3),
+        "unit": "Mtri/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded generator)",
+        "config": {"workload": "meshopt_simplifySloppy, icosphere L8 (655,362 V / 1,310,720 T), target 10%, error 1e-2",
+                   "input_triangles": m.triangle_count, "output_triangles": n_gpu // 3},
+        "p50_ms": round(statistics.median(times) * 1e3, 4),
+        "p99_ms": round(float(np.percentile(np.array(times) * 1e3, 99)), 4),
+    }
+    if not args.no_cpu:
+        from oracle import oracle as O
+        impl = "ref" if O.have_ref() else "oracle"
+        v3 = np.ascontiguousarray(m.vertices[:, :3])
+        ro, eo = O.simplify_sloppy(m.indices, v3, target, 1e-2, impl=impl)
+        out["bit_exact"] = bool(np.array_equal(ro, gpu_out) and np.float32(eo).tobytes() == np.float32(e_gpu).tobytes())
+        cpu_t = []
+        t_start = time.perf_counter()
+        while time.perf_counter() - t_start < min(args.cpu_budget, 5.0) and len(cpu_t) < 20:
+            ts = time.perf_counter()
+            O.simplify_sloppy(m.indices, v3, target, 1e-2, impl=impl)
+            cpu_t.append(time.perf_counter() - ts)
+        med = statistics.median(cpu_t)
+        out["cpu_baseline"] = {"value": round(m.triangle_count / med / 1e6, 3), "unit": "Mtri/s", "cores": 1,
+                               "kind": "reference" if impl == "ref" else "port",
+                               "sample": f"{len(cpu_t)} single-threaded calls on the same mesh, median {med * 1e3:.2f} ms"}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -383,6 +445,8 @@ def main():
         return bench_patches(args)
     if args.workload == "serving":
         return bench_serving(args)
+    if args.workload == "sloppy":
+        return bench_sloppy(args)
 
     import torch
     import meshoptimizer_amd as M
