@@ -29,7 +29,7 @@ struct SloppyBuffers
 	uint32_t* entry_vals;    // 3T: entry ids 3t + k
 	uint32_t* entry_keys_alt;
 	uint32_t* entry_vals_alt;
-	uint32_t* cell_off;      // V + 1: entries per cell, then their exclusive scan
+	uint32_t* cell_off;      // V + 2: start of each cell's run in the sorted entries
 	uint32_t* sort_temp;
 	uint32_t* scan_temp;
 	Quad* cq;                // V: cell quadrics

@@ -42,10 +42,12 @@ __device__ __forceinline__ uint32_t mix32(uint32_t h)
 	return h;
 }
 
-// countTriangles (src/simplifier.cpp:2107-2121) at one grid size, ids computed on the fly
+// countTriangles (src/simplifier.cpp:2107-2121) at one grid size, ids computed on the fly; one
+// atomic per workgroup (same-address atomics serialise in L2, so the grid is kept small)
 __global__ void __launch_bounds__(kThreads) k_sloppy_count(const uint32_t* idx, uint32_t tris, const float4* pos, const uint8_t* lock, float cell_scale,
                                                            uint32_t* count)
 {
+	__shared__ uint32_t part[kThreads / 64];
 	uint32_t local = 0;
 	for (uint32_t t = blockIdx.x * kThreads + threadIdx.x; t < tris; t += gridDim.x * kThreads)
 	{
@@ -56,8 +58,17 @@ __global__ void __launch_bounds__(kThreads) k_sloppy_count(const uint32_t* idx, 
 	}
 	for (int o = 32; o > 0; o >>= 1)
 		local += __shfl_xor(local, o, 64);
-	if (__lane_id() == 0 && local)
-		atomicAdd(count, local);
+	if (__lane_id() == 0)
+		part[threadIdx.x >> 6] = local;
+	__syncthreads();
+	if (threadIdx.x == 0)
+	{
+		uint32_t sum = 0;
+		for (int w = 0; w < kThreads / 64; ++w)
+			sum += part[w];
+		if (sum)
+			atomicAdd(count, sum);
+	}
 }
 
 __global__ void k_sloppy_ids(const float4* pos, const uint8_t* lock, uint32_t n, float cell_scale, uint32_t* ids)
@@ -128,8 +139,7 @@ __global__ void k_cells(const uint32_t* rep, const uint32_t* scan, uint32_t n, u
 // fillCellQuadrics (src/simplifier.cpp:2169-2199), step 1: one entry 3t + k per corner that adds
 // triangle t's quadric to a cell (only corner 0 when the triangle lies in one cell); other
 // entries get the key cell_count and sort to the end
-__global__ void k_cell_entries(const uint32_t* idx, uint32_t tris, const uint32_t* cells, uint32_t cell_count, uint32_t* keys, uint32_t* vals,
-                               uint32_t* per_cell)
+__global__ void k_cell_entries(const uint32_t* idx, uint32_t tris, const uint32_t* cells, uint32_t cell_count, uint32_t* keys, uint32_t* vals)
 {
 	for (uint32_t e = blockIdx.x * kThreads + threadIdx.x; e < 3 * tris; e += gridDim.x * kThreads)
 	{
@@ -140,12 +150,29 @@ __global__ void k_cell_entries(const uint32_t* idx, uint32_t tris, const uint32_
 		const uint32_t key = (single && k) ? cell_count : ck;
 		keys[e] = key;
 		vals[e] = e;
-		if (key < cell_count)
-			atomicAdd(&per_cell[key], 1u);
 	}
 }
 
-// step 2: each cell sums its entries' triangle quadrics from zero in entry order (the stable sort
+// step 2: the start of each cell's run in the sorted keys (lower bound), off[cell_count] = the
+// number of entries that add to a cell
+__global__ void k_cell_runs(const uint32_t* keys, uint32_t n, uint32_t cell_count, uint32_t* off)
+{
+	for (uint32_t c = blockIdx.x * kThreads + threadIdx.x; c <= cell_count; c += gridDim.x * kThreads)
+	{
+		uint32_t lo = 0, hi = n;
+		while (lo < hi)
+		{
+			const uint32_t mid = lo + (hi - lo) / 2;
+			if (keys[mid] < c)
+				lo = mid + 1;
+			else
+				hi = mid;
+		}
+		off[c] = lo;
+	}
+}
+
+// step 3: each cell sums its entries' triangle quadrics from zero in entry order (the stable sort
 // kept the reference's (triangle, corner) order); weight 3 for a triangle inside one cell
 __global__ void k_cell_quadrics(const uint32_t* vals, const uint32_t* off, uint32_t cell_count, const uint32_t* idx, const uint32_t* cells,
                                 const float4* pos, Quad* cq)
@@ -315,7 +342,7 @@ void launch_sloppy_count(hipStream_t s, const SloppyBuffers& b, int grid_size)
 	const uint32_t tris = b.index_count / 3;
 	HIP_CHECK(hipMemsetAsync(b.count, 0, sizeof(uint32_t), s));
 	if (tris)
-		k_sloppy_count<<<grid_for(tris, 4096), kThreads, 0, s>>>(b.indices, tris, b.pos, b.lock, cell_scale_of(grid_size), b.count);
+		k_sloppy_count<<<grid_for(tris, 1024), kThreads, 0, s>>>(b.indices, tris, b.pos, b.lock, cell_scale_of(grid_size), b.count);
 }
 
 void launch_sloppy_cells(hipStream_t s, const SloppyBuffers& b, int grid_size, uint32_t* cell_count_dev)
@@ -340,14 +367,13 @@ static int bits_for(uint32_t max_value)
 void launch_sloppy_quadrics(hipStream_t s, const SloppyBuffers& b, uint32_t cell_count)
 {
 	const uint32_t tris = b.index_count / 3, n = b.vertex_count;
-	fill_u32(s, b.cell_off, 0, (size_t)cell_count + 1);
 	if (tris)
 	{
-		k_cell_entries<<<grid_for(3 * (size_t)tris), kThreads, 0, s>>>(b.indices, tris, b.cells, cell_count, b.entry_keys, b.entry_vals, b.cell_off);
+		k_cell_entries<<<grid_for(3 * (size_t)tris), kThreads, 0, s>>>(b.indices, tris, b.cells, cell_count, b.entry_keys, b.entry_vals);
 		radix_sort_pairs(s, b.entry_keys, b.entry_vals, b.entry_keys_alt, b.entry_vals_alt, 3 * (size_t)tris, nullptr, bits_for(cell_count),
 		                 b.sort_temp);
 	}
-	scan_exclusive(s, b.cell_off, b.cell_off, (size_t)cell_count + 1, nullptr, nullptr, b.scan_temp);
+	k_cell_runs<<<grid_for((size_t)cell_count + 1), kThreads, 0, s>>>(b.entry_keys, 3 * tris, cell_count, b.cell_off);
 	k_cell_quadrics<<<grid_for(cell_count), kThreads, 0, s>>>(b.entry_vals, b.cell_off, cell_count, b.indices, b.cells, b.pos, b.cq);
 	fill_u32(s, reinterpret_cast<uint32_t*>(b.best), kNone, 2 * (size_t)cell_count);
 	HIP_CHECK(hipMemsetAsync(b.error_bits, 0, sizeof(uint32_t), s));
