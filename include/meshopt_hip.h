@@ -25,6 +25,15 @@ extern "C"
 size_t meshopt_hip_simplify_device(unsigned int* destination, const unsigned int* indices, size_t index_count, const float* vertex_positions, size_t vertex_count, size_t vertex_positions_stride, const float* vertex_attributes, size_t vertex_attributes_stride, const float* attribute_weights, size_t attribute_count, const unsigned char* vertex_lock, size_t target_index_count, float target_error, unsigned int options, float* result_error);
 
 /**
+ * meshopt_simplifyWithUpdate on device-resident arrays: indices, vertex_positions,
+ * vertex_attributes and vertex_lock are device pointers, updated in place as the host entry
+ * updates its arrays; attribute_weights and result_error are host pointers.  Replaces the
+ * reference's meshopt_simplifyWithUpdate (src/meshoptimizer.h:560) for callers whose buffers
+ * live in HBM.  Returns the index count, or (size_t)-1 with meshopt_hip_last_error set.
+ */
+size_t meshopt_hip_simplify_with_update_device(unsigned int* indices, size_t index_count, float* vertex_positions, size_t vertex_count, size_t vertex_positions_stride, float* vertex_attributes, size_t vertex_attributes_stride, const float* attribute_weights, size_t attribute_count, const unsigned char* vertex_lock, size_t target_index_count, float target_error, unsigned int options, float* result_error);
+
+/**
  * meshopt_simplifySloppy on device-resident arrays (every pointer but result_error is a device
  * pointer); no counterpart in the reference (its entry point takes host arrays).  Returns the
  * number of indices written, or (size_t)-1 with meshopt_hip_last_error set.
@@ -68,6 +77,8 @@ typedef struct meshopt_hip_call_stats
 	unsigned int kinds[5];         /* primaries per kind: manifold, border, seam, complex, locked */
 	float result_error_sq;
 	float vertex_error_sq;
+	unsigned int solve_vertices; /* meshopt_simplifyWithUpdate: primaries the position solve visited */
+	unsigned int solve_rounds;   /* rounds of the ordered position solve */
 } meshopt_hip_call_stats;
 
 /* Copies up to max_passes pass records of the calling thread's last call; returns the pass count. */

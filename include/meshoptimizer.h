@@ -68,6 +68,17 @@ MESHOPTIMIZER_API size_t meshopt_simplify(unsigned int* destination, const unsig
 MESHOPTIMIZER_API size_t meshopt_simplifyWithAttributes(unsigned int* destination, const unsigned int* indices, size_t index_count, const float* vertex_positions, size_t vertex_count, size_t vertex_positions_stride, const float* vertex_attributes, size_t vertex_attributes_stride, const float* attribute_weights, size_t attribute_count, const unsigned char* vertex_lock, size_t target_index_count, float target_error, unsigned int options, float* result_error);
 
 /**
+ * Edge-collapse simplifier that also moves the surviving vertices: after the collapses, each
+ * remaining vertex that is not locked, on a border or on an attribute seam is solved to the
+ * optimum of its accumulated quadric (with attributes: of every wedge's attribute quadric, under
+ * a volume-preservation constraint), and the attributes are re-evaluated there.  The index
+ * buffer is simplified in place; vertex_positions and vertex_attributes are updated in place
+ * (externally locked vertices are left untouched).  Replaces meshopt_simplifyWithUpdate,
+ * src/meshoptimizer.h:560 (implementation src/simplifier.cpp:2642-2645, 2575-2599).
+ */
+MESHOPTIMIZER_API size_t meshopt_simplifyWithUpdate(unsigned int* indices, size_t index_count, float* vertex_positions, size_t vertex_count, size_t vertex_positions_stride, float* vertex_attributes, size_t vertex_attributes_stride, const float* attribute_weights, size_t attribute_count, const unsigned char* vertex_lock, size_t target_index_count, float target_error, unsigned int options, float* result_error);
+
+/**
  * Sloppy simplifier: vertices snap to a regular grid whose size is searched for the target, each
  * grid cell keeps its lowest-error vertex, collapsed and duplicate triangles are dropped.
  * Replaces meshopt_simplifySloppy, src/meshoptimizer.h:576 (implementation
@@ -116,6 +127,16 @@ inline size_t meshopt_simplifyWithAttributes(T* destination, const T* indices, s
 	size_t n = meshopt_simplifyWithAttributes(out.data(), in.data(), index_count, vertex_positions, vertex_count, vertex_positions_stride, vertex_attributes, vertex_attributes_stride, attribute_weights, attribute_count, vertex_lock, target_index_count, target_error, options, result_error);
 	for (size_t i = 0; i < n; ++i)
 		destination[i] = T(out[i]);
+	return n;
+}
+
+template <typename T>
+inline size_t meshopt_simplifyWithUpdate(T* indices, size_t index_count, float* vertex_positions, size_t vertex_count, size_t vertex_positions_stride, float* vertex_attributes, size_t vertex_attributes_stride, const float* attribute_weights, size_t attribute_count, const unsigned char* vertex_lock, size_t target_index_count, float target_error, unsigned int options = 0, float* result_error = NULL)
+{
+	std::vector<unsigned int> io(indices, indices + index_count);
+	size_t n = meshopt_simplifyWithUpdate(io.data(), index_count, vertex_positions, vertex_count, vertex_positions_stride, vertex_attributes, vertex_attributes_stride, attribute_weights, attribute_count, vertex_lock, target_index_count, target_error, options, result_error);
+	for (size_t i = 0; i < n; ++i)
+		indices[i] = T(io[i]);
 	return n;
 }
 #endif

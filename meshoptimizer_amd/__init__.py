@@ -10,6 +10,11 @@ with the reference's names and argument meaning:
   simplify_with_attributes(indices, positions, attributes, weights, lock, target_index_count,
                            target_error, options)
       -> meshopt_simplifyWithAttributes (src/meshoptimizer.h:537)
+  simplify_with_update(indices, positions, target_index_count, target_error, options,
+                       attributes, weights, lock)
+      -> meshopt_simplifyWithUpdate (src/meshoptimizer.h:560); updates its arrays in place
+  simplify_sloppy(indices, positions, target_index_count, target_error, lock)
+      -> meshopt_simplifySloppy (src/meshoptimizer.h:576)
   simplify_scale(positions)
       -> meshopt_simplifyScale (src/meshoptimizer.h:611)
   simplify_device(...)  device-resident variant (buffers already in HBM)
@@ -47,7 +52,9 @@ EXPORTED_SYMBOLS = (
     "meshopt_simplifyWithAttributes",
     "meshopt_simplifyScale",
     "meshopt_simplifySloppy",
+    "meshopt_simplifyWithUpdate",
     "meshopt_setAllocator",
+    "meshopt_hip_simplify_with_update_device",
     "meshopt_hip_simplify_sloppy_device",
     "meshopt_hip_simplify_device",
     "meshopt_hip_simplify_batch",
@@ -78,6 +85,8 @@ class CallStats(ctypes.Structure):
         ("kinds", ctypes.c_uint * 5),
         ("result_error_sq", ctypes.c_float),
         ("vertex_error_sq", ctypes.c_float),
+        ("solve_vertices", ctypes.c_uint),
+        ("solve_rounds", ctypes.c_uint),
     ]
 
 
@@ -107,6 +116,10 @@ def library() -> ctypes.CDLL:
         fn = getattr(lib, name)
         fn.restype = _SZ
         fn.argtypes = [_P, _P, _SZ, _P, _SZ, _SZ, _P, _SZ, _F, _P]
+    for name in ("meshopt_simplifyWithUpdate", "meshopt_hip_simplify_with_update_device"):
+        fn = getattr(lib, name)
+        fn.restype = _SZ
+        fn.argtypes = [_P, _SZ, _P, _SZ, _SZ, _P, _SZ, _P, _SZ, _P, _SZ, _F, _U, _P]
     lib.meshopt_hip_simplify_device.restype = _SZ
     lib.meshopt_hip_simplify_device.argtypes = [_P, _P, _SZ, _P, _SZ, _SZ, _P, _SZ, _P, _SZ, _P, _SZ, _F, _U, _P]
     lib.meshopt_hip_last_call_stats.restype = ctypes.c_int
@@ -186,6 +199,51 @@ def simplify_with_attributes(indices, positions, attributes, weights, lock, targ
         None if lk is None else lk.ctypes.data, int(target_index_count), float(target_error), int(options),
         ctypes.byref(err))
     return dest[:n].copy(), err.value
+
+
+def simplify_with_update(indices, positions, target_index_count, target_error=1e-2, options=0, *,
+                         attributes=None, weights=None, lock=None):
+    """meshopt_simplifyWithUpdate.  positions: float32 [V, >=3] row view and attributes: float32
+    [V, A] row view or None (they may view one interleaved buffer); both are UPDATED IN PLACE, as
+    the reference updates its arrays.  weights [A]; lock: uint8 [V] or None.
+    Returns (indices uint32 [n], result_error)."""
+    lib = library()
+    ib = np.array(indices, dtype=np.uint32, copy=True).reshape(-1)
+    positions = _rows(positions, "positions")
+    if positions.size and not positions.flags.writeable:
+        raise ValueError("positions must be writable (they are updated in place)")
+    aptr, astride, acount = None, 0, 0
+    if attributes is not None and np.asarray(attributes).size:
+        attributes = _rows(attributes, "attributes")
+        if not attributes.flags.writeable:
+            raise ValueError("attributes must be writable (they are updated in place)")
+        aptr, astride, acount = attributes.ctypes.data, attributes.strides[0], attributes.shape[1]
+    w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float32)
+    lk = None if lock is None else np.ascontiguousarray(lock, dtype=np.uint8)
+    err = ctypes.c_float(0)
+    n = lib.meshopt_simplifyWithUpdate(
+        ib.ctypes.data if ib.size else None, ib.size, positions.ctypes.data, positions.shape[0], positions.strides[0],
+        aptr, astride, None if w is None else w.ctypes.data, acount, None if lk is None else lk.ctypes.data,
+        int(target_index_count), float(target_error), int(options), ctypes.byref(err))
+    return ib[:n].copy(), err.value
+
+
+def simplify_with_update_device(indices_ptr, index_count, positions_ptr, vertex_count, position_stride,
+                                target_index_count, target_error, options=0, attributes_ptr=None, attribute_stride=0,
+                                weights=None, lock_ptr=None):
+    """meshopt_hip_simplify_with_update_device on device pointers (integers): indices, positions and
+    attributes are updated in place on the device.  Returns (index count, result_error); raises on error."""
+    lib = library()
+    w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float32)
+    acount = 0 if w is None else w.size
+    err = ctypes.c_float(0)
+    n = lib.meshopt_hip_simplify_with_update_device(
+        indices_ptr, index_count, positions_ptr, vertex_count, position_stride, attributes_ptr, attribute_stride,
+        None if w is None else w.ctypes.data, acount, lock_ptr, int(target_index_count), float(target_error), int(options),
+        ctypes.byref(err))
+    if n == ctypes.c_size_t(-1).value:
+        raise RuntimeError(lib.meshopt_hip_last_error().decode())
+    return n, err.value
 
 
 def simplify_sloppy(indices, positions, target_index_count, target_error=1e-2, lock=None):

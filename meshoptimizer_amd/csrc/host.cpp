@@ -317,6 +317,7 @@ struct Plan
 	uint32_t* kind_counts;
 	uint32_t* comp; // Prune: component per vertex
 	float* cerr;    // Prune: error per component
+	SolveBuffers solve; // meshopt_simplifyWithUpdate only
 };
 
 static size_t pow2_at_least(size_t n)
@@ -327,7 +328,7 @@ static size_t pow2_at_least(size_t n)
 	return p;
 }
 
-static void carve(Arena& a, Plan& p, size_t V_in, size_t V, size_t I, size_t A, bool sparse, bool prune)
+static void carve(Arena& a, Plan& p, size_t V_in, size_t V, size_t I, size_t A, bool sparse, bool prune, bool solve)
 {
 	Mesh& m = p.m;
 	size_t T = I / 3 + 1;
@@ -352,6 +353,7 @@ static void carve(Arena& a, Plan& p, size_t V_in, size_t V, size_t I, size_t A, 
 		m.loopback[k] = a.take<uint32_t>(Vp);
 	}
 	m.vq = a.take<Quad>(Vp);
+	m.vg = (A && solve) ? a.take<Grad>(Vp) : nullptr;
 	m.aq = A ? a.take<Quad>(Vp) : nullptr;
 	m.ag = A ? a.take<Grad>(Vp * A) : nullptr;
 	m.va = A ? a.take<float>(Vp * A) : nullptr;
@@ -408,6 +410,10 @@ static void carve(Arena& a, Plan& p, size_t V_in, size_t V, size_t I, size_t A, 
 	p.kind_counts = a.take<uint32_t>(8);
 	p.comp = prune ? a.take<uint32_t>(Vp) : nullptr;
 	p.cerr = prune ? a.take<float>(Vp) : nullptr;
+	p.solve.update = solve ? a.take<uint8_t>(Vp + 8) : nullptr;
+	p.solve.flags = solve ? a.take<uint32_t>(Vp + 1) : nullptr;
+	p.solve.list = solve ? a.take<uint32_t>(Vp) : nullptr;
+	p.solve.state = solve ? a.take<uint32_t>(Vp) : nullptr;
 
 	size_t biggest = std::max(std::max(I, cap), Vp + 1);
 	size_t temp_words = std::max(sort_temp_words(biggest), scan_temp_words(biggest + 16));
@@ -479,6 +485,8 @@ struct CallArgs
 	size_t attribute_stride;
 	const float* weights; // host
 	size_t attribute_count;
+	float* out_positions;  // device, meshopt_simplifyWithUpdate: positions written back (same layout as positions)
+	float* out_attributes; // device, the same for the attributes
 	const unsigned char* lock; // device
 	size_t target_index_count;
 	float target_error;
@@ -491,7 +499,10 @@ static size_t simplify_on_device(Context& ctx, const CallArgs& c, float* result_
 	hipStream_t s = ctx.stream;
 	const bool sparse = (c.options & kOptSparse) != 0;
 	const bool prune = (c.options & kOptPrune) != 0;
+	const bool solve = (c.options & kOptInternalSolve) != 0;
 	const size_t I = c.index_count;
+	if (solve && (!c.out_positions || (c.attribute_count && !c.out_attributes)))
+		throw std::invalid_argument("the solve needs writable positions (and attributes)");
 	const size_t V_in = c.vertex_count;
 
 	uint32_t amap[kMaxAttributes];
@@ -509,12 +520,12 @@ static size_t simplify_on_device(Context& ctx, const CallArgs& c, float* result_
 
 	Arena dry;
 	Plan plan;
-	carve(dry, plan, V_in, V_alloc, I, A, sparse, prune);
+	carve(dry, plan, V_in, V_alloc, I, A, sparse, prune, solve);
 	ctx.ensure_arena(dry.used + 4096);
 	Arena arena;
 	arena.base = ctx.arena;
 	arena.size = ctx.arena_size;
-	carve(arena, plan, V_in, V_alloc, I, A, sparse, prune);
+	carve(arena, plan, V_in, V_alloc, I, A, sparse, prune, solve);
 	Mesh& m = plan.m;
 
 	m.index_count = (uint32_t)I;
@@ -743,6 +754,42 @@ static size_t simplify_on_device(Context& ctx, const CallArgs& c, float* result_
 		vertex_error_sq = vertex_error_sq < cmax ? cmax : vertex_error_sq;
 	}
 
+	// meshopt_simplifyWithUpdate: move the surviving vertices and write them back
+	// (src/simplifier.cpp:2575-2599)
+	if (solve)
+	{
+		launch_solve_setup(s, m, plan.solve, result_count);
+		sync_scalars(ctx, m);
+		const uint32_t listed = h.work[4];
+		uint32_t round = 1, batch = 4;
+		while (h.work[5] < listed)
+		{
+			// every round solves at least the lowest pending primary
+			if (round > listed + 1)
+				throw std::runtime_error("position solve did not converge (internal error)");
+			launch_solve_rounds(s, m, plan.solve, listed, round, batch);
+			round += batch;
+			batch = std::min<uint32_t>(batch * 2, 64);
+			sync_scalars(ctx, m);
+		}
+		t_call.solve_vertices = listed;
+		t_call.solve_rounds = round - 1;
+
+		SolveOutput o;
+		o.positions = c.out_positions;
+		o.position_stride = c.position_stride / sizeof(float);
+		o.attributes = c.out_attributes;
+		o.attribute_stride = c.attribute_stride / sizeof(float);
+		o.vertex_lock = c.lock;
+		o.sparse_remap = sparse_remap;
+		for (size_t k = 0; k < A; ++k)
+		{
+			o.map.index[k] = amap[k];
+			o.map.weight[k] = aw[k];
+		}
+		launch_solve_finish(s, m, plan.solve, o);
+	}
+
 	if ((c.options & kOptInternalDebug) && !sparse)
 		launch_debug_bits(s, m, result_count);
 	if (sparse)
@@ -823,12 +870,15 @@ static HostStaging plan_staging(size_t index_count, const float* positions, size
 }
 
 // One host call on the given context (the caller holds whatever lock guards it): stage the used
-// spans of the inputs, run the device path, copy the result back.
+// spans of the inputs, run the device path, copy the result back.  With the solve bit the
+// staged position and attribute spans are updated on the device and copied back over the
+// caller's (whose buffers are then writable; bytes the solve leaves alone return unchanged).
 static size_t simplify_host_on(Context& ctx, const unsigned int* indices, unsigned int* destination, size_t index_count, const float* positions,
                                size_t vertex_count, size_t position_stride, const float* attributes, size_t attribute_stride, const float* weights,
                                size_t attribute_count, const unsigned char* lock, size_t target_index_count, float target_error, unsigned int options,
                                float* result_error)
 {
+	const bool update = (options & kOptInternalSolve) != 0;
 	ctx.ensure_init();
 	hipStream_t s = ctx.stream;
 
@@ -851,7 +901,7 @@ static size_t simplify_host_on(Context& ctx, const unsigned int* indices, unsign
 	if (lock_bytes)
 		HIP_CHECK(hipMemcpyAsync(d + off_lock, lock, lock_bytes, hipMemcpyHostToDevice, s));
 
-	CallArgs c;
+	CallArgs c{};
 	c.destination = reinterpret_cast<unsigned int*>(d + off_dst);
 	c.indices = reinterpret_cast<const unsigned int*>(d + off_idx);
 	c.index_count = index_count;
@@ -866,10 +916,16 @@ static size_t simplify_host_on(Context& ctx, const unsigned int* indices, unsign
 	c.target_index_count = target_index_count;
 	c.target_error = target_error;
 	c.options = options;
+	c.out_positions = update ? reinterpret_cast<float*>(d + off_pos) : nullptr;
+	c.out_attributes = update ? const_cast<float*>(d_attr) : nullptr;
 
 	size_t n = simplify_on_device(ctx, c, result_error);
 	if (n)
 		HIP_CHECK(hipMemcpyAsync(destination, c.destination, n * 4, hipMemcpyDeviceToHost, s));
+	if (update && h.pos_bytes)
+		HIP_CHECK(hipMemcpyAsync(const_cast<float*>(positions), d + off_pos, h.pos_bytes, hipMemcpyDeviceToHost, s));
+	if (update && h.attr_span)
+		HIP_CHECK(hipMemcpyAsync(const_cast<float*>(attributes), d + off_attr, h.attr_span, hipMemcpyDeviceToHost, s));
 	HIP_CHECK(hipStreamSynchronize(s));
 	return n;
 }
@@ -1104,6 +1160,24 @@ extern "C" size_t meshopt_simplifyWithAttributes(unsigned int* destination, cons
 	}
 }
 
+extern "C" size_t meshopt_simplifyWithUpdate(unsigned int* indices, size_t index_count, float* vertex_positions, size_t vertex_count,
+                                             size_t vertex_positions_stride, float* vertex_attributes, size_t vertex_attributes_stride,
+                                             const float* attribute_weights, size_t attribute_count, const unsigned char* vertex_lock,
+                                             size_t target_index_count, float target_error, unsigned int options, float* result_error)
+{
+	try
+	{
+		check_args(index_count, vertex_positions_stride, target_index_count, target_error, vertex_attributes_stride, attribute_count, attribute_weights);
+		return simplify_host(indices, indices, index_count, vertex_positions, vertex_count, vertex_positions_stride, vertex_attributes,
+		                     vertex_attributes_stride, attribute_weights, attribute_count, vertex_lock, target_index_count, target_error,
+		                     options | kOptInternalSolve, result_error);
+	}
+	catch (const std::exception& e)
+	{
+		die(e.what());
+	}
+}
+
 extern "C" size_t meshopt_simplifySloppy(unsigned int* destination, const unsigned int* indices, size_t index_count, const float* vertex_positions,
                                          size_t vertex_count, size_t vertex_positions_stride, const unsigned char* vertex_lock, size_t target_index_count,
                                          float target_error, float* result_error)
@@ -1174,7 +1248,7 @@ extern "C" size_t meshopt_hip_simplify_device(unsigned int* destination, const u
 		Context& ctx = context();
 		std::lock_guard<std::mutex> guard(ctx.mutex);
 		ctx.ensure_init();
-		CallArgs c;
+		CallArgs c{};
 		c.destination = destination;
 		c.indices = indices;
 		c.index_count = index_count;
@@ -1190,6 +1264,46 @@ extern "C" size_t meshopt_hip_simplify_device(unsigned int* destination, const u
 		c.target_error = target_error;
 		c.options = options & ~(unsigned int)kOptInternalSolve;
 		size_t n = simplify_on_device(ctx, c, result_error);
+		t_last_error.clear();
+		return n;
+	}
+	catch (const std::exception& e)
+	{
+		t_last_error = e.what();
+		return (size_t)-1;
+	}
+}
+
+extern "C" size_t meshopt_hip_simplify_with_update_device(unsigned int* indices, size_t index_count, float* vertex_positions, size_t vertex_count,
+                                                          size_t vertex_positions_stride, float* vertex_attributes, size_t vertex_attributes_stride,
+                                                          const float* attribute_weights, size_t attribute_count, const unsigned char* vertex_lock,
+                                                          size_t target_index_count, float target_error, unsigned int options, float* result_error)
+{
+	try
+	{
+		check_args(index_count, vertex_positions_stride, target_index_count, target_error, vertex_attributes_stride, attribute_count, attribute_weights);
+		Context& ctx = context();
+		std::lock_guard<std::mutex> guard(ctx.mutex);
+		ctx.ensure_init();
+		CallArgs c{};
+		c.destination = indices;
+		c.indices = indices;
+		c.index_count = index_count;
+		c.positions = vertex_positions;
+		c.vertex_count = vertex_count;
+		c.position_stride = vertex_positions_stride;
+		c.attributes = attribute_count ? vertex_attributes : nullptr;
+		c.attribute_stride = vertex_attributes_stride;
+		c.weights = attribute_weights;
+		c.attribute_count = attribute_count;
+		c.lock = vertex_lock;
+		c.target_index_count = target_index_count;
+		c.target_error = target_error;
+		c.options = options | kOptInternalSolve;
+		c.out_positions = vertex_positions;
+		c.out_attributes = attribute_count ? vertex_attributes : nullptr;
+		size_t n = simplify_on_device(ctx, c, result_error);
+		HIP_CHECK(hipStreamSynchronize(ctx.stream));
 		t_last_error.clear();
 		return n;
 	}
@@ -1221,6 +1335,8 @@ static int simplify_batch_on_device(Context& ctx, unsigned int* destination, uns
 	hipStream_t s = ctx.stream;
 	if (stride < 12 || stride > 256 || stride % 4)
 		throw std::invalid_argument("meshopt_hip_simplify_batch: invalid vertex stride");
+	if (options & kOptInternalSolve)
+		throw std::invalid_argument("meshopt_hip_simplify_batch: the solve bit is not accepted here");
 	if (!patch_count)
 		return 0;
 
@@ -1358,7 +1474,7 @@ static int simplify_batch_on_device(Context& ctx, unsigned int* destination, uns
 		if (h_index_offsets[p + 1] < h_index_offsets[p] || h_vertex_offsets[p + 1] < h_vertex_offsets[p] ||
 		    (h_index_offsets[p + 1] - h_index_offsets[p]) % 3 || h_targets[p] > h_index_offsets[p + 1] - h_index_offsets[p])
 			throw std::invalid_argument("meshopt_hip_simplify_batch: invalid index count, target or offsets");
-		CallArgs c;
+		CallArgs c{};
 		c.destination = destination + h_index_offsets[p];
 		c.indices = indices + h_index_offsets[p];
 		c.index_count = h_index_offsets[p + 1] - h_index_offsets[p];
@@ -1569,7 +1685,7 @@ static size_t request_device_bytes(const meshopt_hip_request& r)
 	Arena dry;
 	Plan plan;
 	carve(dry, plan, r.vertex_count, sparse ? std::min(r.vertex_count, r.index_count) : r.vertex_count, r.index_count, A, sparse,
-	      (r.options & kOptPrune) != 0);
+	      (r.options & kOptPrune) != 0, false);
 	size_t arena = dry.used + 4096;
 	arena += arena / 4;
 	const HostStaging h = plan_staging(r.index_count, r.vertex_positions, r.vertex_count, r.vertex_positions_stride, r.vertex_attributes,
@@ -1743,6 +1859,8 @@ extern "C" size_t meshopt_hip_server_submit(meshopt_hip_server* server, const me
 		const meshopt_hip_request& r = *request;
 		check_args(r.index_count, r.vertex_positions_stride, r.target_index_count, r.target_error, r.vertex_attributes_stride, r.attribute_count,
 		           r.attribute_weights);
+		if (r.options & kOptInternalSolve)
+			throw std::invalid_argument("meshopt_hip_server_submit: the solve bit is not accepted here");
 		std::unique_ptr<ServerTicket> t(new ServerTicket());
 		t->req = r;
 		if (r.attribute_count)

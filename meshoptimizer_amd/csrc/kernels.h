@@ -34,6 +34,7 @@ struct Mesh
 	int loop_cur;
 
 	Quad* vq;                // vertex quadrics (primaries)
+	Grad* vg;                // volume gradients (primaries; the position solve with attributes only)
 	Quad* aq;                // attribute quadrics (per wedge)
 	Grad* ag;                // attribute gradients (V x A)
 	float* va;               // weighted attributes (V x A)
@@ -78,6 +79,13 @@ struct Mesh
 	CallScalars* sc;
 };
 
+// attributes with weight > 0: caller's attribute index and weight of each
+struct AttrMap
+{
+	uint32_t index[kMaxAttributes];
+	float weight[kMaxAttributes];
+};
+
 // ---------------------------------------------------------------- setup (setup.hip)
 void launch_check_indices(hipStream_t s, const uint32_t* ib, size_t index_count, size_t vertex_count, uint32_t* flag);
 void launch_sparse_remap(hipStream_t s, const uint32_t* indices_in, uint32_t* indices_out, size_t index_count, size_t vertex_count,
@@ -113,6 +121,35 @@ void launch_apply(hipStream_t s, Mesh& m, size_t cand_count);
 void launch_remap_triangles(hipStream_t s, Mesh& m, size_t tri_count);
 void launch_debug_bits(hipStream_t s, Mesh& m, size_t index_count);
 void launch_sparse_backmap(hipStream_t s, uint32_t* ib, size_t index_count, const uint32_t* sparse_remap);
+
+// ---------------------------------------------------------------- vertex solve (solve.hip)
+// buffers of meshopt_simplifyWithUpdate's solve, carved next to the mesh
+struct SolveBuffers
+{
+	uint8_t* update;   // V: vertex of the result, or primary of one
+	uint32_t* flags;   // V + 1: primaries the position solve visits, then their scan
+	uint32_t* list;    // V: those primaries in index order
+	uint32_t* state;   // V: round that solved a listed primary (all-ones while pending, 0 otherwise)
+};
+// the caller's arrays written back (finalizeVertices); strides in floats
+struct SolveOutput
+{
+	float* positions;
+	size_t position_stride;
+	float* attributes;
+	size_t attribute_stride;
+	const uint8_t* vertex_lock;
+	const uint32_t* sparse_remap;
+	AttrMap map;
+};
+// marks the result's vertices, rebuilds the welded adjacency of the result, lists the
+// primaries to solve; returns nothing, the list size lands in sc->work[4]
+void launch_solve_setup(hipStream_t s, Mesh& m, const SolveBuffers& b, size_t result_count);
+// `rounds` rounds of the ordered position solve starting at round `first`; sc->work[5] counts
+// the primaries solved so far
+void launch_solve_rounds(hipStream_t s, Mesh& m, const SolveBuffers& b, uint32_t list_count, uint32_t first, uint32_t rounds);
+// wedge copies, attribute solve and the write-back into the caller's arrays
+void launch_solve_finish(hipStream_t s, Mesh& m, const SolveBuffers& b, const SolveOutput& o);
 
 // ---------------------------------------------------------------- Prune (prune.hip)
 // components of the welded mesh over the input index buffer: comp[V] (ids in order of each

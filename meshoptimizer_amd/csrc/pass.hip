@@ -1094,7 +1094,7 @@ __device__ __forceinline__ uint32_t next_wedge_above(const uint32_t* wedge, uint
 }
 
 __global__ void k_update_quadrics(const uint8_t* status, const uint32_t* r_r0, const uint32_t* r_r1, const CallScalars* sc_in, const uint32_t* cremap,
-                                  const uint32_t* wedge, const float4* pos, int attr_count, Quad* vq, Quad* aq, Grad* ag, CallScalars* sc)
+                                  const uint32_t* wedge, const float4* pos, int attr_count, Quad* vq, Grad* vg, Quad* aq, Grad* ag, CallScalars* sc)
 {
 	size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
 	if (k >= sc_in->stop || status[k] != kCommitted)
@@ -1105,6 +1105,13 @@ __global__ void k_update_quadrics(const uint8_t* status, const uint32_t* r_r0, c
 	Quad Q1 = load_quad(vq, r1);
 	quad_add(Q1, Q0);
 	store_quad(vq, r1, Q1);
+	if (vg)
+	{
+		// the class is a target of one collapse per pass at most: no other update of vg[r1]
+		Grad G1 = vg[r1];
+		grad_add(G1, vg[r0]);
+		vg[r1] = G1;
+	}
 
 	if (attr_count)
 	{
@@ -1179,7 +1186,7 @@ void launch_apply(hipStream_t s, Mesh& m, size_t cand_count)
 {
 	if (cand_count)
 		k_update_quadrics<<<grid_for(cand_count), kThreads, 0, s>>>(m.status, m.r_r0, m.r_r1, m.sc, m.cremap, m.wedge, m.pos, (int)m.attr_count,
-		                                                           m.vq, m.aq, m.ag, m.sc);
+		                                                           m.vq, m.vg, m.aq, m.ag, m.sc);
 	if (m.attr_count)
 		k_vertex_error_from_bits<<<1, 1, 0, s>>>(m.sc);
 	else

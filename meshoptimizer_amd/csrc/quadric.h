@@ -281,6 +281,21 @@ __device__ __forceinline__ void attr_term(Quad& Q, Grad& g, const AttrFrame& f, 
 	g.gw = w * gw;
 }
 
+// quadricVolumeGradient, src/simplifier.cpp:952-965: unit normal scaled by half the area, and
+// the plane offset through p0 with the same scale
+__device__ __forceinline__ void grad_volume(Grad& G, const V3& p0, const V3& p1, const V3& p2)
+{
+	V3 e1 = {p1.x - p0.x, p1.y - p0.y, p1.z - p0.z};
+	V3 e2 = {p2.x - p0.x, p2.y - p0.y, p2.z - p0.z};
+	V3 n = {e1.y * e2.z - e1.z * e2.y, e1.z * e2.x - e1.x * e2.z, e1.x * e2.y - e1.y * e2.x};
+	float half_area = v3_normalize(n) * 0.5f;
+
+	G.gx = n.x * half_area;
+	G.gy = n.y * half_area;
+	G.gz = n.z * half_area;
+	G.gw = (-p0.x * n.x - p0.y * n.y - p0.z * n.z) * half_area;
+}
+
 // hasTriangleFlip, src/simplifier.cpp:1182-1198
 __device__ __forceinline__ bool triangle_flips(const V3& a, const V3& b, const V3& c, const V3& d)
 {

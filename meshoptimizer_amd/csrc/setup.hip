@@ -601,11 +601,6 @@ void launch_extent_only(hipStream_t s, Mesh& m)
 }
 
 // rescaleAttributes, src/simplifier.cpp:605-621 (only attributes with weight > 0)
-struct AttrMap
-{
-	uint32_t index[kMaxAttributes];
-	float weight[kMaxAttributes];
-};
 
 __global__ void k_rescale_attributes(const float* attrs, size_t stride, const uint32_t* sparse_remap, size_t n, uint32_t count, AttrMap map, float* va)
 {
@@ -655,7 +650,7 @@ __device__ __forceinline__ bool edge_loop_qualifies(const uint8_t* kind, const u
 
 __global__ void k_vertex_quadrics(const uint32_t* ib, const float4* pos, const uint32_t* remap, const uint8_t* kind, const uint32_t* loop,
                                   const uint32_t* loopback, const uint32_t* woff, const uint32_t* wcorners, const uint8_t* vertex_lock,
-                                  const uint32_t* sparse_remap, size_t n, float factor, Quad* vq)
+                                  const uint32_t* sparse_remap, size_t n, float factor, Quad* vq, Grad* vg)
 {
 	size_t ri = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
 	if (ri >= n)
@@ -666,15 +661,26 @@ __global__ void k_vertex_quadrics(const uint32_t* ib, const float4* pos, const u
 
 	Quad Q;
 	quad_zero(Q);
+	Grad GV = {0.f, 0.f, 0.f, 0.f};
 	uint32_t begin = woff[r], end = woff[r + 1];
 
 	for (uint32_t p = begin; p < end; ++p)
 	{
 		uint32_t t = wcorners[p] / 3;
+		V3 p0 = load_pos(pos, ib[t * 3 + 0]), p1 = load_pos(pos, ib[t * 3 + 1]), p2 = load_pos(pos, ib[t * 3 + 2]);
 		Quad F;
-		quad_from_triangle(F, load_pos(pos, ib[t * 3 + 0]), load_pos(pos, ib[t * 3 + 1]), load_pos(pos, ib[t * 3 + 2]), 1.f);
+		quad_from_triangle(F, p0, p1, p2, 1.f);
 		quad_add(Q, F);
+		if (vg)
+		{
+			// volume gradients of the position solve, summed in the same corner order
+			Grad G;
+			grad_volume(G, p0, p1, p2);
+			grad_add(GV, G);
+		}
 	}
+	if (vg)
+		vg[r] = GV;
 
 	{
 		size_t rr = sparse_remap ? sparse_remap[r] : r;
@@ -725,7 +731,7 @@ void launch_vertex_quadrics(hipStream_t s, Mesh& m, const uint32_t* welded_off, 
 		return;
 	float factor = (options & kOptRegularizeLight) ? 1e-2f : ((options & kOptRegularize) ? 1e-1f : 1e-7f);
 	k_vertex_quadrics<<<grid_for(n), kThreads, 0, s>>>(m.ib[0], m.pos, m.remap, m.kind, m.loop[0], m.loopback[0], welded_off, welded_corners,
-	                                                   vertex_lock, sparse_remap, n, factor, m.vq);
+	                                                   vertex_lock, sparse_remap, n, factor, m.vq, m.vg);
 }
 
 // attribute quadrics and gradients (src/simplifier.cpp:1159-1179), per wedge (not welded),

@@ -34,6 +34,7 @@ class Stats(ctypes.Structure):
         ("kinds", ctypes.c_uint * 5),
         ("result_error_sq", ctypes.c_float),
         ("vertex_error_sq", ctypes.c_float),
+        ("solve_rounds", ctypes.c_uint),
         ("pass_", PassStats * MAX_PASSES),
     ]
 
@@ -46,7 +47,7 @@ class Stats(ctypes.Structure):
 
 
 class Config(ctypes.Structure):
-    _fields_ = [("perform_rounds", ctypes.c_int), ("round_window", ctypes.c_size_t)]
+    _fields_ = [("perform_rounds", ctypes.c_int), ("round_window", ctypes.c_size_t), ("solve_rounds", ctypes.c_int)]
 
 
 _P = ctypes.c_void_p
@@ -151,7 +152,7 @@ def simplify(indices, positions, target_index_count, target_error, options=0, *,
     st = Stats()
     if impl == "oracle":
         lib = oracle_lib()
-        cfg = Config(1 if rounds else 0, window)
+        cfg = Config(1 if rounds else 0, window, 0)
         n = lib.moracle_simplify_edge(
             dest.ctypes.data, indices.ctypes.data, indices.size, positions.ctypes.data, vcount, positions.strides[0],
             aptr, astride, _ptr(w), acount, _ptr(lk),
@@ -173,6 +174,57 @@ def simplify(indices, positions, target_index_count, target_error, options=0, *,
     if stats:
         return out, err.value, st
     return out, err.value
+
+
+def _writable_rows(a, name):
+    a = _rows(a, name)
+    if a.size and not a.flags.writeable:
+        raise ValueError(f"{name} must be writable (it is updated in place)")
+    return a
+
+
+SOLVE = 1 << 29  # the internal option bit meshopt_simplifyWithUpdate adds (src/simplifier.cpp:2340)
+
+
+def simplify_with_update(indices, positions, target_index_count, target_error, options=0, *,
+                         attributes=None, weights=None, lock=None, impl="oracle", solve_rounds=False, stats=False):
+    """meshopt_simplifyWithUpdate on the CPU: the C restatement (impl="oracle") or the compiled
+    reference (impl="ref").  positions [V, >=3] and attributes [V, A] are float32 row views (they
+    may share one interleaved buffer) and are UPDATED IN PLACE, as the reference does.
+    solve_rounds runs the oracle's position solve in dependency rounds (the device's schedule).
+    Returns (result_indices, result_error[, Stats])."""
+    ib = np.array(indices, dtype=np.uint32, copy=True).reshape(-1)
+    if ib.size == 0:
+        ib = np.zeros(1, dtype=np.uint32)[:0]
+    positions = _writable_rows(positions, "positions")
+    vcount = positions.shape[0]
+    err = ctypes.c_float(0)
+    w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float32)
+    lk = None if lock is None else np.ascontiguousarray(lock, dtype=np.uint8)
+    acount, aptr, astride = 0, None, 0
+    if attributes is not None and np.asarray(attributes).size:
+        attributes = _writable_rows(attributes, "attributes")
+        acount, aptr, astride = attributes.shape[1], attributes.ctypes.data, attributes.strides[0]
+    args = (ib.ctypes.data, ib.size, positions.ctypes.data, vcount, positions.strides[0], aptr, astride, _ptr(w), acount,
+            _ptr(lk), target_index_count, target_error, options, ctypes.byref(err))
+    sig = [_P, _SZ, _P, _SZ, _SZ, _P, _SZ, _P, _SZ, _P, _SZ, _F, _U, _P]
+    st = Stats()
+    if impl == "oracle":
+        cfg = Config(0, 0, 1 if solve_rounds else 0)
+        n = oracle_lib().moracle_simplify_edge(
+            ib.ctypes.data, ib.ctypes.data, ib.size, positions.ctypes.data, vcount, positions.strides[0], aptr, astride,
+            _ptr(w), acount, _ptr(lk), target_index_count, target_error, options | SOLVE, ctypes.byref(err),
+            ctypes.byref(cfg), ctypes.byref(st))
+    elif impl == "ref":
+        fn = ref_lib().meshopt_simplifyWithUpdate
+        fn.restype = _SZ
+        fn.argtypes = sig
+        n = fn(*args)
+    else:
+        raise ValueError(impl)
+    if stats:
+        return ib[:n].copy(), err.value, st
+    return ib[:n].copy(), err.value
 
 
 class SloppyTrace(ctypes.Structure):

@@ -38,6 +38,7 @@ enum
 	OPT_REGULARIZE = 1u << 4,
 	OPT_PERMISSIVE = 1u << 5,
 	OPT_REGULARIZE_LIGHT = 1u << 6,
+	OPT_INTERNAL_SOLVE = 1u << 29,
 	OPT_INTERNAL_DEBUG = 1u << 30,
 };
 
@@ -409,7 +410,7 @@ static void classify(unsigned char* kind, unsigned int* loop, unsigned int* loop
 /* ------------------------------------------------------------------------------------------ */
 /* rescale, src/simplifier.cpp:549-621 */
 
-static float rescale_positions(vec3* out, const float* pos, size_t vertex_count, size_t stride_float, const unsigned int* sparse)
+static float rescale_positions(vec3* out, const float* pos, size_t vertex_count, size_t stride_float, const unsigned int* sparse, float* offset)
 {
 	float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX};
 	float hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
@@ -448,6 +449,9 @@ static float rescale_positions(vec3* out, const float* pos, size_t vertex_count,
 			out[i].z = (out[i].z - lo[2]) * scale;
 		}
 	}
+
+	if (offset)
+		memcpy(offset, lo, sizeof(lo));
 
 	return extent;
 }
@@ -655,7 +659,23 @@ static void q_from_attributes(quadric* Q, qgrad* G, vec3 p0, vec3 p1, vec3 p2, c
 /* ------------------------------------------------------------------------------------------ */
 /* quadric fills, src/simplifier.cpp:1057-1179 */
 
-static void fill_face_quadrics(quadric* vq, const unsigned int* indices, size_t index_count, const vec3* vp, const unsigned int* remap)
+/* quadricVolumeGradient, src/simplifier.cpp:952-965: unit normal scaled by half the area, and
+ * the plane offset through p0 with the same scale */
+static void q_volume_gradient(qgrad* G, vec3 p0, vec3 p1, vec3 p2)
+{
+	vec3 e1 = {p1.x - p0.x, p1.y - p0.y, p1.z - p0.z};
+	vec3 e2 = {p2.x - p0.x, p2.y - p0.y, p2.z - p0.z};
+	vec3 n = {e1.y * e2.z - e1.z * e2.y, e1.z * e2.x - e1.x * e2.z, e1.x * e2.y - e1.y * e2.x};
+	float half_area = vnormalize(&n) * 0.5f;
+
+	G->gx = n.x * half_area;
+	G->gy = n.y * half_area;
+	G->gz = n.z * half_area;
+	G->gw = (-p0.x * n.x - p0.y * n.y - p0.z * n.z) * half_area;
+}
+
+/* fillFaceQuadrics, src/simplifier.cpp:1057-1083; vg (volume gradients) only for the solve */
+static void fill_face_quadrics(quadric* vq, qgrad* vg, const unsigned int* indices, size_t index_count, const vec3* vp, const unsigned int* remap)
 {
 	for (size_t i = 0; i < index_count; i += 3)
 	{
@@ -666,6 +686,15 @@ static void fill_face_quadrics(quadric* vq, const unsigned int* indices, size_t 
 		q_add(&vq[remap[i0]], &Q);
 		q_add(&vq[remap[i1]], &Q);
 		q_add(&vq[remap[i2]], &Q);
+
+		if (vg)
+		{
+			qgrad G;
+			q_volume_gradient(&G, vp[i0], vp[i1], vp[i2]);
+			g_add(&vg[remap[i0]], &G, 1);
+			g_add(&vg[remap[i1]], &G, 1);
+			g_add(&vg[remap[i2]], &G, 1);
+		}
 	}
 }
 
@@ -1486,7 +1515,7 @@ static perform_result perform_rounds(round_state* rs, unsigned int* cremap, unsi
 /* ------------------------------------------------------------------------------------------ */
 /* updateQuadrics, src/simplifier.cpp:1638-1673 */
 
-static void update_quadrics(const unsigned int* cremap, size_t vertex_count, quadric* vq, quadric* aq, qgrad* ag, size_t n, const vec3* vp, const unsigned int* remap, float* vertex_error)
+static void update_quadrics(const unsigned int* cremap, size_t vertex_count, quadric* vq, qgrad* vg, quadric* aq, qgrad* ag, size_t n, const vec3* vp, const unsigned int* remap, float* vertex_error)
 {
 	for (size_t i = 0; i < vertex_count; ++i)
 	{
@@ -1497,7 +1526,11 @@ static void update_quadrics(const unsigned int* cremap, size_t vertex_count, qua
 		unsigned int r0 = remap[i0], r1 = remap[i1];
 
 		if (i0 == r0)
+		{
 			q_add(&vq[r1], &vq[r0]);
+			if (vg)
+				g_add(&vg[r1], &vg[r0], 1);
+		}
 
 		if (n)
 		{
@@ -1510,6 +1543,306 @@ static void update_quadrics(const unsigned int* cremap, size_t vertex_count, qua
 				*vertex_error = *vertex_error < derr ? derr : *vertex_error;
 			}
 		}
+	}
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* the vertex solve of meshopt_simplifyWithUpdate, src/simplifier.cpp:2575-2599 */
+
+/* quadricSolve, src/simplifier.cpp:967-1022: LDL^T factorisation of the 3x3 block, the volume
+ * gradient appended as a fourth row (a linear constraint with a Lagrange multiplier, dropped
+ * when its pivot is tiny); returns 0 when one of the first three pivots is tiny */
+static int q_solve(vec3* p, const quadric* Q, const qgrad* GV)
+{
+	float eps = 1e-6f * Q->w;
+
+	float d0 = Q->a00;
+	float l10 = Q->a10 / d0;
+	float l20 = Q->a20 / d0;
+	float d1 = Q->a11 - Q->a10 * l10;
+	float dl21 = Q->a21 - Q->a20 * l10;
+	float l21 = dl21 / d1;
+	float d2 = Q->a22 - Q->a20 * l20 - dl21 * l21;
+
+	/* forward substitution of -b, then the diagonal */
+	float y0 = -Q->b0;
+	float y1 = -Q->b1 - l10 * y0;
+	float y2 = -Q->b2 - l20 * y0 - l21 * y1;
+	float z0 = y0 / d0;
+	float z1 = y1 / d1;
+	float z2 = y2 / d2;
+
+	/* constraint row */
+	float l30 = GV->gx / d0;
+	float dl31 = GV->gy - GV->gx * l10;
+	float l31 = dl31 / d1;
+	float dl32 = GV->gz - GV->gx * l20 - dl31 * l21;
+	float l32 = dl32 / d2;
+	float d3 = 0.f - GV->gx * l30 - dl31 * l31 - dl32 * l32;
+	float y3 = -GV->gw - l30 * y0 - l31 * y1 - l32 * y2;
+	float lambda = fabsf(d3) > eps ? y3 / d3 : 0.f;
+
+	/* back substitution */
+	p->z = z2 - l32 * lambda;
+	p->y = z1 - l21 * p->z - l31 * lambda;
+	p->x = z0 - l10 * p->y - l20 * p->z - l30 * lambda;
+
+	return fabsf(d0) > eps && fabsf(d1) > eps && fabsf(d2) > eps;
+}
+
+/* quadricReduceAttributes, src/simplifier.cpp:1024-1055: fold one wedge's attribute quadric
+ * (scaled by the vertex weight) and its gradients (the B B^T / a Schur term) into Q */
+static void q_reduce_attributes(quadric* Q, const quadric* A, const qgrad* G, size_t n)
+{
+	float w = Q->w;
+	Q->a00 += A->a00 * w;
+	Q->a11 += A->a11 * w;
+	Q->a22 += A->a22 * w;
+	Q->a10 += A->a10 * w;
+	Q->a20 += A->a20 * w;
+	Q->a21 += A->a21 * w;
+	Q->b0 += A->b0 * w;
+	Q->b1 += A->b1 * w;
+	Q->b2 += A->b2 * w;
+
+	float iaw = A->w == 0 ? 0.f : w / A->w;
+
+	for (size_t k = 0; k < n; ++k)
+	{
+		qgrad g = G[k];
+		Q->a00 -= (g.gx * g.gx) * iaw;
+		Q->a11 -= (g.gy * g.gy) * iaw;
+		Q->a22 -= (g.gz * g.gz) * iaw;
+		Q->a10 -= (g.gx * g.gy) * iaw;
+		Q->a20 -= (g.gx * g.gz) * iaw;
+		Q->a21 -= (g.gy * g.gz) * iaw;
+		Q->b0 -= (g.gx * g.gw) * iaw;
+		Q->b1 -= (g.gy * g.gw) * iaw;
+		Q->b2 -= (g.gz * g.gw) * iaw;
+	}
+}
+
+/* getNeighborhoodRadius, src/simplifier.cpp:1252-1276: distance to the farthest welded
+ * neighbour (both ends of every corner at i) */
+static float neighborhood_radius(const corner_csr* adj, const vec3* vp, unsigned int i)
+{
+	vec3 c = vp[i];
+	float far2 = 0.f;
+
+	for (unsigned int e = adj->offsets[i]; e < adj->offsets[i + 1]; ++e)
+	{
+		unsigned int ends[2] = {adj->next[e], adj->prev[e]};
+		for (int k = 0; k < 2; ++k)
+		{
+			vec3 q = vp[ends[k]];
+			float d = (q.x - c.x) * (q.x - c.x) + (q.y - c.y) * (q.y - c.y) + (q.z - c.z) * (q.z - c.z);
+			far2 = far2 < d ? d : far2;
+		}
+	}
+
+	return sqrtf(far2);
+}
+
+/* hasTriangleFlips for a point move, src/simplifier.cpp:1234-1250 */
+static int move_flips(const corner_csr* adj, const vec3* vp, unsigned int i, vec3 target)
+{
+	for (unsigned int e = adj->offsets[i]; e < adj->offsets[i + 1]; ++e)
+		if (tri_flips(vp[adj->next[e]], vp[adj->prev[e]], vp[i], target))
+			return 1;
+	return 0;
+}
+
+/* kinds whose position the solve keeps (attribute seams, borders, locked) */
+static int solve_pinned(unsigned char k)
+{
+	return k == K_LOCKED || k == K_SEAM || k == K_BORDER;
+}
+
+/* one vertex of solvePositions, src/simplifier.cpp:1697-1760: 1 and the new position, or 0 when
+ * a check keeps the old one */
+static int solve_one(vec3* out, const vec3* vp, unsigned int i, const quadric* vq, const qgrad* vg, const quadric* aq, const qgrad* ag, size_t n, const unsigned int* wedge, const corner_csr* adj)
+{
+	vec3 cur = vp[i];
+	quadric Q = vq[i];
+	quadric R;
+	q_from_point(&R, cur.x, cur.y, cur.z, Q.w * 1e-4f);
+	q_add(&Q, &R);
+
+	qgrad GV = {0, 0, 0, 0};
+	if (n)
+	{
+		unsigned int v = i;
+		do
+		{
+			q_reduce_attributes(&Q, &aq[v], &ag[(size_t)v * n], n);
+			v = wedge[v];
+		} while (v != i);
+
+		if (vg)
+			GV = vg[i];
+	}
+
+	vec3 p;
+	if (!q_solve(&p, &Q, &GV))
+		return 0;
+
+	float nr = neighborhood_radius(adj, vp, i);
+	float dp = (p.x - cur.x) * (p.x - cur.x) + (p.y - cur.y) * (p.y - cur.y) + (p.z - cur.z) * (p.z - cur.z);
+	if (dp > nr * nr)
+		return 0;
+
+	if (move_flips(adj, vp, i, p))
+		return 0;
+
+	if (q_error(&vq[i], p) > q_error(&vq[i], cur) * 1.5f + 1e-6f)
+		return 0;
+
+	*out = p;
+	return 1;
+}
+
+/* solvePositions, src/simplifier.cpp:1675-1763: vertices in index order; each reads the
+ * positions its lower-index neighbours already received.
+ * in_rounds != 0 replays the same walk the way the device runs it: in each round every listed
+ * primary whose listed lower-index neighbours were all solved in earlier rounds is solved from
+ * the positions as they stood at the start of the round, and only then are the round's results
+ * written.  It must give the sequential walk's positions bit for bit; *rounds gets the count. */
+static void solve_positions(vec3* vp, size_t vertex_count, const quadric* vq, const qgrad* vg, const quadric* aq, const qgrad* ag, size_t n, const unsigned int* remap, const unsigned int* wedge, const corner_csr* adj, const unsigned char* kind, const unsigned char* update, int in_rounds, unsigned int* rounds)
+{
+	if (!in_rounds)
+	{
+		for (size_t ii = 0; ii < vertex_count; ++ii)
+		{
+			unsigned int i = (unsigned int)ii;
+			if (!update[i] || solve_pinned(kind[i]))
+				continue;
+
+			vec3 p;
+			if (remap[i] != i)
+				vp[i] = vp[remap[i]];
+			else if (solve_one(&p, vp, i, vq, vg, aq, ag, n, wedge, adj))
+				vp[i] = p;
+		}
+		return;
+	}
+
+	unsigned int* state = (unsigned int*)xalloc(vertex_count * sizeof(unsigned int) + 4);
+	unsigned char* ready = (unsigned char*)xalloc(vertex_count + 1);
+	unsigned char* moved = (unsigned char*)xalloc(vertex_count + 1);
+	vec3* next = (vec3*)xalloc(vertex_count * sizeof(vec3) + 16);
+	size_t pending = 0;
+	for (size_t i = 0; i < vertex_count; ++i)
+	{
+		int listed = update[i] && remap[i] == i && !solve_pinned(kind[i]);
+		state[i] = listed ? NONE : 0;
+		pending += listed;
+	}
+
+	unsigned int round = 0;
+	while (pending)
+	{
+		round++;
+		for (size_t ii = 0; ii < vertex_count; ++ii)
+		{
+			unsigned int i = (unsigned int)ii;
+			ready[i] = state[i] == NONE;
+			for (unsigned int e = adj->offsets[i]; ready[i] && e < adj->offsets[i + 1]; ++e)
+			{
+				unsigned int a = adj->next[e], b = adj->prev[e];
+				if ((a < i && state[a] >= round) || (b < i && state[b] >= round))
+					ready[i] = 0;
+			}
+		}
+		for (size_t i = 0; i < vertex_count; ++i)
+			if (ready[i])
+				moved[i] = (unsigned char)solve_one(&next[i], vp, (unsigned int)i, vq, vg, aq, ag, n, wedge, adj);
+		for (size_t i = 0; i < vertex_count; ++i)
+			if (ready[i])
+			{
+				if (moved[i])
+					vp[i] = next[i];
+				state[i] = round;
+				pending--;
+			}
+	}
+
+	for (size_t i = 0; i < vertex_count; ++i)
+		if (update[i] && !solve_pinned(kind[i]) && remap[i] != i)
+			vp[i] = vp[remap[i]];
+
+	if (rounds)
+		*rounds = round;
+	free(state);
+	free(ready);
+	free(moved);
+	free(next);
+}
+
+/* solveAttributes, src/simplifier.cpp:1765-1809: each attribute of every wedge from its
+ * gradient at the solved position; complex vertices whose wedges shared a value keep sharing
+ * the value of the heaviest wedge */
+static void solve_attributes(const vec3* vp, float* va, size_t vertex_count, const quadric* aq, const qgrad* ag, size_t n, const unsigned int* remap, const unsigned int* wedge, const unsigned char* kind, const unsigned char* update)
+{
+	for (size_t ii = 0; ii < vertex_count; ++ii)
+	{
+		unsigned int i = (unsigned int)ii;
+		if (!update[i] || remap[i] != i)
+			continue;
+
+		for (size_t k = 0; k < n; ++k)
+		{
+			unsigned int shared = NONE;
+
+			if (kind[i] == K_COMPLEX)
+			{
+				shared = i;
+				for (unsigned int v = wedge[i]; v != i; v = wedge[v])
+				{
+					if (va[(size_t)v * n + k] != va[(size_t)i * n + k])
+						shared = NONE;
+					else if (shared != NONE && aq[v].w > aq[shared].w)
+						shared = v;
+				}
+			}
+
+			vec3 p = vp[i];
+			unsigned int v = i;
+			do
+			{
+				unsigned int src = shared == NONE ? v : shared;
+				const quadric* A = &aq[src];
+				const qgrad* G = &ag[(size_t)src * n + k];
+				float iw = A->w == 0 ? 0.f : 1.f / A->w;
+				va[(size_t)v * n + k] = (G->gx * p.x + G->gy * p.y + G->gz * p.z + G->gw) * iw;
+				v = wedge[v];
+			} while (v != i);
+		}
+	}
+}
+
+/* finalizeVertices, src/simplifier.cpp:623-663: positions back to the caller's space and
+ * attributes back to their unweighted values, skipping externally locked vertices */
+static void finalize_vertices(float* positions, size_t pos_stride_f, float* attributes, size_t attr_stride_f, const float* weights, const unsigned int* amap, size_t n, size_t vertex_count, const vec3* vp, const float* va, const unsigned int* sparse, float scale, const float* offset, const unsigned char* kind, const unsigned char* update, const unsigned char* vertex_lock)
+{
+	for (size_t i = 0; i < vertex_count; ++i)
+	{
+		if (!update[i])
+			continue;
+
+		size_t ri = sparse ? sparse[i] : i;
+		if (vertex_lock && (vertex_lock[ri] & VLOCK_LOCK))
+			continue;
+
+		if (kind[i] != K_LOCKED)
+		{
+			float* out = positions + ri * pos_stride_f;
+			out[0] = vp[i].x * scale + offset[0];
+			out[1] = vp[i].y * scale + offset[1];
+			out[2] = vp[i].z * scale + offset[2];
+		}
+
+		for (size_t k = 0; k < n; ++k)
+			attributes[ri * attr_stride_f + amap[k]] = va[i * n + k] / weights[amap[k]];
 	}
 }
 
@@ -1713,13 +2046,14 @@ size_t moracle_simplify_edge(unsigned int* destination, const unsigned int* indi
 	}
 
 	vec3* vp = (vec3*)xalloc(vertex_count * sizeof(vec3) + 16);
-	float vertex_scale = rescale_positions(vp, vertex_positions, vertex_count, stride_f, sparse);
+	float vertex_offset[3] = {0, 0, 0};
+	float vertex_scale = rescale_positions(vp, vertex_positions, vertex_count, stride_f, sparse, vertex_offset);
 
 	float* va = NULL;
 	size_t n = 0;
+	unsigned int amap[MAX_ATTR];
 	if (attribute_count)
 	{
-		unsigned int amap[MAX_ATTR];
 		for (size_t k = 0; k < attribute_count; ++k)
 			if (attribute_weights[k] > 0)
 				amap[n++] = (unsigned int)k;
@@ -1739,15 +2073,21 @@ size_t moracle_simplify_edge(unsigned int* destination, const unsigned int* indi
 
 	quadric* aq = NULL;
 	qgrad* ag = NULL;
+	qgrad* vg = NULL;
 	if (n)
 	{
+		if (options & OPT_INTERNAL_SOLVE)
+		{
+			vg = (qgrad*)xalloc(vertex_count * sizeof(qgrad) + 4);
+			memset(vg, 0, vertex_count * sizeof(qgrad));
+		}
 		aq = (quadric*)xalloc(vertex_count * sizeof(quadric));
 		memset(aq, 0, vertex_count * sizeof(quadric));
 		ag = (qgrad*)xalloc(vertex_count * n * sizeof(qgrad));
 		memset(ag, 0, vertex_count * n * sizeof(qgrad));
 	}
 
-	fill_face_quadrics(vq, result, index_count, vp, remap);
+	fill_face_quadrics(vq, vg, result, index_count, vp, remap);
 	fill_vertex_quadrics(vq, vp, vertex_count, remap, vertex_lock, sparse, options);
 	fill_edge_quadrics(vq, result, index_count, vp, remap, kind, loop, loopback);
 	if (n)
@@ -1830,7 +2170,7 @@ size_t moracle_simplify_edge(unsigned int* destination, const unsigned int* indi
 		if (pr.committed == 0)
 			break;
 
-		update_quadrics(cremap, vertex_count, vq, aq, ag, n, vp, remap, &vertex_error);
+		update_quadrics(cremap, vertex_count, vq, vg, aq, ag, n, vp, remap, &vertex_error);
 		vertex_error = n == 0 ? result_error : vertex_error;
 
 		remap_edge_loops(loop, vertex_count, cremap);
@@ -1860,6 +2200,27 @@ size_t moracle_simplify_edge(unsigned int* destination, const unsigned int* indi
 		result_count = new_count;
 		result_error = result_error < cmax ? cmax : result_error;
 		vertex_error = vertex_error < cmax ? cmax : vertex_error;
+	}
+
+	/* meshopt_simplifyWithUpdate: move the surviving vertices and write them back
+	 * (src/simplifier.cpp:2575-2599) */
+	if (options & OPT_INTERNAL_SOLVE)
+	{
+		unsigned char* update = locked;
+		memset(update, 0, vertex_count);
+		for (size_t i = 0; i < result_count; ++i)
+			update[result[i]] = update[remap[result[i]]] = 1;
+
+		csr_build(&adj, result, result_count, vertex_count, remap);
+
+		unsigned int solve_rounds = 0;
+		solve_positions(vp, vertex_count, vq, vg, aq, ag, n, remap, wedge, &adj, kind, update, config ? config->solve_rounds : 0, &solve_rounds);
+		if (stats)
+			stats->solve_rounds = solve_rounds;
+		if (n)
+			solve_attributes(vp, va, vertex_count, aq, ag, n, remap, wedge, kind, update);
+
+		finalize_vertices((float*)vertex_positions, stride_f, (float*)vertex_attributes, vertex_attributes_stride / sizeof(float), attribute_weights, amap, n, vertex_count, vp, va, sparse, vertex_scale, vertex_offset, kind, update, vertex_lock);
 	}
 
 	if ((options & OPT_INTERNAL_DEBUG) && !sparse)
@@ -1899,6 +2260,7 @@ size_t moracle_simplify_edge(unsigned int* destination, const unsigned int* indi
 	free(vq);
 	free(aq);
 	free(ag);
+	free(vg);
 	free(comp);
 	free(cerr);
 	free(cs);
@@ -1922,10 +2284,17 @@ size_t moracle_simplify_with_attributes(unsigned int* destination, const unsigne
 	return moracle_simplify_edge(destination, indices, index_count, vertex_positions, vertex_count, vertex_positions_stride, vertex_attributes, vertex_attributes_stride, attribute_weights, attribute_count, vertex_lock, target_index_count, target_error, options, result_error, NULL, NULL);
 }
 
+/* meshopt_simplifyWithUpdate, src/simplifier.cpp:2642-2645: in place, positions and attributes
+ * updated through the caller's pointers */
+size_t moracle_simplify_with_update(unsigned int* indices, size_t index_count, float* vertex_positions, size_t vertex_count, size_t vertex_positions_stride, float* vertex_attributes, size_t vertex_attributes_stride, const float* attribute_weights, size_t attribute_count, const unsigned char* vertex_lock, size_t target_index_count, float target_error, unsigned int options, float* result_error)
+{
+	return moracle_simplify_edge(indices, indices, index_count, vertex_positions, vertex_count, vertex_positions_stride, vertex_attributes, vertex_attributes_stride, attribute_weights, attribute_count, vertex_lock, target_index_count, target_error, options | OPT_INTERNAL_SOLVE, result_error, NULL, NULL);
+}
+
 /* meshopt_simplifyScale, src/simplifier.cpp:2938-2948 */
 float moracle_simplify_scale(const float* vertex_positions, size_t vertex_count, size_t vertex_positions_stride)
 {
-	return rescale_positions(NULL, vertex_positions, vertex_count, vertex_positions_stride / sizeof(float), NULL);
+	return rescale_positions(NULL, vertex_positions, vertex_count, vertex_positions_stride / sizeof(float), NULL, NULL);
 }
 
 /* ------------------------------------------------------------------------------------------ */
@@ -2120,7 +2489,7 @@ size_t moracle_simplify_sloppy(unsigned int* destination, const unsigned int* in
 	size_t target_cell_count = target_index_count / 6;
 
 	vec3* vp = (vec3*)xalloc((vertex_count + 1) * sizeof(vec3));
-	rescale_positions(vp, vertex_positions, vertex_count, vertex_positions_stride / sizeof(float), NULL);
+	rescale_positions(vp, vertex_positions, vertex_count, vertex_positions_stride / sizeof(float), NULL, NULL);
 
 	unsigned int* ids = (unsigned int*)xalloc((vertex_count + 1) * sizeof(unsigned int));
 

@@ -17,6 +17,7 @@ typedef struct
 {
 	int perform_rounds;  /* 0: sequential greedy (src/simplifier.cpp:1498); 1: round-based commit */
 	size_t round_window; /* initial rank window for the round-based commit; 0 = all candidates */
+	int solve_rounds;    /* meshopt_simplifyWithUpdate's position solve: 0 in index order, 1 in dependency rounds */
 } moracle_config;
 
 typedef struct
@@ -32,6 +33,7 @@ typedef struct
 	unsigned int kinds[5];
 	float result_error_sq;
 	float vertex_error_sq;
+	unsigned int solve_rounds; /* rounds of the position solve when run in rounds */
 	moracle_pass_stats pass[MORACLE_MAX_PASSES];
 } moracle_stats;
 
@@ -40,6 +42,9 @@ size_t moracle_simplify_edge(unsigned int* destination, const unsigned int* indi
 size_t moracle_simplify(unsigned int* destination, const unsigned int* indices, size_t index_count, const float* vertex_positions, size_t vertex_count, size_t vertex_positions_stride, size_t target_index_count, float target_error, unsigned int options, float* result_error);
 
 size_t moracle_simplify_with_attributes(unsigned int* destination, const unsigned int* indices, size_t index_count, const float* vertex_positions, size_t vertex_count, size_t vertex_positions_stride, const float* vertex_attributes, size_t vertex_attributes_stride, const float* attribute_weights, size_t attribute_count, const unsigned char* vertex_lock, size_t target_index_count, float target_error, unsigned int options, float* result_error);
+
+/* meshopt_simplifyWithUpdate: indices, positions and attributes updated in place */
+size_t moracle_simplify_with_update(unsigned int* indices, size_t index_count, float* vertex_positions, size_t vertex_count, size_t vertex_positions_stride, float* vertex_attributes, size_t vertex_attributes_stride, const float* attribute_weights, size_t attribute_count, const unsigned char* vertex_lock, size_t target_index_count, float target_error, unsigned int options, float* result_error);
 
 float moracle_simplify_scale(const float* vertex_positions, size_t vertex_count, size_t vertex_positions_stride);
 

@@ -4,6 +4,8 @@ reference's own tests, as data.
 Sources (inputs and expected outputs transcribed as values):
   demo/tests.cpp:1771-2558           C known-answer tests of meshopt_simplify* (1.2)
   demo/tests.cpp:1833-1882           meshopt_simplifySloppy (written to sloppy_known_answers.json)
+  demo/tests.cpp:2590-2670,          meshopt_simplifyWithUpdate (written to update_known_answers.json)
+  js/meshopt_simplifier.test.js:115-138
   js/meshopt_simplifier.test.js:25-163  the WASM build's plain-text expected arrays
 
 Each case: call ("simplify" | "simplifyWithAttributes"), vertices [V][F] float32 values,
@@ -240,6 +242,38 @@ def build_sloppy():
     return cases
 
 
+def build_update():
+    """meshopt_simplifyWithUpdate known answers: demo/tests.cpp:2590-2670 (simplifyUpdate,
+    simplifyUpdateLocked with options 0 and Sparse) and js/meshopt_simplifier.test.js:115-138 (the
+    same mesh with separate position and attribute arrays).  "update_checks" hold the tests'
+    assertions on the updated vertex buffer: rows of the vertex array (positions then the
+    attribute column), each "approx" [value, tolerance], "abs_lt" tolerance or "eq" value."""
+    vb = [[0, 0, 0, 0], [1, 1, 0, 0], [2, 0, 0, 0], [0.9, 0.2, 0.1, 0.2], [1.1, 0.2, 0.1, 0.1]]
+    ib = [0, 1, 3, 3, 1, 4, 4, 1, 2, 0, 3, 2, 3, 4, 2]
+    expected = [0, 1, 3, 3, 1, 2, 0, 3, 2]
+    border = [{"row": i, "col": 3, "abs_lt": 1e-6} for i in range(3)]
+    moved = [{"row": 3, "col": c, "approx": [v, 1e-2]} for c, v in enumerate((0.88, 0.19, 0.11, 0.18))]
+    exact = [{"row": 3, "col": c, "eq": float(f32(v))} for c, v in enumerate((0.9, 0.2, 0.1, 0.2))]
+    cases = []
+    c = case("simplifyUpdate", "demo/tests.cpp:2590-2626", vb, ib, 9, 1.0, call="simplifyWithUpdate", vertex_floats=4,
+             attr={"interleaved": 3, "count": 1}, weights=[1.0], expected_count=9, expected=expected)
+    c["update_checks"] = border + moved
+    cases.append(c)
+    for name, opts in (("simplifyUpdateLocked", 0), ("simplifyUpdateLocked.sparse", SPARSE)):
+        c = case(name, "demo/tests.cpp:2628-2670, 3654-3655", vb, ib, 9, 1.0, opts, call="simplifyWithUpdate",
+                 vertex_floats=4, attr={"interleaved": 3, "count": 1}, weights=[1.0], lock=[0, 0, 0, 1, 0],
+                 expected_count=9, expected=expected)
+        c["update_checks"] = border + exact
+        cases.append(c)
+    pos = [row[:3] for row in vb]
+    c = case("js.simplifyUpdate", "js/meshopt_simplifier.test.js:115-138", pos, ib, 9, 1.0, call="simplifyWithUpdate",
+             attr=[[row[3]] for row in vb], weights=[1.0], expected_count=9, expected=expected)
+    # the separate attribute array is column 3 of the checked rows
+    c["update_checks"] = border + moved
+    cases.append(c)
+    return cases
+
+
 def main():
     cases = build()
     here = os.path.dirname(os.path.abspath(__file__))
@@ -252,6 +286,11 @@ def main():
     with open(path, "w") as f:
         json.dump({"reference": "meshoptimizer 1.2 (MESHOPTIMIZER_VERSION 1020)", "cases": sloppy}, f, indent=1)
     print(f"wrote {len(sloppy)} cases to {path}")
+    update = build_update()
+    path = os.path.join(here, "update_known_answers.json")
+    with open(path, "w") as f:
+        json.dump({"reference": "meshoptimizer 1.2 (MESHOPTIMIZER_VERSION 1020)", "cases": update}, f, indent=1)
+    print(f"wrote {len(update)} cases to {path}")
 
 
 if __name__ == "__main__":

@@ -48,3 +48,27 @@ def check(case, result, error):
     if "error_approx" in case:
         val, tol = case["error_approx"]
         assert abs(error - val) < tol, f"{name}: error {error} vs {val}"
+
+
+def updated_rows(positions, attributes):
+    """The vertex rows an update check reads: xyz then the attribute columns."""
+    pos = np.asarray(positions)[:, :3]
+    if attributes is None or np.asarray(attributes).size == 0:
+        return np.array(pos, dtype=np.float32)
+    return np.concatenate([pos, np.asarray(attributes)], axis=1).astype(np.float32)
+
+
+def check_update(case, positions, attributes):
+    """Asserts the reference test's expectations on the vertex buffer a
+    meshopt_simplifyWithUpdate call left behind."""
+    rows = updated_rows(positions, attributes)
+    for c in case.get("update_checks", []):
+        v = float(rows[c["row"], c["col"]])
+        where = f"{case['name']}: vertex {c['row']} column {c['col']} = {v!r}"
+        if "approx" in c:
+            val, tol = c["approx"]
+            assert abs(v - val) < tol, where
+        if "abs_lt" in c:
+            assert abs(v) < c["abs_lt"], where
+        if "eq" in c:
+            assert v == c["eq"], where

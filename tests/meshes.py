@@ -57,3 +57,26 @@ def scattered_components(rng, pieces):
     tri = np.concatenate(ibs)
     tri = tri[rng.permutation(tri.shape[0])]
     return tri.reshape(-1).astype(np.uint32), np.ascontiguousarray(np.concatenate(vs))
+
+
+# option sets the update and parity fuzzers cycle through (bits of src/meshoptimizer.h:466-482)
+UPDATE_OPTIONS = [0, 1, 2, 4, 16, 32, 64, 1 | 32, 2 | 8, 8, 1 | 8, 32 | 16]
+
+
+def update_case(seed):
+    """A random meshopt_simplifyWithUpdate call: (indices, vertex buffer [V, 7] = xyz + 4 more
+    columns, attribute count 0-3 taken from column 3 on, target, error, options, weights (one
+    attribute of weight 0 when there are two or more), lock or None)."""
+    rng = np.random.default_rng(seed)
+    opts = UPDATE_OPTIONS[seed % len(UPDATE_OPTIONS)]
+    if seed % 2:
+        ib, v = random_soup(rng, nv=int(rng.integers(20, 120)), ntri=int(rng.integers(10, 300)))
+    else:
+        ib, v = random_grid(rng, n=int(rng.integers(5, 16)))
+    na = int(rng.integers(0, 4))
+    vb = np.concatenate([v, rng.random((v.shape[0], 3), dtype=np.float32)], axis=1)
+    target = (ib.size // 3 * int(rng.integers(0, 4)) // 4) * 3
+    err = float(rng.choice([1e-3, 1e-2, 1e-1, 1.0]))
+    weights = np.array([0.5, 0.0, 2.0][:na], np.float32) if na else None
+    lock = rng.integers(0, 8, size=v.shape[0]).astype(np.uint8) if seed % 5 == 0 else None
+    return ib, vb, na, target, err, opts, weights, lock

@@ -1,0 +1,113 @@
+"""GPU parity of meshopt_simplifyWithUpdate (src/simplifier.cpp:2642-2645, 2575-2599): the HIP path
+against the oracle (the C restatement, pinned to the compiled reference in test_oracle.py) on
+the reference's known answers, random meshes and full-size workloads.  The index buffer, the
+result_error bits and every byte of the updated vertex buffer must be identical."""
+import sys
+
+import numpy as np
+import pytest
+
+import kat
+from meshes import same_bits, update_case
+from oracle import oracle as O
+import meshoptimizer_amd as M
+from meshoptimizer_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+CASES = kat.load_cases("update_known_answers.json")
+
+
+def announce(what):
+    print(f"[case] {what}", flush=True)
+    sys.stdout.flush()
+
+
+def run(ib, vb, na, target, err, opts, weights, lock, impl, separate=False):
+    """One call on a copy of vb (xyz + attribute columns from 3 on).  separate=True hands the
+    library packed position and attribute arrays instead of views of one interleaved buffer.
+    Returns (indices, error, updated vertex rows [V, 3 + na])."""
+    if separate:
+        pos = np.ascontiguousarray(vb[:, :3]).copy()
+        attr = np.ascontiguousarray(vb[:, 3:3 + na]).copy() if na else None
+    else:
+        b = vb.copy()
+        pos, attr = b[:, :3], (b[:, 3:3 + na] if na else None)
+    if impl == "gpu":
+        r, e = M.simplify_with_update(ib, pos, target, err, opts, attributes=attr, weights=weights, lock=lock)
+    else:
+        r, e = O.simplify_with_update(ib, pos, target, err, opts, attributes=attr, weights=weights, lock=lock)
+    rows = pos if attr is None else np.concatenate([pos, attr], axis=1)
+    return r, e, np.ascontiguousarray(rows)
+
+
+def assert_parity(ib, vb, na, target, err, opts=0, weights=None, lock=None, separate=False):
+    rg, eg, bg = run(ib, vb, na, target, err, opts, weights, lock, "gpu", separate)
+    ro, eo, bo = run(ib, vb, na, target, err, opts, weights, lock, "oracle")
+    assert rg.size == ro.size, f"index count {rg.size} != oracle {ro.size}"
+    assert np.array_equal(rg, ro), f"first index difference at {int(np.argmax(rg != ro))}"
+    assert same_bits(eg, eo), f"result_error {eg!r} != oracle {eo!r}"
+    diff = np.argwhere(bg.view(np.uint32) != bo.view(np.uint32))
+    assert diff.size == 0, f"{len(diff)} vertex floats differ, first (vertex, column) {diff[0].tolist()}: {bg[tuple(diff[0])]!r} vs {bo[tuple(diff[0])]!r}"
+    return rg, eg, bg
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c["name"] for c in CASES])
+def test_update_known_answers(case):
+    announce(case["name"])
+    a = kat.arrays(case)
+    r, e = M.simplify_with_update(a["indices"], a["positions"], case["target_index_count"], case["target_error"],
+                                  case["options"], attributes=a["attributes"], weights=a["weights"], lock=a["lock"])
+    kat.check(case, r, e)
+    kat.check_update(case, a["positions"], a["attributes"])
+    b = kat.arrays(case)
+    ro, eo = O.simplify_with_update(b["indices"], b["positions"], case["target_index_count"], case["target_error"],
+                                    case["options"], attributes=b["attributes"], weights=b["weights"], lock=b["lock"])
+    assert np.array_equal(r, ro) and same_bits(e, eo)
+    assert kat.updated_rows(a["positions"], a["attributes"]).tobytes() == kat.updated_rows(b["positions"], b["attributes"]).tobytes()
+
+
+@pytest.mark.parametrize("seed", range(60))
+def test_update_random(seed):
+    ib, vb, na, target, err, opts, weights, lock = update_case(seed)
+    announce(f"update random seed={seed} I={ib.size} attrs={na} opts={opts} target={target} err={err} lock={lock is not None}")
+    assert_parity(ib, vb, na, target, err, opts, weights, lock, separate=seed % 4 == 3)
+
+
+@pytest.mark.parametrize("name,den,opts,attrs", [("grid256", 10, 0, 0), ("icosphere6", 10, 0, 0), ("seamed256", 5, 0, 2),
+                                                 ("seamed256", 5, 2, 2), ("seamed256", 20, 8, 2), ("grid256", 4, 1, 0)])
+def test_update_workloads(name, den, opts, attrs):
+    """Full-size meshes: a 256x256 row-major grid (a dependency chain of a few hundred rounds), a
+    subdivided sphere (a dozen rounds), a seamed grid with two attribute columns (wedges, the
+    volume constraint, complex vertices) under Sparse and Prune."""
+    m = {"grid256": lambda: W.grid(256), "icosphere6": lambda: W.icosphere(6),
+         "seamed256": lambda: W.seamed_grid(256, seams=8, seed=5)}[name]()
+    target = W.target_count(m.index_count, 1, den)
+    announce(f"update {name} target 1/{den} opts {opts} attrs {attrs}")
+    # attribute columns: a normal component (equal across a seam's wedges) and U (cut by seams)
+    vb = np.ascontiguousarray(m.vertices[:, [0, 1, 2, 3, 6][:3 + attrs]], dtype=np.float32)
+    na = attrs
+    w = np.array([1.0, 0.5][:na], np.float32) if na else None
+    r, e, b = assert_parity(m.indices, vb, na, target, 1e-2, opts, w)
+    assert 0 < r.size <= m.index_count
+    call, _ = M.last_call_stats()
+    assert call["solve_vertices"] > 0 and call["solve_rounds"] > 0
+
+
+def test_update_device_entry():
+    """meshopt_hip_simplify_with_update_device on device-resident arrays updates them exactly as the
+    host entry updates host arrays."""
+    from devmem import DeviceArray
+    m = W.seamed_grid(64, seams=4, seed=9)
+    vb = np.ascontiguousarray(m.vertices[:, [0, 1, 2, 3, 6]], dtype=np.float32)
+    w = np.array([1.0, 0.25], np.float32)
+    target = W.target_count(m.index_count, 1, 6)
+    announce("update device entry")
+    rh, eh, bh = run(m.indices, vb, 2, target, 1e-2, 0, w, None, "gpu")
+    with DeviceArray(m.indices) as d_ib, DeviceArray(vb) as d_vb:
+        n, e = M.simplify_with_update_device(d_ib.ptr, m.index_count, d_vb.ptr, vb.shape[0], 20, target, 1e-2, 0,
+                                             attributes_ptr=d_vb.ptr + 12, attribute_stride=20, weights=w)
+        rd = d_ib.to_host(np.uint32, n)
+        bd = d_vb.to_host(np.float32, vb.size).reshape(vb.shape)
+    assert np.array_equal(rd, rh) and same_bits(e, eh)
+    assert bd.tobytes() == bh.tobytes()

@@ -4,7 +4,7 @@ import numpy as np
 import pytest
 
 import kat
-from meshes import random_grid, random_soup, same_bits
+from meshes import random_grid, random_soup, same_bits, update_case
 from oracle import oracle as O
 from meshoptimizer_amd import workloads as W
 
@@ -179,3 +179,93 @@ def test_sloppy_closed_form():
     tri = m.indices.reshape(-1, 3)
     rot = [tuple(np.roll(t, -int(np.argmin(t)))) for t in tri]
     assert [tuple(x) for x in r.reshape(-1, 3)] == rot
+
+
+UPDATE_CASES = kat.load_cases("update_known_answers.json")
+
+
+def _update_run(ib, vb, na, target, err, opts, weights, lock, impl, rounds=False):
+    """meshopt_simplifyWithUpdate on a copy of the vertex buffer vb (xyz, then na attribute
+    columns); returns (indices, error, updated copy)."""
+    b = vb.copy()
+    r, e = O.simplify_with_update(ib, b[:, :3], target, err, opts, attributes=b[:, 3:3 + na] if na else None,
+                                  weights=weights, lock=lock, impl=impl, solve_rounds=rounds)
+    return r, e, b
+
+
+@pytest.mark.parametrize("impl", ["oracle", "rounds", "ref"])
+@pytest.mark.parametrize("case", UPDATE_CASES, ids=[c["name"] for c in UPDATE_CASES])
+def test_update_known_answers(case, impl):
+    """meshopt_simplifyWithUpdate known answers (demo/tests.cpp:2590-2670,
+    js/meshopt_simplifier.test.js:115-138): index buffer, moved centre vertex, border vertices
+    without drift, locked vertex untouched; in index order, in dependency rounds, and on the
+    compiled reference."""
+    if impl == "ref":
+        _ref_or_skip()
+    a = kat.arrays(case)
+    r, e = O.simplify_with_update(a["indices"], a["positions"], case["target_index_count"], case["target_error"],
+                                  case["options"], attributes=a["attributes"], weights=a["weights"], lock=a["lock"],
+                                  impl="ref" if impl == "ref" else "oracle", solve_rounds=impl == "rounds")
+    kat.check(case, r, e)
+    kat.check_update(case, a["positions"], a["attributes"])
+
+
+@pytest.mark.parametrize("seed", range(40))
+def test_update_oracle_matches_reference(seed):
+    """The solve restatement against the compiled reference on random meshes with every option
+    set, 0-3 attributes (one of weight 0) and random locks: index buffer, error bits and every
+    byte of the updated vertex buffer; the round schedule must agree bit for bit as well."""
+    _ref_or_skip()
+    args = update_case(seed)
+    r1, e1, b1 = _update_run(*args, impl="ref")
+    r2, e2, b2 = _update_run(*args, impl="oracle")
+    r3, e3, b3 = _update_run(*args, impl="oracle", rounds=True)
+    assert r1.tolist() == r2.tolist() == r3.tolist()
+    assert same_bits(e1, e2) and same_bits(e1, e3)
+    assert b1.tobytes() == b2.tobytes() == b3.tobytes()
+
+
+@pytest.mark.parametrize("name,opts", [("grid64", 0), ("seamed64", 0), ("seamed64", 2), ("icosphere4", 8)])
+def test_update_oracle_matches_reference_workloads(name, opts):
+    """Workload meshes through the solve: a row-major grid (a long dependency chain), a seamed
+    grid with two attribute columns (wedges, complex vertices, the volume constraint)."""
+    _ref_or_skip()
+    m = {"grid64": lambda: W.grid(64), "seamed64": lambda: W.seamed_grid(64, seams=4, seed=3),
+         "icosphere4": lambda: W.icosphere(4)}[name]()
+    vb = np.ascontiguousarray(m.vertices, dtype=np.float32)
+    na = vb.shape[1] - 3
+    w = np.ones(na, np.float32) if na else None
+    args = (m.indices, vb, na, W.target_count(m.index_count, 1, 10), 1e-2, opts, w, None)
+    r1, e1, b1 = _update_run(*args, impl="ref")
+    r2, e2, b2 = _update_run(*args, impl="oracle", rounds=True)
+    assert r1.tolist() == r2.tolist() and same_bits(e1, e2)
+    assert b1.tobytes() == b2.tobytes()
+    assert (b1[:, :3] != vb[:, :3]).any()  # the solve moved something
+
+
+@pytest.mark.parametrize("seed", range(20))
+def test_update_rounds_match_index_order(seed):
+    """The device's schedule (dependency rounds, each solved from the positions at its start)
+    replays the reference's index-order walk exactly; checked on the restatement alone."""
+    args = update_case(1000 + seed)
+    r1, e1, b1 = _update_run(*args, impl="oracle")
+    r2, e2, b2 = _update_run(*args, impl="oracle", rounds=True)
+    assert r1.tolist() == r2.tolist() and b1.tobytes() == b2.tobytes()
+
+
+def test_update_closed_form():
+    """Mathematics the solve must respect without any collapse (target = every index):
+    every face plane of a vertex's fan passes through the vertex, so its quadric error is zero
+    there and the optimum (with the point regulariser) is the vertex itself up to rounding; and
+    a mesh whose vertices all lie on one line has zero-area faces, an all-zero quadric, a
+    singular system, and keeps every vertex exactly (up to the rescale round trip)."""
+    m = W.grid(9)
+    vb = np.ascontiguousarray(m.vertices[:, :3], dtype=np.float32)
+    r, e, b = _update_run(m.indices, vb, 0, m.index_count, 1e-2, 0, None, None, impl="oracle")
+    assert r.size == m.index_count and e == 0
+    assert np.allclose(b, vb, atol=1e-5)
+    flat = vb.copy()
+    flat[:, 1] = flat[:, 0]  # every vertex on the line x = y = z: zero-area faces
+    flat[:, 2] = flat[:, 0]
+    r, e, b = _update_run(m.indices, flat, 0, m.index_count, 1e-2, 0, None, None, impl="oracle")
+    assert np.allclose(b, flat, atol=1e-5)
