@@ -1,7 +1,7 @@
 """Benchmark of the MI355X simplifier: input triangles per second for one plain
 meshopt_simplify call on a device-resident mesh, next to the reference CPU library.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload icosphere8|grid256|seamed1024|patches4096|serving|sloppy]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload icosphere8|grid256|seamed1024|patches4096|serving|sloppy|update]
 
 A step is one whole simplify call (setup, every pass, compaction of the result) on inputs
 already resident in HBM; the timed region is bracketed by device synchronisation.  Prints one
@@ -419,6 +419,94 @@ def bench_sloppy(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_update(args):
+    """meshopt_simplifyWithUpdate on the seamed 1024x1024 grid with its normal/UV attributes (target
+    20%, error 1e-2), inputs resident in HBM: one step is one whole call, the vertex solve and
+    write-back included.  The call updates its arrays in place, so every step starts from a pristine
+    device copy restored (and synchronised) outside the timed call."""
+    import torch
+    import meshoptimizer_amd as M
+
+    m, target, weights, desc = make_workload("seamed1024")
+    dev = torch.device("cuda:0")
+    ib0 = torch.from_numpy(m.indices.astype(np.int32)).to(dev)
+    vb0 = torch.from_numpy(m.vertices).to(dev)
+    ib, vb = ib0.clone(), vb0.clone()
+    dest = torch.empty(m.index_count, dtype=torch.int32, device=dev)
+    stride = m.stride
+
+    def restore():
+        ib.copy_(ib0)
+        vb.copy_(vb0)
+        torch.cuda.synchronize()
+
+    def step():
+        return M.simplify_with_update_device(ib.data_ptr(), m.index_count, vb.data_ptr(), m.vertex_count, stride, target, 1e-2, 0,
+                                             attributes_ptr=vb.data_ptr() + 12, attribute_stride=stride, weights=weights)
+
+    def collapse_only():
+        return M.simplify_device(dest.data_ptr(), ib0.data_ptr(), m.index_count, vb0.data_ptr(), m.vertex_count, stride, target, 1e-2, 0,
+                                 attributes_ptr=vb0.data_ptr() + 12, attribute_stride=stride, weights=weights)
+
+    for _ in range(args.warmup):
+        restore()
+        step()
+    restore()
+    n_gpu, e_gpu = step()
+    gpu_ib = ib[:n_gpu].cpu().numpy().astype(np.uint32)
+    gpu_vb = vb.cpu().numpy()
+    call, _ = M.last_call_stats()
+
+    times, plain = [], []
+    for _ in range(args.steps):
+        restore()
+        ts = time.perf_counter()
+        step()
+        times.append(time.perf_counter() - ts)
+    for _ in range(args.steps):
+        ts = time.perf_counter()
+        collapse_only()
+        plain.append(time.perf_counter() - ts)
+    mean = sum(times) / len(times)
+    out = {
+        "metric": "input Mtri/s simplified and solved (bit-exact vs CPU), meshopt_simplifyWithUpdate",
+        "value": round(m.triangle_count / mean / 1e6, 3),
+        "unit": "Mtri/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(mean * 1e3, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded generator)",
+        "config": {"workload": "meshopt_simplifyWithUpdate, " + desc, "input_triangles": m.triangle_count,
+                   "output_triangles": n_gpu // 3, "solved_vertices": call["solve_vertices"],
+                   "solve_rounds_launched": call["solve_rounds"]},
+        "p50_ms": round(statistics.median(times) * 1e3, 4),
+        "p99_ms": round(float(np.percentile(np.array(times) * 1e3, 99)), 4),
+        "same_call_without_solve_ms": round(statistics.median(plain) * 1e3, 4),
+    }
+    if not args.no_cpu:
+        from oracle import oracle as O
+        impl = "ref" if O.have_ref() else "oracle"
+        na = len(weights)
+
+        def cpu_call():
+            b = m.vertices.copy()
+            r, e = O.simplify_with_update(m.indices, b, target, 1e-2, 0, attributes=b[:, 3:3 + na], weights=weights, impl=impl)
+            return r, e, b
+
+        ts = time.perf_counter()
+        ro, eo, bo = cpu_call()
+        cpu_t = [time.perf_counter() - ts]
+        out["bit_exact"] = bool(np.array_equal(ro, gpu_ib) and np.float32(eo).tobytes() == np.float32(e_gpu).tobytes()
+                                and bo.tobytes() == gpu_vb.tobytes())
+        t_start = time.perf_counter()
+        while time.perf_counter() - t_start < min(args.cpu_budget, 10.0) and len(cpu_t) < 5:
+            ts = time.perf_counter()
+            cpu_call()
+            cpu_t.append(time.perf_counter() - ts)
+        med = statistics.median(cpu_t)
+        out["cpu_baseline"] = {"value": round(m.triangle_count / med / 1e6, 3), "unit": "Mtri/s", "cores": 1,
+                               "kind": "reference" if impl == "ref" else "port",
+                               "sample": f"{len(cpu_t)} single-threaded calls on the same mesh, median {med * 1e3:.1f} ms"}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -447,6 +535,8 @@ def main():
         return bench_serving(args)
     if args.workload == "sloppy":
         return bench_sloppy(args)
+    if args.workload == "update":
+        return bench_update(args)
 
     import torch
     import meshoptimizer_amd as M

@@ -411,9 +411,10 @@ static void carve(Arena& a, Plan& p, size_t V_in, size_t V, size_t I, size_t A, 
 	p.comp = prune ? a.take<uint32_t>(Vp) : nullptr;
 	p.cerr = prune ? a.take<float>(Vp) : nullptr;
 	p.solve.update = solve ? a.take<uint8_t>(Vp + 8) : nullptr;
-	p.solve.flags = solve ? a.take<uint32_t>(Vp + 1) : nullptr;
-	p.solve.list = solve ? a.take<uint32_t>(Vp) : nullptr;
-	p.solve.state = solve ? a.take<uint32_t>(Vp) : nullptr;
+	p.solve.listed = solve ? a.take<uint32_t>(Vp) : nullptr;
+	p.solve.waiting = solve ? a.take<uint32_t>(Vp) : nullptr;
+	p.solve.queue[0] = solve ? a.take<uint32_t>(Vp) : nullptr;
+	p.solve.queue[1] = solve ? a.take<uint32_t>(Vp) : nullptr;
 
 	size_t biggest = std::max(std::max(I, cap), Vp + 1);
 	size_t temp_words = std::max(sort_temp_words(biggest), scan_temp_words(biggest + 16));
@@ -761,11 +762,12 @@ static size_t simplify_on_device(Context& ctx, const CallArgs& c, float* result_
 		launch_solve_setup(s, m, plan.solve, result_count);
 		sync_scalars(ctx, m);
 		const uint32_t listed = h.work[4];
-		uint32_t round = 1, batch = 4;
-		while (h.work[5] < listed)
+		uint32_t round = 0, batch = 4;
+		// Kahn's order: the queue empties only once every listed primary is solved, and every
+		// round solves at least the lowest unsolved one
+		while (h.work[5 + round % 3] != 0)
 		{
-			// every round solves at least the lowest pending primary
-			if (round > listed + 1)
+			if (round > listed)
 				throw std::runtime_error("position solve did not converge (internal error)");
 			launch_solve_rounds(s, m, plan.solve, listed, round, batch);
 			round += batch;
@@ -773,7 +775,7 @@ static size_t simplify_on_device(Context& ctx, const CallArgs& c, float* result_
 			sync_scalars(ctx, m);
 		}
 		t_call.solve_vertices = listed;
-		t_call.solve_rounds = round - 1;
+		t_call.solve_rounds = round;
 
 		SolveOutput o;
 		o.positions = c.out_positions;

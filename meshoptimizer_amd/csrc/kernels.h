@@ -127,9 +127,9 @@ void launch_sparse_backmap(hipStream_t s, uint32_t* ib, size_t index_count, cons
 struct SolveBuffers
 {
 	uint8_t* update;   // V: vertex of the result, or primary of one
-	uint32_t* flags;   // V + 1: primaries the position solve visits, then their scan
-	uint32_t* list;    // V: those primaries in index order
-	uint32_t* state;   // V: round that solved a listed primary (all-ones while pending, 0 otherwise)
+	uint32_t* listed;  // V: 1 for the primaries the position solve visits
+	uint32_t* waiting; // V: listed lower-index neighbour entries not solved yet
+	uint32_t* queue[2]; // V each: primaries ready in the current / next round
 };
 // the caller's arrays written back (finalizeVertices); strides in floats
 struct SolveOutput
@@ -143,11 +143,12 @@ struct SolveOutput
 	AttrMap map;
 };
 // marks the result's vertices, rebuilds the welded adjacency of the result, lists the
-// primaries to solve; returns nothing, the list size lands in sc->work[4]
+// primaries to solve (sc->work[4] = how many) and queues those with no listed lower-index
+// neighbour for the first round; sc->work[5 + r % 3] holds round r's queue length
 void launch_solve_setup(hipStream_t s, Mesh& m, const SolveBuffers& b, size_t result_count);
-// `rounds` rounds of the ordered position solve starting at round `first`; sc->work[5] counts
-// the primaries solved so far
-void launch_solve_rounds(hipStream_t s, Mesh& m, const SolveBuffers& b, uint32_t list_count, uint32_t first, uint32_t rounds);
+// `rounds` rounds of the ordered position solve starting at round `first` (0-based); the solve
+// is complete once the queue of the next round to run is empty; `listed` sizes the grid
+void launch_solve_rounds(hipStream_t s, Mesh& m, const SolveBuffers& b, uint32_t listed, uint32_t first, uint32_t rounds);
 // wedge copies, attribute solve and the write-back into the caller's arrays
 void launch_solve_finish(hipStream_t s, Mesh& m, const SolveBuffers& b, const SolveOutput& o);
 

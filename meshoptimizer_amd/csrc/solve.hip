@@ -3,13 +3,18 @@
 // solvePositions (src/simplifier.cpp:1675-1763) walks the primaries in index order, and each
 // vertex reads the current positions of its welded neighbours: a lower-index neighbour has
 // already been moved (or kept), a higher-index one has not.  The walk is replayed exactly in
-// rounds: a listed primary is solved in the first round in which every listed lower-index
-// neighbour was solved in an earlier round.  Its higher-index neighbours wait for it, so
-// nothing a round reads is written in the same round, and each vertex sees precisely the
-// positions the sequential walk would show it.  Rounds = the longest index-increasing path
-// through listed neighbours (tens for subdivided meshes, a few hundred for row-major grids).
+// rounds, Kahn-style: every listed primary counts its listed lower-index neighbour entries
+// (`waiting`); those with none form the first round's queue; a primary solved in round r takes
+// one off the count of each listed higher-index neighbour entry, and the neighbour whose count
+// reaches zero joins round r + 1's queue.  Two adjacent listed primaries are never in one round
+// (the higher waits for the lower), so nothing a round reads is written in that round and each
+// vertex sees precisely the positions the sequential walk would show it.  Rounds = the longest
+// index-increasing path through listed neighbours (tens for subdivided meshes, hundreds for
+// row-major grids); work = O(V + E) over all rounds.
 //
 // solveAttributes (1765-1809) and finalizeVertices (623-663) are independent per vertex.
+#include <algorithm>
+
 #include "kernels.h"
 #include "prims.h"
 #include "quadric.h"
@@ -116,28 +121,61 @@ __global__ void k_solve_mark(const uint32_t* ib, size_t n, const uint32_t* remap
 	update[remap[v]] = 1;
 }
 
-__global__ void k_solve_flags(const uint8_t* update, const uint32_t* remap, const uint8_t* kind, size_t n, uint32_t* flags, uint32_t* state)
+__global__ void k_solve_flags(const uint8_t* update, const uint32_t* remap, const uint8_t* kind, size_t n, uint32_t* listed)
 {
 	size_t ii = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
 	if (ii >= n)
 		return;
 	uint32_t i = (uint32_t)ii;
-	bool listed = update[i] && remap[i] == i && !solve_pinned(kind[i]);
-	flags[i] = listed;
-	state[i] = listed ? kPending : 0u;
+	listed[i] = update[i] && remap[i] == i && !solve_pinned(kind[i]);
 }
 
-__global__ void k_solve_list(const uint32_t* state, const uint32_t* offs, size_t n, uint32_t* list)
+// wave-aggregated append of `item` (for the lanes with `take`) to a queue
+__device__ __forceinline__ void queue_append(bool take, uint32_t item, uint32_t* count, uint32_t* queue)
 {
-	size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-	if (i < n && state[i] == kPending)
-		list[offs[i]] = (uint32_t)i;
+	const uint64_t mask = __ballot(take);
+	if (!mask)
+		return;
+	const uint32_t lane = __lane_id();
+	const uint32_t leader = (uint32_t)__builtin_ctzll(mask);
+	uint32_t base = 0;
+	if (lane == leader)
+		base = atomicAdd(count, (uint32_t)__popcll(mask));
+	base = __shfl(base, (int)leader, 64);
+	if (take)
+		queue[base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull))] = item;
+}
+
+// listed lower-index neighbour entries of every listed primary; the primaries with none are the
+// first round's queue; sc->work[4] = listed primaries
+__global__ void __launch_bounds__(kThreads) k_solve_waiting(const uint32_t* listed, const uint32_t* adj_off, const uint2* adj, size_t n, uint32_t* waiting,
+                                                             uint32_t* queue0, uint32_t* counts, CallScalars* sc)
+{
+	size_t ii = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+	uint32_t i = (uint32_t)ii;
+	bool mine = ii < n && listed[i];
+	uint32_t w = 0;
+	if (mine)
+	{
+		for (uint32_t e = adj_off[i], end = adj_off[i + 1]; e < end; ++e)
+		{
+			uint2 ed = adj[e];
+			w += (ed.x < i && listed[ed.x]) + (ed.y < i && listed[ed.y]);
+		}
+	}
+	// unlisted vertices get a count no release can bring to zero (at most 2 per corner entry)
+	if (ii < n)
+		waiting[i] = mine ? w : 0x80000000u;
+	queue_append(mine && w == 0, i, &counts[0], queue0);
+	const uint64_t all = __ballot(mine);
+	if (__lane_id() == 0 && all)
+		atomicAdd(&sc->work[4], (uint32_t)__popcll(all));
 }
 
 __global__ void k_solve_reset(CallScalars* sc)
 {
 	sc->work[4] = 0;
-	sc->work[5] = 0;
+	sc->work[5] = sc->work[6] = sc->work[7] = 0;
 }
 
 void launch_solve_setup(hipStream_t s, Mesh& m, const SolveBuffers& b, size_t result_count)
@@ -151,9 +189,8 @@ void launch_solve_setup(hipStream_t s, Mesh& m, const SolveBuffers& b, size_t re
 		k_solve_mark<<<grid_for(result_count), kThreads, 0, s>>>(m.ib[m.ib_cur], result_count, m.remap, b.update);
 	// updateEdgeAdjacency over the result (src/simplifier.cpp:2590-2591)
 	launch_adjacency(s, m, result_count / 3);
-	k_solve_flags<<<grid_for(n), kThreads, 0, s>>>(b.update, m.remap, m.kind, n, b.flags, b.state);
-	scan_exclusive(s, b.flags, b.flags, n, nullptr, &m.sc->work[4], m.temp);
-	k_solve_list<<<grid_for(n), kThreads, 0, s>>>(b.state, b.flags, n, b.list);
+	k_solve_flags<<<grid_for(n), kThreads, 0, s>>>(b.update, m.remap, m.kind, n, b.listed);
+	k_solve_waiting<<<grid_for(n), kThreads, 0, s>>>(b.listed, m.adj_off, m.adj, n, b.waiting, b.queue[0], &m.sc->work[5], m.sc);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -161,8 +198,9 @@ void launch_solve_setup(hipStream_t s, Mesh& m, const SolveBuffers& b, size_t re
 
 struct SolveRound
 {
-	const uint32_t* list;
-	uint32_t* state;
+	uint32_t* waiting;
+	uint32_t* queue[2];
+	uint32_t* counts;
 	float4* pos;
 	const Quad* vq;
 	const Grad* vg;
@@ -172,111 +210,135 @@ struct SolveRound
 	const uint32_t* wedge;
 	const uint32_t* adj_off;
 	const uint2* adj;
-	uint32_t* solved;
 };
 
-// the position the solve gives primary i, or false when one of the reference's checks keeps
-// the old one (singular system, outside the neighbourhood, a flipped triangle, error growth)
-__device__ bool solve_position(const SolveRound& a, uint32_t i, V3& out)
+// neighbourhood checks of a solved position over `n` corner entries (src/simplifier.cpp:1734-1748):
+// getNeighborhoodRadius (1252-1276) and hasTriangleFlips for a point move (1234-1250)
+__device__ __forceinline__ void sweep_entry(const V3& q0, const V3& q1, const V3& cur, const V3& p, float& far2, bool& flips)
 {
-	const V3 cur = load_pos(a.pos, i);
-	const Quad Qv = load_quad(a.vq, i);
-
-	// a point quadric regularises the solution
-	Quad Q = Qv;
-	Quad R;
-	quad_from_point(R, cur.x, cur.y, cur.z, Qv.w * 1e-4f);
-	quad_add(Q, R);
-
-	Grad GV = {0.f, 0.f, 0.f, 0.f};
-	if (a.attr_count)
-	{
-		// the optimum minimises every wedge's attribute quadric at once
-		uint32_t v = i;
-		do
-		{
-			quad_reduce_attributes(Q, load_quad(a.aq, v), a.ag + (size_t)v * a.attr_count, a.attr_count);
-			v = a.wedge[v];
-		} while (v != i);
-		// volume preservation as a constraint
-		if (a.vg)
-			GV = a.vg[i];
-	}
-
-	V3 p;
-	if (!quad_solve(p, Q, GV))
-		return false;
-
-	const uint32_t begin = a.adj_off[i], end = a.adj_off[i + 1];
-
-	// getNeighborhoodRadius, src/simplifier.cpp:1252-1276
-	float far2 = 0.f;
-	for (uint32_t e = begin; e < end; ++e)
-	{
-		uint2 ed = a.adj[e];
-		V3 q0 = load_pos(a.pos, ed.x), q1 = load_pos(a.pos, ed.y);
-		float d0 = (q0.x - cur.x) * (q0.x - cur.x) + (q0.y - cur.y) * (q0.y - cur.y) + (q0.z - cur.z) * (q0.z - cur.z);
-		float d1 = (q1.x - cur.x) * (q1.x - cur.x) + (q1.y - cur.y) * (q1.y - cur.y) + (q1.z - cur.z) * (q1.z - cur.z);
-		far2 = far2 < d0 ? d0 : far2;
-		far2 = far2 < d1 ? d1 : far2;
-	}
-	float nr = sqrtf(far2);
-	float dp = (p.x - cur.x) * (p.x - cur.x) + (p.y - cur.y) * (p.y - cur.y) + (p.z - cur.z) * (p.z - cur.z);
-	if (dp > nr * nr)
-		return false;
-
-	// hasTriangleFlips for a point move, src/simplifier.cpp:1234-1250
-	for (uint32_t e = begin; e < end; ++e)
-	{
-		uint2 ed = a.adj[e];
-		if (triangle_flips(load_pos(a.pos, ed.x), load_pos(a.pos, ed.y), cur, p))
-			return false;
-	}
-
-	if (quad_error(Qv, p) > quad_error(Qv, cur) * 1.5f + 1e-6f)
-		return false;
-
-	out = p;
-	return true;
+	float d0 = (q0.x - cur.x) * (q0.x - cur.x) + (q0.y - cur.y) * (q0.y - cur.y) + (q0.z - cur.z) * (q0.z - cur.z);
+	float d1 = (q1.x - cur.x) * (q1.x - cur.x) + (q1.y - cur.y) * (q1.y - cur.y) + (q1.z - cur.z) * (q1.z - cur.z);
+	far2 = far2 < d0 ? d0 : far2;
+	far2 = far2 < d1 ? d1 : far2;
+	flips |= triangle_flips(q0, q1, cur, p);
 }
 
-__global__ void __launch_bounds__(kThreads) k_solve_round(SolveRound a, uint32_t count, uint32_t round)
+// round r: solve the queued primaries, one wavefront per primary, and release their higher-index
+// neighbours into round r + 1's queue.  Queue lengths rotate over three words: round r reads
+// counts[r % 3], appends to counts[(r + 1) % 3] and clears counts[(r + 2) % 3] (round r - 1's,
+// read by nobody after it).
+//
+// Per primary, every lane computes the same quadric solve (its loads are broadcasts) while the
+// lanes share the corner entries.  The release does not depend on the solve's outcome (the
+// neighbour reads the new position only in a later launch), so each lane issues the atomics of
+// its first entry before the solve and uses their results after it: the latencies overlap.
+// Both neighbourhood checks only reject, so evaluating both changes no outcome; the radius is a
+// maximum of non-negative values that never takes a NaN (a maximum of bit patterns, order-free)
+// and the flip test an any().
+__global__ void __launch_bounds__(kThreads) k_solve_round(SolveRound a, uint32_t round)
 {
-	const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-	uint32_t i = 0;
-	bool ready = false;
-	if (k < count)
+	const uint32_t next = (round + 1) % 3;
+	if (blockIdx.x == 0 && threadIdx.x == 0)
+		a.counts[(round + 2) % 3] = 0;
+	const uint32_t count = a.counts[round % 3];
+	const uint32_t* queue = a.queue[round & 1];
+	uint32_t* out = a.queue[(round + 1) & 1];
+	const uint32_t lane = __lane_id();
+	const uint32_t waves = gridDim.x * (blockDim.x / 64);
+
+	for (uint32_t k = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64; k < count; k += waves)
 	{
-		i = a.list[k];
-		ready = a.state[i] == kPending;
-		// wait for every listed lower-index neighbour to be solved in an earlier round (pending
-		// ones read all-ones, ones solved in this round read `round`)
-		for (uint32_t e = a.adj_off[i], end = a.adj_off[i + 1]; ready && e < end; ++e)
+		const uint32_t i = queue[k];
+		const uint32_t begin = a.adj_off[i], end = a.adj_off[i + 1];
+		const V3 cur = load_pos(a.pos, i);
+		const Quad Qv = load_quad(a.vq, i);
+
+		// this lane's first corner entry: release its higher-index ends now (unlisted vertices
+		// hold a count that never reaches zero), load the positions the sweep needs
+		const uint32_t e0 = begin + lane;
+		const bool has0 = e0 < end;
+		const uint2 ed0 = has0 ? a.adj[e0] : make_uint2(i, i);
+		const uint32_t old0 = ed0.x > i ? atomicSub(&a.waiting[ed0.x], 1u) : 0u;
+		const uint32_t old1 = ed0.y > i ? atomicSub(&a.waiting[ed0.y], 1u) : 0u;
+		const V3 q0 = load_pos(a.pos, ed0.x), q1 = load_pos(a.pos, ed0.y);
+
+		// solvePositions' system (src/simplifier.cpp:1705-1729): a point quadric regularises the
+		// solution; with attributes the optimum minimises every wedge's attribute quadric at once,
+		// under the volume gradient as a constraint
+		Quad Q = Qv;
+		Quad R;
+		quad_from_point(R, cur.x, cur.y, cur.z, Qv.w * 1e-4f);
+		quad_add(Q, R);
+		Grad GV = {0.f, 0.f, 0.f, 0.f};
+		if (a.attr_count)
 		{
-			uint2 ed = a.adj[e];
-			if ((ed.x < i && a.state[ed.x] >= round) || (ed.y < i && a.state[ed.y] >= round))
-				ready = false;
+			uint32_t v = i;
+			do
+			{
+				quad_reduce_attributes(Q, load_quad(a.aq, v), a.ag + (size_t)v * a.attr_count, a.attr_count);
+				v = a.wedge[v];
+			} while (v != i);
+			if (a.vg)
+				GV = a.vg[i];
+		}
+
+		V3 p;
+		bool moved = quad_solve(p, Q, GV);
+		if (moved)
+		{
+			float far2 = 0.f;
+			bool flips = false;
+			if (has0)
+				sweep_entry(q0, q1, cur, p, far2, flips);
+			for (uint32_t e = e0 + 64; e < end; e += 64)
+			{
+				uint2 ed = a.adj[e];
+				sweep_entry(load_pos(a.pos, ed.x), load_pos(a.pos, ed.y), cur, p, far2, flips);
+			}
+			uint32_t far_bits = __float_as_uint(far2);
+			for (int o = 32; o > 0; o >>= 1)
+			{
+				uint32_t t = __shfl_xor(far_bits, o, 64);
+				far_bits = t > far_bits ? t : far_bits;
+			}
+			float nr = sqrtf(__uint_as_float(far_bits));
+			float dp = (p.x - cur.x) * (p.x - cur.x) + (p.y - cur.y) * (p.y - cur.y) + (p.z - cur.z) * (p.z - cur.z);
+			moved = !(dp > nr * nr) && !__ballot(flips) && !(quad_error(Qv, p) > quad_error(Qv, cur) * 1.5f + 1e-6f);
+		}
+		if (moved && lane == 0)
+			a.pos[i] = make_float4(p.x, p.y, p.z, 0.f);
+
+		queue_append(old0 == 1u, ed0.x, &a.counts[next], out);
+		queue_append(old1 == 1u, ed0.y, &a.counts[next], out);
+
+		// corner entries past the first 64 (high-valence vertices)
+		for (uint32_t base = begin + 64; base < end; base += 64)
+		{
+			const uint32_t e = base + lane;
+			uint32_t r0 = kNone, r1 = kNone;
+			if (e < end)
+			{
+				uint2 ed = a.adj[e];
+				if (ed.x > i && atomicSub(&a.waiting[ed.x], 1u) == 1u)
+					r0 = ed.x;
+				if (ed.y > i && atomicSub(&a.waiting[ed.y], 1u) == 1u)
+					r1 = ed.y;
+			}
+			queue_append(r0 != kNone, r0, &a.counts[next], out);
+			queue_append(r1 != kNone, r1, &a.counts[next], out);
 		}
 	}
-	if (ready)
-	{
-		V3 p;
-		if (solve_position(a, i, p))
-			a.pos[i] = make_float4(p.x, p.y, p.z, 0.f);
-		a.state[i] = round;
-	}
-	const uint64_t done = __ballot(ready);
-	if (__lane_id() == 0 && done)
-		atomicAdd(a.solved, (uint32_t)__popcll(done));
 }
 
-void launch_solve_rounds(hipStream_t s, Mesh& m, const SolveBuffers& b, uint32_t list_count, uint32_t first, uint32_t rounds)
+void launch_solve_rounds(hipStream_t s, Mesh& m, const SolveBuffers& b, uint32_t listed, uint32_t first, uint32_t rounds)
 {
-	if (!list_count)
+	if (!listed)
 		return;
 	SolveRound a;
-	a.list = b.list;
-	a.state = b.state;
+	a.waiting = b.waiting;
+	a.queue[0] = b.queue[0];
+	a.queue[1] = b.queue[1];
+	a.counts = &m.sc->work[5];
 	a.pos = m.pos;
 	a.vq = m.vq;
 	a.vg = m.vg;
@@ -286,9 +348,10 @@ void launch_solve_rounds(hipStream_t s, Mesh& m, const SolveBuffers& b, uint32_t
 	a.wedge = m.wedge;
 	a.adj_off = m.adj_off;
 	a.adj = m.adj;
-	a.solved = &m.sc->work[5];
+	// one wavefront per queued primary, striding over the queue
+	const unsigned int grid = (unsigned int)std::min<size_t>((listed + 3) / 4, 1024);
 	for (uint32_t r = 0; r < rounds; ++r)
-		k_solve_round<<<grid_for(list_count), kThreads, 0, s>>>(a, list_count, first + r);
+		k_solve_round<<<grid, kThreads, 0, s>>>(a, first + r);
 }
 
 // ------------------------------------------------------------------------------------------
