@@ -980,7 +980,7 @@ static size_t sloppy_on_device(Context& ctx, unsigned int* destination, const un
 	const size_t o_ek = take(4 * (3 * T + 1)), o_ev = take(4 * (3 * T + 1)), o_eka = take(4 * (3 * T + 1)), o_eva = take(4 * (3 * T + 1));
 	const size_t o_coff = take(4 * (V + 2)), o_stemp = take(4 * sort_words), o_ctemp = take(4 * scan_words), o_cq = take(sizeof(Quad) * (V + 1));
 	const size_t o_best = take(8 * (V + 1)), o_cremap = take(4 * (V + 1)), o_tri = take(16 * (T + 1)), o_ttable = take(4 * (size_t)ttable_size);
-	const size_t o_keep = take(4 * (T + 2));
+	const size_t o_keep = take(4 * (T + 2)), o_face = take(sizeof(Quad) * (T + 1));
 	ctx.ensure_arena(off);
 	char* base = ctx.arena;
 
@@ -1020,6 +1020,7 @@ static size_t sloppy_on_device(Context& ctx, unsigned int* destination, const un
 	b.ttable = reinterpret_cast<uint32_t*>(base + o_ttable);
 	b.ttable_size = ttable_size;
 	b.keep = reinterpret_cast<uint32_t*>(base + o_keep);
+	b.face = reinterpret_cast<Quad*>(base + o_face);
 
 	uint32_t* h = ctx.sloppy_pinned();
 	auto read_words = [&](int first, int count) {

@@ -33,6 +33,7 @@ struct SloppyBuffers
 	uint32_t* sort_temp;
 	uint32_t* scan_temp;
 	Quad* cq;                // V: cell quadrics
+	Quad* face;              // T: face quadric of each triangle (weight 3 inside one cell)
 	uint64_t* best;          // V: (error bits << 32 | vertex) minimum per cell
 	uint32_t* cell_remap;    // V
 	uint32_t* error_bits;    // 1 word: largest cell error (bits of a non-negative float)

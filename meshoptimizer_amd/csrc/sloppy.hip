@@ -172,25 +172,44 @@ __global__ void k_cell_runs(const uint32_t* keys, uint32_t n, uint32_t cell_coun
 	}
 }
 
-// step 3: each cell sums its entries' triangle quadrics from zero in entry order (the stable sort
-// kept the reference's (triangle, corner) order); weight 3 for a triangle inside one cell
-__global__ void k_cell_quadrics(const uint32_t* vals, const uint32_t* off, uint32_t cell_count, const uint32_t* idx, const uint32_t* cells,
-                                const float4* pos, Quad* cq)
+// step 3a: the face quadric of every triangle, once (a triangle with corners in two or three
+// cells adds the same weight-1 quadric to each; one inside a single cell adds weight 3)
+__global__ void k_face_quadrics(const uint32_t* idx, uint32_t tris, const uint32_t* cells, const float4* pos, Quad* face)
+{
+	const uint32_t t = blockIdx.x * kThreads + threadIdx.x;
+	if (t >= tris)
+		return;
+	const uint32_t i0 = idx[3 * t + 0], i1 = idx[3 * t + 1], i2 = idx[3 * t + 2];
+	const uint32_t c0 = cells[i0], c1 = cells[i1], c2 = cells[i2];
+	const bool single = c0 == c1 && c0 == c2;
+	Quad F;
+	quad_from_triangle(F, load_pos(pos, i0), load_pos(pos, i1), load_pos(pos, i2), single ? 3.f : 1.f);
+	store_quad(face, t, F);
+}
+
+// step 3b: each cell sums its entries' face quadrics from zero in entry order (the stable sort
+// kept the reference's (triangle, corner) order).  Four entries' quadrics are loaded before they
+// are added one after the other, so the loads overlap and the additions keep the sequential order.
+__global__ void k_cell_quadrics(const uint32_t* vals, const uint32_t* off, uint32_t cell_count, const Quad* face, Quad* cq)
 {
 	for (uint32_t c = blockIdx.x * kThreads + threadIdx.x; c < cell_count; c += gridDim.x * kThreads)
 	{
 		Quad Q;
 		quad_zero(Q);
-		for (uint32_t p = off[c], e = off[c + 1]; p < e; ++p)
+		uint32_t p = off[c];
+		const uint32_t e = off[c + 1];
+		for (; p + 4 <= e; p += 4)
 		{
-			const uint32_t t = vals[p] / 3;
-			const uint32_t i0 = idx[3 * t + 0], i1 = idx[3 * t + 1], i2 = idx[3 * t + 2];
-			const uint32_t c0 = cells[i0], c1 = cells[i1], c2 = cells[i2];
-			const bool single = c0 == c1 && c0 == c2;
-			Quad F;
-			quad_from_triangle(F, load_pos(pos, i0), load_pos(pos, i1), load_pos(pos, i2), single ? 3.f : 1.f);
-			quad_add(Q, F);
+			Quad F[4];
+#pragma unroll
+			for (int u = 0; u < 4; ++u)
+				F[u] = load_quad(face, vals[p + u] / 3);
+#pragma unroll
+			for (int u = 0; u < 4; ++u)
+				quad_add(Q, F[u]);
 		}
+		for (; p < e; ++p)
+			quad_add(Q, load_quad(face, vals[p] / 3));
 		store_quad(cq, c, Q);
 	}
 }
@@ -374,7 +393,8 @@ void launch_sloppy_quadrics(hipStream_t s, const SloppyBuffers& b, uint32_t cell
 		                 b.sort_temp);
 	}
 	k_cell_runs<<<grid_for((size_t)cell_count + 1), kThreads, 0, s>>>(b.entry_keys, 3 * tris, cell_count, b.cell_off);
-	k_cell_quadrics<<<grid_for(cell_count), kThreads, 0, s>>>(b.entry_vals, b.cell_off, cell_count, b.indices, b.cells, b.pos, b.cq);
+	k_face_quadrics<<<grid_for(tris), kThreads, 0, s>>>(b.indices, tris, b.cells, b.pos, b.face);
+	k_cell_quadrics<<<grid_for(cell_count), kThreads, 0, s>>>(b.entry_vals, b.cell_off, cell_count, b.face, b.cq);
 	fill_u32(s, reinterpret_cast<uint32_t*>(b.best), kNone, 2 * (size_t)cell_count);
 	HIP_CHECK(hipMemsetAsync(b.error_bits, 0, sizeof(uint32_t), s));
 	k_cell_best<<<grid_for(n), kThreads, 0, s>>>(b.cells, b.cq, b.pos, n, reinterpret_cast<unsigned long long*>(b.best));

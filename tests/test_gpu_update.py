@@ -111,3 +111,41 @@ def test_update_device_entry():
         bd = d_vb.to_host(np.float32, vb.size).reshape(vb.shape)
     assert np.array_equal(rd, rh) and same_bits(e, eh)
     assert bd.tobytes() == bh.tobytes()
+
+
+@pytest.mark.parametrize("name", ["empty", "no_collapse", "zero_weights", "sparse_lock", "absolute_protect"])
+def test_update_edge_cases(name):
+    """An empty index buffer (nothing moves), a target that allows no collapse (the solve still
+    moves every free vertex), attributes whose weights are all zero (nothing written back to them),
+    Sparse on a sub-range with locks, ErrorAbsolute with Protect bits and a zero target."""
+    m = W.seamed_grid(32, seams=3, seed=4)
+    vb = np.ascontiguousarray(m.vertices[:, [0, 1, 2, 3, 6]], dtype=np.float32)
+    n = vb.shape[0]
+    args = {
+        "empty": (np.zeros(0, np.uint32), 0, 1e-2, 0, [1, 1], None),
+        "no_collapse": (m.indices, m.index_count, 1e-2, 0, [1, 1], None),
+        "zero_weights": (m.indices, m.index_count // 4, 1e-2, 0, [0, 0], None),
+        "sparse_lock": (m.indices[:600], 150, 1e-1, 2 | 4 | 32, [0.5, 2], (np.arange(n) % 7 == 0).astype(np.uint8)),
+        "absolute_protect": (m.indices, 0, 1e-3, 4 | 32, [0.5, 2], (np.arange(n) % 5 == 0).astype(np.uint8) * 2),
+    }[name]
+    ib, target, err, opts, w, lock = args
+    announce(f"update edge case {name}")
+    r, e, b = assert_parity(ib, vb, 2, target, err, opts, np.array(w, np.float32), lock)
+    if name == "empty":
+        assert r.size == 0 and b.tobytes() == vb.tobytes()
+    else:
+        assert not np.array_equal(b[:, :3], vb[:, :3])
+
+
+def test_update_device_rejects_bad_index():
+    """The device entry reports an out-of-range index as an error and leaves the arrays alone."""
+    from devmem import DeviceArray
+    m = W.grid(16)
+    vb = np.ascontiguousarray(m.vertices[:, :3], dtype=np.float32)
+    bad = m.indices.copy()
+    bad[7] = vb.shape[0] + 3
+    announce("update device bad index")
+    with DeviceArray(bad) as d_ib, DeviceArray(vb) as d_vb:
+        with pytest.raises(RuntimeError, match="vertex_count"):
+            M.simplify_with_update_device(d_ib.ptr, bad.size, d_vb.ptr, vb.shape[0], 12, bad.size // 4, 1e-2)
+        assert d_vb.to_host(np.float32, vb.size).tobytes() == vb.tobytes()
