@@ -53,59 +53,103 @@ __device__ __forceinline__ size_t live_count(const uint32_t* n_dev, size_t n_max
 	return n < n_max ? n : n_max;
 }
 
+// Tiles of kScanTile elements, one per block; thread t owns the kScanItems consecutive elements
+// at tile offset t * kScanItems (16-byte vector accesses when the arrays allow them, scalar ones
+// for the tail), so a tile needs one block scan instead of one per item.
+template <bool Vec>
+__device__ __forceinline__ void load_run(const uint32_t* in, size_t e, size_t n, uint32_t v[kScanItems])
+{
+	if (Vec && e + kScanItems <= n)
+	{
+		const uint4* p = reinterpret_cast<const uint4*>(in + e);
+#pragma unroll
+		for (int q = 0; q < kScanItems / 4; ++q)
+		{
+			uint4 x = p[q];
+			v[4 * q + 0] = x.x;
+			v[4 * q + 1] = x.y;
+			v[4 * q + 2] = x.z;
+			v[4 * q + 3] = x.w;
+		}
+	}
+	else
+	{
+#pragma unroll
+		for (int k = 0; k < kScanItems; ++k)
+			v[k] = e + k < n ? in[e + k] : 0u;
+	}
+}
+
+template <bool Vec>
 __global__ void __launch_bounds__(kScanThreads) k_scan_reduce(const uint32_t* in, size_t n_max, const uint32_t* n_dev, uint32_t* block_sums)
 {
 	__shared__ uint32_t lds[4];
-	size_t n = live_count(n_dev, n_max);
-	size_t base = (size_t)blockIdx.x * kScanTile;
+	const size_t n = live_count(n_dev, n_max);
+	const size_t e = (size_t)blockIdx.x * kScanTile + (size_t)threadIdx.x * kScanItems;
+	uint32_t v[kScanItems];
+	load_run<Vec>(in, e, n, v);
 	uint32_t sum = 0;
-#pragma unroll 4
-	for (int j = 0; j < kScanItems; ++j)
-	{
-		size_t e = base + (size_t)j * kScanThreads + threadIdx.x;
-		sum += e < n ? in[e] : 0u;
-	}
+#pragma unroll
+	for (int k = 0; k < kScanItems; ++k)
+		sum += v[k];
 	sum = block_reduce_sum(sum, lds);
 	if (threadIdx.x == 0)
 		block_sums[blockIdx.x] = sum;
 }
 
-// single block: exclusive scan of the block sums in place, total to *total
-__global__ void __launch_bounds__(kScanThreads) k_scan_blocks(uint32_t* sums, uint32_t count, uint32_t* total)
+// each block adds the sums of the tiles before it (read directly: a few per thread), scans its
+// tile and writes it; the last block also writes the total
+template <bool Vec>
+__global__ void __launch_bounds__(kScanThreads) k_scan_apply(const uint32_t* in, uint32_t* out, size_t n_max, const uint32_t* n_dev, const uint32_t* block_sums,
+                                                           uint32_t* total)
 {
 	__shared__ uint32_t lds[4];
-	uint32_t carry = 0;
-	for (uint32_t base = 0; base < count; base += kScanThreads)
-	{
-		uint32_t e = base + threadIdx.x;
-		uint32_t v = e < count ? sums[e] : 0u;
-		uint32_t tot;
-		uint32_t ex = block_exclusive_scan(v, lds, tot);
-		if (e < count)
-			sums[e] = carry + ex;
-		carry += tot;
-	}
-	if (threadIdx.x == 0 && total)
-		*total = carry;
-}
+	const size_t n = live_count(n_dev, n_max);
+	uint32_t before = 0;
+	for (uint32_t b = threadIdx.x; b < blockIdx.x; b += kScanThreads)
+		before += block_sums[b];
+	before = block_reduce_sum(before, lds);
+	if (blockIdx.x + 1 == gridDim.x && threadIdx.x == 0 && total)
+		*total = before + block_sums[blockIdx.x];
 
-__global__ void __launch_bounds__(kScanThreads) k_scan_apply(const uint32_t* in, uint32_t* out, size_t n_max, const uint32_t* n_dev, const uint32_t* block_offsets)
-{
-	__shared__ uint32_t lds[4];
-	size_t n = live_count(n_dev, n_max);
-	size_t base = (size_t)blockIdx.x * kScanTile;
-	if (base >= n)
-		return;
-	uint32_t carry = block_offsets[blockIdx.x];
-	for (int j = 0; j < kScanItems; ++j)
+	const size_t e = (size_t)blockIdx.x * kScanTile + (size_t)threadIdx.x * kScanItems;
+	if ((size_t)blockIdx.x * kScanTile >= n)
+		return; // uniform per block: no barrier follows for the blocks that leave
+	uint32_t v[kScanItems];
+	load_run<Vec>(in, e, n, v);
+	uint32_t run = 0;
+#pragma unroll
+	for (int k = 0; k < kScanItems; ++k)
+		run += v[k];
+	uint32_t tile_total;
+	uint32_t carry = before + block_exclusive_scan(run, lds, tile_total);
+	if (Vec && e + kScanItems <= n)
 	{
-		size_t e = base + (size_t)j * kScanThreads + threadIdx.x;
-		uint32_t v = e < n ? in[e] : 0u;
-		uint32_t tot;
-		uint32_t ex = block_exclusive_scan(v, lds, tot);
-		if (e < n)
-			out[e] = carry + ex;
-		carry += tot;
+		uint4* p = reinterpret_cast<uint4*>(out + e);
+#pragma unroll
+		for (int q = 0; q < kScanItems / 4; ++q)
+		{
+			uint4 x;
+			x.x = carry;
+			carry += v[4 * q + 0];
+			x.y = carry;
+			carry += v[4 * q + 1];
+			x.z = carry;
+			carry += v[4 * q + 2];
+			x.w = carry;
+			carry += v[4 * q + 3];
+			p[q] = x;
+		}
+	}
+	else
+	{
+#pragma unroll
+		for (int k = 0; k < kScanItems; ++k)
+		{
+			if (e + k < n)
+				out[e + k] = carry;
+			carry += v[k];
+		}
 	}
 }
 
@@ -117,9 +161,17 @@ size_t scan_temp_words(size_t n)
 void scan_exclusive(hipStream_t s, const uint32_t* in, uint32_t* out, size_t n, const uint32_t* n_dev, uint32_t* total, uint32_t* temp)
 {
 	unsigned int blocks = div_up(n ? n : 1, kScanTile);
-	k_scan_reduce<<<blocks, kScanThreads, 0, s>>>(in, n, n_dev, temp);
-	k_scan_blocks<<<1, kScanThreads, 0, s>>>(temp, blocks, total);
-	k_scan_apply<<<blocks, kScanThreads, 0, s>>>(in, out, n, n_dev, temp);
+	const bool vec = ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out)) & 15) == 0;
+	if (vec)
+	{
+		k_scan_reduce<true><<<blocks, kScanThreads, 0, s>>>(in, n, n_dev, temp);
+		k_scan_apply<true><<<blocks, kScanThreads, 0, s>>>(in, out, n, n_dev, temp, total);
+	}
+	else
+	{
+		k_scan_reduce<false><<<blocks, kScanThreads, 0, s>>>(in, n, n_dev, temp);
+		k_scan_apply<false><<<blocks, kScanThreads, 0, s>>>(in, out, n, n_dev, temp, total);
+	}
 }
 
 // ------------------------------------------------------------------------------------------
