@@ -367,7 +367,6 @@ static void carve(Arena& a, Plan& p, size_t V_in, size_t V, size_t I, size_t A, 
 	m.src_off = a.take<uint32_t>(Vp + 1);
 	m.adj_off = a.take<uint32_t>(Vp + 1);
 	m.adj = a.take<uint2>(I + 4);
-	m.adj_cursor = a.take<uint32_t>(Vp);
 
 	m.tri_cnt = a.take<uint32_t>(T);
 	m.tri_off = a.take<uint32_t>(T);

@@ -50,7 +50,6 @@ struct Mesh
 
 	uint32_t* adj_off;       // V + 1: welded corner CSR offsets of the current pass
 	uint2* adj;              // I: (next, prev) per corner
-	uint32_t* adj_cursor;    // V
 
 	uint32_t* ib[2];         // index buffer ping-pong
 	int ib_cur;

@@ -34,36 +34,39 @@ __constant__ uint8_t c_has_opposite[5][5] = {
 
 // ------------------------------------------------------------------------------------------
 // welded corner CSR for this pass (src/simplifier.cpp:52-109).  Segment order is free: every
-// consumer (open-edge tests, flip tests, neighbourhood scans) is a count or an any().
+// consumer (open-edge tests, flip tests, neighbourhood scans) is a count or an any().  One atomic
+// pass counts the corners per class and hands each corner its slot in the class's segment; after
+// the scan the fill places the corners without atomics.
 
-__global__ void k_adj_count(const uint32_t* ib, size_t n, const uint32_t* remap, uint32_t* counts)
+__global__ void k_adj_count(const uint32_t* ib, size_t n, const uint32_t* remap, uint32_t* counts, uint32_t* slot)
 {
 	size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
 	if (j < n)
-		atomicAdd(&counts[remap[ib[j]]], 1u);
+		slot[j] = atomicAdd(&counts[remap[ib[j]]], 1u);
 }
 
-__global__ void k_adj_fill(const uint32_t* ib, size_t tris, const uint32_t* remap, uint32_t* cursor, uint2* adj)
+__global__ void k_adj_fill(const uint32_t* ib, size_t tris, const uint32_t* remap, const uint32_t* off, const uint32_t* slot, uint2* adj)
 {
 	size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
 	if (t >= tris)
 		return;
 	uint32_t a = remap[ib[t * 3 + 0]], b = remap[ib[t * 3 + 1]], c = remap[ib[t * 3 + 2]];
-	adj[atomicAdd(&cursor[a], 1u)] = make_uint2(b, c);
-	adj[atomicAdd(&cursor[b], 1u)] = make_uint2(c, a);
-	adj[atomicAdd(&cursor[c], 1u)] = make_uint2(a, b);
+	adj[off[a] + slot[t * 3 + 0]] = make_uint2(b, c);
+	adj[off[b] + slot[t * 3 + 1]] = make_uint2(c, a);
+	adj[off[c] + slot[t * 3 + 2]] = make_uint2(a, b);
 }
 
+// the corner slots go to m.keys, free whenever the adjacency is rebuilt (before pick, after the
+// pass loop)
 void launch_adjacency(hipStream_t s, Mesh& m, size_t tri_count)
 {
 	size_t n = m.vertex_count;
 	fill_u32(s, m.adj_off, 0, n + 1);
 	if (tri_count)
-		k_adj_count<<<grid_for(tri_count * 3), kThreads, 0, s>>>(m.ib[m.ib_cur], tri_count * 3, m.remap, m.adj_off);
+		k_adj_count<<<grid_for(tri_count * 3), kThreads, 0, s>>>(m.ib[m.ib_cur], tri_count * 3, m.remap, m.adj_off, m.keys);
 	scan_exclusive(s, m.adj_off, m.adj_off, n, nullptr, m.adj_off + n, m.temp);
-	HIP_CHECK(hipMemcpyAsync(m.adj_cursor, m.adj_off, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
 	if (tri_count)
-		k_adj_fill<<<grid_for(tri_count), kThreads, 0, s>>>(m.ib[m.ib_cur], tri_count, m.remap, m.adj_cursor, m.adj);
+		k_adj_fill<<<grid_for(tri_count), kThreads, 0, s>>>(m.ib[m.ib_cur], tri_count, m.remap, m.adj_off, m.keys, m.adj);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -640,13 +643,6 @@ __global__ void __launch_bounds__(kTailThreads) k_round_tail(RoundState a, uint3
 	}
 }
 
-__global__ void k_perform_init_status(uint8_t* status, size_t count)
-{
-	size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-	if (k < count)
-		status[k] = kUndecided;
-}
-
 __global__ void k_perform_init_scalars(CallScalars* sc)
 {
 	sc->list_count[0] = 0;
@@ -657,22 +653,27 @@ __global__ void k_perform_init_scalars(CallScalars* sc)
 }
 
 // candidate ranks grouped by source class (order inside a group is irrelevant)
-__global__ void k_src_count(const uint32_t* r_r0, size_t count, uint32_t* counts)
+// one atomic pass: count the ranks per source class and hand each rank its slot in the class's
+// group (and reset its status); after the scan the fill places the ranks without atomics
+__global__ void k_src_count(const uint32_t* r_r0, size_t count, uint32_t* counts, uint32_t* slot, uint8_t* status)
 {
 	size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
 	if (k < count)
-		atomicAdd(&counts[r_r0[k]], 1u);
+	{
+		slot[k] = atomicAdd(&counts[r_r0[k]], 1u);
+		status[k] = kUndecided;
+	}
 }
 
-__global__ void k_src_fill(RoundState a, const uint32_t* r_r0, size_t count, uint32_t* cursor, uint32_t* src_rank, uint32_t* src_val)
+__global__ void k_src_fill(RoundState a, const uint32_t* r_r0, size_t count, const uint32_t* slot, uint32_t* src_rank, uint32_t* src_val)
 {
 	size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
 	if (k >= count)
 		return;
 	uint32_t r0 = r_r0[k];
-	uint32_t slot = atomicAdd(&cursor[r0], 1u);
-	src_rank[slot] = (uint32_t)k;
-	src_val[slot] = commit_value_at(a, (uint32_t)k, r0);
+	uint32_t at = a.src_off[r0] + slot[k];
+	src_rank[at] = (uint32_t)k;
+	src_val[at] = commit_value_at(a, (uint32_t)k, r0);
 }
 
 static RoundState round_state(Mesh& m)
@@ -711,17 +712,12 @@ void launch_perform_init(hipStream_t s, Mesh& m, size_t cand_count, size_t windo
 	iota_u32(s, m.cremap, n);
 	fill_u32(s, m.lock_rank, kNone, n);
 	fill_u32(s, m.src_off, 0, n + 1);
+	// the rank slots go to m.keys, free once the ranked copies are gathered
 	if (cand_count)
-	{
-		k_perform_init_status<<<grid_for(cand_count), kThreads, 0, s>>>(m.status, cand_count);
-		k_src_count<<<grid_for(cand_count), kThreads, 0, s>>>(m.r_r0, cand_count, m.src_off);
-	}
+		k_src_count<<<grid_for(cand_count), kThreads, 0, s>>>(m.r_r0, cand_count, m.src_off, m.keys, m.status);
 	scan_exclusive(s, m.src_off, m.src_off, n, nullptr, m.src_off + n, m.temp);
 	if (cand_count)
-	{
-		HIP_CHECK(hipMemcpyAsync(m.adj_cursor, m.src_off, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
-		k_src_fill<<<grid_for(cand_count), kThreads, 0, s>>>(round_state(m), m.r_r0, cand_count, m.adj_cursor, m.src_rank, m.src_val);
-	}
+		k_src_fill<<<grid_for(cand_count), kThreads, 0, s>>>(round_state(m), m.r_r0, cand_count, m.keys, m.src_rank, m.src_val);
 	k_perform_init_scalars<<<1, 1, 0, s>>>(m.sc);
 }
 
