@@ -917,6 +917,9 @@ __global__ void __launch_bounds__(kThreads) k_stop_find(const uint8_t* status, c
 	size_t base = (size_t)blockIdx.x * kScanTile;
 	StopAgg carry{tiles[blockIdx.x * 3 + 0], tiles[blockIdx.x * 3 + 1], tiles[blockIdx.x * 3 + 2]};
 	const uint32_t half = goal / 2, sixth = goal / 6;
+	// ranks grow with the lane and the item, so a wavefront's first stopping rank is its minimum:
+	// one atomic per wavefront (stopping ranks form long runs; same-address atomics serialise)
+	bool found = false;
 	for (int j = 0; j < kScanItems; ++j)
 	{
 		size_t k = base + (size_t)j * kThreads + threadIdx.x;
@@ -929,6 +932,7 @@ __global__ void __launch_bounds__(kThreads) k_stop_find(const uint8_t* status, c
 		}
 		StopAgg total;
 		StopAgg ex = block_scan3(v, total);
+		bool stop = false;
 		if (k < count)
 		{
 			uint32_t tri = carry.tri + ex.tri;
@@ -937,8 +941,13 @@ __global__ void __launch_bounds__(kThreads) k_stop_find(const uint8_t* status, c
 			float result_error = __uint_as_float(eb);
 			size_t eg = (size_t)half + rej;
 			float error_goal = eg < cand_count ? 1.5f * r_err[eg] : FLT_MAX;
-			bool stop = err > error_limit || tri >= goal || (err > error_goal && err > result_error && tri > sixth);
-			if (stop)
+			stop = err > error_limit || tri >= goal || (err > error_goal && err > result_error && tri > sixth);
+		}
+		const uint64_t stops = __ballot(stop);
+		if (stops && !found)
+		{
+			found = true;
+			if (__lane_id() == (uint32_t)__builtin_ctzll(stops))
 				atomicMin(&sc->stop, (uint32_t)k);
 		}
 		carry.tri += total.tri;
