@@ -1098,12 +1098,11 @@ __device__ __forceinline__ uint32_t next_wedge_above(const uint32_t* wedge, uint
 	return best;
 }
 
-__global__ void k_update_quadrics(const uint8_t* status, const uint32_t* r_r0, const uint32_t* r_r1, const CallScalars* sc_in, const uint32_t* cremap,
-                                  const uint32_t* wedge, const float4* pos, int attr_count, Quad* vq, Grad* vg, Quad* aq, Grad* ag, CallScalars* sc)
+// one committed collapse of k_update_quadrics; returns the bits of its non-NaN positional error
+// (non-negative floats order like their bits), 0 without attributes
+__device__ __forceinline__ uint32_t update_one(size_t k, const uint32_t* r_r0, const uint32_t* r_r1, const uint32_t* cremap, const uint32_t* wedge,
+                                               const float4* pos, int attr_count, Quad* vq, Grad* vg, Quad* aq, Grad* ag)
 {
-	size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-	if (k >= sc_in->stop || status[k] != kCommitted)
-		return;
 	uint32_t r0 = r_r0[k], r1 = r_r1[k];
 
 	Quad Q0 = load_quad(vq, r0);
@@ -1140,7 +1139,31 @@ __global__ void k_update_quadrics(const uint8_t* status, const uint32_t* r_r0, c
 
 		float derr = quad_error(Q0, load_pos(pos, r1));
 		if (derr == derr)
-			atomicMax(&sc->vertex_error_bits, __float_as_uint(derr));
+			return __float_as_uint(derr);
+	}
+	return 0;
+}
+
+__global__ void k_update_quadrics(const uint8_t* status, const uint32_t* r_r0, const uint32_t* r_r1, const CallScalars* sc_in, const uint32_t* cremap,
+                                  const uint32_t* wedge, const float4* pos, int attr_count, Quad* vq, Grad* vg, Quad* aq, Grad* ag, CallScalars* sc)
+{
+	size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+	const bool commit = k < sc_in->stop && status[k] == kCommitted;
+	// the largest positional error of the pass's commits (attribute path): one atomic per
+	// wavefront, so every lane stays to the reduction
+	uint32_t err_bits = 0;
+	if (commit)
+		err_bits = update_one(k, r_r0, r_r1, cremap, wedge, pos, attr_count, vq, vg, aq, ag);
+	if (attr_count)
+	{
+#pragma unroll
+		for (int o = 32; o > 0; o >>= 1)
+		{
+			uint32_t t = __shfl_xor(err_bits, o, 64);
+			err_bits = t > err_bits ? t : err_bits;
+		}
+		if (__lane_id() == 0 && err_bits)
+			atomicMax(&sc->vertex_error_bits, err_bits);
 	}
 }
 

@@ -197,13 +197,25 @@ __global__ void k_prune_keep(const uint32_t* ib, size_t tris, const uint32_t* co
                              uint32_t* min_bits)
 {
 	size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-	if (t >= tris)
-		return;
-	float e = cerr[comp[ib[t * 3]]];
-	bool k = e > cutoff;
-	keep[t] = k;
-	if (k)
-		atomicMin(min_bits, __float_as_uint(e));
+	// the smallest kept error (non-negative floats order like their bits): one atomic per
+	// wavefront, so every lane stays to the reduction
+	uint32_t bits = 0xffffffffu;
+	if (t < tris)
+	{
+		float e = cerr[comp[ib[t * 3]]];
+		bool k = e > cutoff;
+		keep[t] = k;
+		if (k)
+			bits = __float_as_uint(e);
+	}
+#pragma unroll
+	for (int o = 32; o > 0; o >>= 1)
+	{
+		uint32_t x = __shfl_xor(bits, o, 64);
+		bits = x < bits ? x : bits;
+	}
+	if (__lane_id() == 0 && bits != 0xffffffffu)
+		atomicMin(min_bits, bits);
 }
 
 __global__ void k_prune_write(const uint32_t* ib, size_t tris, const uint32_t* keep, const uint32_t* off, uint32_t* out)

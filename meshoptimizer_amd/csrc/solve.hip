@@ -147,10 +147,12 @@ __device__ __forceinline__ void queue_append(bool take, uint32_t item, uint32_t*
 }
 
 // listed lower-index neighbour entries of every listed primary; the primaries with none are the
-// first round's queue; sc->work[4] = listed primaries
+// first round's queue; sc->work[4] = listed primaries.  Both counters take one atomic per
+// workgroup (same-address atomics serialise in L2).
 __global__ void __launch_bounds__(kThreads) k_solve_waiting(const uint32_t* listed, const uint32_t* adj_off, const uint2* adj, size_t n, uint32_t* waiting,
                                                              uint32_t* queue0, uint32_t* counts, CallScalars* sc)
 {
+	__shared__ uint32_t s_take[kThreads / 64], s_listed[kThreads / 64], s_base;
 	size_t ii = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
 	uint32_t i = (uint32_t)ii;
 	bool mine = ii < n && listed[i];
@@ -166,10 +168,36 @@ __global__ void __launch_bounds__(kThreads) k_solve_waiting(const uint32_t* list
 	// unlisted vertices get a count no release can bring to zero (at most 2 per corner entry)
 	if (ii < n)
 		waiting[i] = mine ? w : 0x80000000u;
-	queue_append(mine && w == 0, i, &counts[0], queue0);
-	const uint64_t all = __ballot(mine);
-	if (__lane_id() == 0 && all)
-		atomicAdd(&sc->work[4], (uint32_t)__popcll(all));
+
+	const bool take = mine && w == 0;
+	const uint64_t tm = __ballot(take), lm = __ballot(mine);
+	const uint32_t lane = __lane_id(), wave = threadIdx.x / 64;
+	if (lane == 0)
+	{
+		s_take[wave] = (uint32_t)__popcll(tm);
+		s_listed[wave] = (uint32_t)__popcll(lm);
+	}
+	__syncthreads();
+	if (threadIdx.x == 0)
+	{
+		uint32_t tt = 0, lt = 0;
+		for (int q = 0; q < kThreads / 64; ++q)
+		{
+			tt += s_take[q];
+			lt += s_listed[q];
+		}
+		s_base = tt ? atomicAdd(&counts[0], tt) : 0u;
+		if (lt)
+			atomicAdd(&sc->work[4], lt);
+	}
+	__syncthreads();
+	if (take)
+	{
+		uint32_t at = s_base + (uint32_t)__popcll(tm & ((1ull << lane) - 1ull));
+		for (uint32_t q = 0; q < wave; ++q)
+			at += s_take[q];
+		queue0[at] = i;
+	}
 }
 
 __global__ void k_solve_reset(CallScalars* sc)
