@@ -139,6 +139,22 @@ inline size_t meshopt_simplifyWithUpdate(T* indices, size_t index_count, float* 
 		indices[i] = T(io[i]);
 	return n;
 }
+
+template <typename T>
+inline size_t meshopt_simplifySloppy(T* destination, const T* indices, size_t index_count, const float* vertex_positions, size_t vertex_count, size_t vertex_positions_stride, const unsigned char* vertex_lock, size_t target_index_count, float target_error, float* result_error = NULL)
+{
+	std::vector<unsigned int> in(indices, indices + index_count), out(index_count);
+	size_t n = meshopt_simplifySloppy(out.data(), in.data(), index_count, vertex_positions, vertex_count, vertex_positions_stride, vertex_lock, target_index_count, target_error, result_error);
+	for (size_t i = 0; i < n; ++i)
+		destination[i] = T(out[i]);
+	return n;
+}
+
+template <typename T>
+inline size_t meshopt_simplifySloppy(T* destination, const T* indices, size_t index_count, const float* vertex_positions, size_t vertex_count, size_t vertex_positions_stride, size_t target_index_count, float target_error, float* result_error = NULL)
+{
+	return meshopt_simplifySloppy(destination, indices, index_count, vertex_positions, vertex_count, vertex_positions_stride, (const unsigned char*)NULL, target_index_count, target_error, result_error);
+}
 #endif
 
 #endif /* MESHOPT_HIP_MESHOPTIMIZER_H */

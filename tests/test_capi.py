@@ -48,3 +48,20 @@ def test_product_does_not_link_the_oracle():
     assert "moracle" not in out
     ldd = subprocess.run(["ldd", M.LIB_PATH], capture_output=True, text=True).stdout
     assert "oracle" not in ldd and "meshopt_ref" not in ldd
+
+
+def test_index_adapters_compile(tmp_path):
+    """The 16-bit index adapters (src/meshoptimizer.h:1514-1559) instantiate for unsigned short."""
+    src = tmp_path / "adapters.cpp"
+    src.write_text(
+        '#include "meshoptimizer.h"\n'
+        "size_t f(unsigned short* d, const unsigned short* i, float* v, unsigned char* l) {\n"
+        "  size_t n = meshopt_simplify(d, i, 3, v, 3, 12, 3, 1e-2f);\n"
+        "  n += meshopt_simplifyWithAttributes(d, i, 3, v, 3, 12, v, 12, v, 0, l, 3, 1e-2f);\n"
+        "  n += meshopt_simplifyWithUpdate(d, 3, v, 3, 12, v, 12, v, 0, l, 3, 1e-2f);\n"
+        "  n += meshopt_simplifySloppy(d, i, 3, v, 3, 12, l, 3, 1e-2f);\n"
+        "  n += meshopt_simplifySloppy(d, i, 3, v, 3, 12, 3, 1e-2f);\n"
+        "  return n;\n}\n")
+    res = subprocess.run(["g++", "-std=c++11", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), "-c", str(src),
+                          "-o", str(tmp_path / "adapters.o")], capture_output=True, text=True)
+    assert res.returncode == 0, res.stderr
