@@ -46,6 +46,7 @@ BatchClasses batch_classes();
 // per mesh: validate sizes (status 4), pick its class (ids[c * patch_count + slot], counts[c])
 // or mark it for the plain path (status 1); counts must be zero on entry
 void launch_batch_classify(hipStream_t s, const BatchArgs& a, const BatchClasses& classes, uint32_t* ids, uint32_t* counts);
+void launch_batch_order(hipStream_t s, const BatchArgs& a, uint32_t* ids, const uint32_t* counts, uint32_t* temp);
 
 // one workgroup per mesh of a class (ids, *count_dev of them; grid = upper bound), every one with
 // the (vmax, imax, cmax) layout

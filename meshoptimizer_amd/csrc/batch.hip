@@ -115,6 +115,58 @@ extern "C" void meshopt_hip_batch_profile(unsigned long long* out, int reset)
 }
 #endif
 
+// each class's mesh list, larger meshes first (a counting sort by triangle count, one block per
+// class): long workgroups start early and the launch ends with short ones
+__global__ void __launch_bounds__(1024) k_batch_order(const uint32_t* index_offsets, uint32_t* ids, const uint32_t* counts, size_t P, uint32_t* temp)
+{
+	__shared__ uint32_t hist[1024];
+	__shared__ uint32_t wsum[16];
+	const uint32_t tid = threadIdx.x;
+	const uint32_t n = counts[blockIdx.x];
+	uint32_t* list = ids + (size_t)blockIdx.x * P;
+	uint32_t* tmp = temp + (size_t)blockIdx.x * P;
+	hist[tid] = 0;
+	__syncthreads();
+	for (uint32_t i = tid; i < n; i += 1024)
+	{
+		const uint32_t p = list[i];
+		const uint32_t t = (index_offsets[p + 1] - index_offsets[p]) / 3;
+		atomicAdd(&hist[1023 - (t < 1023 ? t : 1023)], 1u);
+	}
+	__syncthreads();
+	uint32_t v = hist[tid], incl = v;
+	for (int o = 1; o < 64; o <<= 1)
+	{
+		uint32_t x = __shfl_up(incl, o, 64);
+		if (__lane_id() >= (uint32_t)o)
+			incl += x;
+	}
+	if (__lane_id() == 63)
+		wsum[tid / 64] = incl;
+	__syncthreads();
+	uint32_t before = 0;
+	for (uint32_t w = 0; w < tid / 64; ++w)
+		before += wsum[w];
+	__syncthreads();
+	hist[tid] = before + incl - v;
+	__syncthreads();
+	for (uint32_t i = tid; i < n; i += 1024)
+	{
+		const uint32_t p = list[i];
+		const uint32_t t = (index_offsets[p + 1] - index_offsets[p]) / 3;
+		tmp[atomicAdd(&hist[1023 - (t < 1023 ? t : 1023)], 1u)] = p;
+	}
+	__syncthreads();
+	for (uint32_t i = tid; i < n; i += 1024)
+		list[i] = tmp[i];
+}
+
+void launch_batch_order(hipStream_t s, const BatchArgs& a, uint32_t* ids, const uint32_t* counts, uint32_t* temp)
+{
+	if (a.patch_count)
+		k_batch_order<<<kBatchClasses, 1024, 0, s>>>(a.index_offsets, ids, counts, a.patch_count, temp);
+}
+
 size_t batch_lds_bytes(uint32_t vmax, uint32_t imax, uint32_t cmax)
 {
 	return patch_layout(vmax, imax, cmax).total;

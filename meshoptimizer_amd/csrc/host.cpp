@@ -135,7 +135,7 @@ struct Context
 
 	void ensure_batch_buffers(size_t meshes)
 	{
-		size_t words = meshes + 4 + (size_t)kBatchClasses * meshes;
+		size_t words = meshes + 4 + 2 * (size_t)kBatchClasses * meshes;
 		if (words > batch_dev_words)
 		{
 			if (batch_dev)
@@ -1385,6 +1385,7 @@ static int simplify_batch_on_device(Context& ctx, unsigned int* destination, uns
 			HIP_CHECK(hipEventRecord(ctx.batch_ev[0], s));
 		HIP_CHECK(hipMemsetAsync(counts, 0, 4 * sizeof(uint32_t), s));
 		launch_batch_classify(s, a, classes, ids, counts);
+		launch_batch_order(s, a, ids, counts, ids + (size_t)kBatchClasses * P);
 		ctx.ensure_aux_streams();
 		HIP_CHECK(hipEventRecord(ctx.fork_ev, s));
 		// the class of larger meshes goes first: its longer workgroups start early and leave a
