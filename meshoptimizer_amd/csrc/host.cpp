@@ -641,12 +641,19 @@ static size_t simplify_on_device(Context& ctx, const CallArgs& c, float* result_
 	// Prune: smallest component error still present (src/simplifier.cpp:2466-2470)
 	float prune_next = prune ? host_float_bits(h.prune_min_bits) : 0.f;
 
+	// outside Prune the next pass's adjacency and pick are enqueued right after the index remap,
+	// reading the new triangle count on the device, so one read-back returns both counts
+	bool picked = false;
 	while (result_count > c.target_index_count)
 	{
 		size_t T = result_count / 3;
-		launch_adjacency(s, m, T);
-		launch_pick(s, m, T);
-		sync_scalars(ctx, m);
+		if (!picked)
+		{
+			launch_adjacency(s, m, T);
+			launch_pick(s, m, T);
+			sync_scalars(ctx, m);
+		}
+		picked = false;
 		size_t C = h.candidates;
 		if (C == 0)
 			break;
@@ -717,6 +724,12 @@ static size_t simplify_on_device(Context& ctx, const CallArgs& c, float* result_
 
 		launch_apply(s, m, C);
 		launch_remap_triangles(s, m, T);
+		if (!prune)
+		{
+			launch_adjacency(s, m, T, &m.sc->next_triangles);
+			launch_pick(s, m, T, &m.sc->next_triangles);
+			picked = true;
+		}
 		sync_scalars(ctx, m);
 		result_count = (size_t)h.next_triangles * 3;
 

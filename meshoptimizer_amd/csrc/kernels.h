@@ -106,8 +106,10 @@ void launch_attribute_quadrics(hipStream_t s, Mesh& m, const uint32_t* raw_off, 
 void launch_dual_count(hipStream_t s, Mesh& m, const uint32_t* raw_off);
 
 // ---------------------------------------------------------------- passes (pass.hip)
-void launch_adjacency(hipStream_t s, Mesh& m, size_t tri_count);
-void launch_pick(hipStream_t s, Mesh& m, size_t tri_count);
+// tri_count sizes the grids; `live`, when given, is a device word holding the actual count (at
+// most tri_count), so a pass can be enqueued before the host has read it
+void launch_adjacency(hipStream_t s, Mesh& m, size_t tri_count, const uint32_t* live = nullptr);
+void launch_pick(hipStream_t s, Mesh& m, size_t tri_count, const uint32_t* live = nullptr);
 void launch_rank_sort(hipStream_t s, Mesh& m, size_t cand_count);
 void launch_perform_init(hipStream_t s, Mesh& m, size_t cand_count, size_t window);
 void launch_round(hipStream_t s, Mesh& m, size_t list_bound, int round, int head_index);

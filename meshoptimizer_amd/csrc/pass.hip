@@ -38,17 +38,28 @@ __constant__ uint8_t c_has_opposite[5][5] = {
 // pass counts the corners per class and hands each corner its slot in the class's segment; after
 // the scan the fill places the corners without atomics.
 
-__global__ void k_adj_count(const uint32_t* ib, size_t n, const uint32_t* remap, uint32_t* counts, uint32_t* slot)
+// the live triangle count: the host's when `live` is null, else the device word (at most the
+// host's, which sizes the grid)
+__device__ __forceinline__ size_t live_tris(size_t tris_max, const uint32_t* live)
+{
+	if (!live)
+		return tris_max;
+	size_t n = *live;
+	return n < tris_max ? n : tris_max;
+}
+
+__global__ void k_adj_count(const uint32_t* ib, size_t tris_max, const uint32_t* live, const uint32_t* remap, uint32_t* counts, uint32_t* slot)
 {
 	size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-	if (j < n)
+	if (j < 3 * live_tris(tris_max, live))
 		slot[j] = atomicAdd(&counts[remap[ib[j]]], 1u);
 }
 
-__global__ void k_adj_fill(const uint32_t* ib, size_t tris, const uint32_t* remap, const uint32_t* off, const uint32_t* slot, uint2* adj)
+__global__ void k_adj_fill(const uint32_t* ib, size_t tris_max, const uint32_t* live, const uint32_t* remap, const uint32_t* off, const uint32_t* slot,
+                           uint2* adj)
 {
 	size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-	if (t >= tris)
+	if (t >= live_tris(tris_max, live))
 		return;
 	uint32_t a = remap[ib[t * 3 + 0]], b = remap[ib[t * 3 + 1]], c = remap[ib[t * 3 + 2]];
 	adj[off[a] + slot[t * 3 + 0]] = make_uint2(b, c);
@@ -58,15 +69,15 @@ __global__ void k_adj_fill(const uint32_t* ib, size_t tris, const uint32_t* rema
 
 // the corner slots go to m.keys, free whenever the adjacency is rebuilt (before pick, after the
 // pass loop)
-void launch_adjacency(hipStream_t s, Mesh& m, size_t tri_count)
+void launch_adjacency(hipStream_t s, Mesh& m, size_t tri_count, const uint32_t* live)
 {
 	size_t n = m.vertex_count;
 	fill_u32(s, m.adj_off, 0, n + 1);
 	if (tri_count)
-		k_adj_count<<<grid_for(tri_count * 3), kThreads, 0, s>>>(m.ib[m.ib_cur], tri_count * 3, m.remap, m.adj_off, m.keys);
+		k_adj_count<<<grid_for(tri_count * 3), kThreads, 0, s>>>(m.ib[m.ib_cur], tri_count, live, m.remap, m.adj_off, m.keys);
 	scan_exclusive(s, m.adj_off, m.adj_off, n, nullptr, m.adj_off + n, m.temp);
 	if (tri_count)
-		k_adj_fill<<<grid_for(tri_count), kThreads, 0, s>>>(m.ib[m.ib_cur], tri_count, m.remap, m.adj_off, m.keys, m.adj);
+		k_adj_fill<<<grid_for(tri_count), kThreads, 0, s>>>(m.ib[m.ib_cur], tri_count, live, m.remap, m.adj_off, m.keys, m.adj);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -112,22 +123,23 @@ __device__ __forceinline__ int pick_triangle(const uint32_t* ib, size_t t, const
 	return count;
 }
 
-__global__ void k_pick_count(const uint32_t* ib, size_t tris, const uint32_t* remap, const uint8_t* kind, const uint32_t* loop,
+__global__ void k_pick_count(const uint32_t* ib, size_t tris_max, const uint32_t* live, const uint32_t* remap, const uint8_t* kind, const uint32_t* loop,
                              const uint32_t* loopback, uint32_t* counts)
 {
 	size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-	if (t >= tris)
+	if (t >= live_tris(tris_max, live))
 		return;
 	uint32_t v0[3], v1[3], bd[3];
 	counts[t] = (uint32_t)pick_triangle(ib, t, remap, kind, loop, loopback, v0, v1, bd);
 }
 
 // triangles are taken while `written + 3 <= capacity` (src/simplifier.cpp:1318-1320)
-__global__ void k_pick_write(const uint32_t* ib, size_t tris, const uint32_t* remap, const uint8_t* kind, const uint32_t* loop,
+__global__ void k_pick_write(const uint32_t* ib, size_t tris_max, const uint32_t* live, const uint32_t* remap, const uint8_t* kind, const uint32_t* loop,
                              const uint32_t* loopback, const uint32_t* offsets, uint32_t capacity, uint32_t* c_v0, uint32_t* c_v1,
                              float* c_err, uint32_t* out_count)
 {
 	size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+	const size_t tris = live_tris(tris_max, live);
 	if (t >= tris)
 		return;
 	uint32_t base = offsets[t];
@@ -146,16 +158,16 @@ __global__ void k_pick_write(const uint32_t* ib, size_t tris, const uint32_t* re
 		*out_count = base + (uint32_t)count;
 }
 
-void launch_pick(hipStream_t s, Mesh& m, size_t tri_count)
+void launch_pick(hipStream_t s, Mesh& m, size_t tri_count, const uint32_t* live)
 {
 	uint32_t* loop = m.loop[m.loop_cur];
 	uint32_t* loopback = m.loopback[m.loop_cur];
 	HIP_CHECK(hipMemsetAsync(&m.sc->candidates, 0, sizeof(uint32_t), s));
 	if (!tri_count)
 		return;
-	k_pick_count<<<grid_for(tri_count), kThreads, 0, s>>>(m.ib[m.ib_cur], tri_count, m.remap, m.kind, loop, loopback, m.tri_cnt);
-	scan_exclusive(s, m.tri_cnt, m.tri_off, tri_count, nullptr, nullptr, m.temp);
-	k_pick_write<<<grid_for(tri_count), kThreads, 0, s>>>(m.ib[m.ib_cur], tri_count, m.remap, m.kind, loop, loopback, m.tri_off, m.capacity,
+	k_pick_count<<<grid_for(tri_count), kThreads, 0, s>>>(m.ib[m.ib_cur], tri_count, live, m.remap, m.kind, loop, loopback, m.tri_cnt);
+	scan_exclusive(s, m.tri_cnt, m.tri_off, tri_count, live, nullptr, m.temp);
+	k_pick_write<<<grid_for(tri_count), kThreads, 0, s>>>(m.ib[m.ib_cur], tri_count, live, m.remap, m.kind, loop, loopback, m.tri_off, m.capacity,
 	                                                      m.c_v0, m.c_v1, m.c_err, &m.sc->candidates);
 }
 
