@@ -42,18 +42,22 @@ size_t meshopt_hip_simplify_sloppy_device(unsigned int* destination, const unsig
 
 /**
  * Simplifies many independent meshes (for example the cluster groups of a clusterlod build) in
- * one launch; every mesh gets exactly what meshopt_simplify (src/meshoptimizer.h:513) returns for
- * it.  Mesh p owns indices [index_offsets[p], index_offsets[p+1]) (local to its vertices) and
- * vertices [vertex_offsets[p], vertex_offsets[p+1]) of the packed arrays, and writes its result
- * at destination + index_offsets[p]; result_counts[p] and result_errors[p] (may be NULL) receive
- * the index count and error.  target_error and options are shared by all meshes (Sparse and
- * Prune batches run mesh by mesh on the plain path).  Returns 0, or -1 with meshopt_hip_last_error set.
- * The _device variant takes device pointers for every array and the number of vertices the
- * positions hold (vertex_offsets[mesh_count] must not exceed it); the other copies host arrays
- * in and out around the same launch.
+ * one launch; every mesh gets exactly what meshopt_simplifyWithAttributes (src/meshoptimizer.h:537)
+ * returns for it with no attributes, its slice of vertex_lock and the shared options.  Mesh p
+ * owns indices [index_offsets[p], index_offsets[p+1]) (local to its vertices) and vertices
+ * [vertex_offsets[p], vertex_offsets[p+1]) of the packed arrays, and writes its result at
+ * destination + index_offsets[p]; result_counts[p] and result_errors[p] (may be NULL) receive
+ * the index count and error.  vertex_lock (may be NULL) holds one meshopt_SimplifyVertex_* byte
+ * per packed vertex.  target_error and options are shared by all meshes (Sparse and Prune
+ * batches run mesh by mesh on the plain path; a clusterlod group packed with its vertices in
+ * first-use order is what Sparse would make of it).  Returns 0, or -1 with meshopt_hip_last_error
+ * set.  The _device variant takes device pointers for every array and the number of vertices the
+ * positions hold (vertex_offsets[mesh_count] must not exceed it); the other copies host arrays in
+ * and out around the same launch.  No counterpart in the reference, whose callers loop over
+ * meshopt_simplify* per group (demo/clusterlod.h:470-479).
  */
-int meshopt_hip_simplify_batch_device(unsigned int* destination, unsigned int* result_counts, float* result_errors, const unsigned int* indices, const unsigned int* index_offsets, const float* vertex_positions, const unsigned int* vertex_offsets, size_t vertex_count, size_t vertex_positions_stride, const unsigned int* target_index_counts, size_t mesh_count, float target_error, unsigned int options);
-int meshopt_hip_simplify_batch(unsigned int* destination, unsigned int* result_counts, float* result_errors, const unsigned int* indices, const unsigned int* index_offsets, const float* vertex_positions, const unsigned int* vertex_offsets, size_t vertex_positions_stride, const unsigned int* target_index_counts, size_t mesh_count, float target_error, unsigned int options);
+int meshopt_hip_simplify_batch_device(unsigned int* destination, unsigned int* result_counts, float* result_errors, const unsigned int* indices, const unsigned int* index_offsets, const float* vertex_positions, const unsigned int* vertex_offsets, const unsigned char* vertex_lock, size_t vertex_count, size_t vertex_positions_stride, const unsigned int* target_index_counts, size_t mesh_count, float target_error, unsigned int options);
+int meshopt_hip_simplify_batch(unsigned int* destination, unsigned int* result_counts, float* result_errors, const unsigned int* indices, const unsigned int* index_offsets, const float* vertex_positions, const unsigned int* vertex_offsets, const unsigned char* vertex_lock, size_t vertex_positions_stride, const unsigned int* target_index_counts, size_t mesh_count, float target_error, unsigned int options);
 
 /* Per-pass counters of the last call made on the calling thread; the same quantities the
  * reference prints under TRACE (src/simplifier.cpp:1625-1633). */

@@ -127,9 +127,9 @@ def library() -> ctypes.CDLL:
     lib.meshopt_hip_last_error.restype = ctypes.c_char_p
     lib.meshopt_hip_last_error.argtypes = []
     lib.meshopt_hip_simplify_batch.restype = ctypes.c_int
-    lib.meshopt_hip_simplify_batch.argtypes = [_P, _P, _P, _P, _P, _P, _P, _SZ, _P, _SZ, _F, _U]
+    lib.meshopt_hip_simplify_batch.argtypes = [_P, _P, _P, _P, _P, _P, _P, _P, _SZ, _P, _SZ, _F, _U]
     lib.meshopt_hip_simplify_batch_device.restype = ctypes.c_int
-    lib.meshopt_hip_simplify_batch_device.argtypes = [_P, _P, _P, _P, _P, _P, _P, _SZ, _SZ, _P, _SZ, _F, _U]
+    lib.meshopt_hip_simplify_batch_device.argtypes = [_P, _P, _P, _P, _P, _P, _P, _P, _SZ, _SZ, _P, _SZ, _F, _U]
     lib.meshopt_hip_batch_kernel_timing.restype = None
     lib.meshopt_hip_batch_kernel_timing.argtypes = [_P, _P, _P]
     lib.meshopt_hip_kernel_timing.restype = None
@@ -316,17 +316,24 @@ def pack_meshes(meshes):
     return idx, io, verts, vo, width * 4
 
 
-def simplify_batch(meshes, target_index_counts, target_error=1e-2, options=0):
-    """Simplifies every (indices, positions) mesh in one batched launch (meshopt_simplify per
-    mesh).  Returns [(indices, result_error)]."""
+def simplify_batch(meshes, target_index_counts, target_error=1e-2, options=0, locks=None):
+    """Simplifies every (indices, positions) mesh in one batched launch (meshopt_simplify per mesh,
+    or meshopt_simplifyWithAttributes with no attributes and the mesh's vertex_lock when `locks`
+    gives one uint8 array per mesh).  Returns [(indices, result_error)]."""
     lib = library()
     idx, io, verts, vo, stride = pack_meshes(meshes)
+    lk = None
+    if locks is not None:
+        lk = np.zeros(max(int(vo[-1]), 1), dtype=np.uint8)
+        for p, l in enumerate(locks):
+            lk[vo[p]:vo[p + 1]] = np.asarray(l, dtype=np.uint8)
     tg = np.ascontiguousarray(target_index_counts, dtype=np.uint32)
     dest = np.zeros(max(idx.size, 1), dtype=np.uint32)
     counts = np.zeros(max(len(meshes), 1), dtype=np.uint32)
     errors = np.zeros(max(len(meshes), 1), dtype=np.float32)
     rc = lib.meshopt_hip_simplify_batch(dest.ctypes.data, counts.ctypes.data, errors.ctypes.data, idx.ctypes.data, io.ctypes.data,
-                                        verts.ctypes.data, vo.ctypes.data, stride, tg.ctypes.data, len(meshes),
+                                        verts.ctypes.data, vo.ctypes.data, None if lk is None else lk.ctypes.data, stride,
+                                        tg.ctypes.data, len(meshes),
                                         float(target_error), int(options))
     if rc != 0:
         raise RuntimeError(lib.meshopt_hip_last_error().decode())
@@ -357,13 +364,14 @@ def batch_kernel_timing():
 
 def simplify_batch_device(destination_ptr, result_counts_ptr, result_errors_ptr, indices_ptr, index_offsets_ptr,
                           positions_ptr, vertex_offsets_ptr, vertex_count, stride, target_counts_ptr, mesh_count,
-                          target_error, options=0):
+                          target_error, options=0, lock_ptr=None):
     """meshopt_hip_simplify_batch_device on device pointers (integers); vertex_count is the number of
-    vertices the positions hold.  Raises on error."""
+    vertices the positions hold, lock_ptr one vertex_lock byte per packed vertex or None.  Raises on
+    error."""
     lib = library()
     rc = lib.meshopt_hip_simplify_batch_device(destination_ptr, result_counts_ptr, result_errors_ptr, indices_ptr,
-                                               index_offsets_ptr, positions_ptr, vertex_offsets_ptr, vertex_count, stride,
-                                               target_counts_ptr, mesh_count, float(target_error), int(options))
+                                               index_offsets_ptr, positions_ptr, vertex_offsets_ptr, lock_ptr, vertex_count,
+                                               stride, target_counts_ptr, mesh_count, float(target_error), int(options))
     if rc != 0:
         raise RuntimeError(lib.meshopt_hip_last_error().decode())
 

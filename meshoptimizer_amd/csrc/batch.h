@@ -21,6 +21,7 @@ struct BatchArgs
 	const uint32_t* index_offsets;     // mesh_count + 1
 	const float* positions;            // packed vertices (position in the first 12 bytes)
 	const uint32_t* vertex_offsets;    // mesh_count + 1
+	const uint8_t* vertex_lock;        // per packed vertex (like the positions), or null
 	size_t vertex_count;               // vertices the positions hold; every vertex_offsets[p + 1] is at most this
 	Quad* quadrics;                    // vertex_count quadrics: mesh p keeps its vertex quadrics at vertex_offsets[p]
 	size_t position_stride_floats;

@@ -603,6 +603,13 @@ __device__ __forceinline__ void simplify_patch(const BatchArgs& a, const PatchLa
 				continue;
 			bool protect = false;
 			uint32_t v = i;
+			// a Protect bit on any wedge keeps the vertex (src/simplifier.cpp:498-505)
+			if (a.vertex_lock)
+				do
+				{
+					protect |= (a.vertex_lock[vb_begin + v] & kVertexProtect) != 0;
+					v = s.wedge[v];
+				} while (v != i);
 			do
 			{
 				for (uint32_t p = s.off[v], e = s.off[v + 1]; p < e && !protect; ++p)
@@ -612,6 +619,15 @@ __device__ __forceinline__ void simplify_patch(const BatchArgs& a, const PatchLa
 			if (!protect)
 				s.kind[i] = kComplex;
 		}
+		__syncthreads();
+	}
+	// a Lock bit on any wedge locks its primary (src/simplifier.cpp:518-530); the loop below
+	// carries the primary's kind to every wedge
+	if (a.vertex_lock)
+	{
+		for (uint32_t i = tid; i < V; i += kBatchThreads)
+			if (a.vertex_lock[vb_begin + i] & kVertexLock)
+				s.kind[s.remap[i]] = kLocked;
 		__syncthreads();
 	}
 	for (uint32_t i = tid; i < V; i += kBatchThreads)
@@ -729,9 +745,11 @@ __device__ __forceinline__ void simplify_patch(const BatchArgs& a, const PatchLa
 					quad_add(Q, F);
 				}
 				{
+					// Priority raises the regularising weight of a primary (src/simplifier.cpp:1095-1098)
+					const bool priority = a.vertex_lock && (a.vertex_lock[vb_begin + r] & kVertexPriority) != 0;
 					V3 v = P(s, r);
 					Quad Pq;
-					quad_from_point(Pq, v.x, v.y, v.z, Q.w * factor);
+					quad_from_point(Pq, v.x, v.y, v.z, Q.w * (priority ? 1.0f : factor));
 					quad_add(Q, Pq);
 				}
 				uint32_t last_t = kNone;

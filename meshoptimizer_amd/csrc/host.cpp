@@ -1343,7 +1343,8 @@ namespace mhip
 // are read back only then (h_* may be null: device-resident offsets).
 static int simplify_batch_on_device(Context& ctx, unsigned int* destination, unsigned int* result_counts, float* result_errors,
                                     const unsigned int* indices, const unsigned int* index_offsets, const float* positions,
-                                    const unsigned int* vertex_offsets, size_t vertex_count, size_t stride, const unsigned int* target_index_counts,
+                                    const unsigned int* vertex_offsets, const unsigned char* vertex_lock, size_t vertex_count, size_t stride,
+                                    const unsigned int* target_index_counts,
                                     size_t patch_count, float target_error, unsigned int options, const unsigned int* h_index_offsets,
                                     const unsigned int* h_vertex_offsets, const unsigned int* h_targets)
 {
@@ -1373,6 +1374,7 @@ static int simplify_batch_on_device(Context& ctx, unsigned int* destination, uns
 	a.index_offsets = index_offsets;
 	a.positions = positions;
 	a.vertex_offsets = vertex_offsets;
+	a.vertex_lock = vertex_lock;
 	a.vertex_count = vertex_count;
 	a.quadrics = nullptr;
 	a.position_stride_floats = stride / sizeof(float);
@@ -1501,7 +1503,7 @@ static int simplify_batch_on_device(Context& ctx, unsigned int* destination, uns
 		c.attribute_stride = 0;
 		c.weights = nullptr;
 		c.attribute_count = 0;
-		c.lock = nullptr;
+		c.lock = vertex_lock ? vertex_lock + h_vertex_offsets[p] : nullptr;
 		c.target_index_count = h_targets[p];
 		c.target_error = target_error;
 		c.options = options;
@@ -1519,8 +1521,8 @@ static int simplify_batch_on_device(Context& ctx, unsigned int* destination, uns
 
 extern "C" int meshopt_hip_simplify_batch_device(unsigned int* destination, unsigned int* result_counts, float* result_errors, const unsigned int* indices,
                                                  const unsigned int* index_offsets, const float* vertex_positions, const unsigned int* vertex_offsets,
-                                                 size_t vertex_count, size_t vertex_positions_stride, const unsigned int* target_index_counts,
-                                                 size_t mesh_count, float target_error, unsigned int options)
+                                                 const unsigned char* vertex_lock, size_t vertex_count, size_t vertex_positions_stride,
+                                                 const unsigned int* target_index_counts, size_t mesh_count, float target_error, unsigned int options)
 {
 	try
 	{
@@ -1528,8 +1530,8 @@ extern "C" int meshopt_hip_simplify_batch_device(unsigned int* destination, unsi
 		std::lock_guard<std::mutex> guard(ctx.mutex);
 		ctx.ensure_init();
 		simplify_batch_on_device(ctx, destination, result_counts, result_errors, indices, index_offsets, vertex_positions, vertex_offsets,
-		                         vertex_count, vertex_positions_stride, target_index_counts, mesh_count, target_error, options, nullptr, nullptr,
-		                         nullptr);
+		                         vertex_lock, vertex_count, vertex_positions_stride, target_index_counts, mesh_count, target_error, options, nullptr,
+		                         nullptr, nullptr);
 		t_last_error.clear();
 		return 0;
 	}
@@ -1542,8 +1544,8 @@ extern "C" int meshopt_hip_simplify_batch_device(unsigned int* destination, unsi
 
 extern "C" int meshopt_hip_simplify_batch(unsigned int* destination, unsigned int* result_counts, float* result_errors, const unsigned int* indices,
                                           const unsigned int* index_offsets, const float* vertex_positions, const unsigned int* vertex_offsets,
-                                          size_t vertex_positions_stride, const unsigned int* target_index_counts, size_t mesh_count, float target_error,
-                                          unsigned int options)
+                                          const unsigned char* vertex_lock, size_t vertex_positions_stride, const unsigned int* target_index_counts,
+                                          size_t mesh_count, float target_error, unsigned int options)
 {
 	try
 	{
@@ -1561,7 +1563,9 @@ extern "C" int meshopt_hip_simplify_batch(unsigned int* destination, unsigned in
 		size_t off_rc = off_tg + ((n1 * 4 + 255) & ~size_t(255));
 		size_t off_re = off_rc + ((n1 * 4 + 255) & ~size_t(255));
 		size_t off_pos = off_re + ((n1 * 4 + 255) & ~size_t(255));
-		char* d = static_cast<char*>(ctx.staging.get(off_pos + pos_bytes + 256));
+		size_t off_lock = off_pos + ((pos_bytes + 255) & ~size_t(255));
+		size_t lock_bytes = vertex_lock ? V : 0;
+		char* d = static_cast<char*>(ctx.staging.get(off_lock + lock_bytes + 256));
 		if (I)
 			HIP_CHECK(hipMemcpyAsync(d + off_idx, indices, I * 4, hipMemcpyHostToDevice, s));
 		HIP_CHECK(hipMemcpyAsync(d + off_io, index_offsets, n1 * 4, hipMemcpyHostToDevice, s));
@@ -1570,10 +1574,13 @@ extern "C" int meshopt_hip_simplify_batch(unsigned int* destination, unsigned in
 			HIP_CHECK(hipMemcpyAsync(d + off_tg, target_index_counts, mesh_count * 4, hipMemcpyHostToDevice, s));
 		if (pos_bytes)
 			HIP_CHECK(hipMemcpyAsync(d + off_pos, vertex_positions, pos_bytes, hipMemcpyHostToDevice, s));
+		if (lock_bytes)
+			HIP_CHECK(hipMemcpyAsync(d + off_lock, vertex_lock, lock_bytes, hipMemcpyHostToDevice, s));
 		simplify_batch_on_device(ctx, reinterpret_cast<unsigned int*>(d + off_dst), reinterpret_cast<unsigned int*>(d + off_rc),
 		                         reinterpret_cast<float*>(d + off_re), reinterpret_cast<const unsigned int*>(d + off_idx),
 		                         reinterpret_cast<const unsigned int*>(d + off_io), reinterpret_cast<const float*>(d + off_pos),
-		                         reinterpret_cast<const unsigned int*>(d + off_vo), V, vertex_positions_stride,
+		                         reinterpret_cast<const unsigned int*>(d + off_vo), lock_bytes ? reinterpret_cast<const unsigned char*>(d + off_lock) : nullptr,
+		                         V, vertex_positions_stride,
 		                         reinterpret_cast<const unsigned int*>(d + off_tg), mesh_count, target_error, options, index_offsets, vertex_offsets,
 		                         target_index_counts);
 		if (mesh_count)

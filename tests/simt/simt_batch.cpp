@@ -260,6 +260,14 @@ static int run_group(int threads, bool shuffle)
 
 // Simplifies every mesh of a packed batch with the kernel's device code, mesh by mesh, using the
 // launch's LDS layout for (vmax, imax).  status[p] receives the kernel's status word.
+// vertex_lock of the next simt_simplify_batch call (one byte per packed vertex, or null)
+static const unsigned char* g_vertex_lock = nullptr;
+
+extern "C" void simt_set_vertex_lock(const unsigned char* lock)
+{
+	g_vertex_lock = lock;
+}
+
 extern "C" int simt_simplify_batch(unsigned int* destination, unsigned int* result_counts, float* result_errors, unsigned int* status,
                                    unsigned int* passes, const unsigned int* indices, const unsigned int* index_offsets, const float* positions,
                                    const unsigned int* vertex_offsets, size_t stride, const unsigned int* targets, size_t count,
@@ -279,6 +287,7 @@ extern "C" int simt_simplify_batch(unsigned int* destination, unsigned int* resu
 	a.index_offsets = index_offsets;
 	a.positions = positions;
 	a.vertex_offsets = vertex_offsets;
+	a.vertex_lock = g_vertex_lock;
 	a.vertex_count = count ? vertex_offsets[count] : 0;
 	// the quadric buffer in global memory, undefined at the start like a fresh device allocation
 	std::vector<mhip::Quad> quadrics(a.vertex_count + 1);

@@ -19,11 +19,11 @@ def announce(what):
     sys.stdout.flush()
 
 
-def check_batch(meshes, targets, err, opts):
-    got = M.simplify_batch(meshes, targets, err, opts)
+def check_batch(meshes, targets, err, opts, locks=None):
+    got = M.simplify_batch(meshes, targets, err, opts, locks=locks)
     assert len(got) == len(meshes)
     for p, ((ib, v), t, (rg, eg)) in enumerate(zip(meshes, targets, got)):
-        ro, eo = O.simplify(ib, v, int(t), err, opts)
+        ro, eo = O.simplify(ib, v, int(t), err, opts, lock=None if locks is None else locks[p])
         assert rg.size == ro.size, f"mesh {p}: index count {rg.size} != oracle {ro.size}"
         assert np.array_equal(rg, ro), f"mesh {p}: first difference at {int(np.argmax(rg != ro))}"
         assert same_bits(eg, eo), f"mesh {p}: result_error {eg!r} != oracle {eo!r}"
@@ -123,3 +123,67 @@ def test_prune_batch():
         targets.append(0 if k % 2 else (ib.size // 3 // 2) * 3)
     announce("prune batch")
     check_batch(meshes, targets, 5e-2, M.SimplifyPrune)
+
+
+@pytest.mark.parametrize("seed", range(10))
+def test_vertex_locks(seed):
+    """Per-vertex Lock, Protect and Priority bits in the batched kernel (one byte per packed vertex),
+    with and without Permissive and ErrorAbsolute, against the oracle given each mesh's slice."""
+    rng = np.random.default_rng(7100 + seed)
+    meshes, targets, locks = [], [], []
+    for q in range(int(rng.integers(4, 24))):
+        if q % 2:
+            ib, v = random_grid(rng, n=int(rng.integers(4, 14)))
+        else:
+            ib, v = random_soup(rng, nv=int(rng.integers(12, 150)), ntri=int(rng.integers(8, 250)))
+        v = np.ascontiguousarray(v[:, :3])
+        meshes.append((ib, v))
+        targets.append((ib.size // 3 * int(rng.integers(0, 3)) // 4) * 3)
+        locks.append((rng.integers(0, 8, size=v.shape[0]) * (rng.random(v.shape[0]) < 0.3)).astype(np.uint8))
+    opts = [0, M.SimplifyPermissive, M.SimplifyLockBorder | M.SimplifyPermissive, M.SimplifyErrorAbsolute | M.SimplifyPermissive,
+            M.SimplifyRegularize][seed % 5]
+    err = float(rng.choice([1e-2, 1e-1, 1.0]))
+    announce(f"batch vertex locks seed={seed} meshes={len(meshes)} opts={opts} err={err}")
+    check_batch(meshes, targets, err, opts, locks=locks)
+
+
+def test_clusterlod_level_groups():
+    """One clusterlod level in one batched call: the triangles of a grid split into spatial groups,
+    vertices used by two groups locked (lockBoundary, demo/clusterlod.h:372-419), every group
+    simplified to half with ErrorAbsolute | Permissive and no error limit (demo/clusterlod.h:442-451).
+    Each group is packed with its vertices in first-use order, which is what Sparse makes of it, so
+    every result must equal the oracle's Sparse call on the shared vertex buffer, mapped back."""
+    m = W.grid(64)
+    v = np.ascontiguousarray(m.vertices[:, :3])
+    tri = m.indices.reshape(-1, 3)
+    centre = v[tri].mean(axis=1)
+    tiles = 4
+    g = np.minimum((centre[:, 1] * tiles).astype(int), tiles - 1) * tiles + np.minimum((centre[:, 0] * tiles).astype(int), tiles - 1)
+    groups = [tri[g == k].reshape(-1).astype(np.uint32) for k in range(tiles * tiles) if (g == k).any()]
+    _, first, inv = np.unique(v, axis=0, return_index=True, return_inverse=True)
+    remap = first[inv.reshape(-1)]
+    uses = np.zeros(v.shape[0], np.int32)
+    for grp in groups:
+        uses[np.unique(remap[grp])] += 1
+    locks = (uses >= 2).astype(np.uint8)[remap]
+    flt_max = float(np.finfo(np.float32).max)
+
+    packed, targets, plocks, orders = [], [], [], []
+    for grp in groups:
+        _, at = np.unique(grp, return_index=True)
+        order = grp[np.sort(at)]
+        local = np.zeros(v.shape[0], np.uint32)
+        local[order] = np.arange(order.size, dtype=np.uint32)
+        packed.append((local[grp], np.ascontiguousarray(v[order])))
+        plocks.append(locks[order])
+        orders.append(order)
+        targets.append(grp.size // 3 // 2 * 3)
+    announce(f"clusterlod level: {len(groups)} groups, {int(locks.sum())} locked vertices")
+    opts = M.SimplifyErrorAbsolute | M.SimplifyPermissive
+    got = M.simplify_batch(packed, targets, flt_max, opts, locks=plocks)
+    for p, (grp, t, order, (rg, eg)) in enumerate(zip(groups, targets, orders, got)):
+        ro, eo = O.simplify(grp, v, t, flt_max, opts | M.SimplifySparse, attributes=np.zeros((v.shape[0], 0), np.float32),
+                            weights=np.zeros(0, np.float32), lock=locks)
+        assert np.array_equal(order[rg], ro), f"group {p}: indices differ from the Sparse call"
+        assert same_bits(eg, eo), f"group {p}: result_error {eg!r} != {eo!r}"
+        assert rg.size < grp.size

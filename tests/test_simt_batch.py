@@ -29,21 +29,32 @@ def simt():
     if gxx is None:
         pytest.skip("g++ not available")
     os.makedirs(os.path.dirname(OUT), exist_ok=True)
+    # one build per process: parallel test workers must not load each other's half-written file
+    out = f"{OUT}.{os.getpid()}"
     cmd = [gxx, "-std=c++17", "-O1", "-g", "-fPIC", "-shared", "-ffp-contract=off", "-fno-fast-math", "-Wno-unknown-pragmas",
-           "-I", os.path.join(SIMT, "shim"), "-I", CSRC, os.path.join(SIMT, "simt_batch.cpp"), "-o", OUT]
+           "-I", os.path.join(SIMT, "shim"), "-I", CSRC, os.path.join(SIMT, "simt_batch.cpp"), "-o", out]
     subprocess.run(cmd, check=True)
-    lib = ctypes.CDLL(OUT)
+    lib = ctypes.CDLL(out)
+    os.replace(out, OUT)
     P, SZ = ctypes.c_void_p, ctypes.c_size_t
     lib.simt_simplify_batch.restype = ctypes.c_int
     lib.simt_simplify_batch.argtypes = [P, P, P, P, P, P, P, P, P, SZ, P, SZ, ctypes.c_float, ctypes.c_uint, ctypes.c_uint,
                                         ctypes.c_uint, ctypes.c_int, ctypes.c_uint, ctypes.c_ulonglong]
     lib.simt_retries.restype = ctypes.c_uint
     lib.simt_retries.argtypes = []
+    lib.simt_set_vertex_lock.restype = None
+    lib.simt_set_vertex_lock.argtypes = [P]
     return lib
 
 
-def run_simt(lib, meshes, targets, err, opts, mode=3, seed=1):
+def run_simt(lib, meshes, targets, err, opts, mode=3, seed=1, locks=None):
     idx, io, verts, vo, stride = M.pack_meshes(meshes)
+    lk = None
+    if locks is not None:
+        lk = np.zeros(max(int(vo[-1]), 1), np.uint8)
+        for p, l in enumerate(locks):
+            lk[vo[p]:vo[p + 1]] = l
+    lib.simt_set_vertex_lock(None if lk is None else lk.ctypes.data)
     tg = np.ascontiguousarray(targets, dtype=np.uint32)
     n = len(meshes)
     vmax = (int(np.diff(vo).max()) + 31) & ~31
@@ -55,14 +66,16 @@ def run_simt(lib, meshes, targets, err, opts, mode=3, seed=1):
     rc = lib.simt_simplify_batch(dest.ctypes.data, cnt.ctypes.data, errs.ctypes.data, status.ctypes.data, None,
                                  idx.ctypes.data, io.ctypes.data, verts.ctypes.data, vo.ctypes.data, stride, tg.ctypes.data,
                                  n, float(err), int(opts), vmax, imax, 120, mode, seed)
+    lib.simt_set_vertex_lock(None)
     assert rc == 0, "deadlock in the device code (see stderr)"
     return [(dest[io[p]:io[p] + cnt[p]].copy(), float(errs[p]), int(status[p])) for p in range(n)]
 
 
-def check(lib, meshes, targets, err, opts, **kw):
-    for p, ((ib, v), t, (r, e, st)) in enumerate(zip(meshes, targets, run_simt(lib, meshes, targets, err, opts, **kw))):
+def check(lib, meshes, targets, err, opts, locks=None, **kw):
+    got = run_simt(lib, meshes, targets, err, opts, locks=locks, **kw)
+    for p, ((ib, v), t, (r, e, st)) in enumerate(zip(meshes, targets, got)):
         assert st == 0, f"mesh {p}: status {st}"
-        ro, eo = O.simplify(ib, v, int(t), err, opts)
+        ro, eo = O.simplify(ib, v, int(t), err, opts, lock=None if locks is None else locks[p])
         assert r.size == ro.size and np.array_equal(r, ro), f"mesh {p}: indices differ from the oracle"
         assert same_bits(e, eo), f"mesh {p}: result_error {e!r} != {eo!r}"
 
@@ -100,3 +113,22 @@ def test_candidate_overflow_runs_again(simt):
     meshes = [(ib, v), (g.indices, g.vertices)]
     check(simt, meshes, [ib.size // 2 // 3 * 3, W.target_count(g.index_count, 1, 2)], 1.0, 0)
     assert simt.simt_retries() >= 1
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_vertex_locks(simt, seed):
+    """Per-vertex Lock, Protect and Priority bits (src/simplifier.cpp:498-530, 1095-1098) in the
+    batched device code, with and without Permissive, against the oracle given each mesh's slice."""
+    rng = np.random.default_rng(700 + seed)
+    meshes, targets, locks = [], [], []
+    for q in range(4):
+        if q % 2:
+            ib, v = random_grid(rng, n=int(rng.integers(4, 12)))
+        else:
+            ib, v = random_soup(rng, nv=int(rng.integers(12, 90)), ntri=int(rng.integers(8, 160)))
+        v = np.ascontiguousarray(v[:, :3])
+        meshes.append((ib, v))
+        targets.append((ib.size // 3 * int(rng.integers(0, 3)) // 4) * 3)
+        locks.append(rng.integers(0, 8, size=v.shape[0]).astype(np.uint8) * (rng.random(v.shape[0]) < 0.3))
+    opts = [0, 32, 1 | 32, 4 | 32, 16][seed % 5]
+    check(simt, meshes, targets, float(rng.choice([1e-2, 1e-1, 1.0])), opts, locks=locks, seed=seed)
