@@ -41,6 +41,23 @@ size_t meshopt_hip_simplify_with_update_device(unsigned int* indices, size_t ind
 size_t meshopt_hip_simplify_sloppy_device(unsigned int* destination, const unsigned int* indices, size_t index_count, const float* vertex_positions, size_t vertex_count, size_t vertex_positions_stride, const unsigned char* vertex_lock, size_t target_index_count, float target_error, float* result_error);
 
 /**
+ * One clusterlod level on the device (the group loop of clodBuild, demo/clusterlod.h:640-690,
+ * without its fallbacks): group g owns indices [index_offsets[g], index_offsets[g+1]) of
+ * `indices`, which reference the shared vertex buffer (vertex_count vertices, remap = its position
+ * remap, as meshopt_generatePositionRemap gives it).  locks[i] (vertex_count bytes, written)
+ * receives the level's locks: meshopt_SimplifyVertex_Lock on every vertex whose position two or
+ * more groups use (lockBoundary, demo/clusterlod.h:372-419), ORed with base_locks[i] (may be NULL:
+ * the caller's protect and lock bits).  Every group is then simplified as
+ * meshopt_simplifyWithAttributes(group indices, the shared vertices, no attributes, locks,
+ * target_index_counts[g], target_error, options | meshopt_SimplifySparse) returns it
+ * (demo/clusterlod.h:442-451 passes Sparse | ErrorAbsolute | Permissive and FLT_MAX), all groups
+ * in one batched launch: result at destination + index_offsets[g] in shared vertex ids,
+ * result_counts[g], result_errors[g].  All arrays are device pointers.  Returns 0, or -1 with
+ * meshopt_hip_last_error set.  No counterpart in the reference library (clusterlod is its demo).
+ */
+int meshopt_hip_clod_level_device(unsigned int* destination, unsigned int* result_counts, float* result_errors, const unsigned int* indices, const unsigned int* index_offsets, size_t group_count, const float* vertex_positions, size_t vertex_count, size_t vertex_positions_stride, const unsigned int* remap, const unsigned char* base_locks, unsigned char* locks, const unsigned int* target_index_counts, float target_error, unsigned int options);
+
+/**
  * Simplifies many independent meshes (for example the cluster groups of a clusterlod build) in
  * one launch; every mesh gets exactly what meshopt_simplifyWithAttributes (src/meshoptimizer.h:537)
  * returns for it with no attributes, its slice of vertex_lock and the shared options.  Mesh p

@@ -59,6 +59,7 @@ EXPORTED_SYMBOLS = (
     "meshopt_hip_simplify_device",
     "meshopt_hip_simplify_batch",
     "meshopt_hip_simplify_batch_device",
+    "meshopt_hip_clod_level_device",
     "meshopt_hip_batch_kernel_timing",
     "meshopt_hip_server_create",
     "meshopt_hip_server_submit",
@@ -128,6 +129,8 @@ def library() -> ctypes.CDLL:
     lib.meshopt_hip_last_error.argtypes = []
     lib.meshopt_hip_simplify_batch.restype = ctypes.c_int
     lib.meshopt_hip_simplify_batch.argtypes = [_P, _P, _P, _P, _P, _P, _P, _P, _SZ, _P, _SZ, _F, _U]
+    lib.meshopt_hip_clod_level_device.restype = ctypes.c_int
+    lib.meshopt_hip_clod_level_device.argtypes = [_P, _P, _P, _P, _P, _SZ, _P, _SZ, _SZ, _P, _P, _P, _P, _F, _U]
     lib.meshopt_hip_simplify_batch_device.restype = ctypes.c_int
     lib.meshopt_hip_simplify_batch_device.argtypes = [_P, _P, _P, _P, _P, _P, _P, _P, _SZ, _SZ, _P, _SZ, _F, _U]
     lib.meshopt_hip_batch_kernel_timing.restype = None
@@ -372,6 +375,18 @@ def simplify_batch_device(destination_ptr, result_counts_ptr, result_errors_ptr,
     rc = lib.meshopt_hip_simplify_batch_device(destination_ptr, result_counts_ptr, result_errors_ptr, indices_ptr,
                                                index_offsets_ptr, positions_ptr, vertex_offsets_ptr, lock_ptr, vertex_count,
                                                stride, target_counts_ptr, mesh_count, float(target_error), int(options))
+    if rc != 0:
+        raise RuntimeError(lib.meshopt_hip_last_error().decode())
+
+
+def clod_level_device(destination_ptr, result_counts_ptr, result_errors_ptr, indices_ptr, index_offsets_ptr, group_count,
+                      positions_ptr, vertex_count, stride, remap_ptr, base_locks_ptr, locks_ptr, target_counts_ptr,
+                      target_error, options):
+    """meshopt_hip_clod_level_device on device pointers (integers).  Raises on error."""
+    lib = library()
+    rc = lib.meshopt_hip_clod_level_device(destination_ptr, result_counts_ptr, result_errors_ptr, indices_ptr, index_offsets_ptr,
+                                           group_count, positions_ptr, vertex_count, stride, remap_ptr, base_locks_ptr, locks_ptr,
+                                           target_counts_ptr, float(target_error), int(options))
     if rc != 0:
         raise RuntimeError(lib.meshopt_hip_last_error().decode())
 

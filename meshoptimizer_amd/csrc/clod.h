@@ -1,0 +1,43 @@
+// One clusterlod level on the device (demo/clusterlod.h:640-690): boundary locks over the level's
+// groups, each group packed with its vertices in first-use order (what Sparse does inside the
+// simplifier), the batched simplification of the packed groups, and the results mapped back.
+#pragma once
+
+#include "common.h"
+
+namespace mhip
+{
+
+// device buffers of one level (carved by the host); I = indices of all groups, G = groups,
+// V = vertices of the shared buffer
+struct ClodBuffers
+{
+	const uint32_t* indices;       // I: every group's index buffer (shared vertex ids), concatenated
+	const uint32_t* index_offsets; // G + 1
+	const uint32_t* remap;         // V: position remap (lowest vertex with the same position)
+	uint32_t group_count;
+	uint32_t vertex_count;
+
+	uint32_t* first_group;         // V: lowest group that uses each remapped vertex
+	uint8_t* boundary;             // V: 1 when a second group uses the remapped vertex
+	uint32_t* slot_key;            // 2I: per-group hash slices (at 2 * index_offsets[g]): vertex + 1, 0 = empty
+	uint32_t* slot_first;          // 2I: first position of the vertex in its group
+	uint32_t* slot_local;          // 2I: its packed id
+	uint32_t* unique;              // G + 1: vertices per group, then their exclusive scan (packed vertex offsets)
+	uint32_t* order;               // I: packed vertex -> shared vertex id
+	uint32_t* packed_indices;      // I
+	float* packed_positions;       // 3I
+	uint8_t* packed_locks;         // I
+	uint32_t* packed_result;       // I
+};
+
+// locks[i] = (a second group uses remap[i] ? Lock : 0) | base_locks[i] (base_locks may be null)
+// (lockBoundary, demo/clusterlod.h:372-419, with the caller's protect and lock bits as base_locks)
+void launch_clod_locks(hipStream_t s, const ClodBuffers& b, const uint8_t* base_locks, uint8_t* locks);
+// per-group first-use hashing and counts; then (after the host scans `unique`) the packing
+void launch_clod_hash(hipStream_t s, const ClodBuffers& b);
+void launch_clod_pack(hipStream_t s, const ClodBuffers& b, const float* positions, size_t stride_floats, const uint8_t* locks);
+// destination[index_offsets[g] + k] = order[unique[g] + packed_result[index_offsets[g] + k]], k < counts[g]
+void launch_clod_unpack(hipStream_t s, const ClodBuffers& b, const uint32_t* counts, uint32_t* destination);
+
+} // namespace mhip

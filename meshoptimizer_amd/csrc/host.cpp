@@ -25,6 +25,7 @@
 #include "../../include/meshopt_hip.h"
 #include "../../include/meshoptimizer.h"
 #include "batch.h"
+#include "clod.h"
 #include "sloppy.h"
 #include "common.h"
 #include "kernels.h"
@@ -167,6 +168,24 @@ struct Context
 		size_t grow = vertices + vertices / 4;
 		HIP_CHECK(hipMalloc((void**)&batch_quadrics, grow * sizeof(Quad)));
 		batch_quadric_count = grow;
+	}
+
+	// one clusterlod level's packing buffers (meshopt_hip_clod_level_device)
+	void* clod_buf = nullptr;
+	size_t clod_bytes = 0;
+	void* ensure_clod(size_t bytes)
+	{
+		if (bytes > clod_bytes)
+		{
+			if (clod_buf)
+				HIP_CHECK(hipFree(clod_buf));
+			clod_buf = nullptr;
+			clod_bytes = 0;
+			size_t grow = bytes + bytes / 4;
+			HIP_CHECK(hipMalloc(&clod_buf, grow));
+			clod_bytes = grow;
+		}
+		return clod_buf;
 	}
 
 	void ensure_aux_streams()
@@ -1591,6 +1610,110 @@ extern "C" int meshopt_hip_simplify_batch(unsigned int* destination, unsigned in
 		}
 		if (I)
 			HIP_CHECK(hipMemcpyAsync(destination, d + off_dst, I * 4, hipMemcpyDeviceToHost, s));
+		HIP_CHECK(hipStreamSynchronize(s));
+		t_last_error.clear();
+		return 0;
+	}
+	catch (const std::exception& e)
+	{
+		t_last_error = e.what();
+		return -1;
+	}
+}
+
+extern "C" int meshopt_hip_clod_level_device(unsigned int* destination, unsigned int* result_counts, float* result_errors, const unsigned int* indices,
+                                             const unsigned int* index_offsets, size_t group_count, const float* vertex_positions,
+                                             size_t vertex_count, size_t vertex_positions_stride, const unsigned int* remap,
+                                             const unsigned char* base_locks, unsigned char* locks, const unsigned int* target_index_counts,
+                                             float target_error, unsigned int options)
+{
+	try
+	{
+		if (vertex_positions_stride < 12 || vertex_positions_stride > 256 || vertex_positions_stride % 4)
+			throw std::invalid_argument("meshopt_hip_clod_level_device: invalid vertex stride");
+		if (!locks || !remap)
+			throw std::invalid_argument("meshopt_hip_clod_level_device: remap and locks are required");
+		if (vertex_count >= 0xffffffffu)
+			throw std::invalid_argument("meshopt_hip_clod_level_device: too many vertices");
+		Context& ctx = context();
+		std::lock_guard<std::mutex> guard(ctx.mutex);
+		ctx.ensure_init();
+		hipStream_t s = ctx.stream;
+		const size_t G = group_count;
+
+		// the group offsets decide every buffer size and bound every probe loop: checked on the host
+		std::vector<uint32_t> io(G + 1);
+		HIP_CHECK(hipMemcpyAsync(io.data(), index_offsets, (G + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+		HIP_CHECK(hipStreamSynchronize(s));
+		if (io[0] != 0)
+			throw std::invalid_argument("meshopt_hip_clod_level_device: index_offsets[0] must be 0");
+		for (size_t g = 0; g < G; ++g)
+			if (io[g + 1] < io[g] || (io[g + 1] - io[g]) % 3)
+				throw std::invalid_argument("meshopt_hip_clod_level_device: invalid group index counts");
+		const size_t I = io[G];
+
+		// carve the level's buffers (256-byte aligned)
+		size_t used = 0;
+		auto take = [&](size_t bytes) {
+			size_t off = (used + 255) & ~size_t(255);
+			used = off + bytes;
+			return off;
+		};
+		const size_t V = vertex_count;
+		const size_t o_first = take(4 * (V + 1)), o_boundary = take(V + 4), o_key = take(8 * I + 4), o_sfirst = take(8 * I + 4);
+		const size_t o_local = take(8 * I + 4), o_unique = take(4 * (G + 2)), o_order = take(4 * I + 4), o_pidx = take(4 * I + 4);
+		const size_t o_ppos = take(12 * I + 4), o_plock = take(I + 4), o_pres = take(4 * I + 4);
+		const size_t o_temp = take(4 * (scan_temp_words(G + 1) + 64)), o_flag = take(16);
+		char* base = static_cast<char*>(ctx.ensure_clod(used + 256));
+
+		ClodBuffers b;
+		b.indices = indices;
+		b.index_offsets = index_offsets;
+		b.remap = remap;
+		b.group_count = (uint32_t)G;
+		b.vertex_count = (uint32_t)V;
+		b.first_group = reinterpret_cast<uint32_t*>(base + o_first);
+		b.boundary = reinterpret_cast<uint8_t*>(base + o_boundary);
+		b.slot_key = reinterpret_cast<uint32_t*>(base + o_key);
+		b.slot_first = reinterpret_cast<uint32_t*>(base + o_sfirst);
+		b.slot_local = reinterpret_cast<uint32_t*>(base + o_local);
+		b.unique = reinterpret_cast<uint32_t*>(base + o_unique);
+		b.order = reinterpret_cast<uint32_t*>(base + o_order);
+		b.packed_indices = reinterpret_cast<uint32_t*>(base + o_pidx);
+		b.packed_positions = reinterpret_cast<float*>(base + o_ppos);
+		b.packed_locks = reinterpret_cast<uint8_t*>(base + o_plock);
+		b.packed_result = reinterpret_cast<uint32_t*>(base + o_pres);
+		uint32_t* temp = reinterpret_cast<uint32_t*>(base + o_temp);
+		uint32_t* flag = reinterpret_cast<uint32_t*>(base + o_flag);
+
+		// every index below vertex_count before anything reads through it
+		HIP_CHECK(hipMemsetAsync(flag, 0, sizeof(uint32_t), s));
+		if (I)
+		{
+			launch_check_indices(s, indices, I, V, flag);
+			uint32_t bad = 0;
+			HIP_CHECK(hipMemcpyAsync(&bad, flag, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+			HIP_CHECK(hipStreamSynchronize(s));
+			if (bad)
+				throw std::invalid_argument("meshopt_hip_clod_level_device: index buffer references a vertex >= vertex_count");
+		}
+
+		launch_clod_locks(s, b, base_locks, locks);
+		if (I)
+		{
+			HIP_CHECK(hipMemsetAsync(b.slot_key, 0, 8 * I, s));
+			HIP_CHECK(hipMemsetAsync(b.slot_first, 0xff, 8 * I, s));
+		}
+		HIP_CHECK(hipMemsetAsync(b.unique, 0, 4 * (G + 1), s));
+		launch_clod_hash(s, b);
+		scan_exclusive(s, b.unique, b.unique, G, nullptr, b.unique + G, temp);
+		launch_clod_pack(s, b, vertex_positions, vertex_positions_stride / sizeof(float), locks);
+
+		// the packed groups are the meshes of one batched call (Sparse is what the packing did)
+		simplify_batch_on_device(ctx, b.packed_result, result_counts, result_errors, b.packed_indices, index_offsets, b.packed_positions, b.unique,
+		                         b.packed_locks, I, 12, target_index_counts, G, target_error, options & ~(unsigned int)kOptSparse, nullptr, nullptr,
+		                         nullptr);
+		launch_clod_unpack(s, b, result_counts, destination);
 		HIP_CHECK(hipStreamSynchronize(s));
 		t_last_error.clear();
 		return 0;

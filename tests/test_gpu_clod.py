@@ -1,0 +1,98 @@
+"""GPU parity of one clusterlod level on the device (meshopt_hip_clod_level_device): the level's
+boundary locks and every group's simplification, against the oracle's Sparse call per group on the
+shared vertex buffer (the call demo/clusterlod.h:442-451 makes) with the locks lockBoundary
+(demo/clusterlod.h:372-419) computes."""
+import sys
+
+import numpy as np
+import pytest
+
+from meshes import same_bits
+from oracle import oracle as O
+import meshoptimizer_amd as M
+from meshoptimizer_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+FLT_MAX = float(np.finfo(np.float32).max)
+
+
+def announce(what):
+    print(f"[case] {what}", flush=True)
+    sys.stdout.flush()
+
+
+def level_case(n, tiles, seed):
+    """A seamed grid's positions, its triangles split into tiles x tiles spatial groups (shuffled
+    group order), the position remap, random caller lock/protect bits, and the expected locks."""
+    rng = np.random.default_rng(seed)
+    m = W.seamed_grid(n, seams=4, seed=seed)
+    v = np.ascontiguousarray(m.vertices[:, :3])
+    tri = m.indices.reshape(-1, 3)
+    c = v[tri].mean(axis=1)
+    g = np.minimum((c[:, 1] * tiles).astype(int), tiles - 1) * tiles + np.minimum((c[:, 0] * tiles).astype(int), tiles - 1)
+    ids = rng.permutation(tiles * tiles)
+    groups = [tri[g == k].reshape(-1).astype(np.uint32) for k in ids if (g == k).any()]
+    _, first, inv = np.unique(v, axis=0, return_index=True, return_inverse=True)
+    remap = first[inv.reshape(-1)].astype(np.uint32)
+    base = ((rng.random(v.shape[0]) < 0.05) * M.SimplifyVertex_Lock + (rng.random(v.shape[0]) < 0.1) * M.SimplifyVertex_Protect).astype(np.uint8)
+    uses = np.zeros(v.shape[0], np.int32)
+    for grp in groups:
+        uses[np.unique(remap[grp])] += 1
+    locks = ((uses >= 2)[remap] * M.SimplifyVertex_Lock).astype(np.uint8) | base
+    return v, groups, remap, base, locks
+
+
+@pytest.mark.parametrize("n,tiles,seed,opts", [(48, 3, 1, M.SimplifyErrorAbsolute | M.SimplifyPermissive),
+                                               (64, 4, 2, M.SimplifyErrorAbsolute | M.SimplifyPermissive),
+                                               (96, 6, 3, M.SimplifyErrorAbsolute | M.SimplifyPermissive | M.SimplifySparse),
+                                               (64, 5, 4, M.SimplifyErrorAbsolute)])
+def test_clod_level(n, tiles, seed, opts):
+    from devmem import DeviceArray
+    v, groups, remap, base, locks_expected = level_case(n, tiles, seed)
+    io = np.zeros(len(groups) + 1, np.uint32)
+    io[1:] = np.cumsum([grp.size for grp in groups])
+    idx = np.concatenate(groups).astype(np.uint32)
+    targets = np.array([grp.size // 3 // 2 * 3 for grp in groups], np.uint32)
+    G = len(groups)
+    announce(f"clod level grid {n} groups {G} seed {seed} opts {opts}")
+    with DeviceArray(idx) as d_idx, DeviceArray(io) as d_io, DeviceArray(v) as d_v, DeviceArray(remap) as d_remap, \
+            DeviceArray(base) as d_base, DeviceArray(nbytes=v.shape[0]) as d_locks, DeviceArray(targets) as d_tg, \
+            DeviceArray(nbytes=idx.nbytes) as d_dst, DeviceArray(nbytes=4 * G) as d_cnt, DeviceArray(nbytes=4 * G) as d_err:
+        M.clod_level_device(d_dst.ptr, d_cnt.ptr, d_err.ptr, d_idx.ptr, d_io.ptr, G, d_v.ptr, v.shape[0], 12, d_remap.ptr, d_base.ptr,
+                            d_locks.ptr, d_tg.ptr, FLT_MAX, opts)
+        locks = d_locks.to_host(np.uint8, v.shape[0])
+        counts = d_cnt.to_host(np.uint32, G)
+        errors = d_err.to_host(np.float32, G)
+        dst = d_dst.to_host(np.uint32, idx.size)
+    assert np.array_equal(locks, locks_expected), f"{int((locks != locks_expected).sum())} lock bytes differ"
+    moved = 0
+    for p, grp in enumerate(groups):
+        ro, eo = O.simplify(grp, v, int(targets[p]), FLT_MAX, opts | M.SimplifySparse, attributes=np.zeros((v.shape[0], 0), np.float32),
+                            weights=np.zeros(0, np.float32), lock=locks_expected)
+        rg = dst[io[p]:io[p] + counts[p]]
+        assert rg.size == ro.size and np.array_equal(rg, ro), f"group {p}: indices differ from the Sparse call"
+        assert same_bits(errors[p], eo), f"group {p}: result_error {errors[p]!r} != {eo!r}"
+        moved += rg.size < grp.size
+    assert moved > 0
+
+
+def test_clod_level_rejects_bad_input():
+    """An index beyond the vertex buffer and inconsistent group offsets are reported as errors."""
+    from devmem import DeviceArray
+    v, groups, remap, base, _ = level_case(16, 2, 5)
+    idx = np.concatenate(groups).astype(np.uint32)
+    io = np.array([0, idx.size], np.uint32)
+    bad = idx.copy()
+    bad[4] = v.shape[0] + 2
+    tg = np.array([0], np.uint32)
+    announce("clod level bad input")
+    with DeviceArray(bad) as d_bad, DeviceArray(np.array([0, 7], np.uint32)) as d_io_bad, DeviceArray(io) as d_io, DeviceArray(v) as d_v, \
+            DeviceArray(remap) as d_remap, DeviceArray(nbytes=v.shape[0]) as d_locks, DeviceArray(tg) as d_tg, \
+            DeviceArray(nbytes=idx.nbytes) as d_dst, DeviceArray(nbytes=4) as d_cnt:
+        with pytest.raises(RuntimeError, match="vertex_count"):
+            M.clod_level_device(d_dst.ptr, d_cnt.ptr, None, d_bad.ptr, d_io.ptr, 1, d_v.ptr, v.shape[0], 12, d_remap.ptr, None, d_locks.ptr,
+                                d_tg.ptr, FLT_MAX, 0)
+        with pytest.raises(RuntimeError, match="index counts"):
+            M.clod_level_device(d_dst.ptr, d_cnt.ptr, None, d_bad.ptr, d_io_bad.ptr, 1, d_v.ptr, v.shape[0], 12, d_remap.ptr, None,
+                                d_locks.ptr, d_tg.ptr, FLT_MAX, 0)

@@ -269,3 +269,39 @@ def test_update_closed_form():
     flat[:, 2] = flat[:, 0]
     r, e, b = _update_run(m.indices, flat, 0, m.index_count, 1e-2, 0, None, None, impl="oracle")
     assert np.allclose(b, flat, atol=1e-5)
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_boundary_locks_rule(seed):
+    """lockBoundary (demo/clusterlod.h:372-419) marks, group by group, every remapped vertex already
+    seen by an earlier group (bit 7 as 'seen', bit 0 as the lock); the device computes the same set
+    as 'used by two or more groups' (the lowest group that uses it and any other).  Both forms on
+    random group sets, with the caller's bits carried through as the reference carries them."""
+    rng = np.random.default_rng(seed)
+    V = int(rng.integers(20, 200))
+    remap = np.arange(V)
+    dup = rng.random(V) < 0.2
+    remap[dup] = rng.integers(0, V, size=int(dup.sum()))
+    remap = np.minimum(remap, np.arange(V))  # a remap points at a lower (or the same) index
+    remap = remap[remap]                     # and at a primary
+    groups = [rng.integers(0, V, size=3 * int(rng.integers(1, 30))) for _ in range(int(rng.integers(1, 12)))]
+    base = rng.integers(0, 8, size=V).astype(np.uint8)
+
+    # the reference's loop, step by step
+    locks = (rng.integers(0, 256, size=V).astype(np.uint8) & 2) | 0  # earlier levels' protect bits
+    prior = locks.copy()
+    locks &= ~np.uint8(1 | 128)
+    for grp in groups:
+        for v in grp:
+            r = remap[v]
+            locks[r] |= locks[r] >> 7
+        for v in grp:
+            locks[remap[v]] |= 128
+    ref = np.array([(locks[remap[i]] & 1) | (prior[i] & 2) | base[i] for i in range(V)], np.uint8)
+
+    # the device's rule
+    uses = np.zeros(V, np.int32)
+    for grp in groups:
+        uses[np.unique(remap[grp])] += 1
+    dev = ((uses >= 2)[remap].astype(np.uint8)) | (prior & 2) | base
+    assert np.array_equal(ref, dev)
