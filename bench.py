@@ -1,7 +1,7 @@
 """Benchmark of the MI355X simplifier: input triangles per second for one plain
 meshopt_simplify call on a device-resident mesh, next to the reference CPU library.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload icosphere8|grid256|seamed1024|patches4096|serving|sloppy|update]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload icosphere8|grid256|seamed1024|patches4096|serving|sloppy|update|clod]
 
 A step is one whole simplify call (setup, every pass, compaction of the result) on inputs
 already resident in HBM; the timed region is bracketed by device synchronisation.  Prints one
@@ -507,6 +507,100 @@ def bench_update(args):
     print(json.dumps(out), flush=True)
 
 
+def clod_level_workload(n=1024, tiles=64):
+    """One clusterlod level: a displaced n x n grid's triangles split into tiles x tiles spatial
+    groups (about 512 triangles each at the defaults), the position remap, no caller locks."""
+    from meshoptimizer_amd import workloads as W
+    m = W.grid(n)
+    v = np.ascontiguousarray(m.vertices[:, :3])
+    tri = m.indices.reshape(-1, 3)
+    c = v[tri].mean(axis=1)
+    g = np.minimum((c[:, 1] * tiles).astype(np.int64), tiles - 1) * tiles + np.minimum((c[:, 0] * tiles).astype(np.int64), tiles - 1)
+    order = np.argsort(g, kind="stable")
+    counts = np.bincount(g, minlength=tiles * tiles)
+    io = np.zeros(tiles * tiles + 1, np.uint32)
+    io[1:] = np.cumsum(counts) * 3
+    idx = np.ascontiguousarray(tri[order].reshape(-1)).astype(np.uint32)
+    remap = np.arange(v.shape[0], dtype=np.uint32)  # a grid has no repeated positions
+    targets = (np.diff(io) // 3 // 2 * 3).astype(np.uint32)
+    return m, v, idx, io, remap, targets
+
+
+def bench_clod(args):
+    """One clusterlod level on the device (meshopt_hip_clod_level_device): boundary locks, the
+    packing of 4096 groups of about 512 triangles cut from a 1024 x 1024 grid, their batched
+    simplification to half (ErrorAbsolute | Permissive, no error limit, as clodBuild calls it) and
+    the mapping back; inputs resident in HBM.  The reference runs the same level as clodBuild does:
+    its lockBoundary, then one meshopt_simplifyWithAttributes(Sparse) per group."""
+    import torch
+    import meshoptimizer_amd as M
+
+    m, v, idx, io, remap, targets = clod_level_workload()
+    G = targets.size
+    flt_max = float(np.finfo(np.float32).max)
+    opts = M.SimplifyErrorAbsolute | M.SimplifyPermissive
+    dev = torch.device("cuda:0")
+    t = lambda a: torch.from_numpy(a.view(np.int32) if a.dtype == np.uint32 else a).to(dev)
+    d_idx, d_io, d_remap, d_tg, d_v = t(idx), t(io), t(remap), t(targets), torch.from_numpy(v).to(dev)
+    d_locks = torch.empty(v.shape[0], dtype=torch.uint8, device=dev)
+    d_dst = torch.empty(idx.size, dtype=torch.int32, device=dev)
+    d_cnt = torch.empty(G, dtype=torch.int32, device=dev)
+    d_err = torch.empty(G, dtype=torch.float32, device=dev)
+
+    def step():
+        M.clod_level_device(d_dst.data_ptr(), d_cnt.data_ptr(), d_err.data_ptr(), d_idx.data_ptr(), d_io.data_ptr(), G, d_v.data_ptr(),
+                            v.shape[0], 12, d_remap.data_ptr(), None, d_locks.data_ptr(), d_tg.data_ptr(), flt_max, opts)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    times = []
+    for _ in range(args.steps):
+        ts = time.perf_counter()
+        step()
+        times.append(time.perf_counter() - ts)
+    mean = sum(times) / len(times)
+    out = {
+        "metric": "input Mtri/s simplified (bit-exact vs CPU), one clusterlod level on the device",
+        "value": round(m.triangle_count / mean / 1e6, 3),
+        "unit": "Mtri/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(mean * 1e3, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded generator)",
+        "config": {"workload": f"clusterlod level: grid 1024x1024 ({m.triangle_count:,} T) in {G} spatial groups, boundary locks, "
+                               "each group to 50%, ErrorAbsolute | Permissive, no error limit", "groups": int(G)},
+        "p50_ms": round(statistics.median(times) * 1e3, 4),
+        "p99_ms": round(float(np.percentile(np.array(times) * 1e3, 99)), 4),
+    }
+    if not args.no_cpu:
+        from oracle import oracle as O
+        impl = "ref" if O.have_ref() else "oracle"
+        counts = d_cnt.cpu().numpy().astype(np.uint32)
+        dst = d_dst.cpu().numpy().astype(np.uint32)
+        errs = d_err.cpu().numpy()
+        uses = np.zeros(v.shape[0], np.int32)
+        for p in range(G):
+            uses[np.unique(remap[idx[io[p]:io[p + 1]]])] += 1
+        locks = (uses >= 2)[remap].astype(np.uint8)
+        exact = bool(np.array_equal(d_locks.cpu().numpy(), locks))
+        # a bounded, evenly spaced sample of groups, timed and checked
+        sample = list(range(0, G, max(1, G // 256)))
+        empty_a, empty_w = np.zeros((v.shape[0], 0), np.float32), np.zeros(0, np.float32)
+        cpu_t, tris = 0.0, 0
+        for p in sample:
+            grp = idx[io[p]:io[p + 1]]
+            ts = time.perf_counter()
+            ro, eo = O.simplify(grp, v, int(targets[p]), flt_max, opts | M.SimplifySparse, attributes=empty_a, weights=empty_w,
+                                lock=locks, impl=impl)
+            cpu_t += time.perf_counter() - ts
+            tris += grp.size // 3
+            exact &= bool(np.array_equal(dst[io[p]:io[p] + counts[p]], ro) and np.float32(eo).tobytes() == np.float32(errs[p]).tobytes())
+        out["bit_exact"] = exact
+        out["cpu_baseline"] = {"value": round(tris / cpu_t / 1e6, 3), "unit": "Mtri/s", "cores": 1,
+                               "kind": "reference" if impl == "ref" else "port",
+                               "sample": f"{len(sample)} of {G} groups, one Sparse call each on one thread ({cpu_t * 1e3:.0f} ms); "
+                                         "every sampled group checked bit-exact"}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -537,6 +631,8 @@ def main():
         return bench_sloppy(args)
     if args.workload == "update":
         return bench_update(args)
+    if args.workload == "clod":
+        return bench_clod(args)
 
     import torch
     import meshoptimizer_amd as M
