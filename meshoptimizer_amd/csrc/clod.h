@@ -8,6 +8,10 @@
 namespace mhip
 {
 
+// groups of at most kClodLdsSlots / 2 indices (every group the batched kernel's classes take) are
+// packed by one kernel in LDS; larger groups use hash slices in global memory
+constexpr uint32_t kClodLdsSlots = 4608;
+
 // device buffers of one level (carved by the host); I = indices of all groups, G = groups,
 // V = vertices of the shared buffer
 struct ClodBuffers
@@ -18,11 +22,12 @@ struct ClodBuffers
 	uint32_t group_count;
 	uint32_t vertex_count;
 
-	uint32_t* first_group;         // V: lowest group that uses each remapped vertex
-	uint8_t* boundary;             // V: 1 when a second group uses the remapped vertex
-	uint32_t* slot_key;            // 2I: per-group hash slices (at 2 * index_offsets[g]): vertex + 1, 0 = empty
+	uint32_t* first_group;         // V: lowest group that uses each remapped vertex (kNone: none)
+	uint32_t* last_group;          // V: highest group that uses it (0: none)
+	uint32_t* slot_key;            // 2I: hash slices of the large groups (at 2 * index_offsets[g]): vertex + 1, 0 = empty
 	uint32_t* slot_first;          // 2I: first position of the vertex in its group
 	uint32_t* slot_local;          // 2I: its packed id
+	uint32_t* staged;              // I: group g's first-use vertices in order, at index_offsets[g]
 	uint32_t* unique;              // G + 1: vertices per group, then their exclusive scan (packed vertex offsets)
 	uint32_t* order;               // I: packed vertex -> shared vertex id
 	uint32_t* packed_indices;      // I
@@ -31,12 +36,15 @@ struct ClodBuffers
 	uint32_t* packed_result;       // I
 };
 
-// locks[i] = (a second group uses remap[i] ? Lock : 0) | base_locks[i] (base_locks may be null)
+// per group: first-use ranking (packed_indices, staged, unique counts) and the first/last group of
+// every used remapped vertex; needs first_group = kNone, last_group = 0 and, when large_groups,
+// slot_key = 0 and slot_first = kNone beforehand
+void launch_clod_group(hipStream_t s, const ClodBuffers& b, bool large_groups);
+// locks[i] = (two or more groups use remap[i] ? Lock : 0) | base_locks[i] (base_locks may be null)
 // (lockBoundary, demo/clusterlod.h:372-419, with the caller's protect and lock bits as base_locks)
 void launch_clod_locks(hipStream_t s, const ClodBuffers& b, const uint8_t* base_locks, uint8_t* locks);
-// per-group first-use hashing and counts; then (after the host scans `unique`) the packing
-void launch_clod_hash(hipStream_t s, const ClodBuffers& b);
-void launch_clod_pack(hipStream_t s, const ClodBuffers& b, const float* positions, size_t stride_floats, const uint8_t* locks);
+// after the host scans `unique`: the packed vertex list, positions and locks of every group
+void launch_clod_place(hipStream_t s, const ClodBuffers& b, const float* positions, size_t stride_floats, const uint8_t* locks);
 // destination[index_offsets[g] + k] = order[unique[g] + packed_result[index_offsets[g] + k]], k < counts[g]
 void launch_clod_unpack(hipStream_t s, const ClodBuffers& b, const uint32_t* counts, uint32_t* destination);
 

@@ -15,54 +15,12 @@ static inline unsigned int clod_grid(size_t n)
 }
 
 // ------------------------------------------------------------------------------------------
-// lockBoundary (demo/clusterlod.h:372-419): a remapped vertex is locked when a group other than
-// the lowest one that uses it uses it too, i.e. when two or more groups use it
-
-__global__ void __launch_bounds__(kClodThreads) k_clod_first(ClodBuffers b)
-{
-	const uint32_t g = blockIdx.x;
-	for (uint32_t j = b.index_offsets[g] + threadIdx.x, e = b.index_offsets[g + 1]; j < e; j += kClodThreads)
-		atomicMin(&b.first_group[b.remap[b.indices[j]]], g);
-}
-
-__global__ void __launch_bounds__(kClodThreads) k_clod_boundary(ClodBuffers b)
-{
-	const uint32_t g = blockIdx.x;
-	for (uint32_t j = b.index_offsets[g] + threadIdx.x, e = b.index_offsets[g + 1]; j < e; j += kClodThreads)
-	{
-		const uint32_t r = b.remap[b.indices[j]];
-		if (b.first_group[r] != g)
-			b.boundary[r] = 1;
-	}
-}
-
-__global__ void k_clod_locks(const uint8_t* boundary, const uint32_t* remap, const uint8_t* base, size_t n, uint8_t* locks)
-{
-	size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-	if (i < n)
-		locks[i] = (boundary[remap[i]] ? kVertexLock : 0) | (base ? base[i] : 0);
-}
-
-void launch_clod_locks(hipStream_t s, const ClodBuffers& b, const uint8_t* base_locks, uint8_t* locks)
-{
-	const size_t n = b.vertex_count;
-	if (!n)
-		return;
-	HIP_CHECK(hipMemsetAsync(b.first_group, 0xff, n * sizeof(uint32_t), s));
-	HIP_CHECK(hipMemsetAsync(b.boundary, 0, n, s));
-	if (b.group_count)
-	{
-		k_clod_first<<<b.group_count, kClodThreads, 0, s>>>(b);
-		k_clod_boundary<<<b.group_count, kClodThreads, 0, s>>>(b);
-	}
-	k_clod_locks<<<clod_grid(n), kClodThreads, 0, s>>>(b.boundary, b.remap, base_locks, n, locks);
-}
-
-// ------------------------------------------------------------------------------------------
-// first-use packing: group g hashes its vertices into its own slice of 2 * (its index count)
-// slots (at 2 * index_offsets[g]), keeping the first position of each; a vertex's packed id is
-// the number of first uses before its own, so the packed order is the order Sparse gives
-// (src/simplifier.cpp:151-199)
+// first-use packing: group g hashes its vertices into 2 * (its index count) slots, keeping the
+// first position of each; a vertex's packed id is the number of first uses before its own, so the
+// packed order is the order Sparse gives (src/simplifier.cpp:151-199).  Each first use also
+// records its group in first_group / last_group (lowest and highest group using the remapped
+// vertex), which gives lockBoundary's rule (demo/clusterlod.h:372-419) without another pass: a
+// remapped vertex is locked when two or more groups use it, i.e. when the two differ.
 
 __device__ __forceinline__ uint32_t clod_slot0(uint32_t v, uint32_t slots)
 {
@@ -79,29 +37,20 @@ __device__ __forceinline__ uint32_t clod_find(const uint32_t* key, uint32_t v, u
 	return slot;
 }
 
-__global__ void __launch_bounds__(kClodThreads) k_clod_insert(ClodBuffers b)
+// insert v at its first position p: at most (index count) distinct keys in twice as many slots,
+// so a free or matching slot exists within one sweep of the slice
+__device__ __forceinline__ void clod_insert(uint32_t* key, uint32_t* first, uint32_t v, uint32_t p, uint32_t slots)
 {
-	const uint32_t g = blockIdx.x;
-	const uint32_t begin = b.index_offsets[g], end = b.index_offsets[g + 1];
-	const uint32_t slots = 2 * (end - begin);
-	uint32_t* key = b.slot_key + 2 * (size_t)begin;
-	uint32_t* first = b.slot_first + 2 * (size_t)begin;
-	for (uint32_t j = begin + threadIdx.x; j < end; j += kClodThreads)
+	uint32_t slot = clod_slot0(v, slots);
+	for (uint32_t probe = 0; probe < slots; ++probe)
 	{
-		const uint32_t v = b.indices[j];
-		uint32_t slot = clod_slot0(v, slots);
-		// at most end - begin distinct keys in twice as many slots: a free or matching slot exists
-		// within one sweep of the slice
-		for (uint32_t probe = 0; probe < slots; ++probe)
+		const uint32_t k = atomicCAS(&key[slot], 0u, v + 1);
+		if (k == 0 || k == v + 1)
 		{
-			const uint32_t k = atomicCAS(&key[slot], 0u, v + 1);
-			if (k == 0 || k == v + 1)
-			{
-				atomicMin(&first[slot], j - begin);
-				break;
-			}
-			slot = slot + 1 == slots ? 0 : slot + 1;
+			atomicMin(&first[slot], p);
+			return;
 		}
+		slot = slot + 1 == slots ? 0 : slot + 1;
 	}
 }
 
@@ -129,43 +78,91 @@ __device__ __forceinline__ uint32_t clod_block_scan(uint32_t v, uint32_t* lds, u
 	return before + incl - v;
 }
 
-// vertices per group (its first uses)
-__global__ void __launch_bounds__(kClodThreads) k_clod_count(ClodBuffers b)
+__device__ __forceinline__ void clod_first_use(const ClodBuffers& b, uint32_t g, uint32_t at, uint32_t v)
 {
+	b.staged[at] = v;
+	const uint32_t r = b.remap[v];
+	atomicMin(&b.first_group[r], g);
+	atomicMax(&b.last_group[r], g);
+}
+
+static constexpr uint32_t kRanked = 0x80000000u;
+
+// groups of at most kClodLdsSlots slots: the whole packing of the group in LDS.  The ordered pass
+// replaces a ranked vertex's first position by kRanked | id (positions stay below 2^31), so later
+// uses read the id from the same slot and no position compares equal to a ranked entry.
+__global__ void __launch_bounds__(kClodThreads) k_clod_group(ClodBuffers b)
+{
+	__shared__ uint32_t lkey[kClodLdsSlots], lfirst[kClodLdsSlots];
 	__shared__ uint32_t lds[kClodThreads / 64];
 	const uint32_t g = blockIdx.x;
 	const uint32_t begin = b.index_offsets[g], end = b.index_offsets[g + 1];
 	const uint32_t slots = 2 * (end - begin);
-	const uint32_t* key = b.slot_key + 2 * (size_t)begin;
-	const uint32_t* first = b.slot_first + 2 * (size_t)begin;
-	uint32_t n = 0;
-	for (uint32_t j = begin + threadIdx.x; j < end; j += kClodThreads)
-		n += first[clod_find(key, b.indices[j], slots)] == j - begin;
-	uint32_t total;
-	clod_block_scan(n, lds, total);
-	if (threadIdx.x == 0)
-		b.unique[g] = total;
-}
-
-void launch_clod_hash(hipStream_t s, const ClodBuffers& b)
-{
-	if (!b.group_count)
+	if (slots > kClodLdsSlots)
 		return;
-	k_clod_insert<<<b.group_count, kClodThreads, 0, s>>>(b);
-	k_clod_count<<<b.group_count, kClodThreads, 0, s>>>(b);
+	for (uint32_t q = threadIdx.x; q < slots; q += kClodThreads)
+	{
+		lkey[q] = 0;
+		lfirst[q] = kNone;
+	}
+	__syncthreads();
+	for (uint32_t j = begin + threadIdx.x; j < end; j += kClodThreads)
+		clod_insert(lkey, lfirst, b.indices[j], j - begin, slots);
+	__syncthreads();
+	uint32_t carry = 0;
+	for (uint32_t j0 = begin; j0 < end; j0 += kClodThreads)
+	{
+		const uint32_t j = j0 + threadIdx.x;
+		uint32_t v = 0, slot = 0;
+		bool is_first = false;
+		if (j < end)
+		{
+			v = b.indices[j];
+			slot = clod_find(lkey, v, slots);
+			is_first = lfirst[slot] == j - begin;
+		}
+		uint32_t total;
+		const uint32_t id = carry + clod_block_scan(is_first ? 1u : 0u, lds, total);
+		if (is_first)
+		{
+			lfirst[slot] = kRanked | id;
+			clod_first_use(b, g, begin + id, v);
+		}
+		__syncthreads();
+		if (j < end)
+			b.packed_indices[j] = lfirst[slot] & ~kRanked;
+		carry += total;
+	}
+	if (threadIdx.x == 0)
+		b.unique[g] = carry;
 }
 
-// packed ids of the first uses, in order; the packed vertex list, positions and locks
-__global__ void __launch_bounds__(kClodThreads) k_clod_first_ids(ClodBuffers b, const float* positions, size_t stride, const uint8_t* locks)
+// larger groups (beyond the batched kernel's caps; they take the plain path) hash in their global
+// slices at 2 * index_offsets[g]: insert, ordered ranking, ids per index as three launches
+__global__ void __launch_bounds__(kClodThreads) k_clod_insert(ClodBuffers b)
+{
+	const uint32_t g = blockIdx.x;
+	const uint32_t begin = b.index_offsets[g], end = b.index_offsets[g + 1];
+	const uint32_t slots = 2 * (end - begin);
+	if (slots <= kClodLdsSlots)
+		return;
+	uint32_t* key = b.slot_key + 2 * (size_t)begin;
+	uint32_t* first = b.slot_first + 2 * (size_t)begin;
+	for (uint32_t j = begin + threadIdx.x; j < end; j += kClodThreads)
+		clod_insert(key, first, b.indices[j], j - begin, slots);
+}
+
+__global__ void __launch_bounds__(kClodThreads) k_clod_rank(ClodBuffers b)
 {
 	__shared__ uint32_t lds[kClodThreads / 64];
 	const uint32_t g = blockIdx.x;
 	const uint32_t begin = b.index_offsets[g], end = b.index_offsets[g + 1];
 	const uint32_t slots = 2 * (end - begin);
+	if (slots <= kClodLdsSlots)
+		return;
 	const uint32_t* key = b.slot_key + 2 * (size_t)begin;
 	const uint32_t* first = b.slot_first + 2 * (size_t)begin;
 	uint32_t* local = b.slot_local + 2 * (size_t)begin;
-	const uint32_t vbase = b.unique[g];
 	uint32_t carry = 0;
 	for (uint32_t j0 = begin; j0 < end; j0 += kClodThreads)
 	{
@@ -183,16 +180,12 @@ __global__ void __launch_bounds__(kClodThreads) k_clod_first_ids(ClodBuffers b, 
 		if (is_first)
 		{
 			local[slot] = id;
-			const uint32_t at = vbase + id;
-			b.order[at] = v;
-			const float* p = positions + (size_t)v * stride;
-			b.packed_positions[3 * (size_t)at + 0] = p[0];
-			b.packed_positions[3 * (size_t)at + 1] = p[1];
-			b.packed_positions[3 * (size_t)at + 2] = p[2];
-			b.packed_locks[at] = locks ? locks[v] : 0;
+			clod_first_use(b, g, begin + id, v);
 		}
 		carry += total;
 	}
+	if (threadIdx.x == 0)
+		b.unique[g] = carry;
 }
 
 __global__ void __launch_bounds__(kClodThreads) k_clod_local(ClodBuffers b)
@@ -200,18 +193,65 @@ __global__ void __launch_bounds__(kClodThreads) k_clod_local(ClodBuffers b)
 	const uint32_t g = blockIdx.x;
 	const uint32_t begin = b.index_offsets[g], end = b.index_offsets[g + 1];
 	const uint32_t slots = 2 * (end - begin);
+	if (slots <= kClodLdsSlots)
+		return;
 	const uint32_t* key = b.slot_key + 2 * (size_t)begin;
 	const uint32_t* local = b.slot_local + 2 * (size_t)begin;
 	for (uint32_t j = begin + threadIdx.x; j < end; j += kClodThreads)
 		b.packed_indices[j] = local[clod_find(key, b.indices[j], slots)];
 }
 
-void launch_clod_pack(hipStream_t s, const ClodBuffers& b, const float* positions, size_t stride_floats, const uint8_t* locks)
+void launch_clod_group(hipStream_t s, const ClodBuffers& b, bool large_groups)
 {
 	if (!b.group_count)
 		return;
-	k_clod_first_ids<<<b.group_count, kClodThreads, 0, s>>>(b, positions, stride_floats, locks);
-	k_clod_local<<<b.group_count, kClodThreads, 0, s>>>(b);
+	k_clod_group<<<b.group_count, kClodThreads, 0, s>>>(b);
+	if (large_groups)
+	{
+		k_clod_insert<<<b.group_count, kClodThreads, 0, s>>>(b);
+		k_clod_rank<<<b.group_count, kClodThreads, 0, s>>>(b);
+		k_clod_local<<<b.group_count, kClodThreads, 0, s>>>(b);
+	}
+}
+
+__global__ void k_clod_locks(const uint32_t* first_group, const uint32_t* last_group, const uint32_t* remap, const uint8_t* base, size_t n,
+                             uint8_t* locks)
+{
+	size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n)
+	{
+		const uint32_t r = remap[i], f = first_group[r];
+		locks[i] = (f != kNone && f != last_group[r] ? kVertexLock : 0) | (base ? base[i] : 0);
+	}
+}
+
+void launch_clod_locks(hipStream_t s, const ClodBuffers& b, const uint8_t* base_locks, uint8_t* locks)
+{
+	if (b.vertex_count)
+		k_clod_locks<<<clod_grid(b.vertex_count), kClodThreads, 0, s>>>(b.first_group, b.last_group, b.remap, base_locks, b.vertex_count, locks);
+}
+
+// group g's packed vertices at unique[g] (the scanned counts): shared id, position, lock
+__global__ void __launch_bounds__(kClodThreads) k_clod_place(ClodBuffers b, const float* positions, size_t stride, const uint8_t* locks)
+{
+	const uint32_t g = blockIdx.x;
+	const uint32_t begin = b.index_offsets[g], vbase = b.unique[g], n = b.unique[g + 1] - vbase;
+	for (uint32_t k = threadIdx.x; k < n; k += kClodThreads)
+	{
+		const uint32_t v = b.staged[begin + k], at = vbase + k;
+		b.order[at] = v;
+		const float* p = positions + (size_t)v * stride;
+		b.packed_positions[3 * (size_t)at + 0] = p[0];
+		b.packed_positions[3 * (size_t)at + 1] = p[1];
+		b.packed_positions[3 * (size_t)at + 2] = p[2];
+		b.packed_locks[at] = locks[v];
+	}
+}
+
+void launch_clod_place(hipStream_t s, const ClodBuffers& b, const float* positions, size_t stride_floats, const uint8_t* locks)
+{
+	if (b.group_count)
+		k_clod_place<<<b.group_count, kClodThreads, 0, s>>>(b, positions, stride_floats, locks);
 }
 
 __global__ void __launch_bounds__(kClodThreads) k_clod_unpack(ClodBuffers b, const uint32_t* counts, uint32_t* destination)

@@ -1651,6 +1651,9 @@ extern "C" int meshopt_hip_clod_level_device(unsigned int* destination, unsigned
 			if (io[g + 1] < io[g] || (io[g + 1] - io[g]) % 3)
 				throw std::invalid_argument("meshopt_hip_clod_level_device: invalid group index counts");
 		const size_t I = io[G];
+		bool large_groups = false;
+		for (size_t g = 0; g < G; ++g)
+			large_groups |= 2 * (size_t)(io[g + 1] - io[g]) > kClodLdsSlots;
 
 		// carve the level's buffers (256-byte aligned)
 		size_t used = 0;
@@ -1660,8 +1663,8 @@ extern "C" int meshopt_hip_clod_level_device(unsigned int* destination, unsigned
 			return off;
 		};
 		const size_t V = vertex_count;
-		const size_t o_first = take(4 * (V + 1)), o_boundary = take(V + 4), o_key = take(8 * I + 4), o_sfirst = take(8 * I + 4);
-		const size_t o_local = take(8 * I + 4), o_unique = take(4 * (G + 2)), o_order = take(4 * I + 4), o_pidx = take(4 * I + 4);
+		const size_t o_first = take(4 * (V + 1)), o_last = take(4 * (V + 1)), o_key = take(8 * I + 4), o_sfirst = take(8 * I + 4);
+		const size_t o_local = take(8 * I + 4), o_staged = take(4 * I + 4), o_unique = take(4 * (G + 2)), o_order = take(4 * I + 4), o_pidx = take(4 * I + 4);
 		const size_t o_ppos = take(12 * I + 4), o_plock = take(I + 4), o_pres = take(4 * I + 4);
 		const size_t o_temp = take(4 * (scan_temp_words(G + 1) + 64)), o_flag = take(16);
 		char* base = static_cast<char*>(ctx.ensure_clod(used + 256));
@@ -1673,10 +1676,11 @@ extern "C" int meshopt_hip_clod_level_device(unsigned int* destination, unsigned
 		b.group_count = (uint32_t)G;
 		b.vertex_count = (uint32_t)V;
 		b.first_group = reinterpret_cast<uint32_t*>(base + o_first);
-		b.boundary = reinterpret_cast<uint8_t*>(base + o_boundary);
+		b.last_group = reinterpret_cast<uint32_t*>(base + o_last);
 		b.slot_key = reinterpret_cast<uint32_t*>(base + o_key);
 		b.slot_first = reinterpret_cast<uint32_t*>(base + o_sfirst);
 		b.slot_local = reinterpret_cast<uint32_t*>(base + o_local);
+		b.staged = reinterpret_cast<uint32_t*>(base + o_staged);
 		b.unique = reinterpret_cast<uint32_t*>(base + o_unique);
 		b.order = reinterpret_cast<uint32_t*>(base + o_order);
 		b.packed_indices = reinterpret_cast<uint32_t*>(base + o_pidx);
@@ -1698,16 +1702,21 @@ extern "C" int meshopt_hip_clod_level_device(unsigned int* destination, unsigned
 				throw std::invalid_argument("meshopt_hip_clod_level_device: index buffer references a vertex >= vertex_count");
 		}
 
-		launch_clod_locks(s, b, base_locks, locks);
-		if (I)
+		if (V)
+		{
+			HIP_CHECK(hipMemsetAsync(b.first_group, 0xff, 4 * V, s));
+			HIP_CHECK(hipMemsetAsync(b.last_group, 0, 4 * V, s));
+		}
+		if (large_groups)
 		{
 			HIP_CHECK(hipMemsetAsync(b.slot_key, 0, 8 * I, s));
 			HIP_CHECK(hipMemsetAsync(b.slot_first, 0xff, 8 * I, s));
 		}
 		HIP_CHECK(hipMemsetAsync(b.unique, 0, 4 * (G + 1), s));
-		launch_clod_hash(s, b);
+		launch_clod_group(s, b, large_groups);
+		launch_clod_locks(s, b, base_locks, locks);
 		scan_exclusive(s, b.unique, b.unique, G, nullptr, b.unique + G, temp);
-		launch_clod_pack(s, b, vertex_positions, vertex_positions_stride / sizeof(float), locks);
+		launch_clod_place(s, b, vertex_positions, vertex_positions_stride / sizeof(float), locks);
 
 		// the packed groups are the meshes of one batched call (Sparse is what the packing did)
 		simplify_batch_on_device(ctx, b.packed_result, result_counts, result_errors, b.packed_indices, index_offsets, b.packed_positions, b.unique,
