@@ -198,7 +198,7 @@ def bench_patches(args):
         ab = batch_alg_bytes(np.diff(io), np.diff(vo), g_cnt)
         per_launch_s = k_ms * 1e-3 / k_launches
         achieved = ab / per_launch_s / 1e9
-        out["roofline"] = {"bound": "hbm", "kernel": "k_simplify_patches: one LDS-resident workgroup per patch, 3 size classes "
+        out["roofline"] = {"bound": "hbm", "kernel": "k_simplify_patches: one LDS-resident workgroup per patch, LDS size classes "
                                                   "launched side by side, timed with the classify kernel before them",
                            "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                            "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": measured_traffic("patches4096", "k_simplify_patches"),
@@ -535,7 +535,7 @@ def bench_clod(args):
     import torch
     import meshoptimizer_amd as M
 
-    m, v, idx, io, remap, targets = clod_level_workload()
+    m, v, idx, io, remap, targets = clod_level_workload(tiles=args.clod_tiles)
     G = targets.size
     flt_max = float(np.finfo(np.float32).max)
     opts = M.SimplifyErrorAbsolute | M.SimplifyPermissive
@@ -565,8 +565,9 @@ def bench_clod(args):
         "value": round(m.triangle_count / mean / 1e6, 3),
         "unit": "Mtri/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(mean * 1e3, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded generator)",
-        "config": {"workload": f"clusterlod level: grid 1024x1024 ({m.triangle_count:,} T) in {G} spatial groups, boundary locks, "
-                               "each group to 50%, ErrorAbsolute | Permissive, no error limit", "groups": int(G)},
+        "config": {"workload": f"clusterlod level: grid 1024x1024 ({m.triangle_count:,} T) in {G} spatial groups "
+                               f"(about {m.triangle_count // G} T each), boundary locks, each group to 50%, ErrorAbsolute | Permissive, "
+                               "no error limit", "groups": int(G)},
         "p50_ms": round(statistics.median(times) * 1e3, 4),
         "p99_ms": round(float(np.percentile(np.array(times) * 1e3, 99)), 4),
     }
@@ -607,6 +608,9 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="icosphere8")
+    ap.add_argument("--clod-tiles", type=int, default=64,
+                    help="clod workload: the grid is cut into tiles x tiles groups (64: about 512 triangles each; 34: about "
+                         "1810, sixteen clusters of the clusterlod demo's default size)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-sample", type=int, default=256, help="patches run through the plain per-call path")
     ap.add_argument("--requests", type=int, default=64, help="serving: requests in the stream")

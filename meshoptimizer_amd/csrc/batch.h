@@ -6,8 +6,11 @@
 namespace mhip
 {
 
-// LDS size classes of one batched call (launched side by side)
-constexpr int kBatchClasses = 3;
+// LDS size classes of one batched call (launched side by side): 6, 5, 2 and 1 workgroups per CU
+constexpr int kBatchClasses = 4;
+// the class that a mesh overflowing the candidate arrays of class 0 or 1 runs again in (classes
+// 2 and 3 hold the full bound)
+constexpr int kBatchRetryClass = 2;
 
 struct BatchArgs
 {
@@ -15,7 +18,8 @@ struct BatchArgs
 	uint32_t* result_counts;           // per mesh
 	float* result_errors;              // per mesh (may be null)
 	uint32_t* status;                  // per mesh: 0 done, 1 plain path, 2 bad index, 3 pass bound hit (internal error), 4 invalid sizes,
-	                                   // 5 more candidates than the layout holds (run again in the last class)
+	                                   // 5 more candidates than the layout holds (run again in kBatchRetryClass),
+	                                   // 6 waiting for the last class's launch
 	uint32_t* passes;                  // per mesh (may be null)
 	const uint32_t* indices;           // packed local indices
 	const uint32_t* index_offsets;     // mesh_count + 1
@@ -33,7 +37,7 @@ struct BatchArgs
 
 size_t batch_lds_bytes(uint32_t vmax, uint32_t imax, uint32_t cmax);
 // size classes: a mesh belongs to the first class whose vertex and index caps it fits; ccap is the
-// class's candidate capacity (the last class holds the full bound, so it never returns status 5)
+// class's candidate capacity (classes 2 and 3 hold the full bound, so they never return status 5)
 struct BatchClasses
 {
 	uint32_t vcap[kBatchClasses];
@@ -41,17 +45,22 @@ struct BatchClasses
 	uint32_t ccap[kBatchClasses];
 };
 
-// the largest caps per class whose LDS layout lets 6, 5 and 2 workgroups share a CU
-BatchClasses batch_classes();
+// the largest caps per class whose LDS layout lets 6, 5, 2 and 1 workgroups share a CU, given the
+// LDS one workgroup may take (a class that the limit leaves no larger than the one before it is
+// empty: caps 0)
+BatchClasses batch_classes(uint32_t max_lds_per_workgroup);
+// lets k_simplify_patches take `bytes` of dynamic LDS; false when the runtime refuses
+bool batch_allow_lds(size_t bytes);
 
 // per mesh: validate sizes (status 4), pick its class (ids[c * patch_count + slot], counts[c])
 // or mark it for the plain path (status 1); counts must be zero on entry
 void launch_batch_classify(hipStream_t s, const BatchArgs& a, const BatchClasses& classes, uint32_t* ids, uint32_t* counts);
 void launch_batch_order(hipStream_t s, const BatchArgs& a, uint32_t* ids, const uint32_t* counts, uint32_t* temp);
 
-// one workgroup per mesh of a class (ids, *count_dev of them; grid = upper bound), every one with
-// the (vmax, imax, cmax) layout
+// the meshes of a class (ids, *count_dev of them), every one with the (vmax, imax, cmax) layout:
+// workgroup b takes mesh b (grid at least the count; the workgroups past it leave at once) or,
+// with loop, the meshes b, b + grid, ...
 void launch_batch(hipStream_t s, const BatchArgs& a, uint32_t vmax, uint32_t imax, uint32_t cmax, const uint32_t* ids, const uint32_t* count_dev,
-                  size_t grid);
+                  size_t grid, bool loop);
 
 } // namespace mhip

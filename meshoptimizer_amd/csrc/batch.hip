@@ -4,9 +4,12 @@
 namespace mhip
 {
 
-// One workgroup per mesh of one LDS size class; ids lists the class's *count meshes (the grid
-// is an upper bound, workgroups past the count leave at once).  At most 80 VGPRs, so that six
-// wavefronts share a SIMD and six workgroups a CU when their LDS layouts allow it.
+// The meshes of one LDS size class; ids lists the class's *count meshes.  Without kLoop each
+// workgroup takes one mesh (the grid is an upper bound, workgroups past the count leave at once);
+// with it workgroup b takes meshes b, b + grid, ... (the loop is kept out of the one-mesh form,
+// where it measured 7% slower).  At most 80 VGPRs, so that six wavefronts share a SIMD and six
+// workgroups a CU when their LDS layouts allow it.
+template <bool kLoop>
 __global__ void __launch_bounds__(kBatchThreads) __attribute__((amdgpu_waves_per_eu(6))) k_simplify_patches(BatchArgs a, PatchLayout L, const uint32_t* ids, const uint32_t* count)
 {
 	extern __shared__ __align__(16) uint8_t smem[];
@@ -15,10 +18,18 @@ __global__ void __launch_bounds__(kBatchThreads) __attribute__((amdgpu_waves_per
 	__shared__ uint64_t box[6];
 	__shared__ float shf[8];
 
-	if (blockIdx.x >= *count)
+	const uint32_t n = *count;
+	if (!kLoop)
+	{
+		if (blockIdx.x < n)
+			simplify_patch(a, L, ids[blockIdx.x], smem, part, sh, box, shf);
 		return;
-	const uint32_t patch = ids[blockIdx.x];
-	simplify_patch(a, L, patch, smem, part, sh, box, shf);
+	}
+	for (uint32_t i = blockIdx.x; i < n; i += gridDim.x)
+	{
+		simplify_patch(a, L, ids[i], smem, part, sh, box, shf);
+		__syncthreads();
+	}
 }
 
 // one slot per mesh in its class's id list; the slots of a wavefront's meshes come from one
@@ -41,6 +52,8 @@ __global__ void k_batch_classify(BatchArgs a, BatchClasses c, uint32_t* ids, uin
 					cls = k;
 			if (cls < 0)
 				a.status[p] = 1; // too large for the LDS layouts: the plain path
+			else if (cls == kBatchClasses - 1)
+				a.status[p] = 6; // runs after the other classes (its workgroup takes a whole CU)
 		}
 	}
 	const uint32_t lane = __lane_id();
@@ -60,26 +73,30 @@ __global__ void k_batch_classify(BatchArgs a, BatchClasses c, uint32_t* ids, uin
 	}
 }
 
-// LDS bytes one workgroup may take so that `per_cu` of them fit a CU (160 KB, allocated in
-// 512-byte granules, the kernel's static words included)
-static uint32_t lds_limit(uint32_t per_cu)
+// dynamic LDS bytes one workgroup may take so that `per_cu` of them fit a CU (160 KB, allocated
+// in 512-byte granules, the kernel's static words included) and the workgroup limit allows it
+static uint32_t lds_limit(uint32_t per_cu, uint32_t max_block)
 {
 	const uint32_t cu_lds = 160 * 1024, granule = 512, static_words = 256;
 	uint32_t share = cu_lds / per_cu / granule * granule;
-	share = share < 64 * 1024 ? share : 64 * 1024;
+	share = share < max_block ? share : max_block;
 	return share - static_words;
 }
 
-BatchClasses batch_classes()
+BatchClasses batch_classes(uint32_t max_block)
 {
-	// 6, 5 and 2 workgroups per CU (the kernel's 80 VGPRs allow 6 wavefronts per SIMD); the last
-	// class takes meshes with relatively more vertices and holds full candidate arrays
-	const uint32_t limits[kBatchClasses] = {lds_limit(6), lds_limit(5), lds_limit(2)};
-	const uint32_t vdiv[kBatchClasses] = {4, 4, 3};
+	// 6, 5, 2 and 1 workgroups per CU (the kernel's 80 VGPRs allow 6 wavefronts per SIMD); the
+	// last two classes take meshes with relatively more vertices and hold full candidate arrays
+	if (max_block < 64 * 1024)
+		max_block = 64 * 1024;
+	const uint32_t limits[kBatchClasses] = {lds_limit(6, max_block), lds_limit(5, max_block), lds_limit(2, max_block), lds_limit(1, max_block)};
+	const uint32_t vdiv[kBatchClasses] = {4, 4, 3, 3};
 	BatchClasses c{};
 	for (int k = 0; k < kBatchClasses; ++k)
 	{
-		const bool last = k == kBatchClasses - 1;
+		if (k && limits[k] <= limits[k - 1])
+			continue; // caps 0: no mesh fits
+		const bool last = k >= kBatchRetryClass;
 		uint32_t vbest = 32, ibest = 96;
 		for (uint32_t icap = 96; icap < 0xffffu; icap += 96)
 		{
@@ -167,17 +184,29 @@ void launch_batch_order(hipStream_t s, const BatchArgs& a, uint32_t* ids, const 
 		k_batch_order<<<kBatchClasses, 1024, 0, s>>>(a.index_offsets, ids, counts, a.patch_count, temp);
 }
 
+bool batch_allow_lds(size_t bytes)
+{
+	for (const void* f : {reinterpret_cast<const void*>(&k_simplify_patches<false>), reinterpret_cast<const void*>(&k_simplify_patches<true>)})
+		if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) != hipSuccess)
+			return false;
+	return true;
+}
+
 size_t batch_lds_bytes(uint32_t vmax, uint32_t imax, uint32_t cmax)
 {
 	return patch_layout(vmax, imax, cmax).total;
 }
 
 void launch_batch(hipStream_t s, const BatchArgs& a, uint32_t vmax, uint32_t imax, uint32_t cmax, const uint32_t* ids, const uint32_t* count_dev,
-                  size_t grid)
+                  size_t grid, bool loop)
 {
 	PatchLayout L = patch_layout(vmax, imax, cmax);
-	if (grid)
-		k_simplify_patches<<<(unsigned int)grid, kBatchThreads, L.total, s>>>(a, L, ids, count_dev);
+	if (!grid)
+		return;
+	if (loop)
+		k_simplify_patches<true><<<(unsigned int)grid, kBatchThreads, L.total, s>>>(a, L, ids, count_dev);
+	else
+		k_simplify_patches<false><<<(unsigned int)grid, kBatchThreads, L.total, s>>>(a, L, ids, count_dev);
 }
 
 } // namespace mhip

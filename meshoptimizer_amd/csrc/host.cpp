@@ -146,14 +146,15 @@ struct Context
 			HIP_CHECK(hipMalloc((void**)&batch_dev, words * sizeof(uint32_t)));
 			batch_dev_words = words;
 		}
-		if (meshes > batch_pinned_words)
+		// the status of every mesh, then the class counts (they follow it on the device too)
+		if (meshes + kBatchClasses > batch_pinned_words)
 		{
 			if (batch_pinned)
 				HIP_CHECK(hipHostFree(batch_pinned));
 			batch_pinned = nullptr;
 			batch_pinned_words = 0;
-			HIP_CHECK(hipHostMalloc((void**)&batch_pinned, meshes * sizeof(uint32_t), hipHostMallocDefault));
-			batch_pinned_words = meshes;
+			HIP_CHECK(hipHostMalloc((void**)&batch_pinned, (meshes + kBatchClasses) * sizeof(uint32_t), hipHostMallocDefault));
+			batch_pinned_words = meshes + kBatchClasses;
 		}
 	}
 
@@ -1375,7 +1376,30 @@ static int simplify_batch_on_device(Context& ctx, unsigned int* destination, uns
 	if (!patch_count)
 		return 0;
 
-	static const BatchClasses classes = batch_classes();
+	// the classes beyond 64 KB of LDS need the device's limit and the kernel's permission; without
+	// them the classes are those of a 64 KB workgroup
+	static const BatchClasses classes = [&] {
+		int max_block = 0;
+		HIP_CHECK(hipDeviceGetAttribute(&max_block, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx.device));
+		BatchClasses c = batch_classes((uint32_t)max_block);
+		size_t top = 0;
+		for (int k = 0; k < kBatchClasses; ++k)
+			if (c.icap[k])
+				top = std::max(top, batch_lds_bytes(c.vcap[k], c.icap[k], c.ccap[k]));
+		if (top > 64 * 1024 && !batch_allow_lds(top))
+		{
+			(void)hipGetLastError();
+			c = batch_classes(64 * 1024);
+		}
+		return c;
+	}();
+	// the one-workgroup-per-CU class loops over its meshes on one workgroup per CU: a grid of
+	// patch_count workgroups of the whole LDS would make each leaving workgroup wait for a free CU
+	static const size_t cu_count = [&] {
+		int n = 0;
+		HIP_CHECK(hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, ctx.device));
+		return (size_t)(n > 0 ? n : 1);
+	}();
 	const size_t P = patch_count;
 	ctx.ensure_batch_buffers(P);
 	uint32_t* status = ctx.batch_dev;
@@ -1417,21 +1441,28 @@ static int simplify_batch_on_device(Context& ctx, unsigned int* destination, uns
 		}
 		if (timed)
 			HIP_CHECK(hipEventRecord(ctx.batch_ev[0], s));
-		HIP_CHECK(hipMemsetAsync(counts, 0, 4 * sizeof(uint32_t), s));
+		static_assert(kBatchClasses <= 4, "one count word per class before the id lists");
+		HIP_CHECK(hipMemsetAsync(counts, 0, kBatchClasses * sizeof(uint32_t), s));
 		launch_batch_classify(s, a, classes, ids, counts);
 		launch_batch_order(s, a, ids, counts, ids + (size_t)kBatchClasses * P);
 		ctx.ensure_aux_streams();
 		HIP_CHECK(hipEventRecord(ctx.fork_ev, s));
-		// the class of larger meshes goes first: its longer workgroups start early and leave a
-		// shorter tail (measured 0.6% faster on the 4096-patch batch than class order)
-		static constexpr int kLaunchOrder[kBatchClasses] = {1, 0, 2};
-		for (int c = 0; c < kBatchClasses; ++c)
+		// classes 0, 1 and 2 launch at once with one workgroup per mesh slot (those past the class's
+		// count leave at once), the class of larger meshes first: its longer workgroups start early
+		// and leave a shorter tail (measured 0.6% faster on the 4096-patch batch than class order).
+		// A workgroup of class 3 takes a whole CU's LDS, so even one that leaves at once waits for
+		// a free CU: class 3 launches after the status read-back, only when it has meshes, with at
+		// most one workgroup per CU looping over them (reading the counts back before launching
+		// classes 2 and 3 instead measured 4.5% slower on the 4096-patch batch).
+		static constexpr int kLaunchOrder[kBatchClasses - 1] = {1, 0, 2};
+		for (int c = 0; c < kBatchClasses - 1; ++c)
 		{
 			const int k = kLaunchOrder[c];
 			hipStream_t cs = c == 0 ? s : ctx.aux[c - 1];
 			if (c)
 				HIP_CHECK(hipStreamWaitEvent(cs, ctx.fork_ev, 0));
-			launch_batch(cs, a, classes.vcap[k], classes.icap[k], classes.ccap[k], ids + (size_t)k * P, counts + k, P);
+			if (classes.icap[k])
+				launch_batch(cs, a, classes.vcap[k], classes.icap[k], classes.ccap[k], ids + (size_t)k * P, counts + k, P, false);
 			if (c)
 			{
 				HIP_CHECK(hipEventRecord(ctx.join_ev[c - 1], cs));
@@ -1441,33 +1472,60 @@ static int simplify_batch_on_device(Context& ctx, unsigned int* destination, uns
 		if (timed)
 			HIP_CHECK(hipEventRecord(ctx.batch_ev[1], s));
 		HIP_CHECK(hipGetLastError());
-		HIP_CHECK(hipMemcpyAsync(h_status, status, P * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+		HIP_CHECK(hipMemcpyAsync(h_status, status, (P + kBatchClasses) * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
 		HIP_CHECK(hipStreamSynchronize(s));
-		if (timed)
-		{
+		auto add_time = [&] {
 			float ms = 0;
 			HIP_CHECK(hipEventElapsedTime(&ms, ctx.batch_ev[0], ctx.batch_ev[1]));
 			ctx.batch_ms += ms;
+		};
+		if (timed)
+		{
+			add_time();
 			ctx.batch_launches += 1;
 			ctx.batch_meshes += patch_count;
 		}
-		// meshes with more candidates than their class's arrays hold (status 5) run again in the
-		// last class, whose arrays hold the full bound; its workgroups take the plain-path exit
-		// (status 1) for meshes beyond its vertex or index caps
+		// class 3's meshes (status 6 until they run), and the meshes with more candidates than
+		// their class's arrays hold (status 5), which run again in the retry class: its arrays
+		// hold the full bound and its caps every mesh of the classes before it
+		const uint32_t large = h_status[P + kBatchClasses - 1];
 		std::vector<uint32_t> retry;
 		for (size_t p = 0; p < P; ++p)
 			if (h_status[p] == 5)
 				retry.push_back((uint32_t)p);
-		if (!retry.empty())
+		if (!retry.empty() || large)
 		{
-			const int c = kBatchClasses - 1;
-			const uint32_t n = (uint32_t)retry.size();
-			HIP_CHECK(hipMemcpyAsync(ids, retry.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, s));
-			HIP_CHECK(hipMemcpyAsync(counts, &n, sizeof(uint32_t), hipMemcpyHostToDevice, s));
-			launch_batch(s, a, classes.vcap[c], classes.icap[c], classes.ccap[c], ids, counts, n);
+			if (timed)
+				HIP_CHECK(hipEventRecord(ctx.batch_ev[0], s));
+			hipStream_t ls = ctx.aux[kBatchClasses - 2];
+			if (large)
+			{
+				const int c = kBatchClasses - 1;
+				HIP_CHECK(hipEventRecord(ctx.fork_ev, s));
+				HIP_CHECK(hipStreamWaitEvent(ls, ctx.fork_ev, 0));
+				launch_batch(ls, a, classes.vcap[c], classes.icap[c], classes.ccap[c], ids + (size_t)c * P, counts + c,
+				             std::min<size_t>(large, cu_count), true);
+			}
+			if (!retry.empty())
+			{
+				const int c = kBatchRetryClass;
+				const uint32_t n = (uint32_t)retry.size();
+				HIP_CHECK(hipMemcpyAsync(ids, retry.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+				HIP_CHECK(hipMemcpyAsync(counts, &n, sizeof(uint32_t), hipMemcpyHostToDevice, s));
+				launch_batch(s, a, classes.vcap[c], classes.icap[c], classes.ccap[c], ids, counts, n, false);
+			}
+			if (large)
+			{
+				HIP_CHECK(hipEventRecord(ctx.join_ev[kBatchClasses - 2], ls));
+				HIP_CHECK(hipStreamWaitEvent(s, ctx.join_ev[kBatchClasses - 2], 0));
+			}
+			if (timed)
+				HIP_CHECK(hipEventRecord(ctx.batch_ev[1], s));
 			HIP_CHECK(hipGetLastError());
 			HIP_CHECK(hipMemcpyAsync(h_status, status, P * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
 			HIP_CHECK(hipStreamSynchronize(s));
+			if (timed)
+				add_time();
 		}
 	}
 	else
@@ -1484,6 +1542,8 @@ static int simplify_batch_on_device(Context& ctx, unsigned int* destination, uns
 			throw std::runtime_error("meshopt_hip_simplify_batch: internal error (pass bound exceeded)");
 		if (h_status[p] == 5)
 			throw std::runtime_error("meshopt_hip_simplify_batch: internal error (candidate arrays of the last class exceeded)");
+		if (h_status[p] == 6)
+			throw std::runtime_error("meshopt_hip_simplify_batch: internal error (a mesh of the largest class did not run)");
 		fallback |= h_status[p] == 1;
 	}
 	if (!fallback)

@@ -82,6 +82,30 @@ def test_oversized_mesh_takes_plain_path():
     check_batch(meshes, targets, 1e-2, M.SimplifyLockBorder)
 
 
+@pytest.mark.parametrize("opts", [M.SimplifyLockBorder, M.SimplifyErrorAbsolute | M.SimplifyPermissive])
+def test_size_classes(opts):
+    """Meshes for every LDS size class in one call: patches (6 and 5 workgroups per CU), seamed
+    grids of 2400-2904 indices (2 per CU) and 300 of 3174-5400 indices (one workgroup per CU
+    with the whole LDS, which loops over its meshes: more of them than CUs), and one of 6534
+    indices beyond every class (the plain path)."""
+    rng = np.random.default_rng(4242 + opts)
+    meshes, targets = patch_meshes(40, seed=31)
+    for k in range(320):
+        n = int(rng.integers(21, 24)) if k < 20 else int(rng.integers(24, 32))
+        ib, v = random_grid(rng, n=n, seams=3)
+        meshes.append((ib, v))
+        targets.append((ib.size // 3 * int(rng.integers(1, 4)) // 4) * 3)
+    big = W.grid(34)
+    assert big.index_count > 6048
+    meshes.append((big.indices, big.vertices))
+    targets.append(W.target_count(big.index_count, 1, 2))
+    order = rng.permutation(len(meshes))
+    meshes = [meshes[i] for i in order]
+    targets = [targets[i] for i in order]
+    announce(f"size classes: {len(meshes)} meshes, largest {max(m[0].size for m in meshes)} indices, opts {opts}")
+    check_batch(meshes, targets, 1e-2, opts)
+
+
 def test_sparse_batch():
     rng = np.random.default_rng(99)
     meshes, targets = [], []

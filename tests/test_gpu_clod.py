@@ -47,11 +47,13 @@ def level_case(n, tiles, seed):
                                                (64, 4, 2, M.SimplifyErrorAbsolute | M.SimplifyPermissive),
                                                (96, 6, 3, M.SimplifyErrorAbsolute | M.SimplifyPermissive | M.SimplifySparse),
                                                (64, 5, 4, M.SimplifyErrorAbsolute),
+                                               (60, 2, 7, M.SimplifyErrorAbsolute | M.SimplifyPermissive),
                                                (96, 2, 6, M.SimplifyErrorAbsolute | M.SimplifyPermissive)])
 def test_clod_level(n, tiles, seed, opts):
-    """Groups of about 1000-1500 indices (hashed in LDS, simplified in the batched kernel) and, on a
-    96x96 grid in 2x2 tiles, groups of about 14000 indices (hashed in their global slices and
-    simplified on the plain path)."""
+    """Groups of about 1000-1500 indices (hashed in LDS, simplified in the batched kernel), of about
+    5400 (60x60 grid in 2x2 tiles: the default clusterlod group size, global hash slices, the
+    one-workgroup-per-CU class) and, on a 96x96 grid in 2x2 tiles, of about 14000 indices
+    (global hash slices, the plain path)."""
     from devmem import DeviceArray
     v, groups, remap, base, locks_expected = level_case(n, tiles, seed)
     io = np.zeros(len(groups) + 1, np.uint32)

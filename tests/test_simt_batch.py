@@ -47,7 +47,7 @@ def simt():
     return lib
 
 
-def run_simt(lib, meshes, targets, err, opts, mode=3, seed=1, locks=None):
+def run_simt(lib, meshes, targets, err, opts, mode=3, seed=1, locks=None, caps=None):
     idx, io, verts, vo, stride = M.pack_meshes(meshes)
     lk = None
     if locks is not None:
@@ -59,6 +59,9 @@ def run_simt(lib, meshes, targets, err, opts, mode=3, seed=1, locks=None):
     n = len(meshes)
     vmax = (int(np.diff(vo).max()) + 31) & ~31
     imax = (int(np.diff(io).max()) + 95) // 96 * 96
+    if caps is not None:
+        assert caps[0] >= vmax and caps[1] >= imax
+        vmax, imax = caps
     dest = np.zeros(max(idx.size, 1), np.uint32)
     cnt = np.zeros(n, np.uint32)
     errs = np.zeros(n, np.float32)
@@ -132,3 +135,18 @@ def test_vertex_locks(simt, seed):
         locks.append(rng.integers(0, 8, size=v.shape[0]).astype(np.uint8) * (rng.random(v.shape[0]) < 0.3))
     opts = [0, 32, 1 | 32, 4 | 32, 16][seed % 5]
     check(simt, meshes, targets, float(rng.choice([1e-2, 1e-1, 1.0])), opts, locks=locks, seed=seed)
+
+
+def test_one_per_cu_layout(simt):
+    """The largest size class (one workgroup per CU, the whole 160 KB of LDS: 2016 vertices, 6048
+    indices) holds groups of sixteen 128-triangle clusters, the clusterlod demo's default
+    (demo/nanite.cpp:80, demo/clusterlod.h:552-566); its layout runs the same device code."""
+    g = W.grid(31)
+    s = W.seamed_grid(28, seams=3, seed=8)
+    assert g.index_count <= 6048 and s.index_count <= 6048 and s.vertex_count <= 2016
+    vs = np.ascontiguousarray(s.vertices[:, :3])
+    lk = ((np.arange(s.vertex_count) % 11) == 0).astype(np.uint8) * M.SimplifyVertex_Lock
+    meshes = [(g.indices, np.ascontiguousarray(g.vertices[:, :3])), (s.indices, vs)]
+    targets = [W.target_count(g.index_count, 1, 2), W.target_count(s.index_count, 1, 2)]
+    check(simt, meshes, targets, 1e-2, M.SimplifyErrorAbsolute | M.SimplifyPermissive, locks=[np.zeros(g.vertex_count, np.uint8), lk],
+          caps=(2016, 6048))
