@@ -8,9 +8,12 @@
 namespace mhip
 {
 
-// groups of at most kClodLdsSlots / 2 indices (every group the batched kernel's classes take) are
-// packed by one kernel in LDS; larger groups use hash slices in global memory
-constexpr uint32_t kClodLdsSlots = 4608;
+// groups of at most lds_slots / 2 indices are packed by one kernel in LDS (8 bytes per slot; a
+// launch's lds_slots fits its largest group, up to kClodLdsSlotsMax, which holds every group the
+// batched kernel's classes take); larger groups use hash slices in global memory
+constexpr uint32_t kClodLdsSlotsMax = 12288;
+// the LDS slot capacity of a level whose largest group has max_group_indices indices
+uint32_t clod_lds_slots(uint32_t max_group_indices);
 
 // device buffers of one level (carved by the host); I = indices of all groups, G = groups,
 // V = vertices of the shared buffer
@@ -21,6 +24,7 @@ struct ClodBuffers
 	const uint32_t* remap;         // V: position remap (lowest vertex with the same position)
 	uint32_t group_count;
 	uint32_t vertex_count;
+	uint32_t lds_slots;            // LDS hash capacity of this launch (clod_lds_slots)
 
 	uint32_t* first_group;         // V: lowest group that uses each remapped vertex (kNone: none)
 	uint32_t* last_group;          // V: highest group that uses it (0: none)

@@ -88,17 +88,19 @@ __device__ __forceinline__ void clod_first_use(const ClodBuffers& b, uint32_t g,
 
 static constexpr uint32_t kRanked = 0x80000000u;
 
-// groups of at most kClodLdsSlots slots: the whole packing of the group in LDS.  The ordered pass
+// groups of at most lds_slots slots: the whole packing of the group in LDS.  The ordered pass
 // replaces a ranked vertex's first position by kRanked | id (positions stay below 2^31), so later
 // uses read the id from the same slot and no position compares equal to a ranked entry.
 __global__ void __launch_bounds__(kClodThreads) k_clod_group(ClodBuffers b)
 {
-	__shared__ uint32_t lkey[kClodLdsSlots], lfirst[kClodLdsSlots];
+	extern __shared__ uint32_t lhash[]; // lds_slots keys, then lds_slots first positions
 	__shared__ uint32_t lds[kClodThreads / 64];
+	uint32_t* lkey = lhash;
+	uint32_t* lfirst = lhash + b.lds_slots;
 	const uint32_t g = blockIdx.x;
 	const uint32_t begin = b.index_offsets[g], end = b.index_offsets[g + 1];
 	const uint32_t slots = 2 * (end - begin);
-	if (slots > kClodLdsSlots)
+	if (slots > b.lds_slots)
 		return;
 	for (uint32_t q = threadIdx.x; q < slots; q += kClodThreads)
 	{
@@ -144,7 +146,7 @@ __global__ void __launch_bounds__(kClodThreads) k_clod_insert(ClodBuffers b)
 	const uint32_t g = blockIdx.x;
 	const uint32_t begin = b.index_offsets[g], end = b.index_offsets[g + 1];
 	const uint32_t slots = 2 * (end - begin);
-	if (slots <= kClodLdsSlots)
+	if (slots <= b.lds_slots)
 		return;
 	uint32_t* key = b.slot_key + 2 * (size_t)begin;
 	uint32_t* first = b.slot_first + 2 * (size_t)begin;
@@ -158,7 +160,7 @@ __global__ void __launch_bounds__(kClodThreads) k_clod_rank(ClodBuffers b)
 	const uint32_t g = blockIdx.x;
 	const uint32_t begin = b.index_offsets[g], end = b.index_offsets[g + 1];
 	const uint32_t slots = 2 * (end - begin);
-	if (slots <= kClodLdsSlots)
+	if (slots <= b.lds_slots)
 		return;
 	const uint32_t* key = b.slot_key + 2 * (size_t)begin;
 	const uint32_t* first = b.slot_first + 2 * (size_t)begin;
@@ -193,7 +195,7 @@ __global__ void __launch_bounds__(kClodThreads) k_clod_local(ClodBuffers b)
 	const uint32_t g = blockIdx.x;
 	const uint32_t begin = b.index_offsets[g], end = b.index_offsets[g + 1];
 	const uint32_t slots = 2 * (end - begin);
-	if (slots <= kClodLdsSlots)
+	if (slots <= b.lds_slots)
 		return;
 	const uint32_t* key = b.slot_key + 2 * (size_t)begin;
 	const uint32_t* local = b.slot_local + 2 * (size_t)begin;
@@ -201,11 +203,25 @@ __global__ void __launch_bounds__(kClodThreads) k_clod_local(ClodBuffers b)
 		b.packed_indices[j] = local[clod_find(key, b.indices[j], slots)];
 }
 
+uint32_t clod_lds_slots(uint32_t max_group_indices)
+{
+	// beyond 64 KB of dynamic LDS the kernel needs the runtime's permission
+	static const uint32_t cap = [] {
+		if (hipFuncSetAttribute(reinterpret_cast<const void*>(&k_clod_group), hipFuncAttributeMaxDynamicSharedMemorySize,
+		                        (int)(kClodLdsSlotsMax * 8)) == hipSuccess)
+			return kClodLdsSlotsMax;
+		(void)hipGetLastError();
+		return 8000u;
+	}();
+	const uint32_t need = (2 * max_group_indices + 63) & ~63u;
+	return need < 64 ? 64 : need < cap ? need : cap;
+}
+
 void launch_clod_group(hipStream_t s, const ClodBuffers& b, bool large_groups)
 {
 	if (!b.group_count)
 		return;
-	k_clod_group<<<b.group_count, kClodThreads, 0, s>>>(b);
+	k_clod_group<<<b.group_count, kClodThreads, 8 * (size_t)b.lds_slots, s>>>(b);
 	if (large_groups)
 	{
 		k_clod_insert<<<b.group_count, kClodThreads, 0, s>>>(b);

@@ -1711,9 +1711,9 @@ extern "C" int meshopt_hip_clod_level_device(unsigned int* destination, unsigned
 			if (io[g + 1] < io[g] || (io[g + 1] - io[g]) % 3)
 				throw std::invalid_argument("meshopt_hip_clod_level_device: invalid group index counts");
 		const size_t I = io[G];
-		bool large_groups = false;
+		uint32_t max_group = 0;
 		for (size_t g = 0; g < G; ++g)
-			large_groups |= 2 * (size_t)(io[g + 1] - io[g]) > kClodLdsSlots;
+			max_group = std::max(max_group, io[g + 1] - io[g]);
 
 		// carve the level's buffers (256-byte aligned)
 		size_t used = 0;
@@ -1735,6 +1735,8 @@ extern "C" int meshopt_hip_clod_level_device(unsigned int* destination, unsigned
 		b.remap = remap;
 		b.group_count = (uint32_t)G;
 		b.vertex_count = (uint32_t)V;
+		b.lds_slots = clod_lds_slots(max_group);
+		const bool large_groups = 2 * (size_t)max_group > b.lds_slots;
 		b.first_group = reinterpret_cast<uint32_t*>(base + o_first);
 		b.last_group = reinterpret_cast<uint32_t*>(base + o_last);
 		b.slot_key = reinterpret_cast<uint32_t*>(base + o_key);

@@ -22,15 +22,20 @@ def announce(what):
     sys.stdout.flush()
 
 
-def level_case(n, tiles, seed):
-    """A seamed grid's positions, its triangles split into tiles x tiles spatial groups (shuffled
-    group order), the position remap, random caller lock/protect bits, and the expected locks."""
+def level_case(n, tiles, seed, bands=None):
+    """A seamed grid's positions, its triangles split into tiles x tiles spatial groups (or, with
+    bands, into vertical bands cut at those x fractions), in shuffled group order, the position
+    remap, random caller lock/protect bits, and the expected locks."""
     rng = np.random.default_rng(seed)
     m = W.seamed_grid(n, seams=4, seed=seed)
     v = np.ascontiguousarray(m.vertices[:, :3])
     tri = m.indices.reshape(-1, 3)
     c = v[tri].mean(axis=1)
-    g = np.minimum((c[:, 1] * tiles).astype(int), tiles - 1) * tiles + np.minimum((c[:, 0] * tiles).astype(int), tiles - 1)
+    if bands is None:
+        g = np.minimum((c[:, 1] * tiles).astype(int), tiles - 1) * tiles + np.minimum((c[:, 0] * tiles).astype(int), tiles - 1)
+    else:
+        g = np.searchsorted(np.array(bands), c[:, 0])
+        tiles = int(np.ceil(np.sqrt(len(bands) + 1)))
     ids = rng.permutation(tiles * tiles)
     groups = [tri[g == k].reshape(-1).astype(np.uint32) for k in ids if (g == k).any()]
     _, first, inv = np.unique(v, axis=0, return_index=True, return_inverse=True)
@@ -81,6 +86,38 @@ def test_clod_level(n, tiles, seed, opts):
         assert same_bits(errors[p], eo), f"group {p}: result_error {errors[p]!r} != {eo!r}"
         moved += rg.size < grp.size
     assert moved > 0
+
+
+def test_clod_level_mixed_group_sizes():
+    """One level whose groups span every path: about 2000 and 5000 indices (LDS hash; the 2- and
+    1-per-CU classes), about 8000-27000 (global hash slices; the plain path)."""
+    from devmem import DeviceArray
+    v, groups, remap, base, locks_expected = level_case(96, 0, 8, bands=[0.04, 0.14, 0.29, 0.5])
+    sizes = sorted(grp.size for grp in groups)
+    assert sizes[0] < 3000 and 3000 < sizes[1] <= 6048 and sizes[-1] > 20000
+    io = np.zeros(len(groups) + 1, np.uint32)
+    io[1:] = np.cumsum([grp.size for grp in groups])
+    idx = np.concatenate(groups).astype(np.uint32)
+    targets = np.array([grp.size // 3 // 3 * 3 for grp in groups], np.uint32)
+    G = len(groups)
+    opts = M.SimplifyErrorAbsolute | M.SimplifyPermissive
+    announce(f"clod level mixed group sizes {sizes}")
+    with DeviceArray(idx) as d_idx, DeviceArray(io) as d_io, DeviceArray(v) as d_v, DeviceArray(remap) as d_remap, \
+            DeviceArray(base) as d_base, DeviceArray(nbytes=v.shape[0]) as d_locks, DeviceArray(targets) as d_tg, \
+            DeviceArray(nbytes=idx.nbytes) as d_dst, DeviceArray(nbytes=4 * G) as d_cnt, DeviceArray(nbytes=4 * G) as d_err:
+        M.clod_level_device(d_dst.ptr, d_cnt.ptr, d_err.ptr, d_idx.ptr, d_io.ptr, G, d_v.ptr, v.shape[0], 12, d_remap.ptr, d_base.ptr,
+                            d_locks.ptr, d_tg.ptr, FLT_MAX, opts)
+        locks = d_locks.to_host(np.uint8, v.shape[0])
+        counts = d_cnt.to_host(np.uint32, G)
+        errors = d_err.to_host(np.float32, G)
+        dst = d_dst.to_host(np.uint32, idx.size)
+    assert np.array_equal(locks, locks_expected)
+    for p, grp in enumerate(groups):
+        ro, eo = O.simplify(grp, v, int(targets[p]), FLT_MAX, opts | M.SimplifySparse, attributes=np.zeros((v.shape[0], 0), np.float32),
+                            weights=np.zeros(0, np.float32), lock=locks_expected)
+        rg = dst[io[p]:io[p] + counts[p]]
+        assert rg.size == ro.size and np.array_equal(rg, ro), f"group {p} ({grp.size} indices): indices differ from the Sparse call"
+        assert same_bits(errors[p], eo), f"group {p}: result_error {errors[p]!r} != {eo!r}"
 
 
 def test_clod_level_rejects_bad_input():
