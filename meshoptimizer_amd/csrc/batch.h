@@ -59,8 +59,9 @@ void launch_batch_order(hipStream_t s, const BatchArgs& a, uint32_t* ids, const 
 
 // the meshes of a class (ids, *count_dev of them), every one with the (vmax, imax, cmax) layout:
 // workgroup b takes mesh b (grid at least the count; the workgroups past it leave at once) or,
-// with loop, the meshes b, b + grid, ...
+// given a cursor word (cleared here on the stream), each workgroup takes meshes from the list in
+// order until it ends
 void launch_batch(hipStream_t s, const BatchArgs& a, uint32_t vmax, uint32_t imax, uint32_t cmax, const uint32_t* ids, const uint32_t* count_dev,
-                  size_t grid, bool loop);
+                  size_t grid, uint32_t* cursor);
 
 } // namespace mhip

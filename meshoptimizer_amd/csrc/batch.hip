@@ -6,17 +6,19 @@ namespace mhip
 
 // The meshes of one LDS size class; ids lists the class's *count meshes.  Without kLoop each
 // workgroup takes one mesh (the grid is an upper bound, workgroups past the count leave at once);
-// with it workgroup b takes meshes b, b + grid, ... (the loop is kept out of the one-mesh form,
-// where it measured 7% slower).  At most 80 VGPRs, so that six wavefronts share a SIMD and six
-// workgroups a CU when their LDS layouts allow it.
+// with it each workgroup takes the next mesh of the list from *cursor (zero at the launch) until
+// the list ends, so a workgroup that finishes early takes the next largest mesh (the loop is kept
+// out of the one-mesh form, where it measured 7% slower).  At most 80 VGPRs, so that six
+// wavefronts share a SIMD and six workgroups a CU when their LDS layouts allow it.
 template <bool kLoop>
-__global__ void __launch_bounds__(kBatchThreads) __attribute__((amdgpu_waves_per_eu(6))) k_simplify_patches(BatchArgs a, PatchLayout L, const uint32_t* ids, const uint32_t* count)
+__global__ void __launch_bounds__(kBatchThreads) __attribute__((amdgpu_waves_per_eu(6))) k_simplify_patches(BatchArgs a, PatchLayout L, const uint32_t* ids, const uint32_t* count, uint32_t* cursor)
 {
 	extern __shared__ __align__(16) uint8_t smem[];
 	__shared__ uint32_t part[kBatchThreads / 64];
 	__shared__ uint32_t sh[16];
 	__shared__ uint64_t box[6];
 	__shared__ float shf[8];
+	__shared__ uint32_t next;
 
 	const uint32_t n = *count;
 	if (!kLoop)
@@ -25,8 +27,16 @@ __global__ void __launch_bounds__(kBatchThreads) __attribute__((amdgpu_waves_per
 			simplify_patch(a, L, ids[blockIdx.x], smem, part, sh, box, shf);
 		return;
 	}
-	for (uint32_t i = blockIdx.x; i < n; i += gridDim.x)
+	for (;;)
 	{
+		// thread 0 writes `next` again only after the barrier that ends the mesh, which every
+		// thread reaches after reading it
+		if (threadIdx.x == 0)
+			next = atomicAdd(cursor, 1u);
+		__syncthreads();
+		const uint32_t i = next;
+		if (i >= n)
+			break;
 		simplify_patch(a, L, ids[i], smem, part, sh, box, shf);
 		__syncthreads();
 	}
@@ -198,15 +208,18 @@ size_t batch_lds_bytes(uint32_t vmax, uint32_t imax, uint32_t cmax)
 }
 
 void launch_batch(hipStream_t s, const BatchArgs& a, uint32_t vmax, uint32_t imax, uint32_t cmax, const uint32_t* ids, const uint32_t* count_dev,
-                  size_t grid, bool loop)
+                  size_t grid, uint32_t* cursor)
 {
 	PatchLayout L = patch_layout(vmax, imax, cmax);
 	if (!grid)
 		return;
-	if (loop)
-		k_simplify_patches<true><<<(unsigned int)grid, kBatchThreads, L.total, s>>>(a, L, ids, count_dev);
+	if (cursor)
+	{
+		HIP_CHECK(hipMemsetAsync(cursor, 0, sizeof(uint32_t), s));
+		k_simplify_patches<true><<<(unsigned int)grid, kBatchThreads, L.total, s>>>(a, L, ids, count_dev, cursor);
+	}
 	else
-		k_simplify_patches<false><<<(unsigned int)grid, kBatchThreads, L.total, s>>>(a, L, ids, count_dev);
+		k_simplify_patches<false><<<(unsigned int)grid, kBatchThreads, L.total, s>>>(a, L, ids, count_dev, nullptr);
 }
 
 } // namespace mhip

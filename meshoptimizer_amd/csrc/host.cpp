@@ -136,7 +136,8 @@ struct Context
 
 	void ensure_batch_buffers(size_t meshes)
 	{
-		size_t words = meshes + 4 + 2 * (size_t)kBatchClasses * meshes;
+		// status, class counts, id lists, their ordering's scratch, the largest class's cursor
+		size_t words = meshes + 4 + 2 * (size_t)kBatchClasses * meshes + 4;
 		if (words > batch_dev_words)
 		{
 			if (batch_dev)
@@ -1462,7 +1463,7 @@ static int simplify_batch_on_device(Context& ctx, unsigned int* destination, uns
 			if (c)
 				HIP_CHECK(hipStreamWaitEvent(cs, ctx.fork_ev, 0));
 			if (classes.icap[k])
-				launch_batch(cs, a, classes.vcap[k], classes.icap[k], classes.ccap[k], ids + (size_t)k * P, counts + k, P, false);
+				launch_batch(cs, a, classes.vcap[k], classes.icap[k], classes.ccap[k], ids + (size_t)k * P, counts + k, P, nullptr);
 			if (c)
 			{
 				HIP_CHECK(hipEventRecord(ctx.join_ev[c - 1], cs));
@@ -1504,7 +1505,7 @@ static int simplify_batch_on_device(Context& ctx, unsigned int* destination, uns
 				HIP_CHECK(hipEventRecord(ctx.fork_ev, s));
 				HIP_CHECK(hipStreamWaitEvent(ls, ctx.fork_ev, 0));
 				launch_batch(ls, a, classes.vcap[c], classes.icap[c], classes.ccap[c], ids + (size_t)c * P, counts + c,
-				             std::min<size_t>(large, cu_count), true);
+				             std::min<size_t>(large, cu_count), ids + 2 * (size_t)kBatchClasses * P);
 			}
 			if (!retry.empty())
 			{
@@ -1512,7 +1513,7 @@ static int simplify_batch_on_device(Context& ctx, unsigned int* destination, uns
 				const uint32_t n = (uint32_t)retry.size();
 				HIP_CHECK(hipMemcpyAsync(ids, retry.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, s));
 				HIP_CHECK(hipMemcpyAsync(counts, &n, sizeof(uint32_t), hipMemcpyHostToDevice, s));
-				launch_batch(s, a, classes.vcap[c], classes.icap[c], classes.ccap[c], ids, counts, n, false);
+				launch_batch(s, a, classes.vcap[c], classes.icap[c], classes.ccap[c], ids, counts, n, nullptr);
 			}
 			if (large)
 			{
