@@ -1367,7 +1367,7 @@ static int simplify_batch_on_device(Context& ctx, unsigned int* destination, uns
                                     const unsigned int* vertex_offsets, const unsigned char* vertex_lock, size_t vertex_count, size_t stride,
                                     const unsigned int* target_index_counts,
                                     size_t patch_count, float target_error, unsigned int options, const unsigned int* h_index_offsets,
-                                    const unsigned int* h_vertex_offsets, const unsigned int* h_targets)
+                                    const unsigned int* h_vertex_offsets, const unsigned int* h_targets, size_t max_mesh_indices = 0)
 {
 	hipStream_t s = ctx.stream;
 	if (stride < 12 || stride > 256 || stride % 4)
@@ -1448,6 +1448,21 @@ static int simplify_batch_on_device(Context& ctx, unsigned int* destination, uns
 		launch_batch_order(s, a, ids, counts, ids + (size_t)kBatchClasses * P);
 		ctx.ensure_aux_streams();
 		HIP_CHECK(hipEventRecord(ctx.fork_ev, s));
+		// when the host knows of a mesh beyond class 2's index cap (from host offsets or the
+		// caller's max_mesh_indices), class 3 launches at once beside the others instead
+		if (h_index_offsets)
+			for (size_t p = 0; p < P; ++p)
+				max_mesh_indices = std::max<size_t>(max_mesh_indices, h_index_offsets[p + 1] - h_index_offsets[p]);
+		const int big = kBatchClasses - 1;
+		const bool big_early = classes.icap[big] && max_mesh_indices > classes.icap[kBatchRetryClass];
+		if (big_early)
+		{
+			HIP_CHECK(hipStreamWaitEvent(ctx.aux[big - 1], ctx.fork_ev, 0));
+			launch_batch(ctx.aux[big - 1], a, classes.vcap[big], classes.icap[big], classes.ccap[big], ids + (size_t)big * P, counts + big,
+			             std::min(P, cu_count), ids + 2 * (size_t)kBatchClasses * P);
+			HIP_CHECK(hipEventRecord(ctx.join_ev[big - 1], ctx.aux[big - 1]));
+			HIP_CHECK(hipStreamWaitEvent(s, ctx.join_ev[big - 1], 0));
+		}
 		// classes 0, 1 and 2 launch at once with one workgroup per mesh slot (those past the class's
 		// count leave at once), the class of larger meshes first: its longer workgroups start early
 		// and leave a shorter tail (measured 0.6% faster on the 4096-patch batch than class order).
@@ -1489,7 +1504,7 @@ static int simplify_batch_on_device(Context& ctx, unsigned int* destination, uns
 		// class 3's meshes (status 6 until they run), and the meshes with more candidates than
 		// their class's arrays hold (status 5), which run again in the retry class: its arrays
 		// hold the full bound and its caps every mesh of the classes before it
-		const uint32_t large = h_status[P + kBatchClasses - 1];
+		const uint32_t large = big_early ? 0 : h_status[P + kBatchClasses - 1];
 		std::vector<uint32_t> retry;
 		for (size_t p = 0; p < P; ++p)
 			if (h_status[p] == 5)
@@ -1784,7 +1799,7 @@ extern "C" int meshopt_hip_clod_level_device(unsigned int* destination, unsigned
 		// the packed groups are the meshes of one batched call (Sparse is what the packing did)
 		simplify_batch_on_device(ctx, b.packed_result, result_counts, result_errors, b.packed_indices, index_offsets, b.packed_positions, b.unique,
 		                         b.packed_locks, I, 12, target_index_counts, G, target_error, options & ~(unsigned int)kOptSparse, nullptr, nullptr,
-		                         nullptr);
+		                         nullptr, max_group);
 		launch_clod_unpack(s, b, result_counts, destination);
 		HIP_CHECK(hipStreamSynchronize(s));
 		t_last_error.clear();

@@ -82,12 +82,33 @@ def test_oversized_mesh_takes_plain_path():
     check_batch(meshes, targets, 1e-2, M.SimplifyLockBorder)
 
 
+def check_batch_device(meshes, targets, err, opts):
+    """The device entry (offsets on the device only) against the oracle, mesh by mesh."""
+    from devmem import DeviceArray
+    idx, io, verts, vo, stride = M.pack_meshes(meshes)
+    tg = np.ascontiguousarray(targets, dtype=np.uint32)
+    n = len(meshes)
+    with DeviceArray(idx) as d_idx, DeviceArray(io) as d_io, DeviceArray(verts) as d_v, DeviceArray(vo) as d_vo, DeviceArray(tg) as d_tg, \
+            DeviceArray(nbytes=idx.nbytes) as d_dst, DeviceArray(nbytes=4 * n) as d_cnt, DeviceArray(nbytes=4 * n) as d_err:
+        M.simplify_batch_device(d_dst.ptr, d_cnt.ptr, d_err.ptr, d_idx.ptr, d_io.ptr, d_v.ptr, d_vo.ptr, int(vo[-1]), stride, d_tg.ptr, n, err, opts)
+        dst = d_dst.to_host(np.uint32, idx.size)
+        cnt = d_cnt.to_host(np.uint32, n)
+        errs = d_err.to_host(np.float32, n)
+    for p, ((ib, v), t) in enumerate(zip(meshes, targets)):
+        ro, eo = O.simplify(ib, v, int(t), err, opts)
+        rg = dst[io[p]:io[p] + cnt[p]]
+        assert rg.size == ro.size and np.array_equal(rg, ro), f"mesh {p}: indices differ from the oracle"
+        assert same_bits(errs[p], eo), f"mesh {p}: result_error {errs[p]!r} != {eo!r}"
+
+
+@pytest.mark.parametrize("entry", ["host", "device"])
 @pytest.mark.parametrize("opts", [M.SimplifyLockBorder, M.SimplifyErrorAbsolute | M.SimplifyPermissive])
-def test_size_classes(opts):
+def test_size_classes(opts, entry):
     """Meshes for every LDS size class in one call: patches (6 and 5 workgroups per CU), seamed
     grids of 2400-2904 indices (2 per CU) and 300 of 3174-5400 indices (one workgroup per CU
-    with the whole LDS, which loops over its meshes: more of them than CUs), and one of 6534
-    indices beyond every class (the plain path)."""
+    with the whole LDS, taking meshes from a cursor: more of them than CUs), and one of 6534
+    indices beyond every class (the plain path).  The host entry knows the sizes and launches the
+    largest class beside the others; the device entry launches it after the status read-back."""
     rng = np.random.default_rng(4242 + opts)
     meshes, targets = patch_meshes(40, seed=31)
     for k in range(320):
@@ -102,8 +123,11 @@ def test_size_classes(opts):
     order = rng.permutation(len(meshes))
     meshes = [meshes[i] for i in order]
     targets = [targets[i] for i in order]
-    announce(f"size classes: {len(meshes)} meshes, largest {max(m[0].size for m in meshes)} indices, opts {opts}")
-    check_batch(meshes, targets, 1e-2, opts)
+    announce(f"size classes: {len(meshes)} meshes, largest {max(m[0].size for m in meshes)} indices, opts {opts}, {entry} entry")
+    if entry == "host":
+        check_batch(meshes, targets, 1e-2, opts)
+    else:
+        check_batch_device(meshes, targets, 1e-2, opts)
 
 
 def test_sparse_batch():
