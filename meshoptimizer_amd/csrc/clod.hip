@@ -54,6 +54,8 @@ __device__ __forceinline__ void clod_insert(uint32_t* key, uint32_t* first, uint
 	}
 }
 
+// exclusive scan of one value per thread over an NT-thread workgroup; lds holds NT / 64 words
+template <int NT = kClodThreads>
 __device__ __forceinline__ uint32_t clod_block_scan(uint32_t v, uint32_t* lds, uint32_t& total)
 {
 	uint32_t incl = v;
@@ -68,7 +70,7 @@ __device__ __forceinline__ uint32_t clod_block_scan(uint32_t v, uint32_t* lds, u
 		lds[wave] = incl;
 	__syncthreads();
 	uint32_t before = 0, all = 0;
-	for (uint32_t w = 0; w < kClodThreads / 64; ++w)
+	for (uint32_t w = 0; w < NT / 64; ++w)
 	{
 		before += w < wave ? lds[w] : 0;
 		all += lds[w];
@@ -90,11 +92,14 @@ static constexpr uint32_t kRanked = 0x80000000u;
 
 // groups of at most lds_slots slots: the whole packing of the group in LDS.  The ordered pass
 // replaces a ranked vertex's first position by kRanked | id (positions stay below 2^31), so later
-// uses read the id from the same slot and no position compares equal to a ranked entry.
-__global__ void __launch_bounds__(kClodThreads) k_clod_group(ClodBuffers b)
+// uses read the id from the same slot and no position compares equal to a ranked entry.  NT
+// threads: 256 while several workgroups share a CU's LDS, 1024 when one workgroup takes most of it
+// (more wavefronts to hide the LDS atomics and the global loads).
+template <int NT>
+__global__ void __launch_bounds__(NT) k_clod_group(ClodBuffers b)
 {
 	extern __shared__ uint32_t lhash[]; // lds_slots keys, then lds_slots first positions
-	__shared__ uint32_t lds[kClodThreads / 64];
+	__shared__ uint32_t lds[NT / 64];
 	uint32_t* lkey = lhash;
 	uint32_t* lfirst = lhash + b.lds_slots;
 	const uint32_t g = blockIdx.x;
@@ -102,17 +107,17 @@ __global__ void __launch_bounds__(kClodThreads) k_clod_group(ClodBuffers b)
 	const uint32_t slots = 2 * (end - begin);
 	if (slots > b.lds_slots)
 		return;
-	for (uint32_t q = threadIdx.x; q < slots; q += kClodThreads)
+	for (uint32_t q = threadIdx.x; q < slots; q += NT)
 	{
 		lkey[q] = 0;
 		lfirst[q] = kNone;
 	}
 	__syncthreads();
-	for (uint32_t j = begin + threadIdx.x; j < end; j += kClodThreads)
+	for (uint32_t j = begin + threadIdx.x; j < end; j += NT)
 		clod_insert(lkey, lfirst, b.indices[j], j - begin, slots);
 	__syncthreads();
 	uint32_t carry = 0;
-	for (uint32_t j0 = begin; j0 < end; j0 += kClodThreads)
+	for (uint32_t j0 = begin; j0 < end; j0 += NT)
 	{
 		const uint32_t j = j0 + threadIdx.x;
 		uint32_t v = 0, slot = 0;
@@ -124,7 +129,7 @@ __global__ void __launch_bounds__(kClodThreads) k_clod_group(ClodBuffers b)
 			is_first = lfirst[slot] == j - begin;
 		}
 		uint32_t total;
-		const uint32_t id = carry + clod_block_scan(is_first ? 1u : 0u, lds, total);
+		const uint32_t id = carry + clod_block_scan<NT>(is_first ? 1u : 0u, lds, total);
 		if (is_first)
 		{
 			lfirst[slot] = kRanked | id;
@@ -207,7 +212,9 @@ uint32_t clod_lds_slots(uint32_t max_group_indices)
 {
 	// beyond 64 KB of dynamic LDS the kernel needs the runtime's permission
 	static const uint32_t cap = [] {
-		if (hipFuncSetAttribute(reinterpret_cast<const void*>(&k_clod_group), hipFuncAttributeMaxDynamicSharedMemorySize,
+		if (hipFuncSetAttribute(reinterpret_cast<const void*>(&k_clod_group<kClodThreads>), hipFuncAttributeMaxDynamicSharedMemorySize,
+		                        (int)(kClodLdsSlotsMax * 8)) == hipSuccess &&
+		    hipFuncSetAttribute(reinterpret_cast<const void*>(&k_clod_group<4 * kClodThreads>), hipFuncAttributeMaxDynamicSharedMemorySize,
 		                        (int)(kClodLdsSlotsMax * 8)) == hipSuccess)
 			return kClodLdsSlotsMax;
 		(void)hipGetLastError();
@@ -221,7 +228,11 @@ void launch_clod_group(hipStream_t s, const ClodBuffers& b, bool large_groups)
 {
 	if (!b.group_count)
 		return;
-	k_clod_group<<<b.group_count, kClodThreads, 8 * (size_t)b.lds_slots, s>>>(b);
+	// above 32 KB of hash (4096 slots) fewer than five workgroups share a CU: the wide form
+	if (b.lds_slots > 4096)
+		k_clod_group<4 * kClodThreads><<<b.group_count, 4 * kClodThreads, 8 * (size_t)b.lds_slots, s>>>(b);
+	else
+		k_clod_group<kClodThreads><<<b.group_count, kClodThreads, 8 * (size_t)b.lds_slots, s>>>(b);
 	if (large_groups)
 	{
 		k_clod_insert<<<b.group_count, kClodThreads, 0, s>>>(b);
