@@ -58,10 +58,14 @@ void launch_batch_classify(hipStream_t s, const BatchArgs& a, const BatchClasses
 void launch_batch_order(hipStream_t s, const BatchArgs& a, uint32_t* ids, const uint32_t* counts, uint32_t* temp);
 
 // the meshes of a class (ids, *count_dev of them), every one with the (vmax, imax, cmax) layout:
-// workgroup b takes mesh b (grid at least the count; the workgroups past it leave at once) or,
-// given a cursor word (cleared here on the stream), each workgroup takes meshes from the list in
-// order until it ends
+// workgroup b takes mesh b (grid at least the count; the workgroups past it leave at once)
 void launch_batch(hipStream_t s, const BatchArgs& a, uint32_t vmax, uint32_t imax, uint32_t cmax, const uint32_t* ids, const uint32_t* count_dev,
-                  size_t grid, uint32_t* cursor);
+                  size_t grid);
+// the largest class with 1024-thread workgroups (batch_wide.hip): each workgroup takes meshes from
+// the list in order through a cursor word (cleared here on the stream) until it ends
+size_t batch_wide_lds_bytes(uint32_t vmax, uint32_t imax, uint32_t cmax);
+bool batch_wide_allow_lds(size_t bytes);
+void launch_batch_wide(hipStream_t s, const BatchArgs& a, uint32_t vmax, uint32_t imax, uint32_t cmax, const uint32_t* ids, const uint32_t* count_dev,
+                       size_t grid, uint32_t* cursor);
 
 } // namespace mhip

@@ -4,42 +4,20 @@
 namespace mhip
 {
 
-// The meshes of one LDS size class; ids lists the class's *count meshes.  Without kLoop each
-// workgroup takes one mesh (the grid is an upper bound, workgroups past the count leave at once);
-// with it each workgroup takes the next mesh of the list from *cursor (zero at the launch) until
-// the list ends, so a workgroup that finishes early takes the next largest mesh (the loop is kept
-// out of the one-mesh form, where it measured 7% slower).  At most 80 VGPRs, so that six
-// wavefronts share a SIMD and six workgroups a CU when their LDS layouts allow it.
-template <bool kLoop>
-__global__ void __launch_bounds__(kBatchThreads) __attribute__((amdgpu_waves_per_eu(6))) k_simplify_patches(BatchArgs a, PatchLayout L, const uint32_t* ids, const uint32_t* count, uint32_t* cursor)
+// One workgroup per mesh of one LDS size class; ids lists the class's *count meshes (the grid
+// is an upper bound, workgroups past the count leave at once).  At most 80 VGPRs, so that six
+// wavefronts share a SIMD and six workgroups a CU when their LDS layouts allow it.  (The largest
+// class runs the 1024-thread build of the same device code, batch_wide.hip.)
+__global__ void __launch_bounds__(kBatchThreads) __attribute__((amdgpu_waves_per_eu(6))) k_simplify_patches(BatchArgs a, PatchLayout L, const uint32_t* ids, const uint32_t* count)
 {
 	extern __shared__ __align__(16) uint8_t smem[];
 	__shared__ uint32_t part[kBatchThreads / 64];
 	__shared__ uint32_t sh[16];
 	__shared__ uint64_t box[6];
 	__shared__ float shf[8];
-	__shared__ uint32_t next;
 
-	const uint32_t n = *count;
-	if (!kLoop)
-	{
-		if (blockIdx.x < n)
-			simplify_patch(a, L, ids[blockIdx.x], smem, part, sh, box, shf);
-		return;
-	}
-	for (;;)
-	{
-		// thread 0 writes `next` again only after the barrier that ends the mesh, which every
-		// thread reaches after reading it
-		if (threadIdx.x == 0)
-			next = atomicAdd(cursor, 1u);
-		__syncthreads();
-		const uint32_t i = next;
-		if (i >= n)
-			break;
-		simplify_patch(a, L, ids[i], smem, part, sh, box, shf);
-		__syncthreads();
-	}
+	if (blockIdx.x < *count)
+		simplify_patch(a, L, ids[blockIdx.x], smem, part, sh, box, shf);
 }
 
 // one slot per mesh in its class's id list; the slots of a wavefront's meshes come from one
@@ -112,7 +90,8 @@ BatchClasses batch_classes(uint32_t max_block)
 		{
 			uint32_t vcap = ((icap / vdiv[k]) + 31) & ~31u;
 			uint32_t ccap = last ? icap + 4 : batch_candidate_cap(icap);
-			if (patch_layout(vcap, icap, ccap).total > limits[k])
+			const size_t bytes = k == kBatchClasses - 1 ? batch_wide_lds_bytes(vcap, icap, ccap) : patch_layout(vcap, icap, ccap).total;
+			if (bytes > limits[k])
 				break;
 			vbest = vcap;
 			ibest = icap;
@@ -196,10 +175,8 @@ void launch_batch_order(hipStream_t s, const BatchArgs& a, uint32_t* ids, const 
 
 bool batch_allow_lds(size_t bytes)
 {
-	for (const void* f : {reinterpret_cast<const void*>(&k_simplify_patches<false>), reinterpret_cast<const void*>(&k_simplify_patches<true>)})
-		if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) != hipSuccess)
-			return false;
-	return true;
+	return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_simplify_patches), hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) ==
+	       hipSuccess;
 }
 
 size_t batch_lds_bytes(uint32_t vmax, uint32_t imax, uint32_t cmax)
@@ -208,18 +185,11 @@ size_t batch_lds_bytes(uint32_t vmax, uint32_t imax, uint32_t cmax)
 }
 
 void launch_batch(hipStream_t s, const BatchArgs& a, uint32_t vmax, uint32_t imax, uint32_t cmax, const uint32_t* ids, const uint32_t* count_dev,
-                  size_t grid, uint32_t* cursor)
+                  size_t grid)
 {
 	PatchLayout L = patch_layout(vmax, imax, cmax);
-	if (!grid)
-		return;
-	if (cursor)
-	{
-		HIP_CHECK(hipMemsetAsync(cursor, 0, sizeof(uint32_t), s));
-		k_simplify_patches<true><<<(unsigned int)grid, kBatchThreads, L.total, s>>>(a, L, ids, count_dev, cursor);
-	}
-	else
-		k_simplify_patches<false><<<(unsigned int)grid, kBatchThreads, L.total, s>>>(a, L, ids, count_dev, nullptr);
+	if (grid)
+		k_simplify_patches<<<(unsigned int)grid, kBatchThreads, L.total, s>>>(a, L, ids, count_dev);
 }
 
 } // namespace mhip

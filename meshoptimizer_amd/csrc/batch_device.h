@@ -12,10 +12,22 @@
 #include "batch.h"
 #include "quadric.h"
 
+// threads per workgroup (a multiple of 64); a translation unit may build the device code with
+// another count by defining MHIP_BATCH_THREADS first.  Everything below lives in an inline
+// namespace named for the count, so two builds linked into one library stay apart.
+#ifndef MHIP_BATCH_THREADS
+#define MHIP_BATCH_THREADS 256
+#endif
+#define MHIP_BATCH_NS_CAT(a, b) a##b
+#define MHIP_BATCH_NS(n) MHIP_BATCH_NS_CAT(threads, n)
+
 namespace mhip
 {
-
-constexpr int kBatchThreads = 256;
+inline namespace MHIP_BATCH_NS(MHIP_BATCH_THREADS)
+{
+constexpr int kBatchThreads = MHIP_BATCH_THREADS;
+static_assert(kBatchThreads % 64 == 0 && kBatchThreads >= 64 && kBatchThreads <= 1024, "workgroup of whole wavefronts");
+constexpr int kBatchWaves = kBatchThreads / 64;
 
 // kCanCollapse / kHasOpposite, src/simplifier.cpp:312-331, as 25-bit masks (bit k0 * 5 + k1)
 // so that the lookups compile to shifts instead of per-lane memory loads
@@ -126,8 +138,8 @@ inline PatchLayout patch_layout(uint32_t vmax, uint32_t imax, uint32_t cmax)
 	L.lock = take(vmax);
 	L.off = take((vmax + 2) * 2);
 	L.adj = take(imax * 4);
-	// the sort's ping-pong half and digit counts (64 offsets + 4 x 64 wavefront counts)
-	L.sortbuf = take(align16(cmax * 2) + 5 * 64 * 4);
+	// the sort's ping-pong half and digit counts (64 offsets + 64 counts per wavefront)
+	L.sortbuf = take(align16(cmax * 2) + (1 + kBatchWaves) * 64 * 4);
 	// one region for uses whose lifetimes do not overlap: the candidate arrays (pick to commit)
 	// and the scratch of setup (position hash, wedge maxima), of build_adjacency (cursor, corner
 	// ids; the ordered welded corners live on through the quadrics) and of the edge-loop update
@@ -953,7 +965,7 @@ __device__ __forceinline__ void simplify_patch(const BatchArgs& a, const PatchLa
 			for (uint32_t c = tid; c < C; c += kBatchThreads)
 				src[c] = (uint16_t)c;
 			__syncthreads();
-			uint32_t* hist = (uint32_t*)((uint8_t*)s.sortbuf + ((s.cmax * 2 + 15) & ~15u)); // 64 digit offsets + [4][64] wave counts
+			uint32_t* hist = (uint32_t*)((uint8_t*)s.sortbuf + ((s.cmax * 2 + 15) & ~15u)); // 64 digit offsets + [waves][64] counts
 			for (int pass = 0; pass < 2; ++pass)
 			{
 				const int shift = pass * 6;
@@ -991,9 +1003,9 @@ __device__ __forceinline__ void simplify_patch(const BatchArgs& a, const PatchLa
 					}
 					uint32_t lane = __lane_id();
 					uint32_t rank_in_wave = (uint32_t)__popcll(peers & ((1ull << lane) - 1ull));
-					uint32_t* wcount = hist + 64; // [4][64]
+					uint32_t* wcount = hist + 64; // [kBatchWaves][64]
 					const int wave = tid >> 6;
-					for (uint32_t b = tid; b < 4 * 64; b += kBatchThreads)
+					for (uint32_t b = tid; b < kBatchWaves * 64; b += kBatchThreads)
 						wcount[b] = 0;
 					__syncthreads();
 					if (valid && (63 - __clzll((long long)peers)) == (int)lane)
@@ -1006,7 +1018,12 @@ __device__ __forceinline__ void simplify_patch(const BatchArgs& a, const PatchLa
 						dst[hist[d] + before + rank_in_wave] = (uint16_t)item;
 					__syncthreads();
 					if (tid < 64)
-						hist[tid] += wcount[0 * 64 + tid] + wcount[1 * 64 + tid] + wcount[2 * 64 + tid] + wcount[3 * 64 + tid];
+					{
+						uint32_t all = 0;
+						for (int w = 0; w < kBatchWaves; ++w)
+							all += wcount[w * 64 + tid];
+						hist[tid] += all;
+					}
 					__syncthreads();
 				}
 				uint16_t* t = src;
@@ -1345,4 +1362,5 @@ __device__ __forceinline__ void simplify_patch(const BatchArgs& a, const PatchLa
 	PROFILE_FLUSH();
 }
 
+} // inline namespace
 } // namespace mhip

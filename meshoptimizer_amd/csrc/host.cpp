@@ -1384,10 +1384,12 @@ static int simplify_batch_on_device(Context& ctx, unsigned int* destination, uns
 		HIP_CHECK(hipDeviceGetAttribute(&max_block, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx.device));
 		BatchClasses c = batch_classes((uint32_t)max_block);
 		size_t top = 0;
-		for (int k = 0; k < kBatchClasses; ++k)
+		for (int k = 0; k < kBatchClasses - 1; ++k)
 			if (c.icap[k])
 				top = std::max(top, batch_lds_bytes(c.vcap[k], c.icap[k], c.ccap[k]));
-		if (top > 64 * 1024 && !batch_allow_lds(top))
+		const int w = kBatchClasses - 1;
+		const size_t wide = c.icap[w] ? batch_wide_lds_bytes(c.vcap[w], c.icap[w], c.ccap[w]) : 0;
+		if ((top > 64 * 1024 && !batch_allow_lds(top)) || (wide > 64 * 1024 && !batch_wide_allow_lds(wide)))
 		{
 			(void)hipGetLastError();
 			c = batch_classes(64 * 1024);
@@ -1458,8 +1460,8 @@ static int simplify_batch_on_device(Context& ctx, unsigned int* destination, uns
 		if (big_early)
 		{
 			HIP_CHECK(hipStreamWaitEvent(ctx.aux[big - 1], ctx.fork_ev, 0));
-			launch_batch(ctx.aux[big - 1], a, classes.vcap[big], classes.icap[big], classes.ccap[big], ids + (size_t)big * P, counts + big,
-			             std::min(P, cu_count), ids + 2 * (size_t)kBatchClasses * P);
+			launch_batch_wide(ctx.aux[big - 1], a, classes.vcap[big], classes.icap[big], classes.ccap[big], ids + (size_t)big * P, counts + big,
+			                  std::min(P, cu_count), ids + 2 * (size_t)kBatchClasses * P);
 			HIP_CHECK(hipEventRecord(ctx.join_ev[big - 1], ctx.aux[big - 1]));
 			HIP_CHECK(hipStreamWaitEvent(s, ctx.join_ev[big - 1], 0));
 		}
@@ -1478,7 +1480,7 @@ static int simplify_batch_on_device(Context& ctx, unsigned int* destination, uns
 			if (c)
 				HIP_CHECK(hipStreamWaitEvent(cs, ctx.fork_ev, 0));
 			if (classes.icap[k])
-				launch_batch(cs, a, classes.vcap[k], classes.icap[k], classes.ccap[k], ids + (size_t)k * P, counts + k, P, nullptr);
+				launch_batch(cs, a, classes.vcap[k], classes.icap[k], classes.ccap[k], ids + (size_t)k * P, counts + k, P);
 			if (c)
 			{
 				HIP_CHECK(hipEventRecord(ctx.join_ev[c - 1], cs));
@@ -1519,8 +1521,8 @@ static int simplify_batch_on_device(Context& ctx, unsigned int* destination, uns
 				const int c = kBatchClasses - 1;
 				HIP_CHECK(hipEventRecord(ctx.fork_ev, s));
 				HIP_CHECK(hipStreamWaitEvent(ls, ctx.fork_ev, 0));
-				launch_batch(ls, a, classes.vcap[c], classes.icap[c], classes.ccap[c], ids + (size_t)c * P, counts + c,
-				             std::min<size_t>(large, cu_count), ids + 2 * (size_t)kBatchClasses * P);
+				launch_batch_wide(ls, a, classes.vcap[c], classes.icap[c], classes.ccap[c], ids + (size_t)c * P, counts + c,
+				                  std::min<size_t>(large, cu_count), ids + 2 * (size_t)kBatchClasses * P);
 			}
 			if (!retry.empty())
 			{
@@ -1528,7 +1530,7 @@ static int simplify_batch_on_device(Context& ctx, unsigned int* destination, uns
 				const uint32_t n = (uint32_t)retry.size();
 				HIP_CHECK(hipMemcpyAsync(ids, retry.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, s));
 				HIP_CHECK(hipMemcpyAsync(counts, &n, sizeof(uint32_t), hipMemcpyHostToDevice, s));
-				launch_batch(s, a, classes.vcap[c], classes.icap[c], classes.ccap[c], ids, counts, n, nullptr);
+				launch_batch(s, a, classes.vcap[c], classes.icap[c], classes.ccap[c], ids, counts, n);
 			}
 			if (large)
 			{

@@ -94,7 +94,7 @@ def test_clod_level_mixed_group_sizes():
     from devmem import DeviceArray
     v, groups, remap, base, locks_expected = level_case(96, 0, 8, bands=[0.04, 0.14, 0.29, 0.5])
     sizes = sorted(grp.size for grp in groups)
-    assert sizes[0] < 3000 and 3000 < sizes[1] <= 6048 and sizes[-1] > 20000
+    assert sizes[0] < 3000 and 3000 < sizes[1] <= 5952 and sizes[-1] > 20000
     io = np.zeros(len(groups) + 1, np.uint32)
     io[1:] = np.cumsum([grp.size for grp in groups])
     idx = np.concatenate(groups).astype(np.uint32)

@@ -23,19 +23,19 @@ CSRC = os.path.join(os.path.dirname(HERE), "meshoptimizer_amd", "csrc")
 OUT = os.path.join(SIMT, "_build", "libsimt_batch.so")
 
 
-@pytest.fixture(scope="module")
-def simt():
+def build_simt(threads=256):
     gxx = shutil.which("g++")
     if gxx is None:
         pytest.skip("g++ not available")
     os.makedirs(os.path.dirname(OUT), exist_ok=True)
     # one build per process: parallel test workers must not load each other's half-written file
-    out = f"{OUT}.{os.getpid()}"
+    final = OUT if threads == 256 else OUT.replace(".so", f"_{threads}.so")
+    out = f"{final}.{os.getpid()}"
     cmd = [gxx, "-std=c++17", "-O1", "-g", "-fPIC", "-shared", "-ffp-contract=off", "-fno-fast-math", "-Wno-unknown-pragmas",
-           "-I", os.path.join(SIMT, "shim"), "-I", CSRC, os.path.join(SIMT, "simt_batch.cpp"), "-o", out]
+           f"-DMHIP_BATCH_THREADS={threads}", "-I", os.path.join(SIMT, "shim"), "-I", CSRC, os.path.join(SIMT, "simt_batch.cpp"), "-o", out]
     subprocess.run(cmd, check=True)
     lib = ctypes.CDLL(out)
-    os.replace(out, OUT)
+    os.replace(out, final)
     P, SZ = ctypes.c_void_p, ctypes.c_size_t
     lib.simt_simplify_batch.restype = ctypes.c_int
     lib.simt_simplify_batch.argtypes = [P, P, P, P, P, P, P, P, P, SZ, P, SZ, ctypes.c_float, ctypes.c_uint, ctypes.c_uint,
@@ -45,6 +45,17 @@ def simt():
     lib.simt_set_vertex_lock.restype = None
     lib.simt_set_vertex_lock.argtypes = [P]
     return lib
+
+
+@pytest.fixture(scope="module")
+def simt():
+    return build_simt()
+
+
+@pytest.fixture(scope="module")
+def simt_wide():
+    """The same device code built for 1024-thread workgroups (16 wavefronts)."""
+    return build_simt(1024)
 
 
 def run_simt(lib, meshes, targets, err, opts, mode=3, seed=1, locks=None, caps=None):
@@ -137,16 +148,34 @@ def test_vertex_locks(simt, seed):
     check(simt, meshes, targets, float(rng.choice([1e-2, 1e-1, 1.0])), opts, locks=locks, seed=seed)
 
 
-def test_one_per_cu_layout(simt):
-    """The largest size class (one workgroup per CU, the whole 160 KB of LDS: 2016 vertices, 6048
-    indices) holds groups of sixteen 128-triangle clusters, the clusterlod demo's default
-    (demo/nanite.cpp:80, demo/clusterlod.h:552-566); its layout runs the same device code."""
+def test_one_per_cu_layout(simt_wide):
+    """The largest size class (one 1024-thread workgroup per CU, the whole 160 KB of LDS: 1984
+    vertices, 5952 indices) holds groups of sixteen 128-triangle clusters, the clusterlod demo's
+    default (demo/nanite.cpp:80, demo/clusterlod.h:552-566)."""
     g = W.grid(31)
     s = W.seamed_grid(28, seams=3, seed=8)
-    assert g.index_count <= 6048 and s.index_count <= 6048 and s.vertex_count <= 2016
+    assert g.index_count <= 5952 and s.index_count <= 5952 and s.vertex_count <= 1984
     vs = np.ascontiguousarray(s.vertices[:, :3])
     lk = ((np.arange(s.vertex_count) % 11) == 0).astype(np.uint8) * M.SimplifyVertex_Lock
     meshes = [(g.indices, np.ascontiguousarray(g.vertices[:, :3])), (s.indices, vs)]
     targets = [W.target_count(g.index_count, 1, 2), W.target_count(s.index_count, 1, 2)]
-    check(simt, meshes, targets, 1e-2, M.SimplifyErrorAbsolute | M.SimplifyPermissive, locks=[np.zeros(g.vertex_count, np.uint8), lk],
-          caps=(2016, 6048))
+    check(simt_wide, meshes, targets, 1e-2, M.SimplifyErrorAbsolute | M.SimplifyPermissive, locks=[np.zeros(g.vertex_count, np.uint8), lk],
+          caps=(1984, 5952))
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_wide_workgroup(simt_wide, seed):
+    """With 1024 threads per workgroup every loop strides by 1024 and the sort keeps 16 wavefront
+    counts; meshes of every size (patches, random soups and grids, the largest class's layout)
+    come out exactly as the oracle's."""
+    rng = np.random.default_rng(990 + seed)
+    meshes, targets = [], []
+    for k in range(5):
+        if k % 2:
+            ib, v = random_soup(rng, nv=int(rng.integers(3, 200)), ntri=int(rng.integers(1, 400)))
+        else:
+            ib, v = random_grid(rng, n=int(rng.integers(3, 30)))
+        meshes.append((ib, np.ascontiguousarray(v[:, :3])))
+        targets.append((ib.size // 3 * int(rng.integers(0, 4)) // 4) * 3)
+    opts = [M.SimplifyErrorAbsolute | M.SimplifyPermissive, M.SimplifyLockBorder, 0][seed]
+    check(simt_wide, meshes, targets, 1e-2, opts, caps=(1984, 5952) if seed == 0 else None)
