@@ -1367,7 +1367,8 @@ static int simplify_batch_on_device(Context& ctx, unsigned int* destination, uns
                                     const unsigned int* vertex_offsets, const unsigned char* vertex_lock, size_t vertex_count, size_t stride,
                                     const unsigned int* target_index_counts,
                                     size_t patch_count, float target_error, unsigned int options, const unsigned int* h_index_offsets,
-                                    const unsigned int* h_vertex_offsets, const unsigned int* h_targets, size_t max_mesh_indices = 0)
+                                    const unsigned int* h_vertex_offsets, const unsigned int* h_targets, size_t max_mesh_indices = 0,
+                                    size_t min_mesh_indices = 0)
 {
 	hipStream_t s = ctx.stream;
 	if (stride < 12 || stride > 256 || stride % 4)
@@ -1453,8 +1454,16 @@ static int simplify_batch_on_device(Context& ctx, unsigned int* destination, uns
 		// when the host knows of a mesh beyond class 2's index cap (from host offsets or the
 		// caller's max_mesh_indices), class 3 launches at once beside the others instead
 		if (h_index_offsets)
+		{
+			min_mesh_indices = SIZE_MAX;
 			for (size_t p = 0; p < P; ++p)
+			{
 				max_mesh_indices = std::max<size_t>(max_mesh_indices, h_index_offsets[p + 1] - h_index_offsets[p]);
+				min_mesh_indices = std::min<size_t>(min_mesh_indices, h_index_offsets[p + 1] - h_index_offsets[p]);
+			}
+		}
+		// and when every mesh is beyond class 2's index cap, classes 0-2 are empty: not launched
+		const bool small_empty = min_mesh_indices > classes.icap[kBatchRetryClass];
 		const int big = kBatchClasses - 1;
 		const bool big_early = classes.icap[big] && max_mesh_indices > classes.icap[kBatchRetryClass];
 		if (big_early)
@@ -1479,7 +1488,7 @@ static int simplify_batch_on_device(Context& ctx, unsigned int* destination, uns
 			hipStream_t cs = c == 0 ? s : ctx.aux[c - 1];
 			if (c)
 				HIP_CHECK(hipStreamWaitEvent(cs, ctx.fork_ev, 0));
-			if (classes.icap[k])
+			if (classes.icap[k] && !small_empty)
 				launch_batch(cs, a, classes.vcap[k], classes.icap[k], classes.ccap[k], ids + (size_t)k * P, counts + k, P);
 			if (c)
 			{
@@ -1729,9 +1738,12 @@ extern "C" int meshopt_hip_clod_level_device(unsigned int* destination, unsigned
 			if (io[g + 1] < io[g] || (io[g + 1] - io[g]) % 3)
 				throw std::invalid_argument("meshopt_hip_clod_level_device: invalid group index counts");
 		const size_t I = io[G];
-		uint32_t max_group = 0;
+		uint32_t max_group = 0, min_group = G ? 0xffffffffu : 0;
 		for (size_t g = 0; g < G; ++g)
+		{
 			max_group = std::max(max_group, io[g + 1] - io[g]);
+			min_group = std::min(min_group, io[g + 1] - io[g]);
+		}
 
 		// carve the level's buffers (256-byte aligned)
 		size_t used = 0;
@@ -1801,7 +1813,7 @@ extern "C" int meshopt_hip_clod_level_device(unsigned int* destination, unsigned
 		// the packed groups are the meshes of one batched call (Sparse is what the packing did)
 		simplify_batch_on_device(ctx, b.packed_result, result_counts, result_errors, b.packed_indices, index_offsets, b.packed_positions, b.unique,
 		                         b.packed_locks, I, 12, target_index_counts, G, target_error, options & ~(unsigned int)kOptSparse, nullptr, nullptr,
-		                         nullptr, max_group);
+		                         nullptr, max_group, min_group);
 		launch_clod_unpack(s, b, result_counts, destination);
 		HIP_CHECK(hipStreamSynchronize(s));
 		t_last_error.clear();
