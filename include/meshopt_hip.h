@@ -155,7 +155,8 @@ meshopt_hip_server* meshopt_hip_server_create(size_t memory_budget_bytes, int sl
 /* Queues a request; its arrays must stay valid until the ticket is waited on.  Returns the
  * ticket, or 0 if the request is invalid or needs more than the whole budget. */
 size_t meshopt_hip_server_submit(meshopt_hip_server* server, const meshopt_hip_request* request);
-/* Blocks until the ticket's request has finished; fills *response.  Returns 0, or -1 on failure. */
+/* Blocks until the ticket's request has finished; fills *response and frees the ticket.  Returns 0,
+ * or -1 on failure (also for an unknown ticket, or one another thread already waits on). */
 int meshopt_hip_server_wait(meshopt_hip_server* server, size_t ticket, meshopt_hip_response* response);
 double meshopt_hip_server_clock_ms(const meshopt_hip_server* server);
 /* Finishes every queued request, then frees the server's contexts. */

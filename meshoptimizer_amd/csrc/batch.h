@@ -65,6 +65,10 @@ void launch_batch(hipStream_t s, const BatchArgs& a, uint32_t vmax, uint32_t ima
 // the list in order through a cursor word (cleared here on the stream) until it ends
 size_t batch_wide_lds_bytes(uint32_t vmax, uint32_t imax, uint32_t cmax);
 bool batch_wide_allow_lds(size_t bytes);
+#ifdef MESHOPT_BATCH_PROFILE
+// adds the 1024-thread build's phase counters (MESHOPT_BATCH_PROFILE builds) to out[32]
+void batch_wide_profile_add(unsigned long long* out, int reset);
+#endif
 void launch_batch_wide(hipStream_t s, const BatchArgs& a, uint32_t vmax, uint32_t imax, uint32_t cmax, const uint32_t* ids, const uint32_t* count_dev,
                        size_t grid, uint32_t* cursor);
 

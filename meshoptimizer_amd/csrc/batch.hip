@@ -110,6 +110,8 @@ void launch_batch_classify(hipStream_t s, const BatchArgs& a, const BatchClasses
 }
 
 #ifdef MESHOPT_BATCH_PROFILE
+// the phase counters of both builds of the device code (256 threads here, 1024 in batch_wide.hip),
+// summed
 extern "C" void meshopt_hip_batch_profile(unsigned long long* out, int reset)
 {
 	HIP_CHECK(hipMemcpyFromSymbol(out, HIP_SYMBOL(g_batch_profile), sizeof(g_batch_profile)));
@@ -118,6 +120,7 @@ extern "C" void meshopt_hip_batch_profile(unsigned long long* out, int reset)
 		unsigned long long z[32] = {};
 		HIP_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(g_batch_profile), z, sizeof(z)));
 	}
+	batch_wide_profile_add(out, reset);
 }
 #endif
 

@@ -35,6 +35,21 @@ __global__ void __launch_bounds__(kBatchThreads) k_simplify_patches_wide(BatchAr
 	}
 }
 
+#ifdef MESHOPT_BATCH_PROFILE
+void batch_wide_profile_add(unsigned long long* out, int reset)
+{
+	unsigned long long w[32];
+	HIP_CHECK(hipMemcpyFromSymbol(w, HIP_SYMBOL(g_batch_profile), sizeof(w)));
+	for (int k = 0; k < 32; ++k)
+		out[k] += w[k];
+	if (reset)
+	{
+		unsigned long long z[32] = {};
+		HIP_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(g_batch_profile), z, sizeof(z)));
+	}
+}
+#endif
+
 size_t batch_wide_lds_bytes(uint32_t vmax, uint32_t imax, uint32_t cmax)
 {
 	return patch_layout(vmax, imax, cmax).total;

@@ -241,6 +241,10 @@ void launch_clod_group(hipStream_t s, const ClodBuffers& b, bool large_groups)
 	}
 }
 
+// lockBoundary's last loop (demo/clusterlod.h:410-418) runs in index order with remap[i] <= i, so
+// when vertex i reads locks[r] & Lock for r < i, vertex r's own caller bits are already in it:
+// vertex i gets the boundary bit of its position, the caller's Lock bit of the position's first
+// vertex, and its own caller bits
 __global__ void k_clod_locks(const uint32_t* first_group, const uint32_t* last_group, const uint32_t* remap, const uint8_t* base, size_t n,
                              uint8_t* locks)
 {
@@ -248,7 +252,8 @@ __global__ void k_clod_locks(const uint32_t* first_group, const uint32_t* last_g
 	if (i < n)
 	{
 		const uint32_t r = remap[i], f = first_group[r];
-		locks[i] = (f != kNone && f != last_group[r] ? kVertexLock : 0) | (base ? base[i] : 0);
+		const uint8_t boundary = f != kNone && f != last_group[r] ? kVertexLock : 0;
+		locks[i] = boundary | (base ? (uint8_t)((base[r] & kVertexLock) | base[i]) : 0);
 	}
 }
 

@@ -1367,8 +1367,8 @@ static int simplify_batch_on_device(Context& ctx, unsigned int* destination, uns
                                     const unsigned int* vertex_offsets, const unsigned char* vertex_lock, size_t vertex_count, size_t stride,
                                     const unsigned int* target_index_counts,
                                     size_t patch_count, float target_error, unsigned int options, const unsigned int* h_index_offsets,
-                                    const unsigned int* h_vertex_offsets, const unsigned int* h_targets, size_t max_mesh_indices = 0,
-                                    size_t min_mesh_indices = 0)
+                                    const unsigned int* h_vertex_offsets, const unsigned int* h_targets,
+                                    const unsigned int* h_size_offsets = nullptr)
 {
 	hipStream_t s = ctx.stream;
 	if (stride < 12 || stride > 256 || stride % 4)
@@ -1451,21 +1451,23 @@ static int simplify_batch_on_device(Context& ctx, unsigned int* destination, uns
 		launch_batch_order(s, a, ids, counts, ids + (size_t)kBatchClasses * P);
 		ctx.ensure_aux_streams();
 		HIP_CHECK(hipEventRecord(ctx.fork_ev, s));
-		// when the host knows of a mesh beyond class 2's index cap (from host offsets or the
-		// caller's max_mesh_indices), class 3 launches at once beside the others instead
-		if (h_index_offsets)
+		// when the host knows the index counts (host offsets, or the caller's host copy of them)
+		// and one of them lies within class 3's index caps only, class 3 launches at once beside
+		// the others instead; when every mesh is beyond class 2's index cap, classes 0-2 are empty
+		// and not launched
+		const int big = kBatchClasses - 1;
+		const unsigned int* known = h_index_offsets ? h_index_offsets : h_size_offsets;
+		bool small_empty = false, big_early = false;
+		if (known && classes.icap[big])
 		{
-			min_mesh_indices = SIZE_MAX;
+			small_empty = true;
 			for (size_t p = 0; p < P; ++p)
 			{
-				max_mesh_indices = std::max<size_t>(max_mesh_indices, h_index_offsets[p + 1] - h_index_offsets[p]);
-				min_mesh_indices = std::min<size_t>(min_mesh_indices, h_index_offsets[p + 1] - h_index_offsets[p]);
+				const size_t n = known[p + 1] - known[p];
+				small_empty &= n > classes.icap[kBatchRetryClass];
+				big_early |= n > classes.icap[kBatchRetryClass] && n <= classes.icap[big];
 			}
 		}
-		// and when every mesh is beyond class 2's index cap, classes 0-2 are empty: not launched
-		const bool small_empty = min_mesh_indices > classes.icap[kBatchRetryClass];
-		const int big = kBatchClasses - 1;
-		const bool big_early = classes.icap[big] && max_mesh_indices > classes.icap[kBatchRetryClass];
 		if (big_early)
 		{
 			HIP_CHECK(hipStreamWaitEvent(ctx.aux[big - 1], ctx.fork_ev, 0));
@@ -1655,6 +1657,14 @@ extern "C" int meshopt_hip_simplify_batch(unsigned int* destination, unsigned in
 {
 	try
 	{
+		// the offsets size every staging copy and bound every kernel's reads and writes: all of
+		// them are checked here, before any copy or launch
+		if (vertex_positions_stride < 12 || vertex_positions_stride > 256 || vertex_positions_stride % 4)
+			throw std::invalid_argument("meshopt_hip_simplify_batch: invalid vertex stride");
+		for (size_t p = 0; p < mesh_count; ++p)
+			if (index_offsets[p + 1] < index_offsets[p] || vertex_offsets[p + 1] < vertex_offsets[p] ||
+			    (index_offsets[p + 1] - index_offsets[p]) % 3 || target_index_counts[p] > index_offsets[p + 1] - index_offsets[p])
+				throw std::invalid_argument("meshopt_hip_simplify_batch: invalid index count, target or offsets");
 		Context& ctx = context();
 		std::lock_guard<std::mutex> guard(ctx.mutex);
 		ctx.ensure_init();
@@ -1738,12 +1748,9 @@ extern "C" int meshopt_hip_clod_level_device(unsigned int* destination, unsigned
 			if (io[g + 1] < io[g] || (io[g + 1] - io[g]) % 3)
 				throw std::invalid_argument("meshopt_hip_clod_level_device: invalid group index counts");
 		const size_t I = io[G];
-		uint32_t max_group = 0, min_group = G ? 0xffffffffu : 0;
+		uint32_t max_group = 0;
 		for (size_t g = 0; g < G; ++g)
-		{
 			max_group = std::max(max_group, io[g + 1] - io[g]);
-			min_group = std::min(min_group, io[g + 1] - io[g]);
-		}
 
 		// carve the level's buffers (256-byte aligned)
 		size_t used = 0;
@@ -1813,7 +1820,7 @@ extern "C" int meshopt_hip_clod_level_device(unsigned int* destination, unsigned
 		// the packed groups are the meshes of one batched call (Sparse is what the packing did)
 		simplify_batch_on_device(ctx, b.packed_result, result_counts, result_errors, b.packed_indices, index_offsets, b.packed_positions, b.unique,
 		                         b.packed_locks, I, 12, target_index_counts, G, target_error, options & ~(unsigned int)kOptSparse, nullptr, nullptr,
-		                         nullptr, max_group, min_group);
+		                         nullptr, io.data());
 		launch_clod_unpack(s, b, result_counts, destination);
 		HIP_CHECK(hipStreamSynchronize(s));
 		t_last_error.clear();
@@ -1950,6 +1957,7 @@ struct ServerTicket
 	float error = 0;
 	int slot = -1;
 	bool done = false;
+	bool claimed = false; // a thread waits on it (only one may: that thread frees it)
 	std::string failure;
 };
 
@@ -2149,8 +2157,15 @@ extern "C" int meshopt_hip_server_wait(meshopt_hip_server* server, size_t ticket
 		t_last_error = "meshopt_hip_server_wait: unknown ticket";
 		return -1;
 	}
-	// the ticket object keeps its address; the map may rehash while this thread waits
+	// the ticket object keeps its address; the map may rehash while this thread waits.  Only one
+	// thread may wait on a ticket, since the waiter frees it
 	ServerTicket* tp = it->second.get();
+	if (tp->claimed)
+	{
+		t_last_error = "meshopt_hip_server_wait: another thread is already waiting on this ticket";
+		return -1;
+	}
+	tp->claimed = true;
 	srv->cv_done.wait(lk, [&] { return tp->done; });
 	it = srv->tickets.find(ticket);
 	std::unique_ptr<ServerTicket> t = std::move(it->second);

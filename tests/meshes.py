@@ -80,3 +80,28 @@ def update_case(seed):
     weights = np.array([0.5, 0.0, 2.0][:na], np.float32) if na else None
     lock = rng.integers(0, 8, size=v.shape[0]).astype(np.uint8) if seed % 5 == 0 else None
     return ib, vb, na, target, err, opts, weights, lock
+
+
+def lock_boundary(remap, groups, vertex_lock=None, prior=None):
+    """lockBoundary (demo/clusterlod.h:372-419) step by step, in the reference's loop order:
+    bit 7 marks a remapped vertex seen by an earlier group, bit 0 locks it when a later group
+    uses it again; the last loop runs in index order and writes locks[i] in place, so a vertex
+    after the first one at its position reads the first one's already-final Lock bit (caller
+    bits included).  `groups` are index arrays in group order; `prior` holds the bits an earlier
+    level left (only its Protect bit survives)."""
+    V = int(remap.size)
+    locks = np.zeros(V, np.uint8) if prior is None else prior.astype(np.uint8).copy()
+    locks &= np.uint8(~(1 | 128) & 0xFF)
+    for grp in groups:
+        for v in grp:
+            r = remap[v]
+            locks[r] |= locks[r] >> 7
+        for v in grp:
+            locks[remap[v]] |= 128
+    locks = [int(x) for x in locks]
+    for i in range(V):
+        r = int(remap[i])
+        locks[i] = (locks[r] & 1) | (locks[i] & 2)
+        if vertex_lock is not None:
+            locks[i] |= int(vertex_lock[i])
+    return np.array(locks, np.uint8)

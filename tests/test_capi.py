@@ -65,3 +65,29 @@ def test_index_adapters_compile(tmp_path):
     res = subprocess.run(["g++", "-std=c++11", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), "-c", str(src),
                           "-o", str(tmp_path / "adapters.o")], capture_output=True, text=True)
     assert res.returncode == 0, res.stderr
+
+
+def test_batch_rejects_bad_offsets_before_any_copy():
+    """meshopt_hip_simplify_batch checks every host offset before it stages or launches anything:
+    offsets that are not monotone (mesh 0 would read 3000 indices where 30 are staged), an index
+    count that is not a multiple of 3, and a target above its mesh's index count are rejected with
+    an error.  The checks precede every HIP call, so they run on a host without a GPU."""
+    if not os.path.exists(M.LIB_PATH):
+        pytest.skip("libmeshopt_hip.so not built")
+    import numpy as np
+    lib = M.library()
+    idx = np.zeros(30, np.uint32)
+    verts = np.zeros((8, 3), np.float32)
+    dest = np.zeros(3000, np.uint32)
+    counts = np.zeros(2, np.uint32)
+    cases = [
+        (np.array([0, 3000, 30], np.uint32), np.array([0, 4, 8], np.uint32), np.array([0, 0], np.uint32)),
+        (np.array([0, 10, 30], np.uint32), np.array([0, 4, 8], np.uint32), np.array([0, 0], np.uint32)),
+        (np.array([0, 15, 30], np.uint32), np.array([0, 4, 8], np.uint32), np.array([18, 0], np.uint32)),
+        (np.array([0, 15, 30], np.uint32), np.array([0, 8, 4], np.uint32), np.array([0, 0], np.uint32)),
+    ]
+    for io, vo, tg in cases:
+        rc = lib.meshopt_hip_simplify_batch(dest.ctypes.data, counts.ctypes.data, None, idx.ctypes.data, io.ctypes.data,
+                                            verts.ctypes.data, vo.ctypes.data, None, 12, tg.ctypes.data, 2, 1e-2, 0)
+        assert rc == -1
+        assert "offsets" in lib.meshopt_hip_last_error().decode()

@@ -7,7 +7,7 @@ import sys
 import numpy as np
 import pytest
 
-from meshes import same_bits
+from meshes import lock_boundary, same_bits
 from oracle import oracle as O
 import meshoptimizer_amd as M
 from meshoptimizer_amd import workloads as W
@@ -25,7 +25,8 @@ def announce(what):
 def level_case(n, tiles, seed, bands=None):
     """A seamed grid's positions, its triangles split into tiles x tiles spatial groups (or, with
     bands, into vertical bands cut at those x fractions), in shuffled group order, the position
-    remap, random caller lock/protect bits, and the expected locks."""
+    remap, random caller lock/protect bits, and the expected locks: lockBoundary's own loops
+    restated step by step (tests/meshes.py)."""
     rng = np.random.default_rng(seed)
     m = W.seamed_grid(n, seams=4, seed=seed)
     v = np.ascontiguousarray(m.vertices[:, :3])
@@ -41,10 +42,7 @@ def level_case(n, tiles, seed, bands=None):
     _, first, inv = np.unique(v, axis=0, return_index=True, return_inverse=True)
     remap = first[inv.reshape(-1)].astype(np.uint32)
     base = ((rng.random(v.shape[0]) < 0.05) * M.SimplifyVertex_Lock + (rng.random(v.shape[0]) < 0.1) * M.SimplifyVertex_Protect).astype(np.uint8)
-    uses = np.zeros(v.shape[0], np.int32)
-    for grp in groups:
-        uses[np.unique(remap[grp])] += 1
-    locks = ((uses >= 2)[remap] * M.SimplifyVertex_Lock).astype(np.uint8) | base
+    locks = lock_boundary(remap, groups, base)
     return v, groups, remap, base, locks
 
 

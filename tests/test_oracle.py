@@ -274,34 +274,31 @@ def test_update_closed_form():
 @pytest.mark.parametrize("seed", range(8))
 def test_boundary_locks_rule(seed):
     """lockBoundary (demo/clusterlod.h:372-419) marks, group by group, every remapped vertex already
-    seen by an earlier group (bit 7 as 'seen', bit 0 as the lock); the device computes the same set
-    as 'used by two or more groups' (the lowest group that uses it and any other).  Both forms on
-    random group sets, with the caller's bits carried through as the reference carries them."""
+    seen by an earlier group (bit 7 as 'seen', bit 0 as the lock), then rewrites every vertex in
+    index order; the device computes the same bytes as 'the position is used by two or more groups'
+    (the lowest group that uses it and any other), plus the caller's Lock bit of the position's
+    first vertex, plus the vertex's own caller bits.  Both forms on random group sets, and the form
+    without the first vertex's Lock bit must differ from the loop somewhere."""
+    from meshes import lock_boundary
     rng = np.random.default_rng(seed)
     V = int(rng.integers(20, 200))
     remap = np.arange(V)
-    dup = rng.random(V) < 0.2
+    dup = rng.random(V) < 0.3
     remap[dup] = rng.integers(0, V, size=int(dup.sum()))
     remap = np.minimum(remap, np.arange(V))  # a remap points at a lower (or the same) index
     remap = remap[remap]                     # and at a primary
     groups = [rng.integers(0, V, size=3 * int(rng.integers(1, 30))) for _ in range(int(rng.integers(1, 12)))]
     base = rng.integers(0, 8, size=V).astype(np.uint8)
+    prior = (rng.integers(0, 256, size=V).astype(np.uint8) & 2)  # earlier levels' protect bits
 
-    # the reference's loop, step by step
-    locks = (rng.integers(0, 256, size=V).astype(np.uint8) & 2) | 0  # earlier levels' protect bits
-    prior = locks.copy()
-    locks &= ~np.uint8(1 | 128)
-    for grp in groups:
-        for v in grp:
-            r = remap[v]
-            locks[r] |= locks[r] >> 7
-        for v in grp:
-            locks[remap[v]] |= 128
-    ref = np.array([(locks[remap[i]] & 1) | (prior[i] & 2) | base[i] for i in range(V)], np.uint8)
+    ref = lock_boundary(remap, groups, base, prior)
 
-    # the device's rule
     uses = np.zeros(V, np.int32)
     for grp in groups:
         uses[np.unique(remap[grp])] += 1
-    dev = ((uses >= 2)[remap].astype(np.uint8)) | (prior & 2) | base
+    boundary = (uses >= 2)[remap].astype(np.uint8)
+    dev = boundary | (base[remap] & 1) | (prior & 2) | base
     assert np.array_equal(ref, dev)
+    without_first = boundary | (prior & 2) | base
+    gains = (remap != np.arange(V)) & ((base[remap] & 1) == 1) & ((base & 1) == 0) & (boundary == 0)
+    assert np.array_equal(np.flatnonzero(ref != without_first), np.flatnonzero(gains))
