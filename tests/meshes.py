@@ -105,3 +105,62 @@ def lock_boundary(remap, groups, vertex_lock=None, prior=None):
         if vertex_lock is not None:
             locks[i] |= int(vertex_lock[i])
     return np.array(locks, np.uint8)
+
+
+EDGE_KINDS = ("signed_zero", "nan", "denormal", "tiny", "mixed")
+
+
+def edge_value_case(seed):
+    """A mesh whose positions (and attribute columns 3..) hold the IEEE edge values the simplifier
+    treats specially (src/simplifier.cpp:111-148 hashes -0 as +0, 201-240 compares with ==, so +-0
+    weld and NaN never does; 572-573 keep the first of +-0 in the box and skip NaN):
+      signed_zero  every zero coordinate negated at random, so positions that differ only in the
+                   sign of a zero coincide (wedges and seams across the sign);
+      nan          a few vertices with a NaN coordinate, a few NaN attribute values;
+      denormal     some coordinates and attributes replaced by denormals (2^-149 .. 2^-127);
+      tiny         the whole mesh scaled into the denormal range (the box extent is a denormal,
+                   1/extent overflows to inf);
+      mixed        all of the above at once.
+    Returns (indices, vertices [V, 4]: xyz + one attribute column, kind)."""
+    rng = np.random.default_rng(90000 + seed)
+    kind = EDGE_KINDS[seed % len(EDGE_KINDS)]
+    if seed % 2:
+        ib, v = random_soup(rng, nv=int(rng.integers(12, 150)), ntri=int(rng.integers(8, 300)))
+        v[:, :3] -= np.float32(0.5)  # quarters in [-0.5, 0.5): zeros on every axis
+    else:
+        ib, v = random_grid(rng, n=int(rng.integers(4, 16)))
+        v = np.ascontiguousarray(v[:, :4])
+    v = np.ascontiguousarray(v[:, :4]).copy()
+    V = v.shape[0]
+    u = v.view(np.uint32)
+    if kind in ("signed_zero", "mixed"):
+        flip = (v[:, :3] == 0) & (rng.random((V, 3)) < 0.5)
+        u[:, :3][flip] |= np.uint32(0x80000000)
+        # some exact duplicates of zero-carrying vertices with the other sign
+        zero_rows = np.flatnonzero((v[:, :3] == 0).any(axis=1))
+        if zero_rows.size:
+            pick = rng.choice(zero_rows, size=min(zero_rows.size, 6), replace=False)
+            dup = v[pick].copy()
+            du = dup.view(np.uint32)
+            du[:, :3][dup[:, :3] == 0] ^= np.uint32(0x80000000)
+            v = np.concatenate([v, dup])
+            u = v.view(np.uint32)
+            tri = ib.reshape(-1, 3).copy()
+            for k, src in enumerate(pick):
+                hit = rng.random(tri.shape) < 0.5
+                tri[(tri == src) & hit] = V + k
+            ib = tri.reshape(-1).astype(np.uint32)
+            V = v.shape[0]
+    if kind in ("nan", "mixed"):
+        rows = rng.choice(V, size=max(1, V // 25), replace=False)
+        v[rows, rng.integers(0, 3, size=rows.size)] = np.nan
+        arows = rng.choice(V, size=max(1, V // 30), replace=False)
+        v[arows, 3] = np.nan
+    if kind in ("denormal", "mixed"):
+        den = (rng.integers(1, 1 << 23, size=(V, 4)).astype(np.uint32) | (rng.integers(0, 2, size=(V, 4)).astype(np.uint32) << 31))
+        mask = rng.random((V, 4)) < 0.15
+        u[mask] = den[mask]
+    if kind == "tiny":
+        v[:, :3] *= np.float32(2.0 ** -130)
+        v[:, 3] *= np.float32(2.0 ** -128)
+    return ib.astype(np.uint32), np.ascontiguousarray(v), kind

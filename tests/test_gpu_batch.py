@@ -6,7 +6,7 @@ import sys
 import numpy as np
 import pytest
 
-from meshes import random_grid, random_soup, same_bits, scattered_components
+from meshes import edge_value_case, random_grid, random_soup, same_bits, scattered_components
 from oracle import oracle as O
 import meshoptimizer_amd as M
 from meshoptimizer_amd import workloads as W
@@ -235,3 +235,17 @@ def test_clusterlod_level_groups():
         assert np.array_equal(order[rg], ro), f"group {p}: indices differ from the Sparse call"
         assert same_bits(eg, eo), f"group {p}: result_error {eg!r} != {eo!r}"
         assert rg.size < grp.size
+
+
+@pytest.mark.parametrize("opts", [0, M.SimplifyLockBorder, M.SimplifyErrorAbsolute | M.SimplifyPermissive, M.SimplifyRegularize])
+def test_edge_value_batch(opts):
+    """NaN, +-0 and denormal positions in the batched kernel's LDS remap and box (one batch of 40
+    meshes, every edge-value kind), each mesh against the oracle alone."""
+    meshes, targets = [], []
+    for seed in range(40):
+        ib, v, kind = edge_value_case(seed)
+        rng = np.random.default_rng(seed)
+        meshes.append((ib, np.ascontiguousarray(v[:, :3])))
+        targets.append((ib.size // 3 * int(rng.integers(0, 4)) // 4) * 3)
+    announce(f"edge value batch opts={opts}")
+    check_batch(meshes, targets, 1e-1, opts)

@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 
 import kat
-from meshes import scattered_components, random_grid, random_soup, same_bits
+from meshes import edge_value_case, scattered_components, random_grid, random_soup, same_bits
 from oracle import oracle as O
 import meshoptimizer_amd as M
 from meshoptimizer_amd import workloads as W
@@ -127,3 +127,22 @@ def test_prune_components(seed):
     opts = M.SimplifyPrune | int(rng.choice([0, M.SimplifyLockBorder, M.SimplifySparse, M.SimplifyErrorAbsolute]))
     announce(f"prune seed={seed} I={ib.size} V={v.shape[0]} opts={opts} err={err}")
     assert_parity(ib, v, target, err, opts)
+
+
+@pytest.mark.parametrize("seed", range(40))
+def test_edge_values(seed):
+    """NaN, +-0 and denormal coordinates and attributes (tests/meshes.py edge_value_case): the
+    position remap's -0 hashing and == welding (src/simplifier.cpp:111-148, 201-240), the box's
+    first-of-+-0 and NaN-skipping compares (572-573), denormals kept through every kernel, and a
+    denormal extent whose reciprocal overflows.  Same parameters as the oracle's pin against the
+    reference library."""
+    ib, v, kind = edge_value_case(seed)
+    rng = np.random.default_rng(seed)
+    opts = [0, 1, 2, 4, 16, 32, 64, 1 | 32, 2 | 8, 8, 1 | 8, 32 | 16][seed % 12]
+    target = (ib.size // 3 * int(rng.integers(0, 4)) // 4) * 3
+    err = float(rng.choice([1e-3, 1e-2, 1e-1, 1.0]))
+    attrs = weights = None
+    if seed % 3 == 0:
+        attrs, weights = v[:, 3:4], np.array([0.5], np.float32)
+    announce(f"edge values seed={seed} kind={kind} opts={opts} I={ib.size} V={v.shape[0]}")
+    assert_parity(ib, v, target, err, opts, attrs, weights)

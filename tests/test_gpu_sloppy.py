@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 import kat
-from meshes import random_grid, random_soup, same_bits
+from meshes import edge_value_case, random_grid, random_soup, same_bits
 from oracle import oracle as O
 import meshoptimizer_amd as M
 from meshoptimizer_amd import workloads as W
@@ -84,3 +84,16 @@ def test_sloppy_device_entry():
         assert np.array_equal(rg, rh) and same_bits(e, eh)
         with pytest.raises(RuntimeError, match="vertex_count"):
             M.simplify_sloppy_device(d_dst.ptr, d_bad.ptr, bad.size, d_v.ptr, v.shape[0], 12, None, target, 1e-2)
+
+
+@pytest.mark.parametrize("seed", range(20))
+def test_sloppy_edge_values(seed):
+    """NaN, +-0 and denormal positions: cell ids of NaN and denormal coordinates, and the rule that
+    a NaN error at a cell's first vertex keeps that vertex (the reference's comparisons)."""
+    ib, v, kind = edge_value_case(seed)
+    rng = np.random.default_rng(seed)
+    v = np.ascontiguousarray(v[:, :3])
+    target = (ib.size // 3 * int(rng.integers(0, 4)) // 4) * 3
+    err = float(rng.choice([1e-3, 1e-2, 1e-1, 1.0]))
+    announce(f"sloppy edge values seed={seed} kind={kind} I={ib.size} target={target} err={err}")
+    assert_parity(ib, v, target, err)

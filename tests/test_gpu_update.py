@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 import kat
-from meshes import same_bits, update_case
+from meshes import edge_value_case, same_bits, update_case
 from oracle import oracle as O
 import meshoptimizer_amd as M
 from meshoptimizer_amd import workloads as W
@@ -149,3 +149,18 @@ def test_update_device_rejects_bad_index():
         with pytest.raises(RuntimeError, match="vertex_count"):
             M.simplify_with_update_device(d_ib.ptr, bad.size, d_vb.ptr, vb.shape[0], 12, bad.size // 4, 1e-2)
         assert d_vb.to_host(np.float32, vb.size).tobytes() == vb.tobytes()
+
+
+@pytest.mark.parametrize("seed", range(20))
+def test_update_edge_values(seed):
+    """NaN, +-0 and denormal inputs through the solve, with the oracle's parameters of the same
+    seed (pinned to the reference library there): indices, error and every vertex byte."""
+    ib, v, kind = edge_value_case(seed)
+    rng = np.random.default_rng(500 + seed)
+    na = int(seed % 2)
+    target = (ib.size // 3 * int(rng.integers(0, 4)) // 4) * 3
+    err = float(rng.choice([1e-2, 1e-1, 1.0]))
+    opts = [0, 2, 8, 1][seed % 4]
+    w = np.array([0.5], np.float32) if na else None
+    announce(f"update edge values seed={seed} kind={kind} I={ib.size} attrs={na} opts={opts}")
+    assert_parity(ib, v, na, target, err, opts, w, None)

@@ -179,3 +179,17 @@ def test_wide_workgroup(simt_wide, seed):
         targets.append((ib.size // 3 * int(rng.integers(0, 4)) // 4) * 3)
     opts = [M.SimplifyErrorAbsolute | M.SimplifyPermissive, M.SimplifyLockBorder, 0][seed]
     check(simt_wide, meshes, targets, 1e-2, opts, caps=(1984, 5952) if seed == 0 else None)
+
+
+@pytest.mark.parametrize("opts", [0, M.SimplifyLockBorder | M.SimplifyPermissive])
+def test_edge_values(simt, opts):
+    """NaN, +-0 and denormal positions (tests/meshes.py edge_value_case) through the LDS position
+    remap and box of the batched device code, each mesh against the oracle."""
+    from meshes import edge_value_case
+    meshes, targets = [], []
+    for seed in range(20):
+        ib, v, kind = edge_value_case(seed)
+        rng = np.random.default_rng(seed)
+        meshes.append((ib, np.ascontiguousarray(v[:, :3])))
+        targets.append((ib.size // 3 * int(rng.integers(0, 4)) // 4) * 3)
+    check(simt, meshes, targets, 1e-1, opts, seed=int(opts) + 3)
