@@ -1362,5 +1362,28 @@ __device__ __forceinline__ void simplify_patch(const BatchArgs& a, const PatchLa
 	PROFILE_FLUSH();
 }
 
+// The meshes ids[0..n) taken from a cursor word shared by the launch (zero at its start): thread 0
+// takes the next list position with one atomic and hands it to the workgroup through the LDS word
+// `next`; the next mesh starts only after the barrier that ends the last one, which every thread
+// reaches after reading `next`, so thread 0 never overwrites a value still unread.  The position
+// read back is made wave-uniform (every thread reads the same word after the barrier, so the
+// first lane's value is every lane's): the loop exit and every barrier loop inside simplify_patch
+// then sit in uniform control flow, as in the one-mesh-per-workgroup form.
+__device__ __forceinline__ void simplify_from_cursor(const BatchArgs& a, const PatchLayout& L, const uint32_t* ids, uint32_t n, uint32_t* cursor,
+                                                     uint8_t* smem, uint32_t* part, uint32_t* sh, uint64_t* box, float* shf, uint32_t* next)
+{
+	for (;;)
+	{
+		if (threadIdx.x == 0)
+			*next = atomicAdd(cursor, 1u);
+		__syncthreads();
+		const uint32_t i = __builtin_amdgcn_readfirstlane(*next);
+		if (i >= n)
+			break;
+		simplify_patch(a, L, ids[i], smem, part, sh, box, shf);
+		__syncthreads();
+	}
+}
+
 } // inline namespace
 } // namespace mhip

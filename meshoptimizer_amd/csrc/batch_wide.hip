@@ -8,7 +8,7 @@ namespace mhip
 {
 
 // each workgroup takes the next mesh of the list from *cursor (zero at the launch) until the
-// list ends, so a workgroup that finishes early takes the next largest mesh
+// list ends, so a workgroup that finishes early takes the next largest mesh (simplify_from_cursor)
 __global__ void __launch_bounds__(kBatchThreads) k_simplify_patches_wide(BatchArgs a, PatchLayout L, const uint32_t* ids, const uint32_t* count,
                                                                          uint32_t* cursor)
 {
@@ -19,20 +19,7 @@ __global__ void __launch_bounds__(kBatchThreads) k_simplify_patches_wide(BatchAr
 	__shared__ float shf[8];
 	__shared__ uint32_t next;
 
-	const uint32_t n = *count;
-	for (;;)
-	{
-		// thread 0 writes `next` again only after the barrier that ends the mesh, which every
-		// thread reaches after reading it
-		if (threadIdx.x == 0)
-			next = atomicAdd(cursor, 1u);
-		__syncthreads();
-		const uint32_t i = next;
-		if (i >= n)
-			break;
-		simplify_patch(a, L, ids[i], smem, part, sh, box, shf);
-		__syncthreads();
-	}
+	simplify_from_cursor(a, L, ids, *count, cursor, smem, part, sh, box, shf, &next);
 }
 
 #ifdef MESHOPT_BATCH_PROFILE

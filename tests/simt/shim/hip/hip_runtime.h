@@ -108,6 +108,8 @@ inline T readlane(T v, unsigned int j, int line)
 #define __shfl_xor(v, m, w) ::simt::shfl_xor((v), (m), __LINE__)
 #define __ballot(p) ::simt::ballot((p), __LINE__)
 #define __builtin_amdgcn_readlane(v, j) ::simt::readlane((v), (j), __LINE__)
+// every fiber of the wavefront must reach it (all lanes are active in this model: lane 0 is first)
+#define __builtin_amdgcn_readfirstlane(v) ::simt::readlane((v), 0u, __LINE__)
 #define __builtin_amdgcn_wave_barrier() ((void)::simt::wave_sync(0, __LINE__))
 #define __builtin_amdgcn_fence(...) ((void)0)
 #define __lane_id() (::simt::lane())

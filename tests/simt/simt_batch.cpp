@@ -9,6 +9,7 @@
 #include <ucontext.h>
 #include <unistd.h>
 
+#include <memory>
 #include <vector>
 
 #include "batch_device.h"
@@ -34,8 +35,11 @@ struct Fiber
 	std::vector<char> stack;
 };
 
+constexpr int kMaxGroups = 8;
+
 struct Group
 {
+	int id = 0;
 	int threads = 0;
 	std::vector<Fiber> fibers;
 	int current = -1;
@@ -48,7 +52,10 @@ struct Group
 	std::vector<uint64_t> wave_buf; // [waves][2][64]
 };
 
-static Group g;
+// the workgroup whose fibers run now (several may be in flight, interleaved sweep by sweep)
+static Group g_groups[kMaxGroups];
+static Group* gp = &g_groups[0];
+#define g (*gp)
 static volatile sig_atomic_t g_watchdog = 0;
 
 const Dim& thread_idx()
@@ -114,30 +121,40 @@ const uint64_t* wave_sync(uint64_t v, int line)
 	return buf;
 }
 
+// what a workgroup runs: one mesh (simplify_patch), or meshes from a shared cursor
+// (simplify_from_cursor, the loop of the largest class's kernel)
 struct Task
 {
 	const mhip::BatchArgs* a;
 	const mhip::PatchLayout* L;
-	uint32_t patch;
+	uint32_t patch; // one-mesh task
+	const uint32_t* ids; // cursor task: the list, its length and the shared cursor
+	uint32_t n;
+	uint32_t* cursor;
 	uint8_t* smem;
 	uint32_t* part;
 	uint32_t* sh;
 	uint64_t* box;
 	float* shf;
+	uint32_t* next;
 };
 
-static Task g_task;
+static Task g_tasks[kMaxGroups];
 
 static void fiber_main()
 {
-	mhip::simplify_patch(*g_task.a, *g_task.L, g_task.patch, g_task.smem, g_task.part, g_task.sh, g_task.box, g_task.shf);
+	const Task& t = g_tasks[g.id];
+	if (t.ids)
+		mhip::simplify_from_cursor(*t.a, *t.L, t.ids, t.n, t.cursor, t.smem, t.part, t.sh, t.box, t.shf, t.next);
+	else
+		mhip::simplify_patch(*t.a, *t.L, t.patch, t.smem, t.part, t.sh, t.box, t.shf);
 	g.fibers[g.current].state = kDone;
 	swapcontext(&g.fibers[g.current].ctx, &g.scheduler);
 }
 
 static void report(const char* why)
 {
-	fprintf(stderr, "simt: %s (patch %u)\n", why, g_task.patch);
+	fprintf(stderr, "simt: %s (workgroup %d, patch %u)\n", why, g.id, g_tasks[g.id].patch);
 	for (int t = 0; t < g.threads; ++t)
 	{
 		const Fiber& f = g.fibers[t];
@@ -194,21 +211,19 @@ static uint64_t next_random()
 	return g_rng;
 }
 
-// runs one workgroup of `threads` fibers over one mesh; returns 0, or -1 on a deadlock.  With
-// shuffle set, every scheduling sweep visits the fibers in a fresh random order (a different
-// interleaving of the threads between rendezvous points).
-static int run_group(int threads, bool shuffle)
+static void init_group(Group& grp, int id, int threads)
 {
-	g.threads = threads;
-	g.fibers.resize(threads);
+	grp.id = id;
+	grp.threads = threads;
+	grp.fibers.resize(threads);
 	const int waves = (threads + 63) / 64;
-	g.wave_count.assign(waves, 0);
-	g.wave_gen.assign(waves, 0);
-	g.wave_buf.assign((size_t)waves * 2 * 64, 0);
-	g.barrier_count = 0;
+	grp.wave_count.assign(waves, 0);
+	grp.wave_gen.assign(waves, 0);
+	grp.wave_buf.assign((size_t)waves * 2 * 64, 0);
+	grp.barrier_count = 0;
 	for (int t = 0; t < threads; ++t)
 	{
-		Fiber& f = g.fibers[t];
+		Fiber& f = grp.fibers[t];
 		f.idx = Dim{(unsigned int)t, 0, 0};
 		f.state = kRunnable;
 		f.line = 0;
@@ -220,40 +235,93 @@ static int run_group(int threads, bool shuffle)
 		f.ctx.uc_link = nullptr;
 		makecontext(&f.ctx, fiber_main, 0);
 	}
-	std::vector<int> order(threads);
-	for (int t = 0; t < threads; ++t)
-		order[t] = t;
+}
+
+// one scheduling sweep over the current group's runnable fibers (in a fresh random order with
+// shuffle set); sets done when every fiber has finished, returns whether any fiber ran
+static bool sweep(std::vector<int>& order, bool shuffle, bool& done)
+{
+	const int threads = g.threads;
+	bool progressed = false, all_done = true;
+	if (shuffle)
+		for (int t = threads - 1; t > 0; --t)
+		{
+			int u = (int)(next_random() % (uint64_t)(t + 1));
+			int x = order[t];
+			order[t] = order[u];
+			order[u] = x;
+		}
+	for (int i = 0; i < threads; ++i)
+	{
+		const int t = order[i];
+		Fiber& f = g.fibers[t];
+		if (f.state != kDone)
+			all_done = false;
+		if (f.state != kRunnable)
+			continue;
+		g.current = t;
+		swapcontext(&g.scheduler, &f.ctx);
+		progressed = true;
+	}
+	done = all_done;
+	return progressed;
+}
+
+// runs `groups` workgroups of `threads` fibers each (their tasks in g_tasks), interleaved: every
+// round visits the unfinished groups in a random order (with shuffle set) and gives each one
+// sweep, so the groups' atomics on shared words interleave in varying orders.  Returns 0, or -1
+// on a deadlock within a group.
+static int run_groups(int groups, int threads, bool shuffle)
+{
+	std::vector<std::vector<int>> orders(groups, std::vector<int>(threads));
+	std::vector<char> finished(groups, 0);
+	for (int k = 0; k < groups; ++k)
+	{
+		init_group(g_groups[k], k, threads);
+		for (int t = 0; t < threads; ++t)
+			orders[k][t] = t;
+	}
+	std::vector<int> visit(groups);
+	for (int k = 0; k < groups; ++k)
+		visit[k] = k;
 	for (;;)
 	{
-		bool progressed = false, all_done = true;
 		if (shuffle)
-			for (int t = threads - 1; t > 0; --t)
+			for (int k = groups - 1; k > 0; --k)
 			{
-				int u = (int)(next_random() % (uint64_t)(t + 1));
-				int x = order[t];
-				order[t] = order[u];
-				order[u] = x;
+				int u = (int)(next_random() % (uint64_t)(k + 1));
+				int x = visit[k];
+				visit[k] = visit[u];
+				visit[u] = x;
 			}
-		for (int i = 0; i < threads; ++i)
+		bool any_left = false;
+		for (int k : visit)
 		{
-			const int t = order[i];
-			Fiber& f = g.fibers[t];
-			if (f.state != kDone)
-				all_done = false;
-			if (f.state != kRunnable)
+			if (finished[k])
 				continue;
-			g.current = t;
-			swapcontext(&g.scheduler, &f.ctx);
-			progressed = true;
+			gp = &g_groups[k];
+			bool done = false;
+			const bool progressed = sweep(orders[k], shuffle, done);
+			if (done)
+			{
+				finished[k] = 1;
+				continue;
+			}
+			any_left = true;
+			if (!progressed)
+			{
+				report("deadlock: no thread can proceed");
+				return -1;
+			}
 		}
-		if (all_done)
+		if (!any_left)
 			return 0;
-		if (!progressed)
-		{
-			report("deadlock: no thread can proceed");
-			return -1;
-		}
 	}
+}
+
+static int run_group(int threads, bool shuffle)
+{
+	return run_groups(1, threads, shuffle);
 }
 
 } // namespace simt
@@ -332,7 +400,7 @@ extern "C" int simt_simplify_batch(unsigned int* destination, unsigned int* resu
 				for (auto& w : part)
 					w = (uint32_t)simt::next_random();
 			}
-			simt::g_task = simt::Task{&a, &L, (uint32_t)p, smem, part, sh, box, shf};
+			simt::g_tasks[0] = simt::Task{&a, &L, (uint32_t)p, nullptr, 0, nullptr, smem, part, sh, box, shf, nullptr};
 			rc = simt::run_group(mhip::kBatchThreads, (mode & 2) != 0);
 			if (status[p] != 5)
 				break;
@@ -353,4 +421,81 @@ extern "C" unsigned int simt_layout_bytes(unsigned int vmax, unsigned int imax, 
 extern "C" unsigned int simt_retries()
 {
 	return simt::g_retries;
+}
+
+// The largest class's wrapper (simplify_from_cursor): `groups` workgroups, each with its own LDS
+// image (full candidate arrays, as that class holds them; random bytes at the start, then whatever
+// the workgroup's previous mesh left), take the meshes ids[0..count) from one shared cursor,
+// interleaved sweep by sweep in random orders.  Every mesh must be taken exactly once (status
+// words start at 99) and the cursor must end at count + groups (each workgroup's last take is the
+// one that finds the list empty).  Returns 0, -1 on a deadlock, -2 if the cursor is off.
+extern "C" int simt_simplify_batch_cursor(unsigned int* destination, unsigned int* result_counts, float* result_errors, unsigned int* status,
+                                          const unsigned int* ids, const unsigned int* indices, const unsigned int* index_offsets,
+                                          const float* positions, const unsigned int* vertex_offsets, size_t stride, const unsigned int* targets,
+                                          size_t count, float target_error, unsigned int options, unsigned int vmax, unsigned int imax,
+                                          int groups, int watchdog_s, unsigned long long seed)
+{
+	if (groups < 1 || groups > simt::kMaxGroups)
+		return -3;
+	simt::g_rng = seed * 0x9E3779B97F4A7C15ull + 1;
+	mhip::BatchArgs a = {};
+	a.destination = destination;
+	a.result_counts = result_counts;
+	a.result_errors = result_errors;
+	a.status = status;
+	a.passes = nullptr;
+	a.indices = indices;
+	a.index_offsets = index_offsets;
+	a.positions = positions;
+	a.vertex_offsets = vertex_offsets;
+	a.vertex_lock = g_vertex_lock;
+	a.vertex_count = count ? vertex_offsets[count] : 0;
+	std::vector<mhip::Quad> quadrics(a.vertex_count + 1);
+	for (auto& q : quadrics)
+		q.a00 = q.a11 = q.a22 = q.a10 = q.a20 = q.a21 = q.b0 = q.b1 = q.b2 = q.c = q.w = q.pad = (float)(simt::next_random() % 1000) - 500.f;
+	a.quadrics = quadrics.data();
+	a.position_stride_floats = stride / sizeof(float);
+	a.target_index_counts = targets;
+	a.patch_count = count;
+	a.target_error = target_error;
+	a.options = options;
+	const mhip::PatchLayout L = mhip::patch_layout(vmax, imax, imax + 4);
+	std::vector<std::unique_ptr<simt::GuardedImage>> images;
+	struct Shared
+	{
+		uint32_t part[mhip::kBatchThreads / 64];
+		uint32_t sh[16];
+		uint64_t box[6];
+		float shf[8];
+		uint32_t next;
+	};
+	std::vector<Shared> shared(groups);
+	uint32_t cursor = 0;
+	for (int k = 0; k < groups; ++k)
+	{
+		images.emplace_back(new simt::GuardedImage(L.total));
+		uint8_t* smem = images.back()->data;
+		for (size_t b = 0; b < L.total; ++b)
+			smem[b] = (uint8_t)simt::next_random();
+		Shared& sd = shared[k];
+		for (auto& w : sd.sh)
+			w = (uint32_t)simt::next_random();
+		for (auto& w : sd.part)
+			w = (uint32_t)simt::next_random();
+		sd.next = (uint32_t)simt::next_random();
+		simt::g_tasks[k] = simt::Task{&a, &L, 0xffffffffu, ids, (uint32_t)count, &cursor, smem, sd.part, sd.sh, sd.box, sd.shf, &sd.next};
+	}
+	signal(SIGSEGV, simt::on_segv);
+	signal(SIGBUS, simt::on_segv);
+	if (watchdog_s > 0)
+	{
+		signal(SIGALRM, simt::on_alarm);
+		alarm((unsigned int)watchdog_s);
+	}
+	int rc = simt::run_groups(groups, mhip::kBatchThreads, true);
+	if (watchdog_s > 0)
+		alarm(0);
+	if (rc == 0 && cursor != (uint32_t)count + (uint32_t)groups)
+		rc = -2;
+	return rc;
 }

@@ -40,6 +40,9 @@ def build_simt(threads=256):
     lib.simt_simplify_batch.restype = ctypes.c_int
     lib.simt_simplify_batch.argtypes = [P, P, P, P, P, P, P, P, P, SZ, P, SZ, ctypes.c_float, ctypes.c_uint, ctypes.c_uint,
                                         ctypes.c_uint, ctypes.c_int, ctypes.c_uint, ctypes.c_ulonglong]
+    lib.simt_simplify_batch_cursor.restype = ctypes.c_int
+    lib.simt_simplify_batch_cursor.argtypes = [P, P, P, P, P, P, P, P, P, SZ, P, SZ, ctypes.c_float, ctypes.c_uint, ctypes.c_uint,
+                                               ctypes.c_uint, ctypes.c_int, ctypes.c_int, ctypes.c_ulonglong]
     lib.simt_retries.restype = ctypes.c_uint
     lib.simt_retries.argtypes = []
     lib.simt_set_vertex_lock.restype = None
@@ -193,3 +196,67 @@ def test_edge_values(simt, opts):
         meshes.append((ib, np.ascontiguousarray(v[:, :3])))
         targets.append((ib.size // 3 * int(rng.integers(0, 4)) // 4) * 3)
     check(simt, meshes, targets, 1e-1, opts, seed=int(opts) + 3)
+
+
+def run_cursor(lib, meshes, targets, err, opts, groups, seed):
+    """The largest class's wrapper (simplify_from_cursor) with `groups` workgroups interleaved over
+    one shared cursor, the list in a random order; returns [(indices, error, status)]."""
+    idx, io, verts, vo, stride = M.pack_meshes(meshes)
+    lib.simt_set_vertex_lock(None)
+    tg = np.ascontiguousarray(targets, dtype=np.uint32)
+    n = len(meshes)
+    ids = np.random.default_rng(seed).permutation(n).astype(np.uint32)
+    vmax = (int(np.diff(vo).max()) + 31) & ~31
+    imax = (int(np.diff(io).max()) + 95) // 96 * 96
+    dest = np.zeros(max(idx.size, 1), np.uint32)
+    cnt = np.zeros(n, np.uint32)
+    errs = np.zeros(n, np.float32)
+    status = np.full(n, 99, np.uint32)
+    rc = lib.simt_simplify_batch_cursor(dest.ctypes.data, cnt.ctypes.data, errs.ctypes.data, status.ctypes.data, ids.ctypes.data,
+                                        idx.ctypes.data, io.ctypes.data, verts.ctypes.data, vo.ctypes.data, stride, tg.ctypes.data,
+                                        n, float(err), int(opts), vmax, imax, groups, 300, seed)
+    assert rc != -1, "deadlock in the cursor wrapper (see stderr)"
+    assert rc == 0, f"cursor wrapper: rc {rc} (-2: the shared cursor did not end at meshes + workgroups)"
+    return [(dest[io[p]:io[p] + cnt[p]].copy(), float(errs[p]), int(status[p])) for p in range(n)]
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_cursor_wrapper(simt, seed):
+    """k_simplify_patches_wide's loop (simplify_from_cursor: thread 0 takes a list position with an
+    atomic, hands it over through LDS, the workgroup reads it back wave-uniform) with more meshes
+    than workgroups, the workgroups interleaved in random orders over the shared cursor: every mesh
+    runs exactly once, on LDS its workgroup's previous mesh left behind, and comes out exactly as
+    the oracle's; a barrier that not every thread reaches is reported as a deadlock."""
+    rng = np.random.default_rng(880 + seed)
+    meshes, targets = [], []
+    for k in range(10):
+        if k % 3 == 1:
+            ib, v = random_soup(rng, nv=int(rng.integers(3, 120)), ntri=int(rng.integers(1, 200)))
+        else:
+            ib, v = random_grid(rng, n=int(rng.integers(3, 14)))
+        meshes.append((ib, np.ascontiguousarray(v[:, :3])))
+        targets.append((ib.size // 3 * int(rng.integers(0, 4)) // 4) * 3)
+    opts = [M.SimplifyErrorAbsolute | M.SimplifyPermissive, M.SimplifyLockBorder, 0][seed]
+    got = run_cursor(simt, meshes, targets, 1e-2, opts, groups=3 + seed, seed=seed)
+    for p, ((ib, v), t, (r, e, st)) in enumerate(zip(meshes, targets, got)):
+        assert st == 0, f"mesh {p}: status {st}"
+        ro, eo = O.simplify(ib, v, int(t), 1e-2, opts)
+        assert r.size == ro.size and np.array_equal(r, ro), f"mesh {p}: indices differ from the oracle"
+        assert same_bits(e, eo), f"mesh {p}: result_error {e!r} != {eo!r}"
+
+
+def test_cursor_wrapper_wide(simt_wide):
+    """The same wrapper in the 1024-thread build the largest class runs: two workgroups, five
+    meshes."""
+    rng = np.random.default_rng(8899)
+    meshes, targets = [], []
+    for k in range(5):
+        ib, v = random_grid(rng, n=int(rng.integers(4, 12)))
+        meshes.append((ib, np.ascontiguousarray(v[:, :3])))
+        targets.append((ib.size // 3 * int(rng.integers(1, 4)) // 4) * 3)
+    opts = M.SimplifyErrorAbsolute | M.SimplifyPermissive
+    got = run_cursor(simt_wide, meshes, targets, 1e-2, opts, groups=2, seed=5)
+    for p, ((ib, v), t, (r, e, st)) in enumerate(zip(meshes, targets, got)):
+        assert st == 0, f"mesh {p}: status {st}"
+        ro, eo = O.simplify(ib, v, int(t), 1e-2, opts)
+        assert r.size == ro.size and np.array_equal(r, ro) and same_bits(e, eo), f"mesh {p}: differs from the oracle"
