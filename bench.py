@@ -1,11 +1,15 @@
-"""Benchmark of the MI355X simplifier: input triangles per second for one plain
-meshopt_simplify call on a device-resident mesh, next to the reference CPU library.
+"""Benchmark of the MI355X simplifier, next to the reference CPU library on the same host.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload icosphere8|grid256|seamed1024|patches4096|serving|sloppy|update|clod]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload patches4096|icosphere8|grid256|seamed1024|serving|sloppy|update|clod]
 
-A step is one whole simplify call (setup, every pass, compaction of the result) on inputs
-already resident in HBM; the timed region is bracketed by device synchronisation.  Prints one
-JSON line.
+Default (patches4096): the batched form.  A step is one meshopt_hip_simplify_batch_device call that
+simplifies 4096 clusterlod-style patches (64-512 triangles each, LockBorder, target 50%, error 1e-2)
+with every array resident in HBM; the line reports input Mtri/s, bit-exactness of every patch against
+the reference library, the reference library on one host core and on all of the host's cores (a
+thread pool, one patch per work item), the plain per-call device path on the same patches, and a
+nested "plain" object: the plain meshopt_simplify call on the icosphere L8 mesh (device-resident and
+through the host entry point, H2D/D2H included).  The plain workloads alone: --workload icosphere8
+and the others.  The timed region is bracketed by device synchronisation.  Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -25,10 +29,6 @@ import numpy as np
 
 HBM_PEAK_GBS = 8000.0
 
-# compulsory bytes of one rank visit in phase X of the commit rounds (the dominant kernel):
-# work-list entry 16 + status 1 + two lock ranks 8 + two previous-round minima 16 +
-# three 64-bit minima published 24 + survivor entry written 16
-PHASE_X_BYTES_PER_RANK = 81
 
 
 def make_workload(name):
@@ -56,6 +56,39 @@ def alg_bytes(T0, V, A, passes):
         Tn = passes[i + 1]["live_triangles"] if i + 1 < len(passes) else Tp
         total += 96 * Tp + 12 * Tn + 64 * p["candidates"] + (109 + W_) * V + (88 + has_a * (88 + 32 * A)) * p["committed"]
     return total
+
+
+def perform_bytes(V, passes):
+    """The perform term of the fixed formula (the commit of the sorted candidates, which phase X of
+    the commit rounds carries): 16 C_p + 24 T_p + 35 V per pass."""
+    return sum(16 * p["candidates"] + 24 * p["live_triangles"] + 35 * V for p in passes)
+
+
+def batch_alg_bytes(index_counts, vertex_counts, out_counts, stats):
+    """The fixed formula summed over the meshes of a batch (no attributes, so W = 56): setup
+    120 T_0 + 338 V, and per pass 96 T_p + 12 T_{p+1} + 64 C_p + 165 V + 88 M_p.  stats[p] holds the
+    mesh's passes and its sums of T_p, C_p and M_p; the sum of T_{p+1} is sum T_p - T_0 + T_final."""
+    T0 = index_counts.astype(np.int64) // 3
+    V = vertex_counts.astype(np.int64)
+    Tf = out_counts.astype(np.int64) // 3
+    st = stats.astype(np.int64)
+    passes, sT, sC, sM = st[:, 0], st[:, 1], st[:, 2], st[:, 3]
+    sTn = np.where(passes > 0, sT - T0 + Tf, 0)
+    total = 120 * T0 + 338 * V + 96 * sT + 12 * sTn + 64 * sC + 165 * V * passes + 88 * sM
+    return int(total.sum())
+
+
+def host_description():
+    """CPU model, logical CPUs, and the CPUs this process may run on."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0))}
 
 
 def measured_traffic(workload, kernel="k_round_x"):
@@ -88,9 +121,6 @@ def cpu_reference(m, target, weights, budget_s=12.0):
             "sample": f"{len(times)} single-threaded calls on the same mesh, median {med * 1e3:.1f} ms"}
 
 
-# compulsory HBM bytes of one mesh in the batched kernel: its indices read (4 I) and positions
-# read (12 V, xyz), the result written (4 I_out), offsets/target read and count/error/status
-# written (6 x 4)
 def calibrate_copy(torch, dev):
     """One device copy of 256 MiB: the reference point for the PMC byte counters (its FETCH_SIZE
     and WRITE_SIZE should read 262144 KiB)."""
@@ -101,7 +131,9 @@ def calibrate_copy(torch, dev):
     del src, dst
 
 
-def batch_alg_bytes(index_counts, vertex_counts, out_counts):
+def batch_compulsory_bytes(index_counts, vertex_counts, out_counts):
+    """HBM bytes a batched call cannot avoid: indices read (4 I) and positions read (12 V, xyz), the
+    result written (4 I_out), offsets/target read and count/error/status written (6 x 4 per mesh)."""
     return int(4 * np.sum(index_counts) + 12 * np.sum(vertex_counts) + 4 * np.sum(out_counts) + 24 * len(index_counts))
 
 
@@ -174,8 +206,9 @@ def bench_patches(args):
     sample_tris = int(sum(meshes[p].triangle_count for p in range(sample)))
     plain_value = sample_tris / (plain_s * sample / count) / 1e6
 
+    stats = M.last_batch_stats(count)
     out = {
-        "metric": "input Mtri/s simplified (bit-exact vs CPU), 4096-patch batch",
+        "metric": "input Mtri/s simplified (bit-exact vs CPU), batched vs plain; p99 latency",
         "value": round(value, 3),
         "unit": "Mtri/s",
         "n_gpus": 1,
@@ -187,7 +220,8 @@ def bench_patches(args):
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded generator)",
-        "config": {"workload": desc, "input_triangles": tris, "output_triangles": int(g_cnt.sum()) // 3, "patches": count},
+        "config": {"workload": desc, "input_triangles": tris, "output_triangles": int(g_cnt.sum()) // 3, "patches": count,
+                   "passes": int(stats[:, 0].sum()), "candidates": int(stats[:, 2].sum()), "committed": int(stats[:, 3].sum())},
         "p50_ms": round(statistics.median(times) * 1e3, 4),
         "p99_ms": round(float(np.percentile(np.array(times) * 1e3, 99)), 4),
         "plain_per_call_gpu": {"value": round(plain_value, 3), "unit": "Mtri/s",
@@ -195,14 +229,20 @@ def bench_patches(args):
                                "batch_speedup": round(value / plain_value, 2)},
     }
     if k_launches:
-        ab = batch_alg_bytes(np.diff(io), np.diff(vo), g_cnt)
+        ab = batch_alg_bytes(np.diff(io), np.diff(vo), g_cnt, stats)
         per_launch_s = k_ms * 1e-3 / k_launches
         achieved = ab / per_launch_s / 1e9
-        out["roofline"] = {"bound": "hbm", "kernel": "k_simplify_patches: one LDS-resident workgroup per patch, LDS size classes "
-                                                  "launched side by side, timed with the classify kernel before them",
+        comp = batch_compulsory_bytes(np.diff(io), np.diff(vo), g_cnt)
+        out["roofline"] = {"bound": "hbm",
+                           "kernel": "one batched call: k_batch_classify, k_batch_order and the k_simplify_patches size classes launched "
+                                     "side by side, timed with HIP events from the classify start to the join of the class streams",
                            "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                           "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": measured_traffic("patches4096", "k_simplify_patches"),
-                           "alg_bytes_per_launch": ab, "avg_launch_us": round(per_launch_s * 1e6, 2)}
+                           "frac": round(achieved / HBM_PEAK_GBS, 6),
+                           "traffic": measured_traffic("patches4096", "k_simplify_patches"),
+                           "alg_bytes_per_launch": ab, "avg_launch_us": round(per_launch_s * 1e6, 2),
+                           "alg_bytes_formula": "per patch: 120 T0 + 338 V + sum over passes (96 T_p + 12 T_p+1 + 64 C_p + 165 V + 88 M_p)",
+                           "compulsory_hbm_bytes": comp,
+                           "compulsory_hbm_frac": round(comp / per_launch_s / 1e9 / HBM_PEAK_GBS, 6)}
 
     if not args.no_cpu:
         from oracle import oracle as O
@@ -215,7 +255,8 @@ def bench_patches(args):
                 break
             exact = np.array_equal(r_dst[io[p]:io[p] + r_cnt[p]], g_dst[io[p]:io[p] + g_cnt[p]])
         out["bit_exact"] = bool(exact)
-        cores = max(1, min(args.cpu_threads, os.cpu_count() or 1))
+        host = host_description()
+        cores = max(1, min(args.cpu_threads, host["affinity_cpus"]))
         base = {}
         for th in (1, cores):
             reps, t_all = 0, 0.0
@@ -227,10 +268,14 @@ def bench_patches(args):
             base[th] = (tris * reps / t_all / 1e6, reps, t_all)
         v1, vN = base[1], base[cores]
         out["cpu_baseline"] = {"value": round(vN[0], 3), "unit": "Mtri/s", "cores": cores, "kind": kind,
-                               "sample": f"the full 4096-patch batch, {vN[1]} runs on {cores} threads (one patch per work item)",
-                               "single_thread": round(v1[0], 3), "single_thread_sample": f"{v1[1]} runs on 1 thread"}
+                               "sample": f"the full 4096-patch batch, {vN[1]} runs on {cores} threads (std::thread pool, one patch per "
+                                         f"work item from an atomic index; {cores} = the CPUs this process may use, at most the GPU's "
+                                         f"16-CPU share of the host)",
+                               "single_thread": round(v1[0], 3), "single_thread_sample": f"{v1[1]} runs of the full batch on 1 thread",
+                               "host": host}
         out["speedup_vs_cpu_all_cores"] = round(value / vN[0], 2)
-    print(json.dumps(out), flush=True)
+        out["speedup_vs_cpu_one_core"] = round(value / v1[0], 2)
+    return out
 
 
 def replay_cpu(reqs, arrivals, threads):
@@ -607,7 +652,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="icosphere8")
+    ap.add_argument("--workload", default="patches4096")
+    ap.add_argument("--no-plain", action="store_true", help="patches4096: leave out the nested plain icosphere L8 line")
     ap.add_argument("--clod-tiles", type=int, default=64,
                     help="clod workload: the grid is cut into tiles x tiles groups (64: about 512 triangles each; 34: about "
                          "1810, sixteen clusters of the clusterlod demo's default size)")
@@ -624,11 +670,15 @@ def main():
 
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              "(meshes and patch batches shard across GPUs with no exchange, see the design notes)", file=sys.stderr)
+              "(meshes and patch batches shard across GPUs with no exchange)", file=sys.stderr)
         sys.exit(2)
 
     if args.workload == "patches4096":
-        return bench_patches(args)
+        out = bench_patches(args)
+        if not args.no_plain:
+            out["plain"] = bench_plain(args, "icosphere8")
+        print(json.dumps(out), flush=True)
+        return
     if args.workload == "serving":
         return bench_serving(args)
     if args.workload == "sloppy":
@@ -637,11 +687,17 @@ def main():
         return bench_update(args)
     if args.workload == "clod":
         return bench_clod(args)
+    print(json.dumps(bench_plain(args, args.workload)), flush=True)
 
+
+def bench_plain(args, workload):
+    """One plain simplify call per step on a device-resident mesh (meshopt_hip_simplify_device), and
+    the same call through the drop-in host entry (meshopt_simplify: H2D of the inputs and D2H of the
+    result inside the timed call)."""
     import torch
     import meshoptimizer_amd as M
 
-    m, target, weights, desc = make_workload(args.workload)
+    m, target, weights, desc = make_workload(workload)
     dev = torch.device("cuda:0")
     ib = torch.from_numpy(m.indices.astype(np.int32)).to(dev)
     vb = torch.from_numpy(m.vertices).to(dev)
@@ -681,6 +737,19 @@ def main():
     ms = elapsed / args.steps * 1e3
     value = m.triangle_count / (elapsed / args.steps) / 1e6
 
+    # the drop-in host entry point on host arrays (staging copies included)
+    attrs = m.vertices[:, 3:3 + len(weights)] if weights is not None else None
+    host_t = []
+    for k in range(max(3, min(args.steps, 10)) + 1):
+        ts = time.perf_counter()
+        if attrs is not None:
+            M.simplify_with_attributes(m.indices, m.vertices, attrs, weights, None, target, 1e-2, 0)
+        else:
+            M.simplify(m.indices, m.vertices, target, 1e-2, 0)
+        if k:
+            host_t.append(time.perf_counter() - ts)
+    host_ms = statistics.median(host_t) * 1e3
+
     A = 0 if weights is None else int((weights > 0).sum())
     ab = alg_bytes(m.triangle_count, call["vertex_count"], A, passes)
 
@@ -701,29 +770,35 @@ def main():
                    "passes": len(passes), "rounds": sum(p["rounds"] for p in passes)},
         "p50_ms": round(statistics.median(times) * 1e3, 4),
         "p99_ms": round(float(np.percentile(np.array(times) * 1e3, 99)), 4),
+        "host_entry": {"value": round(m.triangle_count / (host_ms * 1e-3) / 1e6, 3), "unit": "Mtri/s", "ms_per_call": round(host_ms, 4),
+                       "sample": f"meshopt_simplify{'WithAttributes' if attrs is not None else ''} on host arrays, median of "
+                                 f"{len(host_t)} calls (H2D of the inputs and D2H of the result inside each call)"},
         "call_roofline": {"bound": "hbm", "alg_bytes": ab, "achieved_GBs": round(ab / (ms * 1e-3) / 1e9, 2),
                           "peak_GBs": HBM_PEAK_GBS, "frac": round(ab / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5)},
     }
 
     if x_launches:
-        per_launch_bytes = PHASE_X_BYTES_PER_RANK * x_ranks / x_launches
+        # the fixed formula's perform bytes of one call, spread over the phase-X launches of one call
+        launches_per_call = x_launches / args.steps
+        per_launch_bytes = perform_bytes(call["vertex_count"], passes) / launches_per_call
         per_launch_s = x_ms * 1e-3 / x_launches
         achieved = per_launch_bytes / per_launch_s / 1e9
         out["roofline"] = {"bound": "hbm", "kernel": "k_round_x (commit rounds, phase X)",
                            "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                           "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(args.workload),
-                           "launches_per_step": x_launches / args.steps,
+                           "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(workload),
+                           "alg_bytes_per_launch": round(per_launch_bytes), "alg_bytes_formula":
+                               "perform term of the fixed formula, sum over passes of 16 C_p + 24 T_p + 35 V, per launch",
+                           "launches_per_step": launches_per_call,
                            "avg_launch_us": round(per_launch_s * 1e6, 2),
                            "ranks_per_launch": round(x_ranks / x_launches, 1)}
 
     if not args.no_cpu:
         from oracle import oracle as O
-        ro, eo = O.simplify(m.indices, m.vertices, target, 1e-2, 0,
-                            attributes=m.vertices[:, 3:3 + len(weights)] if weights is not None else None,
-                            weights=weights, impl="ref" if O.have_ref() else "oracle")
+        ro, eo = O.simplify(m.indices, m.vertices, target, 1e-2, 0, attributes=attrs, weights=weights,
+                            impl="ref" if O.have_ref() else "oracle")
         out["bit_exact"] = bool(np.array_equal(ro, gpu_out) and np.float32(eo).tobytes() == np.float32(e_gpu).tobytes())
         out["cpu_baseline"] = cpu_reference(m, target, weights)
-    print(json.dumps(out), flush=True)
+    return out
 
 
 if __name__ == "__main__":

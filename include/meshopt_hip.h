@@ -115,6 +115,13 @@ void meshopt_hip_kernel_timing(int enable, double* total_ms, size_t* launches, s
  * the previous call; resets the totals. */
 void meshopt_hip_batch_kernel_timing(double* total_ms, size_t* launches, size_t* meshes);
 
+/* Per-mesh statistics of the last batched call (meshopt_hip_simplify_batch,
+ * meshopt_hip_simplify_batch_device, meshopt_hip_clod_level_device): 4 words per mesh, the
+ * passes, and summed over them the live triangles at each pass start, the candidates and the
+ * committed collapses (the per-pass counters of src/simplifier.cpp:1629-1632).  Copies
+ * min(max_meshes, meshes of that call) entries; returns their count, or -1 on a HIP error. */
+int meshopt_hip_last_batch_stats(unsigned int* stats, size_t max_meshes);
+
 /* Serving.  A server runs simplify requests of any size as they arrive, on `slots` concurrent
  * device contexts (each its own HIP stream, arena and staging buffer), admitting them in arrival
  * order against a device-memory budget: a request starts once a slot is idle and the slot's

@@ -61,6 +61,7 @@ EXPORTED_SYMBOLS = (
     "meshopt_hip_simplify_batch_device",
     "meshopt_hip_clod_level_device",
     "meshopt_hip_batch_kernel_timing",
+    "meshopt_hip_last_batch_stats",
     "meshopt_hip_server_create",
     "meshopt_hip_server_submit",
     "meshopt_hip_server_wait",
@@ -133,6 +134,8 @@ def library() -> ctypes.CDLL:
     lib.meshopt_hip_clod_level_device.argtypes = [_P, _P, _P, _P, _P, _SZ, _P, _SZ, _SZ, _P, _P, _P, _P, _F, _U]
     lib.meshopt_hip_simplify_batch_device.restype = ctypes.c_int
     lib.meshopt_hip_simplify_batch_device.argtypes = [_P, _P, _P, _P, _P, _P, _P, _P, _SZ, _SZ, _P, _SZ, _F, _U]
+    lib.meshopt_hip_last_batch_stats.restype = ctypes.c_int
+    lib.meshopt_hip_last_batch_stats.argtypes = [_P, _SZ]
     lib.meshopt_hip_batch_kernel_timing.restype = None
     lib.meshopt_hip_batch_kernel_timing.argtypes = [_P, _P, _P]
     lib.meshopt_hip_kernel_timing.restype = None
@@ -363,6 +366,17 @@ def batch_kernel_timing():
     k = ctypes.c_size_t(0)
     lib.meshopt_hip_batch_kernel_timing(ctypes.byref(ms), ctypes.byref(n), ctypes.byref(k))
     return ms.value, n.value, k.value
+
+
+def last_batch_stats(mesh_count):
+    """Per-mesh statistics of the last batched call: a (n, 4) uint32 array of passes and, summed over
+    the passes, live triangles at the pass start, candidates and committed collapses."""
+    lib = library()
+    out = np.zeros((max(mesh_count, 1), 4), dtype=np.uint32)
+    n = lib.meshopt_hip_last_batch_stats(out.ctypes.data, mesh_count)
+    if n < 0:
+        raise RuntimeError(lib.meshopt_hip_last_error().decode())
+    return out[:n]
 
 
 def simplify_batch_device(destination_ptr, result_counts_ptr, result_errors_ptr, indices_ptr, index_offsets_ptr,

@@ -20,7 +20,8 @@ struct BatchArgs
 	uint32_t* status;                  // per mesh: 0 done, 1 plain path, 2 bad index, 3 pass bound hit (internal error), 4 invalid sizes,
 	                                   // 5 more candidates than the layout holds (run again in kBatchRetryClass),
 	                                   // 6 waiting for the last class's launch
-	uint32_t* passes;                  // per mesh (may be null)
+	uint32_t* stats;                   // 4 words per mesh (may be null): passes, and summed over them the live triangles at
+	                                   // the pass start, the candidates and the committed collapses
 	const uint32_t* indices;           // packed local indices
 	const uint32_t* index_offsets;     // mesh_count + 1
 	const float* positions;            // packed vertices (position in the first 12 bytes)

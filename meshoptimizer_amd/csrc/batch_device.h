@@ -810,6 +810,9 @@ __device__ __forceinline__ void simplify_patch(const BatchArgs& a, const PatchLa
 	uint32_t result_count = I;
 	float result_error = 0.f;
 	uint32_t passes = 0;
+	// per-pass totals for the caller's statistics: live triangles at the pass start, candidates,
+	// committed collapses (workgroup-uniform values)
+	uint32_t sum_live = 0, sum_candidates = 0, sum_committed = 0;
 
 	// every pass that commits removes at least one triangle, so I / 3 + 1 passes always suffice;
 	// the bound is an exit every wavefront reaches even on inconsistent state
@@ -1238,6 +1241,9 @@ __device__ __forceinline__ void simplify_patch(const BatchArgs& a, const PatchLa
 		PROFILE_COUNT(22, sh[3]);
 		result_error = shf[5];
 		passes++;
+		sum_live += T;
+		sum_candidates += C;
+		sum_committed += sh[3];
 		if (sh[3] == 0)
 			break;
 
@@ -1355,8 +1361,13 @@ __device__ __forceinline__ void simplify_patch(const BatchArgs& a, const PatchLa
 		if (a.result_errors)
 			a.result_errors[patch] = sqrtf(result_error) * error_scale;
 		a.status[patch] = overrun ? 3u : 0u;
-		if (a.passes)
-			a.passes[patch] = passes;
+		if (a.stats)
+		{
+			a.stats[patch * 4 + 0] = passes;
+			a.stats[patch * 4 + 1] = sum_live;
+			a.stats[patch * 4 + 2] = sum_candidates;
+			a.stats[patch * 4 + 3] = sum_committed;
+		}
 	}
 	PROFILE_PHASE(11);
 	PROFILE_FLUSH();

@@ -3,6 +3,7 @@
 // wavefront.  The device code is the same source as batch.hip's, in its own inline namespace.
 #define MHIP_BATCH_THREADS 1024
 #include "batch_device.h"
+#include "prims.h"
 
 namespace mhip
 {
@@ -54,7 +55,7 @@ void launch_batch_wide(hipStream_t s, const BatchArgs& a, uint32_t vmax, uint32_
 	PatchLayout L = patch_layout(vmax, imax, cmax);
 	if (!grid)
 		return;
-	HIP_CHECK(hipMemsetAsync(cursor, 0, sizeof(uint32_t), s));
+	fill_u32(s, cursor, 0u, 1);
 	k_simplify_patches_wide<<<(unsigned int)grid, kBatchThreads, L.total, s>>>(a, L, ids, count_dev, cursor);
 }
 

@@ -69,6 +69,20 @@ struct PassRecord
 	meshopt_hip_pass_stats s;
 };
 
+// device words of a batched call's lists: the status of every mesh, one count word per class,
+// the class id lists and their ordering's scratch, the largest class's cursor
+static size_t batch_list_words(size_t meshes)
+{
+	return meshes + 4 + 2 * (size_t)kBatchClasses * meshes + 4;
+}
+
+// host words of a batched call: the status read-back (every mesh, then the class counts), the list
+// of meshes run again in the retry class and its length
+static size_t batch_host_words(size_t meshes)
+{
+	return 2 * meshes + kBatchClasses + 1;
+}
+
 // device buffer for the host entry points' input/output staging (grows; released explicitly)
 struct StagingCache
 {
@@ -117,10 +131,11 @@ struct Context
 	size_t batch_launches = 0, batch_meshes = 0;
 	hipEvent_t batch_ev[2] = {nullptr, nullptr};
 	double window_fraction = 0; // evaluated / candidates of the previous pass (0: none yet)
-	uint32_t* batch_dev = nullptr;    // batched calls: status, class counts and class id lists
+	uint32_t* batch_dev = nullptr;    // batched calls: status, class counts and class id lists, then 4 statistics words per mesh
 	size_t batch_dev_words = 0;
-	uint32_t* batch_pinned = nullptr; // batched calls: status read-back
+	uint32_t* batch_pinned = nullptr; // batched calls: status read-back, then the retry list
 	size_t batch_pinned_words = 0;
+	size_t batch_last_meshes = 0;     // meshes of the last batched call (their statistics follow the lists)
 	Quad* batch_quadrics = nullptr; // batched calls: one vertex quadric per vertex of the batch
 	size_t batch_quadric_count = 0;
 	uint32_t* sloppy_words = nullptr; // sloppy calls: pinned read-back words
@@ -136,8 +151,9 @@ struct Context
 
 	void ensure_batch_buffers(size_t meshes)
 	{
-		// status, class counts, id lists, their ordering's scratch, the largest class's cursor
-		size_t words = meshes + 4 + 2 * (size_t)kBatchClasses * meshes + 4;
+		// status, class counts, id lists, their ordering's scratch, the largest class's cursor, and
+		// the per-mesh statistics
+		size_t words = batch_list_words(meshes) + 4 * meshes;
 		if (words > batch_dev_words)
 		{
 			if (batch_dev)
@@ -147,15 +163,16 @@ struct Context
 			HIP_CHECK(hipMalloc((void**)&batch_dev, words * sizeof(uint32_t)));
 			batch_dev_words = words;
 		}
-		// the status of every mesh, then the class counts (they follow it on the device too)
-		if (meshes + kBatchClasses > batch_pinned_words)
+		// the status of every mesh, then the class counts (they follow it on the device too), then
+		// the retry list
+		if (batch_host_words(meshes) > batch_pinned_words)
 		{
 			if (batch_pinned)
 				HIP_CHECK(hipHostFree(batch_pinned));
 			batch_pinned = nullptr;
 			batch_pinned_words = 0;
-			HIP_CHECK(hipHostMalloc((void**)&batch_pinned, (meshes + kBatchClasses) * sizeof(uint32_t), hipHostMallocDefault));
-			batch_pinned_words = meshes + kBatchClasses;
+			HIP_CHECK(hipHostMalloc((void**)&batch_pinned, batch_host_words(meshes) * sizeof(uint32_t), hipHostMallocDefault));
+			batch_pinned_words = batch_host_words(meshes);
 		}
 	}
 
@@ -202,13 +219,20 @@ struct Context
 		HIP_CHECK(hipEventCreateWithFlags(&fork_ev, hipEventDisableTiming));
 	}
 
-	void ensure_init()
+	// the context's device and stream (the allocation-free entry needs nothing more of it)
+	void ensure_stream()
 	{
 		if (stream)
 			return;
 		HIP_CHECK(hipGetDevice(&device));
 		HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-		HIP_CHECK(hipHostMalloc((void**)&sc_host, sizeof(CallScalars), hipHostMallocDefault));
+	}
+
+	void ensure_init()
+	{
+		ensure_stream();
+		if (!sc_host)
+			HIP_CHECK(hipHostMalloc((void**)&sc_host, sizeof(CallScalars), hipHostMallocDefault));
 	}
 
 	void ensure_arena(size_t bytes)
@@ -1357,13 +1381,38 @@ namespace mhip
 {
 
 
+// Memory of one batched call.  words: batch_list_words(P) device words (status, class counts, id
+// lists, cursor); stats: 4 device words per mesh or null; quadrics: one per vertex of the batch;
+// host: batch_host_words(P) words of host memory (status read-back, retry list).  Meshes that fit
+// no LDS class take the plain path when plain_fallback is set, which allocates its own arena;
+// otherwise the call reports them as an error.  timed: the kernel timing of the context applies.
+struct BatchMemory
+{
+	uint32_t* words;
+	uint32_t* stats;
+	Quad* quadrics;
+	uint32_t* host;
+	bool plain_fallback;
+	bool timed;
+};
+
+// the context's own batch buffers for P meshes and `vertices` vertices (grown on demand)
+static BatchMemory context_batch_memory(Context& ctx, size_t P, size_t vertices, bool need_quadrics)
+{
+	ctx.ensure_batch_buffers(P);
+	if (need_quadrics)
+		ctx.ensure_batch_quadrics(vertices ? vertices : 1);
+	ctx.batch_last_meshes = P;
+	return BatchMemory{ctx.batch_dev, ctx.batch_dev + batch_list_words(P), ctx.batch_quadrics, ctx.batch_pinned, true, ctx.timing};
+}
+
 // Batched call: sizes are validated and meshes sorted into LDS size classes on the device, the
 // classes run side by side (the first on the call's stream, the others on auxiliary streams
 // forked from it), and only the per-mesh status comes back to the host.  Meshes that fit no
 // class (and Sparse batches) take the plain device path afterwards; their offsets and targets
 // are read back only then (h_* may be null: device-resident offsets).
-static int simplify_batch_on_device(Context& ctx, unsigned int* destination, unsigned int* result_counts, float* result_errors,
-                                    const unsigned int* indices, const unsigned int* index_offsets, const float* positions,
+static int simplify_batch_on_device(Context& ctx, const BatchMemory& mem, unsigned int* destination, unsigned int* result_counts,
+                                    float* result_errors, const unsigned int* indices, const unsigned int* index_offsets, const float* positions,
                                     const unsigned int* vertex_offsets, const unsigned char* vertex_lock, size_t vertex_count, size_t stride,
                                     const unsigned int* target_index_counts,
                                     size_t patch_count, float target_error, unsigned int options, const unsigned int* h_index_offsets,
@@ -1405,18 +1454,18 @@ static int simplify_batch_on_device(Context& ctx, unsigned int* destination, uns
 		return (size_t)(n > 0 ? n : 1);
 	}();
 	const size_t P = patch_count;
-	ctx.ensure_batch_buffers(P);
-	uint32_t* status = ctx.batch_dev;
-	uint32_t* counts = ctx.batch_dev + P;
-	uint32_t* ids = ctx.batch_dev + P + 4;
-	uint32_t* h_status = ctx.batch_pinned;
+	uint32_t* status = mem.words;
+	uint32_t* counts = mem.words + P;
+	uint32_t* ids = mem.words + P + 4;
+	uint32_t* h_status = mem.host;
+	uint32_t* h_retry = mem.host + P + kBatchClasses;
 
 	BatchArgs a;
 	a.destination = destination;
 	a.result_counts = result_counts;
 	a.result_errors = result_errors;
 	a.status = status;
-	a.passes = nullptr;
+	a.stats = mem.stats;
 	a.indices = indices;
 	a.index_offsets = index_offsets;
 	a.positions = positions;
@@ -1435,9 +1484,8 @@ static int simplify_batch_on_device(Context& ctx, unsigned int* destination, uns
 	const bool batched = !(options & (kOptSparse | kOptPrune));
 	if (batched)
 	{
-		ctx.ensure_batch_quadrics(vertex_count ? vertex_count : 1);
-		a.quadrics = ctx.batch_quadrics;
-		const bool timed = ctx.timing;
+		a.quadrics = mem.quadrics;
+		const bool timed = mem.timed;
 		if (timed && !ctx.batch_ev[0])
 		{
 			HIP_CHECK(hipEventCreate(&ctx.batch_ev[0]));
@@ -1446,7 +1494,7 @@ static int simplify_batch_on_device(Context& ctx, unsigned int* destination, uns
 		if (timed)
 			HIP_CHECK(hipEventRecord(ctx.batch_ev[0], s));
 		static_assert(kBatchClasses <= 4, "one count word per class before the id lists");
-		HIP_CHECK(hipMemsetAsync(counts, 0, kBatchClasses * sizeof(uint32_t), s));
+		fill_u32(s, counts, 0u, kBatchClasses);
 		launch_batch_classify(s, a, classes, ids, counts);
 		launch_batch_order(s, a, ids, counts, ids + (size_t)kBatchClasses * P);
 		ctx.ensure_aux_streams();
@@ -1518,11 +1566,11 @@ static int simplify_batch_on_device(Context& ctx, unsigned int* destination, uns
 		// their class's arrays hold (status 5), which run again in the retry class: its arrays
 		// hold the full bound and its caps every mesh of the classes before it
 		const uint32_t large = big_early ? 0 : h_status[P + kBatchClasses - 1];
-		std::vector<uint32_t> retry;
+		uint32_t retries = 0;
 		for (size_t p = 0; p < P; ++p)
 			if (h_status[p] == 5)
-				retry.push_back((uint32_t)p);
-		if (!retry.empty() || large)
+				h_retry[retries++] = (uint32_t)p;
+		if (retries || large)
 		{
 			if (timed)
 				HIP_CHECK(hipEventRecord(ctx.batch_ev[0], s));
@@ -1535,13 +1583,15 @@ static int simplify_batch_on_device(Context& ctx, unsigned int* destination, uns
 				launch_batch_wide(ls, a, classes.vcap[c], classes.icap[c], classes.ccap[c], ids + (size_t)c * P, counts + c,
 				                  std::min<size_t>(large, cu_count), ids + 2 * (size_t)kBatchClasses * P);
 			}
-			if (!retry.empty())
+			if (retries)
 			{
+				// the id list and count of class 0 hold the retry list (its meshes have run); the
+				// copies read host memory, which stays untouched until the status read-back below
 				const int c = kBatchRetryClass;
-				const uint32_t n = (uint32_t)retry.size();
-				HIP_CHECK(hipMemcpyAsync(ids, retry.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, s));
-				HIP_CHECK(hipMemcpyAsync(counts, &n, sizeof(uint32_t), hipMemcpyHostToDevice, s));
-				launch_batch(s, a, classes.vcap[c], classes.icap[c], classes.ccap[c], ids, counts, n);
+				h_retry[P] = retries;
+				HIP_CHECK(hipMemcpyAsync(ids, h_retry, retries * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+				HIP_CHECK(hipMemcpyAsync(counts, h_retry + P, sizeof(uint32_t), hipMemcpyHostToDevice, s));
+				launch_batch(s, a, classes.vcap[c], classes.icap[c], classes.ccap[c], ids, counts, retries);
 			}
 			if (large)
 			{
@@ -1577,6 +1627,18 @@ static int simplify_batch_on_device(Context& ctx, unsigned int* destination, uns
 	}
 	if (!fallback)
 		return 0;
+	if (!mem.plain_fallback)
+	{
+		size_t first = 0;
+		while (h_status[first] != 1)
+			++first;
+		char msg[256];
+		snprintf(msg, sizeof(msg),
+		         "meshopt_hip_simplify_batch: mesh %zu fits none of the batched kernel's LDS size classes (or the options need the plain path), "
+		         "and this entry point does not run the plain path",
+		         first);
+		throw std::invalid_argument(msg);
+	}
 
 	// the plain device path for the meshes the batched kernel did not take
 	std::vector<unsigned int> io, vo, tg;
@@ -1620,6 +1682,16 @@ static int simplify_batch_on_device(Context& ctx, unsigned int* destination, uns
 		HIP_CHECK(hipMemcpyAsync(result_counts + p, &n, sizeof(uint32_t), hipMemcpyHostToDevice, s));
 		if (result_errors)
 			HIP_CHECK(hipMemcpyAsync(result_errors + p, &err, sizeof(float), hipMemcpyHostToDevice, s));
+		// the mesh's statistics as the batched kernel records them
+		uint32_t st[4] = {(uint32_t)t_passes.size(), 0, 0, 0};
+		for (const meshopt_hip_pass_stats& ps : t_passes)
+		{
+			st[1] += ps.live_triangles;
+			st[2] += ps.candidates;
+			st[3] += ps.committed;
+		}
+		if (mem.stats)
+			HIP_CHECK(hipMemcpyAsync(mem.stats + 4 * p, st, sizeof(st), hipMemcpyHostToDevice, s));
 		HIP_CHECK(hipStreamSynchronize(s));
 	}
 	return 0;
@@ -1637,11 +1709,34 @@ extern "C" int meshopt_hip_simplify_batch_device(unsigned int* destination, unsi
 		Context& ctx = context();
 		std::lock_guard<std::mutex> guard(ctx.mutex);
 		ctx.ensure_init();
-		simplify_batch_on_device(ctx, destination, result_counts, result_errors, indices, index_offsets, vertex_positions, vertex_offsets,
-		                         vertex_lock, vertex_count, vertex_positions_stride, target_index_counts, mesh_count, target_error, options, nullptr,
-		                         nullptr, nullptr);
+		simplify_batch_on_device(ctx, context_batch_memory(ctx, mesh_count, vertex_count, !(options & (kOptSparse | kOptPrune))), destination,
+		                         result_counts, result_errors, indices, index_offsets, vertex_positions, vertex_offsets, vertex_lock, vertex_count,
+		                         vertex_positions_stride, target_index_counts, mesh_count, target_error, options, nullptr, nullptr, nullptr);
 		t_last_error.clear();
 		return 0;
+	}
+	catch (const std::exception& e)
+	{
+		t_last_error = e.what();
+		return -1;
+	}
+}
+
+extern "C" int meshopt_hip_last_batch_stats(unsigned int* stats, size_t max_meshes)
+{
+	try
+	{
+		Context& ctx = context();
+		std::lock_guard<std::mutex> guard(ctx.mutex);
+		const size_t n = std::min(max_meshes, ctx.batch_last_meshes);
+		if (n)
+		{
+			HIP_CHECK(hipMemcpyAsync(stats, ctx.batch_dev + batch_list_words(ctx.batch_last_meshes), 4 * n * sizeof(uint32_t), hipMemcpyDeviceToHost,
+			                         ctx.stream));
+			HIP_CHECK(hipStreamSynchronize(ctx.stream));
+		}
+		t_last_error.clear();
+		return (int)n;
 	}
 	catch (const std::exception& e)
 	{
@@ -1692,7 +1787,8 @@ extern "C" int meshopt_hip_simplify_batch(unsigned int* destination, unsigned in
 			HIP_CHECK(hipMemcpyAsync(d + off_pos, vertex_positions, pos_bytes, hipMemcpyHostToDevice, s));
 		if (lock_bytes)
 			HIP_CHECK(hipMemcpyAsync(d + off_lock, vertex_lock, lock_bytes, hipMemcpyHostToDevice, s));
-		simplify_batch_on_device(ctx, reinterpret_cast<unsigned int*>(d + off_dst), reinterpret_cast<unsigned int*>(d + off_rc),
+		simplify_batch_on_device(ctx, context_batch_memory(ctx, mesh_count, V, !(options & (kOptSparse | kOptPrune))), reinterpret_cast<unsigned int*>(d + off_dst),
+		                         reinterpret_cast<unsigned int*>(d + off_rc),
 		                         reinterpret_cast<float*>(d + off_re), reinterpret_cast<const unsigned int*>(d + off_idx),
 		                         reinterpret_cast<const unsigned int*>(d + off_io), reinterpret_cast<const float*>(d + off_pos),
 		                         reinterpret_cast<const unsigned int*>(d + off_vo), lock_bytes ? reinterpret_cast<const unsigned char*>(d + off_lock) : nullptr,
@@ -1818,7 +1914,8 @@ extern "C" int meshopt_hip_clod_level_device(unsigned int* destination, unsigned
 		launch_clod_place(s, b, vertex_positions, vertex_positions_stride / sizeof(float), locks);
 
 		// the packed groups are the meshes of one batched call (Sparse is what the packing did)
-		simplify_batch_on_device(ctx, b.packed_result, result_counts, result_errors, b.packed_indices, index_offsets, b.packed_positions, b.unique,
+		simplify_batch_on_device(ctx, context_batch_memory(ctx, G, I, !(options & kOptPrune)), b.packed_result, result_counts, result_errors, b.packed_indices,
+		                         index_offsets, b.packed_positions, b.unique,
 		                         b.packed_locks, I, 12, target_index_counts, G, target_error, options & ~(unsigned int)kOptSparse, nullptr, nullptr,
 		                         nullptr, io.data());
 		launch_clod_unpack(s, b, result_counts, destination);

@@ -19,6 +19,15 @@ static inline unsigned int grid_for(size_t n, unsigned int cap = 65535)
 	return g > cap ? cap : g;
 }
 
+// float to int as the reference's x86 build converts it (cvttss2si): truncation toward zero, and
+// the "integer indefinite" 0x80000000 for NaN and for values outside the int range.  The GPU's
+// own conversion gives 0 for NaN and saturates, so a NaN position (the rescale leaves NaN
+// coordinates NaN) would fall into a different cell than on the CPU.
+__device__ __forceinline__ int cvt_trunc_x86(float f)
+{
+	return (f >= -2147483648.f && f < 2147483648.f) ? (int)f : (int)0x80000000u;
+}
+
 // computeVertexIds (src/simplifier.cpp:2087-2105): round-half-up cell coordinates of the
 // rescaled position, 10 bits per axis; a vertex locked by vertex_lock gets an id of its own
 __device__ __forceinline__ uint32_t sloppy_id(const float4* pos, const uint8_t* lock, uint32_t v, float cell_scale)
@@ -26,9 +35,9 @@ __device__ __forceinline__ uint32_t sloppy_id(const float4* pos, const uint8_t* 
 	if (lock && (lock[v] & kVertexLock))
 		return (1u << 30) | v;
 	const float4 p = pos[v];
-	const int xi = (int)(p.x * cell_scale + 0.5f);
-	const int yi = (int)(p.y * cell_scale + 0.5f);
-	const int zi = (int)(p.z * cell_scale + 0.5f);
+	const int xi = cvt_trunc_x86(p.x * cell_scale + 0.5f);
+	const int yi = cvt_trunc_x86(p.y * cell_scale + 0.5f);
+	const int zi = cvt_trunc_x86(p.z * cell_scale + 0.5f);
 	return ((uint32_t)xi << 20) | ((uint32_t)yi << 10) | (uint32_t)zi;
 }
 

@@ -337,7 +337,7 @@ extern "C" void simt_set_vertex_lock(const unsigned char* lock)
 }
 
 extern "C" int simt_simplify_batch(unsigned int* destination, unsigned int* result_counts, float* result_errors, unsigned int* status,
-                                   unsigned int* passes, const unsigned int* indices, const unsigned int* index_offsets, const float* positions,
+                                   unsigned int* stats, const unsigned int* indices, const unsigned int* index_offsets, const float* positions,
                                    const unsigned int* vertex_offsets, size_t stride, const unsigned int* targets, size_t count,
                                    float target_error, unsigned int options, unsigned int vmax, unsigned int imax, int watchdog_s,
                                    unsigned int mode, unsigned long long seed)
@@ -350,7 +350,7 @@ extern "C" int simt_simplify_batch(unsigned int* destination, unsigned int* resu
 	a.result_counts = result_counts;
 	a.result_errors = result_errors;
 	a.status = status;
-	a.passes = passes;
+	a.stats = stats;
 	a.indices = indices;
 	a.index_offsets = index_offsets;
 	a.positions = positions;
@@ -443,7 +443,7 @@ extern "C" int simt_simplify_batch_cursor(unsigned int* destination, unsigned in
 	a.result_counts = result_counts;
 	a.result_errors = result_errors;
 	a.status = status;
-	a.passes = nullptr;
+	a.stats = nullptr;
 	a.indices = indices;
 	a.index_offsets = index_offsets;
 	a.positions = positions;
