@@ -76,6 +76,23 @@ int meshopt_hip_clod_level_device(unsigned int* destination, unsigned int* resul
 int meshopt_hip_simplify_batch_device(unsigned int* destination, unsigned int* result_counts, float* result_errors, const unsigned int* indices, const unsigned int* index_offsets, const float* vertex_positions, const unsigned int* vertex_offsets, const unsigned char* vertex_lock, size_t vertex_count, size_t vertex_positions_stride, const unsigned int* target_index_counts, size_t mesh_count, float target_error, unsigned int options);
 int meshopt_hip_simplify_batch(unsigned int* destination, unsigned int* result_counts, float* result_errors, const unsigned int* indices, const unsigned int* index_offsets, const float* vertex_positions, const unsigned int* vertex_offsets, const unsigned char* vertex_lock, size_t vertex_positions_stride, const unsigned int* target_index_counts, size_t mesh_count, float target_error, unsigned int options);
 
+/**
+ * meshopt_hip_simplify_batch_device that allocates nothing: no counterpart in the reference; it is
+ * for callers that cannot allocate device or host memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Same arguments and results as
+ * meshopt_hip_simplify_batch_device.  Its scratch (status words, size-class lists and the vertex
+ * quadrics of the batch) is one device array of 64 MiB per GPU that the library holds until the
+ * process ends; its host words are a static array.  Limits: at most 16384 meshes, at most
+ * (64 MiB - the lists) / 48 vertices (about 1.39 million), every mesh within the batched kernel's
+ * LDS size classes (a mesh that needs the plain path makes the call fail), and no Sparse or Prune
+ * option; a call beyond them returns -1 with meshopt_hip_last_error set, before any launch for the
+ * size limits.  Every call on one GPU shares that GPU's scratch: calls that overlap on one GPU race
+ * (the library serialises calls made through it; callers that bypass its lock must not overlap).
+ * The first call on a device creates the library's streams there and looks up the scratch array's
+ * device address.
+ */
+int meshopt_hip_simplify_batch_device_noalloc(unsigned int* destination, unsigned int* result_counts, float* result_errors, const unsigned int* indices, const unsigned int* index_offsets, const float* vertex_positions, const unsigned int* vertex_offsets, const unsigned char* vertex_lock, size_t vertex_count, size_t vertex_positions_stride, const unsigned int* target_index_counts, size_t mesh_count, float target_error, unsigned int options);
+
 /* Per-pass counters of the last call made on the calling thread; the same quantities the
  * reference prints under TRACE (src/simplifier.cpp:1625-1633). */
 typedef struct meshopt_hip_pass_stats

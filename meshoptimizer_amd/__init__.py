@@ -59,6 +59,7 @@ EXPORTED_SYMBOLS = (
     "meshopt_hip_simplify_device",
     "meshopt_hip_simplify_batch",
     "meshopt_hip_simplify_batch_device",
+    "meshopt_hip_simplify_batch_device_noalloc",
     "meshopt_hip_clod_level_device",
     "meshopt_hip_batch_kernel_timing",
     "meshopt_hip_last_batch_stats",
@@ -134,6 +135,8 @@ def library() -> ctypes.CDLL:
     lib.meshopt_hip_clod_level_device.argtypes = [_P, _P, _P, _P, _P, _SZ, _P, _SZ, _SZ, _P, _P, _P, _P, _F, _U]
     lib.meshopt_hip_simplify_batch_device.restype = ctypes.c_int
     lib.meshopt_hip_simplify_batch_device.argtypes = [_P, _P, _P, _P, _P, _P, _P, _P, _SZ, _SZ, _P, _SZ, _F, _U]
+    lib.meshopt_hip_simplify_batch_device_noalloc.restype = ctypes.c_int
+    lib.meshopt_hip_simplify_batch_device_noalloc.argtypes = [_P, _P, _P, _P, _P, _P, _P, _P, _SZ, _SZ, _P, _SZ, _F, _U]
     lib.meshopt_hip_last_batch_stats.restype = ctypes.c_int
     lib.meshopt_hip_last_batch_stats.argtypes = [_P, _SZ]
     lib.meshopt_hip_batch_kernel_timing.restype = None
@@ -389,6 +392,19 @@ def simplify_batch_device(destination_ptr, result_counts_ptr, result_errors_ptr,
     rc = lib.meshopt_hip_simplify_batch_device(destination_ptr, result_counts_ptr, result_errors_ptr, indices_ptr,
                                                index_offsets_ptr, positions_ptr, vertex_offsets_ptr, lock_ptr, vertex_count,
                                                stride, target_counts_ptr, mesh_count, float(target_error), int(options))
+    if rc != 0:
+        raise RuntimeError(lib.meshopt_hip_last_error().decode())
+
+
+def simplify_batch_device_noalloc(destination_ptr, result_counts_ptr, result_errors_ptr, indices_ptr, index_offsets_ptr,
+                                  positions_ptr, vertex_offsets_ptr, vertex_count, stride, target_counts_ptr, mesh_count,
+                                  target_error, options=0, lock_ptr=None):
+    """meshopt_hip_simplify_batch_device_noalloc (the batched device entry with no allocation, its
+    scratch a static device array) on device pointers (integers).  Raises on error."""
+    lib = library()
+    rc = lib.meshopt_hip_simplify_batch_device_noalloc(destination_ptr, result_counts_ptr, result_errors_ptr, indices_ptr,
+                                                       index_offsets_ptr, positions_ptr, vertex_offsets_ptr, lock_ptr, vertex_count,
+                                                       stride, target_counts_ptr, mesh_count, float(target_error), int(options))
     if rc != 0:
         raise RuntimeError(lib.meshopt_hip_last_error().decode())
 

@@ -1722,6 +1722,88 @@ extern "C" int meshopt_hip_simplify_batch_device(unsigned int* destination, unsi
 	}
 }
 
+// ------------------------------------------------------------------------------------------
+// allocation-free batched entry: the device scratch of its calls is one file-scope array
+
+namespace mhip
+{
+
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20; // per GPU, held until the process ends
+constexpr size_t kNoAllocMaxMeshes = 16384;
+constexpr int kNoAllocMaxDevices = 64;
+
+// the batch's status words, class lists and vertex quadrics (the quadrics start 256-byte aligned
+// after the lists); in host code this name is a stand-in, not a device address: host code uses
+// the address looked up per device instead
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+// the device address of `scratch` on each device (looked up once per device, under the context lock)
+static void* g_scratch_device[kNoAllocMaxDevices];
+// host words of the calls (status read-back, retry list), under the context lock
+static uint32_t g_noalloc_host[2 * kNoAllocMaxMeshes + kBatchClasses + 1];
+
+// set-up on the first call on a device: the context's streams and the device address of the scratch
+static void* noalloc_setup(Context& ctx)
+{
+	int dev = 0;
+	HIP_CHECK(hipGetDevice(&dev));
+	if (dev < 0 || dev >= kNoAllocMaxDevices)
+		throw std::invalid_argument("meshopt_hip_simplify_batch_device_noalloc: device index out of range");
+	ctx.ensure_stream();
+	if (dev != ctx.device)
+		throw std::invalid_argument("meshopt_hip_simplify_batch_device_noalloc: the library's context lives on another device");
+	ctx.ensure_aux_streams();
+	if (!g_scratch_device[dev])
+	{
+		void* ptr = nullptr;
+		HIP_CHECK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+		if (!ptr)
+			throw std::runtime_error("meshopt_hip_simplify_batch_device_noalloc: no device address for the scratch array");
+		g_scratch_device[dev] = ptr;
+	}
+	return g_scratch_device[dev];
+}
+
+} // namespace mhip
+
+extern "C" int meshopt_hip_simplify_batch_device_noalloc(unsigned int* destination, unsigned int* result_counts, float* result_errors,
+                                                         const unsigned int* indices, const unsigned int* index_offsets, const float* vertex_positions,
+                                                         const unsigned int* vertex_offsets, const unsigned char* vertex_lock, size_t vertex_count,
+                                                         size_t vertex_positions_stride, const unsigned int* target_index_counts, size_t mesh_count,
+                                                         float target_error, unsigned int options)
+{
+	try
+	{
+		if (mesh_count > kNoAllocMaxMeshes)
+			throw std::invalid_argument("meshopt_hip_simplify_batch_device_noalloc: more than 16384 meshes");
+		if (options & (kOptSparse | kOptPrune))
+			throw std::invalid_argument("meshopt_hip_simplify_batch_device_noalloc: Sparse and Prune batches run on the plain path, which allocates");
+		const size_t list_bytes = (batch_list_words(mesh_count) * sizeof(uint32_t) + 255) & ~size_t(255);
+		const size_t need = list_bytes + (vertex_count ? vertex_count : 1) * sizeof(Quad);
+		if (need > kNoAllocScratchBytes)
+		{
+			char msg[200];
+			snprintf(msg, sizeof(msg), "meshopt_hip_simplify_batch_device_noalloc: the batch needs %zu bytes of scratch, more than the %zu it holds", need,
+			         kNoAllocScratchBytes);
+			throw std::invalid_argument(msg);
+		}
+		Context& ctx = context();
+		std::lock_guard<std::mutex> guard(ctx.mutex);
+		char* base = static_cast<char*>(noalloc_setup(ctx));
+		const BatchMemory mem{reinterpret_cast<uint32_t*>(base), nullptr, reinterpret_cast<Quad*>(base + list_bytes), g_noalloc_host, false, false};
+		simplify_batch_on_device(ctx, mem, destination, result_counts, result_errors, indices, index_offsets, vertex_positions, vertex_offsets,
+		                         vertex_lock, vertex_count, vertex_positions_stride, target_index_counts, mesh_count, target_error, options, nullptr,
+		                         nullptr, nullptr);
+		t_last_error.clear();
+		return 0;
+	}
+	catch (const std::exception& e)
+	{
+		t_last_error = e.what();
+		return -1;
+	}
+}
+
 extern "C" int meshopt_hip_last_batch_stats(unsigned int* stats, size_t max_meshes)
 {
 	try
