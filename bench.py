@@ -207,6 +207,23 @@ def bench_patches(args):
     plain_value = sample_tris / (plain_s * sample / count) / 1e6
 
     stats = M.last_batch_stats(count)
+
+    # the launch shape in use (a tuned table, or the default) and, if tuned, the default's time
+    shape = M.batch_get_shape()
+    default_shape = {"per_cu": [6, 5, 2, 1], "threads": [256, 256, 256, 1024], "cursor": [0, 0, 0, 1]}
+    tuned = any(shape[k] != v for k, v in default_shape.items())
+    default_ms = None
+    if tuned:
+        M.batch_set_shape()
+        for _ in range(2):
+            step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+        torch.cuda.synchronize()
+        default_ms = (time.perf_counter() - t0) / args.steps * 1e3
+        M.batch_set_shape(shape["per_cu"], shape["threads"], shape["cursor"])
     out = {
         "metric": "input Mtri/s simplified (bit-exact vs CPU), batched vs plain; p99 latency",
         "value": round(value, 3),
@@ -227,6 +244,10 @@ def bench_patches(args):
         "plain_per_call_gpu": {"value": round(plain_value, 3), "unit": "Mtri/s",
                                "sample": f"meshopt_hip_simplify_device on the first {sample} patches, one call each",
                                "batch_speedup": round(value / plain_value, 2)},
+        "batch_shape": {**{k: shape[k] for k in ("per_cu", "threads", "cursor", "index_caps", "vertex_caps")},
+                        "source": "tuned table (meshoptimizer_amd/tune.py)" if tuned else "default",
+                        "default_shape_ms_per_step": round(default_ms, 4) if default_ms else None,
+                        "gain_over_default": round(default_ms / ms, 4) if default_ms else None},
     }
     if k_launches:
         ab = batch_alg_bytes(np.diff(io), np.diff(vo), g_cnt, stats)

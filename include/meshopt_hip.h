@@ -139,6 +139,22 @@ void meshopt_hip_batch_kernel_timing(double* total_ms, size_t* launches, size_t*
  * min(max_meshes, meshes of that call) entries; returns their count, or -1 on a HIP error. */
 int meshopt_hip_last_batch_stats(unsigned int* stats, size_t max_meshes);
 
+/* Launch shape of the batched calls' four LDS size classes (no counterpart in the reference):
+ * per class, the workgroups that share one CU (the class's LDS share is the CU's 160 KB over that
+ * count, which sets the largest mesh it holds), the threads per workgroup (256, 512 or 1024) and
+ * whether its workgroups loop over the class's meshes through a cursor (cursor != 0) rather than
+ * taking one mesh each.  The default follows from the LDS budget; at the first batched call the
+ * library reads a measured shape from the table that meshoptimizer_amd/tune.py writes
+ * (batch_shape.txt beside the library, or the file the MESHOPT_HIP_BATCH_SHAPE environment
+ * variable names), unless a shape was set here first.  set: checks the shape (widths, the
+ * wavefronts per SIMD each width's registers allow, workgroups per CU not growing over the
+ * classes, the last class one workgroup per CU) and applies it to later calls; three NULL arrays
+ * restore the default; returns 0, or -1 with the reason in meshopt_hip_last_error.  get: the
+ * shape in use and the vertex and index caps it gives each class on the current device (any
+ * array may be NULL); returns 0, or -1 on a HIP error.  A mesh's result never depends on the shape. */
+int meshopt_hip_batch_set_shape(const unsigned int* per_cu, const unsigned int* threads, const unsigned int* cursor);
+int meshopt_hip_batch_get_shape(unsigned int* per_cu, unsigned int* threads, unsigned int* cursor, unsigned int* vertex_caps, unsigned int* index_caps);
+
 /* Serving.  A server runs simplify requests of any size as they arrive, on `slots` concurrent
  * device contexts (each its own HIP stream, arena and staging buffer), admitting them in arrival
  * order against a device-memory budget: a request starts once a slot is idle and the slot's

@@ -63,6 +63,8 @@ EXPORTED_SYMBOLS = (
     "meshopt_hip_clod_level_device",
     "meshopt_hip_batch_kernel_timing",
     "meshopt_hip_last_batch_stats",
+    "meshopt_hip_batch_set_shape",
+    "meshopt_hip_batch_get_shape",
     "meshopt_hip_server_create",
     "meshopt_hip_server_submit",
     "meshopt_hip_server_wait",
@@ -138,6 +140,10 @@ def library() -> ctypes.CDLL:
     lib.meshopt_hip_simplify_batch_device_noalloc.restype = ctypes.c_int
     lib.meshopt_hip_simplify_batch_device_noalloc.argtypes = [_P, _P, _P, _P, _P, _P, _P, _P, _SZ, _SZ, _P, _SZ, _F, _U]
     lib.meshopt_hip_last_batch_stats.restype = ctypes.c_int
+    lib.meshopt_hip_batch_set_shape.restype = ctypes.c_int
+    lib.meshopt_hip_batch_set_shape.argtypes = [_P, _P, _P]
+    lib.meshopt_hip_batch_get_shape.restype = ctypes.c_int
+    lib.meshopt_hip_batch_get_shape.argtypes = [_P, _P, _P, _P, _P]
     lib.meshopt_hip_last_batch_stats.argtypes = [_P, _SZ]
     lib.meshopt_hip_batch_kernel_timing.restype = None
     lib.meshopt_hip_batch_kernel_timing.argtypes = [_P, _P, _P]
@@ -369,6 +375,32 @@ def batch_kernel_timing():
     k = ctypes.c_size_t(0)
     lib.meshopt_hip_batch_kernel_timing(ctypes.byref(ms), ctypes.byref(n), ctypes.byref(k))
     return ms.value, n.value, k.value
+
+
+def batch_set_shape(per_cu=None, threads=None, cursor=None):
+    """Sets the launch shape of the batched calls' four size classes (workgroups per CU, threads per
+    workgroup, cursor form); no arguments restore the default.  Raises with the reason on an invalid
+    shape."""
+    lib = library()
+    if per_cu is None and threads is None and cursor is None:
+        rc = lib.meshopt_hip_batch_set_shape(None, None, None)
+    else:
+        arrs = [np.ascontiguousarray(x, dtype=np.uint32) for x in (per_cu, threads, cursor)]
+        if any(a.size != 4 for a in arrs):
+            raise ValueError("a shape gives four values per field, one per size class")
+        rc = lib.meshopt_hip_batch_set_shape(arrs[0].ctypes.data, arrs[1].ctypes.data, arrs[2].ctypes.data)
+    if rc != 0:
+        raise ValueError(lib.meshopt_hip_last_error().decode())
+
+
+def batch_get_shape():
+    """The launch shape in use and the vertex and index caps it gives each size class on this
+    device: dict of per_cu, threads, cursor, vertex_caps, index_caps (four entries each)."""
+    lib = library()
+    out = [np.zeros(4, dtype=np.uint32) for _ in range(5)]
+    if lib.meshopt_hip_batch_get_shape(*(a.ctypes.data for a in out)) != 0:
+        raise RuntimeError(lib.meshopt_hip_last_error().decode())
+    return dict(zip(("per_cu", "threads", "cursor", "vertex_caps", "index_caps"), (a.tolist() for a in out)))
 
 
 def last_batch_stats(mesh_count):

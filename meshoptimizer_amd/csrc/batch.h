@@ -6,7 +6,7 @@
 namespace mhip
 {
 
-// LDS size classes of one batched call (launched side by side): 6, 5, 2 and 1 workgroups per CU
+// LDS size classes of one batched call (launched side by side)
 constexpr int kBatchClasses = 4;
 // the class that a mesh overflowing the candidate arrays of class 0 or 1 runs again in (classes
 // 2 and 3 hold the full bound)
@@ -36,7 +36,25 @@ struct BatchArgs
 	uint32_t options;
 };
 
-size_t batch_lds_bytes(uint32_t vmax, uint32_t imax, uint32_t cmax);
+// The launch shape of the size classes: per class, the workgroups that share one CU (the class's
+// LDS share is the CU's 160 KB over that count), the threads per workgroup (256, 512 or 1024), and
+// whether its workgroups take meshes from a cursor (grid: per_cu workgroups per CU) instead of one
+// workgroup per mesh.  The last class always runs from a cursor and launches after the status
+// read-back (unless the host knows a mesh in its range).  The default comes from the LDS budget;
+// a measured one (meshoptimizer_amd/tune.py) may replace it.
+struct BatchShape
+{
+	uint32_t per_cu[kBatchClasses];
+	uint32_t threads[kBatchClasses];
+	uint32_t cursor[kBatchClasses];
+};
+
+BatchShape batch_default_shape();
+// null when the shape can run: widths of 256/512/1024, at most 6 wavefronts per SIMD for the
+// 80-VGPR widths (4 for 1024), workgroups per CU non-increasing over the classes, the last class
+// one workgroup per CU; otherwise the reason
+const char* batch_shape_error(const BatchShape& shape);
+
 // size classes: a mesh belongs to the first class whose vertex and index caps it fits; ccap is the
 // class's candidate capacity (classes 2 and 3 hold the full bound, so they never return status 5)
 struct BatchClasses
@@ -44,33 +62,28 @@ struct BatchClasses
 	uint32_t vcap[kBatchClasses];
 	uint32_t icap[kBatchClasses];
 	uint32_t ccap[kBatchClasses];
+	BatchShape shape;
 };
 
-// the largest caps per class whose LDS layout lets 6, 5, 2 and 1 workgroups share a CU, given the
-// LDS one workgroup may take (a class that the limit leaves no larger than the one before it is
-// empty: caps 0)
-BatchClasses batch_classes(uint32_t max_lds_per_workgroup);
-// lets k_simplify_patches take `bytes` of dynamic LDS; false when the runtime refuses
-bool batch_allow_lds(size_t bytes);
+// the largest caps per class whose LDS layout fits the class's share of a CU, given the LDS one
+// workgroup may take (a class that the limit leaves no larger than the one before it is empty:
+// caps 0)
+BatchClasses batch_classes(uint32_t max_lds_per_workgroup, const BatchShape& shape);
+// LDS bytes of a layout at a width; lets both launch forms of a width take `bytes` of dynamic LDS
+// (false when the runtime refuses)
+size_t batch_lds_bytes(uint32_t threads, uint32_t vmax, uint32_t imax, uint32_t cmax);
+bool batch_allow_lds(uint32_t threads, size_t bytes);
 
 // per mesh: validate sizes (status 4), pick its class (ids[c * patch_count + slot], counts[c])
 // or mark it for the plain path (status 1); counts must be zero on entry
 void launch_batch_classify(hipStream_t s, const BatchArgs& a, const BatchClasses& classes, uint32_t* ids, uint32_t* counts);
 void launch_batch_order(hipStream_t s, const BatchArgs& a, uint32_t* ids, const uint32_t* counts, uint32_t* temp);
 
-// the meshes of a class (ids, *count_dev of them), every one with the (vmax, imax, cmax) layout:
-// workgroup b takes mesh b (grid at least the count; the workgroups past it leave at once)
-void launch_batch(hipStream_t s, const BatchArgs& a, uint32_t vmax, uint32_t imax, uint32_t cmax, const uint32_t* ids, const uint32_t* count_dev,
-                  size_t grid);
-// the largest class with 1024-thread workgroups (batch_wide.hip): each workgroup takes meshes from
-// the list in order through a cursor word (cleared here on the stream) until it ends
-size_t batch_wide_lds_bytes(uint32_t vmax, uint32_t imax, uint32_t cmax);
-bool batch_wide_allow_lds(size_t bytes);
-#ifdef MESHOPT_BATCH_PROFILE
-// adds the 1024-thread build's phase counters (MESHOPT_BATCH_PROFILE builds) to out[32]
-void batch_wide_profile_add(unsigned long long* out, int reset);
-#endif
-void launch_batch_wide(hipStream_t s, const BatchArgs& a, uint32_t vmax, uint32_t imax, uint32_t cmax, const uint32_t* ids, const uint32_t* count_dev,
-                       size_t grid, uint32_t* cursor);
+// the meshes of one class (ids, *count_dev of them), every one with the (vmax, imax, cmax) layout,
+// on workgroups of `threads`: cursor == null gives workgroup b mesh b (grid at least the count;
+// the workgroups past it leave at once); otherwise `grid` workgroups take meshes in list order
+// through the cursor word (cleared here on the stream) until the list ends
+void launch_batch(hipStream_t s, uint32_t threads, const BatchArgs& a, uint32_t vmax, uint32_t imax, uint32_t cmax, const uint32_t* ids,
+                  const uint32_t* count_dev, size_t grid, uint32_t* cursor);
 
 } // namespace mhip

@@ -1,23 +1,61 @@
-// Batched simplification: the persistent launch around simplify_patch (batch_device.h).
-#include "batch_device.h"
+// Batched simplification: the size classes, their launch at each workgroup width, and the
+// classify and order kernels.  This translation unit builds the 256-thread device code
+// (batch_device.h); batch_mid.hip and batch_wide.hip build 512 and 1024.
+#define MHIP_BATCH_THREADS 256
+#include "batch_kernels.h"
 
 namespace mhip
 {
 
-// One workgroup per mesh of one LDS size class; ids lists the class's *count meshes (the grid
-// is an upper bound, workgroups past the count leave at once).  At most 80 VGPRs, so that six
-// wavefronts share a SIMD and six workgroups a CU when their LDS layouts allow it.  (The largest
-// class runs the 1024-thread build of the same device code, batch_wide.hip.)
-__global__ void __launch_bounds__(kBatchThreads) __attribute__((amdgpu_waves_per_eu(6))) k_simplify_patches(BatchArgs a, PatchLayout L, const uint32_t* ids, const uint32_t* count)
+// the other widths' helpers (batch_kernels.h, built in batch_mid.hip and batch_wide.hip)
+namespace threads512
 {
-	extern __shared__ __align__(16) uint8_t smem[];
-	__shared__ uint32_t part[kBatchThreads / 64];
-	__shared__ uint32_t sh[16];
-	__shared__ uint64_t box[6];
-	__shared__ float shf[8];
+size_t width_lds_bytes(uint32_t vmax, uint32_t imax, uint32_t cmax);
+bool width_allow_lds(size_t bytes);
+void width_launch(hipStream_t s, const BatchArgs& a, uint32_t vmax, uint32_t imax, uint32_t cmax, const uint32_t* ids, const uint32_t* count_dev,
+                  size_t grid, uint32_t* cursor);
+#ifdef MESHOPT_BATCH_PROFILE
+void width_profile_add(unsigned long long* out, int reset);
+#endif
+} // namespace threads512
+namespace threads1024
+{
+size_t width_lds_bytes(uint32_t vmax, uint32_t imax, uint32_t cmax);
+bool width_allow_lds(size_t bytes);
+void width_launch(hipStream_t s, const BatchArgs& a, uint32_t vmax, uint32_t imax, uint32_t cmax, const uint32_t* ids, const uint32_t* count_dev,
+                  size_t grid, uint32_t* cursor);
+#ifdef MESHOPT_BATCH_PROFILE
+void width_profile_add(unsigned long long* out, int reset);
+#endif
+} // namespace threads1024
 
-	if (blockIdx.x < *count)
-		simplify_patch(a, L, ids[blockIdx.x], smem, part, sh, box, shf);
+size_t batch_lds_bytes(uint32_t threads, uint32_t vmax, uint32_t imax, uint32_t cmax)
+{
+	if (threads == 1024)
+		return threads1024::width_lds_bytes(vmax, imax, cmax);
+	if (threads == 512)
+		return threads512::width_lds_bytes(vmax, imax, cmax);
+	return threads256::width_lds_bytes(vmax, imax, cmax);
+}
+
+bool batch_allow_lds(uint32_t threads, size_t bytes)
+{
+	if (threads == 1024)
+		return threads1024::width_allow_lds(bytes);
+	if (threads == 512)
+		return threads512::width_allow_lds(bytes);
+	return threads256::width_allow_lds(bytes);
+}
+
+void launch_batch(hipStream_t s, uint32_t threads, const BatchArgs& a, uint32_t vmax, uint32_t imax, uint32_t cmax, const uint32_t* ids,
+                  const uint32_t* count_dev, size_t grid, uint32_t* cursor)
+{
+	if (threads == 1024)
+		threads1024::width_launch(s, a, vmax, imax, cmax, ids, count_dev, grid, cursor);
+	else if (threads == 512)
+		threads512::width_launch(s, a, vmax, imax, cmax, ids, count_dev, grid, cursor);
+	else
+		threads256::width_launch(s, a, vmax, imax, cmax, ids, count_dev, grid, cursor);
 }
 
 // one slot per mesh in its class's id list; the slots of a wavefront's meshes come from one
@@ -71,34 +109,71 @@ static uint32_t lds_limit(uint32_t per_cu, uint32_t max_block)
 	return share - static_words;
 }
 
-BatchClasses batch_classes(uint32_t max_block)
+BatchShape batch_default_shape()
 {
-	// 6, 5, 2 and 1 workgroups per CU (the kernel's 80 VGPRs allow 6 wavefronts per SIMD); the
-	// last two classes take meshes with relatively more vertices and hold full candidate arrays
-	if (max_block < 64 * 1024)
-		max_block = 64 * 1024;
-	const uint32_t limits[kBatchClasses] = {lds_limit(6, max_block), lds_limit(5, max_block), lds_limit(2, max_block), lds_limit(1, max_block)};
-	const uint32_t vdiv[kBatchClasses] = {4, 4, 3, 3};
-	BatchClasses c{};
+	// 6, 5, 2 and 1 workgroups per CU: the 80-VGPR 256-thread kernel allows 6 wavefronts per SIMD;
+	// the last class takes a whole CU with 1024 threads
+	BatchShape s{};
+	const uint32_t per_cu[kBatchClasses] = {6, 5, 2, 1}, threads[kBatchClasses] = {256, 256, 256, 1024};
 	for (int k = 0; k < kBatchClasses; ++k)
 	{
-		if (k && limits[k] <= limits[k - 1])
-			continue; // caps 0: no mesh fits
+		s.per_cu[k] = per_cu[k];
+		s.threads[k] = threads[k];
+		s.cursor[k] = k == kBatchClasses - 1;
+	}
+	return s;
+}
+
+const char* batch_shape_error(const BatchShape& s)
+{
+	for (int k = 0; k < kBatchClasses; ++k)
+	{
+		const uint32_t t = s.threads[k];
+		if (t != 256 && t != 512 && t != 1024)
+			return "threads per workgroup must be 256, 512 or 1024";
+		if (s.per_cu[k] < 1)
+			return "every class needs at least one workgroup per CU";
+		// wavefronts per SIMD: 6 at most for the 80-VGPR widths, 4 for the 1024-thread one
+		const uint32_t per_simd = s.per_cu[k] * (t / 64) / 4;
+		if (per_simd > (t == 1024 ? 4u : 6u))
+			return "more wavefronts per SIMD than the width's register budget allows";
+		if (k && s.per_cu[k] > s.per_cu[k - 1])
+			return "workgroups per CU must not grow from one class to the next";
+	}
+	if (s.per_cu[kBatchClasses - 1] != 1)
+		return "the last class takes a whole CU (one workgroup per CU)";
+	return nullptr;
+}
+
+BatchClasses batch_classes(uint32_t max_block, const BatchShape& shape)
+{
+	if (max_block < 64 * 1024)
+		max_block = 64 * 1024;
+	const uint32_t vdiv[kBatchClasses] = {4, 4, 3, 3};
+	BatchClasses c{};
+	c.shape = shape;
+	uint32_t prev_icap = 0;
+	for (int k = 0; k < kBatchClasses; ++k)
+	{
+		const uint32_t limit = lds_limit(shape.per_cu[k], max_block);
+		// classes from the retry class on hold the full candidate bound (boundEdgeCollapses)
 		const bool last = k >= kBatchRetryClass;
-		uint32_t vbest = 32, ibest = 96;
+		uint32_t vbest = 0, ibest = 0;
 		for (uint32_t icap = 96; icap < 0xffffu; icap += 96)
 		{
 			uint32_t vcap = ((icap / vdiv[k]) + 31) & ~31u;
 			uint32_t ccap = last ? icap + 4 : batch_candidate_cap(icap);
-			const size_t bytes = k == kBatchClasses - 1 ? batch_wide_lds_bytes(vcap, icap, ccap) : patch_layout(vcap, icap, ccap).total;
-			if (bytes > limits[k])
+			if (batch_lds_bytes(shape.threads[k], vcap, icap, ccap) > limit)
 				break;
 			vbest = vcap;
 			ibest = icap;
 		}
+		if (ibest <= prev_icap)
+			continue; // caps 0: no mesh fits
 		c.vcap[k] = vbest;
 		c.icap[k] = ibest;
 		c.ccap[k] = last ? ibest + 4 : batch_candidate_cap(ibest);
+		prev_icap = ibest;
 	}
 	return c;
 }
@@ -110,17 +185,14 @@ void launch_batch_classify(hipStream_t s, const BatchArgs& a, const BatchClasses
 }
 
 #ifdef MESHOPT_BATCH_PROFILE
-// the phase counters of both builds of the device code (256 threads here, 1024 in batch_wide.hip),
-// summed
+// the phase counters of the three builds of the device code (256, 512 and 1024 threads), summed
 extern "C" void meshopt_hip_batch_profile(unsigned long long* out, int reset)
 {
-	HIP_CHECK(hipMemcpyFromSymbol(out, HIP_SYMBOL(g_batch_profile), sizeof(g_batch_profile)));
-	if (reset)
-	{
-		unsigned long long z[32] = {};
-		HIP_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(g_batch_profile), z, sizeof(z)));
-	}
-	batch_wide_profile_add(out, reset);
+	for (int k = 0; k < 32; ++k)
+		out[k] = 0;
+	threads256::width_profile_add(out, reset);
+	threads512::width_profile_add(out, reset);
+	threads1024::width_profile_add(out, reset);
 }
 #endif
 
@@ -174,25 +246,6 @@ void launch_batch_order(hipStream_t s, const BatchArgs& a, uint32_t* ids, const 
 {
 	if (a.patch_count)
 		k_batch_order<<<kBatchClasses, 1024, 0, s>>>(a.index_offsets, ids, counts, a.patch_count, temp);
-}
-
-bool batch_allow_lds(size_t bytes)
-{
-	return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_simplify_patches), hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) ==
-	       hipSuccess;
-}
-
-size_t batch_lds_bytes(uint32_t vmax, uint32_t imax, uint32_t cmax)
-{
-	return patch_layout(vmax, imax, cmax).total;
-}
-
-void launch_batch(hipStream_t s, const BatchArgs& a, uint32_t vmax, uint32_t imax, uint32_t cmax, const uint32_t* ids, const uint32_t* count_dev,
-                  size_t grid)
-{
-	PatchLayout L = patch_layout(vmax, imax, cmax);
-	if (grid)
-		k_simplify_patches<<<(unsigned int)grid, kBatchThreads, L.total, s>>>(a, L, ids, count_dev);
 }
 
 } // namespace mhip

@@ -5,6 +5,7 @@
 // through a pinned scalar block.  All state lives in one device arena per context.
 #include <hip/hip_runtime.h>
 
+#include <dlfcn.h>
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -146,6 +147,13 @@ struct Context
 			HIP_CHECK(hipHostMalloc((void**)&sloppy_words, 8 * sizeof(uint32_t), hipHostMallocDefault));
 		return sloppy_words;
 	}
+	// the batched calls' size classes: the launch shape (set through meshopt_hip_batch_set_shape, or
+	// read from the tuning table at the first batched call) and the caps it gives on this device
+	BatchShape batch_shape = batch_default_shape();
+	bool batch_shape_set = false;
+	BatchClasses batch_cls{};
+	bool batch_cls_ready = false;
+	size_t cu_count = 0;
 	hipStream_t aux[kBatchClasses - 1] = {}; // batch size classes beyond the first
 	hipEvent_t fork_ev = nullptr, join_ev[kBatchClasses - 1] = {};
 
@@ -1381,6 +1389,91 @@ namespace mhip
 {
 
 
+// The tuning table of the batched calls' launch shape: the file named by MESHOPT_HIP_BATCH_SHAPE,
+// else batch_shape.txt beside this library (written by meshoptimizer_amd/tune.py).  Lines
+// "per_cu a b c d", "threads a b c d" and "cursor a b c d" (one number per size class); '#' starts
+// a comment.  Returns false (and leaves *shape alone) when there is no valid table.
+static bool read_batch_shape_table(BatchShape* shape)
+{
+	std::string path;
+	if (const char* env = getenv("MESHOPT_HIP_BATCH_SHAPE"))
+		path = env;
+	else
+	{
+		Dl_info info{};
+		if (!dladdr(reinterpret_cast<void*>(&read_batch_shape_table), &info) || !info.dli_fname)
+			return false;
+		path = info.dli_fname;
+		size_t slash = path.rfind('/');
+		path = (slash == std::string::npos ? std::string(".") : path.substr(0, slash)) + "/batch_shape.txt";
+	}
+	FILE* f = fopen(path.c_str(), "r");
+	if (!f)
+		return false;
+	BatchShape t = batch_default_shape();
+	int seen = 0;
+	char line[256];
+	while (fgets(line, sizeof(line), f))
+	{
+		if (char* hash = strchr(line, '#'))
+			*hash = 0;
+		char key[32];
+		unsigned int v[kBatchClasses];
+		static_assert(kBatchClasses == 4, "table rows hold four classes");
+		if (sscanf(line, "%31s %u %u %u %u", key, &v[0], &v[1], &v[2], &v[3]) != 5)
+			continue;
+		uint32_t* dst = !strcmp(key, "per_cu") ? t.per_cu : !strcmp(key, "threads") ? t.threads : !strcmp(key, "cursor") ? t.cursor : nullptr;
+		if (!dst)
+			continue;
+		for (int k = 0; k < kBatchClasses; ++k)
+			dst[k] = v[k];
+		seen |= dst == t.per_cu ? 1 : dst == t.threads ? 2 : 4;
+	}
+	fclose(f);
+	if (seen != 7 || batch_shape_error(t))
+		return false;
+	*shape = t;
+	return true;
+}
+
+// the size classes of the context's shape on its device: classes beyond 64 KB of LDS need the
+// device's limit and the kernels' permission; without them the shape's classes are those of a
+// 64 KB workgroup
+static const BatchClasses& ensure_batch_classes(Context& ctx)
+{
+	if (ctx.batch_cls_ready)
+		return ctx.batch_cls;
+	if (!ctx.batch_shape_set)
+	{
+		BatchShape t;
+		if (read_batch_shape_table(&t))
+			ctx.batch_shape = t;
+		ctx.batch_shape_set = true;
+	}
+	int max_block = 0, cus = 0;
+	HIP_CHECK(hipDeviceGetAttribute(&max_block, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx.device));
+	HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx.device));
+	ctx.cu_count = (size_t)(cus > 0 ? cus : 1);
+	BatchClasses c = batch_classes((uint32_t)max_block, ctx.batch_shape);
+	bool allowed = true;
+	for (int k = 0; k < kBatchClasses; ++k)
+	{
+		if (!c.icap[k])
+			continue;
+		const size_t bytes = batch_lds_bytes(c.shape.threads[k], c.vcap[k], c.icap[k], c.ccap[k]);
+		if (bytes > 64 * 1024 && !batch_allow_lds(c.shape.threads[k], bytes))
+			allowed = false;
+	}
+	if (!allowed)
+	{
+		(void)hipGetLastError();
+		c = batch_classes(64 * 1024, ctx.batch_shape);
+	}
+	ctx.batch_cls = c;
+	ctx.batch_cls_ready = true;
+	return ctx.batch_cls;
+}
+
 // Memory of one batched call.  words: batch_list_words(P) device words (status, class counts, id
 // lists, cursor); stats: 4 device words per mesh or null; quadrics: one per vertex of the batch;
 // host: batch_host_words(P) words of host memory (status read-back, retry list).  Meshes that fit
@@ -1427,32 +1520,9 @@ static int simplify_batch_on_device(Context& ctx, const BatchMemory& mem, unsign
 	if (!patch_count)
 		return 0;
 
-	// the classes beyond 64 KB of LDS need the device's limit and the kernel's permission; without
-	// them the classes are those of a 64 KB workgroup
-	static const BatchClasses classes = [&] {
-		int max_block = 0;
-		HIP_CHECK(hipDeviceGetAttribute(&max_block, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx.device));
-		BatchClasses c = batch_classes((uint32_t)max_block);
-		size_t top = 0;
-		for (int k = 0; k < kBatchClasses - 1; ++k)
-			if (c.icap[k])
-				top = std::max(top, batch_lds_bytes(c.vcap[k], c.icap[k], c.ccap[k]));
-		const int w = kBatchClasses - 1;
-		const size_t wide = c.icap[w] ? batch_wide_lds_bytes(c.vcap[w], c.icap[w], c.ccap[w]) : 0;
-		if ((top > 64 * 1024 && !batch_allow_lds(top)) || (wide > 64 * 1024 && !batch_wide_allow_lds(wide)))
-		{
-			(void)hipGetLastError();
-			c = batch_classes(64 * 1024);
-		}
-		return c;
-	}();
-	// the one-workgroup-per-CU class loops over its meshes on one workgroup per CU: a grid of
-	// patch_count workgroups of the whole LDS would make each leaving workgroup wait for a free CU
-	static const size_t cu_count = [&] {
-		int n = 0;
-		HIP_CHECK(hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, ctx.device));
-		return (size_t)(n > 0 ? n : 1);
-	}();
+	const BatchClasses& classes = ensure_batch_classes(ctx);
+	const size_t cu_count = ctx.cu_count;
+	const BatchShape& shape = classes.shape;
 	const size_t P = patch_count;
 	uint32_t* status = mem.words;
 	uint32_t* counts = mem.words + P;
@@ -1516,16 +1586,23 @@ static int simplify_batch_on_device(Context& ctx, const BatchMemory& mem, unsign
 				big_early |= n > classes.icap[kBatchRetryClass] && n <= classes.icap[big];
 			}
 		}
+		// a class's workgroups: one per mesh slot (those past the class's count leave at once), or
+		// with the cursor form `per_cu` per CU looping over the class's list
+		auto launch_class = [&](hipStream_t cs, int k, const uint32_t* list, const uint32_t* count_dev, size_t meshes_bound) {
+			const bool from_cursor = shape.cursor[k] != 0;
+			const size_t grid = from_cursor ? std::min(meshes_bound, (size_t)shape.per_cu[k] * cu_count) : meshes_bound;
+			launch_batch(cs, shape.threads[k], a, classes.vcap[k], classes.icap[k], classes.ccap[k], list, count_dev, grid,
+			             from_cursor ? ids + 2 * (size_t)kBatchClasses * P + k : nullptr);
+		};
 		if (big_early)
 		{
 			HIP_CHECK(hipStreamWaitEvent(ctx.aux[big - 1], ctx.fork_ev, 0));
-			launch_batch_wide(ctx.aux[big - 1], a, classes.vcap[big], classes.icap[big], classes.ccap[big], ids + (size_t)big * P, counts + big,
-			                  std::min(P, cu_count), ids + 2 * (size_t)kBatchClasses * P);
+			launch_class(ctx.aux[big - 1], big, ids + (size_t)big * P, counts + big, P);
 			HIP_CHECK(hipEventRecord(ctx.join_ev[big - 1], ctx.aux[big - 1]));
 			HIP_CHECK(hipStreamWaitEvent(s, ctx.join_ev[big - 1], 0));
 		}
-		// classes 0, 1 and 2 launch at once with one workgroup per mesh slot (those past the class's
-		// count leave at once), the class of larger meshes first: its longer workgroups start early
+		// classes 0, 1 and 2 launch at once (in the default shape with one workgroup per mesh slot),
+		// the class of larger meshes first: its longer workgroups start early
 		// and leave a shorter tail (measured 0.6% faster on the 4096-patch batch than class order).
 		// A workgroup of class 3 takes a whole CU's LDS, so even one that leaves at once waits for
 		// a free CU: class 3 launches after the status read-back, only when it has meshes, with at
@@ -1539,7 +1616,7 @@ static int simplify_batch_on_device(Context& ctx, const BatchMemory& mem, unsign
 			if (c)
 				HIP_CHECK(hipStreamWaitEvent(cs, ctx.fork_ev, 0));
 			if (classes.icap[k] && !small_empty)
-				launch_batch(cs, a, classes.vcap[k], classes.icap[k], classes.ccap[k], ids + (size_t)k * P, counts + k, P);
+				launch_class(cs, k, ids + (size_t)k * P, counts + k, P);
 			if (c)
 			{
 				HIP_CHECK(hipEventRecord(ctx.join_ev[c - 1], cs));
@@ -1580,8 +1657,7 @@ static int simplify_batch_on_device(Context& ctx, const BatchMemory& mem, unsign
 				const int c = kBatchClasses - 1;
 				HIP_CHECK(hipEventRecord(ctx.fork_ev, s));
 				HIP_CHECK(hipStreamWaitEvent(ls, ctx.fork_ev, 0));
-				launch_batch_wide(ls, a, classes.vcap[c], classes.icap[c], classes.ccap[c], ids + (size_t)c * P, counts + c,
-				                  std::min<size_t>(large, cu_count), ids + 2 * (size_t)kBatchClasses * P);
+				launch_class(ls, c, ids + (size_t)c * P, counts + c, large);
 			}
 			if (retries)
 			{
@@ -1591,7 +1667,7 @@ static int simplify_batch_on_device(Context& ctx, const BatchMemory& mem, unsign
 				h_retry[P] = retries;
 				HIP_CHECK(hipMemcpyAsync(ids, h_retry, retries * sizeof(uint32_t), hipMemcpyHostToDevice, s));
 				HIP_CHECK(hipMemcpyAsync(counts, h_retry + P, sizeof(uint32_t), hipMemcpyHostToDevice, s));
-				launch_batch(s, a, classes.vcap[c], classes.icap[c], classes.ccap[c], ids, counts, retries);
+				launch_class(s, c, ids, counts, retries);
 			}
 			if (large)
 			{
@@ -1794,6 +1870,69 @@ extern "C" int meshopt_hip_simplify_batch_device_noalloc(unsigned int* destinati
 		simplify_batch_on_device(ctx, mem, destination, result_counts, result_errors, indices, index_offsets, vertex_positions, vertex_offsets,
 		                         vertex_lock, vertex_count, vertex_positions_stride, target_index_counts, mesh_count, target_error, options, nullptr,
 		                         nullptr, nullptr);
+		t_last_error.clear();
+		return 0;
+	}
+	catch (const std::exception& e)
+	{
+		t_last_error = e.what();
+		return -1;
+	}
+}
+
+extern "C" int meshopt_hip_batch_set_shape(const unsigned int* per_cu, const unsigned int* threads, const unsigned int* cursor)
+{
+	try
+	{
+		BatchShape t = batch_default_shape();
+		if (per_cu && threads && cursor)
+			for (int k = 0; k < kBatchClasses; ++k)
+			{
+				t.per_cu[k] = per_cu[k];
+				t.threads[k] = threads[k];
+				t.cursor[k] = cursor[k] != 0;
+			}
+		else if (per_cu || threads || cursor)
+			throw std::invalid_argument("meshopt_hip_batch_set_shape: give all three arrays, or none for the default");
+		if (const char* why = batch_shape_error(t))
+			throw std::invalid_argument(std::string("meshopt_hip_batch_set_shape: ") + why);
+		Context& ctx = context();
+		std::lock_guard<std::mutex> guard(ctx.mutex);
+		ctx.batch_shape = t;
+		ctx.batch_shape_set = true;
+		ctx.batch_cls_ready = false;
+		t_last_error.clear();
+		return 0;
+	}
+	catch (const std::exception& e)
+	{
+		t_last_error = e.what();
+		return -1;
+	}
+}
+
+extern "C" int meshopt_hip_batch_get_shape(unsigned int* per_cu, unsigned int* threads, unsigned int* cursor, unsigned int* vertex_caps,
+                                           unsigned int* index_caps)
+{
+	try
+	{
+		Context& ctx = context();
+		std::lock_guard<std::mutex> guard(ctx.mutex);
+		ctx.ensure_stream();
+		const BatchClasses& c = ensure_batch_classes(ctx);
+		for (int k = 0; k < kBatchClasses; ++k)
+		{
+			if (per_cu)
+				per_cu[k] = c.shape.per_cu[k];
+			if (threads)
+				threads[k] = c.shape.threads[k];
+			if (cursor)
+				cursor[k] = c.shape.cursor[k];
+			if (vertex_caps)
+				vertex_caps[k] = c.vcap[k];
+			if (index_caps)
+				index_caps[k] = c.icap[k];
+		}
 		t_last_error.clear();
 		return 0;
 	}

@@ -91,3 +91,23 @@ def test_batch_rejects_bad_offsets_before_any_copy():
                                             verts.ctypes.data, vo.ctypes.data, None, 12, tg.ctypes.data, 2, 1e-2, 0)
         assert rc == -1
         assert "offsets" in lib.meshopt_hip_last_error().decode()
+
+
+def test_batch_shape_validation():
+    """meshopt_hip_batch_set_shape checks a launch shape on the host (no device call): widths of
+    256/512/1024, the wavefronts per SIMD each width's registers allow, workgroups per CU not growing
+    over the classes, the last class one workgroup per CU."""
+    if not os.path.exists(M.LIB_PATH):
+        pytest.skip("libmeshopt_hip.so not built")
+    bad = [
+        ([6, 5, 2, 1], [256, 256, 128, 1024], [0, 0, 0, 1], "256, 512 or 1024"),
+        ([7, 5, 2, 1], [256, 256, 256, 1024], [0, 0, 0, 1], "register budget"),
+        ([4, 5, 2, 1], [256, 256, 256, 1024], [0, 0, 0, 1], "must not grow"),
+        ([6, 5, 2, 2], [256, 256, 256, 256], [0, 0, 0, 1], "whole CU"),
+        ([6, 5, 2, 1], [256, 256, 256, 1024], [0, 0, 0, 1][:3], "four values"),
+    ]
+    for per_cu, threads, cursor, why in bad:
+        with pytest.raises(ValueError, match=why):
+            M.batch_set_shape(per_cu, threads, cursor)
+    M.batch_set_shape([5, 4, 2, 1], [256, 256, 512, 512], [0, 0, 1, 1])
+    M.batch_set_shape()

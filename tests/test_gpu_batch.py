@@ -249,3 +249,31 @@ def test_edge_value_batch(opts):
         targets.append((ib.size // 3 * int(rng.integers(0, 4)) // 4) * 3)
     announce(f"edge value batch opts={opts}")
     check_batch(meshes, targets, 1e-1, opts)
+
+
+@pytest.mark.parametrize("shape", [
+    ([5, 4, 2, 1], [256, 256, 512, 512], [0, 0, 1, 1]),
+    ([3, 3, 2, 1], [512, 512, 256, 1024], [0, 0, 1, 1]),
+    ([6, 5, 3, 1], [256, 256, 256, 1024], [1, 0, 0, 1]),
+], ids=["mid-widths", "512-first", "cursor-first"])
+def test_batch_shapes(shape):
+    """Launch shapes a tuned table may hold (other workgroups per CU, 512-thread classes, cursor
+    forms): every mesh of a mix over all size classes, through the device entry, against the oracle;
+    the default shape is restored afterwards."""
+    rng = np.random.default_rng(515)
+    meshes, targets = patch_meshes(48, seed=33)
+    for k in range(40):
+        n = int(rng.integers(21, 24)) if k < 20 else int(rng.integers(24, 32))
+        ib, v = random_grid(rng, n=n, seams=3)
+        meshes.append((ib, v))
+        targets.append((ib.size // 3 * int(rng.integers(1, 4)) // 4) * 3)
+    order = rng.permutation(len(meshes))
+    meshes = [meshes[i] for i in order]
+    targets = [targets[i] for i in order]
+    M.batch_set_shape(*shape)
+    try:
+        caps = M.batch_get_shape()
+        announce(f"batch shape {shape}: caps {caps['index_caps']}")
+        check_batch_device(meshes, targets, 1e-2, M.SimplifyLockBorder)
+    finally:
+        M.batch_set_shape()

@@ -61,6 +61,13 @@ def simt_wide():
     return build_simt(1024)
 
 
+@pytest.fixture(scope="module")
+def simt_mid():
+    """The same device code built for 512-thread workgroups (8 wavefronts), a width a tuned batch
+    shape may give a size class."""
+    return build_simt(512)
+
+
 def run_simt(lib, meshes, targets, err, opts, mode=3, seed=1, locks=None, caps=None):
     idx, io, verts, vo, stride = M.pack_meshes(meshes)
     lk = None
@@ -182,6 +189,27 @@ def test_wide_workgroup(simt_wide, seed):
         targets.append((ib.size // 3 * int(rng.integers(0, 4)) // 4) * 3)
     opts = [M.SimplifyErrorAbsolute | M.SimplifyPermissive, M.SimplifyLockBorder, 0][seed]
     check(simt_wide, meshes, targets, 1e-2, opts, caps=(1984, 5952) if seed == 0 else None)
+
+
+@pytest.mark.parametrize("seed", range(2))
+def test_mid_workgroup(simt_mid, seed):
+    """The 512-thread build: patches, soups and grids against the oracle, with one workgroup per
+    mesh and with workgroups interleaved over one cursor."""
+    rng = np.random.default_rng(770 + seed)
+    meshes, targets = [], []
+    for k in range(6):
+        if k % 3 == 0:
+            ib, v = random_soup(rng, nv=int(rng.integers(3, 200)), ntri=int(rng.integers(1, 400)))
+        else:
+            ib, v = random_grid(rng, n=int(rng.integers(3, 24)), seams=k % 2)
+        meshes.append((ib, np.ascontiguousarray(v[:, :3])))
+        targets.append((ib.size // 3 * int(rng.integers(0, 4)) // 4) * 3)
+    opts = [M.SimplifyLockBorder, M.SimplifyErrorAbsolute | M.SimplifyPermissive][seed]
+    check(simt_mid, meshes, targets, 1e-2, opts, seed=seed)
+    got = run_cursor(simt_mid, meshes, targets, 1e-2, opts, groups=2 + seed, seed=seed)
+    for p, ((ib, v), t, (rg, eg, st)) in enumerate(zip(meshes, targets, got)):
+        ro, eo = O.simplify(ib, v, int(t), 1e-2, opts)
+        assert st == 0 and np.array_equal(rg, ro) and same_bits(eg, eo), f"mesh {p}: cursor form differs from the oracle"
 
 
 @pytest.mark.parametrize("opts", [0, M.SimplifyLockBorder | M.SimplifyPermissive])
