@@ -8,6 +8,7 @@ import ctypes
 import os
 import shutil
 import subprocess
+import sys
 
 import numpy as np
 import pytest
@@ -288,3 +289,28 @@ def test_cursor_wrapper_wide(simt_wide):
         assert st == 0, f"mesh {p}: status {st}"
         ro, eo = O.simplify(ib, v, int(t), 1e-2, opts)
         assert r.size == ro.size and np.array_equal(r, ro) and same_bits(e, eo), f"mesh {p}: differs from the oracle"
+
+
+@pytest.mark.parametrize("sanitizer", ["undefined", "address"])
+def test_sanitized_build(sanitizer):
+    """The batched device code built on the host with UBSan or ASan (the harness's fibers switch
+    stacks, which ASan tolerates with a warning), run on patches, soups, seamed grids and edge-value
+    meshes in a child process, every mesh checked against the oracle."""
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("g++ not available")
+    os.makedirs(os.path.dirname(OUT), exist_ok=True)
+    out = OUT.replace(".so", f"_{sanitizer}.{os.getpid()}.so")
+    flags = ["-fsanitize=undefined", "-fno-sanitize-recover=all"] if sanitizer == "undefined" else ["-fsanitize=address"]
+    cmd = [gxx, "-std=c++17", "-O1", "-g", "-fPIC", "-shared", "-ffp-contract=off", "-fno-fast-math", "-Wno-unknown-pragmas", *flags,
+           "-DMHIP_BATCH_THREADS=256", "-I", os.path.join(SIMT, "shim"), "-I", CSRC, os.path.join(SIMT, "simt_batch.cpp"), "-o", out]
+    subprocess.run(cmd, check=True)
+    env = dict(os.environ)
+    if sanitizer == "address":
+        env["LD_PRELOAD"] = subprocess.run([gxx, "-print-file-name=libasan.so"], capture_output=True, text=True).stdout.strip()
+        env["ASAN_OPTIONS"] = "detect_leaks=0"
+    try:
+        res = subprocess.run([sys.executable, os.path.join(SIMT, "run_sanitized.py"), out], capture_output=True, text=True, env=env, timeout=600)
+    finally:
+        os.remove(out)
+    assert res.returncode == 0 and "sanitized run ok" in res.stdout, res.stdout[-2000:] + res.stderr[-4000:]
