@@ -94,9 +94,11 @@ MESHOPTIMIZER_API size_t meshopt_simplifySloppy(unsigned int* destination, const
 MESHOPTIMIZER_API float meshopt_simplifyScale(const float* vertex_positions, size_t vertex_count, size_t vertex_positions_stride);
 
 /**
- * Host allocation callbacks for the library's host-side temporaries (staging copies).
+ * Allocation callbacks for host memory: the index adapters of the C++ overloads below (16-bit index
+ * buffers widened for the call) take their buffers through them, and so would any host temporary of
+ * the library (its entry points stage inputs in device memory, which the library manages itself,
+ * and make no host allocation per call).  Every block is freed before the call that took it returns.
  * Replaces meshopt_setAllocator, src/meshoptimizer.h:1042 (implementation src/allocator.cpp:14-21).
- * Device memory is managed by the library itself.
  */
 MESHOPTIMIZER_API void meshopt_setAllocator(void* (MESHOPTIMIZER_ALLOC_CALLCONV* allocate)(size_t), void (MESHOPTIMIZER_ALLOC_CALLCONV* deallocate)(void*));
 
@@ -104,50 +106,99 @@ MESHOPTIMIZER_API void meshopt_setAllocator(void* (MESHOPTIMIZER_ALLOC_CALLCONV*
 } /* extern "C" */
 #endif
 
+#ifdef __cplusplus
+/* The callbacks meshopt_setAllocator sets (operator new / delete until then), shared by the
+ * library and its callers: the library exports storage(), as the reference does when built with
+ * MESHOPTIMIZER_ALLOC_EXPORT (src/meshoptimizer.h:1194-1244, src/allocator.cpp:6-12). */
+class meshopt_Allocator
+{
+public:
+	struct Storage
+	{
+		void* (MESHOPTIMIZER_ALLOC_CALLCONV* allocate)(size_t);
+		void (MESHOPTIMIZER_ALLOC_CALLCONV* deallocate)(void*);
+	};
+
+	MESHOPTIMIZER_API static Storage& storage();
+};
+#endif
+
 #if defined(__cplusplus) && !defined(MESHOPTIMIZER_NO_WRAPPERS)
-/* Index-type adapters: same role as the templated overloads of src/meshoptimizer.h:1514-1538;
- * 16-bit index buffers are widened, simplified through the C entry point, and narrowed back. */
-#include <string.h>
-#include <vector>
+/* Index-type adapters: same role as the templated overloads of src/meshoptimizer.h:1514-1538.
+ * A 16-bit index buffer is widened into a block from meshopt_Allocator (the role of
+ * meshopt_IndexAdapter, src/meshoptimizer.h:1249-1296), simplified through the C entry point, and
+ * narrowed back into `result` when the adapter goes out of scope; 32-bit buffers are used as they
+ * are. */
+template <typename T, bool Wide = sizeof(T) == sizeof(unsigned int)>
+struct meshopt_IndexAdapter
+{
+	T* result;
+	unsigned int* data;
+	size_t count;
+
+	meshopt_IndexAdapter(T* result_, const T* input, size_t count_)
+	    : result(result_)
+	    , data(NULL)
+	    , count(count_)
+	{
+		const size_t bytes = count > size_t(-1) / sizeof(unsigned int) ? size_t(-1) : count * sizeof(unsigned int);
+		data = static_cast<unsigned int*>(meshopt_Allocator::storage().allocate(bytes));
+		for (size_t i = 0; input && i < count; ++i)
+			data[i] = input[i];
+	}
+
+	~meshopt_IndexAdapter()
+	{
+		for (size_t i = 0; result && i < count; ++i)
+			result[i] = T(data[i]);
+		meshopt_Allocator::storage().deallocate(data);
+	}
+
+private:
+	meshopt_IndexAdapter(const meshopt_IndexAdapter&);
+	meshopt_IndexAdapter& operator=(const meshopt_IndexAdapter&);
+};
+
+template <typename T>
+struct meshopt_IndexAdapter<T, true>
+{
+	unsigned int* data;
+
+	meshopt_IndexAdapter(T* result, const T* input, size_t)
+	    : data(reinterpret_cast<unsigned int*>(result ? result : const_cast<T*>(input)))
+	{
+	}
+};
 
 template <typename T>
 inline size_t meshopt_simplify(T* destination, const T* indices, size_t index_count, const float* vertex_positions, size_t vertex_count, size_t vertex_positions_stride, size_t target_index_count, float target_error, unsigned int options = 0, float* result_error = NULL)
 {
-	std::vector<unsigned int> in(indices, indices + index_count), out(index_count);
-	size_t n = meshopt_simplify(out.data(), in.data(), index_count, vertex_positions, vertex_count, vertex_positions_stride, target_index_count, target_error, options, result_error);
-	for (size_t i = 0; i < n; ++i)
-		destination[i] = T(out[i]);
-	return n;
+	meshopt_IndexAdapter<T> in(NULL, indices, index_count);
+	meshopt_IndexAdapter<T> out(destination, NULL, index_count);
+	return meshopt_simplify(out.data, in.data, index_count, vertex_positions, vertex_count, vertex_positions_stride, target_index_count, target_error, options, result_error);
 }
 
 template <typename T>
 inline size_t meshopt_simplifyWithAttributes(T* destination, const T* indices, size_t index_count, const float* vertex_positions, size_t vertex_count, size_t vertex_positions_stride, const float* vertex_attributes, size_t vertex_attributes_stride, const float* attribute_weights, size_t attribute_count, const unsigned char* vertex_lock, size_t target_index_count, float target_error, unsigned int options = 0, float* result_error = NULL)
 {
-	std::vector<unsigned int> in(indices, indices + index_count), out(index_count);
-	size_t n = meshopt_simplifyWithAttributes(out.data(), in.data(), index_count, vertex_positions, vertex_count, vertex_positions_stride, vertex_attributes, vertex_attributes_stride, attribute_weights, attribute_count, vertex_lock, target_index_count, target_error, options, result_error);
-	for (size_t i = 0; i < n; ++i)
-		destination[i] = T(out[i]);
-	return n;
+	meshopt_IndexAdapter<T> in(NULL, indices, index_count);
+	meshopt_IndexAdapter<T> out(destination, NULL, index_count);
+	return meshopt_simplifyWithAttributes(out.data, in.data, index_count, vertex_positions, vertex_count, vertex_positions_stride, vertex_attributes, vertex_attributes_stride, attribute_weights, attribute_count, vertex_lock, target_index_count, target_error, options, result_error);
 }
 
 template <typename T>
 inline size_t meshopt_simplifyWithUpdate(T* indices, size_t index_count, float* vertex_positions, size_t vertex_count, size_t vertex_positions_stride, float* vertex_attributes, size_t vertex_attributes_stride, const float* attribute_weights, size_t attribute_count, const unsigned char* vertex_lock, size_t target_index_count, float target_error, unsigned int options = 0, float* result_error = NULL)
 {
-	std::vector<unsigned int> io(indices, indices + index_count);
-	size_t n = meshopt_simplifyWithUpdate(io.data(), index_count, vertex_positions, vertex_count, vertex_positions_stride, vertex_attributes, vertex_attributes_stride, attribute_weights, attribute_count, vertex_lock, target_index_count, target_error, options, result_error);
-	for (size_t i = 0; i < n; ++i)
-		indices[i] = T(io[i]);
-	return n;
+	meshopt_IndexAdapter<T> inout(indices, indices, index_count);
+	return meshopt_simplifyWithUpdate(inout.data, index_count, vertex_positions, vertex_count, vertex_positions_stride, vertex_attributes, vertex_attributes_stride, attribute_weights, attribute_count, vertex_lock, target_index_count, target_error, options, result_error);
 }
 
 template <typename T>
 inline size_t meshopt_simplifySloppy(T* destination, const T* indices, size_t index_count, const float* vertex_positions, size_t vertex_count, size_t vertex_positions_stride, const unsigned char* vertex_lock, size_t target_index_count, float target_error, float* result_error = NULL)
 {
-	std::vector<unsigned int> in(indices, indices + index_count), out(index_count);
-	size_t n = meshopt_simplifySloppy(out.data(), in.data(), index_count, vertex_positions, vertex_count, vertex_positions_stride, vertex_lock, target_index_count, target_error, result_error);
-	for (size_t i = 0; i < n; ++i)
-		destination[i] = T(out[i]);
-	return n;
+	meshopt_IndexAdapter<T> in(NULL, indices, index_count);
+	meshopt_IndexAdapter<T> out(destination, NULL, index_count);
+	return meshopt_simplifySloppy(out.data, in.data, index_count, vertex_positions, vertex_count, vertex_positions_stride, vertex_lock, target_index_count, target_error, result_error);
 }
 
 template <typename T>

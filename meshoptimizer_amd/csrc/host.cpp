@@ -123,13 +123,10 @@ struct Context
 	std::vector<meshopt_hip_pass_stats> passes;
 	meshopt_hip_call_stats call{};
 	int head_rounds = 8;
-	// phase-X kernel timing (meshopt_hip_kernel_timing)
+	// phase-X kernel timing of this context's calls (meshopt_hip_kernel_timing), and the events of
+	// the batched calls
 	bool timing = false;
 	std::vector<hipEvent_t> ev_begin, ev_end;
-	double timed_ms = 0;
-	size_t timed_launches = 0, timed_visits = 0;
-	double batch_ms = 0; // the batched kernel, timed when timing is on
-	size_t batch_launches = 0, batch_meshes = 0;
 	hipEvent_t batch_ev[2] = {nullptr, nullptr};
 	double window_fraction = 0; // evaluated / candidates of the previous pass (0: none yet)
 	uint32_t* batch_dev = nullptr;    // batched calls: status, class counts and class id lists, then 4 statistics words per mesh
@@ -318,29 +315,95 @@ static Context& context()
 	return ctx;
 }
 
+// Contexts of the single-mesh entry points: a call takes an idle context (a new one when every
+// context is busy) and hands it back when it returns, so calls from several threads run on their
+// own streams and arenas and overlap on the device, as the reference's calls may run concurrently.
+struct ContextPool
+{
+	std::mutex m;
+	std::vector<std::unique_ptr<Context>> all;
+	std::vector<Context*> idle;
+
+	Context* acquire()
+	{
+		std::lock_guard<std::mutex> guard(m);
+		if (idle.empty())
+		{
+			all.emplace_back(new Context());
+			return all.back().get();
+		}
+		Context* c = idle.back();
+		idle.pop_back();
+		return c;
+	}
+
+	void release(Context* c)
+	{
+		std::lock_guard<std::mutex> guard(m);
+		idle.push_back(c);
+	}
+
+	// frees the device memory and streams of the idle contexts
+	void release_idle()
+	{
+		std::lock_guard<std::mutex> guard(m);
+		for (Context* c : idle)
+		{
+			std::lock_guard<std::mutex> g2(c->mutex);
+			c->release();
+		}
+	}
+};
+
+static ContextPool& pool()
+{
+	static ContextPool p;
+	return p;
+}
+
+struct PooledContext
+{
+	Context* ctx;
+	PooledContext()
+	    : ctx(pool().acquire())
+	{
+	}
+	~PooledContext()
+	{
+		pool().release(ctx);
+	}
+	PooledContext(const PooledContext&) = delete;
+	PooledContext& operator=(const PooledContext&) = delete;
+};
+
+// kernel timing (meshopt_hip_kernel_timing, meshopt_hip_batch_kernel_timing): on or off for
+// every context, and the totals gathered since the last query
+struct TimingTotals
+{
+	std::mutex m;
+	bool on = false;
+	double x_ms = 0;
+	size_t x_launches = 0, x_visits = 0;
+	double batch_ms = 0;
+	size_t batch_launches = 0, batch_meshes = 0;
+};
+
+static TimingTotals& timing_totals()
+{
+	static TimingTotals t;
+	return t;
+}
+
+static bool timing_on()
+{
+	TimingTotals& t = timing_totals();
+	std::lock_guard<std::mutex> guard(t.m);
+	return t.on;
+}
+
 static thread_local std::string t_last_error;
 static thread_local std::vector<meshopt_hip_pass_stats> t_passes;
 static thread_local meshopt_hip_call_stats t_call{};
-
-// host allocation hooks (meshopt_setAllocator)
-static void* (*g_allocate)(size_t) = nullptr;
-static void (*g_deallocate)(void*) = nullptr;
-
-struct HostBuffer
-{
-	void* p = nullptr;
-	explicit HostBuffer(size_t bytes)
-	{
-		p = g_allocate ? g_allocate(bytes) : ::operator new(bytes);
-	}
-	~HostBuffer()
-	{
-		if (g_deallocate)
-			g_deallocate(p);
-		else
-			::operator delete(p);
-	}
-};
 
 // ------------------------------------------------------------------------------------------
 // buffer plan of one call
@@ -584,6 +647,7 @@ static size_t simplify_on_device(Context& ctx, const CallArgs& c, float* result_
 	m.index_count = (uint32_t)I;
 	m.attr_count = (uint32_t)A;
 	KernelTimer timer{};
+	ctx.timing = timing_on();
 	if (ctx.timing)
 	{
 		if (ctx.ev_begin.empty())
@@ -867,14 +931,18 @@ static size_t simplify_on_device(Context& ctx, const CallArgs& c, float* result_
 
 	if (ctx.timing)
 	{
+		double total = 0;
 		for (int i = 0; i < timer.count; ++i)
 		{
 			float ms = 0;
 			HIP_CHECK(hipEventElapsedTime(&ms, timer.begin[i], timer.end[i]));
-			ctx.timed_ms += ms;
+			total += ms;
 		}
-		ctx.timed_launches += (size_t)timer.count;
-		ctx.timed_visits += h.x_visits;
+		TimingTotals& t = timing_totals();
+		std::lock_guard<std::mutex> guard(t.m);
+		t.x_ms += total;
+		t.x_launches += (size_t)timer.count;
+		t.x_visits += h.x_visits;
 	}
 
 	t_call.passes = (unsigned int)t_passes.size();
@@ -1001,7 +1069,8 @@ static size_t simplify_host(const unsigned int* indices, unsigned int* destinati
                             size_t position_stride, const float* attributes, size_t attribute_stride, const float* weights, size_t attribute_count,
                             const unsigned char* lock, size_t target_index_count, float target_error, unsigned int options, float* result_error)
 {
-	Context& ctx = context();
+	PooledContext pooled;
+	Context& ctx = *pooled.ctx;
 	std::lock_guard<std::mutex> guard(ctx.mutex);
 	return simplify_host_on(ctx, indices, destination, index_count, positions, vertex_count, position_stride, attributes, attribute_stride, weights,
 	                        attribute_count, lock, target_index_count, target_error, options, result_error);
@@ -1253,7 +1322,8 @@ extern "C" size_t meshopt_simplifySloppy(unsigned int* destination, const unsign
 	try
 	{
 		check_args(index_count, vertex_positions_stride, target_index_count, target_error, 0, 0, nullptr);
-		Context& ctx = context();
+		PooledContext pooled;
+		Context& ctx = *pooled.ctx;
 		std::lock_guard<std::mutex> guard(ctx.mutex);
 		ctx.ensure_init();
 		hipStream_t s = ctx.stream;
@@ -1288,7 +1358,8 @@ extern "C" size_t meshopt_hip_simplify_sloppy_device(unsigned int* destination, 
 	try
 	{
 		check_args(index_count, vertex_positions_stride, target_index_count, target_error, 0, 0, nullptr);
-		Context& ctx = context();
+		PooledContext pooled;
+		Context& ctx = *pooled.ctx;
 		std::lock_guard<std::mutex> guard(ctx.mutex);
 		ctx.ensure_init();
 		size_t n = sloppy_on_device(ctx, destination, indices, index_count, vertex_positions, vertex_count, vertex_positions_stride, vertex_lock,
@@ -1313,7 +1384,8 @@ extern "C" size_t meshopt_hip_simplify_device(unsigned int* destination, const u
 	try
 	{
 		check_args(index_count, vertex_positions_stride, target_index_count, target_error, vertex_attributes_stride, attribute_count, attribute_weights);
-		Context& ctx = context();
+		PooledContext pooled;
+		Context& ctx = *pooled.ctx;
 		std::lock_guard<std::mutex> guard(ctx.mutex);
 		ctx.ensure_init();
 		CallArgs c{};
@@ -1350,7 +1422,8 @@ extern "C" size_t meshopt_hip_simplify_with_update_device(unsigned int* indices,
 	try
 	{
 		check_args(index_count, vertex_positions_stride, target_index_count, target_error, vertex_attributes_stride, attribute_count, attribute_weights);
-		Context& ctx = context();
+		PooledContext pooled;
+		Context& ctx = *pooled.ctx;
 		std::lock_guard<std::mutex> guard(ctx.mutex);
 		ctx.ensure_init();
 		CallArgs c{};
@@ -1496,7 +1569,7 @@ static BatchMemory context_batch_memory(Context& ctx, size_t P, size_t vertices,
 	if (need_quadrics)
 		ctx.ensure_batch_quadrics(vertices ? vertices : 1);
 	ctx.batch_last_meshes = P;
-	return BatchMemory{ctx.batch_dev, ctx.batch_dev + batch_list_words(P), ctx.batch_quadrics, ctx.batch_pinned, true, ctx.timing};
+	return BatchMemory{ctx.batch_dev, ctx.batch_dev + batch_list_words(P), ctx.batch_quadrics, ctx.batch_pinned, true, timing_on()};
 }
 
 // Batched call: sizes are validated and meshes sorted into LDS size classes on the device, the
@@ -1628,17 +1701,17 @@ static int simplify_batch_on_device(Context& ctx, const BatchMemory& mem, unsign
 		HIP_CHECK(hipGetLastError());
 		HIP_CHECK(hipMemcpyAsync(h_status, status, (P + kBatchClasses) * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
 		HIP_CHECK(hipStreamSynchronize(s));
-		auto add_time = [&] {
+		auto add_time = [&](size_t launches, size_t meshes) {
 			float ms = 0;
 			HIP_CHECK(hipEventElapsedTime(&ms, ctx.batch_ev[0], ctx.batch_ev[1]));
-			ctx.batch_ms += ms;
+			TimingTotals& t = timing_totals();
+			std::lock_guard<std::mutex> guard(t.m);
+			t.batch_ms += ms;
+			t.batch_launches += launches;
+			t.batch_meshes += meshes;
 		};
 		if (timed)
-		{
-			add_time();
-			ctx.batch_launches += 1;
-			ctx.batch_meshes += patch_count;
-		}
+			add_time(1, patch_count);
 		// class 3's meshes (status 6 until they run), and the meshes with more candidates than
 		// their class's arrays hold (status 5), which run again in the retry class: its arrays
 		// hold the full bound and its caps every mesh of the classes before it
@@ -1680,7 +1753,7 @@ static int simplify_batch_on_device(Context& ctx, const BatchMemory& mem, unsign
 			HIP_CHECK(hipMemcpyAsync(h_status, status, P * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
 			HIP_CHECK(hipStreamSynchronize(s));
 			if (timed)
-				add_time();
+				add_time(0, 0);
 		}
 	}
 	else
@@ -2155,7 +2228,8 @@ extern "C" float meshopt_simplifyScale(const float* vertex_positions, size_t ver
 {
 	try
 	{
-		Context& ctx = context();
+		PooledContext pooled;
+		Context& ctx = *pooled.ctx;
 		std::lock_guard<std::mutex> guard(ctx.mutex);
 		ctx.ensure_init();
 		hipStream_t s = ctx.stream;
@@ -2181,10 +2255,19 @@ extern "C" float meshopt_simplifyScale(const float* vertex_positions, size_t ver
 	}
 }
 
+// the callbacks meshopt_setAllocator sets, exported for the header's index adapters
+// (src/allocator.cpp:6-12 under MESHOPTIMIZER_ALLOC_EXPORT)
+meshopt_Allocator::Storage& meshopt_Allocator::storage()
+{
+	static Storage s = {::operator new, ::operator delete};
+	return s;
+}
+
 extern "C" void meshopt_setAllocator(void* (*allocate)(size_t), void (*deallocate)(void*))
 {
-	g_allocate = allocate;
-	g_deallocate = deallocate;
+	meshopt_Allocator::Storage& s = meshopt_Allocator::storage();
+	s.allocate = allocate ? allocate : static_cast<void* (*)(size_t)>(::operator new);
+	s.deallocate = deallocate ? deallocate : static_cast<void (*)(void*)>(::operator delete);
 }
 
 extern "C" int meshopt_hip_last_call_stats(meshopt_hip_call_stats* call, meshopt_hip_pass_stats* passes, int max_passes)
@@ -2199,33 +2282,33 @@ extern "C" int meshopt_hip_last_call_stats(meshopt_hip_call_stats* call, meshopt
 
 extern "C" void meshopt_hip_batch_kernel_timing(double* total_ms, size_t* launches, size_t* meshes)
 {
-	Context& ctx = context();
-	std::lock_guard<std::mutex> guard(ctx.mutex);
+	TimingTotals& t = timing_totals();
+	std::lock_guard<std::mutex> guard(t.m);
 	if (total_ms)
-		*total_ms = ctx.batch_ms;
+		*total_ms = t.batch_ms;
 	if (launches)
-		*launches = ctx.batch_launches;
+		*launches = t.batch_launches;
 	if (meshes)
-		*meshes = ctx.batch_meshes;
-	ctx.batch_ms = 0;
-	ctx.batch_launches = 0;
-	ctx.batch_meshes = 0;
+		*meshes = t.batch_meshes;
+	t.batch_ms = 0;
+	t.batch_launches = 0;
+	t.batch_meshes = 0;
 }
 
 extern "C" void meshopt_hip_kernel_timing(int enable, double* total_ms, size_t* launches, size_t* ranks)
 {
-	Context& ctx = context();
-	std::lock_guard<std::mutex> guard(ctx.mutex);
+	TimingTotals& t = timing_totals();
+	std::lock_guard<std::mutex> guard(t.m);
 	if (total_ms)
-		*total_ms = ctx.timed_ms;
+		*total_ms = t.x_ms;
 	if (launches)
-		*launches = ctx.timed_launches;
+		*launches = t.x_launches;
 	if (ranks)
-		*ranks = ctx.timed_visits;
-	ctx.timing = enable != 0;
-	ctx.timed_ms = 0;
-	ctx.timed_launches = 0;
-	ctx.timed_visits = 0;
+		*ranks = t.x_visits;
+	t.on = enable != 0;
+	t.x_ms = 0;
+	t.x_launches = 0;
+	t.x_visits = 0;
 }
 
 extern "C" const char* meshopt_hip_last_error(void)
@@ -2235,9 +2318,12 @@ extern "C" const char* meshopt_hip_last_error(void)
 
 extern "C" void meshopt_hip_release(void)
 {
-	Context& ctx = context();
-	std::lock_guard<std::mutex> guard(ctx.mutex);
-	ctx.release();
+	{
+		Context& ctx = context();
+		std::lock_guard<std::mutex> guard(ctx.mutex);
+		ctx.release();
+	}
+	pool().release_idle();
 }
 
 // ------------------------------------------------------------------------------------------

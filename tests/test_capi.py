@@ -111,3 +111,33 @@ def test_batch_shape_validation():
             M.batch_set_shape(per_cu, threads, cursor)
     M.batch_set_shape([5, 4, 2, 1], [256, 256, 512, 512], [0, 0, 1, 1])
     M.batch_set_shape()
+
+
+ALLOC_SRC = os.path.join(ROOT, "tests", "capi", "alloc_test.cpp")
+
+
+def build_alloc_test(out_dir):
+    """Compiles the C++ allocator test against include/ and libmeshopt_hip.so (host code only)."""
+    exe = os.path.join(str(out_dir), "alloc_test")
+    res = subprocess.run(["g++", "-std=c++11", "-O1", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), ALLOC_SRC, "-o", exe,
+                          "-L", os.path.dirname(M.LIB_PATH), "-lmeshopt_hip", "-Wl,-rpath," + os.path.dirname(M.LIB_PATH)],
+                         capture_output=True, text=True)
+    assert res.returncode == 0, res.stderr
+    return exe
+
+
+def test_cpp_overloads_build(tmp_path):
+    """The header's C++ overloads and index adapters compile as C++11 and link against the library."""
+    if not os.path.exists(M.LIB_PATH):
+        pytest.skip("libmeshopt_hip.so not built")
+    build_alloc_test(tmp_path)
+
+
+@pytest.mark.gpu
+def test_set_allocator_counts_adapter_blocks(tmp_path):
+    """meshopt_setAllocator with a counting allocator (demo/tests.cpp:1712-1762): the 16-bit overloads
+    take their adapter blocks through it and free every one before returning, 32-bit calls take
+    none, the default allocator restored is the one used afterwards; 16-bit results equal 32-bit."""
+    exe = build_alloc_test(tmp_path)
+    res = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0 and "alloc ok" in res.stdout, res.stdout + res.stderr

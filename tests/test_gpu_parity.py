@@ -146,3 +146,20 @@ def test_edge_values(seed):
         attrs, weights = v[:, 3:4], np.array([0.5], np.float32)
     announce(f"edge values seed={seed} kind={kind} opts={opts} I={ib.size} V={v.shape[0]}")
     assert_parity(ib, v, target, err, opts, attrs, weights)
+
+
+def test_concurrent_calls():
+    """meshopt_simplify from eight threads at once (each call takes its own context, stream and
+    arena): every result equals the oracle's, as calls of the reference from several threads do."""
+    import concurrent.futures as cf
+    rng = np.random.default_rng(808)
+    cases = []
+    for k in range(24):
+        ib, v = random_grid(rng, n=int(rng.integers(8, 40))) if k % 2 else random_soup(rng, nv=int(rng.integers(50, 400)), ntri=int(rng.integers(50, 800)))
+        cases.append((ib, v, (ib.size // 3 * int(rng.integers(1, 4)) // 4) * 3, float(rng.choice([1e-2, 1e-1]))))
+    announce(f"concurrent calls: {len(cases)} meshes on 8 threads")
+    with cf.ThreadPoolExecutor(max_workers=8) as ex:
+        got = list(ex.map(lambda c: M.simplify(c[0], c[1], c[2], c[3], 0), cases))
+    for k, ((ib, v, t, e), (rg, eg)) in enumerate(zip(cases, got)):
+        ro, eo = O.simplify(ib, v, t, e, 0)
+        assert np.array_equal(rg, ro) and same_bits(eg, eo), f"case {k}: differs from the oracle"
