@@ -117,6 +117,8 @@ typedef struct meshopt_hip_call_stats
 	float vertex_error_sq;
 	unsigned int solve_vertices; /* meshopt_simplifyWithUpdate: primaries the position solve visited */
 	unsigned int solve_rounds;   /* rounds of the ordered position solve */
+	unsigned int read_backs;     /* host synchronisations with the device during the call */
+	unsigned int device_extensions; /* passes whose rank window the device extended to the whole list */
 } meshopt_hip_call_stats;
 
 /* Copies up to max_passes pass records of the calling thread's last call; returns the pass count. */

@@ -92,6 +92,8 @@ class CallStats(ctypes.Structure):
         ("vertex_error_sq", ctypes.c_float),
         ("solve_vertices", ctypes.c_uint),
         ("solve_rounds", ctypes.c_uint),
+        ("read_backs", ctypes.c_uint),
+        ("device_extensions", ctypes.c_uint),
     ]
 
 

@@ -103,6 +103,10 @@ struct CallScalars
 	uint32_t head_sizes[16]; // work-list size at the start of each head round
 	uint32_t x_visits;      // ranks listed over every phase-X launch of the call (roofline accounting)
 	uint32_t x_launches;
+	uint32_t round;         // the commit rounds' base of the pass (list buffers rotate with the round number)
+	uint32_t epoch;         // the round tag before the base round (round minima are tagged, monotonic over the call)
+	uint32_t extended;      // 1 when the pass's window was extended on the device to the whole list
+	uint32_t pick_capacity; // boundEdgeCollapses (src/simplifier.cpp:1292-1308): pick stops before exceeding it
 	uint32_t work[8];     // small counters of setup kernels
 };
 

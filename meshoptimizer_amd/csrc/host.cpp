@@ -583,10 +583,17 @@ static float host_float_bits(uint32_t u)
 	return f;
 }
 
+// candidate lists up to this length are decided in one rank window (no extension phase)
+constexpr size_t kWholeWindow = 131072;
+
+// read-backs of the calling thread's current call (meshopt_hip_call_stats::read_backs)
+static thread_local unsigned int t_read_backs;
+
 static void sync_scalars(Context& ctx, const Mesh& m)
 {
 	HIP_CHECK(hipMemcpyAsync(ctx.sc_host, m.sc, sizeof(CallScalars), hipMemcpyDeviceToHost, ctx.stream));
 	HIP_CHECK(hipStreamSynchronize(ctx.stream));
+	t_read_backs++;
 }
 
 struct CallArgs
@@ -672,6 +679,7 @@ static size_t simplify_on_device(Context& ctx, const CallArgs& c, float* result_
 	m.loop_cur = 0;
 	m.epoch = 0;
 
+	t_read_backs = 0;
 	HIP_CHECK(hipMemsetAsync(m.sc, 0, sizeof(CallScalars), s));
 	k_scalars_init<<<1, 1, 0, s>>>(m.sc);
 	if (I)
@@ -735,6 +743,14 @@ static size_t simplify_on_device(Context& ctx, const CallArgs& c, float* result_
 		HIP_CHECK(hipMemsetAsync(m.pend_src[b], 0xff, (V + 1) * sizeof(uint64_t), s));
 	}
 
+	// the first pass's adjacency and pick go before the set-up read-back, which returns their count
+	launch_pick_capacity(s, m, I);
+	if (!prune && I)
+	{
+		launch_adjacency(s, m, I / 3);
+		launch_pick(s, m, I / 3);
+	}
+
 	sync_scalars(ctx, m);
 	const CallScalars& h = *ctx.sc_host;
 
@@ -746,8 +762,7 @@ static size_t simplify_on_device(Context& ctx, const CallArgs& c, float* result_
 		t_call.kinds[k] = kind_counts[k];
 	t_passes.clear();
 
-	size_t capacity = (I - h.dual_count / 2) + 3;
-	m.capacity = (uint32_t)std::min<size_t>(capacity, 0xffffffffu);
+	m.capacity = h.pick_capacity;
 
 	float vertex_scale = h.extent;
 	float error_scale = (c.options & kOptErrorAbsolute) ? vertex_scale : 1.f;
@@ -758,9 +773,12 @@ static size_t simplify_on_device(Context& ctx, const CallArgs& c, float* result_
 	// Prune: smallest component error still present (src/simplifier.cpp:2466-2470)
 	float prune_next = prune ? host_float_bits(h.prune_min_bits) : 0.f;
 
-	// outside Prune the next pass's adjacency and pick are enqueued right after the index remap,
-	// reading the new triangle count on the device, so one read-back returns both counts
-	bool picked = false;
+	// Outside Prune a pass makes one read-back: when the stop scan of the first rank window finds
+	// no break, the window is extended on the device to the whole list (its rounds and stop scan
+	// are enqueued and leave at once otherwise), and the next pass's adjacency and pick follow the
+	// index remap, reading the new triangle count on the device.  Prune reads the pass's error back
+	// before its component pruning, and picks after it.
+	bool picked = !prune;
 	while (result_count > c.target_index_count)
 	{
 		size_t T = result_count / 3;
@@ -779,50 +797,59 @@ static size_t simplify_on_device(Context& ctx, const CallArgs& c, float* result_
 		uint32_t goal = (uint32_t)((result_count - c.target_index_count) / 3);
 
 		// rank window: the first pass guesses from the triangle goal, later passes from the
-		// fraction of candidates the previous pass evaluated; doubled while no stop is found
+		// fraction of candidates the previous pass evaluated; a short list is one window
 		size_t W = ctx.window_fraction > 0 ? (size_t)(ctx.window_fraction * (double)C * 1.25) : (size_t)goal * 3 / 4;
-		W = std::min(C, std::max<size_t>(W, 32768));
+		W = C <= kWholeWindow ? C : std::min(C, std::max<size_t>(W, 32768));
 		launch_perform_init(s, m, C, W);
 
-		int round = 0;
-		size_t lo = 0;
-		launch_window_extend(s, m, 0, W, round);
-		for (;;)
-		{
-			int heads = ctx.head_rounds;
-			double estimate = (double)(W - lo);
-			for (int hd = 0; hd < heads; ++hd, ++round)
+		const int heads = ctx.head_rounds;
+		// head rounds sized for the window (their grid strides over any list length, so a grid
+		// capped at `max_threads` stays correct), then the tail and the flip-failure resolution
+		auto window_rounds = [&](size_t lo, size_t hi, int head_base, int head_count, size_t max_threads) {
+			double estimate = (double)(hi - lo);
+			for (int hd = 0; hd < head_count; ++hd)
 			{
-				launch_round(s, m, (size_t)estimate + 256, round, hd);
+				launch_round(s, m, std::min((size_t)estimate + 256, max_threads), head_base + hd, hd);
 				estimate *= hd == 0 ? 0.9 : 0.75; // undecided ranks shrink geometrically
 			}
-			launch_round_tail(s, m, round, (uint32_t)(W - lo + 2));
-			launch_resolve(s, m, lo, W);
-			launch_stop_scan(s, m, W, C, goal, error_limit);
-			sync_scalars(ctx, m);
-			if (h.work[3])
-				throw std::runtime_error("priority rounds did not converge (internal error)");
-			round += (int)h.tail_rounds;
-			m.epoch += h.tail_rounds;
-
-			// head rounds for the next window / pass: enough to bring the list under the size one
-			// tail workgroup takes in a single sweep
-			int next = heads + 3;
-			for (int hd = 0; hd < heads && hd < 16; ++hd)
-				if (h.head_sizes[hd] < 1024)
-				{
-					next = hd;
-					break;
-				}
-			ctx.head_rounds = std::max(2, std::min(16, next));
-
-			if (h.stop < W || W == C)
-				break;
-			lo = W;
-			W = std::min(C, W * 2);
-			launch_window_extend(s, m, lo, W, round);
+			launch_round_tail(s, m, head_count, (uint32_t)(hi - lo + 2));
+			launch_resolve(s, m, lo, hi);
+		};
+		launch_window_extend(s, m, 0, W, false);
+		window_rounds(0, W, 0, heads, (size_t)-1);
+		launch_stop_scan(s, m, W, C, goal, error_limit, false);
+		if (W < C)
+		{
+			// the rest of the list, decided only when the window held no break (otherwise every
+			// kernel below leaves at once: a few small grids)
+			launch_window_extend(s, m, W, C, true);
+			window_rounds(W, C, heads, 2, (size_t)64 * 1024);
+			launch_stop_scan(s, m, C, C, goal, error_limit, true);
 		}
-		launch_stop_commit(s, m, W, goal, error_limit);
+		launch_stop_commit(s, m, C, goal, error_limit);
+		// a pass without commits changes nothing below: the loop then ends on the read-back
+		launch_apply(s, m, C);
+		launch_remap_triangles(s, m, T);
+		if (!prune)
+		{
+			launch_adjacency(s, m, T, &m.sc->next_triangles);
+			launch_pick(s, m, T, &m.sc->next_triangles);
+			picked = true;
+		}
+		sync_scalars(ctx, m);
+		if (h.work[3])
+			throw std::runtime_error("priority rounds did not converge (internal error)");
+
+		// head rounds for the next pass: enough to bring the list under the size one tail
+		// workgroup takes in a single sweep
+		int next = heads + 3;
+		for (int hd = 0; hd < heads && hd < 16; ++hd)
+			if (h.head_sizes[hd] < 1024)
+			{
+				next = hd;
+				break;
+			}
+		ctx.head_rounds = std::max(2, std::min(16, next));
 		ctx.window_fraction = std::min(1.0, (double)(h.stop + 1) / (double)C);
 
 		meshopt_hip_pass_stats ps{};
@@ -833,21 +860,19 @@ static size_t simplify_on_device(Context& ctx, const CallArgs& c, float* result_
 		ps.committed = h.committed;
 		ps.skipped = h.skipped;
 		ps.rejected = h.rejected;
-		ps.rounds = (unsigned int)round;
+		ps.rounds = h.round;
 		t_passes.push_back(ps);
+		t_call.device_extensions += h.extended;
 
 		if (h.committed == 0)
-			break;
-
-		launch_apply(s, m, C);
-		launch_remap_triangles(s, m, T);
-		if (!prune)
 		{
-			launch_adjacency(s, m, T, &m.sc->next_triangles);
-			launch_pick(s, m, T, &m.sc->next_triangles);
-			picked = true;
+			// the index remap and loop update ran before the read-back; without a commit the
+			// reference stops before them (src/simplifier.cpp:2523-2524), and the buffers they read
+			// are untouched (both are double-buffered), so those stay current
+			m.ib_cur ^= 1;
+			m.loop_cur ^= 1;
+			break;
 		}
-		sync_scalars(ctx, m);
 		result_count = (size_t)h.next_triangles * 3;
 
 		// components whose error fell under the pass's error (src/simplifier.cpp:2539-2541)
@@ -860,8 +885,7 @@ static size_t simplify_on_device(Context& ctx, const CallArgs& c, float* result_
 		}
 	}
 
-	// the stop scan advanced result_error even in a pass that broke off without commits
-	sync_scalars(ctx, m);
+	// the last read-back (every pass ends with one, after its stop scan) holds the result error
 	result_error_sq = h.result_error;
 	float vertex_error_sq = h.vertex_error;
 
@@ -946,6 +970,7 @@ static size_t simplify_on_device(Context& ctx, const CallArgs& c, float* result_
 	}
 
 	t_call.passes = (unsigned int)t_passes.size();
+	t_call.read_backs = t_read_backs;
 	t_call.result_error_sq = result_error_sq;
 	t_call.vertex_error_sq = vertex_error_sq;
 

@@ -112,11 +112,12 @@ void launch_adjacency(hipStream_t s, Mesh& m, size_t tri_count, const uint32_t* 
 void launch_pick(hipStream_t s, Mesh& m, size_t tri_count, const uint32_t* live = nullptr);
 void launch_rank_sort(hipStream_t s, Mesh& m, size_t cand_count);
 void launch_perform_init(hipStream_t s, Mesh& m, size_t cand_count, size_t window);
-void launch_round(hipStream_t s, Mesh& m, size_t list_bound, int round, int head_index);
-void launch_round_tail(hipStream_t s, Mesh& m, int round, uint32_t max_rounds);
-void launch_window_extend(hipStream_t s, Mesh& m, size_t lo, size_t hi, int round);
+void launch_pick_capacity(hipStream_t s, Mesh& m, size_t index_count);
+void launch_round(hipStream_t s, Mesh& m, size_t list_bound, int head_index, int rel);
+void launch_round_tail(hipStream_t s, Mesh& m, int rel, uint32_t max_rounds);
+void launch_window_extend(hipStream_t s, Mesh& m, size_t lo, size_t hi, bool if_no_stop);
 void launch_resolve(hipStream_t s, Mesh& m, size_t lo, size_t hi);
-void launch_stop_scan(hipStream_t s, Mesh& m, size_t window, size_t cand_count, uint32_t goal, float error_limit);
+void launch_stop_scan(hipStream_t s, Mesh& m, size_t window, size_t cand_count, uint32_t goal, float error_limit, bool conditional);
 void launch_stop_commit(hipStream_t s, Mesh& m, size_t window, uint32_t goal, float error_limit);
 void launch_apply(hipStream_t s, Mesh& m, size_t cand_count);
 void launch_remap_triangles(hipStream_t s, Mesh& m, size_t tri_count);

@@ -135,13 +135,14 @@ __global__ void k_pick_count(const uint32_t* ib, size_t tris_max, const uint32_t
 
 // triangles are taken while `written + 3 <= capacity` (src/simplifier.cpp:1318-1320)
 __global__ void k_pick_write(const uint32_t* ib, size_t tris_max, const uint32_t* live, const uint32_t* remap, const uint8_t* kind, const uint32_t* loop,
-                             const uint32_t* loopback, const uint32_t* offsets, uint32_t capacity, uint32_t* c_v0, uint32_t* c_v1,
+                             const uint32_t* loopback, const uint32_t* offsets, const uint32_t* capacity_word, uint32_t* c_v0, uint32_t* c_v1,
                              float* c_err, uint32_t* out_count)
 {
 	size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
 	const size_t tris = live_tris(tris_max, live);
 	if (t >= tris)
 		return;
+	const uint32_t capacity = *capacity_word;
 	uint32_t base = offsets[t];
 	if ((uint64_t)base + 3 > capacity)
 		return;
@@ -167,8 +168,21 @@ void launch_pick(hipStream_t s, Mesh& m, size_t tri_count, const uint32_t* live)
 		return;
 	k_pick_count<<<grid_for(tri_count), kThreads, 0, s>>>(m.ib[m.ib_cur], tri_count, live, m.remap, m.kind, loop, loopback, m.tri_cnt);
 	scan_exclusive(s, m.tri_cnt, m.tri_off, tri_count, live, nullptr, m.temp);
-	k_pick_write<<<grid_for(tri_count), kThreads, 0, s>>>(m.ib[m.ib_cur], tri_count, live, m.remap, m.kind, loop, loopback, m.tri_off, m.capacity,
-	                                                      m.c_v0, m.c_v1, m.c_err, &m.sc->candidates);
+	k_pick_write<<<grid_for(tri_count), kThreads, 0, s>>>(m.ib[m.ib_cur], tri_count, live, m.remap, m.kind, loop, loopback, m.tri_off,
+	                                                      &m.sc->pick_capacity, m.c_v0, m.c_v1, m.c_err, &m.sc->candidates);
+}
+
+// boundEdgeCollapses (src/simplifier.cpp:1292-1308) from the dual count on the device:
+// index_count - dual / 2 + 3, so the first pick needs no read-back
+__global__ void k_pick_capacity(CallScalars* sc, uint64_t index_count)
+{
+	const uint64_t c = index_count - sc->dual_count / 2 + 3;
+	sc->pick_capacity = c < 0xffffffffull ? (uint32_t)c : 0xffffffffu;
+}
+
+void launch_pick_capacity(hipStream_t s, Mesh& m, size_t index_count)
+{
+	k_pick_capacity<<<1, 1, 0, s>>>(m.sc, (uint64_t)index_count);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -566,9 +580,13 @@ __device__ __forceinline__ void append_wave(bool keep, uint4 w, uint4* out_list,
 // head round, phase X.  Round r reads list (r & 1) / counter (r % 3), appends the survivors
 // to list ((r + 1) & 1) / counter ((r + 1) % 3) and clears counter ((r + 2) % 3).  One rank
 // per thread when the grid is large enough; the grid-stride loop covers any list length.
-__global__ void __launch_bounds__(kThreads) k_round_x(RoundState a, uint32_t round, uint32_t epoch, uint32_t head_index)
+__global__ void __launch_bounds__(kThreads) k_round_x(RoundState a, uint32_t head_index, uint32_t rel)
 {
 	CallScalars* sc = a.sc;
+	// rounds are numbered from a base on the device (the tail leaves it past its last round), so
+	// rounds can follow a tail or a device-side window extension without a read-back; `rel` counts
+	// the head rounds launched since the base was set
+	const uint32_t round = sc->round + rel, epoch = sc->epoch + 1 + rel;
 	const uint32_t n = sc->list_count[round % 3];
 	if (blockIdx.x == 0 && threadIdx.x == 0)
 	{
@@ -591,9 +609,11 @@ __global__ void __launch_bounds__(kThreads) k_round_x(RoundState a, uint32_t rou
 }
 
 // head round, phase Y over the survivors of phase X
-__global__ void __launch_bounds__(kThreads) k_round_y(RoundState a, uint32_t round, uint32_t epoch)
+__global__ void __launch_bounds__(kThreads) k_round_y(RoundState a, uint32_t rel)
 {
-	const uint32_t n = a.sc->list_count[(round + 1) % 3];
+	const CallScalars* sc = a.sc;
+	const uint32_t round = sc->round + rel, epoch = sc->epoch + 1 + rel;
+	const uint32_t n = sc->list_count[(round + 1) % 3];
 	const uint4* list = a.list[(round + 1) & 1];
 	for (uint32_t idx = blockIdx.x * kThreads + threadIdx.x; idx < n; idx += gridDim.x * kThreads)
 		round_y(a, list[idx], epoch);
@@ -604,10 +624,12 @@ constexpr int kTailThreads = 1024;
 // tail: one workgroup runs the remaining rounds back to back (phases separated by barriers)
 // until the list is empty.  The lowest undecided rank decides in every round, so a list of n
 // ranks empties within n + 1 rounds; max_rounds bounds the loop regardless.
-__global__ void __launch_bounds__(kTailThreads) k_round_tail(RoundState a, uint32_t round0, uint32_t epoch0, uint32_t max_rounds)
+__global__ void __launch_bounds__(kTailThreads) k_round_tail(RoundState a, uint32_t rel, uint32_t max_rounds)
 {
 	__shared__ uint32_t s_n, s_out;
 	CallScalars* sc = a.sc;
+	const uint32_t round0 = sc->round + rel, epoch0 = sc->epoch + 1 + rel;
+	__syncthreads(); // every thread has read the base before thread 0 moves it below
 	if (threadIdx.x == 0)
 	{
 		s_n = sc->list_count[round0 % 3];
@@ -649,7 +671,9 @@ __global__ void __launch_bounds__(kTailThreads) k_round_tail(RoundState a, uint3
 	if (threadIdx.x == 0)
 	{
 		sc->list_count[round % 3] = s_n;
-		sc->tail_rounds = done;
+		sc->tail_rounds += done;
+		sc->round = round;
+		sc->epoch = epoch - 1;
 		if (s_n)
 			sc->work[3] = 1; // round guard hit: the host reports an error
 	}
@@ -662,6 +686,8 @@ __global__ void k_perform_init_scalars(CallScalars* sc)
 	sc->list_count[2] = 0;
 	sc->stop_reason = 0;
 	sc->tail_rounds = 0;
+	sc->round = 0;
+	sc->extended = 0;
 }
 
 // candidate ranks grouped by source class (order inside a group is irrelevant)
@@ -733,30 +759,36 @@ void launch_perform_init(hipStream_t s, Mesh& m, size_t cand_count, size_t windo
 	k_perform_init_scalars<<<1, 1, 0, s>>>(m.sc);
 }
 
-// ranks [lo, hi) become the list of round `round` (its counter reads 0 when this runs)
-__global__ void k_window_open(uint4* list, uint32_t lo, uint32_t hi, const uint32_t* r_r0, const uint32_t* r_r1, uint32_t* count, uint32_t* next_count)
+// ranks [lo, hi) become the list of the next round (sc->round).  With `if_no_stop` the window
+// opens only when the stop scan of [0, lo) found no break (sc->stop == lo) and the list goes on:
+// the rest of the list is then decided in the same pass without a read-back (sc->extended = 1);
+// otherwise the list stays empty and the rounds that follow do nothing.
+__global__ void k_window_open(RoundState a, uint32_t lo, uint32_t hi, const uint32_t* r_r0, const uint32_t* r_r1, bool if_no_stop)
 {
+	CallScalars* sc = a.sc;
+	const uint32_t round = sc->round;
+	const bool open = !if_no_stop || (sc->stop >= lo && hi > lo);
 	uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i == 0)
 	{
-		*count = hi - lo;
-		*next_count = 0;
+		sc->list_count[round % 3] = open ? hi - lo : 0;
+		sc->list_count[(round + 1) % 3] = 0;
+		if (if_no_stop)
+			sc->extended = open ? 1u : 0u;
 	}
 	uint32_t k = lo + i;
-	if (k < hi)
-		list[i] = make_uint4(k, r_r0[k], r_r1[k], 0);
+	if (open && k < hi)
+		a.list[round & 1][i] = make_uint4(k, r_r0[k], r_r1[k], 0);
 }
 
-void launch_window_extend(hipStream_t s, Mesh& m, size_t lo, size_t hi, int round)
+void launch_window_extend(hipStream_t s, Mesh& m, size_t lo, size_t hi, bool if_no_stop)
 {
-	k_window_open<<<grid_for(hi > lo ? hi - lo : 1), kThreads, 0, s>>>(m.list[round & 1], (uint32_t)lo, (uint32_t)hi, m.r_r0, m.r_r1,
-	                                                                    &m.sc->list_count[round % 3], &m.sc->list_count[(round + 1) % 3]);
+	k_window_open<<<grid_for(hi > lo ? hi - lo : 1), kThreads, 0, s>>>(round_state(m), (uint32_t)lo, (uint32_t)hi, m.r_r0, m.r_r1, if_no_stop);
 }
 
 // list_estimate sizes the grid (one rank per thread); the kernels stay correct for any length
-void launch_round(hipStream_t s, Mesh& m, size_t list_estimate, int round, int head_index)
+void launch_round(hipStream_t s, Mesh& m, size_t list_estimate, int head_index, int rel)
 {
-	uint32_t e = ++m.epoch;
 	unsigned int blocks = grid_for(list_estimate);
 	blocks = blocks < 1 ? 1 : (blocks > 8192 ? 8192 : blocks);
 	RoundState a = round_state(m);
@@ -764,15 +796,15 @@ void launch_round(hipStream_t s, Mesh& m, size_t list_estimate, int round, int h
 	bool timed = t && t->count < t->capacity;
 	if (timed)
 		HIP_CHECK(hipEventRecord(t->begin[t->count], s));
-	k_round_x<<<blocks, kThreads, 0, s>>>(a, (uint32_t)round, e, (uint32_t)head_index);
+	k_round_x<<<blocks, kThreads, 0, s>>>(a, (uint32_t)head_index, (uint32_t)rel);
 	if (timed)
 		HIP_CHECK(hipEventRecord(t->end[t->count++], s));
-	k_round_y<<<blocks, kThreads, 0, s>>>(a, (uint32_t)round, e);
+	k_round_y<<<blocks, kThreads, 0, s>>>(a, (uint32_t)rel);
 }
 
-void launch_round_tail(hipStream_t s, Mesh& m, int round, uint32_t max_rounds)
+void launch_round_tail(hipStream_t s, Mesh& m, int rel, uint32_t max_rounds)
 {
-	k_round_tail<<<1, kTailThreads, 0, s>>>(round_state(m), (uint32_t)round, m.epoch + 1, max_rounds);
+	k_round_tail<<<1, kTailThreads, 0, s>>>(round_state(m), (uint32_t)rel, max_rounds);
 }
 
 // a failed flip test is a rejection unless a lower commit locked an endpoint first
@@ -872,8 +904,10 @@ __device__ __forceinline__ StopAgg block_scan3(StopAgg v, StopAgg& total)
 }
 
 __global__ void __launch_bounds__(kThreads) k_stop_reduce(const uint8_t* status, const uint32_t* r_v0, const uint8_t* kind, const float* r_err,
-                                                          size_t count, uint32_t* tiles)
+                                                          size_t count, uint32_t* tiles, const CallScalars* sc, bool conditional)
 {
+	if (conditional && !sc->extended)
+		return;
 	size_t base = (size_t)blockIdx.x * kScanTile;
 	StopAgg acc{0, 0, 0};
 	for (int j = 0; j < kScanItems; ++j)
@@ -898,8 +932,10 @@ __global__ void __launch_bounds__(kThreads) k_stop_reduce(const uint8_t* status,
 }
 
 // single block: exclusive prefix of the tile aggregates; the max starts at the incoming result_error
-__global__ void __launch_bounds__(kThreads) k_stop_tiles(uint32_t* tiles, uint32_t ntiles, const CallScalars* sc)
+__global__ void __launch_bounds__(kThreads) k_stop_tiles(uint32_t* tiles, uint32_t ntiles, const CallScalars* sc, bool conditional)
 {
+	if (conditional && !sc->extended)
+		return;
 	StopAgg carry{0, 0, __float_as_uint(sc->result_error)};
 	for (uint32_t b = 0; b < ntiles; b += kThreads)
 	{
@@ -924,8 +960,10 @@ __global__ void __launch_bounds__(kThreads) k_stop_tiles(uint32_t* tiles, uint32
 
 __global__ void __launch_bounds__(kThreads) k_stop_find(const uint8_t* status, const uint32_t* r_v0, const uint8_t* kind, const float* r_err,
                                                         size_t count, size_t cand_count, const uint32_t* tiles, uint32_t goal, float error_limit,
-                                                        CallScalars* sc)
+                                                        CallScalars* sc, bool conditional)
 {
+	if (conditional && !sc->extended)
+		return;
 	size_t base = (size_t)blockIdx.x * kScanTile;
 	StopAgg carry{tiles[blockIdx.x * 3 + 0], tiles[blockIdx.x * 3 + 1], tiles[blockIdx.x * 3 + 2]};
 	const uint32_t half = goal / 2, sixth = goal / 6;
@@ -970,8 +1008,10 @@ __global__ void __launch_bounds__(kThreads) k_stop_find(const uint8_t* status, c
 
 // totals below the stop rank (one atomic per workgroup and counter)
 __global__ void __launch_bounds__(kThreads) k_stop_totals(const uint8_t* status, const uint32_t* r_v0, const uint8_t* kind, const float* r_err,
-                                                          CallScalars* sc)
+                                                          CallScalars* sc, bool conditional)
 {
+	if (conditional && !sc->extended)
+		return;
 	__shared__ uint32_t red[5][4];
 	uint32_t stop = sc->stop;
 	uint32_t v[5] = {0, 0, 0, 0, 0}; // committed, skipped, rejected, tri, max error bits
@@ -1056,8 +1096,10 @@ __global__ void k_revert(const uint8_t* status, const uint32_t* r_r0, const uint
 	} while (v != r);
 }
 
-__global__ void k_stop_reset(CallScalars* sc, uint32_t window)
+__global__ void k_stop_reset(CallScalars* sc, uint32_t window, bool conditional)
 {
+	if (conditional && !sc->extended)
+		return;
 	sc->stop = window;
 	sc->committed = 0;
 	sc->skipped = 0;
@@ -1067,19 +1109,20 @@ __global__ void k_stop_reset(CallScalars* sc, uint32_t window)
 }
 
 // repeatable: reads the decisions of ranks [0, window) and writes only the stop rank (window
-// if no rank in it breaks the loop) and the totals below it
-void launch_stop_scan(hipStream_t s, Mesh& m, size_t window, size_t cand_count, uint32_t goal, float error_limit)
+// if no rank in it breaks the loop) and the totals below it.  `conditional`: only when the window
+// was extended on the device (sc->extended), else every kernel leaves at once
+void launch_stop_scan(hipStream_t s, Mesh& m, size_t window, size_t cand_count, uint32_t goal, float error_limit, bool conditional)
 {
-	k_stop_reset<<<1, 1, 0, s>>>(m.sc, (uint32_t)window);
+	k_stop_reset<<<1, 1, 0, s>>>(m.sc, (uint32_t)window, conditional);
 	if (!window)
 		return;
 	unsigned int tiles = div_up(window, kScanTile);
-	k_stop_reduce<<<tiles, kThreads, 0, s>>>(m.status, m.r_v0, m.kind, m.r_err, window, m.stop_tile);
-	k_stop_tiles<<<1, kThreads, 0, s>>>(m.stop_tile, tiles, m.sc);
-	k_stop_find<<<tiles, kThreads, 0, s>>>(m.status, m.r_v0, m.kind, m.r_err, window, cand_count, m.stop_tile, goal, error_limit, m.sc);
+	k_stop_reduce<<<tiles, kThreads, 0, s>>>(m.status, m.r_v0, m.kind, m.r_err, window, m.stop_tile, m.sc, conditional);
+	k_stop_tiles<<<1, kThreads, 0, s>>>(m.stop_tile, tiles, m.sc, conditional);
+	k_stop_find<<<tiles, kThreads, 0, s>>>(m.status, m.r_v0, m.kind, m.r_err, window, cand_count, m.stop_tile, goal, error_limit, m.sc, conditional);
 	unsigned int blocks = div_up(window, kThreads * 4);
 	blocks = blocks < 1 ? 1 : (blocks > 512 ? 512 : blocks);
-	k_stop_totals<<<blocks, kThreads, 0, s>>>(m.status, m.r_v0, m.kind, m.r_err, m.sc);
+	k_stop_totals<<<blocks, kThreads, 0, s>>>(m.status, m.r_v0, m.kind, m.r_err, m.sc, conditional);
 }
 
 // once every rank of the window is decided: advance result_error and undo the commits at or

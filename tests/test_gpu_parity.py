@@ -115,6 +115,9 @@ def test_workloads_full(name):
     A = m.vertices[:, 3:8] if attrs else None
     w = W.SEAMED_GRID_WEIGHTS if attrs else None
     assert_parity(m.indices, m.vertices, t, 1e-2, 0, A, w)
+    # one read-back per pass, plus the index check, the set-up and the result copy
+    call, passes = M.last_call_stats()
+    assert call["read_backs"] <= len(passes) + 3, (call["read_backs"], len(passes))
 
 
 @pytest.mark.parametrize("seed", range(8))
