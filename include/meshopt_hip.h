@@ -41,21 +41,37 @@ size_t meshopt_hip_simplify_with_update_device(unsigned int* indices, size_t ind
 size_t meshopt_hip_simplify_sloppy_device(unsigned int* destination, const unsigned int* indices, size_t index_count, const float* vertex_positions, size_t vertex_count, size_t vertex_positions_stride, const unsigned char* vertex_lock, size_t target_index_count, float target_error, float* result_error);
 
 /**
- * One clusterlod level on the device (the group loop of clodBuild, demo/clusterlod.h:640-690,
- * without its fallbacks): group g owns indices [index_offsets[g], index_offsets[g+1]) of
- * `indices`, which reference the shared vertex buffer (vertex_count vertices, remap = its position
- * remap, as meshopt_generatePositionRemap gives it).  locks[i] (vertex_count bytes, written)
- * receives the level's locks: meshopt_SimplifyVertex_Lock on every vertex whose position two or
- * more groups use (lockBoundary, demo/clusterlod.h:372-419), ORed with base_locks[i] (may be NULL:
- * the caller's protect and lock bits).  Every group is then simplified as
- * meshopt_simplifyWithAttributes(group indices, the shared vertices, no attributes, locks,
- * target_index_counts[g], target_error, options | meshopt_SimplifySparse) returns it
- * (demo/clusterlod.h:442-451 passes Sparse | ErrorAbsolute | Permissive and FLT_MAX), all groups
- * in one batched launch: result at destination + index_offsets[g] in shared vertex ids,
- * result_counts[g], result_errors[g].  All arrays are device pointers.  Returns 0, or -1 with
- * meshopt_hip_last_error set.  No counterpart in the reference library (clusterlod is its demo).
+ * The per-group fallbacks of clusterlod's simplify (demo/clusterlod.h:461-518), configured as
+ * clodConfig's simplify_* fields (demo/clusterlod.h:42-53) configure them.
  */
-int meshopt_hip_clod_level_device(unsigned int* destination, unsigned int* result_counts, float* result_errors, const unsigned int* indices, const unsigned int* index_offsets, size_t group_count, const float* vertex_positions, size_t vertex_count, size_t vertex_positions_stride, const unsigned int* remap, const unsigned char* base_locks, unsigned char* locks, const unsigned int* target_index_counts, float target_error, unsigned int options);
+typedef struct meshopt_hip_clod_config
+{
+	int fallback_permissive;   /* simplify_fallback_permissive: a group left above its target runs again with meshopt_SimplifyPermissive (only when options lack it) */
+	int fallback_sloppy;       /* simplify_fallback_sloppy: a group still above its target takes meshopt_simplifySloppy over its deindexed triangles */
+	float error_factor_sloppy; /* simplify_error_factor_sloppy: the sloppy result's absolute error is multiplied by this */
+	float error_edge_limit;    /* simplify_error_edge_limit: when > 0, a group's error is limited by its triangles' edge lengths */
+} meshopt_hip_clod_config;
+
+/**
+ * One clusterlod level on the device (the group loop of clodBuild, demo/clusterlod.h:640-690):
+ * group g owns indices [index_offsets[g], index_offsets[g+1]) of `indices`, which reference the
+ * shared vertex buffer (vertex_count vertices, remap = its position remap, as
+ * meshopt_generatePositionRemap gives it).  locks[i] (vertex_count bytes, written) receives the
+ * level's locks: meshopt_SimplifyVertex_Lock on every vertex whose position two or more groups use
+ * (lockBoundary, demo/clusterlod.h:372-419), ORed with base_locks[i] (may be NULL: the caller's
+ * protect and lock bits).  Every group is then simplified as clodBuild's simplify() does it
+ * (demo/clusterlod.h:461-518): meshopt_simplifyWithAttributes(group indices, the shared vertices,
+ * no attributes, locks, target_index_counts[g], target_error, options | meshopt_SimplifySparse)
+ * (clodBuild passes Sparse | ErrorAbsolute | Permissive and FLT_MAX), all groups in one batched
+ * launch; with config (may be NULL: no fallbacks), the groups left above their target run again
+ * with Permissive and then through meshopt_simplifySloppy on their deindexed triangles, and every
+ * error is limited by edge length, as the config's fields say.  A group whose target exceeds its
+ * index count keeps its indices with error 0.  Results: at destination + index_offsets[g] in
+ * shared vertex ids, result_counts[g], result_errors[g] (may be NULL).  All arrays are device
+ * pointers.  Returns 0, or -1 with meshopt_hip_last_error set.  No counterpart in the reference
+ * library (clusterlod is its demo).
+ */
+int meshopt_hip_clod_level_device(unsigned int* destination, unsigned int* result_counts, float* result_errors, const unsigned int* indices, const unsigned int* index_offsets, size_t group_count, const float* vertex_positions, size_t vertex_count, size_t vertex_positions_stride, const unsigned int* remap, const unsigned char* base_locks, unsigned char* locks, const unsigned int* target_index_counts, float target_error, unsigned int options, const meshopt_hip_clod_config* config);
 
 /**
  * Simplifies many independent meshes (for example the cluster groups of a clusterlod build) in

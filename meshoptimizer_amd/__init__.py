@@ -97,6 +97,16 @@ class CallStats(ctypes.Structure):
     ]
 
 
+class ClodConfig(ctypes.Structure):
+    """meshopt_hip_clod_config: clodConfig's fallback fields (demo/clusterlod.h:42-53)."""
+    _fields_ = [
+        ("fallback_permissive", ctypes.c_int),
+        ("fallback_sloppy", ctypes.c_int),
+        ("error_factor_sloppy", ctypes.c_float),
+        ("error_edge_limit", ctypes.c_float),
+    ]
+
+
 _P = ctypes.c_void_p
 _SZ = ctypes.c_size_t
 _F = ctypes.c_float
@@ -136,7 +146,7 @@ def library() -> ctypes.CDLL:
     lib.meshopt_hip_simplify_batch.restype = ctypes.c_int
     lib.meshopt_hip_simplify_batch.argtypes = [_P, _P, _P, _P, _P, _P, _P, _P, _SZ, _P, _SZ, _F, _U]
     lib.meshopt_hip_clod_level_device.restype = ctypes.c_int
-    lib.meshopt_hip_clod_level_device.argtypes = [_P, _P, _P, _P, _P, _SZ, _P, _SZ, _SZ, _P, _P, _P, _P, _F, _U]
+    lib.meshopt_hip_clod_level_device.argtypes = [_P, _P, _P, _P, _P, _SZ, _P, _SZ, _SZ, _P, _P, _P, _P, _F, _U, ctypes.POINTER(ClodConfig)]
     lib.meshopt_hip_simplify_batch_device.restype = ctypes.c_int
     lib.meshopt_hip_simplify_batch_device.argtypes = [_P, _P, _P, _P, _P, _P, _P, _P, _SZ, _SZ, _P, _SZ, _F, _U]
     lib.meshopt_hip_simplify_batch_device_noalloc.restype = ctypes.c_int
@@ -445,12 +455,19 @@ def simplify_batch_device_noalloc(destination_ptr, result_counts_ptr, result_err
 
 def clod_level_device(destination_ptr, result_counts_ptr, result_errors_ptr, indices_ptr, index_offsets_ptr, group_count,
                       positions_ptr, vertex_count, stride, remap_ptr, base_locks_ptr, locks_ptr, target_counts_ptr,
-                      target_error, options):
-    """meshopt_hip_clod_level_device on device pointers (integers).  Raises on error."""
+                      target_error, options, config=None):
+    """meshopt_hip_clod_level_device on device pointers (integers).  config: None (no fallbacks) or
+    a dict with the keys of meshopt_hip_clod_config (fallback_permissive, fallback_sloppy,
+    error_factor_sloppy, error_edge_limit).  Raises on error."""
     lib = library()
+    cfg = None
+    if config is not None:
+        cfg = ClodConfig(int(bool(config.get("fallback_permissive", 0))), int(bool(config.get("fallback_sloppy", 0))),
+                         float(config.get("error_factor_sloppy", 1.0)), float(config.get("error_edge_limit", 0.0)))
     rc = lib.meshopt_hip_clod_level_device(destination_ptr, result_counts_ptr, result_errors_ptr, indices_ptr, index_offsets_ptr,
                                            group_count, positions_ptr, vertex_count, stride, remap_ptr, base_locks_ptr, locks_ptr,
-                                           target_counts_ptr, float(target_error), int(options))
+                                           target_counts_ptr, float(target_error), int(options),
+                                           ctypes.byref(cfg) if cfg is not None else None)
     if rc != 0:
         raise RuntimeError(lib.meshopt_hip_last_error().decode())
 

@@ -52,4 +52,22 @@ void launch_clod_place(hipStream_t s, const ClodBuffers& b, const float* positio
 // destination[index_offsets[g] + k] = order[unique[g] + packed_result[index_offsets[g] + k]], k < counts[g]
 void launch_clod_unpack(hipStream_t s, const ClodBuffers& b, const uint32_t* counts, uint32_t* destination);
 
+// the fallbacks of clodBuild's per-group simplify (demo/clusterlod.h:461-518): gather the packed
+// data of the groups `list` (n of them) at a sub-batch's offsets, and put the sub-batch's results
+// back in place of theirs
+void launch_clod_gather(hipStream_t s, const ClodBuffers& b, const uint32_t* list, uint32_t n, const uint32_t* sub_io, const uint32_t* sub_vo,
+                        uint32_t* sub_idx, float* sub_pos, uint8_t* sub_locks);
+void launch_clod_scatter(hipStream_t s, const ClodBuffers& b, const uint32_t* list, uint32_t n, const uint32_t* sub_io, const uint32_t* sub_result,
+                         const uint32_t* sub_counts, const float* sub_errors, uint32_t* counts, float* errors);
+// simplifyFallback's deindexed subset of group g (n indices: one vertex per index) and the result
+// of the sloppy call over it mapped back to the group's packed ids
+void launch_clod_deindex(hipStream_t s, const ClodBuffers& b, uint32_t g, uint32_t n, float* subset_pos, uint8_t* subset_locks, uint32_t* subset_idx);
+void launch_clod_sloppy_result(hipStream_t s, const ClodBuffers& b, uint32_t g, const uint32_t* lod, uint32_t count, uint32_t* counts, float* errors,
+                               float error);
+// every group's error limited by its input triangles' edge lengths (groups whose target exceeds
+// their size excepted)
+void launch_clod_edge_limit(hipStream_t s, const ClodBuffers& b, const uint32_t* targets, float limit, float* errors);
+// the groups `list` (n of them) keep their own indices, error 0
+void launch_clod_keep(hipStream_t s, const ClodBuffers& b, const uint32_t* list, uint32_t n, uint32_t* counts, float* errors);
+
 } // namespace mhip

@@ -301,4 +301,175 @@ void launch_clod_unpack(hipStream_t s, const ClodBuffers& b, const uint32_t* cou
 		k_clod_unpack<<<b.group_count, kClodThreads, 0, s>>>(b, counts, destination);
 }
 
+// ------------------------------------------------------------------------------------------
+// the per-group fallbacks of clodBuild's simplify (demo/clusterlod.h:461-518)
+
+// groups list[f] of a sub-batch: their packed indices, positions and locks gathered at the
+// sub-batch's offsets (sub_io, sub_vo, host-computed scans of the groups' sizes)
+__global__ void __launch_bounds__(kClodThreads) k_clod_gather(ClodBuffers b, const uint32_t* list, const uint32_t* sub_io, const uint32_t* sub_vo,
+                                                              uint32_t* sub_idx, float* sub_pos, uint8_t* sub_locks)
+{
+	const uint32_t f = blockIdx.x, g = list[f];
+	const uint32_t ib = b.index_offsets[g], ni = b.index_offsets[g + 1] - ib, vb = b.unique[g], nv = b.unique[g + 1] - vb;
+	for (uint32_t k = threadIdx.x; k < ni; k += kClodThreads)
+		sub_idx[sub_io[f] + k] = b.packed_indices[ib + k];
+	for (uint32_t k = threadIdx.x; k < nv; k += kClodThreads)
+	{
+		const size_t at = sub_vo[f] + k;
+		sub_pos[3 * at + 0] = b.packed_positions[3 * ((size_t)vb + k) + 0];
+		sub_pos[3 * at + 1] = b.packed_positions[3 * ((size_t)vb + k) + 1];
+		sub_pos[3 * at + 2] = b.packed_positions[3 * ((size_t)vb + k) + 2];
+		sub_locks[at] = b.packed_locks[vb + k];
+	}
+}
+
+// the sub-batch's results replace those of its groups (a group's packed ids are the same in both)
+__global__ void __launch_bounds__(kClodThreads) k_clod_scatter(ClodBuffers b, const uint32_t* list, const uint32_t* sub_io, const uint32_t* sub_result,
+                                                               const uint32_t* sub_counts, const float* sub_errors, uint32_t* counts, float* errors)
+{
+	const uint32_t f = blockIdx.x, g = list[f];
+	const uint32_t n = sub_counts[f];
+	for (uint32_t k = threadIdx.x; k < n; k += kClodThreads)
+		b.packed_result[b.index_offsets[g] + k] = sub_result[sub_io[f] + k];
+	if (threadIdx.x == 0)
+	{
+		counts[g] = n;
+		if (errors)
+			errors[g] = sub_errors[f];
+	}
+}
+
+void launch_clod_gather(hipStream_t s, const ClodBuffers& b, const uint32_t* list, uint32_t n, const uint32_t* sub_io, const uint32_t* sub_vo,
+                        uint32_t* sub_idx, float* sub_pos, uint8_t* sub_locks)
+{
+	if (n)
+		k_clod_gather<<<n, kClodThreads, 0, s>>>(b, list, sub_io, sub_vo, sub_idx, sub_pos, sub_locks);
+}
+
+void launch_clod_scatter(hipStream_t s, const ClodBuffers& b, const uint32_t* list, uint32_t n, const uint32_t* sub_io, const uint32_t* sub_result,
+                         const uint32_t* sub_counts, const float* sub_errors, uint32_t* counts, float* errors)
+{
+	if (n)
+		k_clod_scatter<<<n, kClodThreads, 0, s>>>(b, list, sub_io, sub_result, sub_counts, sub_errors, counts, errors);
+}
+
+// simplifyFallback's subset (demo/clusterlod.h:426-458): one vertex per index of group g (its
+// position and lock), and the identity index buffer over them
+__global__ void k_clod_deindex(ClodBuffers b, uint32_t g, float* subset_pos, uint8_t* subset_locks, uint32_t* subset_idx)
+{
+	const uint32_t ib = b.index_offsets[g], n = b.index_offsets[g + 1] - ib, vb = b.unique[g];
+	for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+	{
+		const size_t v = (size_t)vb + b.packed_indices[ib + i];
+		subset_pos[3 * (size_t)i + 0] = b.packed_positions[3 * v + 0];
+		subset_pos[3 * (size_t)i + 1] = b.packed_positions[3 * v + 1];
+		subset_pos[3 * (size_t)i + 2] = b.packed_positions[3 * v + 2];
+		subset_locks[i] = b.packed_locks[v];
+		subset_idx[i] = i;
+	}
+}
+
+// the sloppy result over the subset becomes group g's result: subset vertex i is the group's index i
+__global__ void k_clod_sloppy_result(ClodBuffers b, uint32_t g, const uint32_t* lod, uint32_t count, uint32_t* counts, float* errors, float error)
+{
+	const uint32_t ib = b.index_offsets[g];
+	for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < count; k += gridDim.x * blockDim.x)
+		b.packed_result[ib + k] = b.packed_indices[ib + lod[k]];
+	if (blockIdx.x == 0 && threadIdx.x == 0)
+	{
+		counts[g] = count;
+		if (errors)
+			errors[g] = error;
+	}
+}
+
+void launch_clod_deindex(hipStream_t s, const ClodBuffers& b, uint32_t g, uint32_t n, float* subset_pos, uint8_t* subset_locks, uint32_t* subset_idx)
+{
+	k_clod_deindex<<<clod_grid(n ? n : 1), kClodThreads, 0, s>>>(b, g, subset_pos, subset_locks, subset_idx);
+}
+
+void launch_clod_sloppy_result(hipStream_t s, const ClodBuffers& b, uint32_t g, const uint32_t* lod, uint32_t count, uint32_t* counts, float* errors,
+                               float error)
+{
+	k_clod_sloppy_result<<<clod_grid(count ? count : 1), kClodThreads, 0, s>>>(b, g, lod, count, counts, errors, error);
+}
+
+// the edge-length limit (demo/clusterlod.h:489-516): over the group's input triangles, the largest
+// max(shortest squared edge, longest / 4) with the std::max / std::min of the reference (a NaN
+// never replaces the running maximum, which starts at 0: the result is order-independent), then
+// error = min(error, sqrt(that) * limit)
+__device__ __forceinline__ float std_max(float a, float b)
+{
+	return a < b ? b : a;
+}
+
+__device__ __forceinline__ float std_min(float a, float b)
+{
+	return b < a ? b : a;
+}
+
+__global__ void __launch_bounds__(kClodThreads) k_clod_edge_limit(ClodBuffers b, const uint32_t* targets, float limit, float* errors)
+{
+	__shared__ float part[kClodThreads / 64];
+	const uint32_t g = blockIdx.x;
+	const uint32_t ib = b.index_offsets[g], n = b.index_offsets[g + 1] - ib, vb = b.unique[g];
+	// a group whose target exceeds its size keeps its indices and error unchanged (the early return
+	// of demo/clusterlod.h:463-464 comes before the limit)
+	if (targets[g] > n)
+		return;
+	float m = 0.f;
+	for (uint32_t t = threadIdx.x; t < n / 3; t += kClodThreads)
+	{
+		const float* va = b.packed_positions + 3 * ((size_t)vb + b.packed_indices[ib + 3 * t + 0]);
+		const float* vv = b.packed_positions + 3 * ((size_t)vb + b.packed_indices[ib + 3 * t + 1]);
+		const float* vc = b.packed_positions + 3 * ((size_t)vb + b.packed_indices[ib + 3 * t + 2]);
+		const float eab = (va[0] - vv[0]) * (va[0] - vv[0]) + (va[1] - vv[1]) * (va[1] - vv[1]) + (va[2] - vv[2]) * (va[2] - vv[2]);
+		const float eac = (va[0] - vc[0]) * (va[0] - vc[0]) + (va[1] - vc[1]) * (va[1] - vc[1]) + (va[2] - vc[2]) * (va[2] - vc[2]);
+		const float ebc = (vv[0] - vc[0]) * (vv[0] - vc[0]) + (vv[1] - vc[1]) * (vv[1] - vc[1]) + (vv[2] - vc[2]) * (vv[2] - vc[2]);
+		const float emax = std_max(std_max(eab, eac), ebc);
+		const float emin = std_min(std_min(eab, eac), ebc);
+		m = std_max(m, std_max(emin, emax / 4));
+	}
+	for (int o = 32; o > 0; o >>= 1)
+		m = std_max(m, __shfl_xor(m, o, 64));
+	if (__lane_id() == 0)
+		part[threadIdx.x >> 6] = m;
+	__syncthreads();
+	if (threadIdx.x == 0)
+	{
+		float all = 0.f;
+		for (int w = 0; w < kClodThreads / 64; ++w)
+			all = std_max(all, part[w]);
+		errors[g] = std_min(errors[g], sqrtf(all) * limit);
+	}
+}
+
+void launch_clod_edge_limit(hipStream_t s, const ClodBuffers& b, const uint32_t* targets, float limit, float* errors)
+{
+	if (b.group_count)
+		k_clod_edge_limit<<<b.group_count, kClodThreads, 0, s>>>(b, targets, limit, errors);
+}
+
+// groups whose target exceeds their size: the result is the group's own index buffer, error 0
+// (demo/clusterlod.h:463-464 returns the indices and leaves the caller's error, 0 in clodBuild)
+__global__ void __launch_bounds__(kClodThreads) k_clod_keep(ClodBuffers b, const uint32_t* list, uint32_t* counts, float* errors)
+{
+	const uint32_t g = list[blockIdx.x];
+	const uint32_t ib = b.index_offsets[g], n = b.index_offsets[g + 1] - ib;
+	for (uint32_t k = threadIdx.x; k < n; k += kClodThreads)
+		b.packed_result[ib + k] = b.packed_indices[ib + k];
+	if (threadIdx.x == 0)
+	{
+		counts[g] = n;
+		if (errors)
+			errors[g] = 0.f;
+	}
+}
+
+void launch_clod_keep(hipStream_t s, const ClodBuffers& b, const uint32_t* list, uint32_t n, uint32_t* counts, float* errors)
+{
+	if (n)
+		k_clod_keep<<<n, kClodThreads, 0, s>>>(b, list, counts, errors);
+}
+
 } // namespace mhip

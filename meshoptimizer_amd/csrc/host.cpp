@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <dlfcn.h>
+#include <float.h>
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -1116,7 +1117,7 @@ static float sloppy_interpolate(float y, float x0, float y0, float x1, float y1,
 
 static size_t sloppy_on_device(Context& ctx, unsigned int* destination, const unsigned int* indices, size_t index_count, const float* positions,
                                size_t vertex_count, size_t stride, const unsigned char* lock, size_t target_index_count, float target_error,
-                               float* out_result_error)
+                               float* out_result_error, float* out_extent = nullptr)
 {
 	hipStream_t s = ctx.stream;
 	const size_t V = vertex_count, I = index_count, T = I / 3;
@@ -1201,6 +1202,13 @@ static size_t sloppy_on_device(Context& ctx, unsigned int* destination, const un
 	if (V)
 		launch_gather_positions(s, positions, stride / sizeof(float), nullptr, m.praw, V);
 	launch_rescale(s, m);
+	// the extent meshopt_simplifyScale returns for the same vertices (rescalePositions, without the
+	// rescale), asked for by the clusterlod fallback
+	if (out_extent)
+	{
+		HIP_CHECK(hipMemcpyAsync(out_extent, &m.sc->extent, sizeof(float), hipMemcpyDeviceToHost, s));
+		HIP_CHECK(hipStreamSynchronize(s));
+	}
 
 	auto count_at = [&](int grid_size) -> size_t {
 		launch_sloppy_count(s, b, grid_size);
@@ -2133,11 +2141,82 @@ extern "C" int meshopt_hip_simplify_batch(unsigned int* destination, unsigned in
 	}
 }
 
+// The fallbacks of clodBuild's per-group simplify (demo/clusterlod.h:475-486) for the groups left
+// above their target by the level's batch: with `permissive`, those groups run again as one
+// sub-batch with Permissive added; then, with config.fallback_sloppy, every group still above its
+// target takes meshopt_simplifySloppy over its deindexed triangles (simplifyFallback,
+// demo/clusterlod.h:426-458: one vertex per index, the group's locks, FLT_MAX error), with the
+// error made absolute by the subset's extent and scaled by config.error_factor_sloppy.  Groups
+// whose target exceeds their size (tg[g] > size) have kept their indices and take no fallback.
+static void clod_fallbacks(Context& ctx, const ClodBuffers& b, const meshopt_hip_clod_config& config, const std::vector<uint32_t>& io,
+                           const std::vector<uint32_t>& tg, unsigned int* result_counts, float* result_errors, float target_error,
+                           unsigned int batch_options, bool permissive, uint32_t* d_list, uint32_t* sub_io, uint32_t* sub_vo, uint32_t* sub_tg,
+                           uint32_t* sub_idx, float* sub_pos, uint8_t* sub_locks, uint32_t* sub_result, uint32_t* sub_counts, float* sub_errors)
+{
+	hipStream_t s = ctx.stream;
+	const size_t G = tg.size();
+	std::vector<uint32_t> cnt(G), vo(G + 1);
+	HIP_CHECK(hipMemcpyAsync(cnt.data(), result_counts, 4 * G, hipMemcpyDeviceToHost, s));
+	HIP_CHECK(hipMemcpyAsync(vo.data(), b.unique, 4 * (G + 1), hipMemcpyDeviceToHost, s));
+	HIP_CHECK(hipStreamSynchronize(s));
+	std::vector<uint32_t> fail;
+	for (size_t g = 0; g < G; ++g)
+		if (tg[g] <= io[g + 1] - io[g] && cnt[g] > tg[g])
+			fail.push_back((uint32_t)g);
+	if (fail.empty())
+		return;
+
+	if (permissive)
+	{
+		const size_t F = fail.size();
+		std::vector<uint32_t> h_io(F + 1), h_vo(F + 1), h_tg(F), h_cnt(F);
+		for (size_t f = 0; f < F; ++f)
+		{
+			const uint32_t g = fail[f];
+			h_io[f + 1] = h_io[f] + (io[g + 1] - io[g]);
+			h_vo[f + 1] = h_vo[f] + (vo[g + 1] - vo[g]);
+			h_tg[f] = tg[g];
+		}
+		HIP_CHECK(hipMemcpyAsync(d_list, fail.data(), 4 * F, hipMemcpyHostToDevice, s));
+		HIP_CHECK(hipMemcpyAsync(sub_io, h_io.data(), 4 * (F + 1), hipMemcpyHostToDevice, s));
+		HIP_CHECK(hipMemcpyAsync(sub_vo, h_vo.data(), 4 * (F + 1), hipMemcpyHostToDevice, s));
+		HIP_CHECK(hipMemcpyAsync(sub_tg, h_tg.data(), 4 * F, hipMemcpyHostToDevice, s));
+		launch_clod_gather(s, b, d_list, (uint32_t)F, sub_io, sub_vo, sub_idx, sub_pos, sub_locks);
+		simplify_batch_on_device(ctx, context_batch_memory(ctx, F, h_io[F], !(batch_options & kOptPrune)), sub_result, sub_counts,
+		                         result_errors ? sub_errors : nullptr, sub_idx, sub_io, sub_pos, sub_vo, sub_locks, h_vo[F], 12, sub_tg, F, target_error,
+		                         batch_options | kOptPermissive, nullptr, nullptr, nullptr, h_io.data());
+		launch_clod_scatter(s, b, d_list, (uint32_t)F, sub_io, sub_result, sub_counts, result_errors ? sub_errors : nullptr, result_counts,
+		                    result_errors);
+		if (!config.fallback_sloppy)
+			return;
+		HIP_CHECK(hipMemcpyAsync(h_cnt.data(), sub_counts, 4 * F, hipMemcpyDeviceToHost, s));
+		HIP_CHECK(hipStreamSynchronize(s));
+		size_t still = 0;
+		for (size_t f = 0; f < F; ++f)
+			if (h_cnt[f] > h_tg[f])
+				fail[still++] = fail[f];
+		fail.resize(still);
+	}
+
+	// one group at a time: its subset lives in the sub-batch's buffers (a group has at most I
+	// indices); the sloppy call's own buffers are the context's arena
+	for (uint32_t g : fail)
+	{
+		const uint32_t n = io[g + 1] - io[g];
+		launch_clod_deindex(s, b, g, n, sub_pos, sub_locks, sub_idx);
+		float error = 0.f, extent = 0.f;
+		const size_t count = sloppy_on_device(ctx, sub_result, sub_idx, n, sub_pos, n, 12, sub_locks, tg[g], FLT_MAX, &error, &extent);
+		error *= extent;
+		error *= config.error_factor_sloppy;
+		launch_clod_sloppy_result(s, b, g, sub_result, (uint32_t)count, result_counts, result_errors, error);
+	}
+}
+
 extern "C" int meshopt_hip_clod_level_device(unsigned int* destination, unsigned int* result_counts, float* result_errors, const unsigned int* indices,
                                              const unsigned int* index_offsets, size_t group_count, const float* vertex_positions,
                                              size_t vertex_count, size_t vertex_positions_stride, const unsigned int* remap,
                                              const unsigned char* base_locks, unsigned char* locks, const unsigned int* target_index_counts,
-                                             float target_error, unsigned int options)
+                                             float target_error, unsigned int options, const meshopt_hip_clod_config* config)
 {
 	try
 	{
@@ -2153,9 +2232,12 @@ extern "C" int meshopt_hip_clod_level_device(unsigned int* destination, unsigned
 		hipStream_t s = ctx.stream;
 		const size_t G = group_count;
 
-		// the group offsets decide every buffer size and bound every probe loop: checked on the host
-		std::vector<uint32_t> io(G + 1);
+		// the group offsets decide every buffer size and bound every probe loop, the targets which
+		// groups keep their indices and which take a fallback: both read on the host
+		std::vector<uint32_t> io(G + 1), tg(G);
 		HIP_CHECK(hipMemcpyAsync(io.data(), index_offsets, (G + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+		if (G)
+			HIP_CHECK(hipMemcpyAsync(tg.data(), target_index_counts, G * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
 		HIP_CHECK(hipStreamSynchronize(s));
 		if (io[0] != 0)
 			throw std::invalid_argument("meshopt_hip_clod_level_device: index_offsets[0] must be 0");
@@ -2179,6 +2261,27 @@ extern "C" int meshopt_hip_clod_level_device(unsigned int* destination, unsigned
 		const size_t o_local = take(8 * I + 4), o_staged = take(4 * I + 4), o_unique = take(4 * (G + 2)), o_order = take(4 * I + 4), o_pidx = take(4 * I + 4);
 		const size_t o_ppos = take(12 * I + 4), o_plock = take(I + 4), o_pres = take(4 * I + 4);
 		const size_t o_temp = take(4 * (scan_temp_words(G + 1) + 64)), o_flag = take(16);
+		// a group whose target exceeds its size keeps its indices (demo/clusterlod.h:463-464): the
+		// batch runs it at its size and its result is replaced afterwards
+		std::vector<uint32_t> keep, clamped;
+		for (size_t g = 0; g < G; ++g)
+			if (tg[g] > io[g + 1] - io[g])
+				keep.push_back((uint32_t)g);
+		if (!keep.empty())
+		{
+			clamped = tg;
+			for (uint32_t g : keep)
+				clamped[g] = io[g + 1] - io[g];
+		}
+		const bool want_permissive = config && config->fallback_permissive && !(options & kOptPermissive);
+		const bool want_sloppy = config && config->fallback_sloppy;
+		const bool fallbacks = want_permissive || want_sloppy;
+		// the fallbacks' sub-batch (at most every group: I indices, I packed vertices), reused by the
+		// sloppy fallback's deindexed subsets (one group at a time)
+		const size_t o_tg = take(keep.empty() ? 0 : 4 * G), o_list = take(4 * (G + 1));
+		const size_t o_sio = take(fallbacks ? 4 * (G + 1) : 0), o_svo = take(fallbacks ? 4 * (G + 1) : 0), o_stg = take(fallbacks ? 4 * G : 0);
+		const size_t o_sidx = take(fallbacks ? 4 * I + 4 : 0), o_spos = take(fallbacks ? 12 * I + 4 : 0), o_slock = take(fallbacks ? I + 4 : 0);
+		const size_t o_sres = take(fallbacks ? 4 * I + 4 : 0), o_scnt = take(fallbacks ? 4 * G : 0), o_serr = take(fallbacks ? 4 * G : 0);
 		char* base = static_cast<char*>(ctx.ensure_clod(used + 256));
 
 		ClodBuffers b;
@@ -2233,10 +2336,33 @@ extern "C" int meshopt_hip_clod_level_device(unsigned int* destination, unsigned
 		launch_clod_place(s, b, vertex_positions, vertex_positions_stride / sizeof(float), locks);
 
 		// the packed groups are the meshes of one batched call (Sparse is what the packing did)
+		const unsigned int* batch_targets = target_index_counts;
+		if (!keep.empty())
+		{
+			uint32_t* d_tg = reinterpret_cast<uint32_t*>(base + o_tg);
+			HIP_CHECK(hipMemcpyAsync(d_tg, clamped.data(), 4 * G, hipMemcpyHostToDevice, s));
+			batch_targets = d_tg;
+		}
+		const unsigned int batch_options = options & ~(unsigned int)kOptSparse;
 		simplify_batch_on_device(ctx, context_batch_memory(ctx, G, I, !(options & kOptPrune)), b.packed_result, result_counts, result_errors, b.packed_indices,
 		                         index_offsets, b.packed_positions, b.unique,
-		                         b.packed_locks, I, 12, target_index_counts, G, target_error, options & ~(unsigned int)kOptSparse, nullptr, nullptr,
+		                         b.packed_locks, I, 12, batch_targets, G, target_error, batch_options, nullptr, nullptr,
 		                         nullptr, io.data());
+		uint32_t* d_list = reinterpret_cast<uint32_t*>(base + o_list);
+		if (!keep.empty())
+		{
+			HIP_CHECK(hipMemcpyAsync(d_list, keep.data(), 4 * keep.size(), hipMemcpyHostToDevice, s));
+			launch_clod_keep(s, b, d_list, (uint32_t)keep.size(), result_counts, result_errors);
+		}
+		if (fallbacks)
+			clod_fallbacks(ctx, b, *config, io, tg, result_counts, result_errors, target_error, batch_options, want_permissive, d_list,
+			               reinterpret_cast<uint32_t*>(base + o_sio), reinterpret_cast<uint32_t*>(base + o_svo),
+			               reinterpret_cast<uint32_t*>(base + o_stg), reinterpret_cast<uint32_t*>(base + o_sidx),
+			               reinterpret_cast<float*>(base + o_spos), reinterpret_cast<uint8_t*>(base + o_slock),
+			               reinterpret_cast<uint32_t*>(base + o_sres), reinterpret_cast<uint32_t*>(base + o_scnt),
+			               reinterpret_cast<float*>(base + o_serr));
+		if (config && config->error_edge_limit > 0 && result_errors)
+			launch_clod_edge_limit(s, b, target_index_counts, config->error_edge_limit, result_errors);
 		launch_clod_unpack(s, b, result_counts, destination);
 		HIP_CHECK(hipStreamSynchronize(s));
 		t_last_error.clear();

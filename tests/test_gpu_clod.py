@@ -137,3 +137,86 @@ def test_clod_level_rejects_bad_input():
         with pytest.raises(RuntimeError, match="index counts"):
             M.clod_level_device(d_dst.ptr, d_cnt.ptr, None, d_bad.ptr, d_io_bad.ptr, 1, d_v.ptr, v.shape[0], 12, d_remap.ptr, None,
                                 d_locks.ptr, d_tg.ptr, FLT_MAX, 0)
+
+
+FALLBACK_CASES = [
+    # (grid, tiles, seed, options, lock fraction, target fraction, config)
+    (64, 4, 11, M.SimplifyErrorAbsolute, 0.5, 8,
+     {"fallback_permissive": 1, "fallback_sloppy": 1, "error_factor_sloppy": 2.0, "error_edge_limit": 0.0}),
+    (64, 4, 12, M.SimplifyErrorAbsolute | M.SimplifyPermissive, 0.5, 8,
+     {"fallback_permissive": 0, "fallback_sloppy": 1, "error_factor_sloppy": 2.0, "error_edge_limit": 0.0}),
+    (48, 3, 13, M.SimplifyErrorAbsolute, 0.15, 4,
+     {"fallback_permissive": 1, "fallback_sloppy": 0, "error_factor_sloppy": 1.0, "error_edge_limit": 0.0}),
+    (64, 5, 14, M.SimplifyErrorAbsolute, 0.4, 6,
+     {"fallback_permissive": 1, "fallback_sloppy": 1, "error_factor_sloppy": 3.0, "error_edge_limit": 0.25}),
+    (96, 2, 15, M.SimplifyErrorAbsolute, 0.3, 6,
+     {"fallback_permissive": 1, "fallback_sloppy": 1, "error_factor_sloppy": 2.0, "error_edge_limit": 1.0}),
+]
+
+
+@pytest.mark.parametrize("n,tiles,seed,opts,lock_frac,div,config", FALLBACK_CASES)
+def test_clod_level_fallbacks(n, tiles, seed, opts, lock_frac, div, config):
+    """clodBuild's per-group simplify with its fallbacks (demo/clusterlod.h:461-518): groups that
+    heavy caller locks and low targets leave above their target run again with Permissive and
+    then through meshopt_simplifySloppy on their deindexed triangles; the edge-length limit caps
+    every error; one group's target exceeds its size (it keeps its indices, error 0).  Every
+    group against the oracle's restatement of the same steps (oracle/clod.py).  The last case's
+    groups of about 14000 indices take the plain path in both the level's batch and the
+    Permissive sub-batch."""
+    from devmem import DeviceArray
+    from oracle import clod as C
+    from meshes import lock_boundary
+    v, groups, remap, _, _ = level_case(n, tiles, seed)
+    rng = np.random.default_rng(seed + 100)
+    base = ((rng.random(v.shape[0]) < lock_frac) * M.SimplifyVertex_Lock).astype(np.uint8)
+    locks_expected = lock_boundary(remap, groups, base)
+    io = np.zeros(len(groups) + 1, np.uint32)
+    io[1:] = np.cumsum([grp.size for grp in groups])
+    idx = np.concatenate(groups).astype(np.uint32)
+    targets = np.array([grp.size // 3 // div * 3 for grp in groups], np.uint32)
+    targets[len(groups) // 2] = groups[len(groups) // 2].size + 3
+    G = len(groups)
+    announce(f"clod level fallbacks grid {n} groups {G} seed {seed} opts {opts} config {config}")
+    with DeviceArray(idx) as d_idx, DeviceArray(io) as d_io, DeviceArray(v) as d_v, DeviceArray(remap) as d_remap, \
+            DeviceArray(base) as d_base, DeviceArray(nbytes=v.shape[0]) as d_locks, DeviceArray(targets) as d_tg, \
+            DeviceArray(nbytes=idx.nbytes) as d_dst, DeviceArray(nbytes=4 * G) as d_cnt, DeviceArray(nbytes=4 * G) as d_err:
+        M.clod_level_device(d_dst.ptr, d_cnt.ptr, d_err.ptr, d_idx.ptr, d_io.ptr, G, d_v.ptr, v.shape[0], 12, d_remap.ptr, d_base.ptr,
+                            d_locks.ptr, d_tg.ptr, FLT_MAX, opts, config)
+        locks = d_locks.to_host(np.uint8, v.shape[0])
+        counts = d_cnt.to_host(np.uint32, G)
+        errors = d_err.to_host(np.float32, G)
+        dst = d_dst.to_host(np.uint32, idx.size)
+        # the same level without result_errors: the same indices
+        M.clod_level_device(d_dst.ptr, d_cnt.ptr, None, d_idx.ptr, d_io.ptr, G, d_v.ptr, v.shape[0], 12, d_remap.ptr, d_base.ptr,
+                            d_locks.ptr, d_tg.ptr, FLT_MAX, opts, config)
+        counts2 = d_cnt.to_host(np.uint32, G)
+        dst2 = d_dst.to_host(np.uint32, idx.size)
+    assert np.array_equal(locks, locks_expected)
+    assert np.array_equal(counts, counts2)
+    plain = {"fallback_permissive": 0, "fallback_sloppy": 0, "error_factor_sloppy": 1.0, "error_edge_limit": 0.0}
+    used = {"permissive": 0, "sloppy": 0, "edge": 0, "keep": 0}
+    for p, grp in enumerate(groups):
+        ro, eo = C.clod_simplify(grp, v, locks_expected, int(targets[p]), opts, config)
+        rg = dst[io[p]:io[p] + counts[p]]
+        assert rg.size == ro.size and np.array_equal(rg, ro), f"group {p} ({grp.size} indices, target {targets[p]}): indices differ"
+        assert np.array_equal(dst2[io[p]:io[p] + counts2[p]], rg)
+        assert same_bits(errors[p], eo), f"group {p}: result_error {errors[p]!r} != {eo!r}"
+        if targets[p] > grp.size:
+            used["keep"] += 1
+            continue
+        r0, _ = C.clod_simplify(grp, v, locks_expected, int(targets[p]), opts, plain)
+        if r0.size > targets[p]:
+            rp, _ = C.clod_simplify(grp, v, locks_expected, int(targets[p]), opts,
+                                    dict(plain, fallback_permissive=config["fallback_permissive"]))
+            used["permissive"] += int(not np.array_equal(rp, r0))
+            used["sloppy"] += int(config["fallback_sloppy"] and rp.size > targets[p])
+        if config["error_edge_limit"] > 0:
+            _, e1 = C.clod_simplify(grp, v, locks_expected, int(targets[p]), opts, dict(config, error_edge_limit=0.0))
+            used["edge"] += int(eo < e1)
+    assert used["keep"] == 1
+    if config["fallback_permissive"] and not (opts & M.SimplifyPermissive):
+        assert used["permissive"] > 0, used
+    if config["fallback_sloppy"]:
+        assert used["sloppy"] > 0, used
+    if config["error_edge_limit"] > 0:
+        assert used["edge"] > 0, used

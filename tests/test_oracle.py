@@ -389,3 +389,52 @@ def test_oracle_matches_reference_full_size(name):
     assert r1.size == r2.size and np.array_equal(r1, r2)
     assert same_bits(e1, e2)
     assert r1.size <= t or e1 > 0
+
+
+def test_clod_edge_limit_closed_forms():
+    """The clusterlod edge-length limit (demo/clusterlod.h:489-516) on triangles whose answer is
+    fixed by hand: an equilateral triangle of side s gives s^2 (its shortest edge), a collinear
+    wire of sides 1, 9 and 10 gives 10^2 / 4 (a quarter of its longest), the largest triangle
+    wins in either order, a point triangle gives 0, and a triangle with a NaN vertex never raises
+    the maximum."""
+    from oracle import clod as C
+    h = np.float32(np.sqrt(np.float32(3)) / 2)
+    eq = np.array([[0, 0, 0], [2, 0, 0], [1, 2 * h, 0]], np.float32)
+    assert C.edge_limit_sq([0, 1, 2], eq) == np.float32(4)
+    wire = np.array([[0, 0, 0], [1, 0, 0], [10, 0, 0]], np.float32)
+    assert C.edge_limit_sq([0, 1, 2], wire) == np.float32(25)
+    both = np.concatenate([eq, wire])
+    assert C.edge_limit_sq([0, 1, 2, 3, 4, 5], both) == np.float32(25)
+    assert C.edge_limit_sq([3, 4, 5, 0, 1, 2], both) == np.float32(25)
+    flat = np.array([[1, 1, 1], [1, 1, 1], [1, 1, 1]], np.float32)
+    assert C.edge_limit_sq([0, 1, 2], flat) == np.float32(0)
+    nan = np.concatenate([eq, np.array([[np.nan, 0, 0], [5, 0, 0], [0, 5, 0]], np.float32)])
+    assert C.edge_limit_sq([3, 4, 5, 0, 1, 2], nan) == np.float32(4)
+    assert C.edge_limit_sq([3, 4, 5], nan) == np.float32(0)
+
+
+def test_clod_simplify_paths():
+    """clusterlod's per-group simplify: a target above the group's size returns the group with
+    error 0; with the sloppy fallback a heavily locked group that the edge collapser leaves above
+    its target comes back at most at the target (the sloppy call's own bound) and its error is the
+    sloppy error times the subset's extent times the factor; the edge limit caps the error at
+    sqrt(edge_limit_sq) * limit."""
+    from oracle import clod as C
+    rng = np.random.default_rng(3)
+    ib, v = random_grid(rng, n=12)
+    locks = (rng.random(v.shape[0]) < 0.6).astype(np.uint8)
+    r, e = C.clod_simplify(ib, v, locks, ib.size + 3, 4)
+    assert np.array_equal(r, ib) and e == 0
+    target = ib.size // 3 // 8 * 3
+    r0, _ = C.clod_simplify(ib, v, locks, target, 4)
+    assert r0.size > target
+    cfg = dict(C.DEFAULT_CONFIG)
+    r1, e1 = C.clod_simplify(ib, v, locks, target, 4, cfg)
+    sub = np.ascontiguousarray(v[ib, :3])
+    rs, es = O.simplify_sloppy(np.arange(ib.size, dtype=np.uint32), sub, target, C.FLT_MAX, lock=locks[ib])
+    assert np.array_equal(r1, ib[rs])
+    assert e1 == np.float32(np.float32(es) * np.float32(O.simplify_scale(sub))) * np.float32(2)
+    cfg["error_edge_limit"] = 1e-3
+    _, e2 = C.clod_simplify(ib, v, locks, target, 4, cfg)
+    assert e2 == min(e1, np.float32(np.sqrt(C.edge_limit_sq(ib, v)) * np.float32(1e-3)))
+    assert e2 < e1
