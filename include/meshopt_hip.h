@@ -159,7 +159,7 @@ int meshopt_hip_last_batch_stats(unsigned int* stats, size_t max_meshes);
 
 /* Launch shape of the batched calls' four LDS size classes (no counterpart in the reference):
  * per class, the workgroups that share one CU (the class's LDS share is the CU's 160 KB over that
- * count, which sets the largest mesh it holds), the threads per workgroup (256, 512 or 1024) and
+ * count, which sets the largest mesh it holds), the threads per workgroup (128, 256, 512 or 1024) and
  * whether its workgroups loop over the class's meshes through a cursor (cursor != 0) rather than
  * taking one mesh each.  The default follows from the LDS budget; at the first batched call the
  * library reads a measured shape from the table that meshoptimizer_amd/tune.py writes

@@ -18,7 +18,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libmeshopt_hip.so")
 BUILD_DIR = os.path.join(HERE, "_build")
 
-SOURCES = ["prims.hip", "setup.hip", "pass.hip", "prune.hip", "batch.hip", "batch_mid.hip", "batch_wide.hip", "sloppy.hip", "solve.hip", "clod.hip", "host.cpp"]
+SOURCES = ["prims.hip", "setup.hip", "pass.hip", "prune.hip", "batch.hip", "batch_narrow.hip", "batch_mid.hip", "batch_wide.hip", "sloppy.hip", "solve.hip", "clod.hip", "host.cpp"]
 
 ARCH = os.environ.get("MESHOPT_HIP_ARCH", "gfx950")
 

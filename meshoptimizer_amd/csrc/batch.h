@@ -37,7 +37,7 @@ struct BatchArgs
 };
 
 // The launch shape of the size classes: per class, the workgroups that share one CU (the class's
-// LDS share is the CU's 160 KB over that count), the threads per workgroup (256, 512 or 1024), and
+// LDS share is the CU's 160 KB over that count), the threads per workgroup (128, 256, 512 or 1024), and
 // whether its workgroups take meshes from a cursor (grid: per_cu workgroups per CU) instead of one
 // workgroup per mesh.  The last class always runs from a cursor and launches after the status
 // read-back (unless the host knows a mesh in its range).  The default comes from the LDS budget;
@@ -50,7 +50,7 @@ struct BatchShape
 };
 
 BatchShape batch_default_shape();
-// null when the shape can run: widths of 256/512/1024, at most 6 wavefronts per SIMD for the
+// null when the shape can run: widths of 128/256/512/1024, at most 6 wavefronts per SIMD for the
 // 80-VGPR widths (4 for 1024), workgroups per CU non-increasing over the classes, the last class
 // one workgroup per CU; otherwise the reason
 const char* batch_shape_error(const BatchShape& shape);

@@ -1,13 +1,24 @@
 // Batched simplification: the size classes, their launch at each workgroup width, and the
 // classify and order kernels.  This translation unit builds the 256-thread device code
-// (batch_device.h); batch_mid.hip and batch_wide.hip build 512 and 1024.
+// (batch_device.h); batch_narrow.hip, batch_mid.hip and batch_wide.hip build 128, 512 and 1024.
 #define MHIP_BATCH_THREADS 256
 #include "batch_kernels.h"
 
 namespace mhip
 {
 
-// the other widths' helpers (batch_kernels.h, built in batch_mid.hip and batch_wide.hip)
+// the other widths' helpers (batch_kernels.h, built in batch_narrow.hip, batch_mid.hip and
+// batch_wide.hip)
+namespace threads128
+{
+size_t width_lds_bytes(uint32_t vmax, uint32_t imax, uint32_t cmax);
+bool width_allow_lds(size_t bytes);
+void width_launch(hipStream_t s, const BatchArgs& a, uint32_t vmax, uint32_t imax, uint32_t cmax, const uint32_t* ids, const uint32_t* count_dev,
+                  size_t grid, uint32_t* cursor);
+#ifdef MESHOPT_BATCH_PROFILE
+void width_profile_add(unsigned long long* out, int reset);
+#endif
+} // namespace threads128
 namespace threads512
 {
 size_t width_lds_bytes(uint32_t vmax, uint32_t imax, uint32_t cmax);
@@ -35,6 +46,8 @@ size_t batch_lds_bytes(uint32_t threads, uint32_t vmax, uint32_t imax, uint32_t 
 		return threads1024::width_lds_bytes(vmax, imax, cmax);
 	if (threads == 512)
 		return threads512::width_lds_bytes(vmax, imax, cmax);
+	if (threads == 128)
+		return threads128::width_lds_bytes(vmax, imax, cmax);
 	return threads256::width_lds_bytes(vmax, imax, cmax);
 }
 
@@ -44,6 +57,8 @@ bool batch_allow_lds(uint32_t threads, size_t bytes)
 		return threads1024::width_allow_lds(bytes);
 	if (threads == 512)
 		return threads512::width_allow_lds(bytes);
+	if (threads == 128)
+		return threads128::width_allow_lds(bytes);
 	return threads256::width_allow_lds(bytes);
 }
 
@@ -54,6 +69,8 @@ void launch_batch(hipStream_t s, uint32_t threads, const BatchArgs& a, uint32_t 
 		threads1024::width_launch(s, a, vmax, imax, cmax, ids, count_dev, grid, cursor);
 	else if (threads == 512)
 		threads512::width_launch(s, a, vmax, imax, cmax, ids, count_dev, grid, cursor);
+	else if (threads == 128)
+		threads128::width_launch(s, a, vmax, imax, cmax, ids, count_dev, grid, cursor);
 	else
 		threads256::width_launch(s, a, vmax, imax, cmax, ids, count_dev, grid, cursor);
 }
@@ -129,12 +146,12 @@ const char* batch_shape_error(const BatchShape& s)
 	for (int k = 0; k < kBatchClasses; ++k)
 	{
 		const uint32_t t = s.threads[k];
-		if (t != 256 && t != 512 && t != 1024)
-			return "threads per workgroup must be 256, 512 or 1024";
+		if (t != 128 && t != 256 && t != 512 && t != 1024)
+			return "threads per workgroup must be 128, 256, 512 or 1024";
 		if (s.per_cu[k] < 1)
 			return "every class needs at least one workgroup per CU";
 		// wavefronts per SIMD: 6 at most for the 80-VGPR widths, 4 for the 1024-thread one
-		const uint32_t per_simd = s.per_cu[k] * (t / 64) / 4;
+		const uint32_t per_simd = (s.per_cu[k] * (t / 64) + 3) / 4;
 		if (per_simd > (t == 1024 ? 4u : 6u))
 			return "more wavefronts per SIMD than the width's register budget allows";
 		if (k && s.per_cu[k] > s.per_cu[k - 1])
@@ -185,11 +202,12 @@ void launch_batch_classify(hipStream_t s, const BatchArgs& a, const BatchClasses
 }
 
 #ifdef MESHOPT_BATCH_PROFILE
-// the phase counters of the three builds of the device code (256, 512 and 1024 threads), summed
+// the phase counters of the four builds of the device code (128, 256, 512 and 1024 threads), summed
 extern "C" void meshopt_hip_batch_profile(unsigned long long* out, int reset)
 {
 	for (int k = 0; k < 32; ++k)
 		out[k] = 0;
+	threads128::width_profile_add(out, reset);
 	threads256::width_profile_add(out, reset);
 	threads512::width_profile_add(out, reset);
 	threads1024::width_profile_add(out, reset);

@@ -6,7 +6,7 @@
 #include "batch_device.h"
 #include "prims.h"
 
-// up to 512 threads the kernel is held to 80 VGPRs, so that six wavefronts share a SIMD; a
+// from 128 to 512 threads the kernel is held to 80 VGPRs, so that six wavefronts share a SIMD; a
 // 1024-thread workgroup has four per SIMD and may take 128
 #if MHIP_BATCH_THREADS < 1024
 #define MHIP_BATCH_OCCUPANCY __attribute__((amdgpu_waves_per_eu(6)))

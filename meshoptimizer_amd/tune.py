@@ -4,7 +4,7 @@
 
 The batched entry points (meshopt_hip_simplify_batch*, the clusterlod level) sort meshes into four
 LDS size classes.  A class's shape is how many of its workgroups share one CU (which sets its LDS
-share and with it the largest mesh it holds), the threads per workgroup (256, 512 or 1024) and
+share and with it the largest mesh it holds), the threads per workgroup (128, 256, 512 or 1024) and
 whether its workgroups take one mesh each or loop over the class's list through a cursor.  The
 library's default shape follows from the LDS budget alone; this tuner measures candidate shapes on a
 batch with the caller's mesh-size histogram (by default the clusterlod-style workload: patches of
@@ -33,13 +33,13 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 TABLE = os.path.join(HERE, "batch_shape.txt")
 
 # per class: (workgroups per CU, threads, cursor form) options; classes must not grow in workgroups
-# per CU, and the wavefronts per SIMD stay within each width's register budget (6 at 256 and 512
-# threads, 4 at 1024)
+# per CU, and the wavefronts per SIMD stay within each width's register budget (6 at 128, 256 and
+# 512 threads, 4 at 1024)
 CLASS_OPTIONS = (
-    [(6, 256, 0), (5, 256, 0), (4, 256, 0), (3, 512, 0)],
-    [(5, 256, 0), (4, 256, 0), (3, 256, 0), (3, 512, 0), (2, 512, 0)],
-    [(2, 256, 0), (2, 256, 1), (3, 256, 0), (2, 512, 0), (2, 512, 1)],
-    [(1, 1024, 1), (1, 512, 1)],
+    [(12, 128, 0), (10, 128, 0), (9, 128, 0), (8, 128, 0), (6, 256, 0), (5, 256, 0)],
+    [(9, 128, 0), (8, 128, 0), (7, 128, 0), (6, 128, 0), (6, 256, 0), (5, 256, 0), (4, 256, 0), (3, 512, 0)],
+    [(5, 256, 0), (4, 256, 0), (3, 256, 0), (2, 256, 0), (2, 256, 1), (2, 512, 0)],
+    [(1, 1024, 1)],
 )
 
 

@@ -95,13 +95,14 @@ def test_batch_rejects_bad_offsets_before_any_copy():
 
 def test_batch_shape_validation():
     """meshopt_hip_batch_set_shape checks a launch shape on the host (no device call): widths of
-    256/512/1024, the wavefronts per SIMD each width's registers allow, workgroups per CU not growing
+    128/256/512/1024, the wavefronts per SIMD each width's registers allow, workgroups per CU not growing
     over the classes, the last class one workgroup per CU."""
     if not os.path.exists(M.LIB_PATH):
         pytest.skip("libmeshopt_hip.so not built")
     bad = [
-        ([6, 5, 2, 1], [256, 256, 128, 1024], [0, 0, 0, 1], "256, 512 or 1024"),
+        ([6, 5, 2, 1], [256, 256, 192, 1024], [0, 0, 0, 1], "128, 256, 512 or 1024"),
         ([7, 5, 2, 1], [256, 256, 256, 1024], [0, 0, 0, 1], "register budget"),
+        ([13, 5, 2, 1], [128, 256, 256, 1024], [0, 0, 0, 1], "register budget"),
         ([4, 5, 2, 1], [256, 256, 256, 1024], [0, 0, 0, 1], "must not grow"),
         ([6, 5, 2, 2], [256, 256, 256, 256], [0, 0, 0, 1], "whole CU"),
         ([6, 5, 2, 1], [256, 256, 256, 1024], [0, 0, 0, 1][:3], "four values"),
@@ -110,6 +111,7 @@ def test_batch_shape_validation():
         with pytest.raises(ValueError, match=why):
             M.batch_set_shape(per_cu, threads, cursor)
     M.batch_set_shape([5, 4, 2, 1], [256, 256, 512, 512], [0, 0, 1, 1])
+    M.batch_set_shape([12, 9, 5, 1], [128, 128, 256, 1024], [0, 0, 0, 1])
     M.batch_set_shape()
 
 

@@ -255,10 +255,12 @@ def test_edge_value_batch(opts):
     ([5, 4, 2, 1], [256, 256, 512, 512], [0, 0, 1, 1]),
     ([3, 3, 2, 1], [512, 512, 256, 1024], [0, 0, 1, 1]),
     ([6, 5, 3, 1], [256, 256, 256, 1024], [1, 0, 0, 1]),
-], ids=["mid-widths", "512-first", "cursor-first"])
+    ([12, 9, 5, 1], [128, 128, 256, 1024], [0, 0, 0, 1]),
+    ([10, 6, 3, 1], [128, 256, 256, 1024], [1, 0, 0, 1]),
+], ids=["mid-widths", "512-first", "cursor-first", "narrow", "narrow-cursor"])
 def test_batch_shapes(shape):
-    """Launch shapes a tuned table may hold (other workgroups per CU, 512-thread classes, cursor
-    forms): every mesh of a mix over all size classes, through the device entry, against the oracle;
+    """Launch shapes a tuned table may hold (other workgroups per CU, 128- and 512-thread classes,
+    cursor forms): every mesh of a mix over all size classes, through the device entry, against the oracle;
     the default shape is restored afterwards."""
     rng = np.random.default_rng(515)
     meshes, targets = patch_meshes(48, seed=33)

@@ -69,6 +69,13 @@ def simt_mid():
     return build_simt(512)
 
 
+@pytest.fixture(scope="module")
+def simt_narrow():
+    """The same device code built for 128-thread workgroups (2 wavefronts), the width of a size
+    class of small meshes at twelve workgroups per CU."""
+    return build_simt(128)
+
+
 def run_simt(lib, meshes, targets, err, opts, mode=3, seed=1, locks=None, caps=None):
     idx, io, verts, vo, stride = M.pack_meshes(meshes)
     lk = None
@@ -192,11 +199,12 @@ def test_wide_workgroup(simt_wide, seed):
     check(simt_wide, meshes, targets, 1e-2, opts, caps=(1984, 5952) if seed == 0 else None)
 
 
-@pytest.mark.parametrize("seed", range(2))
-def test_mid_workgroup(simt_mid, seed):
-    """The 512-thread build: patches, soups and grids against the oracle, with one workgroup per
-    mesh and with workgroups interleaved over one cursor."""
-    rng = np.random.default_rng(770 + seed)
+@pytest.mark.parametrize("width,seed", [(512, 0), (512, 1), (128, 0), (128, 1)])
+def test_mid_workgroup(simt_mid, simt_narrow, width, seed):
+    """The 512- and 128-thread builds: patches, soups and grids against the oracle, with one
+    workgroup per mesh and with workgroups interleaved over one cursor."""
+    lib = simt_mid if width == 512 else simt_narrow
+    rng = np.random.default_rng(770 + seed + (width == 128) * 10)
     meshes, targets = [], []
     for k in range(6):
         if k % 3 == 0:
@@ -206,8 +214,8 @@ def test_mid_workgroup(simt_mid, seed):
         meshes.append((ib, np.ascontiguousarray(v[:, :3])))
         targets.append((ib.size // 3 * int(rng.integers(0, 4)) // 4) * 3)
     opts = [M.SimplifyLockBorder, M.SimplifyErrorAbsolute | M.SimplifyPermissive][seed]
-    check(simt_mid, meshes, targets, 1e-2, opts, seed=seed)
-    got = run_cursor(simt_mid, meshes, targets, 1e-2, opts, groups=2 + seed, seed=seed)
+    check(lib, meshes, targets, 1e-2, opts, seed=seed)
+    got = run_cursor(lib, meshes, targets, 1e-2, opts, groups=2 + seed, seed=seed)
     for p, ((ib, v), t, (rg, eg, st)) in enumerate(zip(meshes, targets, got)):
         ro, eo = O.simplify(ib, v, int(t), 1e-2, opts)
         assert st == 0 and np.array_equal(rg, ro) and same_bits(eg, eo), f"mesh {p}: cursor form differs from the oracle"
