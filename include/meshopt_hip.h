@@ -61,8 +61,11 @@ typedef struct meshopt_hip_clod_config
  * (lockBoundary, demo/clusterlod.h:372-419), ORed with base_locks[i] (may be NULL: the caller's
  * protect and lock bits).  Every group is then simplified as clodBuild's simplify() does it
  * (demo/clusterlod.h:461-518): meshopt_simplifyWithAttributes(group indices, the shared vertices,
- * no attributes, locks, target_index_counts[g], target_error, options | meshopt_SimplifySparse)
- * (clodBuild passes Sparse | ErrorAbsolute | Permissive and FLT_MAX), all groups in one batched
+ * the shared attributes (vertex_attributes: device array of vertex_count rows of
+ * vertex_attributes_stride bytes; attribute_weights: host array of attribute_count weights;
+ * attribute_count 0: none), locks, target_index_counts[g], target_error,
+ * options | meshopt_SimplifySparse) (clodBuild passes Sparse | ErrorAbsolute | Permissive and
+ * FLT_MAX, demo/clusterlod.h:466-472), all groups in one batched
  * launch; with config (may be NULL: no fallbacks), the groups left above their target run again
  * with Permissive and then through meshopt_simplifySloppy on their deindexed triangles, and every
  * error is limited by edge length, as the config's fields say.  A group whose target exceeds its
@@ -71,12 +74,14 @@ typedef struct meshopt_hip_clod_config
  * pointers.  Returns 0, or -1 with meshopt_hip_last_error set.  No counterpart in the reference
  * library (clusterlod is its demo).
  */
-int meshopt_hip_clod_level_device(unsigned int* destination, unsigned int* result_counts, float* result_errors, const unsigned int* indices, const unsigned int* index_offsets, size_t group_count, const float* vertex_positions, size_t vertex_count, size_t vertex_positions_stride, const unsigned int* remap, const unsigned char* base_locks, unsigned char* locks, const unsigned int* target_index_counts, float target_error, unsigned int options, const meshopt_hip_clod_config* config);
+int meshopt_hip_clod_level_device(unsigned int* destination, unsigned int* result_counts, float* result_errors, const unsigned int* indices, const unsigned int* index_offsets, size_t group_count, const float* vertex_positions, size_t vertex_count, size_t vertex_positions_stride, const float* vertex_attributes, size_t vertex_attributes_stride, const float* attribute_weights, size_t attribute_count, const unsigned int* remap, const unsigned char* base_locks, unsigned char* locks, const unsigned int* target_index_counts, float target_error, unsigned int options, const meshopt_hip_clod_config* config);
 
 /**
  * Simplifies many independent meshes (for example the cluster groups of a clusterlod build) in
  * one launch; every mesh gets exactly what meshopt_simplifyWithAttributes (src/meshoptimizer.h:537)
- * returns for it with no attributes, its slice of vertex_lock and the shared options.  Mesh p
+ * returns for it with no attributes (the _with_attributes variants: with its slice of the packed
+ * attributes, vertex_attributes_stride bytes per vertex, and the shared host array of
+ * attribute_count weights), its slice of vertex_lock and the shared options.  Mesh p
  * owns indices [index_offsets[p], index_offsets[p+1]) (local to its vertices) and vertices
  * [vertex_offsets[p], vertex_offsets[p+1]) of the packed arrays, and writes its result at
  * destination + index_offsets[p]; result_counts[p] and result_errors[p] (may be NULL) receive
@@ -91,6 +96,8 @@ int meshopt_hip_clod_level_device(unsigned int* destination, unsigned int* resul
  */
 int meshopt_hip_simplify_batch_device(unsigned int* destination, unsigned int* result_counts, float* result_errors, const unsigned int* indices, const unsigned int* index_offsets, const float* vertex_positions, const unsigned int* vertex_offsets, const unsigned char* vertex_lock, size_t vertex_count, size_t vertex_positions_stride, const unsigned int* target_index_counts, size_t mesh_count, float target_error, unsigned int options);
 int meshopt_hip_simplify_batch(unsigned int* destination, unsigned int* result_counts, float* result_errors, const unsigned int* indices, const unsigned int* index_offsets, const float* vertex_positions, const unsigned int* vertex_offsets, const unsigned char* vertex_lock, size_t vertex_positions_stride, const unsigned int* target_index_counts, size_t mesh_count, float target_error, unsigned int options);
+int meshopt_hip_simplify_batch_with_attributes_device(unsigned int* destination, unsigned int* result_counts, float* result_errors, const unsigned int* indices, const unsigned int* index_offsets, const float* vertex_positions, const unsigned int* vertex_offsets, const float* vertex_attributes, size_t vertex_attributes_stride, const float* attribute_weights, size_t attribute_count, const unsigned char* vertex_lock, size_t vertex_count, size_t vertex_positions_stride, const unsigned int* target_index_counts, size_t mesh_count, float target_error, unsigned int options);
+int meshopt_hip_simplify_batch_with_attributes(unsigned int* destination, unsigned int* result_counts, float* result_errors, const unsigned int* indices, const unsigned int* index_offsets, const float* vertex_positions, const unsigned int* vertex_offsets, const float* vertex_attributes, size_t vertex_attributes_stride, const float* attribute_weights, size_t attribute_count, const unsigned char* vertex_lock, size_t vertex_positions_stride, const unsigned int* target_index_counts, size_t mesh_count, float target_error, unsigned int options);
 
 /**
  * meshopt_hip_simplify_batch_device that allocates nothing: no counterpart in the reference; it is

@@ -60,6 +60,8 @@ EXPORTED_SYMBOLS = (
     "meshopt_hip_simplify_batch",
     "meshopt_hip_simplify_batch_device",
     "meshopt_hip_simplify_batch_device_noalloc",
+    "meshopt_hip_simplify_batch_with_attributes",
+    "meshopt_hip_simplify_batch_with_attributes_device",
     "meshopt_hip_clod_level_device",
     "meshopt_hip_batch_kernel_timing",
     "meshopt_hip_last_batch_stats",
@@ -145,8 +147,13 @@ def library() -> ctypes.CDLL:
     lib.meshopt_hip_last_error.argtypes = []
     lib.meshopt_hip_simplify_batch.restype = ctypes.c_int
     lib.meshopt_hip_simplify_batch.argtypes = [_P, _P, _P, _P, _P, _P, _P, _P, _SZ, _P, _SZ, _F, _U]
+    lib.meshopt_hip_simplify_batch_with_attributes.restype = ctypes.c_int
+    lib.meshopt_hip_simplify_batch_with_attributes.argtypes = [_P, _P, _P, _P, _P, _P, _P, _P, _SZ, _P, _SZ, _P, _SZ, _P, _SZ, _F, _U]
+    lib.meshopt_hip_simplify_batch_with_attributes_device.restype = ctypes.c_int
+    lib.meshopt_hip_simplify_batch_with_attributes_device.argtypes = [_P, _P, _P, _P, _P, _P, _P, _P, _SZ, _P, _SZ, _P, _SZ, _SZ, _P, _SZ, _F, _U]
     lib.meshopt_hip_clod_level_device.restype = ctypes.c_int
-    lib.meshopt_hip_clod_level_device.argtypes = [_P, _P, _P, _P, _P, _SZ, _P, _SZ, _SZ, _P, _P, _P, _P, _F, _U, ctypes.POINTER(ClodConfig)]
+    lib.meshopt_hip_clod_level_device.argtypes = [_P, _P, _P, _P, _P, _SZ, _P, _SZ, _SZ, _P, _SZ, _P, _SZ, _P, _P, _P, _P, _F, _U,
+                                                  ctypes.POINTER(ClodConfig)]
     lib.meshopt_hip_simplify_batch_device.restype = ctypes.c_int
     lib.meshopt_hip_simplify_batch_device.argtypes = [_P, _P, _P, _P, _P, _P, _P, _P, _SZ, _SZ, _P, _SZ, _F, _U]
     lib.meshopt_hip_simplify_batch_device_noalloc.restype = ctypes.c_int
@@ -343,10 +350,11 @@ def pack_meshes(meshes):
     return idx, io, verts, vo, width * 4
 
 
-def simplify_batch(meshes, target_index_counts, target_error=1e-2, options=0, locks=None):
+def simplify_batch(meshes, target_index_counts, target_error=1e-2, options=0, locks=None, attributes=None, weights=None):
     """Simplifies every (indices, positions) mesh in one batched launch (meshopt_simplify per mesh,
-    or meshopt_simplifyWithAttributes with no attributes and the mesh's vertex_lock when `locks`
-    gives one uint8 array per mesh).  Returns [(indices, result_error)]."""
+    or meshopt_simplifyWithAttributes with the mesh's vertex_lock when `locks` gives one uint8
+    array per mesh, and its attributes when `attributes` gives one float32 [V, A] array per mesh
+    with the shared `weights`).  Returns [(indices, result_error)]."""
     lib = library()
     idx, io, verts, vo, stride = pack_meshes(meshes)
     lk = None
@@ -358,10 +366,20 @@ def simplify_batch(meshes, target_index_counts, target_error=1e-2, options=0, lo
     dest = np.zeros(max(idx.size, 1), dtype=np.uint32)
     counts = np.zeros(max(len(meshes), 1), dtype=np.uint32)
     errors = np.zeros(max(len(meshes), 1), dtype=np.float32)
-    rc = lib.meshopt_hip_simplify_batch(dest.ctypes.data, counts.ctypes.data, errors.ctypes.data, idx.ctypes.data, io.ctypes.data,
-                                        verts.ctypes.data, vo.ctypes.data, None if lk is None else lk.ctypes.data, stride,
-                                        tg.ctypes.data, len(meshes),
-                                        float(target_error), int(options))
+    if attributes is not None:
+        w = np.ascontiguousarray(weights, dtype=np.float32)
+        at = np.zeros((max(int(vo[-1]), 1), w.size), dtype=np.float32)
+        for p, a in enumerate(attributes):
+            at[vo[p]:vo[p + 1]] = np.asarray(a, dtype=np.float32).reshape(-1, w.size)
+        rc = lib.meshopt_hip_simplify_batch_with_attributes(dest.ctypes.data, counts.ctypes.data, errors.ctypes.data, idx.ctypes.data,
+                                                            io.ctypes.data, verts.ctypes.data, vo.ctypes.data, at.ctypes.data, 4 * w.size,
+                                                            w.ctypes.data, w.size, None if lk is None else lk.ctypes.data, stride,
+                                                            tg.ctypes.data, len(meshes), float(target_error), int(options))
+    else:
+        rc = lib.meshopt_hip_simplify_batch(dest.ctypes.data, counts.ctypes.data, errors.ctypes.data, idx.ctypes.data, io.ctypes.data,
+                                            verts.ctypes.data, vo.ctypes.data, None if lk is None else lk.ctypes.data, stride,
+                                            tg.ctypes.data, len(meshes),
+                                            float(target_error), int(options))
     if rc != 0:
         raise RuntimeError(lib.meshopt_hip_last_error().decode())
     return [(dest[io[p]:io[p] + counts[p]].copy(), float(errors[p])) for p in range(len(meshes))]
@@ -440,6 +458,22 @@ def simplify_batch_device(destination_ptr, result_counts_ptr, result_errors_ptr,
         raise RuntimeError(lib.meshopt_hip_last_error().decode())
 
 
+def simplify_batch_with_attributes_device(destination_ptr, result_counts_ptr, result_errors_ptr, indices_ptr, index_offsets_ptr,
+                                          positions_ptr, vertex_offsets_ptr, attributes_ptr, attribute_stride, weights, vertex_count, stride,
+                                          target_counts_ptr, mesh_count, target_error, options=0, lock_ptr=None):
+    """meshopt_hip_simplify_batch_with_attributes_device on device pointers (integers); weights is a
+    host float32 array (one weight per attribute, attribute_stride bytes per packed vertex).  Raises
+    on error."""
+    lib = library()
+    w = np.ascontiguousarray(weights, dtype=np.float32)
+    rc = lib.meshopt_hip_simplify_batch_with_attributes_device(destination_ptr, result_counts_ptr, result_errors_ptr, indices_ptr,
+                                                               index_offsets_ptr, positions_ptr, vertex_offsets_ptr, attributes_ptr,
+                                                               attribute_stride, w.ctypes.data, w.size, lock_ptr, vertex_count, stride,
+                                                               target_counts_ptr, mesh_count, float(target_error), int(options))
+    if rc != 0:
+        raise RuntimeError(lib.meshopt_hip_last_error().decode())
+
+
 def simplify_batch_device_noalloc(destination_ptr, result_counts_ptr, result_errors_ptr, indices_ptr, index_offsets_ptr,
                                   positions_ptr, vertex_offsets_ptr, vertex_count, stride, target_counts_ptr, mesh_count,
                                   target_error, options=0, lock_ptr=None):
@@ -455,17 +489,23 @@ def simplify_batch_device_noalloc(destination_ptr, result_counts_ptr, result_err
 
 def clod_level_device(destination_ptr, result_counts_ptr, result_errors_ptr, indices_ptr, index_offsets_ptr, group_count,
                       positions_ptr, vertex_count, stride, remap_ptr, base_locks_ptr, locks_ptr, target_counts_ptr,
-                      target_error, options, config=None):
+                      target_error, options, config=None, attributes_ptr=None, attribute_stride=0, weights=None):
     """meshopt_hip_clod_level_device on device pointers (integers).  config: None (no fallbacks) or
     a dict with the keys of meshopt_hip_clod_config (fallback_permissive, fallback_sloppy,
-    error_factor_sloppy, error_edge_limit).  Raises on error."""
+    error_factor_sloppy, error_edge_limit).  attributes_ptr: the shared vertices' attributes
+    (attribute_stride bytes each) with the host float32 array `weights`, or None.  Raises on
+    error."""
     lib = library()
+    w = np.ascontiguousarray(weights if weights is not None else np.zeros(0), dtype=np.float32)
+    if attributes_ptr is None:
+        w = np.zeros(0, dtype=np.float32)
     cfg = None
     if config is not None:
         cfg = ClodConfig(int(bool(config.get("fallback_permissive", 0))), int(bool(config.get("fallback_sloppy", 0))),
                          float(config.get("error_factor_sloppy", 1.0)), float(config.get("error_edge_limit", 0.0)))
     rc = lib.meshopt_hip_clod_level_device(destination_ptr, result_counts_ptr, result_errors_ptr, indices_ptr, index_offsets_ptr,
-                                           group_count, positions_ptr, vertex_count, stride, remap_ptr, base_locks_ptr, locks_ptr,
+                                           group_count, positions_ptr, vertex_count, stride, attributes_ptr, attribute_stride,
+                                           w.ctypes.data if w.size else None, w.size, remap_ptr, base_locks_ptr, locks_ptr,
                                            target_counts_ptr, float(target_error), int(options),
                                            ctypes.byref(cfg) if cfg is not None else None)
     if rc != 0:

@@ -34,6 +34,17 @@ struct BatchArgs
 	size_t patch_count;
 	float target_error;
 	uint32_t options;
+	// attributes (attribute_count 0: none): packed per vertex like the positions; the attributes
+	// with weight > 0 (attr_map) enter the quadrics, rescaled into attr_values; the per-vertex
+	// attribute quadrics and gradients live in global memory beside the vertex quadrics (mesh p
+	// at vertex_offsets[p], like them)
+	const float* attributes;
+	size_t attribute_stride_floats;
+	uint32_t attribute_count;
+	AttrMap attr_map;
+	float* attr_values;   // vertex_count * attribute_count
+	Quad* attr_quadrics;  // vertex_count
+	Grad* attr_grads;     // vertex_count * attribute_count
 };
 
 // The launch shape of the size classes: per class, the workgroups that share one CU (the class's

@@ -369,8 +369,70 @@ __device__ void build_adjacency(const PatchSmem& s, const uint16_t* ib, uint32_t
 	__syncthreads();
 }
 
+// one mesh's attribute state in global memory: rescaled values, and per wedge the attribute
+// quadric and one gradient per attribute
+struct PatchAttr
+{
+	float* va;
+	Quad* aq;
+	Grad* ag;
+	uint32_t count;
+};
+
+// quadricError with attributes (src/simplifier.cpp:784-799): wedge v's attribute quadric and
+// gradients at wedge t's position and attribute values
+__device__ __forceinline__ float patch_attr_error(const PatchAttr& at, const PatchSmem& s, uint32_t v, uint32_t t)
+{
+	return quad_error_attr(load_quad(at.aq, v), at.ag + (size_t)v * at.count, (int)at.count, P(s, t), at.va + (size_t)t * at.count);
+}
+
+// the attribute terms rankEdgeCollapses adds to a candidate v0 -> v1 (src/simplifier.cpp:1391-1431):
+// ei (v0 -> v1) and, for a two-way edge, ej (v1 -> v0), with the seam pair's or the complex
+// wedges' terms after the candidate's own
+__device__ __forceinline__ void patch_attr_rank(const PatchAttr& at, const PatchSmem& s, uint32_t v0, uint32_t v1, bool bidi, float& ei, float& ej)
+{
+	ei += patch_attr_error(at, s, v0, v1);
+	if (bidi)
+		ej += patch_attr_error(at, s, v1, v0);
+	const uint8_t k0 = s.kind[v0];
+	if (k0 == kSeam)
+	{
+		const uint32_t s0 = s.wedge[v0];
+		uint32_t s1 = s.loop[v0] == v1 ? s.loopback[s0] : s.loop[s0];
+		s1 = s1 != kNone16 ? s1 : s.wedge[v1];
+		ei += patch_attr_error(at, s, s0, s1);
+		if (bidi)
+			ej += patch_attr_error(at, s, s1, s0);
+	}
+	else
+	{
+		if (k0 == kComplex)
+			for (uint32_t v = s.wedge[v0]; v != v0; v = s.wedge[v])
+				ei += patch_attr_error(at, s, v, lds_complex_target(s, v, v1));
+		if (s.kind[v1] == kComplex && bidi)
+			for (uint32_t v = s.wedge[v1]; v != v1; v = s.wedge[v])
+				ej += patch_attr_error(at, s, v, lds_complex_target(s, v, v0));
+	}
+}
+
+// smallest member of r's wedge ring above `after` (kNone if none)
+__device__ __forceinline__ uint32_t lds_next_wedge_above(const PatchSmem& s, uint32_t r, uint32_t after)
+{
+	uint32_t best = kNone;
+	uint32_t v = r;
+	do
+	{
+		if (v > after && v < best)
+			best = v;
+		v = s.wedge[v];
+	} while (v != r);
+	return best;
+}
+
 // The whole simplify of one mesh by the calling workgroup; every return is workgroup-uniform.
-// sh[0..4] and shf[] are scratch words of this function.
+// sh[0..4] and shf[] are scratch words of this function.  kAttr: the batch has attributes
+// (a.attribute_count > 0), which only that build of the function reads.
+template <bool kAttr = false>
 __device__ __forceinline__ void simplify_patch(const BatchArgs& a, const PatchLayout& L, uint32_t patch, uint8_t* smem, uint32_t* part,
                                                uint32_t* sh, uint64_t* box, float* shf)
 {
@@ -391,6 +453,14 @@ __device__ __forceinline__ void simplify_patch(const BatchArgs& a, const PatchLa
 
 	PatchSmem s = bind(smem, L);
 	s.gq = a.quadrics + vb_begin;
+	PatchAttr at{};
+	if constexpr (kAttr)
+	{
+		at.count = a.attribute_count;
+		at.va = a.attr_values + (size_t)vb_begin * at.count;
+		at.aq = a.attr_quadrics + vb_begin;
+		at.ag = a.attr_grads + (size_t)vb_begin * at.count;
+	}
 	uint32_t* cursor = (uint32_t*)s.scratch; // V + 1 words
 	PROFILE_DECL;
 
@@ -737,6 +807,60 @@ __device__ __forceinline__ void simplify_patch(const BatchArgs& a, const PatchLa
 	__syncthreads();
 	PROFILE_PHASE(12);
 
+	if constexpr (kAttr)
+	{
+		// rescaleAttributes (src/simplifier.cpp:605-621): the attributes with weight > 0, each
+		// times its weight
+		const uint32_t A = at.count;
+		for (uint32_t i = tid; i < V; i += kBatchThreads)
+		{
+			const float* src = a.attributes + (size_t)(vb_begin + i) * a.attribute_stride_floats;
+			for (uint32_t k = 0; k < A; ++k)
+				at.va[(size_t)i * A + k] = src[a.attr_map.index[k]] * a.attr_map.weight[k];
+		}
+		// fillAttributeQuadrics (src/simplifier.cpp:1159-1179): each wedge sums its triangles in
+		// triangle order (the ordered corners of the unwelded adjacency; its barriers also make
+		// the values above visible); the quadric first, then one gradient sum per attribute, so
+		// that a single accumulator lives in registers
+		build_adjacency(s, s.ib, I, V, false, true, corners, cursor, part);
+		for (uint32_t v = tid; v < V; v += kBatchThreads)
+		{
+			const uint32_t b = s.off[v], e = s.off[v + 1];
+			Quad Q;
+			quad_zero(Q);
+			for (uint32_t p = b; p < e; ++p)
+			{
+				const uint32_t t = corners[p] / 3;
+				const uint32_t i0 = s.ib[t * 3 + 0], i1 = s.ib[t * 3 + 1], i2 = s.ib[t * 3 + 2];
+				const AttrFrame f = attr_frame(P(s, i0), P(s, i1), P(s, i2));
+				Quad QA;
+				quad_zero(QA);
+				QA.w = f.w;
+				for (uint32_t k = 0; k < A; ++k)
+				{
+					Grad g;
+					attr_term(QA, g, f, at.va[(size_t)i0 * A + k], at.va[(size_t)i1 * A + k], at.va[(size_t)i2 * A + k]);
+				}
+				quad_add(Q, QA);
+			}
+			store_quad(at.aq, v, Q);
+			for (uint32_t k = 0; k < A; ++k)
+			{
+				Grad G{0.f, 0.f, 0.f, 0.f};
+				for (uint32_t p = b; p < e; ++p)
+				{
+					const uint32_t t = corners[p] / 3;
+					const uint32_t i0 = s.ib[t * 3 + 0], i1 = s.ib[t * 3 + 1], i2 = s.ib[t * 3 + 2];
+					const AttrFrame f = attr_frame(P(s, i0), P(s, i1), P(s, i2));
+					grad_add(G, attr_weighted_gradient(f, at.va[(size_t)i0 * A + k], at.va[(size_t)i1 * A + k], at.va[(size_t)i2 * A + k]));
+				}
+				at.ag[(size_t)v * A + k] = G;
+			}
+		}
+		// the corner lists are rebuilt below
+		__syncthreads();
+	}
+
 	// ---------------------------------------------------------------- vertex quadrics (src/simplifier.cpp:1057-1157)
 	build_adjacency(s, s.ib, I, V, true, true, corners, cursor, part);
 	PROFILE_PHASE(13);
@@ -889,8 +1013,10 @@ __device__ __forceinline__ void simplify_patch(const BatchArgs& a, const PatchLa
 						const bool bidi = c01 & c10;
 						const bool fwd = bidi || c01;
 						const uint32_t v0 = fwd ? i0 : i1, v1 = fwd ? i1 : i0;
-						const float ei = fwd ? to_next[a] : to_prev[b];
-						const float ej = bidi ? to_prev[b] : FLT_MAX;
+						float ei = fwd ? to_next[a] : to_prev[b];
+						float ej = bidi ? to_prev[b] : FLT_MAX;
+						if constexpr (kAttr)
+							patch_attr_rank(at, s, v0, v1, bidi, ei, ej);
 						const bool rev = bidi & (ej < ei);
 						const uint32_t packed = rev ? (v1 | (v0 << 16)) : (v0 | (v1 << 16));
 						const float err = ej < ei ? ej : ei;
@@ -1247,7 +1373,10 @@ __device__ __forceinline__ void simplify_patch(const BatchArgs& a, const PatchLa
 		if (sh[3] == 0)
 			break;
 
-		// updateQuadrics (src/simplifier.cpp:1638-1673): each target class receives one add
+		// updateQuadrics (src/simplifier.cpp:1638-1673): each target class receives one add; with
+		// attributes, the moved wedges of the source class add their attribute quadrics and
+		// gradients to their targets in ascending wedge order, as the reference's vertex loop
+		// does (every wedge moving into a class comes from the one collapse that targets it)
 		for (uint32_t i = tid; i < V; i += kBatchThreads)
 		{
 			uint32_t t = s.cremap[i];
@@ -1256,6 +1385,25 @@ __device__ __forceinline__ void simplify_patch(const BatchArgs& a, const PatchLa
 				Quad Q1 = QL(s, s.remap[t]);
 				quad_add(Q1, QL(s, i));
 				QS(s, s.remap[t], Q1);
+				if constexpr (kAttr)
+				{
+					const uint32_t A = at.count;
+					for (uint32_t v = i; v != kNone; v = lds_next_wedge_above(s, i, v))
+					{
+						const uint32_t tv = s.cremap[v];
+						if (tv == v)
+							continue;
+						Quad Qt = load_quad(at.aq, tv);
+						quad_add(Qt, load_quad(at.aq, v));
+						store_quad(at.aq, tv, Qt);
+						for (uint32_t k = 0; k < A; ++k)
+						{
+							Grad g = at.ag[(size_t)tv * A + k];
+							grad_add(g, at.ag[(size_t)v * A + k]);
+							at.ag[(size_t)tv * A + k] = g;
+						}
+					}
+				}
 			}
 		}
 		__syncthreads();
@@ -1380,6 +1528,7 @@ __device__ __forceinline__ void simplify_patch(const BatchArgs& a, const PatchLa
 // read back is made wave-uniform (every thread reads the same word after the barrier, so the
 // first lane's value is every lane's): the loop exit and every barrier loop inside simplify_patch
 // then sit in uniform control flow, as in the one-mesh-per-workgroup form.
+template <bool kAttr = false>
 __device__ __forceinline__ void simplify_from_cursor(const BatchArgs& a, const PatchLayout& L, const uint32_t* ids, uint32_t n, uint32_t* cursor,
                                                      uint8_t* smem, uint32_t* part, uint32_t* sh, uint64_t* box, float* shf, uint32_t* next)
 {
@@ -1391,7 +1540,7 @@ __device__ __forceinline__ void simplify_from_cursor(const BatchArgs& a, const P
 		const uint32_t i = __builtin_amdgcn_readfirstlane(*next);
 		if (i >= n)
 			break;
-		simplify_patch(a, L, ids[i], smem, part, sh, box, shf);
+		simplify_patch<kAttr>(a, L, ids[i], smem, part, sh, box, shf);
 		__syncthreads();
 	}
 }

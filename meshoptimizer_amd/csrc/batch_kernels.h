@@ -20,7 +20,9 @@ inline namespace MHIP_BATCH_NS(MHIP_BATCH_THREADS)
 {
 
 // one workgroup per mesh of one LDS size class: ids lists the class's *count meshes (the grid is
-// an upper bound, workgroups past the count leave at once)
+// an upper bound, workgroups past the count leave at once).  kAttr: the build for batches with
+// attributes
+template <bool kAttr>
 __global__ void __launch_bounds__(kBatchThreads) MHIP_BATCH_OCCUPANCY k_simplify_patches(BatchArgs a, PatchLayout L, const uint32_t* ids, const uint32_t* count)
 {
 	extern __shared__ __align__(16) uint8_t smem[];
@@ -30,11 +32,12 @@ __global__ void __launch_bounds__(kBatchThreads) MHIP_BATCH_OCCUPANCY k_simplify
 	__shared__ float shf[8];
 
 	if (blockIdx.x < *count)
-		simplify_patch(a, L, ids[blockIdx.x], smem, part, sh, box, shf);
+		simplify_patch<kAttr>(a, L, ids[blockIdx.x], smem, part, sh, box, shf);
 }
 
 // each workgroup takes the next mesh of the list from *cursor (zero at the launch) until the list
 // ends, so a workgroup that finishes early takes the next largest mesh (simplify_from_cursor)
+template <bool kAttr>
 __global__ void __launch_bounds__(kBatchThreads) MHIP_BATCH_OCCUPANCY k_simplify_patches_cursor(BatchArgs a, PatchLayout L, const uint32_t* ids,
                                                                                                const uint32_t* count, uint32_t* cursor)
 {
@@ -45,7 +48,7 @@ __global__ void __launch_bounds__(kBatchThreads) MHIP_BATCH_OCCUPANCY k_simplify
 	__shared__ float shf[8];
 	__shared__ uint32_t next;
 
-	simplify_from_cursor(a, L, ids, *count, cursor, smem, part, sh, box, shf, &next);
+	simplify_from_cursor<kAttr>(a, L, ids, *count, cursor, smem, part, sh, box, shf, &next);
 }
 
 size_t width_lds_bytes(uint32_t vmax, uint32_t imax, uint32_t cmax)
@@ -53,13 +56,18 @@ size_t width_lds_bytes(uint32_t vmax, uint32_t imax, uint32_t cmax)
 	return patch_layout(vmax, imax, cmax).total;
 }
 
-// both forms may take `bytes` of dynamic LDS; false when the runtime refuses
+static bool allow_lds(const void* kernel, size_t bytes)
+{
+	return hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) == hipSuccess;
+}
+
+// every form may take `bytes` of dynamic LDS; false when the runtime refuses
 bool width_allow_lds(size_t bytes)
 {
-	return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_simplify_patches), hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) ==
-	           hipSuccess &&
-	       hipFuncSetAttribute(reinterpret_cast<const void*>(&k_simplify_patches_cursor), hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) ==
-	           hipSuccess;
+	return allow_lds(reinterpret_cast<const void*>(&k_simplify_patches<false>), bytes) &&
+	       allow_lds(reinterpret_cast<const void*>(&k_simplify_patches_cursor<false>), bytes) &&
+	       allow_lds(reinterpret_cast<const void*>(&k_simplify_patches<true>), bytes) &&
+	       allow_lds(reinterpret_cast<const void*>(&k_simplify_patches_cursor<true>), bytes);
 }
 
 // cursor == null: one workgroup per mesh slot (grid at least the class's count); otherwise `grid`
@@ -70,13 +78,19 @@ void width_launch(hipStream_t s, const BatchArgs& a, uint32_t vmax, uint32_t ima
 	const PatchLayout L = patch_layout(vmax, imax, cmax);
 	if (!grid)
 		return;
+	const bool attr = a.attribute_count != 0;
 	if (cursor)
 	{
 		fill_u32(s, cursor, 0u, 1);
-		k_simplify_patches_cursor<<<(unsigned int)grid, kBatchThreads, L.total, s>>>(a, L, ids, count_dev, cursor);
+		if (attr)
+			k_simplify_patches_cursor<true><<<(unsigned int)grid, kBatchThreads, L.total, s>>>(a, L, ids, count_dev, cursor);
+		else
+			k_simplify_patches_cursor<false><<<(unsigned int)grid, kBatchThreads, L.total, s>>>(a, L, ids, count_dev, cursor);
 	}
+	else if (attr)
+		k_simplify_patches<true><<<(unsigned int)grid, kBatchThreads, L.total, s>>>(a, L, ids, count_dev);
 	else
-		k_simplify_patches<<<(unsigned int)grid, kBatchThreads, L.total, s>>>(a, L, ids, count_dev);
+		k_simplify_patches<false><<<(unsigned int)grid, kBatchThreads, L.total, s>>>(a, L, ids, count_dev);
 }
 
 #ifdef MESHOPT_BATCH_PROFILE

@@ -38,6 +38,8 @@ struct ClodBuffers
 	float* packed_positions;       // 3I
 	uint8_t* packed_locks;         // I
 	uint32_t* packed_result;       // I
+	float* packed_attributes;      // I * attribute_count: each packed vertex's attributes (null: none)
+	uint32_t attribute_count;
 };
 
 // per group: first-use ranking (packed_indices, staged, unique counts) and the first/last group of
@@ -48,7 +50,8 @@ void launch_clod_group(hipStream_t s, const ClodBuffers& b, bool large_groups);
 // (lockBoundary, demo/clusterlod.h:372-419, with the caller's protect and lock bits as base_locks)
 void launch_clod_locks(hipStream_t s, const ClodBuffers& b, const uint8_t* base_locks, uint8_t* locks);
 // after the host scans `unique`: the packed vertex list, positions and locks of every group
-void launch_clod_place(hipStream_t s, const ClodBuffers& b, const float* positions, size_t stride_floats, const uint8_t* locks);
+void launch_clod_place(hipStream_t s, const ClodBuffers& b, const float* positions, size_t stride_floats, const uint8_t* locks,
+                       const float* attributes, size_t attribute_stride_floats);
 // destination[index_offsets[g] + k] = order[unique[g] + packed_result[index_offsets[g] + k]], k < counts[g]
 void launch_clod_unpack(hipStream_t s, const ClodBuffers& b, const uint32_t* counts, uint32_t* destination);
 
@@ -56,7 +59,7 @@ void launch_clod_unpack(hipStream_t s, const ClodBuffers& b, const uint32_t* cou
 // data of the groups `list` (n of them) at a sub-batch's offsets, and put the sub-batch's results
 // back in place of theirs
 void launch_clod_gather(hipStream_t s, const ClodBuffers& b, const uint32_t* list, uint32_t n, const uint32_t* sub_io, const uint32_t* sub_vo,
-                        uint32_t* sub_idx, float* sub_pos, uint8_t* sub_locks);
+                        uint32_t* sub_idx, float* sub_pos, uint8_t* sub_locks, float* sub_attr);
 void launch_clod_scatter(hipStream_t s, const ClodBuffers& b, const uint32_t* list, uint32_t n, const uint32_t* sub_io, const uint32_t* sub_result,
                          const uint32_t* sub_counts, const float* sub_errors, uint32_t* counts, float* errors);
 // simplifyFallback's deindexed subset of group g (n indices: one vertex per index) and the result

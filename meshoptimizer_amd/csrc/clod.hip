@@ -263,11 +263,14 @@ void launch_clod_locks(hipStream_t s, const ClodBuffers& b, const uint8_t* base_
 		k_clod_locks<<<clod_grid(b.vertex_count), kClodThreads, 0, s>>>(b.first_group, b.last_group, b.remap, base_locks, b.vertex_count, locks);
 }
 
-// group g's packed vertices at unique[g] (the scanned counts): shared id, position, lock
-__global__ void __launch_bounds__(kClodThreads) k_clod_place(ClodBuffers b, const float* positions, size_t stride, const uint8_t* locks)
+// group g's packed vertices at unique[g] (the scanned counts): shared id, position, lock and
+// attributes
+__global__ void __launch_bounds__(kClodThreads) k_clod_place(ClodBuffers b, const float* positions, size_t stride, const uint8_t* locks,
+                                                             const float* attributes, size_t attribute_stride)
 {
 	const uint32_t g = blockIdx.x;
 	const uint32_t begin = b.index_offsets[g], vbase = b.unique[g], n = b.unique[g + 1] - vbase;
+	const uint32_t A = b.attribute_count;
 	for (uint32_t k = threadIdx.x; k < n; k += kClodThreads)
 	{
 		const uint32_t v = b.staged[begin + k], at = vbase + k;
@@ -277,13 +280,16 @@ __global__ void __launch_bounds__(kClodThreads) k_clod_place(ClodBuffers b, cons
 		b.packed_positions[3 * (size_t)at + 1] = p[1];
 		b.packed_positions[3 * (size_t)at + 2] = p[2];
 		b.packed_locks[at] = locks[v];
+		for (uint32_t q = 0; q < A; ++q)
+			b.packed_attributes[(size_t)at * A + q] = attributes[(size_t)v * attribute_stride + q];
 	}
 }
 
-void launch_clod_place(hipStream_t s, const ClodBuffers& b, const float* positions, size_t stride_floats, const uint8_t* locks)
+void launch_clod_place(hipStream_t s, const ClodBuffers& b, const float* positions, size_t stride_floats, const uint8_t* locks,
+                       const float* attributes, size_t attribute_stride_floats)
 {
 	if (b.group_count)
-		k_clod_place<<<b.group_count, kClodThreads, 0, s>>>(b, positions, stride_floats, locks);
+		k_clod_place<<<b.group_count, kClodThreads, 0, s>>>(b, positions, stride_floats, locks, attributes, attribute_stride_floats);
 }
 
 __global__ void __launch_bounds__(kClodThreads) k_clod_unpack(ClodBuffers b, const uint32_t* counts, uint32_t* destination)
@@ -307,8 +313,9 @@ void launch_clod_unpack(hipStream_t s, const ClodBuffers& b, const uint32_t* cou
 // groups list[f] of a sub-batch: their packed indices, positions and locks gathered at the
 // sub-batch's offsets (sub_io, sub_vo, host-computed scans of the groups' sizes)
 __global__ void __launch_bounds__(kClodThreads) k_clod_gather(ClodBuffers b, const uint32_t* list, const uint32_t* sub_io, const uint32_t* sub_vo,
-                                                              uint32_t* sub_idx, float* sub_pos, uint8_t* sub_locks)
+                                                              uint32_t* sub_idx, float* sub_pos, uint8_t* sub_locks, float* sub_attr)
 {
+	const uint32_t A = b.attribute_count;
 	const uint32_t f = blockIdx.x, g = list[f];
 	const uint32_t ib = b.index_offsets[g], ni = b.index_offsets[g + 1] - ib, vb = b.unique[g], nv = b.unique[g + 1] - vb;
 	for (uint32_t k = threadIdx.x; k < ni; k += kClodThreads)
@@ -320,6 +327,8 @@ __global__ void __launch_bounds__(kClodThreads) k_clod_gather(ClodBuffers b, con
 		sub_pos[3 * at + 1] = b.packed_positions[3 * ((size_t)vb + k) + 1];
 		sub_pos[3 * at + 2] = b.packed_positions[3 * ((size_t)vb + k) + 2];
 		sub_locks[at] = b.packed_locks[vb + k];
+		for (uint32_t q = 0; q < A; ++q)
+			sub_attr[at * A + q] = b.packed_attributes[((size_t)vb + k) * A + q];
 	}
 }
 
@@ -340,10 +349,10 @@ __global__ void __launch_bounds__(kClodThreads) k_clod_scatter(ClodBuffers b, co
 }
 
 void launch_clod_gather(hipStream_t s, const ClodBuffers& b, const uint32_t* list, uint32_t n, const uint32_t* sub_io, const uint32_t* sub_vo,
-                        uint32_t* sub_idx, float* sub_pos, uint8_t* sub_locks)
+                        uint32_t* sub_idx, float* sub_pos, uint8_t* sub_locks, float* sub_attr)
 {
 	if (n)
-		k_clod_gather<<<n, kClodThreads, 0, s>>>(b, list, sub_io, sub_vo, sub_idx, sub_pos, sub_locks);
+		k_clod_gather<<<n, kClodThreads, 0, s>>>(b, list, sub_io, sub_vo, sub_idx, sub_pos, sub_locks, sub_attr);
 }
 
 void launch_clod_scatter(hipStream_t s, const ClodBuffers& b, const uint32_t* list, uint32_t n, const uint32_t* sub_io, const uint32_t* sub_result,

@@ -72,6 +72,13 @@ struct __align__(16) Grad
 	float gx, gy, gz, gw;
 };
 
+// attributes with weight > 0: caller's attribute index and weight of each
+struct AttrMap
+{
+	uint32_t index[kMaxAttributes];
+	float weight[kMaxAttributes];
+};
+
 // scalars of one simplify call that live on the device
 struct CallScalars
 {

@@ -120,6 +120,7 @@ struct Context
 	char* arena = nullptr;
 	size_t arena_size = 0;
 	StagingCache staging;
+	StagingCache batch_attr; // batched calls with attributes: attribute quadrics, gradients and values
 	CallScalars* sc_host = nullptr; // pinned mirror of the device scalar block
 	std::vector<meshopt_hip_pass_stats> passes;
 	meshopt_hip_call_stats call{};
@@ -297,6 +298,7 @@ struct Context
 			(void)hipEventDestroy(fork_ev);
 		fork_ev = nullptr;
 		staging.release();
+		batch_attr.release();
 		if (arena)
 			(void)hipFree(arena);
 		if (sc_host)
@@ -1593,16 +1595,44 @@ struct BatchMemory
 	uint32_t* host;
 	bool plain_fallback;
 	bool timed;
+	// attribute state of batches with attributes (null otherwise): per vertex, attr_room
+	// rescaled values and gradients, and one attribute quadric
+	float* attr_values;
+	Quad* attr_quadrics;
+	Grad* attr_grads;
+	size_t attr_room;
 };
 
-// the context's own batch buffers for P meshes and `vertices` vertices (grown on demand)
-static BatchMemory context_batch_memory(Context& ctx, size_t P, size_t vertices, bool need_quadrics)
+// the attributes of a batched call: device values packed per vertex like the positions, their
+// stride in bytes, and the host array of weights
+struct BatchAttributes
+{
+	const float* data;
+	size_t stride;
+	const float* weights;
+	size_t count;
+};
+
+// the context's own batch buffers for P meshes and `vertices` vertices (grown on demand), with
+// room for `attributes` attributes per vertex
+static BatchMemory context_batch_memory(Context& ctx, size_t P, size_t vertices, bool need_quadrics, size_t attributes = 0)
 {
 	ctx.ensure_batch_buffers(P);
 	if (need_quadrics)
 		ctx.ensure_batch_quadrics(vertices ? vertices : 1);
 	ctx.batch_last_meshes = P;
-	return BatchMemory{ctx.batch_dev, ctx.batch_dev + batch_list_words(P), ctx.batch_quadrics, ctx.batch_pinned, true, timing_on()};
+	BatchMemory m{ctx.batch_dev, ctx.batch_dev + batch_list_words(P), ctx.batch_quadrics, ctx.batch_pinned, true, timing_on(), nullptr, nullptr, nullptr, 0};
+	if (need_quadrics && attributes)
+	{
+		const size_t n = vertices ? vertices : 1;
+		const size_t o_aq = 0, o_ag = o_aq + n * sizeof(Quad), o_va = o_ag + n * attributes * sizeof(Grad);
+		char* base = static_cast<char*>(ctx.batch_attr.get(o_va + n * attributes * sizeof(float) + 256));
+		m.attr_quadrics = reinterpret_cast<Quad*>(base + o_aq);
+		m.attr_grads = reinterpret_cast<Grad*>(base + o_ag);
+		m.attr_values = reinterpret_cast<float*>(base + o_va);
+		m.attr_room = attributes;
+	}
+	return m;
 }
 
 // Batched call: sizes are validated and meshes sorted into LDS size classes on the device, the
@@ -1616,13 +1646,35 @@ static int simplify_batch_on_device(Context& ctx, const BatchMemory& mem, unsign
                                     const unsigned int* target_index_counts,
                                     size_t patch_count, float target_error, unsigned int options, const unsigned int* h_index_offsets,
                                     const unsigned int* h_vertex_offsets, const unsigned int* h_targets,
-                                    const unsigned int* h_size_offsets = nullptr)
+                                    const unsigned int* h_size_offsets = nullptr, const BatchAttributes* attrs = nullptr)
 {
 	hipStream_t s = ctx.stream;
 	if (stride < 12 || stride > 256 || stride % 4)
 		throw std::invalid_argument("meshopt_hip_simplify_batch: invalid vertex stride");
 	if (options & kOptInternalSolve)
 		throw std::invalid_argument("meshopt_hip_simplify_batch: the solve bit is not accepted here");
+	// the attributes with weight > 0 enter the quadrics (src/simplifier.cpp:2410-2421)
+	const size_t attr_count = attrs ? attrs->count : 0;
+	AttrMap attr_map{};
+	uint32_t attr_used = 0;
+	if (attr_count)
+	{
+		if (attr_count > (size_t)kMaxAttributes || attrs->stride < attr_count * 4 || attrs->stride > 256 || attrs->stride % 4 != 0 || !attrs->weights)
+			throw std::invalid_argument("meshopt_hip_simplify_batch: invalid attribute count, stride or weights");
+		for (size_t k = 0; k < attr_count; ++k)
+		{
+			if (!(attrs->weights[k] >= 0))
+				throw std::invalid_argument("meshopt_hip_simplify_batch: negative attribute weight");
+			if (attrs->weights[k] > 0)
+			{
+				attr_map.index[attr_used] = (uint32_t)k;
+				attr_map.weight[attr_used] = attrs->weights[k];
+				attr_used++;
+			}
+		}
+		if (attr_used > mem.attr_room && !(options & (kOptSparse | kOptPrune)))
+			throw std::logic_error("meshopt_hip_simplify_batch: attribute buffers too small");
+	}
 	if (!patch_count)
 		return 0;
 
@@ -1654,6 +1706,13 @@ static int simplify_batch_on_device(Context& ctx, const BatchMemory& mem, unsign
 	a.patch_count = patch_count;
 	a.target_error = target_error;
 	a.options = options;
+	a.attributes = attr_used ? attrs->data : nullptr;
+	a.attribute_stride_floats = attr_used ? attrs->stride / sizeof(float) : 0;
+	a.attribute_count = attr_used;
+	a.attr_map = attr_map;
+	a.attr_values = mem.attr_values;
+	a.attr_quadrics = mem.attr_quadrics;
+	a.attr_grads = mem.attr_grads;
 
 	// Sparse and Prune meshes take the plain path, mesh by mesh (the batched kernel's LDS layouts
 	// hold neither the sparse remap nor the component arrays)
@@ -1851,10 +1910,10 @@ static int simplify_batch_on_device(Context& ctx, const BatchMemory& mem, unsign
 		c.positions = positions + (size_t)h_vertex_offsets[p] * (stride / sizeof(float));
 		c.vertex_count = h_vertex_offsets[p + 1] - h_vertex_offsets[p];
 		c.position_stride = stride;
-		c.attributes = nullptr;
-		c.attribute_stride = 0;
-		c.weights = nullptr;
-		c.attribute_count = 0;
+		c.attributes = attr_count ? attrs->data + (size_t)h_vertex_offsets[p] * (attrs->stride / sizeof(float)) : nullptr;
+		c.attribute_stride = attr_count ? attrs->stride : 0;
+		c.weights = attr_count ? attrs->weights : nullptr;
+		c.attribute_count = attr_count;
 		c.lock = vertex_lock ? vertex_lock + h_vertex_offsets[p] : nullptr;
 		c.target_index_count = h_targets[p];
 		c.target_error = target_error;
@@ -1881,19 +1940,21 @@ static int simplify_batch_on_device(Context& ctx, const BatchMemory& mem, unsign
 
 } // namespace mhip
 
-extern "C" int meshopt_hip_simplify_batch_device(unsigned int* destination, unsigned int* result_counts, float* result_errors, const unsigned int* indices,
-                                                 const unsigned int* index_offsets, const float* vertex_positions, const unsigned int* vertex_offsets,
-                                                 const unsigned char* vertex_lock, size_t vertex_count, size_t vertex_positions_stride,
-                                                 const unsigned int* target_index_counts, size_t mesh_count, float target_error, unsigned int options)
+static int batch_device_entry(unsigned int* destination, unsigned int* result_counts, float* result_errors, const unsigned int* indices,
+                              const unsigned int* index_offsets, const float* vertex_positions, const unsigned int* vertex_offsets,
+                              const BatchAttributes* attrs, const unsigned char* vertex_lock, size_t vertex_count, size_t vertex_positions_stride,
+                              const unsigned int* target_index_counts, size_t mesh_count, float target_error, unsigned int options)
 {
 	try
 	{
 		Context& ctx = context();
 		std::lock_guard<std::mutex> guard(ctx.mutex);
 		ctx.ensure_init();
-		simplify_batch_on_device(ctx, context_batch_memory(ctx, mesh_count, vertex_count, !(options & (kOptSparse | kOptPrune))), destination,
+		const bool batched = !(options & (kOptSparse | kOptPrune));
+		simplify_batch_on_device(ctx, context_batch_memory(ctx, mesh_count, vertex_count, batched, attrs ? attrs->count : 0), destination,
 		                         result_counts, result_errors, indices, index_offsets, vertex_positions, vertex_offsets, vertex_lock, vertex_count,
-		                         vertex_positions_stride, target_index_counts, mesh_count, target_error, options, nullptr, nullptr, nullptr);
+		                         vertex_positions_stride, target_index_counts, mesh_count, target_error, options, nullptr, nullptr, nullptr, nullptr,
+		                         attrs);
 		t_last_error.clear();
 		return 0;
 	}
@@ -1902,6 +1963,30 @@ extern "C" int meshopt_hip_simplify_batch_device(unsigned int* destination, unsi
 		t_last_error = e.what();
 		return -1;
 	}
+}
+
+extern "C" int meshopt_hip_simplify_batch_device(unsigned int* destination, unsigned int* result_counts, float* result_errors, const unsigned int* indices,
+                                                 const unsigned int* index_offsets, const float* vertex_positions, const unsigned int* vertex_offsets,
+                                                 const unsigned char* vertex_lock, size_t vertex_count, size_t vertex_positions_stride,
+                                                 const unsigned int* target_index_counts, size_t mesh_count, float target_error, unsigned int options)
+{
+	return batch_device_entry(destination, result_counts, result_errors, indices, index_offsets, vertex_positions, vertex_offsets, nullptr, vertex_lock,
+	                          vertex_count, vertex_positions_stride, target_index_counts, mesh_count, target_error, options);
+}
+
+extern "C" int meshopt_hip_simplify_batch_with_attributes_device(unsigned int* destination, unsigned int* result_counts, float* result_errors,
+                                                                 const unsigned int* indices, const unsigned int* index_offsets,
+                                                                 const float* vertex_positions, const unsigned int* vertex_offsets,
+                                                                 const float* vertex_attributes, size_t vertex_attributes_stride,
+                                                                 const float* attribute_weights, size_t attribute_count,
+                                                                 const unsigned char* vertex_lock, size_t vertex_count, size_t vertex_positions_stride,
+                                                                 const unsigned int* target_index_counts, size_t mesh_count, float target_error,
+                                                                 unsigned int options)
+{
+	const BatchAttributes attrs{vertex_attributes, vertex_attributes_stride, attribute_weights, attribute_count};
+	return batch_device_entry(destination, result_counts, result_errors, indices, index_offsets, vertex_positions, vertex_offsets,
+	                          attribute_count ? &attrs : nullptr, vertex_lock, vertex_count, vertex_positions_stride, target_index_counts, mesh_count,
+	                          target_error, options);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -2072,10 +2157,11 @@ extern "C" int meshopt_hip_last_batch_stats(unsigned int* stats, size_t max_mesh
 	}
 }
 
-extern "C" int meshopt_hip_simplify_batch(unsigned int* destination, unsigned int* result_counts, float* result_errors, const unsigned int* indices,
-                                          const unsigned int* index_offsets, const float* vertex_positions, const unsigned int* vertex_offsets,
-                                          const unsigned char* vertex_lock, size_t vertex_positions_stride, const unsigned int* target_index_counts,
-                                          size_t mesh_count, float target_error, unsigned int options)
+static int batch_host_entry(unsigned int* destination, unsigned int* result_counts, float* result_errors, const unsigned int* indices,
+                            const unsigned int* index_offsets, const float* vertex_positions, const unsigned int* vertex_offsets,
+                            const float* vertex_attributes, size_t vertex_attributes_stride, const float* attribute_weights, size_t attribute_count,
+                            const unsigned char* vertex_lock, size_t vertex_positions_stride, const unsigned int* target_index_counts,
+                            size_t mesh_count, float target_error, unsigned int options)
 {
 	try
 	{
@@ -2103,7 +2189,12 @@ extern "C" int meshopt_hip_simplify_batch(unsigned int* destination, unsigned in
 		size_t off_pos = off_re + ((n1 * 4 + 255) & ~size_t(255));
 		size_t off_lock = off_pos + ((pos_bytes + 255) & ~size_t(255));
 		size_t lock_bytes = vertex_lock ? V : 0;
-		char* d = static_cast<char*>(ctx.staging.get(off_lock + lock_bytes + 256));
+		if (attribute_count && (attribute_count > (size_t)kMaxAttributes || vertex_attributes_stride < attribute_count * 4 ||
+		                        vertex_attributes_stride > 256 || vertex_attributes_stride % 4 != 0))
+			throw std::invalid_argument("meshopt_hip_simplify_batch: invalid attribute count or stride");
+		size_t attr_bytes = attribute_count && V ? (V - 1) * vertex_attributes_stride + attribute_count * 4 : 0;
+		size_t off_attr = off_lock + ((lock_bytes + 255) & ~size_t(255));
+		char* d = static_cast<char*>(ctx.staging.get(off_attr + attr_bytes + 256));
 		if (I)
 			HIP_CHECK(hipMemcpyAsync(d + off_idx, indices, I * 4, hipMemcpyHostToDevice, s));
 		HIP_CHECK(hipMemcpyAsync(d + off_io, index_offsets, n1 * 4, hipMemcpyHostToDevice, s));
@@ -2114,14 +2205,18 @@ extern "C" int meshopt_hip_simplify_batch(unsigned int* destination, unsigned in
 			HIP_CHECK(hipMemcpyAsync(d + off_pos, vertex_positions, pos_bytes, hipMemcpyHostToDevice, s));
 		if (lock_bytes)
 			HIP_CHECK(hipMemcpyAsync(d + off_lock, vertex_lock, lock_bytes, hipMemcpyHostToDevice, s));
-		simplify_batch_on_device(ctx, context_batch_memory(ctx, mesh_count, V, !(options & (kOptSparse | kOptPrune))), reinterpret_cast<unsigned int*>(d + off_dst),
+		if (attr_bytes)
+			HIP_CHECK(hipMemcpyAsync(d + off_attr, vertex_attributes, attr_bytes, hipMemcpyHostToDevice, s));
+		const BatchAttributes attrs{reinterpret_cast<const float*>(d + off_attr), vertex_attributes_stride, attribute_weights, attribute_count};
+		simplify_batch_on_device(ctx, context_batch_memory(ctx, mesh_count, V, !(options & (kOptSparse | kOptPrune)), attribute_count),
+		                         reinterpret_cast<unsigned int*>(d + off_dst),
 		                         reinterpret_cast<unsigned int*>(d + off_rc),
 		                         reinterpret_cast<float*>(d + off_re), reinterpret_cast<const unsigned int*>(d + off_idx),
 		                         reinterpret_cast<const unsigned int*>(d + off_io), reinterpret_cast<const float*>(d + off_pos),
 		                         reinterpret_cast<const unsigned int*>(d + off_vo), lock_bytes ? reinterpret_cast<const unsigned char*>(d + off_lock) : nullptr,
 		                         V, vertex_positions_stride,
 		                         reinterpret_cast<const unsigned int*>(d + off_tg), mesh_count, target_error, options, index_offsets, vertex_offsets,
-		                         target_index_counts);
+		                         target_index_counts, nullptr, attribute_count ? &attrs : nullptr);
 		if (mesh_count)
 		{
 			HIP_CHECK(hipMemcpyAsync(result_counts, d + off_rc, mesh_count * 4, hipMemcpyDeviceToHost, s));
@@ -2141,6 +2236,28 @@ extern "C" int meshopt_hip_simplify_batch(unsigned int* destination, unsigned in
 	}
 }
 
+extern "C" int meshopt_hip_simplify_batch(unsigned int* destination, unsigned int* result_counts, float* result_errors, const unsigned int* indices,
+                                          const unsigned int* index_offsets, const float* vertex_positions, const unsigned int* vertex_offsets,
+                                          const unsigned char* vertex_lock, size_t vertex_positions_stride, const unsigned int* target_index_counts,
+                                          size_t mesh_count, float target_error, unsigned int options)
+{
+	return batch_host_entry(destination, result_counts, result_errors, indices, index_offsets, vertex_positions, vertex_offsets, nullptr, 0, nullptr, 0,
+	                        vertex_lock, vertex_positions_stride, target_index_counts, mesh_count, target_error, options);
+}
+
+extern "C" int meshopt_hip_simplify_batch_with_attributes(unsigned int* destination, unsigned int* result_counts, float* result_errors,
+                                                          const unsigned int* indices, const unsigned int* index_offsets, const float* vertex_positions,
+                                                          const unsigned int* vertex_offsets, const float* vertex_attributes,
+                                                          size_t vertex_attributes_stride, const float* attribute_weights, size_t attribute_count,
+                                                          const unsigned char* vertex_lock, size_t vertex_positions_stride,
+                                                          const unsigned int* target_index_counts, size_t mesh_count, float target_error,
+                                                          unsigned int options)
+{
+	return batch_host_entry(destination, result_counts, result_errors, indices, index_offsets, vertex_positions, vertex_offsets, vertex_attributes,
+	                        vertex_attributes_stride, attribute_weights, attribute_count, vertex_lock, vertex_positions_stride, target_index_counts,
+	                        mesh_count, target_error, options);
+}
+
 // The fallbacks of clodBuild's per-group simplify (demo/clusterlod.h:475-486) for the groups left
 // above their target by the level's batch: with `permissive`, those groups run again as one
 // sub-batch with Permissive added; then, with config.fallback_sloppy, every group still above its
@@ -2151,7 +2268,8 @@ extern "C" int meshopt_hip_simplify_batch(unsigned int* destination, unsigned in
 static void clod_fallbacks(Context& ctx, const ClodBuffers& b, const meshopt_hip_clod_config& config, const std::vector<uint32_t>& io,
                            const std::vector<uint32_t>& tg, unsigned int* result_counts, float* result_errors, float target_error,
                            unsigned int batch_options, bool permissive, uint32_t* d_list, uint32_t* sub_io, uint32_t* sub_vo, uint32_t* sub_tg,
-                           uint32_t* sub_idx, float* sub_pos, uint8_t* sub_locks, uint32_t* sub_result, uint32_t* sub_counts, float* sub_errors)
+                           uint32_t* sub_idx, float* sub_pos, uint8_t* sub_locks, uint32_t* sub_result, uint32_t* sub_counts, float* sub_errors,
+                           float* sub_attr, const float* attribute_weights)
 {
 	hipStream_t s = ctx.stream;
 	const size_t G = tg.size();
@@ -2181,10 +2299,12 @@ static void clod_fallbacks(Context& ctx, const ClodBuffers& b, const meshopt_hip
 		HIP_CHECK(hipMemcpyAsync(sub_io, h_io.data(), 4 * (F + 1), hipMemcpyHostToDevice, s));
 		HIP_CHECK(hipMemcpyAsync(sub_vo, h_vo.data(), 4 * (F + 1), hipMemcpyHostToDevice, s));
 		HIP_CHECK(hipMemcpyAsync(sub_tg, h_tg.data(), 4 * F, hipMemcpyHostToDevice, s));
-		launch_clod_gather(s, b, d_list, (uint32_t)F, sub_io, sub_vo, sub_idx, sub_pos, sub_locks);
-		simplify_batch_on_device(ctx, context_batch_memory(ctx, F, h_io[F], !(batch_options & kOptPrune)), sub_result, sub_counts,
+		launch_clod_gather(s, b, d_list, (uint32_t)F, sub_io, sub_vo, sub_idx, sub_pos, sub_locks, sub_attr);
+		const size_t A = b.attribute_count;
+		const BatchAttributes attrs{sub_attr, A * sizeof(float), attribute_weights, A};
+		simplify_batch_on_device(ctx, context_batch_memory(ctx, F, h_vo[F], !(batch_options & kOptPrune), A), sub_result, sub_counts,
 		                         result_errors ? sub_errors : nullptr, sub_idx, sub_io, sub_pos, sub_vo, sub_locks, h_vo[F], 12, sub_tg, F, target_error,
-		                         batch_options | kOptPermissive, nullptr, nullptr, nullptr, h_io.data());
+		                         batch_options | kOptPermissive, nullptr, nullptr, nullptr, h_io.data(), A ? &attrs : nullptr);
 		launch_clod_scatter(s, b, d_list, (uint32_t)F, sub_io, sub_result, sub_counts, result_errors ? sub_errors : nullptr, result_counts,
 		                    result_errors);
 		if (!config.fallback_sloppy)
@@ -2214,7 +2334,9 @@ static void clod_fallbacks(Context& ctx, const ClodBuffers& b, const meshopt_hip
 
 extern "C" int meshopt_hip_clod_level_device(unsigned int* destination, unsigned int* result_counts, float* result_errors, const unsigned int* indices,
                                              const unsigned int* index_offsets, size_t group_count, const float* vertex_positions,
-                                             size_t vertex_count, size_t vertex_positions_stride, const unsigned int* remap,
+                                             size_t vertex_count, size_t vertex_positions_stride, const float* vertex_attributes,
+                                             size_t vertex_attributes_stride, const float* attribute_weights, size_t attribute_count,
+                                             const unsigned int* remap,
                                              const unsigned char* base_locks, unsigned char* locks, const unsigned int* target_index_counts,
                                              float target_error, unsigned int options, const meshopt_hip_clod_config* config)
 {
@@ -2226,6 +2348,9 @@ extern "C" int meshopt_hip_clod_level_device(unsigned int* destination, unsigned
 			throw std::invalid_argument("meshopt_hip_clod_level_device: remap and locks are required");
 		if (vertex_count >= 0xffffffffu)
 			throw std::invalid_argument("meshopt_hip_clod_level_device: too many vertices");
+		if (attribute_count && (attribute_count > (size_t)kMaxAttributes || vertex_attributes_stride < attribute_count * 4 ||
+		                        vertex_attributes_stride > 256 || vertex_attributes_stride % 4 != 0 || !vertex_attributes || !attribute_weights))
+			throw std::invalid_argument("meshopt_hip_clod_level_device: invalid attribute count, stride or weights");
 		Context& ctx = context();
 		std::lock_guard<std::mutex> guard(ctx.mutex);
 		ctx.ensure_init();
@@ -2259,7 +2384,8 @@ extern "C" int meshopt_hip_clod_level_device(unsigned int* destination, unsigned
 		const size_t V = vertex_count;
 		const size_t o_first = take(4 * (V + 1)), o_last = take(4 * (V + 1)), o_key = take(8 * I + 4), o_sfirst = take(8 * I + 4);
 		const size_t o_local = take(8 * I + 4), o_staged = take(4 * I + 4), o_unique = take(4 * (G + 2)), o_order = take(4 * I + 4), o_pidx = take(4 * I + 4);
-		const size_t o_ppos = take(12 * I + 4), o_plock = take(I + 4), o_pres = take(4 * I + 4);
+		const size_t A = attribute_count;
+		const size_t o_ppos = take(12 * I + 4), o_plock = take(I + 4), o_pres = take(4 * I + 4), o_pattr = take(A ? 4 * A * I + 4 : 0);
 		const size_t o_temp = take(4 * (scan_temp_words(G + 1) + 64)), o_flag = take(16);
 		// a group whose target exceeds its size keeps its indices (demo/clusterlod.h:463-464): the
 		// batch runs it at its size and its result is replaced afterwards
@@ -2282,6 +2408,7 @@ extern "C" int meshopt_hip_clod_level_device(unsigned int* destination, unsigned
 		const size_t o_sio = take(fallbacks ? 4 * (G + 1) : 0), o_svo = take(fallbacks ? 4 * (G + 1) : 0), o_stg = take(fallbacks ? 4 * G : 0);
 		const size_t o_sidx = take(fallbacks ? 4 * I + 4 : 0), o_spos = take(fallbacks ? 12 * I + 4 : 0), o_slock = take(fallbacks ? I + 4 : 0);
 		const size_t o_sres = take(fallbacks ? 4 * I + 4 : 0), o_scnt = take(fallbacks ? 4 * G : 0), o_serr = take(fallbacks ? 4 * G : 0);
+		const size_t o_sattr = take(fallbacks && A ? 4 * A * I + 4 : 0);
 		char* base = static_cast<char*>(ctx.ensure_clod(used + 256));
 
 		ClodBuffers b;
@@ -2304,6 +2431,8 @@ extern "C" int meshopt_hip_clod_level_device(unsigned int* destination, unsigned
 		b.packed_positions = reinterpret_cast<float*>(base + o_ppos);
 		b.packed_locks = reinterpret_cast<uint8_t*>(base + o_plock);
 		b.packed_result = reinterpret_cast<uint32_t*>(base + o_pres);
+		b.packed_attributes = A ? reinterpret_cast<float*>(base + o_pattr) : nullptr;
+		b.attribute_count = (uint32_t)A;
 		uint32_t* temp = reinterpret_cast<uint32_t*>(base + o_temp);
 		uint32_t* flag = reinterpret_cast<uint32_t*>(base + o_flag);
 
@@ -2333,7 +2462,7 @@ extern "C" int meshopt_hip_clod_level_device(unsigned int* destination, unsigned
 		launch_clod_group(s, b, large_groups);
 		launch_clod_locks(s, b, base_locks, locks);
 		scan_exclusive(s, b.unique, b.unique, G, nullptr, b.unique + G, temp);
-		launch_clod_place(s, b, vertex_positions, vertex_positions_stride / sizeof(float), locks);
+		launch_clod_place(s, b, vertex_positions, vertex_positions_stride / sizeof(float), locks, vertex_attributes, vertex_attributes_stride / sizeof(float));
 
 		// the packed groups are the meshes of one batched call (Sparse is what the packing did)
 		const unsigned int* batch_targets = target_index_counts;
@@ -2344,10 +2473,11 @@ extern "C" int meshopt_hip_clod_level_device(unsigned int* destination, unsigned
 			batch_targets = d_tg;
 		}
 		const unsigned int batch_options = options & ~(unsigned int)kOptSparse;
-		simplify_batch_on_device(ctx, context_batch_memory(ctx, G, I, !(options & kOptPrune)), b.packed_result, result_counts, result_errors, b.packed_indices,
-		                         index_offsets, b.packed_positions, b.unique,
+		const BatchAttributes attrs{b.packed_attributes, A * sizeof(float), attribute_weights, A};
+		simplify_batch_on_device(ctx, context_batch_memory(ctx, G, I, !(options & kOptPrune), A), b.packed_result, result_counts, result_errors,
+		                         b.packed_indices, index_offsets, b.packed_positions, b.unique,
 		                         b.packed_locks, I, 12, batch_targets, G, target_error, batch_options, nullptr, nullptr,
-		                         nullptr, io.data());
+		                         nullptr, io.data(), A ? &attrs : nullptr);
 		uint32_t* d_list = reinterpret_cast<uint32_t*>(base + o_list);
 		if (!keep.empty())
 		{
@@ -2360,7 +2490,7 @@ extern "C" int meshopt_hip_clod_level_device(unsigned int* destination, unsigned
 			               reinterpret_cast<uint32_t*>(base + o_stg), reinterpret_cast<uint32_t*>(base + o_sidx),
 			               reinterpret_cast<float*>(base + o_spos), reinterpret_cast<uint8_t*>(base + o_slock),
 			               reinterpret_cast<uint32_t*>(base + o_sres), reinterpret_cast<uint32_t*>(base + o_scnt),
-			               reinterpret_cast<float*>(base + o_serr));
+			               reinterpret_cast<float*>(base + o_serr), A ? reinterpret_cast<float*>(base + o_sattr) : nullptr, attribute_weights);
 		if (config && config->error_edge_limit > 0 && result_errors)
 			launch_clod_edge_limit(s, b, target_index_counts, config->error_edge_limit, result_errors);
 		launch_clod_unpack(s, b, result_counts, destination);

@@ -78,13 +78,6 @@ struct Mesh
 	CallScalars* sc;
 };
 
-// attributes with weight > 0: caller's attribute index and weight of each
-struct AttrMap
-{
-	uint32_t index[kMaxAttributes];
-	float weight[kMaxAttributes];
-};
-
 // ---------------------------------------------------------------- setup (setup.hip)
 void launch_check_indices(hipStream_t s, const uint32_t* ib, size_t index_count, size_t vertex_count, uint32_t* flag);
 void launch_sparse_remap(hipStream_t s, const uint32_t* indices_in, uint32_t* indices_out, size_t index_count, size_t vertex_count,

@@ -251,14 +251,30 @@ __device__ __forceinline__ AttrFrame attr_frame(const V3& p0, const V3& p1, cons
 	return f;
 }
 
-// one attribute's gradient and its quadric terms, src/simplifier.cpp:915-948.
-// Q accumulates the attribute terms in k order; g receives the weighted gradient.
-__device__ __forceinline__ void attr_term(Quad& Q, Grad& g, const AttrFrame& f, float a0, float a1, float a2)
+// one attribute's gradient over the triangle before the area weight, src/simplifier.cpp:915-926
+__device__ __forceinline__ Grad attr_gradient(const AttrFrame& f, float a0, float a1, float a2)
 {
 	float gx = f.gx1 * (a1 - a0) + f.gx2 * (a2 - a0);
 	float gy = f.gy1 * (a1 - a0) + f.gy2 * (a2 - a0);
 	float gz = f.gz1 * (a1 - a0) + f.gz2 * (a2 - a0);
 	float gw = a0 - f.p0.x * gx - f.p0.y * gy - f.p0.z * gz;
+	return Grad{gx, gy, gz, gw};
+}
+
+// the weighted gradient alone: what attr_term gives as g
+__device__ __forceinline__ Grad attr_weighted_gradient(const AttrFrame& f, float a0, float a1, float a2)
+{
+	const Grad r = attr_gradient(f, a0, a1, a2);
+	const float w = f.w;
+	return Grad{w * r.gx, w * r.gy, w * r.gz, w * r.gw};
+}
+
+// one attribute's gradient and its quadric terms, src/simplifier.cpp:915-948.
+// Q accumulates the attribute terms in k order; g receives the weighted gradient.
+__device__ __forceinline__ void attr_term(Quad& Q, Grad& g, const AttrFrame& f, float a0, float a1, float a2)
+{
+	const Grad r = attr_gradient(f, a0, a1, a2);
+	const float gx = r.gx, gy = r.gy, gz = r.gz, gw = r.gw;
 
 	float w = f.w;
 	Q.a00 += w * (gx * gx);

@@ -56,20 +56,27 @@ def edge_limit_sq(indices, positions):
     return np.float32(m)
 
 
-def clod_simplify(indices, positions, locks, target_count, options, config=None, impl="oracle"):
+def clod_simplify(indices, positions, locks, target_count, options, config=None, impl="oracle", attributes=None, weights=None):
     """One group as clodBuild's simplify() returns it: (result indices in the shared vertex ids,
     float32 error).  options are the level's (Sparse is added, as the demo always passes it);
-    config holds meshopt_hip_clod_config's fields (None: no fallbacks, no edge limit)."""
+    config holds meshopt_hip_clod_config's fields (None: no fallbacks, no edge limit); attributes
+    (float32 [V, A] of the shared vertices) and weights as the demo's clodMesh carries them (None:
+    none)."""
     indices = np.ascontiguousarray(indices, dtype=np.uint32)
     positions = np.ascontiguousarray(positions, dtype=np.float32)
     cfg = config or {"fallback_permissive": 0, "fallback_sloppy": 0, "error_factor_sloppy": 1.0, "error_edge_limit": 0.0}
     if target_count > indices.size:
         return indices.copy(), np.float32(0)
-    empty_a, empty_w = np.zeros((positions.shape[0], 0), np.float32), np.zeros(0, np.float32)
-    lod, err = O.simplify(indices, positions, target_count, FLT_MAX, options | SPARSE, attributes=empty_a, weights=empty_w, lock=locks, impl=impl)
+    if attributes is None:
+        attributes, weights = np.zeros((positions.shape[0], 0), np.float32), np.zeros(0, np.float32)
+    else:
+        attributes = np.ascontiguousarray(attributes, dtype=np.float32)
+        weights = np.ascontiguousarray(weights, dtype=np.float32)
+    lod, err = O.simplify(indices, positions, target_count, FLT_MAX, options | SPARSE, attributes=attributes, weights=weights, lock=locks,
+                          impl=impl)
     err = np.float32(err)
     if lod.size > target_count and cfg["fallback_permissive"] and not (options & PERMISSIVE):
-        lod, err = O.simplify(indices, positions, target_count, FLT_MAX, options | SPARSE | PERMISSIVE, attributes=empty_a, weights=empty_w,
+        lod, err = O.simplify(indices, positions, target_count, FLT_MAX, options | SPARSE | PERMISSIVE, attributes=attributes, weights=weights,
                               lock=locks, impl=impl)
         err = np.float32(err)
     if lod.size > target_count and cfg["fallback_sloppy"]:

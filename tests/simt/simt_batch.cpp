@@ -144,10 +144,18 @@ static Task g_tasks[kMaxGroups];
 static void fiber_main()
 {
 	const Task& t = g_tasks[g.id];
+	const bool attr = t.a->attribute_count != 0;
 	if (t.ids)
-		mhip::simplify_from_cursor(*t.a, *t.L, t.ids, t.n, t.cursor, t.smem, t.part, t.sh, t.box, t.shf, t.next);
+	{
+		if (attr)
+			mhip::simplify_from_cursor<true>(*t.a, *t.L, t.ids, t.n, t.cursor, t.smem, t.part, t.sh, t.box, t.shf, t.next);
+		else
+			mhip::simplify_from_cursor<false>(*t.a, *t.L, t.ids, t.n, t.cursor, t.smem, t.part, t.sh, t.box, t.shf, t.next);
+	}
+	else if (attr)
+		mhip::simplify_patch<true>(*t.a, *t.L, t.patch, t.smem, t.part, t.sh, t.box, t.shf);
 	else
-		mhip::simplify_patch(*t.a, *t.L, t.patch, t.smem, t.part, t.sh, t.box, t.shf);
+		mhip::simplify_patch<false>(*t.a, *t.L, t.patch, t.smem, t.part, t.sh, t.box, t.shf);
 	g.fibers[g.current].state = kDone;
 	swapcontext(&g.fibers[g.current].ctx, &g.scheduler);
 }
@@ -336,6 +344,58 @@ extern "C" void simt_set_vertex_lock(const unsigned char* lock)
 	g_vertex_lock = lock;
 }
 
+// attributes of the next calls (packed per vertex like the positions, stride in bytes, one weight
+// per attribute; count 0: none)
+static const float* g_attributes = nullptr;
+static size_t g_attribute_stride = 0;
+static std::vector<float> g_weights;
+
+extern "C" void simt_set_attributes(const float* attributes, size_t stride, const float* weights, size_t count)
+{
+	g_attributes = count ? attributes : nullptr;
+	g_attribute_stride = stride;
+	g_weights.assign(weights, weights + count);
+}
+
+// the attribute fields of a batch's arguments, as the host sets them (the attributes with weight
+// > 0), with their global scratch in `store` (undefined at the start, like a fresh allocation)
+struct AttrStore
+{
+	std::vector<float> values;
+	std::vector<mhip::Quad> quadrics;
+	std::vector<mhip::Grad> grads;
+};
+
+static void set_attributes(mhip::BatchArgs& a, AttrStore& store)
+{
+	uint32_t used = 0;
+	for (size_t k = 0; k < g_weights.size(); ++k)
+		if (g_weights[k] > 0)
+		{
+			a.attr_map.index[used] = (uint32_t)k;
+			a.attr_map.weight[used] = g_weights[k];
+			used++;
+		}
+	a.attribute_count = g_attributes ? used : 0;
+	if (!a.attribute_count)
+		return;
+	a.attributes = g_attributes;
+	a.attribute_stride_floats = g_attribute_stride / sizeof(float);
+	const size_t n = a.vertex_count + 1;
+	store.values.resize(n * used);
+	store.quadrics.resize(n);
+	store.grads.resize(n * used);
+	for (auto& v : store.values)
+		v = (float)(simt::next_random() % 1000) - 500.f;
+	for (auto& q : store.quadrics)
+		q.a00 = q.a11 = q.a22 = q.a10 = q.a20 = q.a21 = q.b0 = q.b1 = q.b2 = q.c = q.w = q.pad = (float)(simt::next_random() % 1000) - 500.f;
+	for (auto& gr : store.grads)
+		gr.gx = gr.gy = gr.gz = gr.gw = (float)(simt::next_random() % 1000) - 500.f;
+	a.attr_values = store.values.data();
+	a.attr_quadrics = store.quadrics.data();
+	a.attr_grads = store.grads.data();
+}
+
 extern "C" int simt_simplify_batch(unsigned int* destination, unsigned int* result_counts, float* result_errors, unsigned int* status,
                                    unsigned int* stats, const unsigned int* indices, const unsigned int* index_offsets, const float* positions,
                                    const unsigned int* vertex_offsets, size_t stride, const unsigned int* targets, size_t count,
@@ -367,6 +427,8 @@ extern "C" int simt_simplify_batch(unsigned int* destination, unsigned int* resu
 	a.patch_count = count;
 	a.target_error = target_error;
 	a.options = options;
+	AttrStore attr_store;
+	set_attributes(a, attr_store);
 	// the host's two layouts for this size: capped candidate arrays first, full arrays for a
 	// mesh that leaves with status 5; each LDS image ends at its own guard page
 	const mhip::PatchLayout layouts[2] = {mhip::patch_layout(vmax, imax, mhip::batch_candidate_cap(imax)), mhip::patch_layout(vmax, imax, imax + 4)};
@@ -459,6 +521,8 @@ extern "C" int simt_simplify_batch_cursor(unsigned int* destination, unsigned in
 	a.patch_count = count;
 	a.target_error = target_error;
 	a.options = options;
+	AttrStore attr_store;
+	set_attributes(a, attr_store);
 	const mhip::PatchLayout L = mhip::patch_layout(vmax, imax, imax + 4);
 	std::vector<std::unique_ptr<simt::GuardedImage>> images;
 	struct Shared

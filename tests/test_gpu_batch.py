@@ -279,3 +279,73 @@ def test_batch_shapes(shape):
         check_batch_device(meshes, targets, 1e-2, M.SimplifyLockBorder)
     finally:
         M.batch_set_shape()
+
+
+def check_batch_attributes(meshes, attrs, weights, targets, err, opts, locks=None, entry="host"):
+    """The batched entries with attributes against the oracle's simplifyWithAttributes per mesh."""
+    if entry == "host":
+        got = M.simplify_batch(meshes, targets, err, opts, locks=locks, attributes=attrs, weights=weights)
+        res = [(rg, np.float32(eg)) for rg, eg in got]
+    else:
+        from devmem import DeviceArray
+        idx, io, verts, vo, stride = M.pack_meshes(meshes)
+        w = np.ascontiguousarray(weights, np.float32)
+        at = np.zeros((max(int(vo[-1]), 1), w.size), np.float32)
+        for p, a in enumerate(attrs):
+            at[vo[p]:vo[p + 1]] = a
+        lk = np.zeros(max(int(vo[-1]), 1), np.uint8)
+        if locks is not None:
+            for p, l in enumerate(locks):
+                lk[vo[p]:vo[p + 1]] = l
+        tg = np.ascontiguousarray(targets, dtype=np.uint32)
+        n = len(meshes)
+        with DeviceArray(idx) as d_idx, DeviceArray(io) as d_io, DeviceArray(verts) as d_v, DeviceArray(vo) as d_vo, DeviceArray(tg) as d_tg, \
+                DeviceArray(at) as d_at, DeviceArray(lk) as d_lk, DeviceArray(nbytes=idx.nbytes) as d_dst, DeviceArray(nbytes=4 * n) as d_cnt, \
+                DeviceArray(nbytes=4 * n) as d_err:
+            M.simplify_batch_with_attributes_device(d_dst.ptr, d_cnt.ptr, d_err.ptr, d_idx.ptr, d_io.ptr, d_v.ptr, d_vo.ptr, d_at.ptr, 4 * w.size, w,
+                                                    int(vo[-1]), stride, d_tg.ptr, n, err, opts, None if locks is None else d_lk.ptr)
+            dst = d_dst.to_host(np.uint32, idx.size)
+            cnt = d_cnt.to_host(np.uint32, n)
+            errs = d_err.to_host(np.float32, n)
+        res = [(dst[io[p]:io[p] + cnt[p]], errs[p]) for p in range(n)]
+    for p, ((ib, v), t, (rg, eg)) in enumerate(zip(meshes, targets, res)):
+        ro, eo = O.simplify(ib, v, int(t), err, opts, attributes=np.ascontiguousarray(attrs[p], np.float32), weights=np.asarray(weights, np.float32),
+                            lock=None if locks is None else locks[p])
+        assert rg.size == ro.size and np.array_equal(rg, ro), f"mesh {p} ({ib.size} indices): indices differ from the oracle"
+        assert same_bits(eg, eo), f"mesh {p}: result_error {eg!r} != {eo!r}"
+
+
+ATTRIBUTE_CASES = [
+    (0, list(W.SEAMED_GRID_WEIGHTS)),
+    (M.SimplifyPermissive, [1.0, 0.5, 0.0, 2.0, 0.25]),
+    (M.SimplifyLockBorder | M.SimplifyErrorAbsolute, [0.0, 0.0, 0.3, 0.0, 0.0]),
+    (M.SimplifyRegularize | M.SimplifyPermissive, list(W.SEAMED_GRID_WEIGHTS)),
+]
+
+
+@pytest.mark.parametrize("entry", ["host", "device"])
+@pytest.mark.parametrize("case", range(len(ATTRIBUTE_CASES)))
+def test_batch_attributes(case, entry):
+    """Batches with attributes (the attribute build of the batched kernel: rescaled attributes and
+    attribute quadrics in global memory, the rank's attribute terms with seam pairs and complex
+    wedges, the wedge-ordered attribute update): seamed grids with their five attributes across
+    every size class, caller protect bits, and one mesh beyond every class (the plain path with
+    attributes), against the oracle's simplifyWithAttributes mesh by mesh."""
+    opts, weights = ATTRIBUTE_CASES[case]
+    rng = np.random.default_rng(6600 + case)
+    meshes, attrs, targets, locks = [], [], [], []
+    for k in range(48):
+        n = int(rng.integers(4, 16)) if k < 36 else int(rng.integers(16, 30))
+        ib, v = random_grid(rng, n=n, seams=int(rng.integers(0, 4)))
+        meshes.append((ib, np.ascontiguousarray(v[:, :3])))
+        attrs.append(np.ascontiguousarray(v[:, 3:8]))
+        targets.append((ib.size // 3 * int(rng.integers(0, 4)) // 4) * 3)
+        locks.append(((rng.random(v.shape[0]) < 0.1) * M.SimplifyVertex_Protect).astype(np.uint8))
+    big = W.seamed_grid(44, seams=5, seed=case)
+    assert big.index_count > 5952
+    meshes.append((big.indices, np.ascontiguousarray(big.vertices[:, :3])))
+    attrs.append(np.ascontiguousarray(big.vertices[:, 3:8]))
+    targets.append(W.target_count(big.index_count, 1, 3))
+    locks.append(np.zeros(big.vertices.shape[0], np.uint8))
+    announce(f"batch attributes case {case} entry {entry} opts {opts} weights {weights}")
+    check_batch_attributes(meshes, attrs, np.array(weights, np.float32), targets, 1e-1, opts, locks=locks, entry=entry)

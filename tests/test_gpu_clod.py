@@ -22,14 +22,17 @@ def announce(what):
     sys.stdout.flush()
 
 
-def level_case(n, tiles, seed, bands=None):
+def level_case(n, tiles, seed, bands=None, attributes=None):
     """A seamed grid's positions, its triangles split into tiles x tiles spatial groups (or, with
     bands, into vertical bands cut at those x fractions), in shuffled group order, the position
     remap, random caller lock/protect bits, and the expected locks: lockBoundary's own loops
-    restated step by step (tests/meshes.py)."""
+    restated step by step (tests/meshes.py).  With `attributes` (a list), the grid's five
+    attributes are appended to it."""
     rng = np.random.default_rng(seed)
     m = W.seamed_grid(n, seams=4, seed=seed)
     v = np.ascontiguousarray(m.vertices[:, :3])
+    if attributes is not None:
+        attributes.append(np.ascontiguousarray(m.vertices[:, 3:8]))
     tri = m.indices.reshape(-1, 3)
     c = v[tri].mean(axis=1)
     if bands is None:
@@ -220,3 +223,54 @@ def test_clod_level_fallbacks(n, tiles, seed, opts, lock_frac, div, config):
         assert used["sloppy"] > 0, used
     if config["error_edge_limit"] > 0:
         assert used["edge"] > 0, used
+
+
+@pytest.mark.parametrize("n,tiles,seed,opts,config", [
+    (64, 4, 21, M.SimplifyErrorAbsolute | M.SimplifyPermissive,
+     {"fallback_permissive": 0, "fallback_sloppy": 1, "error_factor_sloppy": 2.0, "error_edge_limit": 0.0}),
+    (64, 5, 22, M.SimplifyErrorAbsolute,
+     {"fallback_permissive": 1, "fallback_sloppy": 1, "error_factor_sloppy": 2.0, "error_edge_limit": 0.5}),
+    (96, 2, 23, M.SimplifyErrorAbsolute | M.SimplifyPermissive, None),
+])
+def test_clod_level_attributes(n, tiles, seed, opts, config):
+    """One clusterlod level with the grid's five attributes (the demo's clodMesh attributes,
+    demo/clusterlod.h:470-479): every group as the oracle's restatement of clodBuild's simplify()
+    with simplifyWithAttributes(Sparse) returns it, the heavier locks of the first two cases
+    sending groups through the fallbacks; the last case's groups of about 14000 indices take the
+    plain path with attributes."""
+    from devmem import DeviceArray
+    from oracle import clod as C
+    from meshes import lock_boundary
+    holder = []
+    v, groups, remap, base, _ = level_case(n, tiles, seed, attributes=holder)
+    attrs = holder[0]
+    weights = np.array([0.5, 0.0, 1.0, 0.25, 2.0], np.float32)
+    rng = np.random.default_rng(seed + 100)
+    if config is not None:
+        base = (base | (rng.random(v.shape[0]) < 0.4) * M.SimplifyVertex_Lock).astype(np.uint8)
+    locks_expected = lock_boundary(remap, groups, base)
+    io = np.zeros(len(groups) + 1, np.uint32)
+    io[1:] = np.cumsum([grp.size for grp in groups])
+    idx = np.concatenate(groups).astype(np.uint32)
+    targets = np.array([grp.size // 3 // (6 if config else 2) * 3 for grp in groups], np.uint32)
+    G = len(groups)
+    announce(f"clod level attributes grid {n} groups {G} seed {seed} opts {opts} config {config}")
+    with DeviceArray(idx) as d_idx, DeviceArray(io) as d_io, DeviceArray(v) as d_v, DeviceArray(remap) as d_remap, \
+            DeviceArray(base) as d_base, DeviceArray(nbytes=v.shape[0]) as d_locks, DeviceArray(targets) as d_tg, DeviceArray(attrs) as d_at, \
+            DeviceArray(nbytes=idx.nbytes) as d_dst, DeviceArray(nbytes=4 * G) as d_cnt, DeviceArray(nbytes=4 * G) as d_err:
+        M.clod_level_device(d_dst.ptr, d_cnt.ptr, d_err.ptr, d_idx.ptr, d_io.ptr, G, d_v.ptr, v.shape[0], 12, d_remap.ptr, d_base.ptr,
+                            d_locks.ptr, d_tg.ptr, FLT_MAX, opts, config, attributes_ptr=d_at.ptr, attribute_stride=4 * attrs.shape[1],
+                            weights=weights)
+        locks = d_locks.to_host(np.uint8, v.shape[0])
+        counts = d_cnt.to_host(np.uint32, G)
+        errors = d_err.to_host(np.float32, G)
+        dst = d_dst.to_host(np.uint32, idx.size)
+    assert np.array_equal(locks, locks_expected)
+    moved = 0
+    for p, grp in enumerate(groups):
+        ro, eo = C.clod_simplify(grp, v, locks_expected, int(targets[p]), opts, config, attributes=attrs, weights=weights)
+        rg = dst[io[p]:io[p] + counts[p]]
+        assert rg.size == ro.size and np.array_equal(rg, ro), f"group {p} ({grp.size} indices): indices differ"
+        assert same_bits(errors[p], eo), f"group {p}: result_error {errors[p]!r} != {eo!r}"
+        moved += rg.size < grp.size
+    assert moved > 0

@@ -48,6 +48,8 @@ def build_simt(threads=256):
     lib.simt_retries.argtypes = []
     lib.simt_set_vertex_lock.restype = None
     lib.simt_set_vertex_lock.argtypes = [P]
+    lib.simt_set_attributes.restype = None
+    lib.simt_set_attributes.argtypes = [P, SZ, P, SZ]
     return lib
 
 
@@ -76,7 +78,7 @@ def simt_narrow():
     return build_simt(128)
 
 
-def run_simt(lib, meshes, targets, err, opts, mode=3, seed=1, locks=None, caps=None):
+def run_simt(lib, meshes, targets, err, opts, mode=3, seed=1, locks=None, caps=None, attrs=None, weights=None):
     idx, io, verts, vo, stride = M.pack_meshes(meshes)
     lk = None
     if locks is not None:
@@ -84,6 +86,13 @@ def run_simt(lib, meshes, targets, err, opts, mode=3, seed=1, locks=None, caps=N
         for p, l in enumerate(locks):
             lk[vo[p]:vo[p + 1]] = l
     lib.simt_set_vertex_lock(None if lk is None else lk.ctypes.data)
+    at = None
+    if attrs is not None:
+        w = np.ascontiguousarray(weights, np.float32)
+        at = np.zeros((max(int(vo[-1]), 1), w.size), np.float32)
+        for p, a in enumerate(attrs):
+            at[vo[p]:vo[p + 1]] = a
+        lib.simt_set_attributes(at.ctypes.data, 4 * w.size, w.ctypes.data, w.size)
     tg = np.ascontiguousarray(targets, dtype=np.uint32)
     n = len(meshes)
     vmax = (int(np.diff(vo).max()) + 31) & ~31
@@ -99,15 +108,20 @@ def run_simt(lib, meshes, targets, err, opts, mode=3, seed=1, locks=None, caps=N
                                  idx.ctypes.data, io.ctypes.data, verts.ctypes.data, vo.ctypes.data, stride, tg.ctypes.data,
                                  n, float(err), int(opts), vmax, imax, 120, mode, seed)
     lib.simt_set_vertex_lock(None)
+    lib.simt_set_attributes(None, 0, None, 0)
     assert rc == 0, "deadlock in the device code (see stderr)"
     return [(dest[io[p]:io[p] + cnt[p]].copy(), float(errs[p]), int(status[p])) for p in range(n)]
 
 
-def check(lib, meshes, targets, err, opts, locks=None, **kw):
-    got = run_simt(lib, meshes, targets, err, opts, locks=locks, **kw)
+def check(lib, meshes, targets, err, opts, locks=None, attrs=None, weights=None, **kw):
+    got = run_simt(lib, meshes, targets, err, opts, locks=locks, attrs=attrs, weights=weights, **kw)
     for p, ((ib, v), t, (r, e, st)) in enumerate(zip(meshes, targets, got)):
         assert st == 0, f"mesh {p}: status {st}"
-        ro, eo = O.simplify(ib, v, int(t), err, opts, lock=None if locks is None else locks[p])
+        if attrs is None:
+            ro, eo = O.simplify(ib, v, int(t), err, opts, lock=None if locks is None else locks[p])
+        else:
+            ro, eo = O.simplify(ib, v, int(t), err, opts, attributes=np.ascontiguousarray(attrs[p], np.float32),
+                                weights=np.asarray(weights, np.float32), lock=None if locks is None else locks[p])
         assert r.size == ro.size and np.array_equal(r, ro), f"mesh {p}: indices differ from the oracle"
         assert same_bits(e, eo), f"mesh {p}: result_error {e!r} != {eo!r}"
 
@@ -322,3 +336,31 @@ def test_sanitized_build(sanitizer):
     finally:
         os.remove(out)
     assert res.returncode == 0 and "sanitized run ok" in res.stdout, res.stdout[-2000:] + res.stderr[-4000:]
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_attributes(simt, simt_narrow, seed):
+    """Batches with attributes (the kAttr build of the device code: rescaled attributes, attribute
+    quadrics per wedge, the attribute terms of the rank with seam pairs and complex wedges, the
+    wedge-ordered attribute update): seamed grids with their five attributes and soups with one,
+    weights with zeros among them, against the oracle's simplifyWithAttributes per mesh."""
+    rng = np.random.default_rng(4400 + seed)
+    lib = simt if seed % 2 == 0 else simt_narrow
+    opts = [0, M.SimplifyPermissive, M.SimplifyLockBorder | M.SimplifyErrorAbsolute, M.SimplifyRegularize,
+            M.SimplifyPermissive | M.SimplifyErrorAbsolute, M.SimplifyRegularizeLight][seed]
+    meshes, targets, attrs, locks = [], [], [], []
+    soups = seed in (1, 4)
+    weights = np.array([0.5], np.float32) if soups else np.array([[1.0, 0.5, 0.0, 2.0, 0.25], [0.0, 0.0, 0.3, 0.0, 0.0],
+                                                                  list(W.SEAMED_GRID_WEIGHTS)][seed % 3], np.float32)
+    for k in range(5):
+        if soups:
+            ib, v = random_soup(rng, nv=int(rng.integers(3, 120)), ntri=int(rng.integers(1, 240)))
+            a = v[:, 3:4]
+        else:
+            ib, v = random_grid(rng, n=int(rng.integers(4, 18)), seams=int(rng.integers(0, 4)))
+            a = v[:, 3:8]
+        meshes.append((ib, np.ascontiguousarray(v[:, :3])))
+        attrs.append(np.ascontiguousarray(a, np.float32))
+        targets.append((ib.size // 3 * int(rng.integers(0, 4)) // 4) * 3)
+        locks.append(((rng.random(v.shape[0]) < 0.1) * M.SimplifyVertex_Protect).astype(np.uint8))
+    check(lib, meshes, targets, 1e-1, opts, locks=locks if seed % 2 else None, attrs=attrs, weights=weights, seed=seed)
