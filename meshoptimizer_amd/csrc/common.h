@@ -17,6 +17,10 @@ namespace mhip
 
 constexpr uint32_t kNone = 0xffffffffu;
 constexpr int kMaxAttributes = 32;
+// head rounds of a pass's commit (launched on the device's own list length), and the slots that
+// record their list sizes (the window's heads and two for the extension)
+constexpr int kMaxHeadRounds = 32;
+constexpr int kHeadSizeSlots = kMaxHeadRounds + 2;
 
 // vertex kinds (src/simplifier.cpp:296-305)
 enum : uint8_t
@@ -107,7 +111,7 @@ struct CallScalars
 	uint32_t component_count; // Prune: connected components of the welded mesh
 	uint32_t prune_min_bits;  // Prune: smallest component error among kept triangles (float bits)
 	uint32_t prune_max_bits;  // Prune: largest component error at or below a cutoff (float bits)
-	uint32_t head_sizes[16]; // work-list size at the start of each head round
+	uint32_t head_sizes[kHeadSizeSlots]; // work-list size at the start of each head round
 	uint32_t x_visits;      // ranks listed over every phase-X launch of the call (roofline accounting)
 	uint32_t x_launches;
 	uint32_t round;         // the commit rounds' base of the pass (list buffers rotate with the round number)

@@ -483,7 +483,6 @@ static void carve(Arena& a, Plan& p, size_t V_in, size_t V, size_t I, size_t A, 
 		m.pend_any[k] = a.take<uint64_t>(Vp);
 		m.pend_src[k] = a.take<uint64_t>(Vp);
 	}
-	m.src_off = a.take<uint32_t>(Vp + 1);
 	m.adj_off = a.take<uint32_t>(Vp + 1);
 	m.adj = a.take<uint2>(I + 4);
 
@@ -503,8 +502,6 @@ static void carve(Arena& a, Plan& p, size_t V_in, size_t V, size_t I, size_t A, 
 	m.r_r1 = a.take<uint32_t>(cap);
 	m.r_err = a.take<float>(cap);
 	m.status = a.take<uint8_t>(cap);
-	m.src_rank = a.take<uint32_t>(cap);
-	m.src_val = a.take<uint32_t>(cap);
 	m.list[0] = a.take<uint4>(cap);
 	m.list[1] = a.take<uint4>(cap);
 	m.stop_tile = a.take<uint32_t>(3 * (cap / kScanTile + 2));
@@ -588,6 +585,8 @@ static float host_float_bits(uint32_t u)
 
 // candidate lists up to this length are decided in one rank window (no extension phase)
 constexpr size_t kWholeWindow = 131072;
+// list length the single-workgroup tail of the commit rounds takes in one sweep
+constexpr uint32_t kTailList = 1024;
 
 // read-backs of the calling thread's current call (meshopt_hip_call_stats::read_backs)
 static thread_local unsigned int t_read_backs;
@@ -806,15 +805,14 @@ static size_t simplify_on_device(Context& ctx, const CallArgs& c, float* result_
 		launch_perform_init(s, m, C, W);
 
 		const int heads = ctx.head_rounds;
-		// head rounds sized for the window (their grid strides over any list length, so a grid
-		// capped at `max_threads` stays correct), then the tail and the flip-failure resolution
+		// head rounds with one thread per rank of the window (the list only shrinks; workgroups
+		// past its current length leave at once, while a grid smaller than the list would make
+		// each thread walk several ranks' dependent gathers in turn), then the tail and the
+		// flip-failure resolution; their grid strides over any list length, so a grid capped at
+		// `max_threads` stays correct
 		auto window_rounds = [&](size_t lo, size_t hi, int head_base, int head_count, size_t max_threads) {
-			double estimate = (double)(hi - lo);
 			for (int hd = 0; hd < head_count; ++hd)
-			{
-				launch_round(s, m, std::min((size_t)estimate + 256, max_threads), head_base + hd, hd);
-				estimate *= hd == 0 ? 0.9 : 0.75; // undecided ranks shrink geometrically
-			}
+				launch_round(s, m, std::min(hi - lo, max_threads), head_base + hd, hd);
 			launch_round_tail(s, m, head_count, (uint32_t)(hi - lo + 2));
 			launch_resolve(s, m, lo, hi);
 		};
@@ -846,13 +844,13 @@ static size_t simplify_on_device(Context& ctx, const CallArgs& c, float* result_
 		// head rounds for the next pass: enough to bring the list under the size one tail
 		// workgroup takes in a single sweep
 		int next = heads + 3;
-		for (int hd = 0; hd < heads && hd < 16; ++hd)
-			if (h.head_sizes[hd] < 1024)
+		for (int hd = 0; hd < heads && hd < kHeadSizeSlots; ++hd)
+			if (h.head_sizes[hd] < kTailList)
 			{
 				next = hd;
 				break;
 			}
-		ctx.head_rounds = std::max(2, std::min(16, next));
+		ctx.head_rounds = std::max(2, std::min(kMaxHeadRounds, next));
 		ctx.window_fraction = std::min(1.0, (double)(h.stop + 1) / (double)C);
 
 		meshopt_hip_pass_stats ps{};

@@ -270,11 +270,12 @@ __global__ void k_rank(RankArgs a, size_t count, uint32_t* c_v0, uint32_t* c_v1,
 }
 
 __global__ void k_gather_ranked(const uint32_t* order, size_t count, const uint32_t* c_v0, const uint32_t* c_v1, const float* c_err,
-                                const uint32_t* remap, uint32_t* r_v0, uint32_t* r_v1, uint32_t* r_r0, uint32_t* r_r1, float* r_err)
+                                const uint32_t* remap, uint32_t* r_v0, uint32_t* r_v1, uint32_t* r_r0, uint32_t* r_r1, float* r_err, uint8_t* status)
 {
 	size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
 	if (k >= count)
 		return;
+	status[k] = kUndecided;
 	uint32_t c = order[k];
 	uint32_t v0 = c_v0[c], v1 = c_v1[c];
 	r_v0[k] = v0;
@@ -304,7 +305,7 @@ void launch_rank_sort(hipStream_t s, Mesh& m, size_t cand_count)
 	// sortEdgeCollapses (src/simplifier.cpp:1452-1496): stable by the 12-bit key, ties in pick order
 	radix_sort_pairs(s, m.keys, m.vals, m.keys_alt, m.vals_alt, cand_count, nullptr, 12, m.temp);
 	k_gather_ranked<<<grid_for(cand_count), kThreads, 0, s>>>(m.vals, cand_count, m.c_v0, m.c_v1, m.c_err, m.remap, m.r_v0, m.r_v1, m.r_r0,
-	                                                          m.r_r1, m.r_err);
+	                                                          m.r_r1, m.r_err, m.status);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -370,12 +371,9 @@ __device__ __forceinline__ void commit_collapse(uint32_t* cremap, uint32_t i0, u
 
 enum : uint8_t
 {
-	kFlipPass = 4, // flip test passes in every remaining scenario; commits once lowest at its endpoints
-	kFlipFail = 5, // flip test fails in every scenario; resolved as skipped or rejected at the end
+	kFlipPass = 4, // flip test passes, now and for good; commits once lowest at its endpoints
+	kFlipFail = 5, // flip test fails for good; resolved as skipped or rejected at the end
 };
-
-constexpr int kMaxAlternatives = 2; // per neighbour: its current value and one pending move
-constexpr int kMaxPendingSlots = 4; // adjacency slots whose neighbour has a pending lower source
 
 struct RoundState
 {
@@ -389,9 +387,6 @@ struct RoundState
 	const uint32_t* remap;
 	const uint32_t* loop;
 	const uint32_t* loopback;
-	const uint32_t* src_off;  // candidate ranks grouped by source class
-	const uint32_t* src_rank;
-	const uint32_t* src_val;  // what each such candidate's commit writes at the class primary
 	uint32_t* cremap;
 	uint32_t* lock_rank;      // rank of the commit that locked a class (kNone: unlocked)
 	uint8_t* status;
@@ -401,133 +396,104 @@ struct RoundState
 	CallScalars* sc;
 };
 
-// collapse_remap[x] as the sequential loop sees it at rank k: a move committed by a higher
-// rank (possible once flip tests are decided early) reads as "not moved"
-__device__ __forceinline__ uint32_t remap_seen(const RoundState& a, uint32_t x, uint32_t k)
-{
-	uint32_t v = a.cremap[x];
-	return (v != x && a.lock_rank[x] > k) ? x : v;
-}
+// flip test (src/simplifier.cpp:1200-1232) of rank k, decided only where it cannot change any
+// more: 0 undetermined, 1 passes, 2 fails.  The test reads collapse_remap at the corners of the
+// triangles around k's source; a corner whose class published a lower source-role rank in the
+// previous round may still move before k decides (every pending rank publishes every round),
+// while every other corner keeps the value k sees now.  A triangle with no such corner that
+// flips fails the test for good; with no such corner anywhere, the test passes for good;
+// otherwise the rank waits.
+//
+// The test is a chain of dependent gathers (adjacency entry, then its corners' collapse_remap,
+// lock rank and pending minimum, then the positions they lead to), so the adjacency is read in
+// chunks of kFlipChunk entries whose gathers at each level are all issued before any is used:
+// a chunk costs three memory latencies rather than three per entry.
 
-// the value rank r's commit writes into collapse_remap[p], p the primary of its source class
-// (src/simplifier.cpp:1573-1605)
-__device__ __forceinline__ uint32_t commit_value_at(const RoundState& a, uint32_t r, uint32_t p)
-{
-	uint32_t i0 = a.r_v0[r], i1 = a.r_v1[r];
-	uint8_t kd = a.kind[i0];
-	if (kd == kComplex)
-		return complex_target(p, i1, a.remap, a.loop, a.loopback);
-	if (kd == kSeam)
-	{
-		uint32_t s0 = a.wedge[i0];
-		uint32_t s1 = a.loop[i0] == i1 ? a.loopback[s0] : a.loop[s0];
-		s1 = (s1 != kNone) ? s1 : a.wedge[i1];
-		uint32_t v = p;
-		if (i0 == p)
-			v = i1;
-		if (s0 == p)
-			v = s1;
-		return v;
-	}
-	return i1;
-}
+constexpr int kFlipChunk = 4;
 
-// flip test (src/simplifier.cpp:1200-1232) of rank k over every way the still-committable
-// lower ranks could move its neighbours: 0 undetermined, 1 passes in all, 2 fails in all.
-// A neighbour whose source-role minimum of the previous round is not below k has no pending
-// lower candidate (every pending rank publishes every round), so its list is not scanned.
-__device__ int flip_outcome(const RoundState& a, uint32_t i0, uint32_t i1, uint32_t k, uint32_t epoch)
+// begin/end: rank k's source's adjacency range; v0, v1: the positions of its source and target
+__device__ int flip_outcome(const RoundState& a, uint32_t i1, uint32_t k, uint32_t epoch, uint32_t begin, uint32_t end, const V3& v0, const V3& v1)
 {
 	const uint64_t* prev_src = a.pend_src[(epoch - 1) & 1];
 	const uint32_t prev_tag = ~(epoch - 1);
-	const uint32_t begin = a.adj_off[i0], end = a.adj_off[i0 + 1];
-
-	// too many moving neighbours: the outcome is rarely invariant, wait instead
-	int pending = 0;
-	for (uint32_t p = begin; p < end; ++p)
+	bool pending = false;
+	for (uint32_t p0 = begin; p0 < end; p0 += kFlipChunk)
 	{
-		uint2 nb2 = a.adj[p];
-		uint64_t pa = prev_src[nb2.x], pb = prev_src[nb2.y];
-		pending += (uint32_t)(pa >> 32) == prev_tag && (uint32_t)pa < k;
-		pending += (uint32_t)(pb >> 32) == prev_tag && (uint32_t)pb < k;
-	}
-	if (pending > kMaxPendingSlots)
-		return 0;
-
-	V3 v0 = load_pos(a.pos, i0), v1 = load_pos(a.pos, i1);
-	bool all_pass = true;
-	for (uint32_t p = begin; p < end; ++p)
-	{
-		uint2 nb2 = a.adj[p];
-		uint32_t nb[2] = {nb2.x, nb2.y};
-		uint32_t alt[2][kMaxAlternatives];
-		int na[2];
+		const uint32_t n = end - p0 < (uint32_t)kFlipChunk ? end - p0 : (uint32_t)kFlipChunk;
+		uint32_t nb[2 * kFlipChunk];
 #pragma unroll
-		for (int q = 0; q < 2; ++q)
+		for (int j = 0; j < kFlipChunk; ++j)
 		{
-			uint32_t x = nb[q];
-			na[q] = 0;
-			alt[q][na[q]++] = remap_seen(a, x, k);
-			uint64_t ps = prev_src[x];
-			if ((uint32_t)(ps >> 32) != prev_tag || (uint32_t)ps >= k)
-				continue;
-			for (uint32_t t = a.src_off[x], te = a.src_off[x + 1]; t < te; ++t)
-			{
-				uint32_t r = a.src_rank[t];
-				if (r >= k)
-					continue;
-				uint8_t st = a.status[r];
-				if (st != kUndecided && st != kFlipPass)
-					continue;
-				if (na[q] == kMaxAlternatives)
-					return 0;
-				alt[q][na[q]++] = a.src_val[t];
-			}
+			const uint2 e = (uint32_t)j < n ? a.adj[p0 + j] : make_uint2(i1, i1);
+			nb[2 * j] = e.x;
+			nb[2 * j + 1] = e.y;
 		}
-
-		bool any_flip = false, any_ok = false;
-		for (int x = 0; x < na[0]; ++x)
-			for (int y = 0; y < na[1]; ++y)
+		uint32_t cm[2 * kFlipChunk], lr[2 * kFlipChunk];
+		uint64_t ps[2 * kFlipChunk];
+#pragma unroll
+		for (int q = 0; q < 2 * kFlipChunk; ++q)
+		{
+			cm[q] = a.cremap[nb[q]];
+			lr[q] = a.lock_rank[nb[q]];
+			ps[q] = prev_src[nb[q]];
+		}
+		// collapse_remap as rank k sees it: a move committed by a higher rank (possible once
+		// flip tests are decided early) reads as "not moved"
+		V3 pa[kFlipChunk], pb[kFlipChunk];
+		uint32_t test_mask = 0;
+#pragma unroll
+		for (int j = 0; j < kFlipChunk; ++j)
+		{
+			bool moving = false;
+			uint32_t seen[2];
+#pragma unroll
+			for (int h = 0; h < 2; ++h)
 			{
-				if (nb[0] == nb[1] && x != y)
-					continue; // one vertex moves one way only
-				uint32_t va = alt[0][x], vb = alt[1][y];
-				if (va == i1 || vb == i1 || va == vb)
-					any_ok = true;
-				else if (triangle_flips(load_pos(a.pos, va), load_pos(a.pos, vb), v0, v1))
-					any_flip = true;
-				else
-					any_ok = true;
+				const int q = 2 * j + h;
+				seen[h] = (cm[q] != nb[q] && lr[q] > k) ? nb[q] : cm[q];
+				moving |= (uint32_t)(ps[q] >> 32) == prev_tag && (uint32_t)ps[q] < k;
 			}
-		if (any_flip && !any_ok)
-			return 2;
-		if (any_flip)
-			all_pass = false;
+			const bool live = (uint32_t)j < n;
+			pending |= live && moving;
+			const bool test = live && !moving && !(seen[0] == i1 || seen[1] == i1 || seen[0] == seen[1]);
+			test_mask |= test ? 1u << j : 0u;
+			pa[j] = test ? load_pos(a.pos, seen[0]) : V3{0.f, 0.f, 0.f};
+			pb[j] = test ? load_pos(a.pos, seen[1]) : V3{0.f, 0.f, 0.f};
+		}
+#pragma unroll
+		for (int j = 0; j < kFlipChunk; ++j)
+			if (((test_mask >> j) & 1u) && triangle_flips(pa[j], pb[j], v0, v1))
+				return 2;
 	}
-	return all_pass ? 1 : 0;
+	return pending ? 0 : 1;
 }
 
 // phase X of round `epoch`: no commit happens during it.  Ranks at a locked endpoint are
 // skipped; pending flip tests are evaluated; ranks that can still commit publish their rank at
-// both endpoint classes.  True = stays on the list.
+// both endpoint classes.  True = stays on the list.  Every gather the decision may need is
+// issued before the first is used.
 __device__ __forceinline__ bool round_x(const RoundState& a, uint4 w, uint32_t epoch)
 {
 	const uint32_t k = w.x, r0 = w.y, r1 = w.z;
-	uint8_t st = a.status[k];
+	const uint64_t* prev = a.pend[(epoch - 1) & 1];
+	const uint8_t st = a.status[k];
+	const uint32_t l0 = a.lock_rank[r0], l1 = a.lock_rank[r1];
+	const uint64_t p0 = prev[r0], p1 = prev[r1];
+	const uint32_t begin = a.adj_off[r0], end = a.adj_off[r0 + 1];
+	const V3 v0 = load_pos(a.pos, r0), v1 = load_pos(a.pos, r1);
 	if (st != kUndecided && st != kFlipPass)
 		return false;
-	if (a.lock_rank[r0] != kNone || a.lock_rank[r1] != kNone)
+	if (l0 != kNone || l1 != kNone)
 	{
 		a.status[k] = kSkipped;
 		return false;
 	}
 	// the flip test is only worth evaluating for a rank that was the lowest published at one of
 	// its endpoints in the previous round (others cannot commit yet)
-	const uint64_t* prev = a.pend[(epoch - 1) & 1];
-	uint64_t was = round_key(epoch - 1, k);
-	if (st == kUndecided && (prev[r0] == was || prev[r1] == was))
+	const uint64_t was = round_key(epoch - 1, k);
+	if (st == kUndecided && (p0 == was || p1 == was))
 	{
-		int o = flip_outcome(a, r0, r1, k, epoch);
+		int o = flip_outcome(a, r1, k, epoch, begin, end, v0, v1);
 		if (o == 2)
 		{
 			a.status[k] = kFlipFail;
@@ -591,7 +557,7 @@ __global__ void __launch_bounds__(kThreads) k_round_x(RoundState a, uint32_t hea
 	if (blockIdx.x == 0 && threadIdx.x == 0)
 	{
 		sc->list_count[(round + 2) % 3] = 0;
-		if (head_index < 16)
+		if (head_index < kHeadSizeSlots)
 			sc->head_sizes[head_index] = n;
 		sc->x_visits += n;
 		sc->x_launches += 1;
@@ -690,30 +656,6 @@ __global__ void k_perform_init_scalars(CallScalars* sc)
 	sc->extended = 0;
 }
 
-// candidate ranks grouped by source class (order inside a group is irrelevant)
-// one atomic pass: count the ranks per source class and hand each rank its slot in the class's
-// group (and reset its status); after the scan the fill places the ranks without atomics
-__global__ void k_src_count(const uint32_t* r_r0, size_t count, uint32_t* counts, uint32_t* slot, uint8_t* status)
-{
-	size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-	if (k < count)
-	{
-		slot[k] = atomicAdd(&counts[r_r0[k]], 1u);
-		status[k] = kUndecided;
-	}
-}
-
-__global__ void k_src_fill(RoundState a, const uint32_t* r_r0, size_t count, const uint32_t* slot, uint32_t* src_rank, uint32_t* src_val)
-{
-	size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-	if (k >= count)
-		return;
-	uint32_t r0 = r_r0[k];
-	uint32_t at = a.src_off[r0] + slot[k];
-	src_rank[at] = (uint32_t)k;
-	src_val[at] = commit_value_at(a, (uint32_t)k, r0);
-}
-
 static RoundState round_state(Mesh& m)
 {
 	RoundState a;
@@ -727,9 +669,6 @@ static RoundState round_state(Mesh& m)
 	a.remap = m.remap;
 	a.loop = m.loop[m.loop_cur];
 	a.loopback = m.loopback[m.loop_cur];
-	a.src_off = m.src_off;
-	a.src_rank = m.src_rank;
-	a.src_val = m.src_val;
 	a.cremap = m.cremap;
 	a.lock_rank = m.lock_rank;
 	a.status = m.status;
@@ -747,15 +686,9 @@ void launch_perform_init(hipStream_t s, Mesh& m, size_t cand_count, size_t windo
 {
 	(void)window;
 	size_t n = m.vertex_count;
+	(void)cand_count;
 	iota_u32(s, m.cremap, n);
 	fill_u32(s, m.lock_rank, kNone, n);
-	fill_u32(s, m.src_off, 0, n + 1);
-	// the rank slots go to m.keys, free once the ranked copies are gathered
-	if (cand_count)
-		k_src_count<<<grid_for(cand_count), kThreads, 0, s>>>(m.r_r0, cand_count, m.src_off, m.keys, m.status);
-	scan_exclusive(s, m.src_off, m.src_off, n, nullptr, m.src_off + n, m.temp);
-	if (cand_count)
-		k_src_fill<<<grid_for(cand_count), kThreads, 0, s>>>(round_state(m), m.r_r0, cand_count, m.keys, m.src_rank, m.src_val);
 	k_perform_init_scalars<<<1, 1, 0, s>>>(m.sc);
 }
 
