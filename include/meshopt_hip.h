@@ -128,6 +128,7 @@ typedef struct meshopt_hip_pass_stats
 	unsigned int rejected;       /* would flip a triangle */
 	unsigned int stop;           /* 0 out of collapses, 1 error limit, 2 count limit, 3 error goal */
 	unsigned int rounds;         /* priority rounds the GPU commit used */
+	unsigned int tail_rounds;    /* of them, the rounds run by the single-workgroup tail */
 } meshopt_hip_pass_stats;
 
 typedef struct meshopt_hip_call_stats
@@ -147,7 +148,7 @@ typedef struct meshopt_hip_call_stats
 /* Copies up to max_passes pass records of the calling thread's last call; returns the pass count. */
 int meshopt_hip_last_call_stats(meshopt_hip_call_stats* call, meshopt_hip_pass_stats* passes, int max_passes);
 
-/* HIP-event timing of the dominant kernel (phase X of the commit rounds) on the library's
+/* HIP-event timing of the dominant kernel (k_round_x, the commit rounds) on the library's
  * stream.  Each call returns the totals gathered since the previous call (summed kernel time in
  * ms, launches, ranks listed over those launches), resets them, and turns timing on or off. */
 void meshopt_hip_kernel_timing(int enable, double* total_ms, size_t* launches, size_t* ranks);

@@ -81,7 +81,7 @@ EXPORTED_SYMBOLS = (
 
 class PassStats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint) for n in (
-        "live_triangles", "candidates", "evaluated", "committed", "skipped", "rejected", "stop", "rounds")]
+        "live_triangles", "candidates", "evaluated", "committed", "skipped", "rejected", "stop", "rounds", "tail_rounds")]
 
 
 class CallStats(ctypes.Structure):
@@ -386,7 +386,7 @@ def simplify_batch(meshes, target_index_counts, target_error=1e-2, options=0, lo
 
 
 def kernel_timing(enable: bool):
-    """Turns HIP-event timing of the phase-X round kernel on/off; returns (ms, launches, ranks)
+    """Turns HIP-event timing of the round kernel (k_round_x) on/off; returns (ms, launches, ranks)
     gathered since the previous call."""
     lib = library()
     ms = ctypes.c_double(0)

@@ -112,7 +112,7 @@ struct CallScalars
 	uint32_t prune_min_bits;  // Prune: smallest component error among kept triangles (float bits)
 	uint32_t prune_max_bits;  // Prune: largest component error at or below a cutoff (float bits)
 	uint32_t head_sizes[kHeadSizeSlots]; // work-list size at the start of each head round
-	uint32_t x_visits;      // ranks listed over every phase-X launch of the call (roofline accounting)
+	uint32_t x_visits;      // ranks listed over every k_round_x launch of the call (roofline accounting)
 	uint32_t x_launches;
 	uint32_t round;         // the commit rounds' base of the pass (list buffers rotate with the round number)
 	uint32_t epoch;         // the round tag before the base round (round minima are tagged, monotonic over the call)

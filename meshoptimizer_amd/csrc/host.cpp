@@ -124,8 +124,10 @@ struct Context
 	CallScalars* sc_host = nullptr; // pinned mirror of the device scalar block
 	std::vector<meshopt_hip_pass_stats> passes;
 	meshopt_hip_call_stats call{};
-	int head_rounds = 8;
-	// phase-X kernel timing of this context's calls (meshopt_hip_kernel_timing), and the events of
+	// head rounds of the commit, learned per pass index from the calls before (a call's first
+	// passes have long lists, its last ones short)
+	std::vector<int> head_rounds;
+	// round-kernel (k_round_x) timing of this context's calls (meshopt_hip_kernel_timing), and the events of
 	// the batched calls
 	bool timing = false;
 	std::vector<hipEvent_t> ev_begin, ev_end;
@@ -804,7 +806,10 @@ static size_t simplify_on_device(Context& ctx, const CallArgs& c, float* result_
 		W = C <= kWholeWindow ? C : std::min(C, std::max<size_t>(W, 32768));
 		launch_perform_init(s, m, C, W);
 
-		const int heads = ctx.head_rounds;
+		const size_t pass_index = t_passes.size();
+		if (ctx.head_rounds.size() <= pass_index)
+			ctx.head_rounds.resize(pass_index + 1, 16);
+		const int heads = ctx.head_rounds[pass_index];
 		// head rounds with one thread per rank of the window (the list only shrinks; workgroups
 		// past its current length leave at once, while a grid smaller than the list would make
 		// each thread walk several ranks' dependent gathers in turn), then the tail and the
@@ -850,7 +855,7 @@ static size_t simplify_on_device(Context& ctx, const CallArgs& c, float* result_
 				next = hd;
 				break;
 			}
-		ctx.head_rounds = std::max(2, std::min(kMaxHeadRounds, next));
+		ctx.head_rounds[pass_index] = std::max(2, std::min(kMaxHeadRounds, next));
 		ctx.window_fraction = std::min(1.0, (double)(h.stop + 1) / (double)C);
 
 		meshopt_hip_pass_stats ps{};
@@ -862,6 +867,7 @@ static size_t simplify_on_device(Context& ctx, const CallArgs& c, float* result_
 		ps.skipped = h.skipped;
 		ps.rejected = h.rejected;
 		ps.rounds = h.round;
+		ps.tail_rounds = h.tail_rounds;
 		t_passes.push_back(ps);
 		t_call.device_extensions += h.extended;
 

@@ -6,7 +6,7 @@
 namespace mhip
 {
 
-// optional HIP-event timing of the phase-X round kernel (the dominant kernel)
+// optional HIP-event timing of the round kernel k_round_x (the dominant kernel)
 struct KernelTimer
 {
 	hipEvent_t* begin;   // event pairs recorded around each launch

@@ -468,10 +468,20 @@ __device__ int flip_outcome(const RoundState& a, uint32_t i1, uint32_t k, uint32
 	return pending ? 0 : 1;
 }
 
-// phase X of round `epoch`: no commit happens during it.  Ranks at a locked endpoint are
-// skipped; pending flip tests are evaluated; ranks that can still commit publish their rank at
-// both endpoint classes.  True = stays on the list.  Every gather the decision may need is
-// issued before the first is used.
+// round `epoch`, one rank.  A rank at a locked endpoint is skipped.  A rank that was the lowest
+// published at one of its endpoints in the previous round gets its flip test; one that was the
+// lowest at both and whose flip test has passed commits now (the commit phase of the previous
+// round: no lower rank is left at its endpoints, and ranks committing in one round share no
+// endpoint and no neighbourhood in which one of them is still pending for the other, so they
+// never touch each other's state).  The rest publish their rank at both endpoint classes and
+// stay on the list (true).  Every gather the decision may need is issued before the first is
+// used.
+//
+// Commits run concurrently with the other ranks' tests of the same round.  A rank's flip test
+// only trusts corners without a lower source published in the previous round, and every rank
+// that commits in this round published its source then, so no test reads a move still in
+// flight.  A lock written in this round may be missed by a rank sharing that endpoint: that rank
+// publishes once more and is skipped in the next round, before it can commit.
 __device__ __forceinline__ bool round_x(const RoundState& a, uint4 w, uint32_t epoch)
 {
 	const uint32_t k = w.x, r0 = w.y, r1 = w.z;
@@ -491,7 +501,9 @@ __device__ __forceinline__ bool round_x(const RoundState& a, uint4 w, uint32_t e
 	// the flip test is only worth evaluating for a rank that was the lowest published at one of
 	// its endpoints in the previous round (others cannot commit yet)
 	const uint64_t was = round_key(epoch - 1, k);
-	if (st == kUndecided && (p0 == was || p1 == was))
+	const bool low0 = p0 == was, low1 = p1 == was;
+	uint8_t now = st;
+	if (st == kUndecided && (low0 || low1))
 	{
 		int o = flip_outcome(a, r1, k, epoch, begin, end, v0, v1);
 		if (o == 2)
@@ -500,31 +512,24 @@ __device__ __forceinline__ bool round_x(const RoundState& a, uint4 w, uint32_t e
 			return false;
 		}
 		if (o == 1)
-			a.status[k] = kFlipPass;
+			now = kFlipPass;
 	}
+	if (now == kFlipPass && low0 && low1)
+	{
+		commit_collapse(a.cremap, a.r_v0[k], a.r_v1[k], a.kind, a.wedge, a.remap, a.loop, a.loopback);
+		a.lock_rank[r0] = k;
+		a.lock_rank[r1] = k;
+		a.status[k] = kCommitted;
+		return false;
+	}
+	if (now != st)
+		a.status[k] = now;
 	uint64_t key = round_key(epoch, k);
 	unsigned long long* wr = (unsigned long long*)a.pend[epoch & 1];
 	atomicMin(&wr[r0], (unsigned long long)key);
 	atomicMin(&wr[r1], (unsigned long long)key);
 	atomicMin((unsigned long long*)&a.pend_src[epoch & 1][r0], (unsigned long long)key);
 	return true;
-}
-
-// phase Y of round `epoch`: a flip-passed rank that is the minimum published at both endpoint
-// classes commits (such ranks share no endpoint, so they never touch each other's state)
-__device__ __forceinline__ void round_y(const RoundState& a, uint4 w, uint32_t epoch)
-{
-	const uint32_t k = w.x, r0 = w.y, r1 = w.z;
-	if (a.status[k] != kFlipPass)
-		return;
-	const uint64_t* rd = a.pend[epoch & 1];
-	uint64_t mine = round_key(epoch, k);
-	if (rd[r0] != mine || rd[r1] != mine)
-		return;
-	commit_collapse(a.cremap, a.r_v0[k], a.r_v1[k], a.kind, a.wedge, a.remap, a.loop, a.loopback);
-	a.lock_rank[r0] = k;
-	a.lock_rank[r1] = k;
-	a.status[k] = kCommitted;
 }
 
 __device__ __forceinline__ void append_wave(bool keep, uint4 w, uint4* out_list, uint32_t* out_count)
@@ -543,9 +548,9 @@ __device__ __forceinline__ void append_wave(bool keep, uint4 w, uint4* out_list,
 	}
 }
 
-// head round, phase X.  Round r reads list (r & 1) / counter (r % 3), appends the survivors
-// to list ((r + 1) & 1) / counter ((r + 1) % 3) and clears counter ((r + 2) % 3).  One rank
-// per thread when the grid is large enough; the grid-stride loop covers any list length.
+// head round.  Round r reads list (r & 1) / counter (r % 3), appends the survivors to list
+// ((r + 1) & 1) / counter ((r + 1) % 3) and clears counter ((r + 2) % 3).  One rank per thread
+// when the grid is large enough; the grid-stride loop covers any list length.
 __global__ void __launch_bounds__(kThreads) k_round_x(RoundState a, uint32_t head_index, uint32_t rel)
 {
 	CallScalars* sc = a.sc;
@@ -572,17 +577,6 @@ __global__ void __launch_bounds__(kThreads) k_round_x(RoundState a, uint32_t hea
 		bool keep = idx < n && round_x(a, w, epoch);
 		append_wave(keep, w, out_list, out_count);
 	}
-}
-
-// head round, phase Y over the survivors of phase X
-__global__ void __launch_bounds__(kThreads) k_round_y(RoundState a, uint32_t rel)
-{
-	const CallScalars* sc = a.sc;
-	const uint32_t round = sc->round + rel, epoch = sc->epoch + 1 + rel;
-	const uint32_t n = sc->list_count[(round + 1) % 3];
-	const uint4* list = a.list[(round + 1) & 1];
-	for (uint32_t idx = blockIdx.x * kThreads + threadIdx.x; idx < n; idx += gridDim.x * kThreads)
-		round_y(a, list[idx], epoch);
 }
 
 constexpr int kTailThreads = 1024;
@@ -618,10 +612,6 @@ __global__ void __launch_bounds__(kTailThreads) k_round_tail(RoundState a, uint3
 			bool keep = idx < n && round_x(a, w, epoch);
 			append_wave(keep, w, out_list, &s_out);
 		}
-		__syncthreads();
-		uint32_t m = s_out;
-		for (uint32_t idx = threadIdx.x; idx < m; idx += kTailThreads)
-			round_y(a, out_list[idx], epoch);
 		__syncthreads();
 		if (threadIdx.x == 0)
 		{
@@ -732,7 +722,6 @@ void launch_round(hipStream_t s, Mesh& m, size_t list_estimate, int head_index, 
 	k_round_x<<<blocks, kThreads, 0, s>>>(a, (uint32_t)head_index, (uint32_t)rel);
 	if (timed)
 		HIP_CHECK(hipEventRecord(t->end[t->count++], s));
-	k_round_y<<<blocks, kThreads, 0, s>>>(a, (uint32_t)rel);
 }
 
 void launch_round_tail(hipStream_t s, Mesh& m, int rel, uint32_t max_rounds)
