@@ -124,9 +124,10 @@ struct Context
 	CallScalars* sc_host = nullptr; // pinned mirror of the device scalar block
 	std::vector<meshopt_hip_pass_stats> passes;
 	meshopt_hip_call_stats call{};
-	// head rounds of the commit, learned per pass index from the calls before (a call's first
-	// passes have long lists, its last ones short)
-	std::vector<int> head_rounds;
+	// head rounds of the commit, learned per window size (by its power of two) from the passes
+	// before: the rounds a list needs grow with its length, and calls of very different sizes
+	// share a context (serving slots)
+	int head_rounds[33] = {};
 	// round-kernel (k_round_x) timing of this context's calls (meshopt_hip_kernel_timing), and the events of
 	// the batched calls
 	bool timing = false;
@@ -806,10 +807,13 @@ static size_t simplify_on_device(Context& ctx, const CallArgs& c, float* result_
 		W = C <= kWholeWindow ? C : std::min(C, std::max<size_t>(W, 32768));
 		launch_perform_init(s, m, C, W);
 
-		const size_t pass_index = t_passes.size();
-		if (ctx.head_rounds.size() <= pass_index)
-			ctx.head_rounds.resize(pass_index + 1, 16);
-		const int heads = ctx.head_rounds[pass_index];
+		// a window the tail takes in one sweep per round needs no head rounds
+		int bucket = 0;
+		while (bucket < 32 && (size_t(1) << bucket) < W)
+			bucket++;
+		if (!ctx.head_rounds[bucket])
+			ctx.head_rounds[bucket] = 16;
+		const int heads = W <= kTailList ? 0 : ctx.head_rounds[bucket];
 		// head rounds with one thread per rank of the window (the list only shrinks; workgroups
 		// past its current length leave at once, while a grid smaller than the list would make
 		// each thread walk several ranks' dependent gathers in turn), then the tail and the
@@ -846,8 +850,8 @@ static size_t simplify_on_device(Context& ctx, const CallArgs& c, float* result_
 		if (h.work[3])
 			throw std::runtime_error("priority rounds did not converge (internal error)");
 
-		// head rounds for the next pass: enough to bring the list under the size one tail
-		// workgroup takes in a single sweep
+		// head rounds for the next window of this size: enough to bring the list under the size
+		// one tail workgroup takes in a single sweep
 		int next = heads + 3;
 		for (int hd = 0; hd < heads && hd < kHeadSizeSlots; ++hd)
 			if (h.head_sizes[hd] < kTailList)
@@ -855,7 +859,8 @@ static size_t simplify_on_device(Context& ctx, const CallArgs& c, float* result_
 				next = hd;
 				break;
 			}
-		ctx.head_rounds[pass_index] = std::max(2, std::min(kMaxHeadRounds, next));
+		if (heads)
+			ctx.head_rounds[bucket] = std::max(2, std::min(kMaxHeadRounds, next));
 		ctx.window_fraction = std::min(1.0, (double)(h.stop + 1) / (double)C);
 
 		meshopt_hip_pass_stats ps{};
