@@ -148,6 +148,16 @@ typedef struct meshopt_hip_call_stats
 /* Copies up to max_passes pass records of the calling thread's last call; returns the pass count. */
 int meshopt_hip_last_call_stats(meshopt_hip_call_stats* call, meshopt_hip_pass_stats* passes, int max_passes);
 
+/* Policy of the plain path's commit rounds (no counterpart in the reference; results never depend
+ * on it).  A rank's flip test is decided early only where it cannot change any more; the cheap
+ * test waits while any corner of the rank's neighbourhood may still move, the deep test also
+ * decides when the outcome is the same for every way the lower candidates could move those
+ * corners (fewer rounds, dearer ones).  A window runs deep rounds when the head-round count the
+ * library learned for windows of its size exceeds deep_heads (-1: every window; below -1: the
+ * default, 64), from head round deep_from_head on (default 8) and in its tail.  Applies to later
+ * calls in the process; returns 0, or -1 with meshopt_hip_last_error set. */
+int meshopt_hip_set_round_policy(int deep_heads, int deep_from_head);
+
 /* HIP-event timing of the dominant kernel (k_round_x, the commit rounds) on the library's
  * stream.  Each call returns the totals gathered since the previous call (summed kernel time in
  * ms, launches, ranks listed over those launches), resets them, and turns timing on or off. */

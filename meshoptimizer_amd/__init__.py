@@ -73,6 +73,7 @@ EXPORTED_SYMBOLS = (
     "meshopt_hip_server_clock_ms",
     "meshopt_hip_server_destroy",
     "meshopt_hip_last_call_stats",
+    "meshopt_hip_set_round_policy",
     "meshopt_hip_last_error",
     "meshopt_hip_kernel_timing",
     "meshopt_hip_release",
@@ -164,6 +165,8 @@ def library() -> ctypes.CDLL:
     lib.meshopt_hip_batch_get_shape.restype = ctypes.c_int
     lib.meshopt_hip_batch_get_shape.argtypes = [_P, _P, _P, _P, _P]
     lib.meshopt_hip_last_batch_stats.argtypes = [_P, _SZ]
+    lib.meshopt_hip_set_round_policy.restype = ctypes.c_int
+    lib.meshopt_hip_set_round_policy.argtypes = [ctypes.c_int, ctypes.c_int]
     lib.meshopt_hip_batch_kernel_timing.restype = None
     lib.meshopt_hip_batch_kernel_timing.argtypes = [_P, _P, _P]
     lib.meshopt_hip_kernel_timing.restype = None
@@ -510,6 +513,14 @@ def clod_level_device(destination_ptr, result_counts_ptr, result_errors_ptr, ind
                                            ctypes.byref(cfg) if cfg is not None else None)
     if rc != 0:
         raise RuntimeError(lib.meshopt_hip_last_error().decode())
+
+
+def set_round_policy(deep_heads=-2, deep_from_head=8):
+    """meshopt_hip_set_round_policy: when the plain path's commit rounds examine the alternatives of
+    moving corners (deep_heads -1: always; below -1: the default).  Raises on error."""
+    lib = library()
+    if lib.meshopt_hip_set_round_policy(int(deep_heads), int(deep_from_head)) != 0:
+        raise ValueError(lib.meshopt_hip_last_error().decode())
 
 
 def last_call_stats():

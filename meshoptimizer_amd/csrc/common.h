@@ -19,7 +19,7 @@ constexpr uint32_t kNone = 0xffffffffu;
 constexpr int kMaxAttributes = 32;
 // head rounds of a pass's commit (launched on the device's own list length), and the slots that
 // record their list sizes (the window's heads and two for the extension)
-constexpr int kMaxHeadRounds = 32;
+constexpr int kMaxHeadRounds = 256;
 constexpr int kHeadSizeSlots = kMaxHeadRounds + 2;
 
 // vertex kinds (src/simplifier.cpp:296-305)

@@ -13,6 +13,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -124,10 +125,6 @@ struct Context
 	CallScalars* sc_host = nullptr; // pinned mirror of the device scalar block
 	std::vector<meshopt_hip_pass_stats> passes;
 	meshopt_hip_call_stats call{};
-	// head rounds of the commit, learned per window size (by its power of two) from the passes
-	// before: the rounds a list needs grow with its length, and calls of very different sizes
-	// share a context (serving slots)
-	int head_rounds[33] = {};
 	// round-kernel (k_round_x) timing of this context's calls (meshopt_hip_kernel_timing), and the events of
 	// the batched calls
 	bool timing = false;
@@ -480,6 +477,7 @@ static void carve(Arena& a, Plan& p, size_t V_in, size_t V, size_t I, size_t A, 
 	m.ag = A ? a.take<Grad>(Vp * A) : nullptr;
 	m.va = A ? a.take<float>(Vp * A) : nullptr;
 	m.cremap = a.take<uint32_t>(Vp);
+	m.src_off = a.take<uint32_t>(Vp + 1);
 	m.lock_rank = a.take<uint32_t>(Vp);
 	for (int k = 0; k < 2; ++k)
 	{
@@ -504,6 +502,8 @@ static void carve(Arena& a, Plan& p, size_t V_in, size_t V, size_t I, size_t A, 
 	m.r_r0 = a.take<uint32_t>(cap);
 	m.r_r1 = a.take<uint32_t>(cap);
 	m.r_err = a.take<float>(cap);
+	m.src_rank = a.take<uint32_t>(cap);
+	m.src_val = a.take<uint32_t>(cap);
 	m.status = a.take<uint8_t>(cap);
 	m.list[0] = a.take<uint4>(cap);
 	m.list[1] = a.take<uint4>(cap);
@@ -590,6 +590,18 @@ static float host_float_bits(uint32_t u)
 constexpr size_t kWholeWindow = 131072;
 // list length the single-workgroup tail of the commit rounds takes in one sweep
 constexpr uint32_t kTailList = 1024;
+// head rounds of a window seen for the first time
+constexpr int kFirstHeads = 16;
+// a window whose learned head-round count exceeds g_deep_heads has long chains of dependent ranks
+// (as on large regular grids) and runs deep rounds: its head rounds from g_deep_from_head on, and
+// its tail, examine the alternatives of moving corners (meshopt_hip_set_round_policy sets both)
+constexpr int kDeepHeads = 64;
+constexpr int kDeepFromHead = 8;
+static std::atomic<int> g_deep_heads{kDeepHeads}, g_deep_from_head{kDeepFromHead};
+// head rounds of the commit, learned per window size (by its power of two) from every call in the
+// process: the rounds a list needs grow with its length, and the contexts of concurrent calls and
+// serving slots see calls of very different sizes (0: not seen yet)
+static std::atomic<int> g_head_rounds[33];
 
 // read-backs of the calling thread's current call (meshopt_hip_call_stats::read_backs)
 static thread_local unsigned int t_read_backs;
@@ -784,6 +796,9 @@ static size_t simplify_on_device(Context& ctx, const CallArgs& c, float* result_
 	// index remap, reading the new triangle count on the device.  Prune reads the pass's error back
 	// before its component pruning, and picks after it.
 	bool picked = !prune;
+	// set once a window of this call needed more rounds than the cheap ones drain (long chains of
+	// dependent ranks, as on regular grids): later windows of the call run deep rounds
+	bool long_chains = false;
 	while (result_count > c.target_index_count)
 	{
 		size_t T = result_count / 3;
@@ -805,15 +820,19 @@ static size_t simplify_on_device(Context& ctx, const CallArgs& c, float* result_
 		// fraction of candidates the previous pass evaluated; a short list is one window
 		size_t W = ctx.window_fraction > 0 ? (size_t)(ctx.window_fraction * (double)C * 1.25) : (size_t)goal * 3 / 4;
 		W = C <= kWholeWindow ? C : std::min(C, std::max<size_t>(W, 32768));
-		launch_perform_init(s, m, C, W);
 
 		// a window the tail takes in one sweep per round needs no head rounds
 		int bucket = 0;
 		while (bucket < 32 && (size_t(1) << bucket) < W)
 			bucket++;
-		if (!ctx.head_rounds[bucket])
-			ctx.head_rounds[bucket] = 16;
-		const int heads = W <= kTailList ? 0 : ctx.head_rounds[bucket];
+		const int learned = g_head_rounds[bucket].load(std::memory_order_relaxed);
+		const int heads = W <= kTailList ? 0 : (learned ? learned : kFirstHeads);
+		// the flip test of the first rounds waits for every moving corner (cheap, and enough for
+		// most meshes); a window known to need long chains of rounds examines the moving corners'
+		// alternatives from round kDeepFromHead on, which decides ranks far earlier on such meshes
+		const int deep_heads = g_deep_heads.load(std::memory_order_relaxed), deep_from = g_deep_from_head.load(std::memory_order_relaxed);
+		const bool deep = long_chains || learned > deep_heads || deep_heads < 0;
+		launch_perform_init(s, m, C, deep);
 		// head rounds with one thread per rank of the window (the list only shrinks; workgroups
 		// past its current length leave at once, while a grid smaller than the list would make
 		// each thread walk several ranks' dependent gathers in turn), then the tail and the
@@ -821,8 +840,8 @@ static size_t simplify_on_device(Context& ctx, const CallArgs& c, float* result_
 		// `max_threads` stays correct
 		auto window_rounds = [&](size_t lo, size_t hi, int head_base, int head_count, size_t max_threads) {
 			for (int hd = 0; hd < head_count; ++hd)
-				launch_round(s, m, std::min(hi - lo, max_threads), head_base + hd, hd);
-			launch_round_tail(s, m, head_count, (uint32_t)(hi - lo + 2));
+				launch_round(s, m, std::min(hi - lo, max_threads), head_base + hd, hd, deep && hd >= deep_from);
+			launch_round_tail(s, m, head_count, (uint32_t)(hi - lo + 2), deep);
 			launch_resolve(s, m, lo, hi);
 		};
 		launch_window_extend(s, m, 0, W, false);
@@ -851,8 +870,8 @@ static size_t simplify_on_device(Context& ctx, const CallArgs& c, float* result_
 			throw std::runtime_error("priority rounds did not converge (internal error)");
 
 		// head rounds for the next window of this size: enough to bring the list under the size
-		// one tail workgroup takes in a single sweep
-		int next = heads + 3;
+		// one tail workgroup takes in a single sweep (twice as many when they were not)
+		int next = heads * 2;
 		for (int hd = 0; hd < heads && hd < kHeadSizeSlots; ++hd)
 			if (h.head_sizes[hd] < kTailList)
 			{
@@ -860,7 +879,8 @@ static size_t simplify_on_device(Context& ctx, const CallArgs& c, float* result_
 				break;
 			}
 		if (heads)
-			ctx.head_rounds[bucket] = std::max(2, std::min(kMaxHeadRounds, next));
+			g_head_rounds[bucket].store(std::max(2, std::min(kMaxHeadRounds, next)), std::memory_order_relaxed);
+		long_chains |= heads >= deep_heads && h.head_sizes[heads - 1] >= kTailList;
 		ctx.window_fraction = std::min(1.0, (double)(h.stop + 1) / (double)C);
 
 		meshopt_hip_pass_stats ps{};
@@ -2558,6 +2578,19 @@ extern "C" void meshopt_setAllocator(void* (*allocate)(size_t), void (*deallocat
 	meshopt_Allocator::Storage& s = meshopt_Allocator::storage();
 	s.allocate = allocate ? allocate : static_cast<void* (*)(size_t)>(::operator new);
 	s.deallocate = deallocate ? deallocate : static_cast<void (*)(void*)>(::operator delete);
+}
+
+extern "C" int meshopt_hip_set_round_policy(int deep_heads, int deep_from_head)
+{
+	if (deep_from_head < 0 || deep_from_head > kMaxHeadRounds)
+	{
+		t_last_error = "meshopt_hip_set_round_policy: deep_from_head out of range";
+		return -1;
+	}
+	g_deep_heads.store(deep_heads < -1 ? kDeepHeads : deep_heads);
+	g_deep_from_head.store(deep_from_head);
+	t_last_error.clear();
+	return 0;
 }
 
 extern "C" int meshopt_hip_last_call_stats(meshopt_hip_call_stats* call, meshopt_hip_pass_stats* passes, int max_passes)

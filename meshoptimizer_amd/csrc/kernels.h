@@ -40,6 +40,9 @@ struct Mesh
 	float* va;               // weighted attributes (V x A)
 
 	uint32_t* cremap;        // collapse_remap
+	uint32_t* src_off;       // V + 1: candidate ranks grouped by source class
+	uint32_t* src_rank;      // candidates
+	uint32_t* src_val;       // the value each listed candidate's commit writes at the class primary
 	uint32_t* lock_rank;     // rank of the commit that locked a class (collapse_locked)
 	uint64_t* pend_any[2];   // per-class round minima of the commit rounds
 	uint64_t* pend_src[2];   // the same over ranks whose source is the class
@@ -101,10 +104,12 @@ void launch_dual_count(hipStream_t s, Mesh& m, const uint32_t* raw_off);
 void launch_adjacency(hipStream_t s, Mesh& m, size_t tri_count, const uint32_t* live = nullptr);
 void launch_pick(hipStream_t s, Mesh& m, size_t tri_count, const uint32_t* live = nullptr);
 void launch_rank_sort(hipStream_t s, Mesh& m, size_t cand_count);
-void launch_perform_init(hipStream_t s, Mesh& m, size_t cand_count, size_t window);
+// commit state of a pass; with source_lists, also the candidate ranks grouped by source class
+// that deep rounds read
+void launch_perform_init(hipStream_t s, Mesh& m, size_t cand_count, bool source_lists);
 void launch_pick_capacity(hipStream_t s, Mesh& m, size_t index_count);
-void launch_round(hipStream_t s, Mesh& m, size_t list_bound, int head_index, int rel);
-void launch_round_tail(hipStream_t s, Mesh& m, int rel, uint32_t max_rounds);
+void launch_round(hipStream_t s, Mesh& m, size_t list_bound, int head_index, int rel, bool deep);
+void launch_round_tail(hipStream_t s, Mesh& m, int rel, uint32_t max_rounds, bool deep);
 void launch_window_extend(hipStream_t s, Mesh& m, size_t lo, size_t hi, bool if_no_stop);
 void launch_resolve(hipStream_t s, Mesh& m, size_t lo, size_t hi);
 void launch_stop_scan(hipStream_t s, Mesh& m, size_t window, size_t cand_count, uint32_t goal, float error_limit, bool conditional);

@@ -387,6 +387,9 @@ struct RoundState
 	const uint32_t* remap;
 	const uint32_t* loop;
 	const uint32_t* loopback;
+	const uint32_t* src_off;  // candidate ranks grouped by source class
+	const uint32_t* src_rank;
+	const uint32_t* src_val;  // what each such candidate's commit writes at the class primary
 	uint32_t* cremap;
 	uint32_t* lock_rank;      // rank of the commit that locked a class (kNone: unlocked)
 	uint8_t* status;
@@ -396,27 +399,130 @@ struct RoundState
 	CallScalars* sc;
 };
 
-// flip test (src/simplifier.cpp:1200-1232) of rank k, decided only where it cannot change any
-// more: 0 undetermined, 1 passes, 2 fails.  The test reads collapse_remap at the corners of the
-// triangles around k's source; a corner whose class published a lower source-role rank in the
-// previous round may still move before k decides (every pending rank publishes every round),
-// while every other corner keeps the value k sees now.  A triangle with no such corner that
-// flips fails the test for good; with no such corner anywhere, the test passes for good;
-// otherwise the rank waits.
+// the value rank r's commit writes into collapse_remap[p], p the primary of its source class
+// (src/simplifier.cpp:1573-1605)
+__device__ __forceinline__ uint32_t commit_value_at(const RoundState& a, uint32_t r, uint32_t p)
+{
+	uint32_t i0 = a.r_v0[r], i1 = a.r_v1[r];
+	uint8_t kd = a.kind[i0];
+	if (kd == kComplex)
+		return complex_target(p, i1, a.remap, a.loop, a.loopback);
+	if (kd == kSeam)
+	{
+		uint32_t s0 = a.wedge[i0];
+		uint32_t s1 = a.loop[i0] == i1 ? a.loopback[s0] : a.loop[s0];
+		s1 = (s1 != kNone) ? s1 : a.wedge[i1];
+		uint32_t v = p;
+		if (i0 == p)
+			v = i1;
+		if (s0 == p)
+			v = s1;
+		return v;
+	}
+	return i1;
+}
+
+// flip test (src/simplifier.cpp:1200-1232) of rank k over every way the lower candidates still in
+// play could move its neighbours: 0 undetermined, 1 passes in all, 2 fails in all.  The test
+// reads collapse_remap at the corners of the triangles around k's source; a corner whose class
+// published a lower source-role rank in the previous round may still move (every pending rank
+// publishes every round), every other corner keeps the value k sees now.  A moving corner's
+// alternatives are the value seen and the values its lower candidates' commits write, taken from
+// the per-source candidate lists; a candidate counts unless it is known never to commit (skipped,
+// flip-failed, rejected), so the set is never smaller than the true one, whatever stores of the
+// same round the test sees.  An outcome that holds in every combination is final; a triangle
+// without a moving corner decides alone.  Limited to kMaxAlternatives per corner and
+// kMaxPendingSlots moving corners; beyond that the rank waits.
 //
-// The test is a chain of dependent gathers (adjacency entry, then its corners' collapse_remap,
-// lock rank and pending minimum, then the positions they lead to), so the adjacency is read in
-// chunks of kFlipChunk entries whose gathers at each level are all issued before any is used:
-// a chunk costs three memory latencies rather than three per entry.
+// The test is a chain of dependent gathers, so the adjacency is read in chunks of kFlipChunk
+// entries, and a source list kSrcChunk ranks at a time, with the gathers of each level issued
+// before any is used.
 
 constexpr int kFlipChunk = 4;
+constexpr int kSrcChunk = 8;
+constexpr int kMaxAlternatives = 2; // per corner: its current value and one possible move
+constexpr int kMaxPendingSlots = 4; // moving corners before the rank waits
 
-// begin/end: rank k's source's adjacency range; v0, v1: the positions of its source and target
+// one adjacency entry with moving corners (pend2: bit h set when corner h moves): 0 mixed, 1 passes
+// in all combinations, 2 fails in all, 3 too many alternatives
+__device__ int flip_entry_pending(const RoundState& a, uint32_t k, const uint32_t nb[2], const uint32_t seen[2], uint32_t pend2, const V3& v0,
+                                  const V3& v1, uint32_t i1)
+{
+	uint32_t alt[2][kMaxAlternatives];
+	int na[2] = {1, 1};
+	alt[0][0] = seen[0];
+	alt[1][0] = seen[1];
+	uint32_t tb[2], te[2];
+#pragma unroll
+	for (int q = 0; q < 2; ++q)
+	{
+		const bool scan = (pend2 >> q) & 1u;
+		tb[q] = scan ? a.src_off[nb[q]] : 0;
+		te[q] = scan ? a.src_off[nb[q] + 1] : 0;
+	}
+	bool more = true;
+	for (uint32_t step = 0; more; step += kSrcChunk)
+	{
+		more = false;
+		uint32_t r[2][kSrcChunk], val[2][kSrcChunk];
+#pragma unroll
+		for (int q = 0; q < 2; ++q)
+#pragma unroll
+			for (int u = 0; u < kSrcChunk; ++u)
+			{
+				const uint32_t t = tb[q] + step + u;
+				const bool in = t < te[q];
+				r[q][u] = in ? a.src_rank[t] : kNone;
+				val[q][u] = in ? a.src_val[t] : 0;
+			}
+		uint8_t st[2][kSrcChunk];
+#pragma unroll
+		for (int q = 0; q < 2; ++q)
+#pragma unroll
+			for (int u = 0; u < kSrcChunk; ++u)
+				st[q][u] = r[q][u] < k ? a.status[r[q][u]] : (uint8_t)kSkipped;
+#pragma unroll
+		for (int q = 0; q < 2; ++q)
+		{
+#pragma unroll
+			for (int u = 0; u < kSrcChunk; ++u)
+			{
+				if (st[q][u] == kSkipped || st[q][u] == kRejected || st[q][u] == kFlipFail || val[q][u] == alt[q][0])
+					continue;
+				if (na[q] == kMaxAlternatives)
+					return 3;
+				alt[q][na[q]++] = val[q][u];
+			}
+			more |= tb[q] + step + kSrcChunk < te[q];
+		}
+	}
+	bool any_flip = false, any_ok = false;
+	for (int x = 0; x < na[0]; ++x)
+		for (int y = 0; y < na[1]; ++y)
+		{
+			if (nb[0] == nb[1] && x != y)
+				continue; // one vertex moves one way only
+			uint32_t va = alt[0][x], vb = alt[1][y];
+			if (va == i1 || vb == i1 || va == vb)
+				any_ok = true;
+			else if (triangle_flips(load_pos(a.pos, va), load_pos(a.pos, vb), v0, v1))
+				any_flip = true;
+			else
+				any_ok = true;
+		}
+	return any_flip ? (any_ok ? 0 : 2) : 1;
+}
+
+// begin/end: rank k's source's adjacency range; v0, v1: the positions of its source and target.
+// kDeep: examine the combinations of the moving corners' alternatives; otherwise a rank with a
+// moving corner waits (the cheap test of the first rounds of a window, when most corners move)
+template <bool kDeep>
 __device__ int flip_outcome(const RoundState& a, uint32_t i1, uint32_t k, uint32_t epoch, uint32_t begin, uint32_t end, const V3& v0, const V3& v1)
 {
 	const uint64_t* prev_src = a.pend_src[(epoch - 1) & 1];
 	const uint32_t prev_tag = ~(epoch - 1);
-	bool pending = false;
+	int pending = 0;
+	bool all_pass = true;
 	for (uint32_t p0 = begin; p0 < end; p0 += kFlipChunk)
 	{
 		const uint32_t n = end - p0 < (uint32_t)kFlipChunk ? end - p0 : (uint32_t)kFlipChunk;
@@ -439,33 +545,57 @@ __device__ int flip_outcome(const RoundState& a, uint32_t i1, uint32_t k, uint32
 		}
 		// collapse_remap as rank k sees it: a move committed by a higher rank (possible once
 		// flip tests are decided early) reads as "not moved"
+		uint32_t seen[2 * kFlipChunk];
+		uint32_t pend_mask = 0;
+#pragma unroll
+		for (int q = 0; q < 2 * kFlipChunk; ++q)
+		{
+			seen[q] = (cm[q] != nb[q] && lr[q] > k) ? nb[q] : cm[q];
+			const bool moving = (uint32_t)(q / 2) < n && (uint32_t)(ps[q] >> 32) == prev_tag && (uint32_t)ps[q] < k;
+			pend_mask |= moving ? 1u << q : 0u;
+		}
+		pending += __builtin_popcount(pend_mask);
+		if (kDeep && pending > kMaxPendingSlots)
+			return 0;
+		// entries without a moving corner have one scenario: the positions they need, all at once
 		V3 pa[kFlipChunk], pb[kFlipChunk];
 		uint32_t test_mask = 0;
 #pragma unroll
 		for (int j = 0; j < kFlipChunk; ++j)
 		{
-			bool moving = false;
-			uint32_t seen[2];
-#pragma unroll
-			for (int h = 0; h < 2; ++h)
-			{
-				const int q = 2 * j + h;
-				seen[h] = (cm[q] != nb[q] && lr[q] > k) ? nb[q] : cm[q];
-				moving |= (uint32_t)(ps[q] >> 32) == prev_tag && (uint32_t)ps[q] < k;
-			}
-			const bool live = (uint32_t)j < n;
-			pending |= live && moving;
-			const bool test = live && !moving && !(seen[0] == i1 || seen[1] == i1 || seen[0] == seen[1]);
+			const uint32_t va = seen[2 * j], vb = seen[2 * j + 1];
+			const bool single = (uint32_t)j < n && !((pend_mask >> (2 * j)) & 3u);
+			const bool test = single && !(va == i1 || vb == i1 || va == vb);
 			test_mask |= test ? 1u << j : 0u;
-			pa[j] = test ? load_pos(a.pos, seen[0]) : V3{0.f, 0.f, 0.f};
-			pb[j] = test ? load_pos(a.pos, seen[1]) : V3{0.f, 0.f, 0.f};
+			pa[j] = test ? load_pos(a.pos, va) : V3{0.f, 0.f, 0.f};
+			pb[j] = test ? load_pos(a.pos, vb) : V3{0.f, 0.f, 0.f};
 		}
 #pragma unroll
 		for (int j = 0; j < kFlipChunk; ++j)
 			if (((test_mask >> j) & 1u) && triangle_flips(pa[j], pb[j], v0, v1))
 				return 2;
+		if (!kDeep)
+		{
+			all_pass &= pending == 0;
+			continue;
+		}
+		// entries with a moving corner: every combination of the alternatives
+		for (int j = 0; j < kFlipChunk; ++j)
+		{
+			if ((uint32_t)j >= n || !((pend_mask >> (2 * j)) & 3u))
+				continue;
+			const uint32_t nbj[2] = {nb[2 * j], nb[2 * j + 1]};
+			const uint32_t seenj[2] = {seen[2 * j], seen[2 * j + 1]};
+			const int o = flip_entry_pending(a, k, nbj, seenj, (pend_mask >> (2 * j)) & 3u, v0, v1, i1);
+			if (o == 3)
+				return 0;
+			if (o == 2)
+				return 2;
+			if (o == 0)
+				all_pass = false;
+		}
 	}
-	return pending ? 0 : 1;
+	return all_pass ? 1 : 0;
 }
 
 // round `epoch`, one rank.  A rank at a locked endpoint is skipped.  A rank that was the lowest
@@ -482,6 +612,7 @@ __device__ int flip_outcome(const RoundState& a, uint32_t i1, uint32_t k, uint32
 // that commits in this round published its source then, so no test reads a move still in
 // flight.  A lock written in this round may be missed by a rank sharing that endpoint: that rank
 // publishes once more and is skipped in the next round, before it can commit.
+template <bool kDeep>
 __device__ __forceinline__ bool round_x(const RoundState& a, uint4 w, uint32_t epoch)
 {
 	const uint32_t k = w.x, r0 = w.y, r1 = w.z;
@@ -505,7 +636,7 @@ __device__ __forceinline__ bool round_x(const RoundState& a, uint4 w, uint32_t e
 	uint8_t now = st;
 	if (st == kUndecided && (low0 || low1))
 	{
-		int o = flip_outcome(a, r1, k, epoch, begin, end, v0, v1);
+		int o = flip_outcome<kDeep>(a, r1, k, epoch, begin, end, v0, v1);
 		if (o == 2)
 		{
 			a.status[k] = kFlipFail;
@@ -551,7 +682,9 @@ __device__ __forceinline__ void append_wave(bool keep, uint4 w, uint4* out_list,
 // head round.  Round r reads list (r & 1) / counter (r % 3), appends the survivors to list
 // ((r + 1) & 1) / counter ((r + 1) % 3) and clears counter ((r + 2) % 3).  One rank per thread
 // when the grid is large enough; the grid-stride loop covers any list length.
-__global__ void __launch_bounds__(kThreads) k_round_x(RoundState a, uint32_t head_index, uint32_t rel)
+// deep: this round examines the alternatives of moving corners (the window's per-source lists
+// are built)
+__global__ void __launch_bounds__(kThreads) k_round_x(RoundState a, uint32_t head_index, uint32_t rel, bool deep)
 {
 	CallScalars* sc = a.sc;
 	// rounds are numbered from a base on the device (the tail leaves it past its last round), so
@@ -574,7 +707,7 @@ __global__ void __launch_bounds__(kThreads) k_round_x(RoundState a, uint32_t hea
 	{
 		uint32_t idx = base + threadIdx.x;
 		uint4 w = idx < n ? in_list[idx] : make_uint4(0, 0, 0, 0);
-		bool keep = idx < n && round_x(a, w, epoch);
+		bool keep = idx < n && (deep ? round_x<true>(a, w, epoch) : round_x<false>(a, w, epoch));
 		append_wave(keep, w, out_list, out_count);
 	}
 }
@@ -584,7 +717,7 @@ constexpr int kTailThreads = 1024;
 // tail: one workgroup runs the remaining rounds back to back (phases separated by barriers)
 // until the list is empty.  The lowest undecided rank decides in every round, so a list of n
 // ranks empties within n + 1 rounds; max_rounds bounds the loop regardless.
-__global__ void __launch_bounds__(kTailThreads) k_round_tail(RoundState a, uint32_t rel, uint32_t max_rounds)
+__global__ void __launch_bounds__(kTailThreads) k_round_tail(RoundState a, uint32_t rel, uint32_t max_rounds, bool deep)
 {
 	__shared__ uint32_t s_n, s_out;
 	CallScalars* sc = a.sc;
@@ -609,7 +742,7 @@ __global__ void __launch_bounds__(kTailThreads) k_round_tail(RoundState a, uint3
 		{
 			uint32_t idx = base + threadIdx.x;
 			uint4 w = idx < n ? in_list[idx] : make_uint4(0, 0, 0, 0);
-			bool keep = idx < n && round_x(a, w, epoch);
+			bool keep = idx < n && (deep ? round_x<true>(a, w, epoch) : round_x<false>(a, w, epoch));
 			append_wave(keep, w, out_list, &s_out);
 		}
 		__syncthreads();
@@ -646,6 +779,27 @@ __global__ void k_perform_init_scalars(CallScalars* sc)
 	sc->extended = 0;
 }
 
+// candidate ranks grouped by source class (order inside a group is irrelevant): one atomic pass
+// counts the ranks per source class and hands each its slot in the class's group; after the scan
+// the fill places the ranks without atomics
+__global__ void k_src_count(const uint32_t* r_r0, size_t count, uint32_t* counts, uint32_t* slot)
+{
+	size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (k < count)
+		slot[k] = atomicAdd(&counts[r_r0[k]], 1u);
+}
+
+__global__ void k_src_fill(RoundState a, const uint32_t* r_r0, size_t count, const uint32_t* slot, uint32_t* src_rank, uint32_t* src_val)
+{
+	size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (k >= count)
+		return;
+	uint32_t r0 = r_r0[k];
+	uint32_t at = a.src_off[r0] + slot[k];
+	src_rank[at] = (uint32_t)k;
+	src_val[at] = commit_value_at(a, (uint32_t)k, r0);
+}
+
 static RoundState round_state(Mesh& m)
 {
 	RoundState a;
@@ -659,6 +813,9 @@ static RoundState round_state(Mesh& m)
 	a.remap = m.remap;
 	a.loop = m.loop[m.loop_cur];
 	a.loopback = m.loopback[m.loop_cur];
+	a.src_off = m.src_off;
+	a.src_rank = m.src_rank;
+	a.src_val = m.src_val;
 	a.cremap = m.cremap;
 	a.lock_rank = m.lock_rank;
 	a.status = m.status;
@@ -672,14 +829,21 @@ static RoundState round_state(Mesh& m)
 	return a;
 }
 
-void launch_perform_init(hipStream_t s, Mesh& m, size_t cand_count, size_t window)
+void launch_perform_init(hipStream_t s, Mesh& m, size_t cand_count, bool source_lists)
 {
-	(void)window;
 	size_t n = m.vertex_count;
-	(void)cand_count;
 	iota_u32(s, m.cremap, n);
 	fill_u32(s, m.lock_rank, kNone, n);
 	k_perform_init_scalars<<<1, 1, 0, s>>>(m.sc);
+	if (!source_lists)
+		return;
+	fill_u32(s, m.src_off, 0, n + 1);
+	// the rank slots go to m.keys, free once the ranked copies are gathered
+	if (cand_count)
+		k_src_count<<<grid_for(cand_count), kThreads, 0, s>>>(m.r_r0, cand_count, m.src_off, m.keys);
+	scan_exclusive(s, m.src_off, m.src_off, n, nullptr, m.src_off + n, m.temp);
+	if (cand_count)
+		k_src_fill<<<grid_for(cand_count), kThreads, 0, s>>>(round_state(m), m.r_r0, cand_count, m.keys, m.src_rank, m.src_val);
 }
 
 // ranks [lo, hi) become the list of the next round (sc->round).  With `if_no_stop` the window
@@ -710,7 +874,7 @@ void launch_window_extend(hipStream_t s, Mesh& m, size_t lo, size_t hi, bool if_
 }
 
 // list_estimate sizes the grid (one rank per thread); the kernels stay correct for any length
-void launch_round(hipStream_t s, Mesh& m, size_t list_estimate, int head_index, int rel)
+void launch_round(hipStream_t s, Mesh& m, size_t list_estimate, int head_index, int rel, bool deep)
 {
 	unsigned int blocks = grid_for(list_estimate);
 	blocks = blocks < 1 ? 1 : (blocks > 8192 ? 8192 : blocks);
@@ -719,14 +883,14 @@ void launch_round(hipStream_t s, Mesh& m, size_t list_estimate, int head_index, 
 	bool timed = t && t->count < t->capacity;
 	if (timed)
 		HIP_CHECK(hipEventRecord(t->begin[t->count], s));
-	k_round_x<<<blocks, kThreads, 0, s>>>(a, (uint32_t)head_index, (uint32_t)rel);
+	k_round_x<<<blocks, kThreads, 0, s>>>(a, (uint32_t)head_index, (uint32_t)rel, deep);
 	if (timed)
 		HIP_CHECK(hipEventRecord(t->end[t->count++], s));
 }
 
-void launch_round_tail(hipStream_t s, Mesh& m, int rel, uint32_t max_rounds)
+void launch_round_tail(hipStream_t s, Mesh& m, int rel, uint32_t max_rounds, bool deep)
 {
-	k_round_tail<<<1, kTailThreads, 0, s>>>(round_state(m), (uint32_t)rel, max_rounds);
+	k_round_tail<<<1, kTailThreads, 0, s>>>(round_state(m), (uint32_t)rel, max_rounds, deep);
 }
 
 // a failed flip test is a rejection unless a lower commit locked an endpoint first

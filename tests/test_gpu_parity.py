@@ -166,3 +166,32 @@ def test_concurrent_calls():
     for k, ((ib, v, t, e), (rg, eg)) in enumerate(zip(cases, got)):
         ro, eo = O.simplify(ib, v, t, e, 0)
         assert np.array_equal(rg, ro) and same_bits(eg, eo), f"case {k}: differs from the oracle"
+
+
+@pytest.mark.parametrize("policy", [(-1, 0), (-1, 2), (1 << 20, 0)], ids=["deep-all", "deep-from-2", "cheap-only"])
+def test_round_policies(policy):
+    """Every commit-round policy gives the same result: deep rounds everywhere (the moving corners'
+    alternatives examined from the first round and in every tail), deep from the third head round,
+    and cheap rounds only; on random meshes of every option set (several windows of rounds for the
+    larger ones), a seamed grid with attributes, and the full-size grid 256 workload (its first
+    pass needs more head rounds than the first guess)."""
+    rng = np.random.default_rng(77)
+    M.set_round_policy(*policy)
+    try:
+        for k in range(14):
+            if k % 2:
+                ib, v = random_soup(rng, nv=int(rng.integers(20, 2000)), ntri=int(rng.integers(10, 4000)))
+            else:
+                ib, v = random_grid(rng, n=int(rng.integers(8, 64)))
+            target = (ib.size // 3 * int(rng.integers(0, 4)) // 4) * 3
+            announce(f"round policy {policy} mesh {k} I={ib.size}")
+            assert_parity(ib, v, target, float(rng.choice([1e-2, 1e-1, 1.0])), OPTION_SETS[k % len(OPTION_SETS)])
+        m = W.seamed_grid(128, seams=6)
+        announce(f"round policy {policy} seamed128 with attributes")
+        assert_parity(m.indices, m.vertices, W.target_count(m.index_count, 1, 5), 1e-2, 0, m.vertices[:, 3:8], W.SEAMED_GRID_WEIGHTS)
+        g = W.grid(256)
+        announce(f"round policy {policy} grid256")
+        for _ in range(2):
+            assert_parity(g.indices, g.vertices, W.target_count(g.index_count, 1, 4), 1e-2, 0)
+    finally:
+        M.set_round_policy()
