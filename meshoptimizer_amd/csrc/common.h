@@ -21,6 +21,9 @@ constexpr int kMaxAttributes = 32;
 // record their list sizes (the window's heads and two for the extension)
 constexpr int kMaxHeadRounds = 256;
 constexpr int kHeadSizeSlots = kMaxHeadRounds + 2;
+// the commit rounds' work list is split into one segment per XCD (8 on the MI355X), by the range
+// of each rank's source class; workgroup i of a round works on segment i mod 8
+constexpr int kListSegments = 8;
 
 // vertex kinds (src/simplifier.cpp:296-305)
 enum : uint8_t
@@ -104,7 +107,9 @@ struct CallScalars
 	uint32_t skipped;
 	uint32_t rejected;
 	uint32_t tri_collapses;
-	uint32_t list_count[3]; // undecided work-list counters of the rounds (rotating)
+	uint32_t list_count[3][kListSegments]; // undecided work-list counters of the rounds (rotating), per segment
+	uint32_t list_off[kListSegments];      // where each segment starts in the list buffers
+	uint32_t window_count[kListSegments];  // ranks per segment of the window being opened (zero otherwise)
 	uint32_t dual_count;  // boundEdgeCollapses reduction
 	uint32_t unique_count; // sparse remap: number of referenced vertices
 	uint32_t tail_rounds;   // rounds run by the single-workgroup tail of the commit

@@ -481,8 +481,7 @@ static void carve(Arena& a, Plan& p, size_t V_in, size_t V, size_t I, size_t A, 
 	m.lock_rank = a.take<uint32_t>(Vp);
 	for (int k = 0; k < 2; ++k)
 	{
-		m.pend_any[k] = a.take<uint64_t>(Vp);
-		m.pend_src[k] = a.take<uint64_t>(Vp);
+		m.pend[k] = a.take<uint64_t>(2 * Vp);
 	}
 	m.adj_off = a.take<uint32_t>(Vp + 1);
 	m.adj = a.take<uint2>(I + 4);
@@ -756,8 +755,7 @@ static size_t simplify_on_device(Context& ctx, const CallArgs& c, float* result_
 	// round minima start as "no entry" (tag of epoch 0, rank all-ones)
 	for (int b = 0; b < 2; ++b)
 	{
-		HIP_CHECK(hipMemsetAsync(m.pend_any[b], 0xff, (V + 1) * sizeof(uint64_t), s));
-		HIP_CHECK(hipMemsetAsync(m.pend_src[b], 0xff, (V + 1) * sizeof(uint64_t), s));
+		HIP_CHECK(hipMemsetAsync(m.pend[b], 0xff, 2 * (V + 1) * sizeof(uint64_t), s));
 	}
 
 	// the first pass's adjacency and pick go before the set-up read-back, which returns their count

@@ -44,8 +44,8 @@ struct Mesh
 	uint32_t* src_rank;      // candidates
 	uint32_t* src_val;       // the value each listed candidate's commit writes at the class primary
 	uint32_t* lock_rank;     // rank of the commit that locked a class (collapse_locked)
-	uint64_t* pend_any[2];   // per-class round minima of the commit rounds
-	uint64_t* pend_src[2];   // the same over ranks whose source is the class
+	uint64_t* pend[2];       // per-class round minima of the commit rounds, two per class (16 B):
+	                         // over ranks whose source is the class, and whose target is
 	uint32_t epoch;
 
 	uint32_t* adj_off;       // V + 1: welded corner CSR offsets of the current pass

@@ -318,7 +318,9 @@ void launch_rank_sort(hipStream_t s, Mesh& m, size_t cand_count)
 // only come from a lower commit); one that was the minimum published at both endpoints in
 // round e-1 and is below every source-role entry of e-1 in its neighbourhood decides now
 // (flip-rejected or committed); the rest publish their rank for round e as 64-bit minima
-// tagged with ~e in the high word, into the buffer of parity e, and stay on the work list.
+// tagged with ~e in the high word, into the buffer of parity e, and stay on the work list.  A
+// class keeps one minimum per role (source, target), so a rank publishes with two atomics, and
+// the minimum over both roles is the smaller of the pair.
 // Ranks that decide in one round never touch each other's state.
 
 __device__ __forceinline__ uint64_t round_key(uint32_t epoch, uint32_t k)
@@ -393,11 +395,21 @@ struct RoundState
 	uint32_t* cremap;
 	uint32_t* lock_rank;      // rank of the commit that locked a class (kNone: unlocked)
 	uint8_t* status;
-	uint64_t* pend[2];        // per-class minimum rank published in a round (tagged with ~round)
-	uint64_t* pend_src[2];    // the same minimum over ranks whose source is the class
+	uint64_t* pend[2];        // per class, the minimum rank published in a round (tagged with ~round)
+	                          // over ranks whose source is the class (entry 2c) and over ranks
+	                          // whose target is the class (entry 2c + 1); the minimum over both
+	                          // roles is the smaller of the two
 	uint4* list[2];           // work-list entries: (rank, source class, target class, 0)
 	CallScalars* sc;
+	uint64_t seg_mul;         // list segment of class c: (c * seg_mul) >> 32, below kListSegments
 };
+
+// classes are numbered close to their neighbours in most meshes (grids, subdivided and
+// locality-optimised meshes), so a segment's ranks touch mostly one range of the class arrays
+__device__ __forceinline__ uint32_t list_segment(const RoundState& a, uint32_t c)
+{
+	return (uint32_t)(((uint64_t)c * a.seg_mul) >> 32);
+}
 
 // the value rank r's commit writes into collapse_remap[p], p the primary of its source class
 // (src/simplifier.cpp:1573-1605)
@@ -438,7 +450,9 @@ __device__ __forceinline__ uint32_t commit_value_at(const RoundState& a, uint32_
 // entries, and a source list kSrcChunk ranks at a time, with the gathers of each level issued
 // before any is used.
 
-constexpr int kFlipChunk = 4;
+constexpr int kFlipChunk = 4;      // the deep test's adjacency chunk
+constexpr int kCheapFlipChunk = 4; // the cheap test's
+constexpr int kCheapWaves = 5;     // waves per SIMD the cheap round kernel is built for
 constexpr int kSrcChunk = 8;
 constexpr int kMaxAlternatives = 2; // per corner: its current value and one possible move
 constexpr int kMaxPendingSlots = 4; // moving corners before the rank waits
@@ -516,39 +530,39 @@ __device__ int flip_entry_pending(const RoundState& a, uint32_t k, const uint32_
 // begin/end: rank k's source's adjacency range; v0, v1: the positions of its source and target.
 // kDeep: examine the combinations of the moving corners' alternatives; otherwise a rank with a
 // moving corner waits (the cheap test of the first rounds of a window, when most corners move)
-template <bool kDeep>
+template <bool kDeep, int kChunk = kDeep ? kFlipChunk : kCheapFlipChunk>
 __device__ int flip_outcome(const RoundState& a, uint32_t i1, uint32_t k, uint32_t epoch, uint32_t begin, uint32_t end, const V3& v0, const V3& v1)
 {
-	const uint64_t* prev_src = a.pend_src[(epoch - 1) & 1];
+	const uint64_t* prev_pend = a.pend[(epoch - 1) & 1];
 	const uint32_t prev_tag = ~(epoch - 1);
 	int pending = 0;
 	bool all_pass = true;
-	for (uint32_t p0 = begin; p0 < end; p0 += kFlipChunk)
+	for (uint32_t p0 = begin; p0 < end; p0 += kChunk)
 	{
-		const uint32_t n = end - p0 < (uint32_t)kFlipChunk ? end - p0 : (uint32_t)kFlipChunk;
-		uint32_t nb[2 * kFlipChunk];
+		const uint32_t n = end - p0 < (uint32_t)kChunk ? end - p0 : (uint32_t)kChunk;
+		uint32_t nb[2 * kChunk];
 #pragma unroll
-		for (int j = 0; j < kFlipChunk; ++j)
+		for (int j = 0; j < kChunk; ++j)
 		{
 			const uint2 e = (uint32_t)j < n ? a.adj[p0 + j] : make_uint2(i1, i1);
 			nb[2 * j] = e.x;
 			nb[2 * j + 1] = e.y;
 		}
-		uint32_t cm[2 * kFlipChunk], lr[2 * kFlipChunk];
-		uint64_t ps[2 * kFlipChunk];
+		uint32_t cm[2 * kChunk], lr[2 * kChunk];
+		uint64_t ps[2 * kChunk];
 #pragma unroll
-		for (int q = 0; q < 2 * kFlipChunk; ++q)
+		for (int q = 0; q < 2 * kChunk; ++q)
 		{
 			cm[q] = a.cremap[nb[q]];
 			lr[q] = a.lock_rank[nb[q]];
-			ps[q] = prev_src[nb[q]];
+			ps[q] = prev_pend[2 * nb[q]];
 		}
 		// collapse_remap as rank k sees it: a move committed by a higher rank (possible once
 		// flip tests are decided early) reads as "not moved"
-		uint32_t seen[2 * kFlipChunk];
+		uint32_t seen[2 * kChunk];
 		uint32_t pend_mask = 0;
 #pragma unroll
-		for (int q = 0; q < 2 * kFlipChunk; ++q)
+		for (int q = 0; q < 2 * kChunk; ++q)
 		{
 			seen[q] = (cm[q] != nb[q] && lr[q] > k) ? nb[q] : cm[q];
 			const bool moving = (uint32_t)(q / 2) < n && (uint32_t)(ps[q] >> 32) == prev_tag && (uint32_t)ps[q] < k;
@@ -558,10 +572,10 @@ __device__ int flip_outcome(const RoundState& a, uint32_t i1, uint32_t k, uint32
 		if (kDeep && pending > kMaxPendingSlots)
 			return 0;
 		// entries without a moving corner have one scenario: the positions they need, all at once
-		V3 pa[kFlipChunk], pb[kFlipChunk];
+		V3 pa[kChunk], pb[kChunk];
 		uint32_t test_mask = 0;
 #pragma unroll
-		for (int j = 0; j < kFlipChunk; ++j)
+		for (int j = 0; j < kChunk; ++j)
 		{
 			const uint32_t va = seen[2 * j], vb = seen[2 * j + 1];
 			const bool single = (uint32_t)j < n && !((pend_mask >> (2 * j)) & 3u);
@@ -571,7 +585,7 @@ __device__ int flip_outcome(const RoundState& a, uint32_t i1, uint32_t k, uint32
 			pb[j] = test ? load_pos(a.pos, vb) : V3{0.f, 0.f, 0.f};
 		}
 #pragma unroll
-		for (int j = 0; j < kFlipChunk; ++j)
+		for (int j = 0; j < kChunk; ++j)
 			if (((test_mask >> j) & 1u) && triangle_flips(pa[j], pb[j], v0, v1))
 				return 2;
 		if (!kDeep)
@@ -580,7 +594,7 @@ __device__ int flip_outcome(const RoundState& a, uint32_t i1, uint32_t k, uint32
 			continue;
 		}
 		// entries with a moving corner: every combination of the alternatives
-		for (int j = 0; j < kFlipChunk; ++j)
+		for (int j = 0; j < kChunk; ++j)
 		{
 			if ((uint32_t)j >= n || !((pend_mask >> (2 * j)) & 3u))
 				continue;
@@ -616,10 +630,11 @@ template <bool kDeep>
 __device__ __forceinline__ bool round_x(const RoundState& a, uint4 w, uint32_t epoch)
 {
 	const uint32_t k = w.x, r0 = w.y, r1 = w.z;
-	const uint64_t* prev = a.pend[(epoch - 1) & 1];
+	const ulonglong2* prev = reinterpret_cast<const ulonglong2*>(a.pend[(epoch - 1) & 1]);
 	const uint8_t st = a.status[k];
 	const uint32_t l0 = a.lock_rank[r0], l1 = a.lock_rank[r1];
-	const uint64_t p0 = prev[r0], p1 = prev[r1];
+	const ulonglong2 q0 = prev[r0], q1 = prev[r1];
+	const uint64_t p0 = q0.x < q0.y ? q0.x : q0.y, p1 = q1.x < q1.y ? q1.x : q1.y;
 	const uint32_t begin = a.adj_off[r0], end = a.adj_off[r0 + 1];
 	const V3 v0 = load_pos(a.pos, r0), v1 = load_pos(a.pos, r1);
 	if (st != kUndecided && st != kFlipPass)
@@ -655,11 +670,11 @@ __device__ __forceinline__ bool round_x(const RoundState& a, uint4 w, uint32_t e
 	}
 	if (now != st)
 		a.status[k] = now;
+	// two atomics: the source-role minimum at r0 and the target-role one at r1
 	uint64_t key = round_key(epoch, k);
 	unsigned long long* wr = (unsigned long long*)a.pend[epoch & 1];
-	atomicMin(&wr[r0], (unsigned long long)key);
-	atomicMin(&wr[r1], (unsigned long long)key);
-	atomicMin((unsigned long long*)&a.pend_src[epoch & 1][r0], (unsigned long long)key);
+	atomicMin(&wr[2 * (size_t)r0], (unsigned long long)key);
+	atomicMin(&wr[2 * (size_t)r1 + 1], (unsigned long long)key);
 	return true;
 }
 
@@ -679,35 +694,50 @@ __device__ __forceinline__ void append_wave(bool keep, uint4 w, uint4* out_list,
 	}
 }
 
-// head round.  Round r reads list (r & 1) / counter (r % 3), appends the survivors to list
-// ((r + 1) & 1) / counter ((r + 1) % 3) and clears counter ((r + 2) % 3).  One rank per thread
-// when the grid is large enough; the grid-stride loop covers any list length.
-// deep: this round examines the alternatives of moving corners (the window's per-source lists
-// are built)
-__global__ void __launch_bounds__(kThreads) k_round_x(RoundState a, uint32_t head_index, uint32_t rel, bool deep)
+// head round.  Round r reads list (r & 1) / counters (r % 3), appends the survivors to list
+// ((r + 1) & 1) / counters ((r + 1) % 3) and clears counters ((r + 2) % 3).  A rank stays in its
+// segment, at the segment's offset in both lists.  Workgroups are dealt to the XCDs in turn, so
+// workgroup i works on segment i mod kListSegments: each XCD's L2 then holds the class state of
+// one segment's ranks rather than of all of them.  The grid is a multiple of kListSegments; one
+// rank per thread when it is large enough, and the grid-stride loop covers any segment length.
+// kDeep: this round examines the alternatives of moving corners (the window's per-source lists
+// are built).  Two kernels rather than a flag, so that the cheap rounds are not held to the deep
+// test's register count.
+template <bool kDeep>
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kDeep ? 4 : kCheapWaves))) k_round_x(RoundState a, uint32_t head_index, uint32_t rel)
 {
 	CallScalars* sc = a.sc;
 	// rounds are numbered from a base on the device (the tail leaves it past its last round), so
 	// rounds can follow a tail or a device-side window extension without a read-back; `rel` counts
 	// the head rounds launched since the base was set
 	const uint32_t round = sc->round + rel, epoch = sc->epoch + 1 + rel;
-	const uint32_t n = sc->list_count[round % 3];
-	if (blockIdx.x == 0 && threadIdx.x == 0)
+	const uint32_t seg = blockIdx.x % kListSegments;
+	const uint32_t n = sc->list_count[round % 3][seg];
+	const uint32_t off = sc->list_off[seg];
+	if (blockIdx.x == 0)
 	{
-		sc->list_count[(round + 2) % 3] = 0;
-		if (head_index < kHeadSizeSlots)
-			sc->head_sizes[head_index] = n;
-		sc->x_visits += n;
-		sc->x_launches += 1;
+		if (threadIdx.x < kListSegments)
+			sc->list_count[(round + 2) % 3][threadIdx.x] = 0;
+		if (threadIdx.x == 0)
+		{
+			uint32_t total = 0;
+			for (int q = 0; q < kListSegments; ++q)
+				total += sc->list_count[round % 3][q];
+			if (head_index < kHeadSizeSlots)
+				sc->head_sizes[head_index] = total;
+			sc->x_visits += total;
+			sc->x_launches += 1;
+		}
 	}
-	const uint4* in_list = a.list[round & 1];
-	uint4* out_list = a.list[(round + 1) & 1];
-	uint32_t* out_count = &sc->list_count[(round + 1) % 3];
-	for (uint32_t base = blockIdx.x * kThreads; base < n; base += gridDim.x * kThreads)
+	const uint4* in_list = a.list[round & 1] + off;
+	uint4* out_list = a.list[(round + 1) & 1] + off;
+	uint32_t* out_count = &sc->list_count[(round + 1) % 3][seg];
+	const uint32_t stride = (gridDim.x / kListSegments) * kThreads;
+	for (uint32_t base = (blockIdx.x / kListSegments) * kThreads; base < n; base += stride)
 	{
 		uint32_t idx = base + threadIdx.x;
 		uint4 w = idx < n ? in_list[idx] : make_uint4(0, 0, 0, 0);
-		bool keep = idx < n && (deep ? round_x<true>(a, w, epoch) : round_x<false>(a, w, epoch));
+		bool keep = idx < n && round_x<kDeep>(a, w, epoch);
 		append_wave(keep, w, out_list, out_count);
 	}
 }
@@ -716,16 +746,27 @@ constexpr int kTailThreads = 1024;
 
 // tail: one workgroup runs the remaining rounds back to back (phases separated by barriers)
 // until the list is empty.  The lowest undecided rank decides in every round, so a list of n
-// ranks empties within n + 1 rounds; max_rounds bounds the loop regardless.
+// ranks empties within n + 1 rounds; max_rounds bounds the loop regardless.  The first round
+// reads the segments of the list; the survivors are appended as one list from offset 0 (segment
+// 0, the others empty), which the later rounds read.
 __global__ void __launch_bounds__(kTailThreads) k_round_tail(RoundState a, uint32_t rel, uint32_t max_rounds, bool deep)
 {
 	__shared__ uint32_t s_n, s_out;
+	__shared__ uint32_t s_pre[kListSegments + 1], s_off[kListSegments];
 	CallScalars* sc = a.sc;
 	const uint32_t round0 = sc->round + rel, epoch0 = sc->epoch + 1 + rel;
 	__syncthreads(); // every thread has read the base before thread 0 moves it below
 	if (threadIdx.x == 0)
 	{
-		s_n = sc->list_count[round0 % 3];
+		uint32_t total = 0;
+		for (int q = 0; q < kListSegments; ++q)
+		{
+			s_pre[q] = total;
+			s_off[q] = sc->list_off[q];
+			total += sc->list_count[round0 % 3][q];
+		}
+		s_pre[kListSegments] = total;
+		s_n = total;
 		s_out = 0;
 	}
 	__syncthreads();
@@ -741,7 +782,14 @@ __global__ void __launch_bounds__(kTailThreads) k_round_tail(RoundState a, uint3
 		for (uint32_t base = 0; base < n; base += kTailThreads)
 		{
 			uint32_t idx = base + threadIdx.x;
-			uint4 w = idx < n ? in_list[idx] : make_uint4(0, 0, 0, 0);
+			uint4 w = make_uint4(0, 0, 0, 0);
+			if (idx < n)
+			{
+				int q = 0;
+				while (idx >= s_pre[q + 1]) // stops below kListSegments: idx < s_pre[kListSegments] = n
+					++q;
+				w = in_list[s_off[q] + (idx - s_pre[q])];
+			}
 			bool keep = idx < n && (deep ? round_x<true>(a, w, epoch) : round_x<false>(a, w, epoch));
 			append_wave(keep, w, out_list, &s_out);
 		}
@@ -750,6 +798,11 @@ __global__ void __launch_bounds__(kTailThreads) k_round_tail(RoundState a, uint3
 		{
 			s_n = s_out;
 			s_out = 0;
+			s_pre[0] = 0;
+			for (int q = 1; q <= kListSegments; ++q)
+				s_pre[q] = s_n;
+			for (int q = 0; q < kListSegments; ++q)
+				s_off[q] = 0;
 		}
 		round++;
 		epoch++;
@@ -759,7 +812,14 @@ __global__ void __launch_bounds__(kTailThreads) k_round_tail(RoundState a, uint3
 
 	if (threadIdx.x == 0)
 	{
-		sc->list_count[round % 3] = s_n;
+		for (int q = 0; q < kListSegments; ++q)
+		{
+			sc->list_count[round % 3][q] = q == 0 ? s_n : 0u;
+			sc->list_count[(round + 1) % 3][q] = 0;
+			sc->list_count[(round + 2) % 3][q] = 0;
+			sc->list_off[q] = 0;
+			sc->window_count[q] = 0; // the next window open counts from zero
+		}
 		sc->tail_rounds += done;
 		sc->round = round;
 		sc->epoch = epoch - 1;
@@ -770,9 +830,14 @@ __global__ void __launch_bounds__(kTailThreads) k_round_tail(RoundState a, uint3
 
 __global__ void k_perform_init_scalars(CallScalars* sc)
 {
-	sc->list_count[0] = 0;
-	sc->list_count[1] = 0;
-	sc->list_count[2] = 0;
+	for (int q = 0; q < kListSegments; ++q)
+	{
+		sc->list_count[0][q] = 0;
+		sc->list_count[1][q] = 0;
+		sc->list_count[2][q] = 0;
+		sc->list_off[q] = 0;
+		sc->window_count[q] = 0;
+	}
 	sc->stop_reason = 0;
 	sc->tail_rounds = 0;
 	sc->round = 0;
@@ -821,11 +886,11 @@ static RoundState round_state(Mesh& m)
 	a.status = m.status;
 	for (int b = 0; b < 2; ++b)
 	{
-		a.pend[b] = m.pend_any[b];
-		a.pend_src[b] = m.pend_src[b];
+		a.pend[b] = m.pend[b];
 		a.list[b] = m.list[b];
 	}
 	a.sc = m.sc;
+	a.seg_mul = m.vertex_count ? ((uint64_t)kListSegments << 32) / m.vertex_count : 0;
 	return a;
 }
 
@@ -846,44 +911,99 @@ void launch_perform_init(hipStream_t s, Mesh& m, size_t cand_count, bool source_
 		k_src_fill<<<grid_for(cand_count), kThreads, 0, s>>>(round_state(m), m.r_r0, cand_count, m.keys, m.src_rank, m.src_val);
 }
 
-// ranks [lo, hi) become the list of the next round (sc->round).  With `if_no_stop` the window
-// opens only when the stop scan of [0, lo) found no break (sc->stop == lo) and the list goes on:
-// the rest of the list is then decided in the same pass without a read-back (sc->extended = 1);
-// otherwise the list stays empty and the rounds that follow do nothing.
-__global__ void k_window_open(RoundState a, uint32_t lo, uint32_t hi, const uint32_t* r_r0, const uint32_t* r_r1, bool if_no_stop)
+// ranks [lo, hi) become the list of the next round (sc->round), in segments by source class.
+// With `if_no_stop` the window opens only when the stop scan of [0, lo) found no break
+// (sc->stop == lo) and the list goes on: the rest of the list is then decided in the same pass
+// without a read-back (sc->extended = 1); otherwise the list stays empty and the rounds that
+// follow do nothing.  Two kernels: the count of each segment (into sc->window_count, zero since
+// the previous window's tail), then the placement, each segment at the sum of the counts before it.
+__device__ __forceinline__ bool window_opens(const CallScalars* sc, uint32_t lo, uint32_t hi, bool if_no_stop)
 {
+	return !if_no_stop || (sc->stop >= lo && hi > lo);
+}
+
+__global__ void __launch_bounds__(kThreads) k_window_count(RoundState a, uint32_t lo, uint32_t hi, const uint32_t* r_r0, bool if_no_stop)
+{
+	__shared__ uint32_t s_cnt[kListSegments];
 	CallScalars* sc = a.sc;
 	const uint32_t round = sc->round;
-	const bool open = !if_no_stop || (sc->stop >= lo && hi > lo);
-	uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-	if (i == 0)
+	const bool open = window_opens(sc, lo, hi, if_no_stop);
+	if (threadIdx.x < kListSegments)
 	{
-		sc->list_count[round % 3] = open ? hi - lo : 0;
-		sc->list_count[(round + 1) % 3] = 0;
-		if (if_no_stop)
-			sc->extended = open ? 1u : 0u;
+		s_cnt[threadIdx.x] = 0;
+		if (blockIdx.x == 0)
+		{
+			sc->list_count[round % 3][threadIdx.x] = 0; // filled by the placement
+			sc->list_count[(round + 1) % 3][threadIdx.x] = 0;
+		}
 	}
-	uint32_t k = lo + i;
+	__syncthreads();
+	const uint32_t k = lo + blockIdx.x * blockDim.x + threadIdx.x;
 	if (open && k < hi)
-		a.list[round & 1][i] = make_uint4(k, r_r0[k], r_r1[k], 0);
+		atomicAdd(&s_cnt[list_segment(a, r_r0[k])], 1u);
+	__syncthreads();
+	if (threadIdx.x < kListSegments && s_cnt[threadIdx.x])
+		atomicAdd(&sc->window_count[threadIdx.x], s_cnt[threadIdx.x]);
+}
+
+__global__ void __launch_bounds__(kThreads) k_window_open(RoundState a, uint32_t lo, uint32_t hi, const uint32_t* r_r0, const uint32_t* r_r1, bool if_no_stop)
+{
+	__shared__ uint32_t s_off[kListSegments], s_cnt[kListSegments], s_base[kListSegments];
+	CallScalars* sc = a.sc;
+	const uint32_t round = sc->round;
+	const bool open = window_opens(sc, lo, hi, if_no_stop);
+	if (threadIdx.x < kListSegments)
+	{
+		uint32_t off = 0;
+		for (uint32_t q = 0; q < threadIdx.x; ++q)
+			off += sc->window_count[q];
+		s_off[threadIdx.x] = off;
+		s_cnt[threadIdx.x] = 0;
+		if (blockIdx.x == 0)
+			sc->list_off[threadIdx.x] = off;
+	}
+	if (if_no_stop && blockIdx.x == 0 && threadIdx.x == 0)
+		sc->extended = open ? 1u : 0u;
+	__syncthreads();
+	const uint32_t k = lo + blockIdx.x * blockDim.x + threadIdx.x;
+	const bool in = open && k < hi;
+	const uint32_t r0 = in ? r_r0[k] : 0;
+	const uint32_t seg = list_segment(a, r0);
+	uint32_t local = 0;
+	if (in)
+		local = atomicAdd(&s_cnt[seg], 1u);
+	__syncthreads();
+	if (threadIdx.x < kListSegments && s_cnt[threadIdx.x])
+		s_base[threadIdx.x] = atomicAdd(&sc->list_count[round % 3][threadIdx.x], s_cnt[threadIdx.x]);
+	__syncthreads();
+	if (in)
+		a.list[round & 1][s_off[seg] + s_base[seg] + local] = make_uint4(k, r0, r_r1[k], 0);
 }
 
 void launch_window_extend(hipStream_t s, Mesh& m, size_t lo, size_t hi, bool if_no_stop)
 {
-	k_window_open<<<grid_for(hi > lo ? hi - lo : 1), kThreads, 0, s>>>(round_state(m), (uint32_t)lo, (uint32_t)hi, m.r_r0, m.r_r1, if_no_stop);
+	const unsigned int grid = grid_for(hi > lo ? hi - lo : 1);
+	const RoundState a = round_state(m);
+	k_window_count<<<grid, kThreads, 0, s>>>(a, (uint32_t)lo, (uint32_t)hi, m.r_r0, if_no_stop);
+	k_window_open<<<grid, kThreads, 0, s>>>(a, (uint32_t)lo, (uint32_t)hi, m.r_r0, m.r_r1, if_no_stop);
 }
 
 // list_estimate sizes the grid (one rank per thread); the kernels stay correct for any length
 void launch_round(hipStream_t s, Mesh& m, size_t list_estimate, int head_index, int rel, bool deep)
 {
+	// a multiple of kListSegments workgroups, at least one per segment
 	unsigned int blocks = grid_for(list_estimate);
-	blocks = blocks < 1 ? 1 : (blocks > 8192 ? 8192 : blocks);
+	blocks = (blocks + kListSegments - 1) / kListSegments * kListSegments;
+	blocks = blocks < kListSegments ? kListSegments : (blocks > 8192 ? 8192 : blocks);
 	RoundState a = round_state(m);
 	KernelTimer* t = m.timer;
 	bool timed = t && t->count < t->capacity;
 	if (timed)
 		HIP_CHECK(hipEventRecord(t->begin[t->count], s));
-	k_round_x<<<blocks, kThreads, 0, s>>>(a, (uint32_t)head_index, (uint32_t)rel, deep);
+	if (deep)
+		k_round_x<true><<<blocks, kThreads, 0, s>>>(a, (uint32_t)head_index, (uint32_t)rel);
+	else
+		k_round_x<false><<<blocks, kThreads, 0, s>>>(a, (uint32_t)head_index, (uint32_t)rel);
 	if (timed)
 		HIP_CHECK(hipEventRecord(t->end[t->count++], s));
 }
