@@ -589,6 +589,9 @@ static float host_float_bits(uint32_t u)
 constexpr size_t kWholeWindow = 131072;
 // list length the single-workgroup tail of the commit rounds takes in one sweep
 constexpr uint32_t kTailList = 1024;
+// list length at which the tail takes over from the head rounds: a tail round costs about 5 us
+// plus 22 ns per listed rank (one workgroup), a head round 5-8 us at any short length
+constexpr uint32_t kTailStart = 256;
 // head rounds of a window seen for the first time
 constexpr int kFirstHeads = 16;
 // a window whose learned head-round count exceeds g_deep_heads has long chains of dependent ranks
@@ -824,7 +827,7 @@ static size_t simplify_on_device(Context& ctx, const CallArgs& c, float* result_
 		while (bucket < 32 && (size_t(1) << bucket) < W)
 			bucket++;
 		const int learned = g_head_rounds[bucket].load(std::memory_order_relaxed);
-		const int heads = W <= kTailList ? 0 : (learned ? learned : kFirstHeads);
+		const int heads = W <= kTailStart ? 0 : (learned ? learned : kFirstHeads);
 		// the flip test of the first rounds waits for every moving corner (cheap, and enough for
 		// most meshes); a window known to need long chains of rounds examines the moving corners'
 		// alternatives from round kDeepFromHead on, which decides ranks far earlier on such meshes
@@ -867,17 +870,30 @@ static size_t simplify_on_device(Context& ctx, const CallArgs& c, float* result_
 		if (h.work[3])
 			throw std::runtime_error("priority rounds did not converge (internal error)");
 
-		// head rounds for the next window of this size: enough to bring the list under the size
-		// one tail workgroup takes in a single sweep (twice as many when they were not)
-		int next = heads * 2;
-		for (int hd = 0; hd < heads && hd < kHeadSizeSlots; ++hd)
-			if (h.head_sizes[hd] < kTailList)
-			{
-				next = hd;
-				break;
-			}
+		// head rounds for the next window of this size: enough to bring the list under kTailStart
+		// (when it was not reached, extrapolated from the last two head rounds' decay, at most
+		// twice as many), an overshoot given back a quarter at a time, so that windows of one size
+		// with different needs (passes of one mesh, meshes of a serving stream) settle near the
+		// larger need rather than alternate
 		if (heads)
+		{
+			int need = -1;
+			for (int hd = 0; hd < heads; ++hd)
+				if (h.head_sizes[hd] < kTailStart)
+				{
+					need = hd;
+					break;
+				}
+			if (need < 0)
+			{
+				need = heads * 2;
+				const double a = h.head_sizes[heads - 2], b = h.head_sizes[heads - 1];
+				if (b > 0 && b < a)
+					need = std::min(need, heads + 1 + (int)std::ceil(std::log((double)kTailStart / b) / std::log(b / a)));
+			}
+			const int next = std::max(need, heads - heads / 4);
 			g_head_rounds[bucket].store(std::max(2, std::min(kMaxHeadRounds, next)), std::memory_order_relaxed);
+		}
 		long_chains |= heads >= deep_heads && h.head_sizes[heads - 1] >= kTailList;
 		ctx.window_fraction = std::min(1.0, (double)(h.stop + 1) / (double)C);
 
