@@ -1678,20 +1678,37 @@ static BatchMemory context_batch_memory(Context& ctx, size_t P, size_t vertices,
 	return m;
 }
 
+// device-to-host copies a caller of the batched call wants along with the status read-back
+struct BatchReadback
+{
+	void* host;
+	const void* device;
+	size_t bytes;
+	bool valid; // set by the call: the copies hold the final results (no plain-path mesh ran after them)
+};
+
 // Batched call: sizes are validated and meshes sorted into LDS size classes on the device, the
 // classes run side by side (the first on the call's stream, the others on auxiliary streams
 // forked from it), and only the per-mesh status comes back to the host.  Meshes that fit no
 // class (and Sparse batches) take the plain device path afterwards; their offsets and targets
-// are read back only then (h_* may be null: device-resident offsets).
+// are read back only then (h_* may be null: device-resident offsets).  `readback` copies go with
+// each status read-back, so a caller that needs the result counts saves a round trip.
 static int simplify_batch_on_device(Context& ctx, const BatchMemory& mem, unsigned int* destination, unsigned int* result_counts,
                                     float* result_errors, const unsigned int* indices, const unsigned int* index_offsets, const float* positions,
                                     const unsigned int* vertex_offsets, const unsigned char* vertex_lock, size_t vertex_count, size_t stride,
                                     const unsigned int* target_index_counts,
                                     size_t patch_count, float target_error, unsigned int options, const unsigned int* h_index_offsets,
                                     const unsigned int* h_vertex_offsets, const unsigned int* h_targets,
-                                    const unsigned int* h_size_offsets = nullptr, const BatchAttributes* attrs = nullptr)
+                                    const unsigned int* h_size_offsets = nullptr, const BatchAttributes* attrs = nullptr,
+                                    BatchReadback* readback = nullptr, int readback_count = 0)
 {
 	hipStream_t s = ctx.stream;
+	auto enqueue_readback = [&]() {
+		for (int r = 0; r < readback_count; ++r)
+			HIP_CHECK(hipMemcpyAsync(readback[r].host, readback[r].device, readback[r].bytes, hipMemcpyDeviceToHost, s));
+	};
+	for (int r = 0; r < readback_count; ++r)
+		readback[r].valid = false;
 	if (stride < 12 || stride > 256 || stride % 4)
 		throw std::invalid_argument("meshopt_hip_simplify_batch: invalid vertex stride");
 	if (options & kOptInternalSolve)
@@ -1835,6 +1852,7 @@ static int simplify_batch_on_device(Context& ctx, const BatchMemory& mem, unsign
 			HIP_CHECK(hipEventRecord(ctx.batch_ev[1], s));
 		HIP_CHECK(hipGetLastError());
 		HIP_CHECK(hipMemcpyAsync(h_status, status, (P + kBatchClasses) * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+		enqueue_readback();
 		HIP_CHECK(hipStreamSynchronize(s));
 		auto add_time = [&](size_t launches, size_t meshes) {
 			float ms = 0;
@@ -1886,6 +1904,7 @@ static int simplify_batch_on_device(Context& ctx, const BatchMemory& mem, unsign
 				HIP_CHECK(hipEventRecord(ctx.batch_ev[1], s));
 			HIP_CHECK(hipGetLastError());
 			HIP_CHECK(hipMemcpyAsync(h_status, status, P * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+			enqueue_readback();
 			HIP_CHECK(hipStreamSynchronize(s));
 			if (timed)
 				add_time(0, 0);
@@ -1910,7 +1929,11 @@ static int simplify_batch_on_device(Context& ctx, const BatchMemory& mem, unsign
 		fallback |= h_status[p] == 1;
 	}
 	if (!fallback)
+	{
+		for (int r = 0; r < readback_count; ++r)
+			readback[r].valid = true;
 		return 0;
+	}
 	if (!mem.plain_fallback)
 	{
 		size_t first = 0;
@@ -2312,14 +2335,17 @@ static void clod_fallbacks(Context& ctx, const ClodBuffers& b, const meshopt_hip
                            const std::vector<uint32_t>& tg, unsigned int* result_counts, float* result_errors, float target_error,
                            unsigned int batch_options, bool permissive, uint32_t* d_list, uint32_t* sub_io, uint32_t* sub_vo, uint32_t* sub_tg,
                            uint32_t* sub_idx, float* sub_pos, uint8_t* sub_locks, uint32_t* sub_result, uint32_t* sub_counts, float* sub_errors,
-                           float* sub_attr, const float* attribute_weights)
+                           float* sub_attr, const float* attribute_weights, std::vector<uint32_t>& cnt, std::vector<uint32_t>& vo, bool have_counts)
 {
 	hipStream_t s = ctx.stream;
 	const size_t G = tg.size();
-	std::vector<uint32_t> cnt(G), vo(G + 1);
-	HIP_CHECK(hipMemcpyAsync(cnt.data(), result_counts, 4 * G, hipMemcpyDeviceToHost, s));
-	HIP_CHECK(hipMemcpyAsync(vo.data(), b.unique, 4 * (G + 1), hipMemcpyDeviceToHost, s));
-	HIP_CHECK(hipStreamSynchronize(s));
+	// the result counts and packed vertex offsets, unless they came back with the batch's status
+	if (!have_counts)
+	{
+		HIP_CHECK(hipMemcpyAsync(cnt.data(), result_counts, 4 * G, hipMemcpyDeviceToHost, s));
+		HIP_CHECK(hipMemcpyAsync(vo.data(), b.unique, 4 * (G + 1), hipMemcpyDeviceToHost, s));
+		HIP_CHECK(hipStreamSynchronize(s));
+	}
 	std::vector<uint32_t> fail;
 	for (size_t g = 0; g < G; ++g)
 		if (tg[g] <= io[g + 1] - io[g] && cnt[g] > tg[g])
@@ -2517,10 +2543,14 @@ extern "C" int meshopt_hip_clod_level_device(unsigned int* destination, unsigned
 		}
 		const unsigned int batch_options = options & ~(unsigned int)kOptSparse;
 		const BatchAttributes attrs{b.packed_attributes, A * sizeof(float), attribute_weights, A};
+		// the fallbacks need the result counts and the packed vertex offsets: they come back with
+		// the batch's status read-back (the kept groups' counts, rewritten below, are not used)
+		std::vector<uint32_t> h_cnt(fallbacks ? G : 0), h_vo(fallbacks ? G + 1 : 0);
+		BatchReadback rb[2] = {{h_cnt.data(), result_counts, 4 * G, false}, {h_vo.data(), b.unique, 4 * (G + 1), false}};
 		simplify_batch_on_device(ctx, context_batch_memory(ctx, G, I, !(options & kOptPrune), A), b.packed_result, result_counts, result_errors,
 		                         b.packed_indices, index_offsets, b.packed_positions, b.unique,
 		                         b.packed_locks, I, 12, batch_targets, G, target_error, batch_options, nullptr, nullptr,
-		                         nullptr, io.data(), A ? &attrs : nullptr);
+		                         nullptr, io.data(), A ? &attrs : nullptr, fallbacks ? rb : nullptr, fallbacks ? 2 : 0);
 		uint32_t* d_list = reinterpret_cast<uint32_t*>(base + o_list);
 		if (!keep.empty())
 		{
@@ -2533,7 +2563,8 @@ extern "C" int meshopt_hip_clod_level_device(unsigned int* destination, unsigned
 			               reinterpret_cast<uint32_t*>(base + o_stg), reinterpret_cast<uint32_t*>(base + o_sidx),
 			               reinterpret_cast<float*>(base + o_spos), reinterpret_cast<uint8_t*>(base + o_slock),
 			               reinterpret_cast<uint32_t*>(base + o_sres), reinterpret_cast<uint32_t*>(base + o_scnt),
-			               reinterpret_cast<float*>(base + o_serr), A ? reinterpret_cast<float*>(base + o_sattr) : nullptr, attribute_weights);
+			               reinterpret_cast<float*>(base + o_serr), A ? reinterpret_cast<float*>(base + o_sattr) : nullptr, attribute_weights, h_cnt, h_vo,
+			               rb[0].valid && rb[1].valid);
 		if (config && config->error_edge_limit > 0 && result_errors)
 			launch_clod_edge_limit(s, b, target_index_counts, config->error_edge_limit, result_errors);
 		launch_clod_unpack(s, b, result_counts, destination);
