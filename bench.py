@@ -540,7 +540,7 @@ def bench_update(args):
         "unit": "Mtri/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(mean * 1e3, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded generator)",
         "config": {"workload": "meshopt_simplifyWithUpdate, " + desc, "input_triangles": m.triangle_count,
-                   "output_triangles": n_gpu // 3, "solved_vertices": call["solve_vertices"],
+                   "output_triangles": n_gpu // 3, "vertices_left_to_rounds": call["solve_vertices"],
                    "solve_rounds_launched": call["solve_rounds"]},
         "p50_ms": round(statistics.median(times) * 1e3, 4),
         "p99_ms": round(float(np.percentile(np.array(times) * 1e3, 99)), 4),

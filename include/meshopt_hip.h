@@ -139,7 +139,8 @@ typedef struct meshopt_hip_call_stats
 	unsigned int kinds[5];         /* primaries per kind: manifold, border, seam, complex, locked */
 	float result_error_sq;
 	float vertex_error_sq;
-	unsigned int solve_vertices; /* meshopt_simplifyWithUpdate: primaries the position solve visited */
+	unsigned int solve_vertices; /* meshopt_simplifyWithUpdate: primaries left to the ordered rounds (the
+	                                outcomes not decided from their neighbours' possible positions) */
 	unsigned int solve_rounds;   /* rounds of the ordered position solve */
 	unsigned int read_backs;     /* host synchronisations with the device during the call */
 	unsigned int device_extensions; /* passes whose rank window the device extended to the whole list */

@@ -532,6 +532,8 @@ static void carve(Arena& a, Plan& p, size_t V_in, size_t V, size_t I, size_t A, 
 	p.solve.waiting = solve ? a.take<uint32_t>(Vp) : nullptr;
 	p.solve.queue[0] = solve ? a.take<uint32_t>(Vp) : nullptr;
 	p.solve.queue[1] = solve ? a.take<uint32_t>(Vp) : nullptr;
+	p.solve.cand = solve ? a.take<float4>(Vp) : nullptr;
+	p.solve.state = solve ? a.take<uint8_t>(Vp + 8) : nullptr;
 
 	size_t biggest = std::max(std::max(I, cap), Vp + 1);
 	size_t temp_words = std::max(sort_temp_words(biggest), scan_temp_words(biggest + 16));

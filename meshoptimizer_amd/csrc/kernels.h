@@ -127,6 +127,8 @@ struct SolveBuffers
 	uint32_t* listed;  // V: 1 for the primaries the position solve visits
 	uint32_t* waiting; // V: listed lower-index neighbour entries not solved yet
 	uint32_t* queue[2]; // V each: primaries ready in the current / next round
+	float4* cand;      // V: a listed primary's solved position (w unused)
+	uint8_t* state;    // V: its outcome as far as decided (open, keep, move)
 };
 // the caller's arrays written back (finalizeVertices); strides in floats
 struct SolveOutput

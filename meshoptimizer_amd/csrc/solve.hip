@@ -12,6 +12,17 @@
 // index-increasing path through listed neighbours (tens for subdivided meshes, hundreds for
 // row-major grids); work = O(V + E) over all rounds.
 //
+// Most outcomes are decided before the rounds.  A primary's solved position and its error test
+// depend on the vertex alone; only the two neighbourhood tests (radius and triangle flips) read
+// the neighbours, and both only reject.  A lower-index neighbour whose own outcome is still open
+// may hold its current or its solved position when the walk reaches the vertex, so the tests are
+// evaluated over both: an outcome that is the same in every combination is the walk's, and the
+// vertex is decided (keep or move) without waiting.  Two such passes (the second sees the first's
+// decisions) leave only the primaries whose outcome hinges on an open neighbour, and the rounds
+// replay the walk for them alone, reading a decided lower-index neighbour's final position.  A
+// decided move is written to the positions only after the rounds, since every undecided primary
+// below it must still see its current position.
+//
 // solveAttributes (1765-1809) and finalizeVertices (623-663) are independent per vertex.
 #include <algorithm>
 
@@ -24,6 +35,16 @@ namespace mhip
 
 constexpr uint32_t kPending = 0xffffffffu;
 static constexpr int kThreads = 256;
+
+// a listed primary's outcome as far as decided before the rounds
+enum : uint8_t
+{
+	kSpecOpen = 0, // hinges on a lower-index neighbour still open: solved in the rounds
+	kSpecKeep = 1, // keeps its position (not listed, no solution, or rejected in every case)
+	kSpecMove = 2, // moves to its solved position in every case
+};
+
+static void launch_solve_decide(hipStream_t s, Mesh& m, const SolveBuffers& b);
 
 static inline unsigned int grid_for(size_t n)
 {
@@ -146,26 +167,27 @@ __device__ __forceinline__ void queue_append(bool take, uint32_t item, uint32_t*
 		queue[base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull))] = item;
 }
 
-// listed lower-index neighbour entries of every listed primary; the primaries with none are the
-// first round's queue; sc->work[4] = listed primaries.  Both counters take one atomic per
-// workgroup (same-address atomics serialise in L2).
-__global__ void __launch_bounds__(kThreads) k_solve_waiting(const uint32_t* listed, const uint32_t* adj_off, const uint2* adj, size_t n, uint32_t* waiting,
+// open lower-index neighbour entries of every open primary (the decided ones need no round);
+// the primaries with none are the first round's queue; sc->work[4] = open primaries.  Both
+// counters take one atomic per workgroup (same-address atomics serialise in L2).
+__global__ void __launch_bounds__(kThreads) k_solve_waiting(const uint8_t* state, const uint32_t* adj_off, const uint2* adj, size_t n, uint32_t* waiting,
                                                              uint32_t* queue0, uint32_t* counts, CallScalars* sc)
 {
 	__shared__ uint32_t s_take[kThreads / 64], s_listed[kThreads / 64], s_base;
 	size_t ii = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
 	uint32_t i = (uint32_t)ii;
-	bool mine = ii < n && listed[i];
+	bool mine = ii < n && state[i] == kSpecOpen;
 	uint32_t w = 0;
 	if (mine)
 	{
 		for (uint32_t e = adj_off[i], end = adj_off[i + 1]; e < end; ++e)
 		{
 			uint2 ed = adj[e];
-			w += (ed.x < i && listed[ed.x]) + (ed.y < i && listed[ed.y]);
+			w += (ed.x < i && state[ed.x] == kSpecOpen) + (ed.y < i && state[ed.y] == kSpecOpen);
 		}
 	}
-	// unlisted vertices get a count no release can bring to zero (at most 2 per corner entry)
+	// decided and unlisted vertices get a count no release can bring to zero (at most 2 per
+	// corner entry)
 	if (ii < n)
 		waiting[i] = mine ? w : 0x80000000u;
 
@@ -218,7 +240,8 @@ void launch_solve_setup(hipStream_t s, Mesh& m, const SolveBuffers& b, size_t re
 	// updateEdgeAdjacency over the result (src/simplifier.cpp:2590-2591)
 	launch_adjacency(s, m, result_count / 3);
 	k_solve_flags<<<grid_for(n), kThreads, 0, s>>>(b.update, m.remap, m.kind, n, b.listed);
-	k_solve_waiting<<<grid_for(n), kThreads, 0, s>>>(b.listed, m.adj_off, m.adj, n, b.waiting, b.queue[0], &m.sc->work[5], m.sc);
+	launch_solve_decide(s, m, b);
+	k_solve_waiting<<<grid_for(n), kThreads, 0, s>>>(b.state, m.adj_off, m.adj, n, b.waiting, b.queue[0], &m.sc->work[5], m.sc);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -238,7 +261,191 @@ struct SolveRound
 	const uint32_t* wedge;
 	const uint32_t* adj_off;
 	const uint2* adj;
+	float4* cand;
+	uint8_t* state;
+	const uint32_t* listed;
 };
+
+// the solved position of listed primary i and the error test (src/simplifier.cpp:1697-1729 and
+// 1750-1755): false when there is no solution or it raises the positional error too much
+__device__ __forceinline__ bool solve_candidate(const SolveRound& a, uint32_t i, const V3& cur, V3& p)
+{
+	// solvePositions' system (src/simplifier.cpp:1705-1729): a point quadric regularises the
+	// solution; with attributes the optimum minimises every wedge's attribute quadric at once,
+	// under the volume gradient as a constraint
+	const Quad Qv = load_quad(a.vq, i);
+	Quad Q = Qv;
+	Quad R;
+	quad_from_point(R, cur.x, cur.y, cur.z, Qv.w * 1e-4f);
+	quad_add(Q, R);
+	Grad GV = {0.f, 0.f, 0.f, 0.f};
+	if (a.attr_count)
+	{
+		uint32_t v = i;
+		do
+		{
+			quad_reduce_attributes(Q, load_quad(a.aq, v), a.ag + (size_t)v * a.attr_count, a.attr_count);
+			v = a.wedge[v];
+		} while (v != i);
+		if (a.vg)
+			GV = a.vg[i];
+	}
+	if (!quad_solve(p, Q, GV))
+		return false;
+	return !(quad_error(Qv, p) > quad_error(Qv, cur) * 1.5f + 1e-6f);
+}
+
+// every vertex: its solved position when listed, kSpecOpen when that is a candidate move,
+// kSpecKeep otherwise (unlisted vertices never move)
+__global__ void k_solve_candidates(SolveRound a, size_t n)
+{
+	size_t ii = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (ii >= n)
+		return;
+	const uint32_t i = (uint32_t)ii;
+	uint8_t st = kSpecKeep;
+	if (a.listed[i])
+	{
+		const V3 cur = load_pos(a.pos, i);
+		V3 p = cur;
+		if (solve_candidate(a, i, cur, p))
+			st = kSpecOpen;
+		a.cand[i] = make_float4(p.x, p.y, p.z, 0.f);
+	}
+	a.state[i] = st;
+}
+
+// the positions corner u may hold when the walk reaches primary i: its current one, or a
+// lower-index primary's decided one, or both for a lower-index primary still open
+__device__ __forceinline__ int corner_options(const SolveRound& a, uint32_t u, uint32_t i, V3 opt[2])
+{
+	const uint8_t st = u < i ? a.state[u] : kSpecKeep;
+	if (st == kSpecMove)
+	{
+		opt[0] = load_pos(a.cand, u);
+		return 1;
+	}
+	opt[0] = load_pos(a.pos, u);
+	if (st == kSpecOpen)
+	{
+		opt[1] = load_pos(a.cand, u);
+		return 2;
+	}
+	return 1;
+}
+
+// the position the walk shows at primary i for corner u once every open lower-index primary is
+// done: a lower-index primary decided to move at its solved position, any other at its current
+__device__ __forceinline__ V3 walk_pos(const SolveRound& a, uint32_t u, uint32_t i)
+{
+	return load_pos(u < i && a.state[u] == kSpecMove ? a.cand : a.pos, u);
+}
+
+// getNeighborhoodRadius's term: a NaN distance never enters its running maximum, which starts
+// at 0, so it counts as 0
+__device__ __forceinline__ float radius_term(const V3& q, const V3& cur)
+{
+	const float d = (q.x - cur.x) * (q.x - cur.x) + (q.y - cur.y) * (q.y - cur.y) + (q.z - cur.z) * (q.z - cur.z);
+	return d == d ? d : 0.f;
+}
+
+// one pass over the open primaries (one thread each): the radius test over the smallest and the
+// largest radius the combinations allow, and the flip test of every corner entry over its
+// combinations (src/simplifier.cpp:1731-1748).  A neighbour decided in this same pass may be read
+// as open or as decided: either way its true position is among the ones examined.
+__global__ void k_solve_speculate(SolveRound a, size_t n)
+{
+	size_t ii = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (ii >= n)
+		return;
+	const uint32_t i = (uint32_t)ii;
+	if (a.state[i] != kSpecOpen)
+		return;
+	const V3 cur = load_pos(a.pos, i), p = load_pos(a.cand, i);
+	const float dp = (p.x - cur.x) * (p.x - cur.x) + (p.y - cur.y) * (p.y - cur.y) + (p.z - cur.z) * (p.z - cur.z);
+	float lo2 = 0.f, hi2 = 0.f;
+	bool flips_always = false, flips_maybe = false;
+	for (uint32_t e = a.adj_off[i], end = a.adj_off[i + 1]; e < end; ++e)
+	{
+		const uint2 ed = a.adj[e];
+		V3 qa[2], qb[2];
+		const int na = corner_options(a, ed.x, i, qa), nb = corner_options(a, ed.y, i, qb);
+		float ra_lo = radius_term(qa[0], cur), ra_hi = ra_lo, rb_lo = radius_term(qb[0], cur), rb_hi = rb_lo;
+		if (na == 2)
+		{
+			const float r = radius_term(qa[1], cur);
+			ra_lo = r < ra_lo ? r : ra_lo;
+			ra_hi = r > ra_hi ? r : ra_hi;
+		}
+		if (nb == 2)
+		{
+			const float r = radius_term(qb[1], cur);
+			rb_lo = r < rb_lo ? r : rb_lo;
+			rb_hi = r > rb_hi ? r : rb_hi;
+		}
+		lo2 = lo2 < ra_lo ? ra_lo : lo2;
+		lo2 = lo2 < rb_lo ? rb_lo : lo2;
+		hi2 = hi2 < ra_hi ? ra_hi : hi2;
+		hi2 = hi2 < rb_hi ? rb_hi : hi2;
+		bool all = true, any = false;
+		for (int x = 0; x < na; ++x)
+			for (int y = 0; y < nb; ++y)
+			{
+				const bool f = triangle_flips(qa[x], qb[y], cur, p);
+				all &= f;
+				any |= f;
+			}
+		flips_always |= all;
+		flips_maybe |= any;
+	}
+	// the radius test passes when !(dp > nr * nr), nr = sqrtf(radius), monotone in the radius
+	const float nlo = sqrtf(lo2), nhi = sqrtf(hi2);
+	if (dp > nhi * nhi || flips_always)
+		a.state[i] = kSpecKeep;
+	else if (!(dp > nlo * nlo) && !flips_maybe)
+		a.state[i] = kSpecMove;
+}
+
+// the decided moves, once the rounds are done
+__global__ void k_solve_apply(const uint8_t* state, const float4* cand, size_t n, float4* pos)
+{
+	size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n && state[i] == kSpecMove)
+		pos[i] = cand[i];
+}
+
+static SolveRound solve_round_args(Mesh& m, const SolveBuffers& b)
+{
+	SolveRound a;
+	a.waiting = b.waiting;
+	a.queue[0] = b.queue[0];
+	a.queue[1] = b.queue[1];
+	a.counts = &m.sc->work[5];
+	a.pos = m.pos;
+	a.vq = m.vq;
+	a.vg = m.vg;
+	a.aq = m.aq;
+	a.ag = m.ag;
+	a.attr_count = (int)m.attr_count;
+	a.wedge = m.wedge;
+	a.adj_off = m.adj_off;
+	a.adj = m.adj;
+	a.cand = b.cand;
+	a.state = b.state;
+	a.listed = b.listed;
+	return a;
+}
+
+static constexpr int kSpeculationPasses = 2;
+
+static void launch_solve_decide(hipStream_t s, Mesh& m, const SolveBuffers& b)
+{
+	const size_t n = m.vertex_count;
+	const SolveRound a = solve_round_args(m, b);
+	k_solve_candidates<<<grid_for(n), kThreads, 0, s>>>(a, n);
+	for (int k = 0; k < kSpeculationPasses; ++k)
+		k_solve_speculate<<<grid_for(n), kThreads, 0, s>>>(a, n);
+}
 
 // neighbourhood checks of a solved position over `n` corner entries (src/simplifier.cpp:1734-1748):
 // getNeighborhoodRadius (1252-1276) and hasTriangleFlips for a point move (1234-1250)
@@ -251,18 +458,20 @@ __device__ __forceinline__ void sweep_entry(const V3& q0, const V3& q1, const V3
 	flips |= triangle_flips(q0, q1, cur, p);
 }
 
-// round r: solve the queued primaries, one wavefront per primary, and release their higher-index
-// neighbours into round r + 1's queue.  Queue lengths rotate over three words: round r reads
-// counts[r % 3], appends to counts[(r + 1) % 3] and clears counts[(r + 2) % 3] (round r - 1's,
-// read by nobody after it).
+// round r: decide the queued (open) primaries, one wavefront per primary, and release their
+// higher-index open neighbours into round r + 1's queue.  Queue lengths rotate over three words:
+// round r reads counts[r % 3], appends to counts[(r + 1) % 3] and clears counts[(r + 2) % 3]
+// (round r - 1's, read by nobody after it).
 //
-// Per primary, every lane computes the same quadric solve (its loads are broadcasts) while the
-// lanes share the corner entries.  The release does not depend on the solve's outcome (the
-// neighbour reads the new position only in a later launch), so each lane issues the atomics of
-// its first entry before the solve and uses their results after it: the latencies overlap.
-// Both neighbourhood checks only reject, so evaluating both changes no outcome; the radius is a
-// maximum of non-negative values that never takes a NaN (a maximum of bit patterns, order-free)
-// and the flip test an any().
+// A queued primary's solved position passed the error test (k_solve_candidates); the lanes share
+// its corner entries.  A corner is read where the walk would show it: a lower-index primary
+// decided to move at its solved position (written to the positions only after the rounds), every
+// other one at the current position (an open lower-index primary was moved in an earlier round).
+// The release does not depend on the outcome (the neighbour reads the new position only in a
+// later launch), so each lane issues the atomics of its first entry before the tests and uses
+// their results after them.  Both neighbourhood checks only reject, so evaluating both changes
+// no outcome; the radius is a maximum of non-negative values that never takes a NaN (a maximum
+// of bit patterns, order-free) and the flip test an any().
 __global__ void __launch_bounds__(kThreads) k_solve_round(SolveRound a, uint32_t round)
 {
 	const uint32_t next = (round + 1) % 3;
@@ -278,41 +487,18 @@ __global__ void __launch_bounds__(kThreads) k_solve_round(SolveRound a, uint32_t
 	{
 		const uint32_t i = queue[k];
 		const uint32_t begin = a.adj_off[i], end = a.adj_off[i + 1];
-		const V3 cur = load_pos(a.pos, i);
-		const Quad Qv = load_quad(a.vq, i);
+		const V3 cur = load_pos(a.pos, i), p = load_pos(a.cand, i);
 
-		// this lane's first corner entry: release its higher-index ends now (unlisted vertices
-		// hold a count that never reaches zero), load the positions the sweep needs
+		// this lane's first corner entry: release its higher-index ends now (decided and unlisted
+		// vertices hold a count that never reaches zero), load the positions the sweep needs
 		const uint32_t e0 = begin + lane;
 		const bool has0 = e0 < end;
 		const uint2 ed0 = has0 ? a.adj[e0] : make_uint2(i, i);
 		const uint32_t old0 = ed0.x > i ? atomicSub(&a.waiting[ed0.x], 1u) : 0u;
 		const uint32_t old1 = ed0.y > i ? atomicSub(&a.waiting[ed0.y], 1u) : 0u;
-		const V3 q0 = load_pos(a.pos, ed0.x), q1 = load_pos(a.pos, ed0.y);
+		const V3 q0 = walk_pos(a, ed0.x, i), q1 = walk_pos(a, ed0.y, i);
 
-		// solvePositions' system (src/simplifier.cpp:1705-1729): a point quadric regularises the
-		// solution; with attributes the optimum minimises every wedge's attribute quadric at once,
-		// under the volume gradient as a constraint
-		Quad Q = Qv;
-		Quad R;
-		quad_from_point(R, cur.x, cur.y, cur.z, Qv.w * 1e-4f);
-		quad_add(Q, R);
-		Grad GV = {0.f, 0.f, 0.f, 0.f};
-		if (a.attr_count)
-		{
-			uint32_t v = i;
-			do
-			{
-				quad_reduce_attributes(Q, load_quad(a.aq, v), a.ag + (size_t)v * a.attr_count, a.attr_count);
-				v = a.wedge[v];
-			} while (v != i);
-			if (a.vg)
-				GV = a.vg[i];
-		}
-
-		V3 p;
-		bool moved = quad_solve(p, Q, GV);
-		if (moved)
+		bool moved = true;
 		{
 			float far2 = 0.f;
 			bool flips = false;
@@ -321,7 +507,7 @@ __global__ void __launch_bounds__(kThreads) k_solve_round(SolveRound a, uint32_t
 			for (uint32_t e = e0 + 64; e < end; e += 64)
 			{
 				uint2 ed = a.adj[e];
-				sweep_entry(load_pos(a.pos, ed.x), load_pos(a.pos, ed.y), cur, p, far2, flips);
+				sweep_entry(walk_pos(a, ed.x, i), walk_pos(a, ed.y, i), cur, p, far2, flips);
 			}
 			uint32_t far_bits = __float_as_uint(far2);
 			for (int o = 32; o > 0; o >>= 1)
@@ -331,7 +517,7 @@ __global__ void __launch_bounds__(kThreads) k_solve_round(SolveRound a, uint32_t
 			}
 			float nr = sqrtf(__uint_as_float(far_bits));
 			float dp = (p.x - cur.x) * (p.x - cur.x) + (p.y - cur.y) * (p.y - cur.y) + (p.z - cur.z) * (p.z - cur.z);
-			moved = !(dp > nr * nr) && !__ballot(flips) && !(quad_error(Qv, p) > quad_error(Qv, cur) * 1.5f + 1e-6f);
+			moved = !(dp > nr * nr) && !__ballot(flips);
 		}
 		if (moved && lane == 0)
 			a.pos[i] = make_float4(p.x, p.y, p.z, 0.f);
@@ -362,20 +548,7 @@ void launch_solve_rounds(hipStream_t s, Mesh& m, const SolveBuffers& b, uint32_t
 {
 	if (!listed)
 		return;
-	SolveRound a;
-	a.waiting = b.waiting;
-	a.queue[0] = b.queue[0];
-	a.queue[1] = b.queue[1];
-	a.counts = &m.sc->work[5];
-	a.pos = m.pos;
-	a.vq = m.vq;
-	a.vg = m.vg;
-	a.aq = m.aq;
-	a.ag = m.ag;
-	a.attr_count = (int)m.attr_count;
-	a.wedge = m.wedge;
-	a.adj_off = m.adj_off;
-	a.adj = m.adj;
+	const SolveRound a = solve_round_args(m, b);
 	// one wavefront per queued primary, striding over the queue
 	const unsigned int grid = (unsigned int)std::min<size_t>((listed + 3) / 4, 1024);
 	for (uint32_t r = 0; r < rounds; ++r)
@@ -473,6 +646,7 @@ void launch_solve_finish(hipStream_t s, Mesh& m, const SolveBuffers& b, const So
 	size_t n = m.vertex_count;
 	if (!n)
 		return;
+	k_solve_apply<<<grid_for(n), kThreads, 0, s>>>(b.state, b.cand, n, m.pos);
 	k_solve_wedges<<<grid_for(n), kThreads, 0, s>>>(b.update, m.remap, m.kind, n, m.pos);
 	if (m.attr_count)
 		k_solve_attributes<<<grid_for(n), kThreads, 0, s>>>(b.update, m.remap, m.wedge, m.kind, m.pos, m.aq, m.ag, (int)m.attr_count, n, m.va);

@@ -97,7 +97,10 @@ def test_update_workloads(name, den, opts, attrs):
     r, e, b = assert_parity(m.indices, vb, na, target, 1e-2, opts, w)
     assert 0 < r.size <= m.index_count
     call, _ = M.last_call_stats()
-    assert call["solve_vertices"] > 0 and call["solve_rounds"] > 0
+    # the primaries whose outcome hinges on an undecided lower-index neighbour go to the ordered
+    # rounds (often none): every round solves at least one of them, and rounds are launched in
+    # batches that double from 4
+    assert call["solve_rounds"] <= 2 * call["solve_vertices"] + 4
 
 
 def test_update_device_entry():
