@@ -77,27 +77,42 @@ void launch_batch(hipStream_t s, uint32_t threads, const BatchArgs& a, uint32_t 
 
 // one slot per mesh in its class's id list; the slots of a wavefront's meshes come from one
 // atomic per class (same-address atomics from every thread serialise in L2)
+// the size class of mesh p (-1: none) and the status word to write for it (0: none; the kernel
+// of its class writes it), with its triangle count
+__device__ __forceinline__ int batch_mesh_class(const BatchArgs& a, const BatchClasses& c, size_t p, uint32_t& status, uint32_t& tris)
+{
+	const uint32_t ib = a.index_offsets[p], ie = a.index_offsets[p + 1];
+	const uint32_t vb = a.vertex_offsets[p], ve = a.vertex_offsets[p + 1];
+	status = 0;
+	tris = 0;
+	if (ie < ib || ve < vb || ve > a.vertex_count || (ie - ib) % 3 != 0 || a.target_index_counts[p] > ie - ib)
+	{
+		status = 4;
+		return -1;
+	}
+	const uint32_t ni = ie - ib, nv = ve - vb;
+	tris = ni / 3;
+	int cls = -1;
+	for (int k = 0; k < kBatchClasses && cls < 0; ++k)
+		if (nv <= c.vcap[k] && ni <= c.icap[k])
+			cls = k;
+	if (cls < 0)
+		status = 1; // too large for the LDS layouts: the plain path
+	else if (cls == kBatchClasses - 1)
+		status = 6; // runs after the other classes (its workgroup takes a whole CU)
+	return cls;
+}
+
 __global__ void k_batch_classify(BatchArgs a, BatchClasses c, uint32_t* ids, uint32_t* counts)
 {
 	size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
 	int cls = -1;
 	if (p < a.patch_count)
 	{
-		const uint32_t ib = a.index_offsets[p], ie = a.index_offsets[p + 1];
-		const uint32_t vb = a.vertex_offsets[p], ve = a.vertex_offsets[p + 1];
-		if (ie < ib || ve < vb || ve > a.vertex_count || (ie - ib) % 3 != 0 || a.target_index_counts[p] > ie - ib)
-			a.status[p] = 4;
-		else
-		{
-			const uint32_t ni = ie - ib, nv = ve - vb;
-			for (int k = 0; k < kBatchClasses && cls < 0; ++k)
-				if (nv <= c.vcap[k] && ni <= c.icap[k])
-					cls = k;
-			if (cls < 0)
-				a.status[p] = 1; // too large for the LDS layouts: the plain path
-			else if (cls == kBatchClasses - 1)
-				a.status[p] = 6; // runs after the other classes (its workgroup takes a whole CU)
-		}
+		uint32_t status, tris;
+		cls = batch_mesh_class(a, c, p, status, tris);
+		if (status)
+			a.status[p] = status;
 	}
 	const uint32_t lane = __lane_id();
 	const uint64_t below = (1ull << lane) - 1ull;
@@ -214,6 +229,76 @@ extern "C" void meshopt_hip_batch_profile(unsigned long long* out, int reset)
 }
 #endif
 
+// exclusive scan of hist[1024] in place by a 1024-thread block; returns the total
+__device__ __forceinline__ uint32_t order_scan(uint32_t* hist, uint32_t* wsum)
+{
+	const uint32_t tid = threadIdx.x;
+	uint32_t v = hist[tid], incl = v;
+	for (int o = 1; o < 64; o <<= 1)
+	{
+		uint32_t x = __shfl_up(incl, o, 64);
+		if (__lane_id() >= (uint32_t)o)
+			incl += x;
+	}
+	if (__lane_id() == 63)
+		wsum[tid / 64] = incl;
+	__syncthreads();
+	uint32_t before = 0, total = 0;
+	for (uint32_t w = 0; w < 16; ++w)
+	{
+		before += w < tid / 64 ? wsum[w] : 0u;
+		total += wsum[w];
+	}
+	__syncthreads();
+	hist[tid] = before + incl - v;
+	__syncthreads();
+	return total;
+}
+
+// classification and ordering in one launch, for batches of up to kFusedOrderMax meshes:
+// workgroup k keeps the meshes of class k (every workgroup classifies every mesh, the first
+// writes the status words) and lists them larger first with a counting sort by triangle count
+__global__ void __launch_bounds__(1024) k_batch_classify_order(BatchArgs a, BatchClasses c, uint32_t* ids, uint32_t* counts)
+{
+	__shared__ uint32_t hist[1024];
+	__shared__ uint32_t wsum[16];
+	const uint32_t tid = threadIdx.x, k = blockIdx.x;
+	const size_t P = a.patch_count;
+	uint32_t* list = ids + (size_t)k * P;
+	hist[tid] = 0;
+	__syncthreads();
+	for (size_t p = tid; p < P; p += 1024)
+	{
+		uint32_t status, tris;
+		const int cls = batch_mesh_class(a, c, p, status, tris);
+		if (k == 0 && status)
+			a.status[p] = status;
+		if (cls == (int)k)
+			atomicAdd(&hist[1023 - (tris < 1023 ? tris : 1023)], 1u);
+	}
+	__syncthreads();
+	const uint32_t total = order_scan(hist, wsum);
+	for (size_t p = tid; p < P; p += 1024)
+	{
+		uint32_t status, tris;
+		if (batch_mesh_class(a, c, p, status, tris) == (int)k)
+			list[atomicAdd(&hist[1023 - (tris < 1023 ? tris : 1023)], 1u)] = (uint32_t)p;
+	}
+	if (tid == 0)
+		counts[k] = total;
+}
+
+bool launch_batch_classify_order(hipStream_t s, const BatchArgs& a, const BatchClasses& classes, uint32_t* ids, uint32_t* counts)
+{
+	if (a.patch_count > kFusedOrderMax)
+		return false;
+	if (a.patch_count)
+		k_batch_classify_order<<<kBatchClasses, 1024, 0, s>>>(a, classes, ids, counts);
+	else
+		fill_u32(s, counts, 0u, kBatchClasses);
+	return true;
+}
+
 // each class's mesh list, larger meshes first (a counting sort by triangle count, one block per
 // class): long workgroups start early and the launch ends with short ones
 __global__ void __launch_bounds__(1024) k_batch_order(const uint32_t* index_offsets, uint32_t* ids, const uint32_t* counts, size_t P, uint32_t* temp)
@@ -233,22 +318,7 @@ __global__ void __launch_bounds__(1024) k_batch_order(const uint32_t* index_offs
 		atomicAdd(&hist[1023 - (t < 1023 ? t : 1023)], 1u);
 	}
 	__syncthreads();
-	uint32_t v = hist[tid], incl = v;
-	for (int o = 1; o < 64; o <<= 1)
-	{
-		uint32_t x = __shfl_up(incl, o, 64);
-		if (__lane_id() >= (uint32_t)o)
-			incl += x;
-	}
-	if (__lane_id() == 63)
-		wsum[tid / 64] = incl;
-	__syncthreads();
-	uint32_t before = 0;
-	for (uint32_t w = 0; w < tid / 64; ++w)
-		before += wsum[w];
-	__syncthreads();
-	hist[tid] = before + incl - v;
-	__syncthreads();
+	order_scan(hist, wsum);
 	for (uint32_t i = tid; i < n; i += 1024)
 	{
 		const uint32_t p = list[i];

@@ -1791,9 +1791,12 @@ static int simplify_batch_on_device(Context& ctx, const BatchMemory& mem, unsign
 		if (timed)
 			HIP_CHECK(hipEventRecord(ctx.batch_ev[0], s));
 		static_assert(kBatchClasses <= 4, "one count word per class before the id lists");
-		fill_u32(s, counts, 0u, kBatchClasses);
-		launch_batch_classify(s, a, classes, ids, counts);
-		launch_batch_order(s, a, ids, counts, ids + (size_t)kBatchClasses * P);
+		if (!launch_batch_classify_order(s, a, classes, ids, counts))
+		{
+			fill_u32(s, counts, 0u, kBatchClasses);
+			launch_batch_classify(s, a, classes, ids, counts);
+			launch_batch_order(s, a, ids, counts, ids + (size_t)kBatchClasses * P);
+		}
 		ctx.ensure_aux_streams();
 		HIP_CHECK(hipEventRecord(ctx.fork_ev, s));
 		// when the host knows the index counts (host offsets, or the caller's host copy of them)
