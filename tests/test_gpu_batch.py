@@ -151,6 +151,18 @@ def test_empty_and_trivial_meshes():
     check_batch(meshes, targets, 1e-2, 0)
 
 
+def test_batch_beyond_fused_ordering():
+    """A batch of more meshes than the one-launch classification takes (16384): the classify and
+    order kernels run apart; every mesh still matches the oracle."""
+    rng = np.random.default_rng(4242)
+    base = [W.grid(n, seed=s) for n, s in ((4, 1), (5, 2), (6, 3), (9, 4), (14, 5), (24, 6))]
+    picks = rng.integers(0, len(base), 16500)
+    meshes = [(base[k].indices, base[k].vertices) for k in picks]
+    targets = [int(rng.integers(0, base[k].index_count // 3 + 1)) * 3 for k in picks]
+    announce(f"batch of {len(meshes)} small grids, random targets")
+    check_batch(meshes, targets, 5e-2, 0)
+
+
 def test_bad_index_raises():
     announce("index out of range")
     m = W.grid(5)
