@@ -587,8 +587,10 @@ static float host_float_bits(uint32_t u)
 	return f;
 }
 
-// candidate lists up to this length are decided in one rank window (no extension phase)
-constexpr size_t kWholeWindow = 131072;
+// candidate lists up to this length are decided in one rank window (no extension phase): a
+// longer list's window holds the ranks its stop is expected within, which spares the rounds the
+// ranks past it (grid 256: 2.76 ms per call with 131072 here, 2.56 ms with 65536)
+constexpr size_t kWholeWindow = 65536;
 // list length the single-workgroup tail of the commit rounds takes in one sweep
 constexpr uint32_t kTailList = 1024;
 // list length at which the tail takes over from the head rounds: a tail round costs about 5 us
@@ -606,6 +608,17 @@ static std::atomic<int> g_deep_heads{kDeepHeads}, g_deep_from_head{kDeepFromHead
 // process: the rounds a list needs grow with its length, and the contexts of concurrent calls and
 // serving slots see calls of very different sizes (0: not seen yet)
 static std::atomic<int> g_head_rounds[33];
+// the same for the extension window [W, C) of a pass, learned only from extensions that opened
+// (most never do: each head of an extension that does not open is an empty launch)
+static std::atomic<int> g_ext_head_rounds[33];
+
+static int size_bucket(size_t n)
+{
+	int bucket = 0;
+	while (bucket < 32 && (size_t(1) << bucket) < n)
+		bucket++;
+	return bucket;
+}
 
 // read-backs of the calling thread's current call (meshopt_hip_call_stats::read_backs)
 static thread_local unsigned int t_read_backs;
@@ -825,9 +838,7 @@ static size_t simplify_on_device(Context& ctx, const CallArgs& c, float* result_
 		W = C <= kWholeWindow ? C : std::min(C, std::max<size_t>(W, 32768));
 
 		// a window the tail takes in one sweep per round needs no head rounds
-		int bucket = 0;
-		while (bucket < 32 && (size_t(1) << bucket) < W)
-			bucket++;
+		const int bucket = size_bucket(W);
 		const int learned = g_head_rounds[bucket].load(std::memory_order_relaxed);
 		const int heads = W <= kTailStart ? 0 : (learned ? learned : kFirstHeads);
 		// the flip test of the first rounds waits for every moving corner (cheap, and enough for
@@ -850,12 +861,15 @@ static size_t simplify_on_device(Context& ctx, const CallArgs& c, float* result_
 		launch_window_extend(s, m, 0, W, false);
 		window_rounds(0, W, 0, heads, (size_t)-1);
 		launch_stop_scan(s, m, W, C, goal, error_limit, false);
+		const int ext_bucket = size_bucket(C - W);
+		const int ext_learned = g_ext_head_rounds[ext_bucket].load(std::memory_order_relaxed);
+		const int ext_heads = std::max(2, std::min(ext_learned ? ext_learned : 2, kHeadSizeSlots - heads));
 		if (W < C)
 		{
 			// the rest of the list, decided only when the window held no break (otherwise every
 			// kernel below leaves at once: a few small grids)
 			launch_window_extend(s, m, W, C, true);
-			window_rounds(W, C, heads, 2, (size_t)64 * 1024);
+			window_rounds(W, C, heads, ext_heads, (size_t)64 * 1024);
 			launch_stop_scan(s, m, C, C, goal, error_limit, true);
 		}
 		launch_stop_commit(s, m, C, goal, error_limit);
@@ -876,25 +890,33 @@ static size_t simplify_on_device(Context& ctx, const CallArgs& c, float* result_
 		// (when it was not reached, extrapolated from the last two head rounds' decay, at most
 		// twice as many), an overshoot given back a quarter at a time, so that windows of one size
 		// with different needs (passes of one mesh, meshes of a serving stream) settle near the
-		// larger need rather than alternate
-		if (heads)
-		{
+		// larger need rather than alternate.  `base`: the first head's slot in head_sizes
+		auto learn_heads = [&](int base, int count) {
 			int need = -1;
-			for (int hd = 0; hd < heads; ++hd)
-				if (h.head_sizes[hd] < kTailStart)
+			for (int hd = 0; hd < count; ++hd)
+				if (h.head_sizes[base + hd] < kTailStart)
 				{
 					need = hd;
 					break;
 				}
 			if (need < 0)
 			{
-				need = heads * 2;
-				const double a = h.head_sizes[heads - 2], b = h.head_sizes[heads - 1];
+				need = count * 2;
+				const double a = h.head_sizes[base + count - 2], b = h.head_sizes[base + count - 1];
 				if (b > 0 && b < a)
-					need = std::min(need, heads + 1 + (int)std::ceil(std::log((double)kTailStart / b) / std::log(b / a)));
+					need = std::min(need, count + 1 + (int)std::ceil(std::log((double)kTailStart / b) / std::log(b / a)));
 			}
-			const int next = std::max(need, heads - heads / 4);
-			g_head_rounds[bucket].store(std::max(2, std::min(kMaxHeadRounds, next)), std::memory_order_relaxed);
+			const int next = std::max(need, count - count / 4);
+			return std::max(2, std::min(kMaxHeadRounds, next));
+		};
+		if (heads)
+			g_head_rounds[bucket].store(learn_heads(0, heads), std::memory_order_relaxed);
+		if (W < C)
+		{
+			if (h.extended)
+				g_ext_head_rounds[ext_bucket].store(learn_heads(heads, ext_heads), std::memory_order_relaxed);
+			else if (ext_learned > 2)
+				g_ext_head_rounds[ext_bucket].store(ext_learned - ext_learned / 4, std::memory_order_relaxed);
 		}
 		long_chains |= heads >= deep_heads && h.head_sizes[heads - 1] >= kTailList;
 		ctx.window_fraction = std::min(1.0, (double)(h.stop + 1) / (double)C);
