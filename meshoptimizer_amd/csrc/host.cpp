@@ -888,9 +888,11 @@ static size_t simplify_on_device(Context& ctx, const CallArgs& c, float* result_
 
 		// head rounds for the next window of this size: enough to bring the list under kTailStart
 		// (when it was not reached, extrapolated from the last two head rounds' decay, at most
-		// twice as many), an overshoot given back a quarter at a time, so that windows of one size
+		// twice as many), an overshoot given back an eighth at a time, so that windows of one size
 		// with different needs (passes of one mesh, meshes of a serving stream) settle near the
-		// larger need rather than alternate.  `base`: the first head's slot in head_sizes
+		// larger need rather than alternate: a head too many costs about what the tail round it
+		// replaces costs, a head too few hands the tail a list it sweeps one workgroup at a time.
+		// `base`: the first head's slot in head_sizes
 		auto learn_heads = [&](int base, int count) {
 			int need = -1;
 			for (int hd = 0; hd < count; ++hd)
@@ -906,7 +908,7 @@ static size_t simplify_on_device(Context& ctx, const CallArgs& c, float* result_
 				if (b > 0 && b < a)
 					need = std::min(need, count + 1 + (int)std::ceil(std::log((double)kTailStart / b) / std::log(b / a)));
 			}
-			const int next = std::max(need, count - count / 4);
+			const int next = std::max(need, count - count / 8);
 			return std::max(2, std::min(kMaxHeadRounds, next));
 		};
 		if (heads)
@@ -916,7 +918,7 @@ static size_t simplify_on_device(Context& ctx, const CallArgs& c, float* result_
 			if (h.extended)
 				g_ext_head_rounds[ext_bucket].store(learn_heads(heads, ext_heads), std::memory_order_relaxed);
 			else if (ext_learned > 2)
-				g_ext_head_rounds[ext_bucket].store(ext_learned - ext_learned / 4, std::memory_order_relaxed);
+				g_ext_head_rounds[ext_bucket].store(ext_learned - ext_learned / 8, std::memory_order_relaxed);
 		}
 		long_chains |= heads >= deep_heads && h.head_sizes[heads - 1] >= kTailList;
 		ctx.window_fraction = std::min(1.0, (double)(h.stop + 1) / (double)C);
