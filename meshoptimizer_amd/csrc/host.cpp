@@ -845,7 +845,7 @@ static size_t simplify_on_device(Context& ctx, const CallArgs& c, float* result_
 		// most meshes); a window known to need long chains of rounds examines the moving corners'
 		// alternatives from round kDeepFromHead on, which decides ranks far earlier on such meshes
 		const int deep_heads = g_deep_heads.load(std::memory_order_relaxed), deep_from = g_deep_from_head.load(std::memory_order_relaxed);
-		const bool deep = long_chains || learned > deep_heads || deep_heads < 0;
+		bool deep = long_chains || learned > deep_heads || deep_heads < 0;
 		launch_perform_init(s, m, C, deep);
 		// head rounds with one thread per rank of the window (the list only shrinks; workgroups
 		// past its current length leave at once, while a grid smaller than the list would make
@@ -859,17 +859,49 @@ static size_t simplify_on_device(Context& ctx, const CallArgs& c, float* result_
 			launch_resolve(s, m, lo, hi);
 		};
 		launch_window_extend(s, m, 0, W, false);
-		window_rounds(0, W, 0, heads, (size_t)-1);
+		int heads_run = heads;
+		bool switched_deep = false;
+		if (!learned && W > kWholeWindow && heads)
+		{
+			// a window size not seen yet: head rounds in doubling batches, each followed by a
+			// read-back of the list length, until the list is short enough for the tail (a first
+			// guess too small would hand the tail a list of hundreds of thousands of ranks).  A list
+			// still long after deep_heads rounds has long chains: deep rounds from then on, with the
+			// per-source lists they read built first
+			int done = 0, batch = heads;
+			for (;;)
+			{
+				for (int k = 0; k < batch && done < kMaxHeadRounds; ++k, ++done)
+					launch_round(s, m, W, done, done, deep && done >= deep_from);
+				sync_scalars(ctx, m);
+				uint32_t left = 0;
+				for (int q = 0; q < kListSegments; ++q)
+					left += h.list_count[(h.round + done) % 3][q];
+				if (left < kTailStart || done >= kMaxHeadRounds)
+					break;
+				if (!deep && deep_heads >= 0 && done >= deep_heads && left >= kTailList)
+				{
+					launch_source_lists(s, m, C);
+					deep = switched_deep = true;
+				}
+				batch = done;
+			}
+			heads_run = done;
+			launch_round_tail(s, m, done, (uint32_t)(W + 2), deep);
+			launch_resolve(s, m, 0, W);
+		}
+		else
+			window_rounds(0, W, 0, heads, (size_t)-1);
 		launch_stop_scan(s, m, W, C, goal, error_limit, false);
 		const int ext_bucket = size_bucket(C - W);
 		const int ext_learned = g_ext_head_rounds[ext_bucket].load(std::memory_order_relaxed);
-		const int ext_heads = std::max(2, std::min(ext_learned ? ext_learned : 2, kHeadSizeSlots - heads));
+		const int ext_heads = std::max(2, std::min(ext_learned ? ext_learned : 2, kHeadSizeSlots - heads_run));
 		if (W < C)
 		{
 			// the rest of the list, decided only when the window held no break (otherwise every
 			// kernel below leaves at once: a few small grids)
 			launch_window_extend(s, m, W, C, true);
-			window_rounds(W, C, heads, ext_heads, (size_t)64 * 1024);
+			window_rounds(W, C, heads_run, ext_heads, (size_t)64 * 1024);
 			launch_stop_scan(s, m, C, C, goal, error_limit, true);
 		}
 		launch_stop_commit(s, m, C, goal, error_limit);
@@ -911,16 +943,16 @@ static size_t simplify_on_device(Context& ctx, const CallArgs& c, float* result_
 			const int next = std::max(need, count - count / 8);
 			return std::max(2, std::min(kMaxHeadRounds, next));
 		};
-		if (heads)
-			g_head_rounds[bucket].store(learn_heads(0, heads), std::memory_order_relaxed);
+		if (heads_run)
+			g_head_rounds[bucket].store(learn_heads(0, heads_run), std::memory_order_relaxed);
 		if (W < C)
 		{
 			if (h.extended)
-				g_ext_head_rounds[ext_bucket].store(learn_heads(heads, ext_heads), std::memory_order_relaxed);
+				g_ext_head_rounds[ext_bucket].store(learn_heads(heads_run, ext_heads), std::memory_order_relaxed);
 			else if (ext_learned > 2)
 				g_ext_head_rounds[ext_bucket].store(ext_learned - ext_learned / 8, std::memory_order_relaxed);
 		}
-		long_chains |= heads >= deep_heads && h.head_sizes[heads - 1] >= kTailList;
+		long_chains |= switched_deep || (heads_run >= deep_heads && h.head_sizes[heads_run - 1] >= kTailList);
 		ctx.window_fraction = std::min(1.0, (double)(h.stop + 1) / (double)C);
 
 		meshopt_hip_pass_stats ps{};

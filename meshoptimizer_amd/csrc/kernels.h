@@ -107,6 +107,7 @@ void launch_rank_sort(hipStream_t s, Mesh& m, size_t cand_count);
 // commit state of a pass; with source_lists, also the candidate ranks grouped by source class
 // that deep rounds read
 void launch_perform_init(hipStream_t s, Mesh& m, size_t cand_count, bool source_lists);
+void launch_source_lists(hipStream_t s, Mesh& m, size_t cand_count);
 void launch_pick_capacity(hipStream_t s, Mesh& m, size_t index_count);
 void launch_round(hipStream_t s, Mesh& m, size_t list_bound, int head_index, int rel, bool deep);
 void launch_round_tail(hipStream_t s, Mesh& m, int rel, uint32_t max_rounds, bool deep);

@@ -900,8 +900,15 @@ void launch_perform_init(hipStream_t s, Mesh& m, size_t cand_count, bool source_
 	iota_u32(s, m.cremap, n);
 	fill_u32(s, m.lock_rank, kNone, n);
 	k_perform_init_scalars<<<1, 1, 0, s>>>(m.sc);
-	if (!source_lists)
-		return;
+	if (source_lists)
+		launch_source_lists(s, m, cand_count);
+}
+
+// the per-source candidate lists deep rounds read (they depend on the ranks alone, so they may
+// also be built between the rounds of a window)
+void launch_source_lists(hipStream_t s, Mesh& m, size_t cand_count)
+{
+	size_t n = m.vertex_count;
 	fill_u32(s, m.src_off, 0, n + 1);
 	// the rank slots go to m.keys, free once the ranked copies are gathered
 	if (cand_count)
