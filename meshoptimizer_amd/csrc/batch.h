@@ -88,12 +88,13 @@ bool batch_allow_lds(uint32_t threads, size_t bytes);
 // per mesh: validate sizes (status 4), pick its class (ids[c * patch_count + slot], counts[c])
 // or mark it for the plain path (status 1); counts must be zero on entry
 void launch_batch_classify(hipStream_t s, const BatchArgs& a, const BatchClasses& classes, uint32_t* ids, uint32_t* counts);
-void launch_batch_order(hipStream_t s, const BatchArgs& a, uint32_t* ids, const uint32_t* counts, uint32_t* temp);
+// host_counts (may be null): a host-visible copy of the class counts
+void launch_batch_order(hipStream_t s, const BatchArgs& a, uint32_t* ids, const uint32_t* counts, uint32_t* temp, uint32_t* host_counts);
 // batches of up to this many meshes are classified and ordered in one launch
 constexpr size_t kFusedOrderMax = 16384;
 // classify and order in one launch (counts written, no clearing needed); false when the batch is
 // larger than kFusedOrderMax (then launch_batch_classify and launch_batch_order)
-bool launch_batch_classify_order(hipStream_t s, const BatchArgs& a, const BatchClasses& classes, uint32_t* ids, uint32_t* counts);
+bool launch_batch_classify_order(hipStream_t s, const BatchArgs& a, const BatchClasses& classes, uint32_t* ids, uint32_t* counts, uint32_t* host_counts);
 
 // the meshes of one class (ids, *count_dev of them), every one with the (vmax, imax, cmax) layout,
 // on workgroups of `threads`: cursor == null gives workgroup b mesh b (grid at least the count;

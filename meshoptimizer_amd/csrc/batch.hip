@@ -258,7 +258,7 @@ __device__ __forceinline__ uint32_t order_scan(uint32_t* hist, uint32_t* wsum)
 // classification and ordering in one launch, for batches of up to kFusedOrderMax meshes:
 // workgroup k keeps the meshes of class k (every workgroup classifies every mesh, the first
 // writes the status words) and lists them larger first with a counting sort by triangle count
-__global__ void __launch_bounds__(1024) k_batch_classify_order(BatchArgs a, BatchClasses c, uint32_t* ids, uint32_t* counts)
+__global__ void __launch_bounds__(1024) k_batch_classify_order(BatchArgs a, BatchClasses c, uint32_t* ids, uint32_t* counts, uint32_t* host_counts)
 {
 	__shared__ uint32_t hist[1024];
 	__shared__ uint32_t wsum[16];
@@ -285,15 +285,19 @@ __global__ void __launch_bounds__(1024) k_batch_classify_order(BatchArgs a, Batc
 			list[atomicAdd(&hist[1023 - (tris < 1023 ? tris : 1023)], 1u)] = (uint32_t)p;
 	}
 	if (tid == 0)
+	{
 		counts[k] = total;
+		if (host_counts)
+			host_counts[k] = total;
+	}
 }
 
-bool launch_batch_classify_order(hipStream_t s, const BatchArgs& a, const BatchClasses& classes, uint32_t* ids, uint32_t* counts)
+bool launch_batch_classify_order(hipStream_t s, const BatchArgs& a, const BatchClasses& classes, uint32_t* ids, uint32_t* counts, uint32_t* host_counts)
 {
 	if (a.patch_count > kFusedOrderMax)
 		return false;
 	if (a.patch_count)
-		k_batch_classify_order<<<kBatchClasses, 1024, 0, s>>>(a, classes, ids, counts);
+		k_batch_classify_order<<<kBatchClasses, 1024, 0, s>>>(a, classes, ids, counts, host_counts);
 	else
 		fill_u32(s, counts, 0u, kBatchClasses);
 	return true;
@@ -301,12 +305,15 @@ bool launch_batch_classify_order(hipStream_t s, const BatchArgs& a, const BatchC
 
 // each class's mesh list, larger meshes first (a counting sort by triangle count, one block per
 // class): long workgroups start early and the launch ends with short ones
-__global__ void __launch_bounds__(1024) k_batch_order(const uint32_t* index_offsets, uint32_t* ids, const uint32_t* counts, size_t P, uint32_t* temp)
+__global__ void __launch_bounds__(1024) k_batch_order(const uint32_t* index_offsets, uint32_t* ids, const uint32_t* counts, size_t P, uint32_t* temp,
+                                                      uint32_t* host_counts)
 {
 	__shared__ uint32_t hist[1024];
 	__shared__ uint32_t wsum[16];
 	const uint32_t tid = threadIdx.x;
 	const uint32_t n = counts[blockIdx.x];
+	if (host_counts && tid == 0)
+		host_counts[blockIdx.x] = n;
 	uint32_t* list = ids + (size_t)blockIdx.x * P;
 	uint32_t* tmp = temp + (size_t)blockIdx.x * P;
 	hist[tid] = 0;
@@ -330,10 +337,10 @@ __global__ void __launch_bounds__(1024) k_batch_order(const uint32_t* index_offs
 		list[i] = tmp[i];
 }
 
-void launch_batch_order(hipStream_t s, const BatchArgs& a, uint32_t* ids, const uint32_t* counts, uint32_t* temp)
+void launch_batch_order(hipStream_t s, const BatchArgs& a, uint32_t* ids, const uint32_t* counts, uint32_t* temp, uint32_t* host_counts)
 {
 	if (a.patch_count)
-		k_batch_order<<<kBatchClasses, 1024, 0, s>>>(a.index_offsets, ids, counts, a.patch_count, temp);
+		k_batch_order<<<kBatchClasses, 1024, 0, s>>>(a.index_offsets, ids, counts, a.patch_count, temp, host_counts);
 }
 
 } // namespace mhip

@@ -134,6 +134,7 @@ struct Context
 	uint32_t* batch_dev = nullptr;    // batched calls: status, class counts and class id lists, then 4 statistics words per mesh
 	size_t batch_dev_words = 0;
 	uint32_t* batch_pinned = nullptr; // batched calls: status read-back, then the retry list
+	uint32_t* batch_pinned_dev = nullptr; // the same memory's device address (kernels write the status there)
 	size_t batch_pinned_words = 0;
 	size_t batch_last_meshes = 0;     // meshes of the last batched call (their statistics follow the lists)
 	Quad* batch_quadrics = nullptr; // batched calls: one vertex quadric per vertex of the batch
@@ -179,6 +180,7 @@ struct Context
 			batch_pinned = nullptr;
 			batch_pinned_words = 0;
 			HIP_CHECK(hipHostMalloc((void**)&batch_pinned, batch_host_words(meshes) * sizeof(uint32_t), hipHostMallocDefault));
+			HIP_CHECK(hipHostGetDevicePointer((void**)&batch_pinned_dev, batch_pinned, 0));
 			batch_pinned_words = batch_host_words(meshes);
 		}
 	}
@@ -1702,6 +1704,9 @@ struct BatchMemory
 	Quad* attr_quadrics;
 	Grad* attr_grads;
 	size_t attr_room;
+	// device address of `host` when it is pinned memory the kernels may write (then the status
+	// words and class counts need no copy back); null for host memory the device cannot write
+	uint32_t* host_dev = nullptr;
 };
 
 // the attributes of a batched call: device values packed per vertex like the positions, their
@@ -1722,7 +1727,8 @@ static BatchMemory context_batch_memory(Context& ctx, size_t P, size_t vertices,
 	if (need_quadrics)
 		ctx.ensure_batch_quadrics(vertices ? vertices : 1);
 	ctx.batch_last_meshes = P;
-	BatchMemory m{ctx.batch_dev, ctx.batch_dev + batch_list_words(P), ctx.batch_quadrics, ctx.batch_pinned, true, timing_on(), nullptr, nullptr, nullptr, 0};
+	BatchMemory m{ctx.batch_dev, ctx.batch_dev + batch_list_words(P), ctx.batch_quadrics, ctx.batch_pinned, true, timing_on(), nullptr, nullptr, nullptr, 0,
+	              ctx.batch_pinned_dev};
 	if (need_quadrics && attributes)
 	{
 		const size_t n = vertices ? vertices : 1;
@@ -1800,8 +1806,11 @@ static int simplify_batch_on_device(Context& ctx, const BatchMemory& mem, unsign
 	const size_t cu_count = ctx.cu_count;
 	const BatchShape& shape = classes.shape;
 	const size_t P = patch_count;
-	uint32_t* status = mem.words;
+	// the status words go straight to pinned host memory when the device may write it (no copy
+	// back); the class counts stay on the device (the class kernels read them), with a host copy
+	uint32_t* status = mem.host_dev ? mem.host_dev : mem.words;
 	uint32_t* counts = mem.words + P;
+	uint32_t* host_counts = mem.host_dev ? mem.host_dev + P : nullptr;
 	uint32_t* ids = mem.words + P + 4;
 	uint32_t* h_status = mem.host;
 	uint32_t* h_retry = mem.host + P + kBatchClasses;
@@ -1847,11 +1856,11 @@ static int simplify_batch_on_device(Context& ctx, const BatchMemory& mem, unsign
 		if (timed)
 			HIP_CHECK(hipEventRecord(ctx.batch_ev[0], s));
 		static_assert(kBatchClasses <= 4, "one count word per class before the id lists");
-		if (!launch_batch_classify_order(s, a, classes, ids, counts))
+		if (!launch_batch_classify_order(s, a, classes, ids, counts, host_counts))
 		{
 			fill_u32(s, counts, 0u, kBatchClasses);
 			launch_batch_classify(s, a, classes, ids, counts);
-			launch_batch_order(s, a, ids, counts, ids + (size_t)kBatchClasses * P);
+			launch_batch_order(s, a, ids, counts, ids + (size_t)kBatchClasses * P, host_counts);
 		}
 		ctx.ensure_aux_streams();
 		HIP_CHECK(hipEventRecord(ctx.fork_ev, s));
@@ -1912,7 +1921,8 @@ static int simplify_batch_on_device(Context& ctx, const BatchMemory& mem, unsign
 		if (timed)
 			HIP_CHECK(hipEventRecord(ctx.batch_ev[1], s));
 		HIP_CHECK(hipGetLastError());
-		HIP_CHECK(hipMemcpyAsync(h_status, status, (P + kBatchClasses) * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+		if (!mem.host_dev)
+			HIP_CHECK(hipMemcpyAsync(h_status, status, (P + kBatchClasses) * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
 		enqueue_readback();
 		HIP_CHECK(hipStreamSynchronize(s));
 		auto add_time = [&](size_t launches, size_t meshes) {
@@ -1964,7 +1974,8 @@ static int simplify_batch_on_device(Context& ctx, const BatchMemory& mem, unsign
 			if (timed)
 				HIP_CHECK(hipEventRecord(ctx.batch_ev[1], s));
 			HIP_CHECK(hipGetLastError());
-			HIP_CHECK(hipMemcpyAsync(h_status, status, P * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+			if (!mem.host_dev)
+				HIP_CHECK(hipMemcpyAsync(h_status, status, P * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
 			enqueue_readback();
 			HIP_CHECK(hipStreamSynchronize(s));
 			if (timed)
