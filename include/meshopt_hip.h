@@ -159,6 +159,12 @@ int meshopt_hip_last_call_stats(meshopt_hip_call_stats* call, meshopt_hip_pass_s
  * calls in the process; returns 0, or -1 with meshopt_hip_last_error set. */
 int meshopt_hip_set_round_policy(int deep_heads, int deep_from_head);
 
+/* Forgets the head-round counts the library learned per window size (no counterpart in the
+ * reference; results never depend on them): later calls start as the process's first did, with
+ * a read-back of the list length after each batch of head rounds of a window size not yet seen.
+ * For callers that time or test first calls; returns 0. */
+int meshopt_hip_reset_round_counts(void);
+
 /* HIP-event timing of the dominant kernel (k_round_x, the commit rounds) on the library's
  * stream.  Each call returns the totals gathered since the previous call (summed kernel time in
  * ms, launches, ranks listed over those launches), resets them, and turns timing on or off. */

@@ -74,6 +74,7 @@ EXPORTED_SYMBOLS = (
     "meshopt_hip_server_destroy",
     "meshopt_hip_last_call_stats",
     "meshopt_hip_set_round_policy",
+    "meshopt_hip_reset_round_counts",
     "meshopt_hip_last_error",
     "meshopt_hip_kernel_timing",
     "meshopt_hip_release",
@@ -167,6 +168,8 @@ def library() -> ctypes.CDLL:
     lib.meshopt_hip_last_batch_stats.argtypes = [_P, _SZ]
     lib.meshopt_hip_set_round_policy.restype = ctypes.c_int
     lib.meshopt_hip_set_round_policy.argtypes = [ctypes.c_int, ctypes.c_int]
+    lib.meshopt_hip_reset_round_counts.restype = ctypes.c_int
+    lib.meshopt_hip_reset_round_counts.argtypes = []
     lib.meshopt_hip_batch_kernel_timing.restype = None
     lib.meshopt_hip_batch_kernel_timing.argtypes = [_P, _P, _P]
     lib.meshopt_hip_kernel_timing.restype = None
@@ -521,6 +524,11 @@ def set_round_policy(deep_heads=-2, deep_from_head=8):
     lib = library()
     if lib.meshopt_hip_set_round_policy(int(deep_heads), int(deep_from_head)) != 0:
         raise ValueError(lib.meshopt_hip_last_error().decode())
+
+
+def reset_round_counts():
+    """meshopt_hip_reset_round_counts: forget the head-round counts learned per window size."""
+    library().meshopt_hip_reset_round_counts()
 
 
 def last_call_stats():

@@ -2710,6 +2710,17 @@ extern "C" int meshopt_hip_set_round_policy(int deep_heads, int deep_from_head)
 	return 0;
 }
 
+extern "C" int meshopt_hip_reset_round_counts(void)
+{
+	for (int b = 0; b < 33; ++b)
+	{
+		g_head_rounds[b].store(0, std::memory_order_relaxed);
+		g_ext_head_rounds[b].store(0, std::memory_order_relaxed);
+	}
+	t_last_error.clear();
+	return 0;
+}
+
 extern "C" int meshopt_hip_last_call_stats(meshopt_hip_call_stats* call, meshopt_hip_pass_stats* passes, int max_passes)
 {
 	if (call)

@@ -198,3 +198,23 @@ def test_round_policies(policy):
             assert_parity(g.indices, g.vertices, W.target_count(g.index_count, 1, 4), 1e-2, 0)
     finally:
         M.set_round_policy()
+
+
+@pytest.mark.slow
+def test_cold_round_counts():
+    """A process that has not seen a mesh's window sizes runs their head rounds in batches with a
+    read-back of the list length; a long-chained mesh (a 998 x 998 row-major grid) goes on to deep
+    rounds within its first window, with the per-source candidate lists built then.  The cold call
+    and the warm one after it both match the oracle."""
+    g = W.grid(998)
+    target = W.target_count(g.index_count, 3, 4)
+    M.reset_round_counts()
+    announce("cold grid998 target 3/4")
+    assert_parity(g.indices, g.vertices, target, 1e-2, 0)
+    call, passes = M.last_call_stats()
+    print("[cold]", call["read_backs"], [(p["rounds"], p["tail_rounds"]) for p in passes], flush=True)
+    assert call["read_backs"] > len(passes) + 3  # the batches of head rounds read the list back
+    announce("warm grid998 target 3/4")
+    assert_parity(g.indices, g.vertices, target, 1e-2, 0)
+    call, passes = M.last_call_stats()
+    assert call["read_backs"] <= len(passes) + 3
