@@ -200,6 +200,19 @@ def test_round_policies(policy):
         M.set_round_policy()
 
 
+@pytest.mark.parametrize("den,err,opts", [(20, 1e-2, 0), (100, 1.0, M.SimplifyLockBorder)])
+def test_window_extension(den, err, opts):
+    """A pass whose first rank window holds no break goes on to the rest of the list on the device
+    (the extension window, with its own head rounds and tail): icosphere L6 at a low target opens
+    one; the result matches the oracle."""
+    m = W.icosphere(6)
+    target = W.target_count(m.index_count, 1, den)
+    announce(f"window extension icosphere6 target 1/{den} err {err} opts {opts}")
+    assert_parity(m.indices, m.vertices, target, err, opts)
+    call, passes = M.last_call_stats()
+    assert call["device_extensions"] >= 1, (call, len(passes))
+
+
 @pytest.mark.slow
 def test_cold_round_counts():
     """A process that has not seen a mesh's window sizes runs their head rounds in batches with a
