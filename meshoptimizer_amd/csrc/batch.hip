@@ -262,6 +262,9 @@ __global__ void __launch_bounds__(1024) k_batch_classify_order(BatchArgs a, Batc
 {
 	__shared__ uint32_t hist[1024];
 	__shared__ uint32_t wsum[16];
+	// each mesh's histogram bucket when it is of this workgroup's class, else kNone16: the
+	// placement pass reads it here instead of classifying again
+	__shared__ uint16_t bucket[kFusedOrderMax];
 	const uint32_t tid = threadIdx.x, k = blockIdx.x;
 	const size_t P = a.patch_count;
 	uint32_t* list = ids + (size_t)k * P;
@@ -273,16 +276,21 @@ __global__ void __launch_bounds__(1024) k_batch_classify_order(BatchArgs a, Batc
 		const int cls = batch_mesh_class(a, c, p, status, tris);
 		if (k == 0 && status)
 			a.status[p] = status;
+		uint16_t b = kNone16;
 		if (cls == (int)k)
-			atomicAdd(&hist[1023 - (tris < 1023 ? tris : 1023)], 1u);
+		{
+			b = (uint16_t)(1023 - (tris < 1023 ? tris : 1023));
+			atomicAdd(&hist[b], 1u);
+		}
+		bucket[p] = b;
 	}
 	__syncthreads();
 	const uint32_t total = order_scan(hist, wsum);
 	for (size_t p = tid; p < P; p += 1024)
 	{
-		uint32_t status, tris;
-		if (batch_mesh_class(a, c, p, status, tris) == (int)k)
-			list[atomicAdd(&hist[1023 - (tris < 1023 ? tris : 1023)], 1u)] = (uint32_t)p;
+		const uint16_t b = bucket[p];
+		if (b != kNone16)
+			list[atomicAdd(&hist[b], 1u)] = (uint32_t)p;
 	}
 	if (tid == 0)
 	{
