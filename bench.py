@@ -261,8 +261,9 @@ def bench_patches(args):
         achieved = ab / per_launch_s / 1e9
         comp = batch_compulsory_bytes(np.diff(io), np.diff(vo), g_cnt)
         out["roofline"] = {"bound": "hbm",
-                           "kernel": "one batched call: k_batch_classify, k_batch_order and the k_simplify_patches size classes launched "
-                                     "side by side, timed with HIP events from the classify start to the join of the class streams",
+                           "kernel": "one batched call: k_batch_classify_order (classification and largest-first order) and the "
+                                     "k_simplify_patches size classes launched side by side, timed with HIP events from the "
+                                     "classification start to the join of the class streams",
                            "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                            "frac": round(achieved / HBM_PEAK_GBS, 6),
                            "traffic": measured_traffic("patches4096", "k_simplify_patches"),
