@@ -8,9 +8,9 @@ namespace mhip
 
 // LDS size classes of one batched call (launched side by side)
 constexpr int kBatchClasses = 4;
-// the class that a mesh overflowing the candidate arrays of class 0 or 1 runs again in (classes
-// 2 and 3 hold the full bound)
-constexpr int kBatchRetryClass = 2;
+// the class that a mesh overflowing the candidate arrays of classes 0-2 runs again in (the last
+// class, one workgroup per CU, holds the full bound)
+constexpr int kBatchRetryClass = 3;
 
 struct BatchArgs
 {
@@ -67,7 +67,7 @@ BatchShape batch_default_shape();
 const char* batch_shape_error(const BatchShape& shape);
 
 // size classes: a mesh belongs to the first class whose vertex and index caps it fits; ccap is the
-// class's candidate capacity (classes 2 and 3 hold the full bound, so they never return status 5)
+// class's candidate capacity (class 3 holds the full bound, so it never returns status 5)
 struct BatchClasses
 {
 	uint32_t vcap[kBatchClasses];

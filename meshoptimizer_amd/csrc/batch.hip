@@ -181,7 +181,7 @@ BatchClasses batch_classes(uint32_t max_block, const BatchShape& shape)
 {
 	if (max_block < 64 * 1024)
 		max_block = 64 * 1024;
-	const uint32_t vdiv[kBatchClasses] = {4, 4, 3, 3};
+	const uint32_t vdiv[kBatchClasses] = {4, 4, 4, 3};
 	BatchClasses c{};
 	c.shape = shape;
 	uint32_t prev_icap = 0;

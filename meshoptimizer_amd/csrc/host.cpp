@@ -76,7 +76,7 @@ struct PassRecord
 // the class id lists and their ordering's scratch, the largest class's cursor
 static size_t batch_list_words(size_t meshes)
 {
-	return meshes + 4 + 2 * (size_t)kBatchClasses * meshes + 4;
+	return meshes + 4 + 2 * (size_t)kBatchClasses * meshes + kBatchClasses + 1;
 }
 
 // host words of a batched call: the status read-back (every mesh, then the class counts), the list
@@ -1877,33 +1877,36 @@ static int simplify_batch_on_device(Context& ctx, const BatchMemory& mem, unsign
 			for (size_t p = 0; p < P; ++p)
 			{
 				const size_t n = known[p + 1] - known[p];
-				small_empty &= n > classes.icap[kBatchRetryClass];
-				big_early |= n > classes.icap[kBatchRetryClass] && n <= classes.icap[big];
+				small_empty &= n > classes.icap[big - 1];
+				big_early |= n > classes.icap[big - 1] && n <= classes.icap[big];
 			}
 		}
 		// a class's workgroups: one per mesh slot (those past the class's count leave at once), or
-		// with the cursor form `per_cu` per CU looping over the class's list
-		auto launch_class = [&](hipStream_t cs, int k, const uint32_t* list, const uint32_t* count_dev, size_t meshes_bound) {
+		// with the cursor form `per_cu` per CU looping over the class's list through cursor word
+		// `slot` (one per class, and one more for the retry launch, which may run beside its
+		// class's own)
+		auto launch_class = [&](hipStream_t cs, int k, const uint32_t* list, const uint32_t* count_dev, size_t meshes_bound, int slot) {
 			const bool from_cursor = shape.cursor[k] != 0;
 			const size_t grid = from_cursor ? std::min(meshes_bound, (size_t)shape.per_cu[k] * cu_count) : meshes_bound;
 			launch_batch(cs, shape.threads[k], a, classes.vcap[k], classes.icap[k], classes.ccap[k], list, count_dev, grid,
-			             from_cursor ? ids + 2 * (size_t)kBatchClasses * P + k : nullptr);
+			             from_cursor ? ids + 2 * (size_t)kBatchClasses * P + slot : nullptr);
 		};
 		if (big_early)
 		{
 			HIP_CHECK(hipStreamWaitEvent(ctx.aux[big - 1], ctx.fork_ev, 0));
-			launch_class(ctx.aux[big - 1], big, ids + (size_t)big * P, counts + big, P);
+			launch_class(ctx.aux[big - 1], big, ids + (size_t)big * P, counts + big, P, big);
 			HIP_CHECK(hipEventRecord(ctx.join_ev[big - 1], ctx.aux[big - 1]));
 			HIP_CHECK(hipStreamWaitEvent(s, ctx.join_ev[big - 1], 0));
 		}
 		// classes 0, 1 and 2 launch at once (in the default shape with one workgroup per mesh slot),
-		// the class of larger meshes first: its longer workgroups start early
-		// and leave a shorter tail (measured 0.6% faster on the 4096-patch batch than class order).
+		// the class of the largest meshes first: its longer workgroups start early and leave a
+		// shorter tail (the 4096-patch batch split over classes 1 and 2 at 6 and 5 per CU measured
+		// 0.557 ms with the largest class launched last, 0.512 ms with it first).
 		// A workgroup of class 3 takes a whole CU's LDS, so even one that leaves at once waits for
 		// a free CU: class 3 launches after the status read-back, only when it has meshes, with at
 		// most one workgroup per CU looping over them (reading the counts back before launching
 		// classes 2 and 3 instead measured 4.5% slower on the 4096-patch batch).
-		static constexpr int kLaunchOrder[kBatchClasses - 1] = {1, 0, 2};
+		static constexpr int kLaunchOrder[kBatchClasses - 1] = {2, 1, 0};
 		for (int c = 0; c < kBatchClasses - 1; ++c)
 		{
 			const int k = kLaunchOrder[c];
@@ -1911,7 +1914,7 @@ static int simplify_batch_on_device(Context& ctx, const BatchMemory& mem, unsign
 			if (c)
 				HIP_CHECK(hipStreamWaitEvent(cs, ctx.fork_ev, 0));
 			if (classes.icap[k] && !small_empty)
-				launch_class(cs, k, ids + (size_t)k * P, counts + k, P);
+				launch_class(cs, k, ids + (size_t)k * P, counts + k, P, k);
 			if (c)
 			{
 				HIP_CHECK(hipEventRecord(ctx.join_ev[c - 1], cs));
@@ -1943,7 +1946,13 @@ static int simplify_batch_on_device(Context& ctx, const BatchMemory& mem, unsign
 		uint32_t retries = 0;
 		for (size_t p = 0; p < P; ++p)
 			if (h_status[p] == 5)
-				h_retry[retries++] = (uint32_t)p;
+			{
+				// a retry class left without caps (LDS limits) sends the mesh to the plain path
+				if (classes.icap[kBatchRetryClass])
+					h_retry[retries++] = (uint32_t)p;
+				else
+					h_status[p] = 1;
+			}
 		if (retries || large)
 		{
 			if (timed)
@@ -1954,7 +1963,7 @@ static int simplify_batch_on_device(Context& ctx, const BatchMemory& mem, unsign
 				const int c = kBatchClasses - 1;
 				HIP_CHECK(hipEventRecord(ctx.fork_ev, s));
 				HIP_CHECK(hipStreamWaitEvent(ls, ctx.fork_ev, 0));
-				launch_class(ls, c, ids + (size_t)c * P, counts + c, large);
+				launch_class(ls, c, ids + (size_t)c * P, counts + c, large, c);
 			}
 			if (retries)
 			{
@@ -1964,7 +1973,7 @@ static int simplify_batch_on_device(Context& ctx, const BatchMemory& mem, unsign
 				h_retry[P] = retries;
 				HIP_CHECK(hipMemcpyAsync(ids, h_retry, retries * sizeof(uint32_t), hipMemcpyHostToDevice, s));
 				HIP_CHECK(hipMemcpyAsync(counts, h_retry + P, sizeof(uint32_t), hipMemcpyHostToDevice, s));
-				launch_class(s, c, ids, counts, retries);
+				launch_class(s, c, ids, counts, retries, kBatchClasses);
 			}
 			if (large)
 			{

@@ -38,7 +38,7 @@ TABLE = os.path.join(HERE, "batch_shape.txt")
 CLASS_OPTIONS = (
     [(12, 128, 0), (10, 128, 0), (9, 128, 0), (8, 128, 0), (6, 256, 0), (5, 256, 0)],
     [(9, 128, 0), (8, 128, 0), (7, 128, 0), (6, 128, 0), (6, 256, 0), (5, 256, 0), (4, 256, 0), (3, 512, 0)],
-    [(5, 256, 0), (4, 256, 0), (3, 256, 0), (2, 256, 0), (2, 256, 1), (2, 512, 0)],
+    [(6, 256, 0), (5, 256, 0), (4, 256, 0), (3, 256, 0), (2, 256, 0), (2, 256, 1), (2, 512, 0)],
     [(1, 1024, 1)],
 )
 
