@@ -82,6 +82,23 @@ def test_oversized_mesh_takes_plain_path():
     check_batch(meshes, targets, 1e-2, M.SimplifyLockBorder)
 
 
+class launch_shape:
+    """Runs a block under the given launch shape (none: the built-in default), then restores the
+    shape that was in use (the shipped table, unless a test set another)."""
+
+    def __init__(self, *shape):
+        self.shape = shape
+
+    def __enter__(self):
+        self.saved = M.batch_get_shape()
+        M.batch_set_shape(*self.shape)
+        return M.batch_get_shape()
+
+    def __exit__(self, *exc):
+        M.batch_set_shape(self.saved["per_cu"], self.saved["threads"], self.saved["cursor"])
+        return False
+
+
 def check_batch_device(meshes, targets, err, opts):
     """The device entry (offsets on the device only) against the oracle, mesh by mesh."""
     from devmem import DeviceArray
@@ -104,15 +121,16 @@ def check_batch_device(meshes, targets, err, opts):
 @pytest.mark.parametrize("entry", ["host", "device"])
 @pytest.mark.parametrize("opts", [M.SimplifyLockBorder, M.SimplifyErrorAbsolute | M.SimplifyPermissive])
 def test_size_classes(opts, entry):
-    """Meshes for every LDS size class in one call: patches (6 and 5 workgroups per CU), seamed
-    grids of 2400-2904 indices (2 per CU) and 300 of 3174-5400 indices (one workgroup per CU
-    with the whole LDS, taking meshes from a cursor: more of them than CUs), and one of 6534
-    indices beyond every class (the plain path).  The host entry knows the sizes and launches the
-    largest class beside the others; the device entry launches it after the status read-back."""
+    """Meshes for every LDS size class of the built-in default shape in one call: patches (6 and
+    5 workgroups per CU), seamed grids of 2400-2904 indices (2 per CU) and 300 of 4056-5400
+    indices (one workgroup per CU with the whole LDS, taking meshes from a cursor: more of them
+    than CUs), and one of 6534 indices beyond every class (the plain path).  The host entry knows
+    the sizes and launches the largest class beside the others; the device entry launches it
+    after the status read-back."""
     rng = np.random.default_rng(4242 + opts)
     meshes, targets = patch_meshes(40, seed=31)
     for k in range(320):
-        n = int(rng.integers(21, 24)) if k < 20 else int(rng.integers(24, 32))
+        n = int(rng.integers(21, 24)) if k < 20 else int(rng.integers(27, 32))
         ib, v = random_grid(rng, n=n, seams=3)
         meshes.append((ib, v))
         targets.append((ib.size // 3 * int(rng.integers(1, 4)) // 4) * 3)
@@ -124,10 +142,13 @@ def test_size_classes(opts, entry):
     meshes = [meshes[i] for i in order]
     targets = [targets[i] for i in order]
     announce(f"size classes: {len(meshes)} meshes, largest {max(m[0].size for m in meshes)} indices, opts {opts}, {entry} entry")
-    if entry == "host":
-        check_batch(meshes, targets, 1e-2, opts)
-    else:
-        check_batch_device(meshes, targets, 1e-2, opts)
+    with launch_shape() as shape:
+        caps = shape["index_caps"]
+        assert 2904 <= caps[2] < 4056 and caps[3] >= 5400, caps
+        if entry == "host":
+            check_batch(meshes, targets, 1e-2, opts)
+        else:
+            check_batch_device(meshes, targets, 1e-2, opts)
 
 
 def test_sparse_batch():
@@ -273,7 +294,7 @@ def test_edge_value_batch(opts):
 def test_batch_shapes(shape):
     """Launch shapes a tuned table may hold (other workgroups per CU, 128- and 512-thread classes,
     cursor forms): every mesh of a mix over all size classes, through the device entry, against the oracle;
-    the default shape is restored afterwards."""
+    the shape in use before is restored afterwards."""
     rng = np.random.default_rng(515)
     meshes, targets = patch_meshes(48, seed=33)
     for k in range(40):
@@ -284,13 +305,9 @@ def test_batch_shapes(shape):
     order = rng.permutation(len(meshes))
     meshes = [meshes[i] for i in order]
     targets = [targets[i] for i in order]
-    M.batch_set_shape(*shape)
-    try:
-        caps = M.batch_get_shape()
+    with launch_shape(*shape) as caps:
         announce(f"batch shape {shape}: caps {caps['index_caps']}")
         check_batch_device(meshes, targets, 1e-2, M.SimplifyLockBorder)
-    finally:
-        M.batch_set_shape()
 
 
 def check_batch_attributes(meshes, attrs, weights, targets, err, opts, locks=None, entry="host"):
