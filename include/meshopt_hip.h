@@ -201,9 +201,12 @@ int meshopt_hip_batch_get_shape(unsigned int* per_cu, unsigned int* threads, uns
 /* Serving.  A server runs simplify requests of any size as they arrive, on `slots` concurrent
  * device contexts (each its own HIP stream, arena and staging buffer), admitting them in arrival
  * order against a device-memory budget: a request starts once a slot is idle and the slot's
- * memory, next to what the other slots hold, fits the budget.  Each result is exactly what
- * meshopt_simplify / meshopt_simplifyWithAttributes (src/meshoptimizer.h:513, 537) returns for
- * the request alone.  This interface has no counterpart in the reference. */
+ * memory, next to what the other slots hold, fits the budget.  A small request without
+ * attributes, Sparse or Prune (at most 1984 vertices and 5952 indices) takes the small requests
+ * queued right behind it with the same target error, options and lock presence into one batched
+ * call on its slot.  Each result is exactly what meshopt_simplify /
+ * meshopt_simplifyWithAttributes (src/meshoptimizer.h:513, 537) returns for the request alone.
+ * This interface has no counterpart in the reference. */
 typedef struct meshopt_hip_server meshopt_hip_server;
 
 typedef struct meshopt_hip_request
@@ -231,6 +234,7 @@ typedef struct meshopt_hip_response
 	int slot;
 	double submit_ms, start_ms, finish_ms; /* server clock (meshopt_hip_server_clock_ms) */
 	size_t device_bytes;                   /* the admission estimate of the request */
+	int batched; /* requests simplified by the same batched call (1: the plain path, alone) */
 } meshopt_hip_response;
 
 /* Returns NULL on failure (meshopt_hip_last_error). */

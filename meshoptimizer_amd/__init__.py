@@ -553,13 +553,15 @@ class ServerRequest(ctypes.Structure):
 class ServerResponse(ctypes.Structure):
     """meshopt_hip_response."""
     _fields_ = [("index_count", _SZ), ("result_error", _F), ("slot", ctypes.c_int), ("submit_ms", ctypes.c_double),
-                ("start_ms", ctypes.c_double), ("finish_ms", ctypes.c_double), ("device_bytes", _SZ)]
+                ("start_ms", ctypes.c_double), ("finish_ms", ctypes.c_double), ("device_bytes", _SZ),
+                ("batched", ctypes.c_int)]
 
 
 class Server:
     """A simplify server (meshopt_hip_server_*): requests run concurrently on `slots` device
-    contexts and are admitted in arrival order against `budget_bytes` of device memory.  Each
-    result equals simplify / simplify_with_attributes on the request alone."""
+    contexts and are admitted in arrival order against `budget_bytes` of device memory; small
+    attribute-free requests queued together run as one batched call.  Each result equals simplify /
+    simplify_with_attributes on the request alone."""
 
     def __init__(self, budget_bytes=64 << 30, slots=4):
         self._lib = library()
@@ -599,7 +601,7 @@ class Server:
         if rc != 0:
             raise RuntimeError(self._lib.meshopt_hip_last_error().decode())
         info = {"slot": resp.slot, "submit_ms": resp.submit_ms, "start_ms": resp.start_ms,
-                "finish_ms": resp.finish_ms, "device_bytes": resp.device_bytes}
+                "finish_ms": resp.finish_ms, "device_bytes": resp.device_bytes, "batched": resp.batched}
         return dest[:resp.index_count].copy(), resp.result_error, info
 
     def clock_ms(self):

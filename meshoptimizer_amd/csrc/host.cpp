@@ -2304,6 +2304,67 @@ extern "C" int meshopt_hip_last_batch_stats(unsigned int* stats, size_t max_mesh
 	}
 }
 
+// a batched call from host arrays on one context (the caller holds the context): stages the
+// arrays, runs the batch and copies every mesh's result back; throws on invalid input
+static void batch_host_on(Context& ctx, unsigned int* destination, unsigned int* result_counts, float* result_errors, const unsigned int* indices,
+                          const unsigned int* index_offsets, const float* vertex_positions, const unsigned int* vertex_offsets,
+                          const float* vertex_attributes, size_t vertex_attributes_stride, const float* attribute_weights, size_t attribute_count,
+                          const unsigned char* vertex_lock, size_t vertex_positions_stride, const unsigned int* target_index_counts,
+                          size_t mesh_count, float target_error, unsigned int options)
+{
+	ctx.ensure_init();
+	hipStream_t s = ctx.stream;
+	size_t I = index_offsets[mesh_count], V = vertex_offsets[mesh_count];
+	size_t pos_bytes = V ? (V - 1) * vertex_positions_stride + 12 : 0;
+	size_t n1 = mesh_count + 1;
+	size_t off_idx = 0, off_dst = (I * 4 + 255) & ~size_t(255);
+	size_t off_io = off_dst + ((I * 4 + 255) & ~size_t(255));
+	size_t off_vo = off_io + ((n1 * 4 + 255) & ~size_t(255));
+	size_t off_tg = off_vo + ((n1 * 4 + 255) & ~size_t(255));
+	size_t off_rc = off_tg + ((n1 * 4 + 255) & ~size_t(255));
+	size_t off_re = off_rc + ((n1 * 4 + 255) & ~size_t(255));
+	size_t off_pos = off_re + ((n1 * 4 + 255) & ~size_t(255));
+	size_t off_lock = off_pos + ((pos_bytes + 255) & ~size_t(255));
+	size_t lock_bytes = vertex_lock ? V : 0;
+	if (attribute_count && (attribute_count > (size_t)kMaxAttributes || vertex_attributes_stride < attribute_count * 4 ||
+	                        vertex_attributes_stride > 256 || vertex_attributes_stride % 4 != 0))
+		throw std::invalid_argument("meshopt_hip_simplify_batch: invalid attribute count or stride");
+	size_t attr_bytes = attribute_count && V ? (V - 1) * vertex_attributes_stride + attribute_count * 4 : 0;
+	size_t off_attr = off_lock + ((lock_bytes + 255) & ~size_t(255));
+	char* d = static_cast<char*>(ctx.staging.get(off_attr + attr_bytes + 256));
+	if (I)
+		HIP_CHECK(hipMemcpyAsync(d + off_idx, indices, I * 4, hipMemcpyHostToDevice, s));
+	HIP_CHECK(hipMemcpyAsync(d + off_io, index_offsets, n1 * 4, hipMemcpyHostToDevice, s));
+	HIP_CHECK(hipMemcpyAsync(d + off_vo, vertex_offsets, n1 * 4, hipMemcpyHostToDevice, s));
+	if (mesh_count)
+		HIP_CHECK(hipMemcpyAsync(d + off_tg, target_index_counts, mesh_count * 4, hipMemcpyHostToDevice, s));
+	if (pos_bytes)
+		HIP_CHECK(hipMemcpyAsync(d + off_pos, vertex_positions, pos_bytes, hipMemcpyHostToDevice, s));
+	if (lock_bytes)
+		HIP_CHECK(hipMemcpyAsync(d + off_lock, vertex_lock, lock_bytes, hipMemcpyHostToDevice, s));
+	if (attr_bytes)
+		HIP_CHECK(hipMemcpyAsync(d + off_attr, vertex_attributes, attr_bytes, hipMemcpyHostToDevice, s));
+	const BatchAttributes attrs{reinterpret_cast<const float*>(d + off_attr), vertex_attributes_stride, attribute_weights, attribute_count};
+	simplify_batch_on_device(ctx, context_batch_memory(ctx, mesh_count, V, !(options & (kOptSparse | kOptPrune)), attribute_count),
+	                         reinterpret_cast<unsigned int*>(d + off_dst),
+	                         reinterpret_cast<unsigned int*>(d + off_rc),
+	                         reinterpret_cast<float*>(d + off_re), reinterpret_cast<const unsigned int*>(d + off_idx),
+	                         reinterpret_cast<const unsigned int*>(d + off_io), reinterpret_cast<const float*>(d + off_pos),
+	                         reinterpret_cast<const unsigned int*>(d + off_vo), lock_bytes ? reinterpret_cast<const unsigned char*>(d + off_lock) : nullptr,
+	                         V, vertex_positions_stride,
+	                         reinterpret_cast<const unsigned int*>(d + off_tg), mesh_count, target_error, options, index_offsets, vertex_offsets,
+	                         target_index_counts, nullptr, attribute_count ? &attrs : nullptr);
+	if (mesh_count)
+	{
+		HIP_CHECK(hipMemcpyAsync(result_counts, d + off_rc, mesh_count * 4, hipMemcpyDeviceToHost, s));
+		if (result_errors)
+			HIP_CHECK(hipMemcpyAsync(result_errors, d + off_re, mesh_count * 4, hipMemcpyDeviceToHost, s));
+	}
+	if (I)
+		HIP_CHECK(hipMemcpyAsync(destination, d + off_dst, I * 4, hipMemcpyDeviceToHost, s));
+	HIP_CHECK(hipStreamSynchronize(s));
+}
+
 static int batch_host_entry(unsigned int* destination, unsigned int* result_counts, float* result_errors, const unsigned int* indices,
                             const unsigned int* index_offsets, const float* vertex_positions, const unsigned int* vertex_offsets,
                             const float* vertex_attributes, size_t vertex_attributes_stride, const float* attribute_weights, size_t attribute_count,
@@ -2322,57 +2383,9 @@ static int batch_host_entry(unsigned int* destination, unsigned int* result_coun
 				throw std::invalid_argument("meshopt_hip_simplify_batch: invalid index count, target or offsets");
 		Context& ctx = context();
 		std::lock_guard<std::mutex> guard(ctx.mutex);
-		ctx.ensure_init();
-		hipStream_t s = ctx.stream;
-		size_t I = index_offsets[mesh_count], V = vertex_offsets[mesh_count];
-		size_t pos_bytes = V ? (V - 1) * vertex_positions_stride + 12 : 0;
-		size_t n1 = mesh_count + 1;
-		size_t off_idx = 0, off_dst = (I * 4 + 255) & ~size_t(255);
-		size_t off_io = off_dst + ((I * 4 + 255) & ~size_t(255));
-		size_t off_vo = off_io + ((n1 * 4 + 255) & ~size_t(255));
-		size_t off_tg = off_vo + ((n1 * 4 + 255) & ~size_t(255));
-		size_t off_rc = off_tg + ((n1 * 4 + 255) & ~size_t(255));
-		size_t off_re = off_rc + ((n1 * 4 + 255) & ~size_t(255));
-		size_t off_pos = off_re + ((n1 * 4 + 255) & ~size_t(255));
-		size_t off_lock = off_pos + ((pos_bytes + 255) & ~size_t(255));
-		size_t lock_bytes = vertex_lock ? V : 0;
-		if (attribute_count && (attribute_count > (size_t)kMaxAttributes || vertex_attributes_stride < attribute_count * 4 ||
-		                        vertex_attributes_stride > 256 || vertex_attributes_stride % 4 != 0))
-			throw std::invalid_argument("meshopt_hip_simplify_batch: invalid attribute count or stride");
-		size_t attr_bytes = attribute_count && V ? (V - 1) * vertex_attributes_stride + attribute_count * 4 : 0;
-		size_t off_attr = off_lock + ((lock_bytes + 255) & ~size_t(255));
-		char* d = static_cast<char*>(ctx.staging.get(off_attr + attr_bytes + 256));
-		if (I)
-			HIP_CHECK(hipMemcpyAsync(d + off_idx, indices, I * 4, hipMemcpyHostToDevice, s));
-		HIP_CHECK(hipMemcpyAsync(d + off_io, index_offsets, n1 * 4, hipMemcpyHostToDevice, s));
-		HIP_CHECK(hipMemcpyAsync(d + off_vo, vertex_offsets, n1 * 4, hipMemcpyHostToDevice, s));
-		if (mesh_count)
-			HIP_CHECK(hipMemcpyAsync(d + off_tg, target_index_counts, mesh_count * 4, hipMemcpyHostToDevice, s));
-		if (pos_bytes)
-			HIP_CHECK(hipMemcpyAsync(d + off_pos, vertex_positions, pos_bytes, hipMemcpyHostToDevice, s));
-		if (lock_bytes)
-			HIP_CHECK(hipMemcpyAsync(d + off_lock, vertex_lock, lock_bytes, hipMemcpyHostToDevice, s));
-		if (attr_bytes)
-			HIP_CHECK(hipMemcpyAsync(d + off_attr, vertex_attributes, attr_bytes, hipMemcpyHostToDevice, s));
-		const BatchAttributes attrs{reinterpret_cast<const float*>(d + off_attr), vertex_attributes_stride, attribute_weights, attribute_count};
-		simplify_batch_on_device(ctx, context_batch_memory(ctx, mesh_count, V, !(options & (kOptSparse | kOptPrune)), attribute_count),
-		                         reinterpret_cast<unsigned int*>(d + off_dst),
-		                         reinterpret_cast<unsigned int*>(d + off_rc),
-		                         reinterpret_cast<float*>(d + off_re), reinterpret_cast<const unsigned int*>(d + off_idx),
-		                         reinterpret_cast<const unsigned int*>(d + off_io), reinterpret_cast<const float*>(d + off_pos),
-		                         reinterpret_cast<const unsigned int*>(d + off_vo), lock_bytes ? reinterpret_cast<const unsigned char*>(d + off_lock) : nullptr,
-		                         V, vertex_positions_stride,
-		                         reinterpret_cast<const unsigned int*>(d + off_tg), mesh_count, target_error, options, index_offsets, vertex_offsets,
-		                         target_index_counts, nullptr, attribute_count ? &attrs : nullptr);
-		if (mesh_count)
-		{
-			HIP_CHECK(hipMemcpyAsync(result_counts, d + off_rc, mesh_count * 4, hipMemcpyDeviceToHost, s));
-			if (result_errors)
-				HIP_CHECK(hipMemcpyAsync(result_errors, d + off_re, mesh_count * 4, hipMemcpyDeviceToHost, s));
-		}
-		if (I)
-			HIP_CHECK(hipMemcpyAsync(destination, d + off_dst, I * 4, hipMemcpyDeviceToHost, s));
-		HIP_CHECK(hipStreamSynchronize(s));
+		batch_host_on(ctx, destination, result_counts, result_errors, indices, index_offsets, vertex_positions, vertex_offsets, vertex_attributes,
+		              vertex_attributes_stride, attribute_weights, attribute_count, vertex_lock, vertex_positions_stride, target_index_counts,
+		              mesh_count, target_error, options);
 		t_last_error.clear();
 		return 0;
 	}
@@ -2820,16 +2833,79 @@ struct ServerTicket
 	size_t result = 0;
 	float error = 0;
 	int slot = -1;
+	int batched = 1; // requests in the call that ran this one
 	bool done = false;
 	bool claimed = false; // a thread waits on it (only one may: that thread frees it)
 	std::string failure;
 };
 
+// requests the batched kernel takes whole (the largest LDS size class: 1984 vertices, 5952
+// indices; a larger mesh would still be simplified exactly, by the plain path inside the call),
+// without attributes, Sparse or Prune
+constexpr size_t kServeBatchVertices = 1984, kServeBatchIndices = 5952;
+// requests one batched call of the server takes at most
+constexpr size_t kServeBatchMax = 4096;
+
+static bool serve_batchable(const meshopt_hip_request& r)
+{
+	return r.attribute_count == 0 && !(r.options & (kOptSparse | kOptPrune)) && r.index_count <= kServeBatchIndices &&
+	       r.vertex_count <= kServeBatchVertices;
+}
+
+// one batched call takes one target error, one option set and locks for all meshes or none
+static bool serve_compatible(const meshopt_hip_request& a, const meshopt_hip_request& b)
+{
+	return a.target_error == b.target_error && a.options == b.options && (a.vertex_lock == nullptr) == (b.vertex_lock == nullptr);
+}
+
+// small requests taken together from the queue: packed (positions as xyz, indices local to each
+// request's vertices) and simplified by one batched call on the slot's context; each gets what the
+// plain path returns for it alone.  Throws when the call fails (a bad index in any request); the
+// worker then runs each request on its own.
+static void serve_batch(Context& ctx, const std::vector<ServerTicket*>& g)
+{
+	const size_t P = g.size();
+	std::vector<uint32_t> io(P + 1, 0), vo(P + 1, 0), tg(P), counts(P);
+	std::vector<float> errs(P);
+	for (size_t p = 0; p < P; ++p)
+	{
+		io[p + 1] = io[p] + (uint32_t)g[p]->req.index_count;
+		vo[p + 1] = vo[p] + (uint32_t)g[p]->req.vertex_count;
+		tg[p] = (uint32_t)g[p]->req.target_index_count;
+	}
+	const bool locks = g[0]->req.vertex_lock != nullptr;
+	std::vector<uint32_t> idx(std::max<size_t>(io[P], 1)), dst(std::max<size_t>(io[P], 1));
+	std::vector<float> pos(3 * std::max<size_t>(vo[P], 1));
+	std::vector<uint8_t> lock(locks ? std::max<size_t>(vo[P], 1) : 0);
+	for (size_t p = 0; p < P; ++p)
+	{
+		const meshopt_hip_request& r = g[p]->req;
+		if (r.index_count)
+			memcpy(idx.data() + io[p], r.indices, r.index_count * sizeof(uint32_t));
+		const char* src = reinterpret_cast<const char*>(r.vertex_positions);
+		for (size_t v = 0; v < r.vertex_count; ++v)
+			memcpy(pos.data() + 3 * (vo[p] + v), src + v * r.vertex_positions_stride, 3 * sizeof(float));
+		if (locks && r.vertex_count)
+			memcpy(lock.data() + vo[p], r.vertex_lock, r.vertex_count);
+	}
+	batch_host_on(ctx, dst.data(), counts.data(), errs.data(), idx.data(), io.data(), pos.data(), vo.data(), nullptr, 0, nullptr, 0,
+	              locks ? lock.data() : nullptr, 3 * sizeof(float), tg.data(), P, g[0]->req.target_error, g[0]->req.options);
+	for (size_t p = 0; p < P; ++p)
+	{
+		if (counts[p])
+			memcpy(g[p]->req.destination, dst.data() + io[p], counts[p] * sizeof(uint32_t));
+		g[p]->result = counts[p];
+		g[p]->error = errs[p];
+		g[p]->batched = (int)P;
+	}
+}
+
 // Requests are admitted in arrival order (FIFO, no overtaking) to the first idle slot whose
 // memory, next to what the other slots hold, fits the budget.  A slot keeps its arena between
 // requests unless it holds more than budget / slots; idle slots give theirs up when the head of
 // the queue needs the room.  Slots run concurrently, each on its own HIP stream, so the device
-// overlaps their work.
+// overlaps their work.  A small request at the head takes the compatible small requests queued
+// right behind it (while their memory fits) into one batched call on its slot.
 struct Server
 {
 	size_t budget = 0;
@@ -2896,26 +2972,64 @@ struct Server
 			queue.pop_front();
 			ServerTicket& t = *tickets[id];
 			busy[s] = 1;
-			held[s] = std::max(held[s], t.need);
 			t.start_ms = now_ms();
 			t.slot = s;
+			std::vector<ServerTicket*> group{&t};
+			size_t need = t.need;
+			if (serve_batchable(t.req))
+			{
+				size_t others = 0;
+				for (int k = 0; k < slots; ++k)
+					if (k != s)
+						others += held[k];
+				while (!queue.empty() && group.size() < kServeBatchMax)
+				{
+					ServerTicket& u = *tickets[queue.front()];
+					if (!serve_batchable(u.req) || !serve_compatible(t.req, u.req) || others + std::max(held[s], need + u.need) > budget)
+						break;
+					queue.pop_front();
+					need += u.need;
+					u.start_ms = t.start_ms;
+					u.slot = s;
+					group.push_back(&u);
+				}
+			}
+			held[s] = std::max(held[s], need);
 			lk.unlock();
 
-			size_t n = 0;
-			float err = 0;
-			std::string failure;
-			try
+			auto run_alone = [&](ServerTicket& r1) {
+				const meshopt_hip_request& r = r1.req;
+				try
+				{
+					r1.result = simplify_host_on(ctx, r.indices, r.destination, r.index_count, r.vertex_positions, r.vertex_count,
+					                             r.vertex_positions_stride, r.attribute_count ? r.vertex_attributes : nullptr,
+					                             r.vertex_attributes_stride, r1.weights.empty() ? nullptr : r1.weights.data(), r.attribute_count,
+					                             r.vertex_lock, r.target_index_count, r.target_error, r.options, &r1.error);
+				}
+				catch (const std::exception& e)
+				{
+					r1.result = 0;
+					r1.error = 0;
+					r1.failure = e.what();
+				}
+			};
+			bool ran = false;
+			if (group.size() > 1)
 			{
-				const meshopt_hip_request& r = t.req;
-				n = simplify_host_on(ctx, r.indices, r.destination, r.index_count, r.vertex_positions, r.vertex_count, r.vertex_positions_stride,
-				                     r.attribute_count ? r.vertex_attributes : nullptr, r.vertex_attributes_stride,
-				                     t.weights.empty() ? nullptr : t.weights.data(), r.attribute_count, r.vertex_lock, r.target_index_count,
-				                     r.target_error, r.options, &err);
+				try
+				{
+					serve_batch(ctx, group);
+					ran = true;
+				}
+				catch (const std::exception&)
+				{
+					// some request is invalid for the device (a bad index): each runs on its own,
+					// so that only the invalid one fails
+				}
 			}
-			catch (const std::exception& e)
-			{
-				failure = e.what();
-			}
+			if (!ran)
+				for (ServerTicket* u : group)
+					run_alone(*u);
 			size_t now_held = ctx.held_bytes();
 			if (now_held > budget / (size_t)slots)
 			{
@@ -2926,11 +3040,12 @@ struct Server
 			lk.lock();
 			held[s] = now_held;
 			busy[s] = 0;
-			t.result = n;
-			t.error = err;
-			t.failure = failure;
-			t.finish_ms = now_ms();
-			t.done = true;
+			const double finish = now_ms();
+			for (ServerTicket* u : group)
+			{
+				u->finish_ms = finish;
+				u->done = true;
+			}
 			cv_done.notify_all();
 			cv_work.notify_all();
 		}
@@ -3044,6 +3159,7 @@ extern "C" int meshopt_hip_server_wait(meshopt_hip_server* server, size_t ticket
 		response->start_ms = t->start_ms;
 		response->finish_ms = t->finish_ms;
 		response->device_bytes = t->need;
+		response->batched = t->batched;
 	}
 	if (!t->failure.empty())
 	{

@@ -116,3 +116,54 @@ def test_rejections():
         ri, re, _ = srv.wait(t_ok)
     e = M.simplify(m.indices, m.vertices, W.target_count(m.index_count, 1, 4))
     assert np.array_equal(ri, e[0]) and same_bits(re, e[1])
+
+
+def test_small_requests_batched():
+    """Small attribute-free requests queued together run as one batched call on their slot, and
+    each result still equals the request alone and the oracle.  A large request submitted first
+    keeps the one slot busy while the small ones queue behind it; a request with other options
+    ends a run of compatible ones; a run whose requests carry locks batches too; and a run holding
+    a request with a bad index falls back to one call per request, failing only that one."""
+    rng = np.random.default_rng(77)
+    big = W.icosphere(8, seed=5)
+    reqs = [dict(ib=big.indices, v=big.vertices, target=W.target_count(big.index_count, 1, 10), err=1e-2, opts=0)]
+    for k in range(60):
+        ib, v = random_grid(rng, n=int(rng.integers(4, 24)))
+        r = dict(ib=ib, v=v, target=(ib.size // 3 * int(rng.integers(1, 4)) // 4) * 3, err=1e-2, opts=M.SimplifyLockBorder)
+        if k == 24:
+            r.update(err=1e-1, opts=0)
+        if k > 24:
+            r["lock"] = (rng.integers(0, 8, size=v.shape[0]) * (rng.random(v.shape[0]) < 0.3)).astype(np.uint8)
+        reqs.append(r)
+    bad_at = 1 + 50
+    bad = reqs[bad_at]["ib"].copy()
+    bad[4] = reqs[bad_at]["v"].shape[0] + 3
+    announce(f"serving: {len(reqs)} requests on one slot, small ones batched, a bad index at {bad_at}")
+
+    def submit_any(srv, r, ib):
+        return srv.submit(ib, r["v"], r["target"], r["err"], r["opts"], lock=r.get("lock"))
+
+    with M.Server(budget_bytes=8 << 30, slots=1) as srv:
+        tickets = [submit_any(srv, r, bad if k == bad_at else r["ib"]) for k, r in enumerate(reqs)]
+        results = []
+        for k, t in enumerate(tickets):
+            if k == bad_at:
+                with pytest.raises(RuntimeError, match="vertex"):
+                    srv.wait(t)
+                results.append(None)
+            else:
+                results.append(srv.wait(t))
+    for k, (r, res) in enumerate(zip(reqs, results)):
+        if res is None:
+            continue
+        ri, re, info = res
+        oi, oe = O.simplify(r["ib"], r["v"], r["target"], r["err"], r["opts"], lock=r.get("lock"))
+        assert np.array_equal(ri, oi) and same_bits(re, oe), f"request {k}: differs from the oracle (batched {info['batched']})"
+    batched = [res[2]["batched"] for res in results if res is not None]
+    assert batched[0] == 1, "the large request runs on the plain path"
+    assert max(batched[1:25]) > 1, f"no small requests were batched: {batched}"
+    assert max(batched) <= 25, f"a batch crossed the request with other options: {batched}"
+    # a group shares its start time; the one holding the bad index ran request by request
+    from collections import Counter
+    starts = Counter(res[2]["start_ms"] for res in results if res is not None and res[2]["batched"] == 1)
+    assert any(n > 1 for n in starts.values()), "the run with a bad index did not fall back to one call per request"
