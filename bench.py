@@ -337,7 +337,7 @@ def replay_cpu(reqs, arrivals, threads):
 
 
 def bench_serving(args):
-    """The serving stream: Poisson arrivals of requests of 1k-2M triangles (grids and icospheres,
+    """The serving stream: Poisson arrivals of requests of 1k-2M triangles (`--min-tris`, `--max-tris`; grids and icospheres,
     a third with normal+UV attributes, target ratio 0.5/0.25/0.1) served by meshopt_hip_server
     with `--slots` concurrent device contexts and a 64 GB admission budget.  First the closed-loop
     capacity (every request queued at once), then an open-loop replay at `--load` of it."""
@@ -347,7 +347,7 @@ def bench_serving(args):
     from meshoptimizer_amd import workloads as W
 
     count, slots, budget, load = args.requests, args.slots, 64 << 30, args.load
-    reqs = W.request_stream(count, 1.0)
+    reqs = W.request_stream(count, 1.0, tmin=args.min_tris, tmax=args.max_tris)
     unit = np.array([r.arrival_s for r in reqs])  # arrival times at one request per second
     tris = np.array([r.mesh.triangle_count for r in reqs], dtype=np.float64)
     total = float(tris.sum())
@@ -406,11 +406,12 @@ def bench_serving(args):
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded generator)",
-        "config": {"workload": f"serving stream: {count} requests, log-uniform 1k-2M triangles (grids and icospheres, "
+        "config": {"workload": f"serving stream: {count} requests, log-uniform {args.min_tris:g}-{args.max_tris:g} triangles (grids and icospheres, "
                                f"1/3 with normal+UV), target ratio 0.5/0.25/0.1, Poisson arrivals at {load:.0%} of the "
                                f"measured capacity, {slots} slots, 64 GB admission budget",
                    "requests": count, "input_triangles": int(total), "slots": slots,
-                   "offered_requests_per_s": round(rate, 2)},
+                   "offered_requests_per_s": round(rate, 2),
+                   "requests_batched": int(sum(1 for _, _, i in opened if i["batched"] > 1))},
         "p50_ms": round(float(np.percentile(lat_ms, 50)), 3),
         "p99_ms": round(float(np.percentile(lat_ms, 99)), 3),
         "closed_loop_capacity": round(capacity / 1e6, 3),
@@ -693,6 +694,8 @@ def main():
     ap.add_argument("--requests", type=int, default=64, help="serving: requests in the stream")
     ap.add_argument("--slots", type=int, default=4, help="serving: concurrent device contexts")
     ap.add_argument("--load", type=float, default=0.7, help="serving: offered load as a fraction of capacity")
+    ap.add_argument("--min-tris", type=float, default=1e3, help="serving: smallest request (triangles, log-uniform sizes)")
+    ap.add_argument("--max-tris", type=float, default=2e6, help="serving: largest request (triangles)")
     ap.add_argument("--cpu-threads", type=int, default=16, help="threads of the all-core CPU baseline (batches)")
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU baseline timing")
     ap.add_argument("--calibrate-copy", action="store_true",

@@ -201,10 +201,10 @@ int meshopt_hip_batch_get_shape(unsigned int* per_cu, unsigned int* threads, uns
 /* Serving.  A server runs simplify requests of any size as they arrive, on `slots` concurrent
  * device contexts (each its own HIP stream, arena and staging buffer), admitting them in arrival
  * order against a device-memory budget: a request starts once a slot is idle and the slot's
- * memory, next to what the other slots hold, fits the budget.  A small request without
- * attributes, Sparse or Prune (at most 1984 vertices and 5952 indices) takes the small requests
- * queued right behind it with the same target error, options and lock presence into one batched
- * call on its slot.  Each result is exactly what meshopt_simplify /
+ * memory, next to what the other slots hold, fits the budget.  A small request without Sparse
+ * or Prune (at most 1984 vertices and 5952 indices) takes the small requests queued behind it with
+ * the same target error, options, lock presence, attribute count and weights into one batched call
+ * on its slot.  Each result is exactly what meshopt_simplify /
  * meshopt_simplifyWithAttributes (src/meshoptimizer.h:513, 537) returns for the request alone.
  * This interface has no counterpart in the reference. */
 typedef struct meshopt_hip_server meshopt_hip_server;

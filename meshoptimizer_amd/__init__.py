@@ -560,7 +560,7 @@ class ServerResponse(ctypes.Structure):
 class Server:
     """A simplify server (meshopt_hip_server_*): requests run concurrently on `slots` device
     contexts and are admitted in arrival order against `budget_bytes` of device memory; small
-    attribute-free requests queued together run as one batched call.  Each result equals simplify /
+    compatible requests queued together run as one batched call.  Each result equals simplify /
     simplify_with_attributes on the request alone."""
 
     def __init__(self, budget_bytes=64 << 30, slots=4):

@@ -2841,26 +2841,30 @@ struct ServerTicket
 
 // requests the batched kernel takes whole (the largest LDS size class: 1984 vertices, 5952
 // indices; a larger mesh would still be simplified exactly, by the plain path inside the call),
-// without attributes, Sparse or Prune
+// without Sparse or Prune
 constexpr size_t kServeBatchVertices = 1984, kServeBatchIndices = 5952;
-// requests one batched call of the server takes at most
-constexpr size_t kServeBatchMax = 4096;
+// requests one batched call of the server takes at most, and queue entries it looks through
+constexpr size_t kServeBatchMax = 4096, kServeScan = 8192;
 
-static bool serve_batchable(const meshopt_hip_request& r)
+static bool serve_batchable(const ServerTicket& t)
 {
-	return r.attribute_count == 0 && !(r.options & (kOptSparse | kOptPrune)) && r.index_count <= kServeBatchIndices &&
-	       r.vertex_count <= kServeBatchVertices;
+	const meshopt_hip_request& r = t.req;
+	return !(r.options & (kOptSparse | kOptPrune)) && r.index_count <= kServeBatchIndices && r.vertex_count <= kServeBatchVertices &&
+	       r.attribute_count <= (size_t)kMaxAttributes && t.weights.size() == r.attribute_count;
 }
 
-// one batched call takes one target error, one option set and locks for all meshes or none
-static bool serve_compatible(const meshopt_hip_request& a, const meshopt_hip_request& b)
+// one batched call takes one target error, one option set, one attribute count with one set of
+// weights, and locks for all meshes or none
+static bool serve_compatible(const ServerTicket& a, const ServerTicket& b)
 {
-	return a.target_error == b.target_error && a.options == b.options && (a.vertex_lock == nullptr) == (b.vertex_lock == nullptr);
+	return a.req.target_error == b.req.target_error && a.req.options == b.req.options &&
+	       (a.req.vertex_lock == nullptr) == (b.req.vertex_lock == nullptr) && a.req.attribute_count == b.req.attribute_count &&
+	       a.weights == b.weights;
 }
 
-// small requests taken together from the queue: packed (positions as xyz, indices local to each
-// request's vertices) and simplified by one batched call on the slot's context; each gets what the
-// plain path returns for it alone.  Throws when the call fails (a bad index in any request); the
+// small requests taken together from the queue: packed (positions as xyz, attributes as
+// attribute_count floats per vertex, indices local to each request's vertices) and simplified by
+// one batched call on the slot's context; each gets what the plain path returns for it alone.  Throws when the call fails (a bad index in any request); the
 // worker then runs each request on its own.
 static void serve_batch(Context& ctx, const std::vector<ServerTicket*>& g)
 {
@@ -2874,8 +2878,9 @@ static void serve_batch(Context& ctx, const std::vector<ServerTicket*>& g)
 		tg[p] = (uint32_t)g[p]->req.target_index_count;
 	}
 	const bool locks = g[0]->req.vertex_lock != nullptr;
+	const size_t A = g[0]->req.attribute_count;
 	std::vector<uint32_t> idx(std::max<size_t>(io[P], 1)), dst(std::max<size_t>(io[P], 1));
-	std::vector<float> pos(3 * std::max<size_t>(vo[P], 1));
+	std::vector<float> pos(3 * std::max<size_t>(vo[P], 1)), attr(A * std::max<size_t>(vo[P], 1));
 	std::vector<uint8_t> lock(locks ? std::max<size_t>(vo[P], 1) : 0);
 	for (size_t p = 0; p < P; ++p)
 	{
@@ -2887,9 +2892,13 @@ static void serve_batch(Context& ctx, const std::vector<ServerTicket*>& g)
 			memcpy(pos.data() + 3 * (vo[p] + v), src + v * r.vertex_positions_stride, 3 * sizeof(float));
 		if (locks && r.vertex_count)
 			memcpy(lock.data() + vo[p], r.vertex_lock, r.vertex_count);
+		const char* asrc = reinterpret_cast<const char*>(r.vertex_attributes);
+		for (size_t v = 0; A && v < r.vertex_count; ++v)
+			memcpy(attr.data() + A * (vo[p] + v), asrc + v * r.vertex_attributes_stride, A * sizeof(float));
 	}
-	batch_host_on(ctx, dst.data(), counts.data(), errs.data(), idx.data(), io.data(), pos.data(), vo.data(), nullptr, 0, nullptr, 0,
-	              locks ? lock.data() : nullptr, 3 * sizeof(float), tg.data(), P, g[0]->req.target_error, g[0]->req.options);
+	batch_host_on(ctx, dst.data(), counts.data(), errs.data(), idx.data(), io.data(), pos.data(), vo.data(), A ? attr.data() : nullptr,
+	              A * sizeof(float), A ? g[0]->weights.data() : nullptr, A, locks ? lock.data() : nullptr, 3 * sizeof(float), tg.data(), P,
+	              g[0]->req.target_error, g[0]->req.options);
 	for (size_t p = 0; p < P; ++p)
 	{
 		if (counts[p])
@@ -2905,7 +2914,7 @@ static void serve_batch(Context& ctx, const std::vector<ServerTicket*>& g)
 // requests unless it holds more than budget / slots; idle slots give theirs up when the head of
 // the queue needs the room.  Slots run concurrently, each on its own HIP stream, so the device
 // overlaps their work.  A small request at the head takes the compatible small requests queued
-// right behind it (while their memory fits) into one batched call on its slot.
+// behind it (while their memory fits) into one batched call on its slot.
 struct Server
 {
 	size_t budget = 0;
@@ -2976,18 +2985,25 @@ struct Server
 			t.slot = s;
 			std::vector<ServerTicket*> group{&t};
 			size_t need = t.need;
-			if (serve_batchable(t.req))
+			if (serve_batchable(t))
 			{
 				size_t others = 0;
 				for (int k = 0; k < slots; ++k)
 					if (k != s)
 						others += held[k];
-				while (!queue.empty() && group.size() < kServeBatchMax)
+				// compatible requests from anywhere in the first kServeScan entries, in queue order;
+				// the others keep their order
+				for (size_t i = 0, seen = 0; i < queue.size() && seen < kServeScan && group.size() < kServeBatchMax; ++seen)
 				{
-					ServerTicket& u = *tickets[queue.front()];
-					if (!serve_batchable(u.req) || !serve_compatible(t.req, u.req) || others + std::max(held[s], need + u.need) > budget)
+					ServerTicket& u = *tickets[queue[i]];
+					if (!serve_batchable(u) || !serve_compatible(t, u))
+					{
+						++i;
+						continue;
+					}
+					if (others + std::max(held[s], need + u.need) > budget)
 						break;
-					queue.pop_front();
+					queue.erase(queue.begin() + (std::ptrdiff_t)i);
 					need += u.need;
 					u.start_ms = t.start_ms;
 					u.slot = s;

@@ -119,32 +119,38 @@ def test_rejections():
 
 
 def test_small_requests_batched():
-    """Small attribute-free requests queued together run as one batched call on their slot, and
-    each result still equals the request alone and the oracle.  A large request submitted first
-    keeps the one slot busy while the small ones queue behind it; a request with other options
-    ends a run of compatible ones; a run whose requests carry locks batches too; and a run holding
-    a request with a bad index falls back to one call per request, failing only that one."""
+    """Small requests queued together run as one batched call on their slot, and each result still
+    equals the oracle.  A large request submitted first keeps the one slot busy while the small
+    ones queue behind it.  A batch takes the compatible requests from anywhere in the queue: the
+    request with other options and the runs with locks or attributes in between do not stop the
+    last four plain requests from joining the first batch.  Requests with attributes (one weight
+    set) batch together; a run with locks batches too; and the run holding a request with a bad
+    index falls back to one call per request, failing only that one."""
     rng = np.random.default_rng(77)
     big = W.icosphere(8, seed=5)
     reqs = [dict(ib=big.indices, v=big.vertices, target=W.target_count(big.index_count, 1, 10), err=1e-2, opts=0)]
-    for k in range(60):
-        ib, v = random_grid(rng, n=int(rng.integers(4, 24)))
+    for k in range(64):
+        if 50 <= k < 60:
+            m = W.seamed_grid(int(rng.integers(6, 20)), seams=2, seed=int(rng.integers(1, 1000)))
+            ib, v = m.indices, m.vertices
+        else:
+            ib, v = random_grid(rng, n=int(rng.integers(4, 24)))
         r = dict(ib=ib, v=v, target=(ib.size // 3 * int(rng.integers(1, 4)) // 4) * 3, err=1e-2, opts=M.SimplifyLockBorder)
         if k == 24:
             r.update(err=1e-1, opts=0)
-        if k > 24:
+        if 25 <= k < 50:
             r["lock"] = (rng.integers(0, 8, size=v.shape[0]) * (rng.random(v.shape[0]) < 0.3)).astype(np.uint8)
+        if 50 <= k < 60:
+            r.update(attrs=v[:, 3:8], weights=W.SEAMED_GRID_WEIGHTS)
         reqs.append(r)
-    bad_at = 1 + 50
+    bad_at = 1 + 40
     bad = reqs[bad_at]["ib"].copy()
     bad[4] = reqs[bad_at]["v"].shape[0] + 3
     announce(f"serving: {len(reqs)} requests on one slot, small ones batched, a bad index at {bad_at}")
 
-    def submit_any(srv, r, ib):
-        return srv.submit(ib, r["v"], r["target"], r["err"], r["opts"], lock=r.get("lock"))
-
     with M.Server(budget_bytes=8 << 30, slots=1) as srv:
-        tickets = [submit_any(srv, r, bad if k == bad_at else r["ib"]) for k, r in enumerate(reqs)]
+        tickets = [srv.submit(bad if k == bad_at else r["ib"], r["v"], r["target"], r["err"], r["opts"], attributes=r.get("attrs"),
+                              weights=r.get("weights"), lock=r.get("lock")) for k, r in enumerate(reqs)]
         results = []
         for k, t in enumerate(tickets):
             if k == bad_at:
@@ -157,13 +163,17 @@ def test_small_requests_batched():
         if res is None:
             continue
         ri, re, info = res
-        oi, oe = O.simplify(r["ib"], r["v"], r["target"], r["err"], r["opts"], lock=r.get("lock"))
+        oi, oe = O.simplify(r["ib"], r["v"], r["target"], r["err"], r["opts"], attributes=r.get("attrs"), weights=r.get("weights"),
+                            lock=r.get("lock"))
         assert np.array_equal(ri, oi) and same_bits(re, oe), f"request {k}: differs from the oracle (batched {info['batched']})"
-    batched = [res[2]["batched"] for res in results if res is not None]
-    assert batched[0] == 1, "the large request runs on the plain path"
-    assert max(batched[1:25]) > 1, f"no small requests were batched: {batched}"
-    assert max(batched) <= 25, f"a batch crossed the request with other options: {batched}"
+    info = [None if res is None else res[2] for res in results]
+    assert info[0]["batched"] == 1, "the large request runs on the plain path"
+    first = [k for k in range(1, len(reqs)) if info[k] is not None and info[k]["start_ms"] == info[1]["start_ms"]]
+    assert first == list(range(1, 25)) + list(range(61, 65)), f"the first batch took {first}"
+    assert all(info[k]["batched"] == 28 for k in first)
+    assert info[25]["batched"] == 1, "the request with other options runs alone"
+    assert all(info[k]["batched"] == 10 for k in range(51, 61)), "the requests with attributes run as one batch"
     # a group shares its start time; the one holding the bad index ran request by request
-    from collections import Counter
-    starts = Counter(res[2]["start_ms"] for res in results if res is not None and res[2]["batched"] == 1)
-    assert any(n > 1 for n in starts.values()), "the run with a bad index did not fall back to one call per request"
+    locked = [info[k] for k in range(26, 51) if k != bad_at]
+    assert len({i["start_ms"] for i in locked}) == 1 and all(i["batched"] == 1 for i in locked), \
+        "the run with a bad index did not fall back to one call per request"
