@@ -177,3 +177,37 @@ def test_small_requests_batched():
     locked = [info[k] for k in range(26, 51) if k != bad_at]
     assert len({i["start_ms"] for i in locked}) == 1 and all(i["batched"] == 1 for i in locked), \
         "the run with a bad index did not fall back to one call per request"
+
+
+def test_batches_split_by_attribute_weights():
+    """Requests with attributes batch only with requests of the same attribute count and weights:
+    32 seamed grids queued behind a large request on one slot, with 3 or 5 attributes and one of
+    two weight sets each, run as four batches, one per (count, weights), and every result equals
+    the oracle."""
+    rng = np.random.default_rng(91)
+    big = W.icosphere(8, seed=9)
+    reqs = [dict(ib=big.indices, v=big.vertices, target=W.target_count(big.index_count, 1, 10), err=1e-2, opts=0)]
+    w5 = [W.SEAMED_GRID_WEIGHTS, np.array([0.25, 0.5, 1.0, 2.0, 0.125], np.float32)]
+    keys = []
+    for k in range(32):
+        m = W.seamed_grid(int(rng.integers(6, 20)), seams=2, seed=int(rng.integers(1, 1000)))
+        count, which = (3, 5)[k % 2], (k // 2) % 2
+        weights = w5[which][:count].copy()
+        reqs.append(dict(ib=m.indices, v=m.vertices, target=(m.index_count // 3 * int(rng.integers(1, 4)) // 4) * 3, err=1e-2,
+                         opts=M.SimplifyLockBorder, attrs=m.vertices[:, 3:3 + count], weights=weights))
+        keys.append((count, which))
+    announce(f"serving: {len(reqs)} requests, attribute batches split by count and weights")
+    with M.Server(budget_bytes=8 << 30, slots=1) as srv:
+        tickets = [srv.submit(r["ib"], r["v"], r["target"], r["err"], r["opts"], attributes=r["attrs"] if "attrs" in r else None,
+                              weights=r.get("weights")) for r in reqs]
+        results = [srv.wait(t) for t in tickets]
+    for k, (r, (ri, re, info)) in enumerate(zip(reqs, results)):
+        oi, oe = O.simplify(r["ib"], r["v"], r["target"], r["err"], r["opts"], attributes=r.get("attrs"), weights=r.get("weights"))
+        assert np.array_equal(ri, oi) and same_bits(re, oe), f"request {k}: differs from the oracle (batched {info['batched']})"
+    assert results[0][2]["batched"] == 1
+    from collections import Counter
+    sizes = Counter(keys)
+    for k, key in enumerate(keys):
+        assert results[k + 1][2]["batched"] == sizes[key], f"request {k + 1}: batched {results[k + 1][2]['batched']}, expected {sizes[key]}"
+    starts = {key: {results[k + 1][2]["start_ms"] for k, kk in enumerate(keys) if kk == key} for key in sizes}
+    assert all(len(v) == 1 for v in starts.values()), f"a (count, weights) group ran in more than one call: {starts}"
